@@ -1,0 +1,128 @@
+"""ctypes binding of libbsql.so (include/bsql/interp.h).
+
+The library is built in-tree by blazingsql_amd/build.py. Loading fails loudly
+if it is missing: there is no fallback implementation.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbsql.so")
+
+# type ids (cudf::type_id numbering)
+EMPTY, INT8, INT16, INT32, INT64 = 0, 1, 2, 3, 4
+UINT8, UINT16, UINT32, UINT64 = 5, 6, 7, 8
+FLOAT32, FLOAT64, BOOL8 = 9, 10, 11
+TIMESTAMP_DAYS, TIMESTAMP_SECONDS, TIMESTAMP_MILLISECONDS = 12, 13, 14
+TIMESTAMP_MICROSECONDS, TIMESTAMP_NANOSECONDS = 15, 16
+DURATION_DAYS, DURATION_SECONDS, DURATION_MILLISECONDS = 17, 18, 19
+DURATION_MICROSECONDS, DURATION_NANOSECONDS = 20, 21
+STRING = 23
+
+UNARY_INDEX, SCALAR_INDEX, SCALAR_NULL_INDEX, NULLARY_INDEX = -1, -2, -3, -4
+
+OK, ERR_OOM, ERR_INVALID, ERR_DEVICE = 0, 1, 2, 3
+
+# operator_type numbering (engine/src/parser/expression_utils.hpp:11-93)
+OPS = ["INVALID_OP", "RAND", "NOT", "ABS", "FLOOR", "CEIL", "SIN", "COS", "ASIN", "ACOS",
+       "TAN", "COTAN", "ATAN", "LN", "LOG", "YEAR", "MONTH", "DAY", "DAYOFWEEK", "HOUR",
+       "MINUTE", "SECOND", "IS_NULL", "IS_NOT_NULL", "IS_TRUE", "IS_NOT_TRUE", "IS_NOT_FALSE",
+       "CAST_TINYINT", "CAST_SMALLINT", "CAST_INTEGER", "CAST_BIGINT", "CAST_FLOAT",
+       "CAST_DOUBLE", "CAST_DATE", "CAST_TIMESTAMP_SECONDS", "CAST_TIMESTAMP_MILLISECONDS",
+       "CAST_TIMESTAMP_MICROSECONDS", "CAST_TIMESTAMP", "CAST_VARCHAR", "CHAR_LENGTH",
+       "STR_LOWER", "STR_UPPER", "STR_INITCAP", "STR_REVERSE", "ADD", "SUB", "MUL", "DIV",
+       "MOD", "POW", "ROUND", "EQUAL", "NOT_EQUAL", "LESS", "GREATER", "LESS_EQUAL",
+       "GREATER_EQUAL", "BITWISE_AND", "BITWISE_OR", "BITWISE_XOR", "LOGICAL_AND",
+       "LOGICAL_OR", "FIRST_NON_MAGIC", "MAGIC_IF_NOT", "STR_LIKE", "STR_SUBSTRING",
+       "STR_REGEXP_REPLACE", "STR_CONCAT", "STR_REPLACE", "STR_LEFT", "STR_RIGHT", "TO_DATE",
+       "TO_TIMESTAMP", "STR_TRIM", "IS_NOT_DISTINCT_FROM"]
+OP = {name: i for i, name in enumerate(OPS)}
+
+
+class Column(ctypes.Structure):
+    _fields_ = [("type_id", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("size", ctypes.c_int64), ("offset", ctypes.c_int64),
+                ("data", ctypes.c_void_p), ("validity", ctypes.c_void_p),
+                ("offsets", ctypes.c_void_p)]
+
+
+class _ScalarValue(ctypes.Union):
+    _fields_ = [("i64", ctypes.c_int64), ("f64", ctypes.c_double)]
+
+
+class Scalar(ctypes.Structure):
+    _fields_ = [("type_id", ctypes.c_int32), ("str_len", ctypes.c_int32),
+                ("v", _ScalarValue), ("str", ctypes.c_char_p)]
+
+
+class LaunchConfig(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_int32), ("rows_per_thread", ctypes.c_int32)]
+
+
+# exported symbols and their signatures, as declared in include/bsql/interp.h
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+SIGNATURES = {
+    "bsql_program_from_arrays": (_I, [_P, _P, _P, _P, _I, _P, _I, _P, _P, ctypes.POINTER(_P)]),
+    "bsql_program_compile": (_I, [ctypes.POINTER(ctypes.c_char_p), _I, _P, _I, ctypes.POINTER(_P)]),
+    "bsql_program_free": (None, [_P]),
+    "bsql_program_num_ops": (_I, [_P]),
+    "bsql_program_num_outputs": (_I, [_P]),
+    "bsql_program_num_inputs": (_I, [_P]),
+    "bsql_program_arrays": (_I, [_P, _P, _P, _P, _P, _P]),
+    "bsql_program_scalar": (_I, [_P, _I, _I, ctypes.POINTER(Scalar)]),
+    "bsql_program_output_type": (_I, [_P, _P, _I, _I, ctypes.POINTER(ctypes.c_int32)]),
+    "bsql_program_used_columns": (_I, [_P, _P, _I]),
+    "bsql_interp_execute": (_I, [_P, _P, _I, _P, _I, _I64, _P]),
+    "bsql_filter": (_I, [_P, _P, _I, _P, _I64, _P, _P]),
+    "bsql_filter_project": (_I, [_P, _P, _P, _I, _P, _I, _I64, _P, _P]),
+    "bsql_program_num_conjuncts": (_I, [_P]),
+    "bsql_program_set_conjunct_order": (_I, [_P, _P, _I]),
+    "bsql_program_set_launch": (_I, [_P, ctypes.POINTER(LaunchConfig)]),
+    "bsql_last_error": (ctypes.c_char_p, []),
+    "bsql_version": (_I, []),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libbsql.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libbsql.so is not built: run python blazingsql_amd/build.py "
+                "(or __graft_entry__.build())")
+        # One HIP runtime per process: torch ships its own libamdhip64.so.7.
+        # Loading torch first makes libbsql.so bind to that same runtime (same
+        # SONAME); loading ours first would bring in a second copy from
+        # /opt/rocm, and the two runtimes do not share the device.
+        import torch  # noqa: F401
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+class BsqlError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bsql error {code}: {msg}")
+        self.code = code
+
+
+class OutOfMemory(BsqlError):
+    pass
+
+
+def check(code):
+    if code != OK:
+        msg = lib().bsql_last_error().decode(errors="replace")
+        if code == ERR_OOM:
+            raise OutOfMemory(code, msg)
+        raise BsqlError(code, msg)
+    return code

@@ -1,0 +1,425 @@
+// capi.cpp -- the C ABI (include/bsql/interp.h) over the host program
+// representation and the gfx950 kernels.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bsql/interp.h"
+#include "program.hpp"
+
+struct bsql_program : bsql::Program {
+  std::mutex cache_mu;
+  std::map<std::string, bsql::Lowered> cache;
+};
+
+namespace bsql {
+
+hipError_t launch_execute(const DevPlan& plan, int nreg, int rpt, hipStream_t stream);
+int64_t compact_tiles(int64_t num_rows, int nreg, int rpt);
+hipError_t launch_compact(const DevPlan& plan, int nreg, int rpt, uint64_t* status,
+                          int64_t* d_count, int64_t capacity, hipStream_t stream);
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int device_error(hipError_t e, const char* where) {
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    return set_error(BSQL_ERR_OOM, std::string(where) + ": out of device memory");
+  return set_error(BSQL_ERR_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+// Compaction scratch (tile status words + ticket), one buffer per stream so
+// that concurrent calls on different streams never share it.
+struct Scratch {
+  uint64_t* p = nullptr;
+  int64_t words = 0;
+};
+std::mutex g_scratch_mu;
+std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
+
+hipError_t get_scratch(hipStream_t stream, int64_t words, uint64_t** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  Scratch& s = g_scratch[{dev, stream}];
+  if (s.words < words) {
+    if (s.p) {
+      // the previous buffer may still be read by queued work on this stream
+      e = hipStreamSynchronize(stream);
+      if (e != hipSuccess) return e;
+      (void)hipFree(s.p);
+      s.p = nullptr;
+      s.words = 0;
+    }
+    const int64_t n = words < 4096 ? 4096 : words + words / 4;
+    e = hipMalloc((void**)&s.p, (size_t)n * sizeof(uint64_t));
+    if (e != hipSuccess) return e;
+    s.words = n;
+  }
+  *out = s.p;
+  return hipSuccess;
+}
+
+// string literals -> one device buffer of {int32 len, bytes} records
+hipError_t ensure_strings(Program& p) {
+  if (p.d_strings || !p.str_offsets.empty()) return hipSuccess;
+  std::vector<char> blob;
+  std::vector<int64_t> offs(2 * p.ops.size(), -1);
+  for (size_t i = 0; i < p.ops.size(); ++i) {
+    for (int side = 0; side < 2; ++side) {
+      const Scalar& s = side ? p.rsc[i] : p.lsc[i];
+      const int pos = side ? p.right[i] : p.left[i];
+      if (pos != BSQL_SCALAR_INDEX || s.type_id != BSQL_STRING) continue;
+      while (blob.size() % 8) blob.push_back(0);
+      offs[2 * i + side] = (int64_t)blob.size();
+      int32_t len = (int32_t)s.str.size();
+      blob.insert(blob.end(), (const char*)&len, (const char*)&len + 4);
+      blob.insert(blob.end(), s.str.begin(), s.str.end());
+    }
+  }
+  if (blob.empty()) {
+    p.str_offsets = offs;
+    return hipSuccess;
+  }
+  void* d = nullptr;
+  hipError_t e = hipMalloc(&d, blob.size());
+  if (e != hipSuccess) return e;
+  e = hipMemcpy(d, blob.data(), blob.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { (void)hipFree(d); return e; }
+  p.d_strings = d;
+  p.str_offsets = offs;
+  return hipSuccess;
+}
+
+DevInCol to_dev_in(const bsql_column& c) {
+  DevInCol d{};
+  d.type = (int16_t)c.type_id;
+  if (c.type_id == BSQL_STRING) {
+    d.data = c.data;
+    d.offsets = c.offsets ? c.offsets + c.offset : nullptr;
+  } else {
+    d.data = (const char*)c.data + c.offset * type_size(c.type_id);
+  }
+  if (c.validity) {
+    d.valid = c.validity + (c.offset >> 5);
+    d.vbit = (int32_t)(c.offset & 31);
+  }
+  d.aligned16 = ((((uintptr_t)d.data) & 15) == 0) ? 1 : 0;
+  return d;
+}
+
+void check_inputs(const bsql_column* in, int n_in, int64_t num_rows) {
+  if (n_in < 0 || (n_in > 0 && !in)) fail("invalid input table");
+  if (num_rows < 0) fail("negative row count");
+  for (int k = 0; k < n_in; ++k) {
+    const bsql_column& c = in[k];
+    if (c.type_id == BSQL_STRING) {
+      if (num_rows > 0 && (!c.offsets || !c.data)) fail("string column without offsets/chars");
+    } else if (type_size(c.type_id) == 0) {
+      fail("unsupported input column type " + std::to_string(c.type_id));
+    } else if (num_rows > 0 && !c.data) {
+      fail("input column " + std::to_string(k) + " has no data");
+    }
+    if (c.size < num_rows) fail("input column " + std::to_string(k) + " is shorter than num_rows");
+  }
+}
+
+std::string cache_key(int mode, const bsql_program* pred, const bsql_program* proj,
+                      const bsql_column* in, int n_in, const bsql_column* out, int n_out) {
+  std::string k = std::to_string(mode) + "|" + std::to_string((uintptr_t)pred) + "|" +
+                  std::to_string((uintptr_t)proj) + "|";
+  for (int i = 0; i < n_in; ++i) k += std::to_string(in[i].type_id) + ",";
+  k += "|";
+  for (int i = 0; i < n_out; ++i) k += std::to_string(out ? out[i].type_id : 0) + ",";
+  if (pred) {
+    k += "|o";
+    for (int32_t c : pred->conj_order) k += std::to_string(c) + ",";
+  }
+  return k;
+}
+
+const Lowered& lowered_for(bsql_program* owner, const LowerOptions& opt, const bsql_column* in,
+                           int n_in, const bsql_column* out, int n_out) {
+  std::string key = cache_key((int)opt.mode, (const bsql_program*)opt.pred,
+                              (const bsql_program*)opt.proj, in, n_in, out, n_out);
+  std::lock_guard<std::mutex> lk(owner->cache_mu);
+  auto it = owner->cache.find(key);
+  if (it != owner->cache.end()) return it->second;
+  std::vector<int32_t> in_types(n_in), out_types(n_out);
+  for (int i = 0; i < n_in; ++i) in_types[i] = in[i].type_id;
+  for (int i = 0; i < n_out; ++i) out_types[i] = out ? out[i].type_id : 0;
+  Lowered l = lower(opt, in_types.data(), n_in, out_types.data(), n_out);
+  return owner->cache.emplace(key, l).first->second;
+}
+
+int rows_per_thread(const bsql_program* p) { return p ? p->launch.rows_per_thread : 0; }
+
+}  // namespace
+}  // namespace bsql
+
+using namespace bsql;
+
+#define BSQL_TRY(...)                                              \
+  try {                                                            \
+    __VA_ARGS__                                                    \
+  } catch (const bsql::Error& e) {                                 \
+    return set_error(e.code, e.what());                            \
+  } catch (const std::bad_alloc&) {                                \
+    return set_error(BSQL_ERR_OOM, "host allocation failed");      \
+  } catch (const std::exception& e) {                              \
+    return set_error(BSQL_ERR_INVALID, e.what());                  \
+  }
+
+extern "C" {
+
+const char* bsql_last_error(void) { return g_last_error.c_str(); }
+int bsql_version(void) { return 1; }
+
+int bsql_program_from_arrays(const int16_t* left, const int16_t* right, const int16_t* out,
+                             const int32_t* ops, int n_ops, const int16_t* final_out, int n_final,
+                             const bsql_scalar* left_scalars, const bsql_scalar* right_scalars,
+                             bsql_program** prog) {
+  BSQL_TRY({
+    if (!prog || n_ops < 0 || n_final <= 0 || !final_out) fail("invalid arguments");
+    if (n_ops > 0 && (!left || !right || !out || !ops)) fail("missing program arrays");
+    std::unique_ptr<bsql_program> p(new bsql_program());
+    p->left.assign(left, left + n_ops);
+    p->right.assign(right, right + n_ops);
+    p->out.assign(out, out + n_ops);
+    p->ops.assign(ops, ops + n_ops);
+    p->final_out.assign(final_out, final_out + n_final);
+    p->lsc.resize(n_ops);
+    p->rsc.resize(n_ops);
+    for (int i = 0; i < n_ops; ++i) {
+      for (int side = 0; side < 2; ++side) {
+        const int pos = side ? right[i] : left[i];
+        if (pos != BSQL_SCALAR_INDEX) continue;
+        const bsql_scalar* arr = side ? right_scalars : left_scalars;
+        if (!arr) fail("op " + std::to_string(i) + " has a scalar operand but no scalar array");
+        const bsql_scalar& s = arr[i];
+        Scalar& d = side ? p->rsc[i] : p->lsc[i];
+        d.type_id = s.type_id;
+        d.is_null = false;
+        d.i64 = s.v.i64;
+        d.f64 = s.v.f64;
+        if (s.type_id == BSQL_STRING) d.str.assign(s.str ? s.str : "", s.str ? (size_t)s.str_len : 0);
+      }
+    }
+    validate(*p, 0);
+    *prog = p.release();
+    return BSQL_OK;
+  })
+}
+
+int bsql_program_compile(const char* const* exprs, int n_exprs, const int32_t* in_types, int n_in,
+                         bsql_program** prog) {
+  BSQL_TRY({
+    if (!prog || !exprs || n_exprs <= 0 || n_in < 0 || (n_in > 0 && !in_types))
+      fail("invalid arguments");
+    std::vector<std::string> v;
+    for (int i = 0; i < n_exprs; ++i) {
+      if (!exprs[i]) fail("null expression");
+      v.emplace_back(exprs[i]);
+    }
+    std::unique_ptr<Program> p = compile_expressions(v, in_types, n_in);
+    std::unique_ptr<bsql_program> bp(new bsql_program());
+    static_cast<Program&>(*bp) = std::move(*p);
+    *prog = bp.release();
+    return BSQL_OK;
+  })
+}
+
+void bsql_program_free(bsql_program* prog) {
+  if (!prog) return;
+  if (prog->d_strings) (void)hipFree(prog->d_strings);
+  delete prog;
+}
+
+int bsql_program_num_ops(const bsql_program* p) { return p ? (int)p->ops.size() : -1; }
+int bsql_program_num_outputs(const bsql_program* p) { return p ? (int)p->final_out.size() : -1; }
+int bsql_program_num_inputs(const bsql_program* p) { return p ? p->n_inputs : -1; }
+
+int bsql_program_arrays(const bsql_program* p, int16_t* left, int16_t* right, int16_t* out,
+                        int32_t* ops, int16_t* final_out) {
+  if (!p) return set_error(BSQL_ERR_INVALID, "null program");
+  const size_t n = p->ops.size();
+  if (left) std::memcpy(left, p->left.data(), n * 2);
+  if (right) std::memcpy(right, p->right.data(), n * 2);
+  if (out) std::memcpy(out, p->out.data(), n * 2);
+  if (ops) std::memcpy(ops, p->ops.data(), n * 4);
+  if (final_out) std::memcpy(final_out, p->final_out.data(), p->final_out.size() * 2);
+  return BSQL_OK;
+}
+
+int bsql_program_scalar(const bsql_program* p, int op, int side, bsql_scalar* out) {
+  if (!p || !out || op < 0 || op >= (int)p->ops.size())
+    return set_error(BSQL_ERR_INVALID, "invalid arguments");
+  const Scalar& s = side ? p->rsc[op] : p->lsc[op];
+  const int pos = side ? p->right[op] : p->left[op];
+  std::memset(out, 0, sizeof(*out));
+  out->type_id = pos == BSQL_SCALAR_INDEX ? s.type_id : BSQL_EMPTY;
+  if (is_float_type(s.type_id)) out->v.f64 = s.f64; else out->v.i64 = s.i64;
+  if (s.type_id == BSQL_STRING) {
+    out->str = s.str.c_str();
+    out->str_len = (int32_t)s.str.size();
+  }
+  return BSQL_OK;
+}
+
+int bsql_program_output_type(const bsql_program* p, const int32_t* in_types, int n_in, int k,
+                             int32_t* type_id) {
+  BSQL_TRY({
+    if (!p || !type_id || k < 0 || k >= (int)p->final_out.size()) fail("invalid arguments");
+    std::vector<int32_t> st = slot_types_for(*p, in_types, n_in);
+    OpTypes t = propagate_types(*p, st.data(), (int)st.size());
+    const int slot = p->final_out[k];
+    int32_t ty = slot < (int)st.size() ? st[(size_t)slot] : BSQL_EMPTY;
+    for (size_t i = 0; i < p->ops.size(); ++i)
+      if (p->out[i] == slot) ty = t.ot[i];
+    *type_id = ty;
+    return BSQL_OK;
+  })
+}
+
+int bsql_program_used_columns(const bsql_program* p, int32_t* cols, int cap) {
+  if (!p) return set_error(BSQL_ERR_INVALID, "null program");
+  const int n = (int)p->used_columns.size();
+  for (int i = 0; i < n && i < cap; ++i) cols[i] = p->used_columns[(size_t)i];
+  return n;
+}
+
+int bsql_program_num_conjuncts(const bsql_program* pred) {
+  if (!pred) return set_error(BSQL_ERR_INVALID, "null program") * -1;
+  return count_conjuncts(*pred);
+}
+
+int bsql_program_set_conjunct_order(bsql_program* pred, const int32_t* order, int n_conj) {
+  BSQL_TRY({
+    if (!pred) fail("null program");
+    if (!order) { pred->conj_order.clear(); return BSQL_OK; }
+    const int n = count_conjuncts(*pred);
+    if (n_conj != n) fail("the predicate has " + std::to_string(n) + " conjuncts");
+    std::vector<int> seen(n, 0);
+    for (int i = 0; i < n_conj; ++i) {
+      if (order[i] < 0 || order[i] >= n || seen[order[i]]++) fail("order is not a permutation");
+    }
+    pred->conj_order.assign(order, order + n_conj);
+    return BSQL_OK;
+  })
+}
+
+int bsql_program_set_launch(bsql_program* p, const bsql_launch_config* cfg) {
+  if (!p || !cfg) return set_error(BSQL_ERR_INVALID, "invalid arguments");
+  const int r = cfg->rows_per_thread;
+  if (r != 0 && r != 1 && r != 2 && r != 4 && r != 8)
+    return set_error(BSQL_ERR_INVALID, "rows_per_thread must be 0, 1, 2, 4 or 8");
+  if (cfg->block_size != 0 && cfg->block_size != 256)
+    return set_error(BSQL_ERR_INVALID, "block_size must be 0 or 256");
+  p->launch = *cfg;
+  return BSQL_OK;
+}
+
+int bsql_interp_execute(const bsql_program* cprog, const bsql_column* in, int n_in,
+                        bsql_column* out, int n_out, int64_t num_rows, void* stream) {
+  BSQL_TRY({
+    bsql_program* prog = const_cast<bsql_program*>(cprog);
+    if (!prog) fail("null program");
+    check_inputs(in, n_in, num_rows);
+    if (n_out != (int)prog->final_out.size()) fail("number of outputs does not match the program");
+    for (int k = 0; k < n_out; ++k) {
+      if (out[k].offset != 0) fail("output columns must have offset 0");
+      if (type_size(out[k].type_id) == 0) fail("unsupported output column type");
+      if (out[k].size < num_rows) fail("output column is shorter than num_rows");
+      if (num_rows > 0 && !out[k].data) fail("output column has no data");
+    }
+    hipError_t e = ensure_strings(*prog);
+    if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
+    LowerOptions opt;
+    opt.mode = LowerOptions::EXECUTE;
+    opt.proj = prog;
+    const Lowered& L = lowered_for(prog, opt, in, n_in, out, n_out);
+    DevPlan plan = L.plan;
+    plan.num_rows = num_rows;
+    for (int k = 0; k < n_in; ++k) plan.in[k] = to_dev_in(in[k]);
+    for (int k = 0; k < n_out; ++k) {
+      plan.out[k].data = out[k].data;
+      plan.out[k].valid = out[k].validity;
+      plan.out[k].type = out[k].type_id;
+    }
+    if (num_rows == 0) return BSQL_OK;
+    e = launch_execute(plan, L.nreg, rows_per_thread(prog), (hipStream_t)stream);
+    if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
+    return BSQL_OK;
+  })
+}
+
+static int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, int n_in,
+                       bsql_column* out, int n_out, int64_t num_rows, int64_t capacity,
+                       int64_t* d_count, void* stream) {
+  if (!pred) fail("null predicate");
+  if (!d_count) fail("null count pointer");
+  check_inputs(in, n_in, num_rows);
+  for (int k = 0; k < n_out; ++k) {
+    if (out[k].offset != 0) fail("output columns must have offset 0");
+    if (type_size(out[k].type_id) == 0) fail("unsupported output column type");
+    if (out[k].size < capacity) fail("output column capacity is smaller than requested");
+    if (capacity > 0 && !out[k].data) fail("output column has no data");
+  }
+  hipError_t e = ensure_strings(*pred);
+  if (e == hipSuccess && proj) e = ensure_strings(*proj);
+  if (e != hipSuccess) return device_error(e, "filter");
+  LowerOptions opt;
+  opt.mode = proj ? LowerOptions::FILTER_PROJECT : LowerOptions::FILTER_GATHER;
+  opt.pred = pred;
+  opt.proj = proj;
+  opt.conj_order = &pred->conj_order;
+  const Lowered& L = lowered_for(pred, opt, in, n_in, out, n_out);
+  DevPlan plan = L.plan;
+  plan.num_rows = num_rows;
+  for (int k = 0; k < n_in; ++k) plan.in[k] = to_dev_in(in[k]);
+  for (int k = 0; k < n_out; ++k) {
+    plan.out[k].data = out[k].data;
+    plan.out[k].valid = out[k].validity;
+    plan.out[k].type = out[k].type_id;
+  }
+  const int rpt = rows_per_thread(proj ? proj : pred);
+  const int64_t tiles = compact_tiles(num_rows, L.nreg, rpt);
+  if (tiles < 0) fail("no kernel variant for this program");
+  uint64_t* scratch = nullptr;
+  e = get_scratch((hipStream_t)stream, tiles + 1, &scratch);
+  if (e != hipSuccess) return device_error(e, "filter scratch");
+  e = launch_compact(plan, L.nreg, rpt, scratch, d_count, capacity, (hipStream_t)stream);
+  if (e != hipSuccess) return device_error(e, "filter");
+  return BSQL_OK;
+}
+
+int bsql_filter(const bsql_program* pred, const bsql_column* in, int n_in, bsql_column* out,
+                int64_t num_rows, int64_t* d_count, void* stream) {
+  BSQL_TRY({
+    return run_compact(const_cast<bsql_program*>(pred), nullptr, in, n_in, out, n_in, num_rows,
+                       num_rows, d_count, stream);
+  })
+}
+
+int bsql_filter_project(const bsql_program* pred, const bsql_program* proj, const bsql_column* in,
+                        int n_in, bsql_column* out, int n_out, int64_t num_rows, int64_t* d_count,
+                        void* stream) {
+  BSQL_TRY({
+    if (!proj) fail("null projection");
+    return run_compact(const_cast<bsql_program*>(pred), const_cast<bsql_program*>(proj), in, n_in,
+                       out, n_out, num_rows, num_rows, d_count, stream);
+  })
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// devplan.h -- the lowered program the gfx950 VM kernel executes.
+//
+// Shared by the host lowering (program.cpp) and the HIP kernels
+// (interp_kernels.hip). Passed by value as a kernel argument, so the whole
+// plan is read through the scalar cache with wave-uniform loads.
+#pragma once
+
+#include <stdint.h>
+
+#define BSQL_DEV_MAX_OPS 96
+#define BSQL_DEV_MAX_IN 24
+#define BSQL_DEV_MAX_OUT 16
+#define BSQL_DEV_MAX_REG 32
+
+// Device opcodes: reference operators specialised by operand kind
+// (I = int64 slot, F = double slot), plus the VM's own load/store/select ops.
+enum DevCode : uint8_t {
+  D_NOP = 0,
+  D_LOAD,        // reg[dst] <- input column a (converted to the slot kind), validity
+  D_LOADV,       // validity of input column a only (string columns)
+  D_LOADN,       // aux (1..4) columns at once: imm bytes 2j / 2j+1 = column / register
+  D_STORE,       // output column aux <- reg[a] (static_cast to its type)
+  D_KILL,        // alive &= valid(a) && truthy(a); wave-uniform early exit
+  D_SEL,         // alive &= valid(a) && bits(a) != 0 (BOOL8 write of the slot)
+  D_ADD_II, D_SUB_II, D_MUL_II, D_DIV_II, D_MOD_II,
+  D_ADD_FF, D_SUB_FF, D_MUL_FF, D_DIV_FF, D_MOD_FF,
+  D_POW, D_ROUND,
+  D_EQ_II, D_NE_II, D_LT_II, D_GT_II, D_LE_II, D_GE_II,
+  D_EQ_FF, D_NE_FF, D_LT_FF, D_GT_FF, D_LE_FF, D_GE_FF,
+  D_EQ_TS, D_NE_TS, D_LT_TS, D_GT_TS, D_LE_TS, D_GE_TS,
+  D_EQ_SS, D_NE_SS, D_LT_SS, D_GT_SS, D_LE_SS, D_GE_SS,
+  D_AND, D_OR, D_INDF_II, D_INDF_FF, D_MAGIC_IF_NOT, D_FIRST_NON_MAGIC,
+  D_NOT, D_ABS_I, D_ABS_F, D_FLOOR, D_CEIL, D_SIN, D_COS, D_ASIN, D_ACOS,
+  D_TAN, D_COTAN, D_ATAN, D_LN, D_LOG, D_EXTRACT,
+  D_IS_NULL, D_IS_NOT_NULL, D_IS_TRUE, D_IS_NOT_TRUE, D_IS_NOT_FALSE,
+  D_TO_I64, D_TO_F64, D_CAST_DATE, D_CAST_TS, D_CHAR_LENGTH, D_RAND,
+  D_NUM_CODES
+};
+
+// DevOp.flags
+enum : uint8_t {
+  F_A_IMM = 1,    // operand a is the literal in imm (already in the op's domain)
+  F_B_IMM = 2,    // operand b is the literal in imm
+  F_A_NULL = 4,   // operand a is a null literal
+  F_B_NULL = 8,   // operand b is a null literal
+  F_A_F = 16,     // operand a holds double bits (else int64)
+  F_B_F = 32,     // operand b holds double bits
+  F_STORE_VALID = 64  // D_STORE: the output column has a validity buffer
+};
+
+typedef struct DevOp {
+  uint8_t code;
+  uint8_t dst;   // destination register
+  uint8_t a;     // register of operand a (D_LOAD/D_LOADV: input column)
+  uint8_t b;     // register of operand b
+  uint8_t flags;
+  uint8_t aux;   // D_STORE: output column; D_EXTRACT: operator; *_SS/CHAR_LENGTH: column of a
+  uint8_t aux2;  // *_TS: (unit of a) | (unit of b) << 4; *_SS: column of b or 0xff;
+                 // D_EXTRACT/D_CAST_DATE/D_CAST_TS: source type id
+  uint8_t pad;
+  int64_t imm;   // literal bits; *_SS: device pointer to {int32 len, bytes}
+} DevOp;
+
+// Input column as the kernel sees it: data already advanced to the view's
+// first row; validity pointer advanced by whole words, vbit = remaining bit.
+typedef struct DevInCol {
+  const void* data;
+  const uint32_t* valid;  // NULL = all valid
+  const int32_t* offsets; // strings
+  int32_t vbit;
+  int16_t type;
+  int16_t aligned16;      // data is 16-byte aligned
+} DevInCol;
+
+typedef struct DevOutCol {
+  void* data;
+  uint32_t* valid;
+  int32_t type;
+  int32_t pad;
+} DevOutCol;
+
+typedef struct DevPlan {
+  int32_t n_ops;
+  int32_t split;   // ops [0, split) decide the row selection, [split, n_ops) write outputs
+  int32_t n_in;
+  int32_t n_out;
+  int64_t num_rows;
+  DevInCol in[BSQL_DEV_MAX_IN];
+  DevOutCol out[BSQL_DEV_MAX_OUT];
+  DevOp ops[BSQL_DEV_MAX_OPS];
+} DevPlan;

@@ -1,0 +1,748 @@
+// parser.cpp -- Calcite expression text -> interpreter program.
+//
+// Host front-end of the interpreter path, following the reference:
+//   replace_calcite_regex        engine/src/parser/expression_utils.cpp:1068-1113
+//   expand_if_logical_op         engine/src/parser/CalciteExpressionParsing.cpp:330-398
+//   fill_minus_op_with_zero      engine/src/parser/expression_utils.cpp:1155-1164
+//   lexer / expr_parser          engine/src/parser/expression_tree.cpp:33-279
+//   literal typing               engine/src/parser/expression_tree.cpp:281-375
+//   custom_op_transformer        engine/src/parser/expression_tree.hpp:181-258
+//   map_to_operator_type         engine/src/parser/expression_utils.cpp:268-352
+//   get_scalar_from_string       engine/src/parser/CalciteExpressionParsing.cpp:138-262
+//   used-column remap            engine/src/execution_kernels/LogicalProject.cpp:859-881
+//   expr_to_plan_visitor         engine/src/Interpreter/interpreter_cpp.cu:274-438
+// The slot numbering is the reference's: inputs 0..n-1 (used columns in table
+// order), final outputs n..n+e-1, temporaries from n+e with first-free reuse.
+#include <algorithm>
+#include <cctype>
+#include <climits>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <regex>
+#include <string>
+#include <vector>
+
+#include "program.hpp"
+
+namespace bsql {
+namespace {
+
+// ------------------------------------------------------------ text rewrites
+
+void replace_all(std::string& data, const std::string& what, const std::string& with) {
+  size_t pos = data.find(what);
+  while (pos != std::string::npos) {
+    data.replace(pos, what.size(), with);
+    pos = data.find(what, pos + with.size());
+  }
+}
+
+std::string replace_calcite_regex(const std::string& expression) {
+  std::string ret = expression;
+  static const std::regex char_collate_re{R"((?:\(\d+\))? CHARACTER SET ".+?" COLLATE ".+?")",
+                                          std::regex_constants::icase};
+  ret = std::regex_replace(ret, char_collate_re, "");
+  static const std::regex timestamp_re{R"(TIMESTAMP\(\d+\))", std::regex_constants::icase};
+  ret = std::regex_replace(ret, timestamp_re, "TIMESTAMP");
+  static const std::regex decimal_re{R"(DECIMAL\(\d+, \d+\))", std::regex_constants::icase};
+  ret = std::regex_replace(ret, decimal_re, "DOUBLE");
+  static const std::regex char_re{R"(:CHAR\(\d+\))", std::regex_constants::icase};
+  ret = std::regex_replace(ret, char_re, ":VARCHAR");
+  replace_all(ret, "IS NOT DISTINCT FROM", "IS_NOT_DISTINCT_FROM");
+  replace_all(ret, "IS NOT NULL", "IS_NOT_NULL");
+  replace_all(ret, "IS NULL", "IS_NULL");
+  replace_all(ret, " NOT NULL", "");
+  replace_all(ret, "RAND()", "BLZ_RND()");
+  replace_all(ret, "EXTRACT(FLAG(YEAR), ", "BL_YEAR(");
+  replace_all(ret, "EXTRACT(FLAG(MONTH), ", "BL_MONTH(");
+  replace_all(ret, "EXTRACT(FLAG(DAY), ", "BL_DAY(");
+  replace_all(ret, "EXTRACT(FLAG(DOW), ", "BL_DOW(");
+  replace_all(ret, "EXTRACT(FLAG(HOUR), ", "BL_HOUR(");
+  replace_all(ret, "EXTRACT(FLAG(MINUTE), ", "BL_MINUTE(");
+  replace_all(ret, "EXTRACT(FLAG(SECOND), ", "BL_SECOND(");
+  replace_all(ret, "DAY TO SECOND", "SECOND");
+  replace_all(ret, "HOUR TO SECOND", "SECOND");
+  replace_all(ret, "MINUTE TO SECOND", "SECOND");
+  replace_all(ret, "/INT(", "/(");
+  return ret;
+}
+
+std::string trim(const std::string& s) {
+  size_t b = 0, e = s.size();
+  while (b < e && std::isspace((unsigned char)s[b])) ++b;
+  while (e > b && std::isspace((unsigned char)s[e - 1])) --e;
+  return s.substr(b, e - b);
+}
+
+// characters inside single-quoted literals ('' is an escaped quote)
+std::vector<bool> quoted_vector(const std::string& s) {
+  std::vector<bool> q(s.size(), false);
+  bool in = false;
+  for (size_t i = 0; i < s.size(); ++i) {
+    if (s[i] == '\'') {
+      if (in && i + 1 < s.size() && s[i + 1] == '\'') { q[i] = q[i + 1] = true; ++i; continue; }
+      in = !in;
+      q[i] = true;
+      continue;
+    }
+    q[i] = in;
+  }
+  return q;
+}
+
+int find_not_in_quotes(const std::string& hay, const std::string& needle, size_t pos) {
+  std::vector<bool> q = quoted_vector(hay);
+  while (pos < hay.size()) {
+    pos = hay.find(needle, pos);
+    if (pos == std::string::npos) return -1;
+    if (!q[pos]) return (int)pos;
+    pos += needle.size();
+  }
+  return -1;
+}
+
+int find_closing(const std::string& e, int start) {
+  const char open = e[(size_t)start];
+  const char close = open == '(' ? ')' : (open == '[' ? ']' : 0);
+  if (!close) return -1;
+  int depth = 1;
+  bool in_q = false;
+  for (size_t i = (size_t)start + 1; i < e.size(); ++i) {
+    if (in_q) {
+      if (e[i] == '\'' && !(i + 1 < e.size() && e[i + 1] == '\'')) in_q = false;
+    } else if (e[i] == '\'') {
+      in_q = true;
+    } else if (e[i] == open) {
+      ++depth;
+    } else if (e[i] == close) {
+      if (--depth == 0) return (int)i;
+    }
+  }
+  return -1;
+}
+
+}  // namespace
+
+// splits a comma separated expression list at depth 0 (expression_utils.cpp:1012-1066)
+std::vector<std::string> split_expression_list(const std::string& combined, bool do_trim) {
+  std::vector<std::string> out;
+  size_t cur = 0, start = 0;
+  bool in_q = false;
+  int paren = 0, sq = 0;
+  while (cur < combined.size()) {
+    const char c = combined[cur];
+    if (in_q) {
+      if (c == '\'' && !(cur + 1 < combined.size() && combined[cur + 1] == '\'')) in_q = false;
+    } else if (c == '\'') {
+      in_q = true;
+    } else if (c == '(') {
+      ++paren;
+    } else if (c == ')') {
+      --paren;
+    } else if (c == '[') {
+      ++sq;
+    } else if (c == ']') {
+      --sq;
+    } else if (c == ',' && paren == 0 && sq == 0) {
+      std::string e = combined.substr(start, cur - start);
+      if (do_trim) {
+        size_t b = 0;
+        while (b < e.size() && std::isspace((unsigned char)e[b])) ++b;
+        e = e.substr(b);
+      }
+      out.push_back(e);
+      start = cur + 1;
+    }
+    ++cur;
+  }
+  if (start < combined.size()) {
+    std::string e = combined.substr(start);
+    out.push_back(do_trim ? trim(e) : e);
+  }
+  return out;
+}
+
+namespace {
+
+// N-ary AND/OR -> left-deep binary chain in plan order
+std::string expand_if_logical_op(std::string expression) {
+  std::string output = expression;
+  size_t start_pos = 0;
+  while (start_pos < expression.size()) {
+    const int first_and = find_not_in_quotes(expression, "AND(", start_pos);
+    int first_or = -1;
+    if (expression.find("FLOOR") == std::string::npos)
+      first_or = find_not_in_quotes(expression, "OR(", start_pos);
+    int first;
+    std::string op;
+    if (first_and >= 0) {
+      if (first_or >= 0 && first_or < first_and) { first = first_or; op = "OR("; }
+      else { first = first_and; op = "AND("; }
+    } else {
+      first = first_or;
+      op = "OR(";
+    }
+    if (first < 0) return output;
+    const int ex_start = first + (int)op.size() - 1;
+    const int ex_end = find_closing(expression, ex_start);
+    if (ex_end < 0) fail("unbalanced parentheses in expression");
+    const std::string rest = expression.substr((size_t)ex_start + 1, (size_t)(ex_end - ex_start - 1));
+    std::vector<std::string> parts = split_expression_list(rest, false);
+    if (parts.size() == 2) {
+      start_pos = (size_t)ex_start;
+      continue;
+    }
+    if (parts.size() < 2) fail("logical operator with fewer than two operands");
+    start_pos = (size_t)first;
+    output = expression.substr(0, (size_t)first);
+    for (size_t i = 0; i + 1 < parts.size(); ++i) {
+      output += op;
+      start_pos += op.size();
+    }
+    output += parts[0] + ",";
+    for (size_t i = 1; i + 1 < parts.size(); ++i) output += parts[i] + "),";
+    output += parts.back() + ")";
+    if ((size_t)ex_end < expression.size() - 1) output += expression.substr((size_t)ex_end + 1);
+    expression = output;
+  }
+  return output;
+}
+
+std::string fill_minus_op_with_zero(std::string e) {
+  const size_t vars = (size_t)std::count(e.begin(), e.end(), '$');
+  if (!e.empty() && e[0] == '-' && vars == 1 && e.find(',') == std::string::npos)
+    e = e.substr(0, 2) + "0, " + e.substr(2);
+  return e;
+}
+
+// ------------------------------------------------------------------- lexer
+
+enum class Tok {
+  Open, Close, Comma, Colon, Variable, Null, Boolean, Number, Duration_s, Duration_ms,
+  Duration_us, Duration_ns, Timestamp_d, Timestamp_s, Timestamp_ms, Timestamp_us, Timestamp_ns,
+  String, Identifier, Eof
+};
+
+struct Token {
+  Tok type;
+  std::string value;
+};
+
+class Lexer {
+ public:
+  explicit Lexer(const std::string& s) : text_(trim(s)), pos_(0) {}
+
+  Token next() {
+    if (pos_ >= text_.size()) return {Tok::Eof, ""};
+    while (pos_ < text_.size() && text_[pos_] == ' ') ++pos_;
+    if (pos_ >= text_.size()) return {Tok::Eof, ""};
+    const char c = text_[pos_];
+    if (c == '(') { ++pos_; return {Tok::Open, "("}; }
+    if (c == ')') { ++pos_; return {Tok::Close, ")"}; }
+    if (c == ',') { ++pos_; return {Tok::Comma, ","}; }
+    if (c == ':') { ++pos_; return {Tok::Colon, ":"}; }
+    static const std::regex variable_re{R"(^\$\d+)"};
+    static const std::regex null_re{R"(^null)"};
+    static const std::regex boolean_re{R"(^(true|false))"};
+    static const std::regex number_re{R"(^[-+]?\d*\.?\d+([eE][-+]?\d+)?)"};
+    static const std::regex dur_s_re{R"(^\d{2}:\d{2}:\d{2})"};
+    static const std::regex dur_ms_re{R"(^\d{2}:\d{2}:\d{2}.\d{3})"};
+    static const std::regex ts_d_re{R"(^\d{4}-\d{2}-\d{2})"};
+    static const std::regex ts_s_re{R"(^\d{4}-\d{2}-\d{2}(?:[ T]?\d{2}:\d{2}:\d{2}))"};
+    static const std::regex ts_ms_re{R"(^\d{4}-\d{2}-\d{2}(?:[ T]?\d{2}:\d{2}:\d{2}.\d{3}))"};
+    static const std::regex ts_us_re{R"(^\d{4}-\d{2}-\d{2}(?:[ T]?\d{2}:\d{2}:\d{2}.\d{6}))"};
+    static const std::regex string_re{R"(^(["'])(?:(?!\1|\\).|\\.)*?\1)"};
+    const std::string rem = text_.substr(pos_);
+    std::smatch m;
+    auto take = [&](const std::regex& re, Tok t, Token* out) {
+      if (!std::regex_search(rem, m, re)) return false;
+      pos_ += (size_t)m.length(0);
+      *out = {t, m.str(0)};
+      return true;
+    };
+    Token t;
+    // same order and patterns as the reference lexer, including its reuse of
+    // the millisecond duration pattern for us/ns and of the us timestamp
+    // pattern for ns (expression_tree.hpp:319-332)
+    if (take(variable_re, Tok::Variable, &t)) return t;
+    if (take(null_re, Tok::Null, &t)) return t;
+    if (take(boolean_re, Tok::Boolean, &t)) return t;
+    if (take(dur_s_re, Tok::Duration_s, &t)) return t;
+    if (take(dur_ms_re, Tok::Duration_ms, &t)) return t;
+    if (take(dur_ms_re, Tok::Duration_us, &t)) return t;
+    if (take(dur_ms_re, Tok::Duration_ms, &t)) return t;
+    if (take(ts_us_re, Tok::Timestamp_ns, &t)) return t;
+    if (take(ts_us_re, Tok::Timestamp_us, &t)) return t;
+    if (take(ts_ms_re, Tok::Timestamp_ms, &t)) return t;
+    if (take(ts_s_re, Tok::Timestamp_s, &t)) return t;
+    if (take(ts_d_re, Tok::Timestamp_d, &t)) return t;
+    if (take(number_re, Tok::Number, &t)) return t;
+    if (take(string_re, Tok::String, &t)) return t;
+    size_t len = 0;
+    do {
+      const char ch = text_[pos_ + len];
+      if (ch == '(' || ch == ')' || ch == ',' || ch == ':') break;
+    } while (pos_ + (++len) < text_.size());
+    std::string v = text_.substr(pos_, len);
+    pos_ += len;
+    return {Tok::Identifier, v};
+  }
+
+ private:
+  std::string text_;
+  size_t pos_;
+};
+
+// ------------------------------------------------------------------- tree
+
+enum class NodeType { Operator, Literal, Variable };
+
+struct Node {
+  NodeType type;
+  std::string value;
+  int32_t lit_type = BSQL_EMPTY;  // literals
+  int var_index = -1;             // variables
+  std::vector<std::unique_ptr<Node>> children;
+};
+
+int32_t infer_literal_type(const Token& t) {
+  switch (t.type) {
+    case Tok::Null: return BSQL_EMPTY;
+    case Tok::Boolean: return BSQL_BOOL8;
+    case Tok::Number:
+      if (t.value.find_first_of(".eE") != std::string::npos) {
+        const double d = std::stod(t.value);
+        return d == (double)(float)d ? BSQL_FLOAT32 : BSQL_FLOAT64;
+      }
+      return std::stoll(t.value) > INT_MAX ? BSQL_INT64 : BSQL_INT32;
+    case Tok::Duration_s: return BSQL_DURATION_SECONDS;
+    case Tok::Duration_ms: return BSQL_DURATION_MILLISECONDS;
+    case Tok::Duration_us: return BSQL_DURATION_MICROSECONDS;
+    case Tok::Duration_ns: return BSQL_DURATION_NANOSECONDS;
+    case Tok::Timestamp_ns: return BSQL_TIMESTAMP_NANOSECONDS;
+    case Tok::Timestamp_us: return BSQL_TIMESTAMP_MICROSECONDS;
+    case Tok::Timestamp_ms: return BSQL_TIMESTAMP_MILLISECONDS;
+    case Tok::Timestamp_s: return BSQL_TIMESTAMP_SECONDS;
+    case Tok::Timestamp_d: return BSQL_TIMESTAMP_DAYS;
+    default: return BSQL_STRING;
+  }
+}
+
+int32_t type_from_type_token(const std::string& v) {
+  if (v == "NULL" || v == "BOOLEAN") return BSQL_BOOL8;
+  if (v == "TINYINT") return BSQL_INT8;
+  if (v == "SMALLINT") return BSQL_INT16;
+  if (v == "INTEGER") return BSQL_INT32;
+  if (v == "INTERVAL SECOND" || v == "INTERVAL MINUTE" || v == "INTERVAL HOUR" ||
+      v == "INTERVAL DAY")
+    return BSQL_DURATION_SECONDS;
+  if (v == "INTERVAL MONTH" || v == "INTERVAL YEAR")
+    fail("operation over MONTH or YEAR units are not currently supported.");
+  if (v == "BIGINT") return BSQL_INT64;
+  if (v == "FLOAT") return BSQL_FLOAT32;
+  if (v == "DOUBLE") return BSQL_FLOAT64;
+  if (v == "DATE") return BSQL_TIMESTAMP_DAYS;
+  if (v == "TIMESTAMP_SECONDS") return BSQL_TIMESTAMP_SECONDS;
+  if (v == "TIMESTAMP_MILLISECONDS") return BSQL_TIMESTAMP_MILLISECONDS;
+  if (v == "TIMESTAMP_MICROSECONDS") return BSQL_TIMESTAMP_MICROSECONDS;
+  if (v == "TIMESTAMP") return BSQL_TIMESTAMP_NANOSECONDS;
+  if (v == "VARCHAR") return BSQL_STRING;
+  fail("Invalid literal cast type");
+}
+
+class Parser {
+ public:
+  explicit Parser(const std::string& s) : lex_(s) {}
+  std::unique_ptr<Node> parse() {
+    tok_ = lex_.next();
+    auto n = expr();
+    return n;
+  }
+
+ private:
+  bool accept(Tok t) {
+    if (tok_.type == t) { tok_ = lex_.next(); return true; }
+    return false;
+  }
+  std::unique_ptr<Node> expr() {
+    auto r = term();
+    if (!r) r = func();
+    if (!r) fail("Couldn't parse calcite expression");
+    return r;
+  }
+  std::unique_ptr<Node> term() {
+    Token v = tok_;
+    if (accept(Tok::Variable)) {
+      auto n = std::make_unique<Node>();
+      n->type = NodeType::Variable;
+      n->value = v.value;
+      n->var_index = std::stoi(v.value.substr(1));
+      return n;
+    }
+    return literal();
+  }
+  std::unique_ptr<Node> func() {
+    Token name = tok_;
+    if (!accept(Tok::Identifier)) return nullptr;
+    accept(Tok::Open);
+    std::vector<std::unique_ptr<Node>> args;
+    if (!accept(Tok::Close)) {
+      args.push_back(expr());
+      while (accept(Tok::Comma)) args.push_back(expr());
+      accept(Tok::Close);
+    }
+    std::string id = name.value;
+    if (accept(Tok::Colon)) {
+      Token rt = tok_;
+      accept(Tok::Identifier);
+      id += "_" + rt.value;
+    }
+    auto n = std::make_unique<Node>();
+    n->type = NodeType::Operator;
+    n->value = id;
+    n->children = std::move(args);
+    return n;
+  }
+  std::unique_ptr<Node> literal() {
+    Token lit = tok_;
+    const Tok t = tok_.type;
+    if (t == Tok::Null || t == Tok::Boolean || t == Tok::Number || t == Tok::Duration_s ||
+        t == Tok::Duration_ms || t == Tok::Duration_us || t == Tok::Duration_ns ||
+        t == Tok::Timestamp_d || t == Tok::Timestamp_s || t == Tok::Timestamp_ms ||
+        t == Tok::Timestamp_us || t == Tok::Timestamp_ns || t == Tok::String) {
+      accept(t);
+      auto n = std::make_unique<Node>();
+      n->type = NodeType::Literal;
+      n->value = lit.value;
+      if (accept(Tok::Colon)) {
+        Token tt = tok_;
+        accept(Tok::Identifier);
+        n->lit_type = type_from_type_token(tt.value);
+      } else {
+        n->lit_type = infer_literal_type(lit);
+      }
+      return n;
+    }
+    return nullptr;
+  }
+  Lexer lex_;
+  Token tok_;
+};
+
+// ------------------------------------------------------------- operators
+
+int map_to_operator(const std::string& tok) {
+  static const std::map<std::string, int> M = {
+      {"BLZ_RND", BSQL_OP_RAND},
+      {"NOT", BSQL_OP_NOT}, {"IS NOT TRUE", BSQL_OP_IS_NOT_TRUE},
+      {"IS NOT FALSE", BSQL_OP_IS_NOT_FALSE}, {"IS TRUE", BSQL_OP_IS_TRUE},
+      {"SIN", BSQL_OP_SIN}, {"ASIN", BSQL_OP_ASIN}, {"COS", BSQL_OP_COS}, {"ACOS", BSQL_OP_ACOS},
+      {"TAN", BSQL_OP_TAN}, {"ATAN", BSQL_OP_ATAN}, {"FLOOR", BSQL_OP_FLOOR},
+      {"CEIL", BSQL_OP_CEIL}, {"ABS", BSQL_OP_ABS}, {"LOG10", BSQL_OP_LOG}, {"LN", BSQL_OP_LN},
+      {"BL_YEAR", BSQL_OP_YEAR}, {"BL_MONTH", BSQL_OP_MONTH}, {"BL_DAY", BSQL_OP_DAY},
+      {"BL_DOW", BSQL_OP_DAYOFWEEK}, {"BL_HOUR", BSQL_OP_HOUR}, {"BL_MINUTE", BSQL_OP_MINUTE},
+      {"BL_SECOND", BSQL_OP_SECOND}, {"IS_NULL", BSQL_OP_IS_NULL},
+      {"IS_NOT_NULL", BSQL_OP_IS_NOT_NULL}, {"CAST_TINYINT", BSQL_OP_CAST_TINYINT},
+      {"CAST_SMALLINT", BSQL_OP_CAST_SMALLINT}, {"CAST_INTEGER", BSQL_OP_CAST_INTEGER},
+      {"CAST_BIGINT", BSQL_OP_CAST_BIGINT}, {"CAST_FLOAT", BSQL_OP_CAST_FLOAT},
+      {"CAST_DOUBLE", BSQL_OP_CAST_DOUBLE}, {"CAST_DATE", BSQL_OP_CAST_DATE},
+      {"CAST_TIMESTAMP_SECONDS", BSQL_OP_CAST_TIMESTAMP_SECONDS},
+      {"CAST_TIMESTAMP_MILLISECONDS", BSQL_OP_CAST_TIMESTAMP_MILLISECONDS},
+      {"CAST_TIMESTAMP_MICROSECONDS", BSQL_OP_CAST_TIMESTAMP_MICROSECONDS},
+      {"CAST_TIMESTAMP", BSQL_OP_CAST_TIMESTAMP}, {"CAST_VARCHAR", BSQL_OP_CAST_VARCHAR},
+      {"CAST_CHAR", BSQL_OP_CAST_VARCHAR}, {"CHAR_LENGTH", BSQL_OP_CHAR_LENGTH},
+      {"LOWER", BSQL_OP_STR_LOWER}, {"UPPER", BSQL_OP_STR_UPPER},
+      {"INITCAP", BSQL_OP_STR_INITCAP}, {"REVERSE", BSQL_OP_STR_REVERSE},
+      {"=", BSQL_OP_EQUAL}, {"<>", BSQL_OP_NOT_EQUAL}, {">", BSQL_OP_GREATER},
+      {">=", BSQL_OP_GREATER_EQUAL}, {"<", BSQL_OP_LESS}, {"<=", BSQL_OP_LESS_EQUAL},
+      {"+", BSQL_OP_ADD}, {"-", BSQL_OP_SUB}, {"*", BSQL_OP_MUL}, {"/", BSQL_OP_DIV},
+      {"POWER", BSQL_OP_POW}, {"ROUND", BSQL_OP_ROUND}, {"MOD", BSQL_OP_MOD},
+      {"AND", BSQL_OP_LOGICAL_AND}, {"OR", BSQL_OP_LOGICAL_OR},
+      {"FIRST_NON_MAGIC", BSQL_OP_FIRST_NON_MAGIC}, {"MAGIC_IF_NOT", BSQL_OP_MAGIC_IF_NOT},
+      {"LIKE", BSQL_OP_STR_LIKE}, {"SUBSTRING", BSQL_OP_STR_SUBSTRING},
+      {"REGEXP_REPLACE", BSQL_OP_STR_REGEXP_REPLACE}, {"REPLACE", BSQL_OP_STR_REPLACE},
+      {"TO_DATE", BSQL_OP_TO_DATE}, {"TO_TIMESTAMP", BSQL_OP_TO_TIMESTAMP},
+      {"||", BSQL_OP_STR_CONCAT}, {"CONCAT", BSQL_OP_STR_CONCAT}, {"TRIM", BSQL_OP_STR_TRIM},
+      {"LEFT", BSQL_OP_STR_LEFT}, {"RIGHT", BSQL_OP_STR_RIGHT},
+      {"IS_NOT_DISTINCT_FROM", BSQL_OP_IS_NOT_DISTINCT_FROM},
+  };
+  auto it = M.find(tok);
+  if (it == M.end()) fail("Unsupported operator: " + tok);
+  return it->second;
+}
+
+// CASE / Reinterpret / ROUND / CAST(literal) rewrites, bottom-up
+std::unique_ptr<Node> transform_custom(std::unique_ptr<Node> n) {
+  if (n->type != NodeType::Operator) return n;
+  for (auto& c : n->children) c = transform_custom(std::move(c));
+  if (n->value == "CASE") {
+    if (n->children.size() < 3 || n->children.size() % 2 == 0) fail("malformed CASE");
+    // CASE(c1, v1, c2, v2, ..., e) -> FIRST_NON_MAGIC(MAGIC_IF_NOT(c1, v1), ...)
+    std::unique_ptr<Node> acc = std::move(n->children.back());
+    for (int i = (int)n->children.size() - 3; i >= 0; i -= 2) {
+      auto mif = std::make_unique<Node>();
+      mif->type = NodeType::Operator;
+      mif->value = "MAGIC_IF_NOT";
+      mif->children.push_back(std::move(n->children[(size_t)i]));
+      mif->children.push_back(std::move(n->children[(size_t)i + 1]));
+      auto fnm = std::make_unique<Node>();
+      fnm->type = NodeType::Operator;
+      fnm->value = "FIRST_NON_MAGIC";
+      fnm->children.push_back(std::move(mif));
+      fnm->children.push_back(std::move(acc));
+      acc = std::move(fnm);
+    }
+    return acc;
+  }
+  if (n->value == "Reinterpret") {
+    if (n->children.size() != 1) fail("malformed Reinterpret");
+    return std::move(n->children[0]);
+  }
+  if (n->value == "ROUND") {
+    if (n->children.size() == 1) {
+      auto z = std::make_unique<Node>();
+      z->type = NodeType::Literal;
+      z->value = "0";
+      z->lit_type = BSQL_INT8;
+      n->children.push_back(std::move(z));
+    }
+    return n;
+  }
+  if (n->value.rfind("CAST", 0) == 0) {
+    if (n->children.size() != 1) fail("malformed CAST");
+    const int cast_op = map_to_operator(n->value);
+    if (n->children[0]->type == NodeType::Literal) {
+      std::unique_ptr<Node> lit = std::move(n->children[0]);
+      lit->lit_type = output_type_unary(cast_op, lit->lit_type);
+      return lit;
+    }
+  }
+  return n;
+}
+
+// ---------------------------------------------------------- literal values
+
+bool is_null_literal(const std::string& v) { return v == "null"; }
+
+int64_t days_from_civil(int64_t y, unsigned m, unsigned d) {
+  y -= m <= 2;
+  const int64_t era = (y >= 0 ? y : y - 399) / 400;
+  const unsigned yoe = (unsigned)(y - era * 400);
+  const unsigned doy = (153 * (m + (m > 2 ? -3 : 9)) + 2) / 5 + d - 1;
+  const unsigned doe = yoe * 365 + yoe / 4 - yoe / 100 + doy;
+  return era * 146097 + (int64_t)doe - 719468;
+}
+
+// "%Y-%m-%d[ %H:%M:%S[.%f]]" (or with '/') -> ticks of the timestamp type
+int64_t parse_timestamp(const std::string& raw, int32_t type) {
+  std::string s = raw;
+  if (!s.empty() && s.front() == '\'') s = s.substr(1);
+  if (!s.empty() && s.back() == '\'') s.pop_back();
+  static const std::regex re{
+      R"(^(\d{4})[-/](\d{2})[-/](\d{2})(?:[ T](\d{2}):(\d{2}):(\d{2})(?:\.(\d{1,9}))?)?Z?$)"};
+  std::smatch m;
+  if (!std::regex_match(s, m, re)) fail("not recognized timestamp literal: " + raw);
+  const int64_t days = days_from_civil(std::stoll(m[1]), (unsigned)std::stoi(m[2]),
+                                       (unsigned)std::stoi(m[3]));
+  int64_t secs = 0, nanos = 0;
+  if (m[4].matched)
+    secs = std::stoll(m[4]) * 3600 + std::stoll(m[5]) * 60 + std::stoll(m[6]);
+  if (m[7].matched) {
+    std::string f = m[7].str();
+    f.resize(9, '0');
+    nanos = std::stoll(f);
+  }
+  switch (type) {
+    case BSQL_TIMESTAMP_DAYS: return days;
+    case BSQL_TIMESTAMP_SECONDS: return days * 86400 + secs;
+    case BSQL_TIMESTAMP_MILLISECONDS: return (days * 86400 + secs) * 1000 + nanos / 1000000;
+    case BSQL_TIMESTAMP_MICROSECONDS: return (days * 86400 + secs) * 1000000 + nanos / 1000;
+    default: return (days * 86400 + secs) * 1000000000LL + nanos;
+  }
+}
+
+Scalar scalar_from_string(const std::string& v, int32_t type) {
+  Scalar s;
+  s.type_id = type;
+  s.is_null = false;
+  switch (type) {
+    case BSQL_BOOL8: s.i64 = v == "true"; break;
+    case BSQL_INT8: s.i64 = (int8_t)std::stoi(v); break;
+    case BSQL_INT16: s.i64 = (int16_t)std::stoi(v); break;
+    case BSQL_INT32: s.i64 = (int32_t)std::stoi(v); break;
+    case BSQL_INT64: s.i64 = (int64_t)std::stoll(v); break;
+    case BSQL_UINT8: s.i64 = (uint8_t)std::stoul(v); break;
+    case BSQL_UINT16: s.i64 = (uint16_t)std::stoul(v); break;
+    case BSQL_UINT32: s.i64 = (uint32_t)std::stoul(v); break;
+    case BSQL_UINT64: s.i64 = (int64_t)std::stoull(v); break;
+    case BSQL_FLOAT32: s.f64 = (double)std::stof(v); break;
+    case BSQL_FLOAT64: s.f64 = std::stod(v); break;
+    case BSQL_STRING: s.str = v.size() >= 2 ? v.substr(1, v.size() - 2) : v; break;
+    case BSQL_DURATION_SECONDS: case BSQL_DURATION_MILLISECONDS:
+    case BSQL_DURATION_MICROSECONDS: case BSQL_DURATION_NANOSECONDS:
+      s.i64 = std::stoll(v); break;
+    case BSQL_TIMESTAMP_DAYS: case BSQL_TIMESTAMP_SECONDS: case BSQL_TIMESTAMP_MILLISECONDS:
+    case BSQL_TIMESTAMP_MICROSECONDS: case BSQL_TIMESTAMP_NANOSECONDS:
+      s.i64 = parse_timestamp(v, type); break;
+    default: fail("unsupported literal type " + std::to_string(type));
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------- encoder
+
+struct Encoder {
+  Program& p;
+  const std::map<int, int>& col_map;
+  int start;
+  std::vector<bool> free_;
+  std::map<const Node*, int> pos_;
+
+  Encoder(Program& prog, const std::map<int, int>& m, int start_pos)
+      : p(prog), col_map(m), start(start_pos), free_(512, true) {
+    std::fill_n(free_.begin(), start_pos, false);
+  }
+
+  int first_free() {
+    for (size_t i = (size_t)start; i < free_.size(); ++i)
+      if (free_[i]) { free_[i] = false; return (int)i; }
+    return -1;
+  }
+
+  void push(int l, int r, const Scalar& ls, const Scalar& rs, int op) {
+    p.left.push_back((int16_t)l);
+    p.right.push_back((int16_t)r);
+    p.lsc.push_back(ls);
+    p.rsc.push_back(rs);
+    p.ops.push_back(op);
+  }
+
+  void visit(const Node* n) {
+    for (auto& c : n->children) visit(c.get());
+    if (n->type == NodeType::Literal) { pos_[n] = BSQL_SCALAR_INDEX; return; }
+    if (n->type == NodeType::Variable) {
+      auto it = col_map.find(n->var_index);
+      if (it == col_map.end()) fail("column $" + std::to_string(n->var_index) + " not in the table");
+      pos_[n] = it->second;
+      return;
+    }
+    const int op = map_to_operator(n->value);
+    const Scalar none;
+    auto literal_scalar = [&](const Node* lit, int* code) {
+      if (is_null_literal(lit->value)) { *code = BSQL_SCALAR_NULL_INDEX; return Scalar(); }
+      *code = BSQL_SCALAR_INDEX;
+      return scalar_from_string(lit->value, lit->lit_type);
+    };
+    if (is_binary_op(op)) {
+      if (n->children.size() != 2) fail("operator " + n->value + " expects two operands");
+      const Node* L = n->children[0].get();
+      const Node* R = n->children[1].get();
+      const int lp = pos_.at(L), rp = pos_.at(R);
+      if (L->type != NodeType::Literal && lp >= start) free_[(size_t)lp] = true;
+      if (R->type != NodeType::Literal && rp >= start) free_[(size_t)rp] = true;
+      if (L->type == NodeType::Literal && R->type == NodeType::Literal)
+        fail("Operations between literals is not supported");
+      if (L->type == NodeType::Literal) {
+        int code;
+        Scalar s = literal_scalar(L, &code);
+        push(code, rp, s, none, op);
+      } else if (R->type == NodeType::Literal) {
+        int code;
+        Scalar s = literal_scalar(R, &code);
+        push(lp, code, none, s, op);
+      } else {
+        push(lp, rp, none, none, op);
+      }
+    } else if (is_unary_op(op)) {
+      if (n->children.size() != 1) fail("operator " + n->value + " expects one operand");
+      const Node* L = n->children[0].get();
+      if (L->type == NodeType::Literal) fail("Unary operations on literals is not supported");
+      const int lp = pos_.at(L);
+      if (lp >= start) free_[(size_t)lp] = true;
+      push(lp, BSQL_UNARY_INDEX, none, none, op);
+    } else {
+      if (!n->children.empty()) fail("operator " + n->value + " takes no operands");
+      push(BSQL_NULLARY_INDEX, BSQL_NULLARY_INDEX, none, none, op);
+    }
+    const int out = first_free();
+    pos_[n] = out;
+    p.out.push_back((int16_t)out);
+  }
+};
+
+void collect_vars(const Node* n, std::vector<int>& acc) {
+  if (n->type == NodeType::Variable) acc.push_back(n->var_index);
+  for (auto& c : n->children) collect_vars(c.get(), acc);
+}
+
+bool is_string_function(const std::string& v) {
+  static const char* names[] = {"LIKE", "SUBSTRING", "REGEXP_REPLACE", "REPLACE", "TO_DATE",
+                                "TO_TIMESTAMP", "||", "CONCAT", "TRIM", "LEFT", "RIGHT",
+                                "LOWER", "UPPER", "INITCAP", "REVERSE", "CAST_VARCHAR",
+                                "CAST_CHAR"};
+  for (const char* s : names)
+    if (v == s) return true;
+  return false;
+}
+
+void check_interpretable(const Node* n) {
+  if (n->type == NodeType::Operator && is_string_function(n->value))
+    fail("string function " + n->value +
+         " is evaluated before the interpreter (see the string column functions)");
+  for (auto& c : n->children) check_interpretable(c.get());
+}
+
+}  // namespace
+
+std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exprs,
+                                             const int32_t* in_types, int n_in) {
+  std::vector<std::unique_ptr<Node>> trees;
+  std::vector<bool> used((size_t)std::max(n_in, 0), false);
+  for (const std::string& raw : exprs) {
+    std::string e = replace_calcite_regex(fill_minus_op_with_zero(raw));
+    e = expand_if_logical_op(e);
+    Parser parser(e);
+    std::unique_ptr<Node> t = transform_custom(parser.parse());
+    check_interpretable(t.get());
+    if (t->type == NodeType::Literal)
+      fail("literal expression: the operator layer fills it without the interpreter");
+    std::vector<int> vars;
+    collect_vars(t.get(), vars);
+    for (int v : vars) {
+      if (v < 0 || v >= n_in) fail("column $" + std::to_string(v) + " not in the table");
+      used[(size_t)v] = true;
+    }
+    trees.push_back(std::move(t));
+  }
+  auto prog = std::make_unique<Program>();
+  std::map<int, int> col_map;
+  for (int i = 0; i < n_in; ++i)
+    if (used[(size_t)i]) {
+      const int slot = (int)col_map.size();
+      col_map[i] = slot;
+      prog->used_columns.push_back(i);
+    }
+  const int n_inputs = (int)col_map.size();
+  const int n_exprs = (int)trees.size();
+  prog->n_inputs = n_inputs;
+  for (int i = 0; i < n_exprs; ++i) {
+    const int final_pos = n_inputs + i;
+    prog->final_out.push_back((int16_t)final_pos);
+    const Node* root = trees[(size_t)i].get();
+    if (root->type == NodeType::Variable) continue;  // pass-through: final slot = input slot
+    Encoder enc(*prog, col_map, n_inputs + n_exprs);
+    enc.visit(root);
+    prog->out.back() = (int16_t)final_pos;
+  }
+  // pass-through expressions read their column's slot directly
+  for (int i = 0; i < n_exprs; ++i) {
+    const Node* root = trees[(size_t)i].get();
+    if (root->type == NodeType::Variable) prog->final_out[(size_t)i] = (int16_t)col_map.at(root->var_index);
+  }
+  validate(*prog, n_inputs);
+  (void)in_types;
+  return prog;
+}
+
+}  // namespace bsql

@@ -1,0 +1,690 @@
+// program.cpp -- type rules, validation and lowering of interpreter programs
+// to the gfx950 device plan.
+#include "program.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <set>
+
+namespace bsql {
+
+bool is_float_type(int t) { return t == BSQL_FLOAT32 || t == BSQL_FLOAT64; }
+bool is_timestamp_type(int t) { return t >= BSQL_TIMESTAMP_DAYS && t <= BSQL_TIMESTAMP_NANOSECONDS; }
+bool is_duration_type(int t) { return t >= BSQL_DURATION_DAYS && t <= BSQL_DURATION_NANOSECONDS; }
+bool is_string_type(int t) { return t == BSQL_STRING; }
+
+int type_size(int t) {
+  switch (t) {
+    case BSQL_INT8: case BSQL_UINT8: case BSQL_BOOL8: return 1;
+    case BSQL_INT16: case BSQL_UINT16: return 2;
+    case BSQL_INT32: case BSQL_UINT32: case BSQL_FLOAT32: case BSQL_TIMESTAMP_DAYS:
+    case BSQL_DURATION_DAYS: case BSQL_DECIMAL32: case BSQL_DICTIONARY32: return 4;
+    case BSQL_INT64: case BSQL_UINT64: case BSQL_FLOAT64: case BSQL_TIMESTAMP_SECONDS:
+    case BSQL_TIMESTAMP_MILLISECONDS: case BSQL_TIMESTAMP_MICROSECONDS:
+    case BSQL_TIMESTAMP_NANOSECONDS: case BSQL_DURATION_SECONDS:
+    case BSQL_DURATION_MILLISECONDS: case BSQL_DURATION_MICROSECONDS:
+    case BSQL_DURATION_NANOSECONDS: case BSQL_DECIMAL64: return 8;
+    default: return 0;
+  }
+}
+
+int64_t ticks_per_day(int t) {
+  switch (t) {
+    case BSQL_TIMESTAMP_DAYS: case BSQL_DURATION_DAYS: return 1;
+    case BSQL_TIMESTAMP_SECONDS: case BSQL_DURATION_SECONDS: return 86400LL;
+    case BSQL_TIMESTAMP_MILLISECONDS: case BSQL_DURATION_MILLISECONDS: return 86400000LL;
+    case BSQL_TIMESTAMP_MICROSECONDS: case BSQL_DURATION_MICROSECONDS: return 86400000000LL;
+    default: return 86400000000000LL;
+  }
+}
+
+int64_t ns_per_tick(int t) { return 86400000000000LL / ticks_per_day(t); }
+
+bool is_nullary_op(int op) { return op == BSQL_OP_RAND; }
+bool is_unary_op(int op) { return op >= BSQL_OP_NOT && op <= BSQL_OP_STR_REVERSE; }
+bool is_binary_op(int op) { return op >= BSQL_OP_ADD && op <= BSQL_OP_IS_NOT_DISTINCT_FROM; }
+
+int output_type_nullary(int op) { return op == BSQL_OP_RAND ? BSQL_FLOAT64 : BSQL_EMPTY; }
+
+int output_type_unary(int op, int lt) {
+  switch (op) {
+    case BSQL_OP_CAST_TINYINT: return BSQL_INT8;
+    case BSQL_OP_CAST_SMALLINT: return BSQL_INT16;
+    case BSQL_OP_CAST_INTEGER: return BSQL_INT32;
+    case BSQL_OP_CAST_BIGINT: return BSQL_INT64;
+    case BSQL_OP_CAST_FLOAT: return BSQL_FLOAT32;
+    case BSQL_OP_CAST_DOUBLE: return BSQL_FLOAT64;
+    case BSQL_OP_CAST_DATE: return BSQL_TIMESTAMP_DAYS;
+    case BSQL_OP_CAST_TIMESTAMP_SECONDS: return BSQL_TIMESTAMP_SECONDS;
+    case BSQL_OP_CAST_TIMESTAMP_MILLISECONDS: return BSQL_TIMESTAMP_MILLISECONDS;
+    case BSQL_OP_CAST_TIMESTAMP_MICROSECONDS: return BSQL_TIMESTAMP_MICROSECONDS;
+    case BSQL_OP_CAST_TIMESTAMP: return BSQL_TIMESTAMP_NANOSECONDS;
+    case BSQL_OP_STR_LOWER: case BSQL_OP_STR_UPPER: case BSQL_OP_STR_INITCAP:
+    case BSQL_OP_STR_REVERSE: case BSQL_OP_CAST_VARCHAR: return BSQL_STRING;
+    case BSQL_OP_YEAR: case BSQL_OP_MONTH: case BSQL_OP_DAY: case BSQL_OP_DAYOFWEEK:
+    case BSQL_OP_HOUR: case BSQL_OP_MINUTE: case BSQL_OP_SECOND: return BSQL_INT16;
+    case BSQL_OP_SIN: case BSQL_OP_COS: case BSQL_OP_ASIN: case BSQL_OP_ACOS: case BSQL_OP_TAN:
+    case BSQL_OP_COTAN: case BSQL_OP_ATAN: case BSQL_OP_LN: case BSQL_OP_LOG: case BSQL_OP_FLOOR:
+    case BSQL_OP_CEIL:
+      return is_float_type(lt) ? lt : BSQL_FLOAT64;
+    case BSQL_OP_ABS: return lt;
+    case BSQL_OP_NOT: case BSQL_OP_IS_TRUE: case BSQL_OP_IS_NULL: case BSQL_OP_IS_NOT_NULL:
+    case BSQL_OP_IS_NOT_TRUE: case BSQL_OP_IS_NOT_FALSE: return BSQL_BOOL8;
+    case BSQL_OP_CHAR_LENGTH: return BSQL_INT32;
+    default: return BSQL_EMPTY;
+  }
+}
+
+int output_type_binary(int op, int lt, int rt) {
+  if (lt == BSQL_EMPTY && rt == BSQL_EMPTY)
+    fail("In get_output_type function: both operands types are empty");
+  if (lt == BSQL_EMPTY) lt = rt;
+  else if (rt == BSQL_EMPTY) rt = lt;
+  switch (op) {
+    case BSQL_OP_ADD: case BSQL_OP_SUB: case BSQL_OP_MUL: case BSQL_OP_DIV: case BSQL_OP_MOD:
+      if (is_float_type(lt) && is_float_type(rt)) return type_size(lt) >= type_size(rt) ? lt : rt;
+      if (is_float_type(lt)) return lt;
+      if (is_float_type(rt)) return rt;
+      return type_size(lt) >= type_size(rt) ? lt : rt;
+    case BSQL_OP_EQUAL: case BSQL_OP_NOT_EQUAL: case BSQL_OP_LESS: case BSQL_OP_GREATER:
+    case BSQL_OP_LESS_EQUAL: case BSQL_OP_GREATER_EQUAL: case BSQL_OP_LOGICAL_AND:
+    case BSQL_OP_LOGICAL_OR: case BSQL_OP_STR_LIKE:
+      return BSQL_BOOL8;
+    case BSQL_OP_POW: case BSQL_OP_ROUND: return type_size(lt) <= 4 ? BSQL_FLOAT32 : BSQL_FLOAT64;
+    case BSQL_OP_MAGIC_IF_NOT: return rt;
+    case BSQL_OP_FIRST_NON_MAGIC: return type_size(lt) >= type_size(rt) ? lt : rt;
+    case BSQL_OP_STR_SUBSTRING: case BSQL_OP_STR_REPLACE: case BSQL_OP_STR_REGEXP_REPLACE:
+    case BSQL_OP_STR_CONCAT: case BSQL_OP_STR_TRIM: return BSQL_STRING;
+    case BSQL_OP_TO_DATE: return BSQL_TIMESTAMP_DAYS;
+    case BSQL_OP_TO_TIMESTAMP: return BSQL_TIMESTAMP_NANOSECONDS;
+    default:
+      // IS_NOT_DISTINCT_FROM is listed only in the unary rule set of the
+      // reference, so the binary rule falls through to EMPTY as it does there.
+      return BSQL_EMPTY;
+  }
+}
+
+OpTypes propagate_types(const Program& p, const int32_t* in_types, int n_in) {
+  const size_t n = p.ops.size();
+  OpTypes t;
+  t.lt.assign(n, BSQL_EMPTY);
+  t.rt.assign(n, BSQL_EMPTY);
+  t.ot.assign(n, BSQL_EMPTY);
+  std::map<int, int32_t> slot_type;  // output_map_type: missing slots read as EMPTY
+  auto type_of = [&](int s) -> int32_t {
+    auto it = slot_type.find(s);
+    return it == slot_type.end() ? BSQL_EMPTY : it->second;
+  };
+  for (size_t i = 0; i < n; ++i) {
+    const int l = p.left[i], r = p.right[i];
+    if (l >= 0 && l < n_in) t.lt[i] = in_types[l];
+    else if (l == BSQL_SCALAR_INDEX) t.lt[i] = p.lsc[i].type_id;
+    else if (l >= 0) t.lt[i] = type_of(l);
+    if (r >= 0 && r < n_in) t.rt[i] = in_types[r];
+    else if (r == BSQL_SCALAR_INDEX) t.rt[i] = p.rsc[i].type_id;
+    else if (r >= 0) t.rt[i] = type_of(r);
+    if (r == BSQL_UNARY_INDEX) t.ot[i] = output_type_unary(p.ops[i], t.lt[i]);
+    else if (r == BSQL_NULLARY_INDEX) t.ot[i] = output_type_nullary(p.ops[i]);
+    else t.ot[i] = output_type_binary(p.ops[i], t.lt[i], t.rt[i]);
+    slot_type[p.out[i]] = t.ot[i];
+  }
+  return t;
+}
+
+void validate(const Program& p, int n_in) {
+  const size_t n = p.ops.size();
+  if (p.left.size() != n || p.right.size() != n || p.out.size() != n || p.lsc.size() != n ||
+      p.rsc.size() != n)
+    fail("program arrays have different lengths");
+  if (p.final_out.empty()) fail("program has no outputs");
+  int mx = -1;
+  for (size_t i = 0; i < n; ++i) {
+    mx = std::max<int>(mx, std::max<int>(p.left[i], std::max<int>(p.right[i], p.out[i])));
+    const int op = p.ops[i];
+    if (op <= BSQL_OP_INVALID || op >= BSQL_OP_COUNT) fail("invalid operator " + std::to_string(op));
+    const int l = p.left[i], r = p.right[i];
+    if (p.out[i] < 0) fail("op " + std::to_string(i) + " has no output slot");
+    if (r == BSQL_NULLARY_INDEX) {
+      if (!is_nullary_op(op)) fail("operator " + std::to_string(op) + " is not nullary");
+    } else if (r == BSQL_UNARY_INDEX) {
+      if (!is_unary_op(op)) fail("operator " + std::to_string(op) + " is not unary");
+      if (l < 0) fail("Unary operations on literals is not supported");
+    } else {
+      if (!is_binary_op(op)) fail("operator " + std::to_string(op) + " is not binary");
+      if (l < 0 && r < 0) fail("Operations between literals is not supported");
+      if (l < BSQL_SCALAR_NULL_INDEX || r < BSQL_SCALAR_NULL_INDEX) fail("invalid operand index");
+    }
+  }
+  for (int16_t f : p.final_out) mx = std::max<int>(mx, f);
+  if (mx > BSQL_MAX_SLOT)
+    fail("Interops does not support plans with an input or output index greater than 63");
+  (void)n_in;
+}
+
+// ------------------------------------------------------------------ lowering
+
+namespace {
+
+// One SSA value of the lowered program.
+struct Value {
+  int def = -1;        // index in the emitted list
+  int last_use = -1;
+  int reg = -1;
+  bool is_load = false;
+  int column = -1;     // is_load: input column
+  bool emitted = false;
+};
+
+// An op of the lowered program before register assignment.
+struct LOp {
+  DevOp d{};
+  int va = -1, vb = -1;  // operand values (-1: literal or none)
+  int vdst = -1;         // defined value
+};
+
+struct SourceOp {  // one reference op in SSA form
+  int prog_op = -1;
+  int va = -1, vb = -1, vdst = -1;
+  DevOp d{};
+};
+
+struct Lowering {
+  const int32_t* in_types;
+  int n_in;
+  std::vector<Value> values;
+  std::vector<int> load_value;  // per input column
+  std::vector<LOp> out;
+
+  int new_value() { values.push_back(Value{}); return (int)values.size() - 1; }
+
+  int load_of(int col) {
+    if (load_value[col] < 0) {
+      int v = new_value();
+      values[v].is_load = true;
+      values[v].column = col;
+      load_value[col] = v;
+    }
+    return load_value[col];
+  }
+
+  void emit_value_load(int v) {
+    if (v < 0 || !values[v].is_load || values[v].emitted) return;
+    LOp l;
+    const int col = values[v].column;
+    l.d.code = is_string_type(in_types[col]) ? D_LOADV : D_LOAD;
+    l.d.a = (uint8_t)col;
+    l.d.aux = (uint8_t)in_types[col];
+    l.vdst = v;
+    values[v].emitted = true;
+    values[v].def = (int)out.size();
+    out.push_back(l);
+  }
+
+  void emit(const LOp& op) {
+    emit_value_load(op.va);
+    emit_value_load(op.vb);
+    LOp o = op;
+    if (o.vdst >= 0) {
+      values[o.vdst].def = (int)out.size();
+      values[o.vdst].emitted = true;
+    }
+    out.push_back(o);
+  }
+};
+
+uint8_t unit_code(int t) {  // 0 days .. 4 ns
+  switch (t) {
+    case BSQL_TIMESTAMP_DAYS: return 0;
+    case BSQL_TIMESTAMP_SECONDS: return 1;
+    case BSQL_TIMESTAMP_MILLISECONDS: return 2;
+    case BSQL_TIMESTAMP_MICROSECONDS: return 3;
+    default: return 4;
+  }
+}
+
+int64_t f64_bits(double d) { int64_t b; std::memcpy(&b, &d, 8); return b; }
+
+// slot bits of a literal, as get_scalar_value<T> yields it for the kind of its type
+int64_t scalar_bits(const Scalar& s) {
+  if (is_float_type(s.type_id)) return f64_bits(s.f64);
+  return s.i64;
+}
+
+// Converts reference op i of program p into SSA form (without emitting).
+SourceOp translate(Lowering& L, const Program& p, const OpTypes& t, size_t i,
+                   std::vector<int>& slot_val, int64_t str_base_a, int64_t str_base_b,
+                   uint64_t seed, int n_slots_in, const std::vector<int>& col_of) {
+  SourceOp s;
+  s.prog_op = (int)i;
+  DevOp& d = s.d;
+  const int op = p.ops[i];
+  const int l = p.left[i], r = p.right[i];
+  const int lt = t.lt[i], rt = t.rt[i];
+  const bool lf = is_float_type(lt), rf = is_float_type(rt);
+  auto operand = [&](int pos) -> int {
+    if (pos < 0) return -1;
+    if (pos < n_slots_in && slot_val[pos] < 0) slot_val[pos] = L.load_of(col_of[(size_t)pos]);
+    if (slot_val[pos] < 0) fail("slot " + std::to_string(pos) + " is read before it is written");
+    return slot_val[pos];
+  };
+  if (r != BSQL_UNARY_INDEX && r != BSQL_NULLARY_INDEX) {
+    s.va = operand(l);
+    s.vb = operand(r);
+    if (l == BSQL_SCALAR_INDEX) { d.flags |= F_A_IMM; d.imm = scalar_bits(p.lsc[i]); }
+    if (l == BSQL_SCALAR_NULL_INDEX) d.flags |= F_A_NULL;
+    if (r == BSQL_SCALAR_INDEX) { d.flags |= F_B_IMM; d.imm = scalar_bits(p.rsc[i]); }
+    if (r == BSQL_SCALAR_NULL_INDEX) d.flags |= F_B_NULL;
+    if (lf) d.flags |= F_A_F;
+    if (rf) d.flags |= F_B_F;
+    const bool ii = !lf && !rf;
+    switch (op) {
+      case BSQL_OP_ADD: d.code = ii ? D_ADD_II : D_ADD_FF; break;
+      case BSQL_OP_SUB: d.code = ii ? D_SUB_II : D_SUB_FF; break;
+      case BSQL_OP_MUL: d.code = ii ? D_MUL_II : D_MUL_FF; break;
+      case BSQL_OP_DIV: d.code = ii ? D_DIV_II : D_DIV_FF; break;
+      case BSQL_OP_MOD: d.code = ii ? D_MOD_II : D_MOD_FF; break;
+      case BSQL_OP_POW: d.code = D_POW; break;
+      case BSQL_OP_ROUND: d.code = D_ROUND; break;
+      case BSQL_OP_EQUAL: case BSQL_OP_NOT_EQUAL: case BSQL_OP_LESS: case BSQL_OP_GREATER:
+      case BSQL_OP_LESS_EQUAL: case BSQL_OP_GREATER_EQUAL: {
+        const int k = op - BSQL_OP_EQUAL;
+        if (is_string_type(lt) || is_string_type(rt)) {
+          if (!(is_string_type(lt) && is_string_type(rt)))
+            fail("comparison between a string and a non-string operand");
+          if (l >= n_slots_in || r >= n_slots_in) fail("string operands must be input columns");
+          d.code = (uint8_t)(D_EQ_SS + k);
+          if (l >= 0) d.aux = (uint8_t)col_of[(size_t)l]; else { d.aux = 0xff; d.imm = str_base_a; }
+          if (r >= 0) d.aux2 = (uint8_t)col_of[(size_t)r]; else { d.aux2 = 0xff; d.imm = str_base_b; }
+          if (l < 0 && r < 0) fail("Operations between literals is not supported");
+        } else if (is_timestamp_type(lt) && is_timestamp_type(rt)) {
+          d.code = (uint8_t)(D_EQ_TS + k);
+          d.aux2 = (uint8_t)(unit_code(lt) | (unit_code(rt) << 4));
+        } else {
+          d.code = (uint8_t)((ii ? D_EQ_II : D_EQ_FF) + k);
+        }
+        break;
+      }
+      case BSQL_OP_LOGICAL_AND: d.code = D_AND; break;
+      case BSQL_OP_LOGICAL_OR: d.code = D_OR; break;
+      case BSQL_OP_IS_NOT_DISTINCT_FROM: d.code = ii ? D_INDF_II : D_INDF_FF; break;
+      case BSQL_OP_MAGIC_IF_NOT: d.code = D_MAGIC_IF_NOT; break;
+      case BSQL_OP_FIRST_NON_MAGIC: d.code = D_FIRST_NON_MAGIC; break;
+      default: fail("operator " + std::to_string(op) + " is not supported by the interpreter");
+    }
+    if (is_string_type(lt) && d.code < D_EQ_SS) fail("string operand to a non-string operator");
+  } else if (r == BSQL_UNARY_INDEX) {
+    s.va = operand(l);
+    if (lf) d.flags |= F_A_F;
+    switch (op) {
+      case BSQL_OP_NOT: d.code = D_NOT; break;
+      case BSQL_OP_ABS: d.code = lf ? D_ABS_F : D_ABS_I; break;
+      case BSQL_OP_FLOOR: d.code = D_FLOOR; break;
+      case BSQL_OP_CEIL: d.code = D_CEIL; break;
+      case BSQL_OP_SIN: d.code = D_SIN; break;
+      case BSQL_OP_COS: d.code = D_COS; break;
+      case BSQL_OP_ASIN: d.code = D_ASIN; break;
+      case BSQL_OP_ACOS: d.code = D_ACOS; break;
+      case BSQL_OP_TAN: d.code = D_TAN; break;
+      case BSQL_OP_COTAN: d.code = D_COTAN; break;
+      case BSQL_OP_ATAN: d.code = D_ATAN; break;
+      case BSQL_OP_LN: d.code = D_LN; break;
+      case BSQL_OP_LOG: d.code = D_LOG; break;
+      case BSQL_OP_YEAR: case BSQL_OP_MONTH: case BSQL_OP_DAY: case BSQL_OP_DAYOFWEEK:
+      case BSQL_OP_HOUR: case BSQL_OP_MINUTE: case BSQL_OP_SECOND:
+        d.code = D_EXTRACT; d.aux = (uint8_t)op; d.aux2 = (uint8_t)lt; break;
+      case BSQL_OP_IS_NULL: d.code = D_IS_NULL; break;
+      case BSQL_OP_IS_NOT_NULL: d.code = D_IS_NOT_NULL; break;
+      case BSQL_OP_IS_TRUE: d.code = D_IS_TRUE; break;
+      case BSQL_OP_IS_NOT_TRUE: d.code = D_IS_NOT_TRUE; break;
+      case BSQL_OP_IS_NOT_FALSE: d.code = D_IS_NOT_FALSE; break;
+      case BSQL_OP_CAST_TINYINT: case BSQL_OP_CAST_SMALLINT: case BSQL_OP_CAST_INTEGER:
+      case BSQL_OP_CAST_BIGINT: d.code = D_TO_I64; break;
+      case BSQL_OP_CAST_FLOAT: case BSQL_OP_CAST_DOUBLE: d.code = D_TO_F64; break;
+      case BSQL_OP_CAST_DATE: d.code = D_CAST_DATE; d.aux2 = (uint8_t)lt; break;
+      case BSQL_OP_CAST_TIMESTAMP: case BSQL_OP_CAST_TIMESTAMP_SECONDS:
+      case BSQL_OP_CAST_TIMESTAMP_MILLISECONDS: case BSQL_OP_CAST_TIMESTAMP_MICROSECONDS:
+        d.code = D_CAST_TS; d.aux2 = (uint8_t)lt; break;
+      case BSQL_OP_CHAR_LENGTH:
+        if (!is_string_type(lt) || l >= n_slots_in) fail("CHAR_LENGTH needs a string input column");
+        d.code = D_CHAR_LENGTH; d.aux = (uint8_t)col_of[(size_t)l]; break;
+      default: fail("operator " + std::to_string(op) + " is not supported by the interpreter");
+    }
+    if (is_string_type(lt) && d.code != D_CHAR_LENGTH && d.code != D_IS_NULL &&
+        d.code != D_IS_NOT_NULL)
+      fail("string operand to a non-string operator");
+  } else {
+    d.code = D_RAND;
+    d.imm = (int64_t)seed;
+  }
+  return s;
+}
+
+// Result value of each op and the ops each value depends on.
+struct SsaProgram {
+  std::vector<SourceOp> ops;
+  std::vector<int> finals;  // value per final output
+  OpTypes types;
+  std::vector<int32_t> slot_types;
+};
+
+// Interpreter input slot -> table column: programs compiled from text refer to
+// the used columns only (col_idx_map, LogicalProject.cpp:859-881); programs
+// built from arrays address the table directly.
+std::vector<int> slot_columns(const Program& p, int n_table) {
+  std::vector<int> col_of;
+  if (!p.used_columns.empty() || p.n_inputs == 0) {
+    for (int32_t c : p.used_columns) {
+      if (c < 0 || c >= n_table) fail("program refers to column " + std::to_string(c) + " outside the table");
+      col_of.push_back(c);
+    }
+  } else {
+    for (int k = 0; k < n_table; ++k) col_of.push_back(k);
+  }
+  return col_of;
+}
+
+}  // namespace
+
+std::vector<int32_t> slot_types_for(const Program& p, const int32_t* table_types, int n_table) {
+  std::vector<int32_t> t;
+  for (int c : slot_columns(p, n_table)) t.push_back(table_types[c]);
+  return t;
+}
+
+namespace {
+
+SsaProgram to_ssa(Lowering& L, const Program& p, uint64_t seed) {
+  const std::vector<int> col_of = slot_columns(p, L.n_in);
+  const int n_slots_in = (int)col_of.size();
+  std::vector<int32_t> slot_types(col_of.size());
+  for (size_t k = 0; k < col_of.size(); ++k) slot_types[k] = L.in_types[col_of[k]];
+  validate(p, n_slots_in);
+  OpTypes t = propagate_types(p, slot_types.data(), n_slots_in);
+  SsaProgram s;
+  std::vector<int> slot_val(512, -1);
+  for (size_t i = 0; i < p.ops.size(); ++i) {
+    int64_t sa = -1, sb = -1;
+    if (p.str_offsets.size() == 2 * p.ops.size()) { sa = p.str_offsets[2 * i]; sb = p.str_offsets[2 * i + 1]; }
+    int64_t base = (int64_t)(intptr_t)p.d_strings;
+    SourceOp so = translate(L, p, t, i, slot_val, sa >= 0 ? base + sa : 0, sb >= 0 ? base + sb : 0,
+                            seed, n_slots_in, col_of);
+    so.vdst = L.new_value();
+    slot_val[p.out[i]] = so.vdst;
+    s.ops.push_back(so);
+  }
+  for (int16_t f : p.final_out) {
+    int v = f < n_slots_in && slot_val[f] < 0 ? L.load_of(col_of[(size_t)f]) : slot_val[f];
+    if (v < 0) fail("output slot " + std::to_string(f) + " is never written");
+    s.finals.push_back(v);
+  }
+  s.types = t;
+  s.slot_types = slot_types;
+  return s;
+}
+
+// producer op (index into ssa.ops) of each value, -1 for loads
+std::vector<int> producers(const SsaProgram& s, size_t nvalues) {
+  std::vector<int> prod(nvalues, -1);
+  for (size_t i = 0; i < s.ops.size(); ++i) prod[s.ops[i].vdst] = (int)i;
+  return prod;
+}
+
+void subtree(const SsaProgram& s, const std::vector<int>& prod, int v, std::set<int>& acc) {
+  if (v < 0) return;
+  int o = prod[v];
+  if (o < 0 || acc.count(o)) return;
+  acc.insert(o);
+  subtree(s, prod, s.ops[o].va, acc);
+  subtree(s, prod, s.ops[o].vb, acc);
+}
+
+// Flattens the AND tree rooted at value v into its conjunct values.
+void conjuncts_of(const Program& p, const SsaProgram& s, const std::vector<int>& prod, int v,
+                  std::vector<int>& acc) {
+  int o = v >= 0 ? prod[v] : -1;
+  if (o >= 0 && p.ops[s.ops[o].prog_op] == BSQL_OP_LOGICAL_AND && s.ops[o].va >= 0 &&
+      s.ops[o].vb >= 0) {
+    conjuncts_of(p, s, prod, s.ops[o].va, acc);
+    conjuncts_of(p, s, prod, s.ops[o].vb, acc);
+  } else {
+    acc.push_back(v);
+  }
+}
+
+LOp to_lop(const SourceOp& so) {
+  LOp l;
+  l.d = so.d;
+  l.va = so.va;
+  l.vb = so.vb;
+  l.vdst = so.vdst;
+  return l;
+}
+
+}  // namespace
+
+int count_conjuncts(const Program& p) {
+  // The tree shape does not depend on the column types: a neutral schema suffices.
+  int n_table = BSQL_MAX_SLOT + 1;
+  for (int32_t c : p.used_columns) n_table = std::max(n_table, c + 1);
+  std::vector<int32_t> types((size_t)n_table, BSQL_INT64);
+  Lowering L{types.data(), n_table, {}, {}, {}};
+  L.load_value.assign((size_t)n_table, -1);
+  try {
+    SsaProgram s = to_ssa(L, p, 0);
+    if (s.finals.size() != 1) return 0;
+    std::vector<int> prod = producers(s, L.values.size());
+    std::vector<int> conj;
+    conjuncts_of(p, s, prod, s.finals[0], conj);
+    return (int)conj.size();
+  } catch (const Error&) {
+    return 0;
+  }
+}
+
+Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
+              const int32_t* out_types, int n_out) {
+  if (n_in > BSQL_DEV_MAX_IN) fail("too many input columns for one launch");
+  if (n_out > BSQL_DEV_MAX_OUT) fail("too many output columns for one launch");
+  Lowering L{in_types, n_in, {}, {}, {}};
+  L.load_value.assign(n_in, -1);
+  Lowered res;
+  std::vector<int> store_value;  // value stored into output k
+
+  // ---- phase 1: row selection
+  if (opt.mode != LowerOptions::EXECUTE) {
+    const Program& pred = *opt.pred;
+    if (pred.final_out.size() != 1) fail("a filter predicate must have exactly one output");
+    SsaProgram s = to_ssa(L, pred, pred.seed);
+    const OpTypes& t = s.types;
+    int root_type = pred.final_out[0] < (int)s.slot_types.size() ? s.slot_types[pred.final_out[0]] : BSQL_EMPTY;
+    for (size_t i = 0; i < pred.ops.size(); ++i)
+      if (pred.out[i] == pred.final_out[0]) root_type = t.ot[i];
+    if (root_type != BSQL_BOOL8)
+      fail("Expression does not evaluate to a boolean mask");
+    std::vector<int> prod = producers(s, L.values.size());
+    std::vector<int> conj;
+    if (opt.conjuncts) conjuncts_of(pred, s, prod, s.finals[0], conj);
+    res.n_conjuncts = (int)conj.size();
+    if (conj.size() >= 2) {
+      std::vector<int32_t> order(conj.size());
+      for (size_t j = 0; j < conj.size(); ++j) order[j] = (int32_t)j;
+      if (opt.conj_order && opt.conj_order->size() == conj.size()) order = *opt.conj_order;
+      std::set<int> done;
+      for (int32_t j : order) {
+        std::set<int> sub;
+        subtree(s, prod, conj[(size_t)j], sub);
+        for (int o : sub)  // std::set iterates in program order
+          if (!done.count(o)) { L.emit(to_lop(s.ops[(size_t)o])); done.insert(o); }
+        LOp k;
+        k.d.code = D_KILL;
+        k.va = conj[(size_t)j];
+        // truthiness follows the kind of the conjunct's value
+        int o = prod[(size_t)k.va];
+        int ty = o >= 0 ? t.ot[(size_t)s.ops[(size_t)o].prog_op] : L.in_types[L.values[(size_t)k.va].column];
+        if (is_float_type(ty)) k.d.flags |= F_A_F;
+        L.emit(k);
+      }
+    } else {
+      for (const SourceOp& so : s.ops) L.emit(to_lop(so));
+      LOp k;
+      k.d.code = D_SEL;
+      k.va = s.finals[0];
+      L.emit(k);
+    }
+  }
+  const int split = (int)L.out.size();
+
+  // ---- phase 2: outputs
+  if (opt.mode == LowerOptions::FILTER_GATHER) {
+    if (n_out != n_in) fail("filter gathers every input column");
+    for (int k = 0; k < n_in; ++k) {
+      if (is_string_type(in_types[k])) fail("string columns are gathered by the string path");
+      store_value.push_back(L.load_of(k));
+    }
+  } else {
+    const Program& proj = *opt.proj;
+    if ((int)proj.final_out.size() != n_out) fail("number of outputs does not match the program");
+    SsaProgram s = to_ssa(L, proj, proj.seed);
+    for (const SourceOp& so : s.ops) L.emit(to_lop(so));
+    store_value = s.finals;
+  }
+  for (int k = 0; k < n_out; ++k) {
+    LOp st;
+    st.d.code = D_STORE;
+    st.va = store_value[(size_t)k];
+    st.d.aux = (uint8_t)k;
+    L.emit(st);
+  }
+  // Place each store right after the definition of its value, but never
+  // inside the selection phase (compacted positions exist only after it).
+  {
+    std::vector<LOp> body, stores;
+    for (const LOp& o : L.out) (o.d.code == D_STORE ? stores : body).push_back(o);
+    std::vector<int> def_at(L.values.size(), -1);
+    for (size_t i = 0; i < body.size(); ++i)
+      if (body[i].vdst >= 0) def_at[(size_t)body[i].vdst] = (int)i;
+    std::vector<std::vector<LOp>> after(body.size() + 1);
+    for (const LOp& st : stores) {
+      int at = std::max(def_at[(size_t)st.va], split - 1);
+      after[(size_t)(at + 1)].push_back(st);
+    }
+    std::vector<LOp> final_list;
+    for (size_t i = 0; i <= body.size(); ++i) {
+      for (const LOp& st : after[i]) final_list.push_back(st);
+      if (i < body.size()) final_list.push_back(body[i]);
+    }
+    L.out = final_list;
+  }
+  const int new_split = opt.mode == LowerOptions::EXECUTE ? 0 : split;
+  int new_split_final = new_split;
+
+  // ---- load placement: column loads move to the start of their segment so
+  // that one LOADN keeps several columns in flight. Segments: the whole
+  // selection phase (eager) or each conjunct (lazy), and the output phase.
+  {
+    std::vector<int> bounds{0};
+    if (opt.lazy_loads)
+      for (int i = 0; i < new_split; ++i)
+        if (L.out[(size_t)i].d.code == D_KILL) bounds.push_back(i + 1);
+    if (new_split > 0) bounds.push_back(new_split);
+    bounds.push_back((int)L.out.size());
+    for (size_t s = 0; s + 1 < bounds.size(); ++s) {
+      auto b = L.out.begin() + bounds[s], e = L.out.begin() + bounds[s + 1];
+      if (b < e)
+        std::stable_partition(b, e, [](const LOp& o) { return o.d.code == D_LOAD; });
+    }
+  }
+
+  // ---- liveness and register assignment (linear scan)
+  for (Value& v : L.values) { v.def = -1; v.last_use = -1; v.reg = -1; }
+  for (size_t i = 0; i < L.out.size(); ++i) {
+    LOp& o = L.out[i];
+    if (o.vdst >= 0) L.values[(size_t)o.vdst].def = (int)i;
+    for (int v : {o.va, o.vb})
+      if (v >= 0) L.values[(size_t)v].last_use = std::max(L.values[(size_t)v].last_use, (int)i);
+  }
+  // values read in phase 2 but defined in phase 1 stay live across the split
+  std::vector<int> free_regs;
+  for (int r = BSQL_DEV_MAX_REG - 1; r >= 0; --r) free_regs.push_back(r);
+  int used = 0;
+  for (size_t i = 0; i < L.out.size(); ++i) {
+    LOp& o = L.out[i];
+    for (int v : {o.va, o.vb}) {
+      if (v < 0) continue;
+      Value& val = L.values[(size_t)v];
+      if (val.reg < 0) fail("internal: operand without a register");
+    }
+    // free operands whose last use is this op (dst may reuse them)
+    for (int v : {o.va, o.vb}) {
+      if (v < 0) continue;
+      Value& val = L.values[(size_t)v];
+      if (val.last_use == (int)i && val.reg >= 0) {
+        if (std::find(free_regs.begin(), free_regs.end(), val.reg) == free_regs.end())
+          free_regs.push_back(val.reg);
+      }
+    }
+    std::sort(free_regs.begin(), free_regs.end(), std::greater<int>());
+    if (o.vdst >= 0) {
+      Value& dv = L.values[(size_t)o.vdst];
+      if (free_regs.empty()) fail("program needs more than 32 live values");
+      dv.reg = free_regs.back();
+      free_regs.pop_back();
+      used = std::max(used, dv.reg + 1);
+      if (dv.last_use < 0) free_regs.push_back(dv.reg);  // dead result
+    }
+    if (o.va >= 0) o.d.a = (uint8_t)L.values[(size_t)o.va].reg;
+    if (o.vb >= 0) o.d.b = (uint8_t)L.values[(size_t)o.vb].reg;
+    if (o.vdst >= 0) o.d.dst = (uint8_t)L.values[(size_t)o.vdst].reg;
+  }
+  // loads keep their column index in `a`
+  for (LOp& o : L.out)
+    if (o.d.code == D_LOAD || o.d.code == D_LOADV) o.d.a = (uint8_t)L.values[(size_t)o.vdst].column;
+
+  // merge runs of column loads (same segment) into LOADN ops of up to 4
+  {
+    std::vector<LOp> merged;
+    int merged_split = new_split;
+    for (size_t i = 0; i < L.out.size();) {
+      if (L.out[i].d.code != D_LOAD) { merged.push_back(L.out[i]); ++i; continue; }
+      LOp m = L.out[i];
+      m.d.code = D_LOADN;
+      m.d.imm = 0;
+      int n = 0;
+      size_t j = i;
+      while (j < L.out.size() && L.out[j].d.code == D_LOAD && n < 4) {
+        const uint64_t pair = (uint64_t)L.out[j].d.a | ((uint64_t)L.out[j].d.dst << 8);
+        m.d.imm |= (int64_t)(pair << (16 * n));
+        ++n;
+        ++j;
+      }
+      m.d.aux = (uint8_t)n;
+      merged.push_back(m);
+      i = j;
+    }
+    L.out = merged;
+    if (opt.mode != LowerOptions::EXECUTE) {
+      // recompute the split: one past the last selection op
+      int last_sel = -1;
+      for (size_t i = 0; i < L.out.size(); ++i)
+        if (L.out[i].d.code == D_KILL || L.out[i].d.code == D_SEL) last_sel = (int)i;
+      merged_split = last_sel + 1;
+    }
+    new_split_final = merged_split;
+  }
+
+  if (L.out.size() > BSQL_DEV_MAX_OPS) fail("program too long for one launch");
+  std::memset(&res.plan, 0, sizeof(res.plan));
+  res.plan.n_ops = (int32_t)L.out.size();
+  res.plan.split = new_split_final;
+  res.plan.n_in = n_in;
+  res.plan.n_out = n_out;
+  for (size_t i = 0; i < L.out.size(); ++i) res.plan.ops[i] = L.out[i].d;
+  (void)out_types;
+  res.nreg = std::max(used, 1);
+  return res;
+}
+
+Program::~Program() = default;
+
+}  // namespace bsql

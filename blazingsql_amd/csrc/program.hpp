@@ -1,0 +1,112 @@
+// program.hpp -- host-side program representation of the row-parallel
+// interpreter, and its lowering to the MI355X device plan.
+//
+// A Program holds the reference encoding (engine/src/Interpreter/interpreter_cpp.h:50-86):
+// parallel int16 vectors left/right/out, the operator per op, the final output
+// slots, and the literal scalars. lower() turns it into a DevPlan: operand
+// types resolved per op (interpreter_cpp.cu:502-550), each op specialised to
+// its int64/double operand kinds, column loads inserted before first use, and
+// the slot numbering replaced by a dense register numbering found by liveness,
+// so that the whole per-row state lives in VGPRs on gfx950.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bsql/interp.h"
+#include "devplan.h"
+
+namespace bsql {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] inline void fail(const std::string& m) { throw Error(BSQL_ERR_INVALID, m); }
+
+// literal scalar with owned string storage
+struct Scalar {
+  int32_t type_id = BSQL_EMPTY;
+  bool is_null = true;
+  int64_t i64 = 0;
+  double f64 = 0.0;
+  std::string str;
+};
+
+struct Program {
+  std::vector<int16_t> left, right, out, final_out;
+  std::vector<int32_t> ops;
+  std::vector<Scalar> lsc, rsc;
+  // schema hints from the compiler: number of interpreter inputs and the
+  // table columns they come from (col_idx_map of LogicalProject.cpp:859-881)
+  int n_inputs = -1;
+  std::vector<int32_t> used_columns;
+  // conjunct order override (empty = plan order) and launch override
+  std::vector<int32_t> conj_order;
+  bsql_launch_config launch{0, 0};
+  uint64_t seed = 0x5eed5eed12345678ULL;
+  // device copies of string literals, one buffer per program
+  void* d_strings = nullptr;
+  std::vector<int64_t> str_offsets;  // per (op, side): offset into d_strings or -1
+  ~Program();
+};
+
+// type helpers (cudf type_id semantics)
+bool is_float_type(int t);
+bool is_timestamp_type(int t);
+bool is_duration_type(int t);
+bool is_string_type(int t);
+int type_size(int t);
+int64_t ticks_per_day(int t);
+int64_t ns_per_tick(int t);
+
+bool is_nullary_op(int op);
+bool is_unary_op(int op);
+bool is_binary_op(int op);
+// get_output_type (engine/src/parser/expression_utils.cpp:115-266)
+int output_type_nullary(int op);
+int output_type_unary(int op, int lt);
+int output_type_binary(int op, int lt, int rt);
+
+// Operand and result types of every op for a given input schema.
+struct OpTypes {
+  std::vector<int32_t> lt, rt, ot;
+};
+OpTypes propagate_types(const Program& p, const int32_t* in_types, int n_in);
+
+// Types of the interpreter input slots of `p` for a table schema: programs
+// compiled from text address their used columns, array-built ones the table.
+std::vector<int32_t> slot_types_for(const Program& p, const int32_t* table_types, int n_table);
+
+// Checks the reference's program limits (interpreter_cpp.cu:329-378, 466).
+void validate(const Program& p, int n_in);
+
+// Which register-file size / rows-per-thread a lowered plan needs.
+struct LowerOptions {
+  enum Mode { EXECUTE, FILTER_PROJECT, FILTER_GATHER } mode = EXECUTE;
+  const Program* pred = nullptr;   // FILTER_*: predicate (1 BOOL8 output)
+  const Program* proj = nullptr;   // EXECUTE / FILTER_PROJECT: outputs
+  bool conjuncts = true;           // wave-level early exit per conjunct
+  bool lazy_loads = false;         // load each conjunct's columns only when it is reached
+  const std::vector<int32_t>* conj_order = nullptr;
+};
+
+struct Lowered {
+  DevPlan plan;          // columns are filled by the launcher
+  int nreg = 0;          // registers used
+  int n_conjuncts = 0;   // conjuncts found in the predicate (0 = not an AND chain)
+};
+Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
+              const int32_t* out_types, int n_out);
+
+// Conjunct decomposition of a predicate whose root is a left-deep AND chain.
+int count_conjuncts(const Program& p);
+
+// Text front-end (parser.cpp): Calcite expression strings -> Program.
+std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exprs,
+                                             const int32_t* in_types, int n_in);
+
+}  // namespace bsql

@@ -1,0 +1,290 @@
+"""Host-side mirror of the interpreter's program and column types over the C ABI.
+
+Program            <-> the postfix plan of add_expression_to_interpreter_plan
+                       (engine/src/Interpreter/interpreter_cpp.h:50-59)
+perform_interpreter_operation <-> interops::perform_interpreter_operation
+                       (engine/src/Interpreter/interpreter_cpp.cu:440-594)
+DeviceColumn       <-> a cudf column view: data, 32-bit validity words, offsets
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+_NP_DTYPE = {
+    L.INT8: np.int8, L.INT16: np.int16, L.INT32: np.int32, L.INT64: np.int64,
+    L.UINT8: np.uint8, L.UINT16: np.uint16, L.UINT32: np.uint32, L.UINT64: np.uint64,
+    L.FLOAT32: np.float32, L.FLOAT64: np.float64, L.BOOL8: np.uint8,
+    L.TIMESTAMP_DAYS: np.int32, L.TIMESTAMP_SECONDS: np.int64,
+    L.TIMESTAMP_MILLISECONDS: np.int64, L.TIMESTAMP_MICROSECONDS: np.int64,
+    L.TIMESTAMP_NANOSECONDS: np.int64, L.DURATION_DAYS: np.int32,
+    L.DURATION_SECONDS: np.int64, L.DURATION_MILLISECONDS: np.int64,
+    L.DURATION_MICROSECONDS: np.int64, L.DURATION_NANOSECONDS: np.int64,
+}
+
+
+def np_dtype(type_id):
+    return _NP_DTYPE[type_id]
+
+
+def is_float(type_id):
+    return type_id in (L.FLOAT32, L.FLOAT64)
+
+
+def pack_validity(valid_bools):
+    """bool array -> LSB-first 32-bit validity words (numpy uint32)."""
+    v = np.asarray(valid_bools, dtype=bool)
+    n = v.shape[0]
+    words = (n + 31) // 32
+    padded = np.zeros(words * 32, dtype=np.uint8)
+    padded[:n] = v
+    bits = np.packbits(padded.reshape(-1, 8), axis=1, bitorder="little").reshape(-1)
+    return bits.view(np.uint32) if words else np.zeros(0, dtype=np.uint32)
+
+
+def unpack_validity(words, n):
+    w = np.asarray(words, dtype=np.uint32)
+    if n == 0:
+        return np.zeros(0, dtype=bool)
+    b = np.unpackbits(w.view(np.uint8), bitorder="little")
+    return b[:n].astype(bool)
+
+
+class Program:
+    """An interpreter program (reference encoding), owned by libbsql."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                L.lib().bsql_program_free(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @classmethod
+    def from_arrays(cls, left, right, out, ops, final_out, left_scalars=None, right_scalars=None):
+        """ops are operator_type values (ints) or names from _lib.OPS; scalars are
+        per-op (type_id, value) pairs or None."""
+        n = len(ops)
+        ops_i = [L.OP[o] if isinstance(o, str) else int(o) for o in ops]
+        a16 = lambda v: (ctypes.c_int16 * max(len(v), 1))(*v)
+        sarr = lambda sc: _scalar_array(sc, n) if sc is not None else None
+        ls, rs = sarr(left_scalars), sarr(right_scalars)
+        h = ctypes.c_void_p()
+        L.check(L.lib().bsql_program_from_arrays(
+            a16(left), a16(right), a16(out), (ctypes.c_int32 * max(n, 1))(*ops_i), n,
+            a16(final_out), len(final_out), ls, rs, ctypes.byref(h)))
+        p = cls(h.value)
+        p._keep = (ls, rs)
+        return p
+
+    @classmethod
+    def compile(cls, exprs, in_types):
+        """Calcite expression text(s) -> program (evaluate_expressions front-end)."""
+        if isinstance(exprs, str):
+            exprs = [exprs]
+        arr = (ctypes.c_char_p * len(exprs))(*[e.encode() for e in exprs])
+        types = (ctypes.c_int32 * max(len(in_types), 1))(*in_types)
+        h = ctypes.c_void_p()
+        L.check(L.lib().bsql_program_compile(arr, len(exprs), types, len(in_types), ctypes.byref(h)))
+        return cls(h.value)
+
+    @property
+    def num_ops(self):
+        return L.lib().bsql_program_num_ops(self._h)
+
+    @property
+    def num_outputs(self):
+        return L.lib().bsql_program_num_outputs(self._h)
+
+    @property
+    def num_inputs(self):
+        return L.lib().bsql_program_num_inputs(self._h)
+
+    def arrays(self):
+        n, m = self.num_ops, self.num_outputs
+        left = (ctypes.c_int16 * max(n, 1))()
+        right = (ctypes.c_int16 * max(n, 1))()
+        out = (ctypes.c_int16 * max(n, 1))()
+        ops = (ctypes.c_int32 * max(n, 1))()
+        fo = (ctypes.c_int16 * max(m, 1))()
+        L.check(L.lib().bsql_program_arrays(self._h, left, right, out, ops, fo))
+        return dict(left=list(left)[:n], right=list(right)[:n], out=list(out)[:n],
+                    ops=list(ops)[:n], final_out=list(fo)[:m])
+
+    def scalar(self, op, side):
+        s = L.Scalar()
+        L.check(L.lib().bsql_program_scalar(self._h, op, side, ctypes.byref(s)))
+        if s.type_id == L.EMPTY:
+            return None
+        if s.type_id == L.STRING:
+            return (s.type_id, ctypes.string_at(s.str, s.str_len).decode())
+        return (s.type_id, s.v.f64 if is_float(s.type_id) else s.v.i64)
+
+    def output_type(self, in_types, k=0):
+        t = ctypes.c_int32()
+        types = (ctypes.c_int32 * max(len(in_types), 1))(*in_types)
+        L.check(L.lib().bsql_program_output_type(self._h, types, len(in_types), k, ctypes.byref(t)))
+        return t.value
+
+    def used_columns(self):
+        buf = (ctypes.c_int32 * 64)()
+        n = L.lib().bsql_program_used_columns(self._h, buf, 64)
+        return list(buf)[:n]
+
+    @property
+    def num_conjuncts(self):
+        return L.lib().bsql_program_num_conjuncts(self._h)
+
+    def set_conjunct_order(self, order):
+        if order is None:
+            L.check(L.lib().bsql_program_set_conjunct_order(self._h, None, 0))
+        else:
+            arr = (ctypes.c_int32 * len(order))(*order)
+            L.check(L.lib().bsql_program_set_conjunct_order(self._h, arr, len(order)))
+
+    def set_launch(self, rows_per_thread=0, block_size=0):
+        cfg = L.LaunchConfig(block_size, rows_per_thread)
+        L.check(L.lib().bsql_program_set_launch(self._h, ctypes.byref(cfg)))
+
+    def to_oracle_dict(self):
+        """Arrays + scalars in the form the CPU oracle consumes."""
+        a = self.arrays()
+        n = len(a["ops"])
+        a["left_scalars"] = [self.scalar(i, 0) if a["left"][i] == L.SCALAR_INDEX else None for i in range(n)]
+        a["right_scalars"] = [self.scalar(i, 1) if a["right"][i] == L.SCALAR_INDEX else None for i in range(n)]
+        return a
+
+
+def _scalar_array(scalars, n):
+    arr = (L.Scalar * max(n, 1))()
+    keep = []
+    for i, sc in enumerate(scalars):
+        if sc is None:
+            continue
+        t, v = sc
+        arr[i].type_id = t
+        if t == L.STRING:
+            b = v.encode() if isinstance(v, str) else bytes(v)
+            keep.append(b)
+            arr[i].str = b
+            arr[i].str_len = len(b)
+        elif is_float(t):
+            arr[i].v.f64 = float(v)
+        else:
+            arr[i].v.i64 = int(v)
+    arr._keep = keep
+    return arr
+
+
+class DeviceColumn:
+    """A column in device memory (torch tensors as storage)."""
+
+    def __init__(self, type_id, data, validity=None, offsets=None, size=None, offset=0):
+        self.type_id = type_id
+        self.data = data          # torch tensor (fixed width) or uint8 chars
+        self.validity = validity  # torch int32 tensor of words, or None
+        self.offsets = offsets    # torch int32 tensor (strings)
+        if size is None:
+            size = (offsets.numel() - 1 - offset) if type_id == L.STRING else data.numel() - offset
+        self.size = size
+        self.offset = offset
+
+    def to_c(self):
+        c = L.Column()
+        c.type_id = self.type_id
+        c.size = self.size
+        c.offset = self.offset
+        c.data = self.data.data_ptr() if self.data is not None and self.data.numel() else None
+        c.validity = self.validity.data_ptr() if self.validity is not None else None
+        c.offsets = self.offsets.data_ptr() if self.offsets is not None else None
+        return c
+
+    @staticmethod
+    def from_numpy(type_id, values, valid=None, device="cuda", chars=None, offsets=None):
+        import torch
+        if type_id == L.STRING:
+            data = torch.from_numpy(np.ascontiguousarray(chars, dtype=np.uint8).copy()).to(device)
+            offs = torch.from_numpy(np.ascontiguousarray(offsets, dtype=np.int32).copy()).to(device)
+            n = len(offsets) - 1
+            vt = None
+            if valid is not None:
+                vt = torch.from_numpy(pack_validity(valid).view(np.int32).copy()).to(device)
+            return DeviceColumn(type_id, data, vt, offs, size=n)
+        arr = np.ascontiguousarray(values, dtype=np_dtype(type_id))
+        data = torch.from_numpy(arr.copy()).to(device)
+        vt = None
+        if valid is not None:
+            vt = torch.from_numpy(pack_validity(valid).view(np.int32).copy()).to(device)
+        return DeviceColumn(type_id, data, vt, size=len(arr))
+
+    @staticmethod
+    def empty(type_id, n, nullable=True, device="cuda"):
+        import torch
+        dt = {1: torch.int8, 2: torch.int16, 4: torch.int32, 8: torch.int64}
+        width = np.dtype(np_dtype(type_id)).itemsize
+        data = torch.empty(max(n, 1), dtype=dt[width], device=device)
+        vt = torch.zeros(max((n + 31) // 32, 1), dtype=torch.int32, device=device) if nullable else None
+        return DeviceColumn(type_id, data, vt, size=n)
+
+    def to_numpy(self, n=None):
+        n = self.size if n is None else n
+        raw = self.data[self.offset:self.offset + n].cpu().numpy() if n else np.zeros(0)
+        vals = raw.view(np_dtype(self.type_id)) if n else np.zeros(0, dtype=np_dtype(self.type_id))
+        if self.validity is None:
+            valid = np.ones(n, dtype=bool)
+        else:
+            words = self.validity.cpu().numpy().view(np.uint32)
+            valid = unpack_validity(words, self.offset + n)[self.offset:]
+        return vals, valid
+
+
+def columns_array(cols):
+    arr = (L.Column * max(len(cols), 1))()
+    for i, c in enumerate(cols):
+        arr[i] = c.to_c()
+    return arr
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return ctypes.c_void_p(stream)
+
+
+def perform_interpreter_operation(prog, in_cols, out_cols, num_rows=None, stream=None):
+    """Evaluates every output of `prog` over `in_cols` into `out_cols`."""
+    if num_rows is None:
+        num_rows = in_cols[0].size if in_cols else 0
+    ins, outs = columns_array(in_cols), columns_array(out_cols)
+    L.check(L.lib().bsql_interp_execute(prog.handle, ins, len(in_cols), outs, len(out_cols),
+                                        num_rows, _stream_ptr(stream)))
+
+
+def filter_columns(pred, in_cols, out_cols, d_count, num_rows=None, stream=None):
+    """Stable compaction of every input column by the predicate (count on device)."""
+    if num_rows is None:
+        num_rows = in_cols[0].size if in_cols else 0
+    ins, outs = columns_array(in_cols), columns_array(out_cols)
+    L.check(L.lib().bsql_filter(pred.handle, ins, len(in_cols), outs, num_rows,
+                                ctypes.c_void_p(d_count.data_ptr()), _stream_ptr(stream)))
+
+
+def filter_project(pred, proj, in_cols, out_cols, d_count, num_rows=None, stream=None):
+    """Fused WHERE + SELECT: projection outputs of the selected rows, in order."""
+    if num_rows is None:
+        num_rows = in_cols[0].size if in_cols else 0
+    ins, outs = columns_array(in_cols), columns_array(out_cols)
+    L.check(L.lib().bsql_filter_project(pred.handle, proj.handle, ins, len(in_cols), outs,
+                                        len(out_cols), num_rows,
+                                        ctypes.c_void_p(d_count.data_ptr()), _stream_ptr(stream)))

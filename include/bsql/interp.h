@@ -1,0 +1,237 @@
+/*
+ * bsql/interp.h -- C ABI of the MI355X row-parallel expression interpreter,
+ * filter (boolean-mask stream compaction) and fused filter+project path.
+ *
+ * Drop-in boundary for BlazingSQL's interpreter/filter path. Each entry point
+ * names the reference interface it replaces (paths relative to the
+ * BlazingSQL 21.08 source tree).
+ *
+ * Conventions
+ *   - Plain C: no exceptions cross the ABI, no torch/cudf types.
+ *   - Column buffers are device pointers (hipMalloc'd or torch-owned).
+ *   - Return codes: BSQL_OK (0); BSQL_ERR_OOM (1, retryable: inputs untouched,
+ *     the operator layer maps it to task_status::RETRY); BSQL_ERR_INVALID (2,
+ *     invalid or unsupported program/arguments: FAIL); BSQL_ERR_DEVICE (3).
+ *     bsql_last_error() returns the message of the calling thread's last error.
+ *   - Every launching call takes an explicit hipStream_t (passed as void*), is
+ *     reentrant per stream and never synchronizes the device.
+ */
+#ifndef BSQL_INTERP_H
+#define BSQL_INTERP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  BSQL_OK = 0,
+  BSQL_ERR_OOM = 1,
+  BSQL_ERR_INVALID = 2,
+  BSQL_ERR_DEVICE = 3
+};
+
+/* ---- type ids: numbering of cudf::type_id (libcudf 21.08) ----------------
+ * The interpreter dispatches on these exactly as the reference does
+ * (engine/src/Interpreter/interpreter_ops.cuh:141-159, 267-395). */
+enum {
+  BSQL_EMPTY = 0,
+  BSQL_INT8 = 1, BSQL_INT16 = 2, BSQL_INT32 = 3, BSQL_INT64 = 4,
+  BSQL_UINT8 = 5, BSQL_UINT16 = 6, BSQL_UINT32 = 7, BSQL_UINT64 = 8,
+  BSQL_FLOAT32 = 9, BSQL_FLOAT64 = 10, BSQL_BOOL8 = 11,
+  BSQL_TIMESTAMP_DAYS = 12, BSQL_TIMESTAMP_SECONDS = 13,
+  BSQL_TIMESTAMP_MILLISECONDS = 14, BSQL_TIMESTAMP_MICROSECONDS = 15,
+  BSQL_TIMESTAMP_NANOSECONDS = 16,
+  BSQL_DURATION_DAYS = 17, BSQL_DURATION_SECONDS = 18,
+  BSQL_DURATION_MILLISECONDS = 19, BSQL_DURATION_MICROSECONDS = 20,
+  BSQL_DURATION_NANOSECONDS = 21,
+  BSQL_DICTIONARY32 = 22, BSQL_STRING = 23, BSQL_LIST = 24,
+  BSQL_DECIMAL32 = 25, BSQL_DECIMAL64 = 26, BSQL_STRUCT = 27
+};
+
+/* ---- operand codes of the postfix program --------------------------------
+ * engine/src/Interpreter/interpreter_cpp.h:19-26 (column_index). */
+enum {
+  BSQL_UNARY_INDEX = -1,
+  BSQL_SCALAR_INDEX = -2,
+  BSQL_SCALAR_NULL_INDEX = -3,
+  BSQL_NULLARY_INDEX = -4
+};
+
+/* ---- opcodes: numeric values of operator_type ------------------------------
+ * engine/src/parser/expression_utils.hpp:11-93 (enumerator order). */
+enum {
+  BSQL_OP_INVALID = 0,
+  BSQL_OP_RAND = 1,
+  BSQL_OP_NOT = 2, BSQL_OP_ABS = 3, BSQL_OP_FLOOR = 4, BSQL_OP_CEIL = 5,
+  BSQL_OP_SIN = 6, BSQL_OP_COS = 7, BSQL_OP_ASIN = 8, BSQL_OP_ACOS = 9,
+  BSQL_OP_TAN = 10, BSQL_OP_COTAN = 11, BSQL_OP_ATAN = 12, BSQL_OP_LN = 13,
+  BSQL_OP_LOG = 14, BSQL_OP_YEAR = 15, BSQL_OP_MONTH = 16, BSQL_OP_DAY = 17,
+  BSQL_OP_DAYOFWEEK = 18, BSQL_OP_HOUR = 19, BSQL_OP_MINUTE = 20,
+  BSQL_OP_SECOND = 21, BSQL_OP_IS_NULL = 22, BSQL_OP_IS_NOT_NULL = 23,
+  BSQL_OP_IS_TRUE = 24, BSQL_OP_IS_NOT_TRUE = 25, BSQL_OP_IS_NOT_FALSE = 26,
+  BSQL_OP_CAST_TINYINT = 27, BSQL_OP_CAST_SMALLINT = 28,
+  BSQL_OP_CAST_INTEGER = 29, BSQL_OP_CAST_BIGINT = 30,
+  BSQL_OP_CAST_FLOAT = 31, BSQL_OP_CAST_DOUBLE = 32, BSQL_OP_CAST_DATE = 33,
+  BSQL_OP_CAST_TIMESTAMP_SECONDS = 34, BSQL_OP_CAST_TIMESTAMP_MILLISECONDS = 35,
+  BSQL_OP_CAST_TIMESTAMP_MICROSECONDS = 36, BSQL_OP_CAST_TIMESTAMP = 37,
+  BSQL_OP_CAST_VARCHAR = 38, BSQL_OP_CHAR_LENGTH = 39,
+  BSQL_OP_STR_LOWER = 40, BSQL_OP_STR_UPPER = 41, BSQL_OP_STR_INITCAP = 42,
+  BSQL_OP_STR_REVERSE = 43,
+  BSQL_OP_ADD = 44, BSQL_OP_SUB = 45, BSQL_OP_MUL = 46, BSQL_OP_DIV = 47,
+  BSQL_OP_MOD = 48, BSQL_OP_POW = 49, BSQL_OP_ROUND = 50,
+  BSQL_OP_EQUAL = 51, BSQL_OP_NOT_EQUAL = 52, BSQL_OP_LESS = 53,
+  BSQL_OP_GREATER = 54, BSQL_OP_LESS_EQUAL = 55, BSQL_OP_GREATER_EQUAL = 56,
+  BSQL_OP_BITWISE_AND = 57, BSQL_OP_BITWISE_OR = 58, BSQL_OP_BITWISE_XOR = 59,
+  BSQL_OP_LOGICAL_AND = 60, BSQL_OP_LOGICAL_OR = 61,
+  BSQL_OP_FIRST_NON_MAGIC = 62, BSQL_OP_MAGIC_IF_NOT = 63,
+  BSQL_OP_STR_LIKE = 64, BSQL_OP_STR_SUBSTRING = 65,
+  BSQL_OP_STR_REGEXP_REPLACE = 66, BSQL_OP_STR_CONCAT = 67,
+  BSQL_OP_STR_REPLACE = 68, BSQL_OP_STR_LEFT = 69, BSQL_OP_STR_RIGHT = 70,
+  BSQL_OP_TO_DATE = 71, BSQL_OP_TO_TIMESTAMP = 72, BSQL_OP_STR_TRIM = 73,
+  BSQL_OP_IS_NOT_DISTINCT_FROM = 74,
+  BSQL_OP_COUNT = 75
+};
+
+/* Largest slot index a program may use (interpreter_cpp.cu:466). */
+#define BSQL_MAX_SLOT 63
+
+/* ---- columns: Arrow/cudf layout -------------------------------------------
+ * Mirrors cudf::column_view as used by BlazingColumn/BlazingTable
+ * (engine/src/execution_kernels/LogicPrimitives.h:23-89).
+ *   data      fixed-width elements (STRING: the chars buffer)
+ *   validity  32-bit words, LSB-first, 1 = valid; NULL means all valid
+ *   offsets   STRING only: size+1 int32 offsets into data
+ *   offset    first row of the view (row units; bit units into validity)   */
+typedef struct bsql_column {
+  int32_t type_id;
+  int32_t reserved;
+  int64_t size;
+  int64_t offset;
+  void* data;
+  uint32_t* validity;
+  int32_t* offsets;
+} bsql_column;
+
+/* ---- scalars (literal operands) --------------------------------------------
+ * One per program operand that is BSQL_SCALAR_INDEX. The typed value follows
+ * get_scalar_from_string (engine/src/parser/CalciteExpressionParsing.cpp:138-262):
+ *   integral/bool/timestamp/duration -> i64 (already narrowed to the type)
+ *   FLOAT32/FLOAT64                  -> f64 (FLOAT32 holds the float value)
+ *   STRING                           -> str/str_len (host memory)            */
+typedef struct bsql_scalar {
+  int32_t type_id;
+  int32_t str_len;
+  union { int64_t i64; double f64; } v;
+  const char* str;
+} bsql_scalar;
+
+typedef struct bsql_program bsql_program;
+
+/* ---- program construction ---------------------------------------------------
+ * bsql_program_from_arrays: the reference encoding as built by
+ * add_expression_to_interpreter_plan (engine/src/Interpreter/interpreter_cpp.h:50-59)
+ * and consumed by perform_interpreter_operation (interpreter_cpp.h:77-86):
+ * four parallel int16 vectors (left, right, out per op, final_out per output),
+ * the operator per op, and one left/right scalar per op (consulted only where
+ * the operand is BSQL_SCALAR_INDEX; may be NULL when no op has a scalar). */
+int bsql_program_from_arrays(const int16_t* left, const int16_t* right,
+                             const int16_t* out, const int32_t* ops, int n_ops,
+                             const int16_t* final_out, int n_final,
+                             const bsql_scalar* left_scalars,
+                             const bsql_scalar* right_scalars,
+                             bsql_program** prog);
+
+/* bsql_program_compile: Calcite expression text -> program, replacing the
+ * per-batch front-end of evaluate_expressions
+ * (engine/src/execution_kernels/LogicalProject.cpp:796-941): regex rewrite,
+ * AND/OR expansion, parse, CASE/ROUND/CAST rewrites, slot allocation.
+ * `in_types[n_in]` is the schema of the table the expressions refer to ($k).
+ * Expressions whose root is a bare column or literal are still compiled
+ * (as a copy) so that every expression gets an output slot.               */
+int bsql_program_compile(const char* const* exprs, int n_exprs,
+                         const int32_t* in_types, int n_in,
+                         bsql_program** prog);
+
+void bsql_program_free(bsql_program* prog);
+
+/* Introspection (reference encoding, after compile): counts and arrays. */
+int bsql_program_num_ops(const bsql_program* prog);
+int bsql_program_num_outputs(const bsql_program* prog);
+int bsql_program_num_inputs(const bsql_program* prog);
+/* Fills caller arrays of length num_ops / num_outputs; any may be NULL. */
+int bsql_program_arrays(const bsql_program* prog, int16_t* left, int16_t* right,
+                        int16_t* out, int32_t* ops, int16_t* final_out);
+/* Scalar of op i (side 0 = left, 1 = right); string pointer stays owned by prog. */
+int bsql_program_scalar(const bsql_program* prog, int op, int side,
+                        bsql_scalar* out);
+/* Output type of output k, for the given input schema (in_types[n_in]);
+ * follows get_output_type (engine/src/parser/expression_utils.cpp:115-266). */
+int bsql_program_output_type(const bsql_program* prog, const int32_t* in_types,
+                             int n_in, int k, int32_t* type_id);
+/* Input columns referenced by a compiled program, in slot order (the
+ * col_idx_map of LogicalProject.cpp:859-881); fills up to cap entries. */
+int bsql_program_used_columns(const bsql_program* prog, int32_t* cols, int cap);
+
+/* ---- execution ---------------------------------------------------------------
+ * bsql_interp_execute  <=> interops::perform_interpreter_operation
+ * (engine/src/Interpreter/interpreter_cpp.cu:440-594). `in` is the interpreter
+ * input table (slot k = in[k]); `out[k]` receives final output k, written with
+ * the column's own type, and its validity words (outputs must have a validity
+ * buffer, offset 0). Rows [0, num_rows).                                      */
+int bsql_interp_execute(const bsql_program* prog, const bsql_column* in,
+                        int n_in, bsql_column* out, int n_out, int64_t num_rows,
+                        void* stream);
+
+/* bsql_filter <=> ral::processor::process_filter + applyBooleanFilter
+ * (engine/src/execution_kernels/LogicalFilter.cpp:22-52): `pred` must have one
+ * BOOL8 output. Every column of `in` is compacted stably into `out[k]`
+ * (same type; capacity >= num_rows rows; validity written iff in[k] has one).
+ * A null or false predicate drops the row. The selected row count is written
+ * to the device int64 `*d_count` (no host sync).                              */
+int bsql_filter(const bsql_program* pred, const bsql_column* in, int n_in,
+                bsql_column* out, int64_t num_rows, int64_t* d_count,
+                void* stream);
+
+/* bsql_filter_project: fused WHERE + SELECT in one pass. Equivalent to
+ * process_filter followed by process_project on the filtered table
+ * (LogicalFilter.cpp:31-52 then LogicalProject.cpp:943-969), whose
+ * expressions `proj` refer to the same input slots. Output k (capacity >=
+ * num_rows, validity buffer required) receives projection output k for the
+ * selected rows, in row order; *d_count receives the selected row count.   */
+int bsql_filter_project(const bsql_program* pred, const bsql_program* proj,
+                        const bsql_column* in, int n_in, bsql_column* out,
+                        int n_out, int64_t num_rows, int64_t* d_count,
+                        void* stream);
+
+/* Conjunct order: for a predicate whose root is a left-deep AND chain (the
+ * shape expand_if_logical_op produces, CalciteExpressionParsing.cpp:330-398),
+ * evaluate its conjuncts in `order` (a permutation of 0..n_conj-1) with a
+ * wave-uniform early exit after each conjunct. The selected rows are the same
+ * for every order. order == NULL restores the static (plan) order.         */
+int bsql_program_num_conjuncts(const bsql_program* pred);
+int bsql_program_set_conjunct_order(bsql_program* pred, const int32_t* order,
+                                    int n_conj);
+
+/* ---- launch configuration (autotuner hook) ------------------------------ */
+typedef struct bsql_launch_config {
+  int32_t block_size;     /* threads per workgroup: 64, 128, 256, 512 */
+  int32_t rows_per_thread;/* 1, 2, 4, 8 */
+} bsql_launch_config;
+/* Sets the configuration used for programs of this one (0 fields = auto). */
+int bsql_program_set_launch(bsql_program* prog, const bsql_launch_config* cfg);
+
+/* ---- diagnostics --------------------------------------------------------- */
+const char* bsql_last_error(void);
+/* Device kernel statistics of the last launch on the calling thread:
+ * name of the dominant kernel and the number of workgroups launched. */
+int bsql_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BSQL_INTERP_H */

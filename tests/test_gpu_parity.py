@@ -1,0 +1,305 @@
+"""GPU parity: the HIP interpreter / filter / fused filter+project through the
+C ABI against the CPU oracle (bit-exact), the reference's golden vectors, and
+size-independent properties at large sizes."""
+import numpy as np
+import pytest
+
+from tests.helpers import (BOOL8, FLOAT64, INT32, INT64, TS_D, ExprGen, assert_same, device_columns,
+                           eval_program_both, host_columns, make_table)
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_prog(p):
+    return p.to_oracle_dict()
+
+
+def test_golden_numeric_program(bsql, gpu, golden):
+    from blazingsql_amd.interop import DeviceColumn, perform_interpreter_operation
+    g = golden["numeric_program"]
+    for name, t in g["types"].items():
+        rs = [None] * 4
+        rs[g["right_scalar_op"]] = (t, g["right_scalar_value"])
+        p = bsql.Program.from_arrays(g["left"], g["right"], g["out"], g["ops"], g["final_out"], right_scalars=rs)
+        conv = (lambda v: np.asarray(v) != 0) if name == "bool" else np.asarray
+        ins = [DeviceColumn.from_numpy(t, conv(g[c])) for c in ("col1", "col2", "col3")]
+        outs = [DeviceColumn.empty(t, 10), DeviceColumn.empty(t, 10)]
+        perform_interpreter_operation(p, ins, outs, 10)
+        (o1, v1), (o2, v2) = outs[0].to_numpy(), outs[1].to_numpy()
+        e1 = np.asarray(g["expected_col1"]).astype(np.int64)
+        e2 = np.asarray(g["expected_col2"]).astype(np.int64)
+        if name == "bool":
+            e1, e2 = e1 != 0, e2 != 0
+        np.testing.assert_array_equal(o1, e1.astype(o1.dtype), err_msg=name)
+        np.testing.assert_array_equal(o2, e2.astype(o2.dtype), err_msg=name)
+        assert v1.all() and v2.all()
+
+
+@pytest.mark.parametrize("unit", ["D", "s", "ms", "us", "ns"])
+def test_golden_timestamps(bsql, gpu, golden, unit):
+    from blazingsql_amd.interop import DeviceColumn, perform_interpreter_operation
+    g = golden["timestamp_components"]
+    t = g["types"][unit]
+    p = bsql.Program.from_arrays([0] * 7, [-1] * 7, [1, 2, 3, 4, 5, 6, 7], g["ops"], [1, 2, 3, 4, 5, 6, 7])
+    ins = [DeviceColumn.from_numpy(t, g["inputs"][unit])]
+    outs = [DeviceColumn.empty(INT32, 10) for _ in range(7)]
+    perform_interpreter_operation(p, ins, outs, 10)
+    exp = g["expected_days"] if unit == "D" else g["expected_other"]
+    for o, e in zip(outs, exp):
+        np.testing.assert_array_equal(o.to_numpy()[0], e)
+    g = golden["day_of_week"]
+    p = bsql.Program.compile(g["expression"], [t])
+    ins = [DeviceColumn.from_numpy(t, g["inputs"][unit])]
+    outs = [DeviceColumn.empty(2, 20)]
+    perform_interpreter_operation(p, ins, outs, 20)
+    np.testing.assert_array_equal(outs[0].to_numpy()[0], g["expected_days"] if unit == "D" else g["expected_other"])
+
+
+@pytest.mark.parametrize("nulls", [False, True])
+def test_golden_filter_mod(bsql, gpu, golden, nulls):
+    import torch
+    from blazingsql_amd.interop import DeviceColumn, filter_columns
+    g = golden["filter_mod"]
+    for name, t in g["types"].items():
+        conv = (lambda v: np.asarray(v) != 0) if name == "bool" else np.asarray
+        valid = g["null_validity"] if nulls else None
+        ins = [DeviceColumn.from_numpy(t, conv(g["col1"]), valid), DeviceColumn.from_numpy(t, conv(g["col2"]), valid)]
+        outs = [DeviceColumn.empty(t, 40, nullable=nulls) for _ in range(2)]
+        p = bsql.Program.compile(g["expression"], [t, t])
+        cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+        filter_columns(p, ins, outs, cnt, 40)
+        n = int(cnt.item())
+        if name == "bool":
+            assert n == (20 if nulls else 40)
+            continue
+        assert n == 20, name
+        (o1, v1), (o2, v2) = outs[0].to_numpy(n), outs[1].to_numpy(n)
+        np.testing.assert_array_equal(o1, np.asarray(g["expected_col1"]).astype(o1.dtype))
+        np.testing.assert_array_equal(o2, np.asarray(g["expected_col2"]).astype(o2.dtype))
+        assert v1.all() and v2.all()
+
+
+def test_golden_round_and_compare(bsql, gpu, golden):
+    from blazingsql_amd.interop import DeviceColumn, perform_interpreter_operation
+    g = golden["round"]
+    for t, dt in ((9, np.float32), (10, np.float64)):
+        p = bsql.Program.compile(["ROUND($0)", "ROUND($0, 2)", "ROUND($0, 5)"], [t])
+        ins = [DeviceColumn.from_numpy(t, np.asarray(g["col1"], dtype=dt))]
+        outs = [DeviceColumn.empty(t, 7) for _ in range(3)]
+        perform_interpreter_operation(p, ins, outs, 7)
+        for o, e in zip(outs, g["expected"]):
+            np.testing.assert_array_equal(o.to_numpy()[0], np.asarray(e, dtype=dt))
+    g = golden["project_compare"]
+    p = bsql.Program.compile([">($0, 3)", "<($2, 3)"], [INT32, INT32, INT32])
+    ins = [DeviceColumn.from_numpy(INT32, g["col1"]), DeviceColumn.from_numpy(INT32, g["col1"]),
+           DeviceColumn.from_numpy(INT32, g["col3"])]
+    outs = [DeviceColumn.empty(BOOL8, 7), DeviceColumn.empty(BOOL8, 7)]
+    perform_interpreter_operation(p, ins, outs, 7)
+    np.testing.assert_array_equal(outs[0].to_numpy()[0], g["expected_col1"])
+    np.testing.assert_array_equal(outs[1].to_numpy()[0], g["expected_col2"])
+
+
+def test_golden_strings(bsql, gpu, golden):
+    """String comparison and CHAR_LENGTH inside the VM (filter_tests.cpp:178-214,
+    process_project.cpp:350-366)."""
+    from blazingsql_amd.interop import DeviceColumn, perform_interpreter_operation
+    g = golden["filter_strings"]
+    strs = [s.encode() for s in g["col1"]]
+    offs = np.concatenate([[0], np.cumsum([len(s) for s in strs])]).astype(np.int32)
+    chars = np.frombuffer(b"".join(strs) + b"\0", dtype=np.uint8)
+    valid = np.asarray(g["null_validity"], dtype=bool)
+    col = DeviceColumn.from_numpy(23, None, valid, chars=chars, offsets=offs)
+    p = bsql.Program.compile(["=($0, 'bar')", ">($0, 'd')"], [23])
+    outs = [DeviceColumn.empty(BOOL8, 10), DeviceColumn.empty(BOOL8, 10)]
+    perform_interpreter_operation(p, [col], outs, 10)
+    eq, veq = outs[0].to_numpy()
+    gt, vgt = outs[1].to_numpy()
+    np.testing.assert_array_equal(veq, valid)
+    sel = [s for s, v, ok in zip(g["col1"], gt, vgt) if ok and v]
+    assert sel == g["gt_expected"]["col1"]
+    assert [s for s, v, ok in zip(g["col1"], eq, veq) if ok and v] == []  # 'bar' is null here
+    g = golden["char_length"]
+    strs = [s.encode() for s in g["col1"]]
+    offs = np.concatenate([[0], np.cumsum([len(s) for s in strs])]).astype(np.int32)
+    chars = np.frombuffer(b"".join(strs) + b"\0", dtype=np.uint8)
+    col = DeviceColumn.from_numpy(23, None, np.asarray(g["valid"], dtype=bool), chars=chars, offsets=offs)
+    p = bsql.Program.compile(["-(CHAR_LENGTH($0), 1)"], [23])
+    outs = [DeviceColumn.empty(INT32, 4)]
+    perform_interpreter_operation(p, [col], outs, 4)
+    o, v = outs[0].to_numpy()
+    np.testing.assert_array_equal(v, np.asarray(g["expected_valid"], dtype=bool))
+    np.testing.assert_array_equal(o[v], np.asarray(g["expected"])[v])
+
+
+TYPES6 = [INT32, INT64, FLOAT64, FLOAT64, TS_D, INT32]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_expressions_vs_oracle(bsql, gpu, seed):
+    rng = np.random.default_rng(100 + seed)
+    types = TYPES6 + [BOOL8]
+    n = [1, 31, 257, 1025, 4099, 70001][seed]
+    table = make_table(rng, types, n, null_rates=list(rng.uniform(0, 0.3, len(types))))
+    gen = ExprGen(rng, types)
+    exprs = []
+    while len(exprs) < 12:
+        e = gen.boolean(3) if rng.random() < 0.4 else gen.numeric(4)
+        if e and "$" in e:
+            exprs.append(e)
+    for e in exprs:
+        try:
+            res = eval_program_both(bsql, [e], table, n)
+        except Exception as err:  # programs the interpreter rejects must be rejected on compile
+            assert "bsql error 2" in str(err), (e, err)
+            continue
+        for (gv, gvalid), (ov, ovalid), t in res:
+            assert_same(gv, gvalid, ov, ovalid, what=e)
+
+
+def _c1_table(n, seed=7):
+    rng = np.random.default_rng(seed)
+    a = rng.integers(0, 1000000, n, dtype=np.int64)
+    b = rng.random(n)
+    c = rng.integers(0, 100, n, dtype=np.int32)
+    cvalid = rng.random(n) >= 0.05
+    return [(INT64, a, None), (FLOAT64, b, None), (INT32, c, cvalid)]
+
+
+C1_PRED = "AND(>($2, 50), <($1, 0.3:DECIMAL(2, 1)))"
+C1_PROJ = ["+($0, *($2, 3))", "-(*($1, 2.0:DECIMAL(2, 1)), 1.0:DECIMAL(2, 1))"]
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, 1 << 20])
+def test_filter_project_c1_vs_oracle(bsql, gpu, n):
+    import oracle
+    import torch
+    from blazingsql_amd.interop import DeviceColumn, filter_project
+    table = _c1_table(n)
+    types = [t for t, _, _ in table]
+    pred = bsql.Program.compile(C1_PRED, types)
+    proj = bsql.Program.compile(C1_PROJ, types)
+    outs = [DeviceColumn.empty(INT64, n), DeviceColumn.empty(FLOAT64, n)]
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    filter_project(pred, proj, device_columns(table), outs, cnt, n)
+    m = int(cnt.item())
+    # the oracle runs the two reference operators: filter, then project
+    on, ores = oracle.filter_project(_full_prog(bsql, C1_PRED, types), _full_prog(bsql, C1_PROJ, types),
+                                     host_columns(table), [INT64, FLOAT64], n)
+    assert m == on
+    for k in range(2):
+        gv, gval = outs[k].to_numpy(m)
+        assert_same(gv, gval, ores[k][0], ores[k][1], what=f"C1 output {k}")
+    if n:
+        # independent restatement of the query in numpy
+        a, b, c = table[0][1], table[1][1], table[2][1]
+        sel = table[2][2] & (c > 50) & (b < 0.3)
+        np.testing.assert_array_equal(outs[0].to_numpy(m)[0], a[sel] + c[sel].astype(np.int64) * 3)
+        np.testing.assert_array_equal(outs[1].to_numpy(m)[0], b[sel] * 2.0 - 1.0)
+
+
+def _full_prog(bsql, exprs, types):
+    """A program over the full table for the oracle (identity column map)."""
+    p = bsql.Program.compile(exprs, types)
+    d = p.to_oracle_dict()
+    used = p.used_columns()
+    # remap input slots back to table columns; shift other slots past the table
+    n_in, n_tab = len(used), len(types)
+    def remap(s):
+        if s < 0:
+            return s
+        return used[s] if s < n_in else s - n_in + n_tab
+    for key in ("left", "right", "out", "final_out"):
+        d[key] = [remap(s) for s in d[key]]
+    return d
+
+
+def c2_table(n, seed=11):
+    rng = np.random.default_rng(seed)
+    qty = rng.integers(1, 51, n, dtype=np.int32)
+    price = rng.uniform(900.0, 105000.0, n)
+    disc = rng.integers(0, 11, n) / 100.0
+    tax = rng.integers(0, 9, n) / 100.0
+    ship = rng.integers(8036, 10562, n, dtype=np.int32)
+    return [(INT32, qty, None), (FLOAT64, price, None), (FLOAT64, disc, None), (FLOAT64, tax, None),
+            (INT32, ship, None)]
+
+
+C2_PRED = "AND(<=($4, 10471), >=($2, 0.05:DECIMAL(3, 2)), <=($2, 0.07:DECIMAL(3, 2)), <($0, 24))"
+C2_PROJ = ["*(*($1, -(1, $2)), +(1, $3))", "*($1, $2)"]
+
+
+@pytest.mark.parametrize("order", [None, [3, 1, 2, 0], [1, 0, 3, 2]])
+def test_filter_project_c2_orders_vs_oracle(bsql, gpu, order):
+    import oracle
+    import torch
+    from blazingsql_amd.interop import DeviceColumn, filter_project
+    n = 3_000_017
+    table = c2_table(n)
+    types = [t for t, _, _ in table]
+    pred = bsql.Program.compile(C2_PRED, types)
+    proj = bsql.Program.compile(C2_PROJ, types)
+    if order is not None:
+        pred.set_conjunct_order(order)
+    outs = [DeviceColumn.empty(FLOAT64, n), DeviceColumn.empty(FLOAT64, n)]
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    filter_project(pred, proj, device_columns(table), outs, cnt, n)
+    m = int(cnt.item())
+    on, ores = oracle.filter_project(_full_prog(bsql, C2_PRED, types), _full_prog(bsql, C2_PROJ, types),
+                                     host_columns(table), [FLOAT64, FLOAT64], n)
+    assert m == on and 0.10 < m / n < 0.14
+    for k in range(2):
+        gv, gval = outs[k].to_numpy(m)
+        assert_same(gv, gval, ores[k][0], ores[k][1], what=f"C2 output {k}")
+
+
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 255, 256, 257, 1023, 1024, 1025, 4096 + 3, 100003])
+@pytest.mark.parametrize("sel", ["none", "all", "half", "sparse"])
+def test_filter_gather_edges_vs_oracle(bsql, gpu, n, sel):
+    import oracle
+    import torch
+    from blazingsql_amd.interop import DeviceColumn, filter_columns
+    rng = np.random.default_rng(n * 7 + len(sel))
+    table = make_table(rng, [INT32, FLOAT64, INT64, INT32], n, null_rates=[0.1, 0.0, 0.3, 0.0])
+    thr = {"none": ">($3, 100000)", "all": ">($3, -100000)", "half": ">($3, 0)", "sparse": ">($3, 990)"}[sel]
+    types = [t for t, _, _ in table]
+    pred = bsql.Program.compile(thr, types)
+    outs = [DeviceColumn.empty(t, n, nullable=v is not None) for t, _, v in table]
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    filter_columns(pred, device_columns(table), outs, cnt, n)
+    m = int(cnt.item())
+    on, ores = oracle.filter_table(_full_prog(bsql, thr, types), host_columns(table), n)
+    assert m == on
+    for k in range(4):
+        gv, gval = outs[k].to_numpy(m)
+        assert_same(gv, gval, ores[k][0], ores[k][1], what=f"col {k}")
+
+
+def test_transcendentals_within_one_ulp(bsql, gpu):
+    """FP64 math functions: within 1 ulp of the host libm."""
+    rng = np.random.default_rng(5)
+    n = 20000
+    y = rng.uniform(1e-3, 1e3, n)
+    table = [(FLOAT64, y, None), (FLOAT64, rng.uniform(-1, 1, n), None)]
+    exprs = ["SIN($0)", "COS($0)", "TAN($0)", "LN($0)", "LOG10($0)", "POWER($0, 2.5:DECIMAL(2, 1))",
+             "ATAN($0)", "ASIN($1)", "ACOS($1)", "FLOOR($0)", "CEIL($0)"]
+    res = eval_program_both(bsql, exprs, table, n)
+    for e, ((gv, gval), (ov, oval), t) in zip(exprs, res):
+        np.testing.assert_array_equal(gval, oval)
+        gi, oi = gv.view(np.int64), ov.view(np.int64)
+        ulp = np.abs(gi - oi)
+        assert ulp.max() <= 1, (e, int(ulp.max()))
+        if e.startswith(("FLOOR", "CEIL")):
+            assert ulp.max() == 0
+
+
+def test_empty_and_single_row_execute(bsql, gpu):
+    from blazingsql_amd.interop import DeviceColumn, perform_interpreter_operation
+    p = bsql.Program.compile(["+($0, 1)"], [INT32])
+    ins = [DeviceColumn.from_numpy(INT32, np.zeros(0, dtype=np.int32))]
+    outs = [DeviceColumn.empty(INT32, 0)]
+    perform_interpreter_operation(p, ins, outs, 0)
+    ins = [DeviceColumn.from_numpy(INT32, np.array([41], dtype=np.int32), np.array([True]))]
+    outs = [DeviceColumn.empty(INT32, 1)]
+    perform_interpreter_operation(p, ins, outs, 1)
+    v, ok = outs[0].to_numpy()
+    assert v[0] == 42 and ok[0]

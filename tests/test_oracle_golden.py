@@ -1,0 +1,264 @@
+"""Pins the CPU oracle (oracle/interp_oracle.cpp) before it is trusted.
+
+1. Against the golden vectors of the reference's own unit tests
+   (tests/golden/reference_vectors.json, transcribed from engine/tests/new-tests).
+2. Against what the mathematics fixes: IEEE-754 binary64 arithmetic (numpy),
+   two's-complement int64 wrap-around (numpy int64), the proleptic Gregorian
+   calendar (Python datetime), and the truth tables of the reference's null
+   handling (interpreter_ops.cuh:559-598, 721-727, 744-755).
+"""
+import datetime
+
+import numpy as np
+import pytest
+
+from blazingsql_amd._lib import OP
+
+pytestmark = []
+
+
+def prog(left, right, out, ops, final_out, left_scalars=None, right_scalars=None):
+    n = len(ops)
+    return dict(left=left, right=right, out=out, ops=[OP[o] if isinstance(o, str) else o for o in ops],
+                final_out=final_out, left_scalars=left_scalars or [None] * n,
+                right_scalars=right_scalars or [None] * n)
+
+
+def test_numeric_program(oracle_lib, golden):
+    g = golden["numeric_program"]
+    from oracle import HostColumn
+    for name, t in g["types"].items():
+        rs = [None] * 4
+        rs[g["right_scalar_op"]] = (t, g["right_scalar_value"])
+        p = prog(g["left"], g["right"], g["out"], g["ops"], g["final_out"], right_scalars=rs)
+        cast = {"bool": lambda v: np.asarray(v) != 0}.get(name, lambda v: np.asarray(v))
+        cols = [HostColumn(t, cast(g[c])) for c in ("col1", "col2", "col3")]
+        (o1, v1), (o2, v2) = oracle_lib.execute(p, cols, [t, t])
+        dt = o1.dtype
+        exp1 = np.asarray(g["expected_col1"]).astype(np.int64)
+        exp2 = np.asarray(g["expected_col2"]).astype(np.int64)
+        if name == "bool":
+            exp1, exp2 = exp1 != 0, exp2 != 0
+        np.testing.assert_array_equal(o1, exp1.astype(dt), err_msg=name)
+        np.testing.assert_array_equal(o2, exp2.astype(dt), err_msg=name)
+        assert v1.all() and v2.all()
+
+
+@pytest.mark.parametrize("unit", ["D", "s", "ms", "us", "ns"])
+def test_day_of_week(oracle_lib, golden, unit):
+    g = golden["day_of_week"]
+    from oracle import HostColumn
+    t = g["types"][unit]
+    p = prog([0], [-1], [1], ["DAYOFWEEK"], [1])
+    (o, v), = oracle_lib.execute(p, [HostColumn(t, g["inputs"][unit])], [3])
+    exp = g["expected_days"] if unit == "D" else g["expected_other"]
+    np.testing.assert_array_equal(o, exp)
+
+
+@pytest.mark.parametrize("unit", ["D", "s", "ms", "us", "ns"])
+def test_timestamp_components(oracle_lib, golden, unit):
+    g = golden["timestamp_components"]
+    from oracle import HostColumn
+    t = g["types"][unit]
+    p = prog([0] * 7, [-1] * 7, [1, 2, 3, 4, 5, 6, 7], g["ops"], [1, 2, 3, 4, 5, 6, 7])
+    outs = oracle_lib.execute(p, [HostColumn(t, g["inputs"][unit])], [3] * 7)
+    exp = g["expected_days"] if unit == "D" else g["expected_other"]
+    for (o, v), e in zip(outs, exp):
+        np.testing.assert_array_equal(o, e)
+
+
+@pytest.mark.parametrize("nulls", [False, True])
+def test_filter_mod(oracle_lib, golden, nulls):
+    g = golden["filter_mod"]
+    from oracle import HostColumn
+    enc = g["reference_encoding"]
+    for name, t in g["types"].items():
+        p = prog(enc["left"], enc["right"], enc["out"], enc["ops"], enc["final_out"],
+                 right_scalars=[tuple(s) if s else None for s in enc["right_scalars"]])
+        conv = (lambda v: np.asarray(v) != 0) if name == "bool" else np.asarray
+        valid = g["null_validity"] if nulls else None
+        cols = [HostColumn(t, conv(g["col1"]), valid), HostColumn(t, conv(g["col2"]), valid)]
+        n, res = oracle_lib.filter_table(p, cols)
+        if name == "bool" and not nulls:
+            # bool(r) + bool(2r) is 0 or 2: every row passes
+            assert n == 40
+            continue
+        if name == "bool":
+            assert n == 20
+            continue
+        assert n == 20, name
+        np.testing.assert_array_equal(res[0][0], conv(g["expected_col1"]).astype(res[0][0].dtype))
+        np.testing.assert_array_equal(res[1][0], conv(g["expected_col2"]).astype(res[1][0].dtype))
+        assert res[0][1].all() and res[1][1].all()
+
+
+def test_filter_strings(oracle_lib, golden):
+    g = golden["filter_strings"]
+    from oracle import HostColumn
+    p = prog([0], [-2], [2], ["EQUAL"], [2], right_scalars=[(23, "bar")])
+    n, res = oracle_lib.filter_table(p, [HostColumn(23, strings=g["col1"]), HostColumn(3, g["col2"])])
+    assert n == 1 and res[0][0] == g["eq_expected"]["col1"] and list(res[1][0]) == g["eq_expected"]["col2"]
+    valid = [bool(x) for x in g["null_validity"]]
+    p = prog([0], [-2], [2], ["GREATER"], [2], right_scalars=[(23, "d")])
+    n, res = oracle_lib.filter_table(p, [HostColumn(23, strings=g["col1"], valid=valid),
+                                         HostColumn(3, g["col2"], valid)])
+    assert res[0][0] == g["gt_expected"]["col1"] and list(res[1][0]) == g["gt_expected"]["col2"]
+    assert res[0][1].all() and res[1][1].all()
+
+
+def test_project_compare(oracle_lib, golden):
+    g = golden["project_compare"]
+    from oracle import HostColumn
+    # >($0, 3) and <($2, 3) over (col1, col3): slots 0, 1 inputs; outputs 2, 3
+    p = prog([0, 1], [-2, -2], [2, 3], ["GREATER", "LESS"], [2, 3], right_scalars=[(3, 3), (3, 3)])
+    for t in (1, 2, 3, 4, 9, 10):
+        (o1, _), (o2, _) = oracle_lib.execute(p, [HostColumn(t, g["col1"]), HostColumn(t, g["col3"])], [11, 11])
+        np.testing.assert_array_equal(o1, g["expected_col1"])
+        np.testing.assert_array_equal(o2, g["expected_col2"])
+    (o1, _), (o2, _) = oracle_lib.execute(
+        p, [HostColumn(11, np.asarray(g["col1"]) != 0), HostColumn(11, np.asarray(g["col3"]) != 0)], [11, 11])
+    np.testing.assert_array_equal(o1, g["expected_bool_col1"])
+    np.testing.assert_array_equal(o2, g["expected_bool_col2"])
+
+
+def test_char_length(oracle_lib, golden):
+    g = golden["char_length"]
+    from oracle import HostColumn
+    p = prog([0, 2], [-1, -2], [2, 1], ["CHAR_LENGTH", "SUB"], [1], right_scalars=[None, (3, 1)])
+    (o, v), = oracle_lib.execute(p, [HostColumn(23, strings=g["col1"], valid=[bool(x) for x in g["valid"]])], [3])
+    np.testing.assert_array_equal(v, np.asarray(g["expected_valid"], dtype=bool))
+    np.testing.assert_array_equal(o[v], np.asarray(g["expected"])[v])
+
+
+@pytest.mark.parametrize("t", [9, 10])
+def test_round(oracle_lib, golden, t):
+    g = golden["round"]
+    from oracle import HostColumn
+    # ROUND($0) gets the default INT8 0 argument (expression_tree.hpp:228-236)
+    p = prog([0, 0, 0], [-2, -2, -2], [1, 2, 3], ["ROUND"] * 3, [1, 2, 3],
+             right_scalars=[(1, 0), (3, 2), (3, 5)])
+    dt = np.float32 if t == 9 else np.float64
+    outs = oracle_lib.execute(p, [HostColumn(t, np.asarray(g["col1"], dtype=dt))], [t] * 3)
+    for (o, v), e in zip(outs, g["expected"]):
+        np.testing.assert_array_equal(o, np.asarray(e, dtype=dt))
+
+
+def test_apply_filter_nulls(oracle_lib, golden):
+    g = golden["apply_filter"]
+    from oracle import HostColumn
+    b = lambda k: [bool(x) for x in g[k]]
+    for t in (1, 2, 3, 4, 9, 10):
+        cols = [HostColumn(t, g["col1"], b("col1_valid")), HostColumn(23, strings=g["col2"], valid=b("col2_valid")),
+                HostColumn(t, g["col3"], b("col3_valid")), HostColumn(11, np.asarray(g["mask"]) != 0, b("mask_valid"))]
+        # the predicate is the BOOL8 mask column itself: no ops, output slot 3
+        p = prog([], [], [], [], [3])
+        n, res = oracle_lib.filter_table(p, cols)
+        assert n == 3
+        np.testing.assert_array_equal(res[0][0], g["expected_col1"])
+        np.testing.assert_array_equal(res[0][1], b("expected_col1_valid"))
+        assert res[1][0] == g["expected_col2"]
+        np.testing.assert_array_equal(res[1][1], b("expected_col2_valid"))
+        np.testing.assert_array_equal(res[2][1], b("expected_col3_valid"))
+        np.testing.assert_array_equal(res[2][0][res[2][1]], np.asarray(g["expected_col3"])[b("expected_col3_valid")])
+
+
+# ---------------------------------------------------------------- mathematics
+
+def test_ieee_binary64_arithmetic(oracle_lib):
+    """+ - * / of two FLOAT64 columns round once (numpy is IEEE binary64)."""
+    from oracle import HostColumn
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(5000) * 10.0 ** rng.integers(-30, 30, 5000)
+    b = rng.standard_normal(5000) * 10.0 ** rng.integers(-30, 30, 5000)
+    p = prog([0, 0, 0, 0], [1, 1, 1, 1], [2, 3, 4, 5], ["ADD", "SUB", "MUL", "DIV"], [2, 3, 4, 5])
+    outs = oracle_lib.execute(p, [HostColumn(10, a), HostColumn(10, b)], [10] * 4)
+    with np.errstate(all="ignore"):
+        for (o, v), e in zip(outs, (a + b, a - b, a * b, a / b)):
+            assert np.array_equal(o.view(np.uint64), e.view(np.uint64))
+
+
+def test_int64_wraparound_and_division(oracle_lib):
+    from oracle import HostColumn
+    rng = np.random.default_rng(2)
+    a = rng.integers(-2**63, 2**63 - 1, 4000, dtype=np.int64)
+    b = rng.integers(-2**63, 2**63 - 1, 4000, dtype=np.int64)
+    b[::7] = 0
+    b[3::11] = -1
+    p = prog([0, 0, 0, 0, 0], [1, 1, 1, 1, 1], [2, 3, 4, 5, 6], ["ADD", "SUB", "MUL", "DIV", "MOD"],
+             [2, 3, 4, 5, 6])
+    (add, _), (sub, _), (mul, _), (div, vdiv), (mod, _) = oracle_lib.execute(
+        p, [HostColumn(4, a), HostColumn(4, b)], [4] * 5)
+    with np.errstate(all="ignore"):
+        np.testing.assert_array_equal(add, a + b)
+        np.testing.assert_array_equal(sub, a - b)
+        np.testing.assert_array_equal(mul, a * b)
+    ok = (b != 0) & (b != -1)
+    # C++ integer division truncates toward zero; % takes the dividend's sign
+    xs, ys = a[ok].tolist(), b[ok].tolist()
+    exp_div = np.array([(abs(x) // abs(y)) * (1 if (x >= 0) == (y > 0) else -1) for x, y in zip(xs, ys)],
+                       dtype=np.int64)
+    np.testing.assert_array_equal(div[ok], exp_div)
+    np.testing.assert_array_equal(vdiv, b != 0)   # x / 0 is null (interpreter_ops.cuh:721-724)
+    exp_mod = np.array([x - q * y for x, y, q in zip(xs, ys, exp_div.tolist())], dtype=np.int64)
+    np.testing.assert_array_equal(mod[ok], exp_mod)
+
+
+def test_calendar_against_datetime(oracle_lib):
+    from oracle import HostColumn
+    rng = np.random.default_rng(3)
+    days = rng.integers(-200000, 200000, 3000)
+    p = prog([0, 0, 0, 0], [-1, -1, -1, -1], [1, 2, 3, 4], ["YEAR", "MONTH", "DAY", "DAYOFWEEK"], [1, 2, 3, 4])
+    (y, _), (m, _), (d, _), (w, _) = oracle_lib.execute(p, [HostColumn(12, days)], [3] * 4)
+    epoch = datetime.date(1970, 1, 1)
+    for i, dd in enumerate(days.tolist()):
+        dt = epoch + datetime.timedelta(days=dd)
+        assert (y[i], m[i], d[i], w[i]) == (dt.year, dt.month, dt.day, dt.isoweekday())
+
+
+def test_null_truth_tables(oracle_lib):
+    """AND/OR/IS_NOT_DISTINCT_FROM/IS_NOT_TRUE/IS_NOT_FALSE over {false, true, null}^2,
+    as the reference's branches define them (interpreter_ops.cuh:575-601, 744-755)."""
+    from oracle import HostColumn
+    # all nine (left, right) pairs over {false, true, null}
+    lvals = [0, 0, 0, 1, 1, 1, 0, 0, 0]
+    lv = [1, 1, 1, 1, 1, 1, 0, 0, 0]
+    rvals = [0, 1, 0, 0, 1, 0, 0, 1, 0]
+    rv = [1, 1, 0, 1, 1, 0, 1, 1, 0]
+    a = HostColumn(11, np.asarray(lvals) != 0, np.asarray(lv, dtype=bool))
+    b = HostColumn(11, np.asarray(rvals) != 0, np.asarray(rv, dtype=bool))
+    p = prog([0, 0, 0, 0, 0], [1, 1, 1, -1, -1], [2, 3, 4, 5, 6],
+             ["LOGICAL_AND", "LOGICAL_OR", "IS_NOT_DISTINCT_FROM", "IS_NOT_TRUE", "IS_NOT_FALSE"],
+             [2, 3, 4, 5, 6])
+    (and_, vand), (or_, vor), (indf, vindf), (nt, vnt), (nf, vnf) = oracle_lib.execute(p, [a, b], [11] * 5)
+    # AND: null if either side is null, else l && r
+    np.testing.assert_array_equal(vand, [1, 1, 0, 1, 1, 0, 0, 0, 0])
+    np.testing.assert_array_equal(and_[vand], [0, 0, 0, 1])
+    # OR: both valid -> l||r; left only -> valid iff left is true (data = left);
+    # right only -> valid (data = right), so OR(null, false) is a valid false
+    np.testing.assert_array_equal(vor, [1, 1, 0, 1, 1, 1, 1, 1, 0])
+    np.testing.assert_array_equal(or_[vor], [0, 1, 1, 1, 1, 0, 1])
+    # IS NOT DISTINCT FROM: always valid; both null -> true; one null -> false
+    assert vindf.all()
+    np.testing.assert_array_equal(indf, [1, 0, 0, 0, 1, 0, 0, 0, 1])
+    np.testing.assert_array_equal(nt, [1, 1, 1, 0, 0, 0, 1, 1, 1])
+    np.testing.assert_array_equal(nf, [0, 0, 0, 1, 1, 1, 1, 1, 1])
+    assert vnt.all() and vnf.all()
+
+
+def test_case_magic_encoding(oracle_lib):
+    """CASE(c, t, e) = FIRST_NON_MAGIC(MAGIC_IF_NOT(c, t), e) (expression_tree.hpp:200-220)."""
+    from oracle import HostColumn
+    c = HostColumn(11, np.array([1, 0, 1, 0]) != 0, np.array([1, 1, 0, 0], dtype=bool))
+    t = HostColumn(4, np.array([10, 20, 30, 40]), np.array([1, 1, 1, 0], dtype=bool))
+    e = HostColumn(4, np.array([-1, -2, -3, -4]))
+    p = prog([0, 4], [1, 2], [4, 3], ["MAGIC_IF_NOT", "FIRST_NON_MAGIC"], [3])
+    (o, v), = oracle_lib.execute(p, [c, t, e], [4])
+    np.testing.assert_array_equal(o, [10, -2, -3, -4])
+    assert v.all()
+
+
+def test_type_propagation(oracle_lib):
+    """Operand types follow interpreter_cpp.cu:502-550 and get_output_type."""
+    p = prog([0, 3, 3], [1, -2, 2], [3, 3, 4], ["ADD", "MUL", "LESS"], [4], right_scalars=[None, (10, 2.0), None])
+    lt, rt, ot = oracle_lib.op_types(p, [3, 4, 10])
+    assert (lt, rt, ot) == ([3, 4, 10], [4, 10, 10], [4, 10, 11])
