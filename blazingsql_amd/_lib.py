@@ -56,7 +56,11 @@ class Scalar(ctypes.Structure):
 
 
 class LaunchConfig(ctypes.Structure):
-    _fields_ = [("block_size", ctypes.c_int32), ("rows_per_thread", ctypes.c_int32)]
+    _fields_ = [("block_size", ctypes.c_int32), ("rows_per_thread", ctypes.c_int32),
+                ("flags", ctypes.c_int32)]
+
+
+LAUNCH_NO_RANGES, LAUNCH_LAZY_LOADS = 1, 2
 
 
 # exported symbols and their signatures, as declared in include/bsql/interp.h

@@ -18,10 +18,11 @@ struct bsql_program : bsql::Program {
 
 namespace bsql {
 
-hipError_t launch_execute(const DevPlan& plan, int nreg, int rpt, hipStream_t stream);
+hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt,
+                          hipStream_t stream);
 int64_t compact_tiles(int64_t num_rows, int nreg, int rpt);
-hipError_t launch_compact(const DevPlan& plan, int nreg, int rpt, uint64_t* status,
-                          int64_t* d_count, int64_t capacity, hipStream_t stream);
+hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt,
+                          uint64_t* status, int64_t* d_count, int64_t capacity, hipStream_t stream);
 
 namespace {
 
@@ -38,8 +39,10 @@ int device_error(hipError_t e, const char* where) {
   return set_error(BSQL_ERR_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
 }
 
-// Compaction scratch (tile status words + ticket), one buffer per stream so
-// that concurrent calls on different streams never share it.
+// Per-stream scratch: the device copy of the launch plan, then the compaction
+// tile status words + ticket. One buffer per stream, so that concurrent calls
+// on different streams never share it.
+constexpr int64_t kPlanWords = (int64_t)((sizeof(DevPlan) + 255) / 256 * 256 / 8);
 struct Scratch {
   uint64_t* p = nullptr;
   int64_t words = 0;
@@ -53,7 +56,7 @@ hipError_t get_scratch(hipStream_t stream, int64_t words, uint64_t** out) {
   if (e != hipSuccess) return e;
   std::lock_guard<std::mutex> lk(g_scratch_mu);
   Scratch& s = g_scratch[{dev, stream}];
-  if (s.words < words) {
+  if (!s.p || s.words < words) {
     if (s.p) {
       // the previous buffer may still be read by queued work on this stream
       e = hipStreamSynchronize(stream);
@@ -62,13 +65,18 @@ hipError_t get_scratch(hipStream_t stream, int64_t words, uint64_t** out) {
       s.p = nullptr;
       s.words = 0;
     }
-    const int64_t n = words < 4096 ? 4096 : words + words / 4;
+    const int64_t n = (words < 4096 ? 4096 : words + words / 4) + kPlanWords;
     e = hipMalloc((void**)&s.p, (size_t)n * sizeof(uint64_t));
     if (e != hipSuccess) return e;
-    s.words = n;
+    s.words = n - kPlanWords;
   }
   *out = s.p;
   return hipSuccess;
+}
+
+// Copies the launch plan into the stream's scratch (stream-ordered).
+hipError_t upload_plan(const DevPlan& plan, uint64_t* scratch, hipStream_t stream) {
+  return hipMemcpyAsync(scratch, &plan, sizeof(DevPlan), hipMemcpyHostToDevice, stream);
 }
 
 // string literals -> one device buffer of {int32 len, bytes} records
@@ -145,6 +153,7 @@ std::string cache_key(int mode, const bsql_program* pred, const bsql_program* pr
   if (pred) {
     k += "|o";
     for (int32_t c : pred->conj_order) k += std::to_string(c) + ",";
+    k += "|f" + std::to_string(pred->launch.flags);
   }
   return k;
 }
@@ -322,10 +331,13 @@ int bsql_program_set_conjunct_order(bsql_program* pred, const int32_t* order, in
 int bsql_program_set_launch(bsql_program* p, const bsql_launch_config* cfg) {
   if (!p || !cfg) return set_error(BSQL_ERR_INVALID, "invalid arguments");
   const int r = cfg->rows_per_thread;
-  if (r != 0 && r != 1 && r != 2 && r != 4 && r != 8)
-    return set_error(BSQL_ERR_INVALID, "rows_per_thread must be 0, 1, 2, 4 or 8");
+  if (r != 0 && r != 1 && r != 2 && r != 4)
+    return set_error(BSQL_ERR_INVALID, "rows_per_thread must be 0, 1, 2 or 4");
   if (cfg->block_size != 0 && cfg->block_size != 256)
     return set_error(BSQL_ERR_INVALID, "block_size must be 0 or 256");
+  if (cfg->flags & ~(BSQL_LAUNCH_NO_RANGES | BSQL_LAUNCH_LAZY_LOADS))
+    return set_error(BSQL_ERR_INVALID, "unknown launch flags");
+  std::lock_guard<std::mutex> lk(p->cache_mu);
   p->launch = *cfg;
   return BSQL_OK;
 }
@@ -358,7 +370,11 @@ int bsql_interp_execute(const bsql_program* cprog, const bsql_column* in, int n_
       plan.out[k].type = out[k].type_id;
     }
     if (num_rows == 0) return BSQL_OK;
-    e = launch_execute(plan, L.nreg, rows_per_thread(prog), (hipStream_t)stream);
+    uint64_t* scratch = nullptr;
+    e = get_scratch((hipStream_t)stream, 0, &scratch);
+    if (e == hipSuccess) e = upload_plan(plan, scratch, (hipStream_t)stream);
+    if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
+    e = launch_execute(plan, (const DevPlan*)scratch, L.nreg, rows_per_thread(prog), (hipStream_t)stream);
     if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
     return BSQL_OK;
   })
@@ -384,6 +400,8 @@ static int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column
   opt.pred = pred;
   opt.proj = proj;
   opt.conj_order = &pred->conj_order;
+  opt.ranges = !(pred->launch.flags & BSQL_LAUNCH_NO_RANGES);
+  opt.lazy_loads = (pred->launch.flags & BSQL_LAUNCH_LAZY_LOADS) != 0;
   const Lowered& L = lowered_for(pred, opt, in, n_in, out, n_out);
   DevPlan plan = L.plan;
   plan.num_rows = num_rows;
@@ -398,8 +416,10 @@ static int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column
   if (tiles < 0) fail("no kernel variant for this program");
   uint64_t* scratch = nullptr;
   e = get_scratch((hipStream_t)stream, tiles + 1, &scratch);
+  if (e == hipSuccess) e = upload_plan(plan, scratch, (hipStream_t)stream);
   if (e != hipSuccess) return device_error(e, "filter scratch");
-  e = launch_compact(plan, L.nreg, rpt, scratch, d_count, capacity, (hipStream_t)stream);
+  e = launch_compact(plan, (const DevPlan*)scratch, L.nreg, rpt, scratch + kPlanWords, d_count,
+                     capacity, (hipStream_t)stream);
   if (e != hipSuccess) return device_error(e, "filter");
   return BSQL_OK;
 }

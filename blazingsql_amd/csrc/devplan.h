@@ -10,7 +10,7 @@
 #define BSQL_DEV_MAX_OPS 96
 #define BSQL_DEV_MAX_IN 24
 #define BSQL_DEV_MAX_OUT 16
-#define BSQL_DEV_MAX_REG 32
+#define BSQL_DEV_MAX_REG 16
 
 // Device opcodes: reference operators specialised by operand kind
 // (I = int64 slot, F = double slot), plus the VM's own load/store/select ops.
@@ -18,7 +18,8 @@ enum DevCode : uint8_t {
   D_NOP = 0,
   D_LOAD,        // reg[dst] <- input column a (converted to the slot kind), validity
   D_LOADV,       // validity of input column a only (string columns)
-  D_LOADN,       // aux (1..4) columns at once: imm bytes 2j / 2j+1 = column / register
+  D_LOADN,       // issue loads of aux (1..4) columns (imm byte j = column j) into temporaries
+  D_COMMIT,      // reg[dst] <- loaded column aux of the last LOADN (data and validity)
   D_STORE,       // output column aux <- reg[a] (static_cast to its type)
   D_KILL,        // alive &= valid(a) && truthy(a); wave-uniform early exit
   D_SEL,         // alive &= valid(a) && bits(a) != 0 (BOOL8 write of the slot)
@@ -50,7 +51,7 @@ enum : uint8_t {
 
 typedef struct DevOp {
   uint8_t code;
-  uint8_t dst;   // destination register
+  uint8_t dst;   // destination register (0xff: no result)
   uint8_t a;     // register of operand a (D_LOAD/D_LOADV: input column)
   uint8_t b;     // register of operand b
   uint8_t flags;
@@ -79,13 +80,31 @@ typedef struct DevOutCol {
   int32_t pad;
 } DevOutCol;
 
+// One conjunct of a range predicate: column `col` (slot kind `is_f`) must be
+// valid and lie in [lo, hi] (bounds optional / strict per flags), or, with
+// R_NE, differ from lo. Literals are already in the comparison domain.
+#define BSQL_DEV_MAX_RANGES 16
+enum : uint8_t {
+  R_HAS_LO = 1, R_LO_STRICT = 2, R_HAS_HI = 4, R_HI_STRICT = 8, R_NE = 16, R_F = 32
+};
+typedef struct DevRange {
+  uint8_t col;
+  uint8_t flags;
+  uint8_t pad[6];
+  int64_t lo;  // int64 or double bits (R_F)
+  int64_t hi;
+} DevRange;
+
 typedef struct DevPlan {
   int32_t n_ops;
-  int32_t split;   // ops [0, split) decide the row selection, [split, n_ops) write outputs
+  int32_t split;     // ops [0, split) decide the row selection, [split, n_ops) write outputs
   int32_t n_in;
   int32_t n_out;
   int64_t num_rows;
+  int32_t n_ranges;  // > 0: the selection is these range conjuncts (ops [0, split) unused)
+  int32_t range_eager;  // load every range column before the first test
   DevInCol in[BSQL_DEV_MAX_IN];
   DevOutCol out[BSQL_DEV_MAX_OUT];
+  DevRange ranges[BSQL_DEV_MAX_RANGES];
   DevOp ops[BSQL_DEV_MAX_OPS];
 } DevPlan;

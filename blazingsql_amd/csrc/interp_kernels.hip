@@ -239,7 +239,7 @@ __device__ __forceinline__ void store_elem(const DevOutCol& c, int64_t row, uint
 template <int R>
 __device__ __forceinline__ void store_rows(const DevOutCol& c, int64_t r0, int n, const uint64_t (&v)[R]) {
   if (n == R && (c.type == BSQL_FLOAT64 || c.type == BSQL_INT64) && (R % 2 == 0) &&
-      ((((uintptr_t)c.data) & 15) == 0)) {
+      ((((uintptr_t)((uint64_t*)c.data + r0)) & 15) == 0)) {
     uint64_t* p = (uint64_t*)c.data + r0;
 #pragma unroll
     for (int k = 0; k < R / 2; ++k) {
@@ -257,50 +257,6 @@ __device__ __forceinline__ void store_rows(const DevOutCol& c, int64_t r0, int n
 
 // ------------------------------------------------------------ the VM core
 //
-// The register file: rv[k][reg] holds the slot bits of register reg for
-// this lane's row k (one array per row, each indexed by the register number); rvalid[k] has one validity bit per register. One VM step
-// reads the register file at exactly two sites (operands a and b) and writes
-// it at one (the result), all with the wave-uniform register number, so the
-// compiler keeps the file in VGPRs; the op itself works on plain locals.
-
-// One register array per row handled by the lane: separate allocas, each
-// indexed only by the wave-uniform register number, are what the compiler
-// promotes to VGPRs (relative addressing) instead of scratch.
-template <int NREG, int R>
-struct RegFile {
-  uint64_t (&r0)[NREG];
-  uint64_t (&r1)[NREG];
-  uint64_t (&r2)[NREG];
-  uint64_t (&r3)[NREG];
-  uint64_t (&r4)[NREG];
-  uint64_t (&r5)[NREG];
-  uint64_t (&r6)[NREG];
-  uint64_t (&r7)[NREG];
-  __device__ __forceinline__ uint64_t& at(int k, int reg) const {
-    switch (k) {
-      case 0: return r0[reg];
-      case 1: return r1[reg];
-      case 2: return r2[reg];
-      case 3: return r3[reg];
-      case 4: return r4[reg];
-      case 5: return r5[reg];
-      case 6: return r6[reg];
-      default: return r7[reg];
-    }
-  }
-};
-#define BSQL_REGFILE(NAME)                                                               \
-  uint64_t NAME##0[NREG], NAME##1[NREG], NAME##2[NREG], NAME##3[NREG], NAME##4[NREG],      \
-      NAME##5[NREG], NAME##6[NREG], NAME##7[NREG];                                         \
-  RegFile<NREG, R> NAME{NAME##0, NAME##1, NAME##2, NAME##3, NAME##4, NAME##5, NAME##6, NAME##7}
-
-struct Operands {
-  uint64_t a, b;
-  uint32_t va, vb;
-};
-
-#define FA ((op.flags & F_A_F) != 0)
-#define FB ((op.flags & F_B_F) != 0)
 
 // Transcendental and calendar ops: kept out of line so that their large
 // register footprint does not lower the occupancy of every program.
@@ -328,110 +284,6 @@ __device__ __noinline__ uint64_t compute_heavy(uint8_t code, uint8_t aux, uint8_
   }
 }
 
-__device__ __forceinline__ bool is_heavy(uint8_t code) {
-  return code == D_MOD_FF || code == D_POW || code == D_ROUND || (code >= D_SIN && code <= D_EXTRACT) ||
-         code == D_RAND;
-}
-
-// Computes the result bits/validity of `op` on one row (binary and unary ops
-// that only need the operand values). Returns false if the op has no result.
-__device__ __forceinline__ void compute(const DevOp& op, const Operands& o, uint64_t& d, uint32_t& vd,
-                                        int64_t row) {
-  const uint64_t a = o.a, b = o.b;
-  const bool both = o.va && o.vb;
-  const double af = to_f(a, FA), bf = to_f(b, FB);
-  if (is_heavy(op.code)) {
-    d = compute_heavy(op.code, op.aux, op.aux2, op.imm, af, bf, a, row);
-    vd = op.code == D_RAND ? 1u : (op.code >= D_SIN ? o.va : (uint32_t)both);
-    return;
-  }
-  switch (op.code) {
-    // binary ops valid only if both operands are (interpreter_ops.cuh:599-727)
-    case D_ADD_II: d = a + b; vd = both; return;
-    case D_SUB_II: d = a - b; vd = both; return;
-    case D_MUL_II: d = a * b; vd = both; return;
-    case D_DIV_II: d = (uint64_t)idiv((int64_t)a, (int64_t)b); vd = both && b != 0; return;
-    case D_MOD_II: d = (uint64_t)imod((int64_t)a, (int64_t)b); vd = both; return;
-    case D_ADD_FF: d = bits(__dadd_rn(af, bf)); vd = both; return;
-    case D_SUB_FF: d = bits(__dsub_rn(af, bf)); vd = both; return;
-    case D_MUL_FF: d = bits(__dmul_rn(af, bf)); vd = both; return;
-    case D_DIV_FF:
-      d = bits(__ddiv_rn(af, bf));
-      vd = both && (FB ? !(as_f(b) == 0.0) : (b != 0));
-      return;
-    case D_EQ_II: d = (int64_t)a == (int64_t)b; vd = both; return;
-    case D_NE_II: d = (int64_t)a != (int64_t)b; vd = both; return;
-    case D_LT_II: d = (int64_t)a < (int64_t)b; vd = both; return;
-    case D_GT_II: d = (int64_t)a > (int64_t)b; vd = both; return;
-    case D_LE_II: d = (int64_t)a <= (int64_t)b; vd = both; return;
-    case D_GE_II: d = (int64_t)a >= (int64_t)b; vd = both; return;
-    case D_EQ_FF: d = af == bf; vd = both; return;
-    case D_NE_FF: d = af != bf; vd = both; return;
-    case D_LT_FF: d = af < bf; vd = both; return;
-    case D_GT_FF: d = af > bf; vd = both; return;
-    case D_LE_FF: d = af <= bf; vd = both; return;
-    case D_GE_FF: d = af >= bf; vd = both; return;
-    case D_EQ_TS: case D_NE_TS: case D_LT_TS: case D_GT_TS: case D_LE_TS: case D_GE_TS: {
-      const int64_t x = (int64_t)(a * (uint64_t)ns_per_unit(op.aux2 & 15));
-      const int64_t y = (int64_t)(b * (uint64_t)ns_per_unit(op.aux2 >> 4));
-      d = (uint64_t)cmp_result(op.code - D_EQ_TS, x < y ? -1 : (x > y ? 1 : 0));
-      vd = both;
-      return;
-    }
-    case D_AND: d = truthy(a, FA) && truthy(b, FB); vd = both; return;
-    case D_OR:  // interpreter_ops.cuh:575-587
-      if (both) { d = truthy(a, FA) || truthy(b, FB); vd = 1; }
-      else if (o.va) { d = a; vd = truthy(a, FA); }
-      else if (o.vb) { d = b; vd = 1; }
-      else { d = 0; vd = 0; }
-      return;
-    case D_INDF_II: case D_INDF_FF:  // interpreter_ops.cuh:588-598
-      if (!o.va && !o.vb) d = 1;
-      else if (!o.va || !o.vb) d = 0;
-      else d = op.code == D_INDF_II ? (a == b) : (af == bf);
-      vd = 1;
-      return;
-    case D_MAGIC_IF_NOT:  // interpreter_ops.cuh:559-566
-      if (o.va && truthy(a, FA)) { d = b; vd = o.vb; }
-      else { d = FB ? bits(MAGIC_F64) : (uint64_t)MAGIC_I64; vd = 0; }
-      return;
-    case D_FIRST_NON_MAGIC: {  // interpreter_ops.cuh:567-574
-      const bool magic = FA ? (as_f(a) == MAGIC_F64) : ((int64_t)a == MAGIC_I64);
-      if (magic) { d = b; vd = o.vb; } else { d = a; vd = o.va; }
-      return;
-    }
-    // unary ops: computed only for a valid operand, validity passes through
-    // (interpreter_ops.cuh:728-917)
-    case D_NOT: d = !truthy(a, FA); vd = o.va; return;
-    case D_ABS_I: d = (int64_t)a < 0 ? 0ULL - a : a; vd = o.va; return;
-    case D_ABS_F: d = bits(fabs(as_f(a))); vd = o.va; return;
-    case D_FLOOR: d = bits(floor(af)); vd = o.va; return;
-    case D_CEIL: d = bits(ceil(af)); vd = o.va; return;
-    case D_IS_NULL: d = !o.va; vd = 1; return;
-    case D_IS_NOT_NULL: d = o.va; vd = 1; return;
-    case D_IS_TRUE: d = FA ? (uint64_t)f64_to_i64(as_f(a)) : a; vd = o.va; return;
-    case D_IS_NOT_TRUE: d = !(o.va && (FA ? as_f(a) == 1.0 : a == 1)); vd = 1; return;
-    case D_IS_NOT_FALSE: d = !(o.va && (FA ? as_f(a) == 0.0 : a == 0)); vd = 1; return;
-    case D_TO_I64: d = FA ? (uint64_t)f64_to_i64(as_f(a)) : a; vd = o.va; return;
-    case D_TO_F64: d = bits(af); vd = o.va; return;
-    case D_CAST_DATE: {
-      int64_t v = FA ? f64_to_i64(as_f(a)) : (int64_t)a;
-      if (is_ts(op.aux2) && op.aux2 != BSQL_TIMESTAMP_DAYS) v = v / ticks_per_day(op.aux2);
-      d = (uint64_t)v;
-      vd = o.va;
-      return;
-    }
-    case D_CAST_TS: {
-      uint64_t v = FA ? (uint64_t)f64_to_i64(as_f(a)) : a;
-      if (is_ts(op.aux2)) v = v * (uint64_t)(86400000000000LL / ticks_per_day(op.aux2));
-      d = v;
-      vd = o.va;
-      return;
-    }
-    default: d = a; vd = 0; return;
-  }
-}
-
 __device__ __forceinline__ void str_at(const DevPlan& plan, uint8_t col, int64_t imm, int64_t row,
                                        const uint8_t** p, int32_t* n) {
   if (col != 0xff) {
@@ -446,99 +298,127 @@ __device__ __forceinline__ void str_at(const DevPlan& plan, uint8_t col, int64_t
   }
 }
 
-// One VM step for this lane's R rows (rows row0 .. row0+nrows-1 exist).
-template <int NREG, int R>
-__device__ __forceinline__ void vm_step(const DevPlan& plan, const DevOp& op,
-                                        const RegFile<NREG, R>& rv, uint32_t (&rvalid)[R],
-                                        uint32_t& alive, int64_t row0, int nrows) {
-  const uint8_t code = op.code;
-  uint64_t D[R];
-  uint32_t VD = 0;  // bit k = result validity of row k
-  if (code == D_LOADN) {
-    // issue every column's loads before the first register write, so that
-    // all of them are in flight together
-    uint64_t T[4][R];
-    uint32_t TV[4] = {0, 0, 0, 0};
-    const int n = op.aux;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (j < n) {
-        const DevInCol& c = plan.in[(op.imm >> (16 * j)) & 0xff];
-        load_rows<R>(c, row0, nrows, T[j]);
-        TV[j] = load_valid<R>(c, row0, nrows);
-      }
+#define FA ((op.flags & F_A_F) != 0)
+#define FB ((op.flags & F_B_F) != 0)
+#define AF to_f(a, FA)
+#define BF to_f(b, FB)
+// one result per row: D[k] and validity bit k of VD
+#define ROWS(EXPR_D, EXPR_V)                                                   \
+  _Pragma("unroll") for (int k = 0; k < R; ++k) {                              \
+    const uint64_t a = A[k], b = B[k];                                         \
+    const uint32_t va = (VA >> k) & 1u, vb = (VB >> k) & 1u;                   \
+    (void)a; (void)b; (void)va; (void)vb;                                      \
+    D[k] = (uint64_t)(EXPR_D);                                                 \
+    VD |= ((uint32_t)(EXPR_V) & 1u) << k;                                      \
+  }                                                                            \
+  return;
+
+// Result of one op for this lane's R rows. Reference semantics per op in
+// engine/src/Interpreter/interpreter_ops.cuh:500-925.
+template <int R> struct Rows;
+
+template <int R>
+__device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& op, const uint64_t (&A)[R],
+                                             const uint64_t (&B)[R], uint32_t VA, uint32_t VB,
+                                             uint64_t (&D)[R], uint32_t& VD, const Rows<R>& rs) {
+  VD = 0;
+  switch (op.code) {
+    // binary ops: valid only if both operands are (:599-727); x/0 is null (:721-724)
+    case D_ADD_II: ROWS(a + b, va & vb)
+    case D_SUB_II: ROWS(a - b, va & vb)
+    case D_MUL_II: ROWS(a * b, va & vb)
+    case D_DIV_II: ROWS(idiv((int64_t)a, (int64_t)b), va & vb & (b != 0))
+    case D_MOD_II: ROWS(imod((int64_t)a, (int64_t)b), va & vb)
+    case D_ADD_FF: ROWS(bits(__dadd_rn(AF, BF)), va & vb)
+    case D_SUB_FF: ROWS(bits(__dsub_rn(AF, BF)), va & vb)
+    case D_MUL_FF: ROWS(bits(__dmul_rn(AF, BF)), va & vb)
+    case D_DIV_FF: ROWS(bits(__ddiv_rn(AF, BF)), va & vb & (FB ? !(as_f(b) == 0.0) : (b != 0)))
+    case D_EQ_II: ROWS((int64_t)a == (int64_t)b, va & vb)
+    case D_NE_II: ROWS((int64_t)a != (int64_t)b, va & vb)
+    case D_LT_II: ROWS((int64_t)a < (int64_t)b, va & vb)
+    case D_GT_II: ROWS((int64_t)a > (int64_t)b, va & vb)
+    case D_LE_II: ROWS((int64_t)a <= (int64_t)b, va & vb)
+    case D_GE_II: ROWS((int64_t)a >= (int64_t)b, va & vb)
+    case D_EQ_FF: ROWS(AF == BF, va & vb)
+    case D_NE_FF: ROWS(AF != BF, va & vb)
+    case D_LT_FF: ROWS(AF < BF, va & vb)
+    case D_GT_FF: ROWS(AF > BF, va & vb)
+    case D_LE_FF: ROWS(AF <= BF, va & vb)
+    case D_GE_FF: ROWS(AF >= BF, va & vb)
+    case D_EQ_TS: case D_NE_TS: case D_LT_TS: case D_GT_TS: case D_LE_TS: case D_GE_TS: {
+      const uint64_t fa = (uint64_t)ns_per_unit(op.aux2 & 15), fb = (uint64_t)ns_per_unit(op.aux2 >> 4);
+      const int kind = op.code - D_EQ_TS;
+      ROWS(cmp_result(kind, (int64_t)(a * fa) < (int64_t)(b * fb) ? -1
+                            : ((int64_t)(a * fa) > (int64_t)(b * fb) ? 1 : 0)), va & vb)
     }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (j < n) {
-        const int reg = (int)((op.imm >> (16 * j + 8)) & 0xff);
-        const uint32_t keep = ~(1u << reg);
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-          rv.at(k, reg) = T[j][k];
-          rvalid[k] = (rvalid[k] & keep) | (((TV[j] >> k) & 1u) << reg);
-        }
-      }
+    case D_AND: ROWS(truthy(a, FA) && truthy(b, FB), va & vb)
+    case D_OR:  // :575-587
+      ROWS((va & vb) ? (uint64_t)(truthy(a, FA) || truthy(b, FB)) : (va ? a : b),
+           (va & vb) | (va & (uint32_t)truthy(a, FA)) | ((va ^ 1u) & vb))
+    case D_INDF_II:  // :588-598
+      ROWS((!va && !vb) ? 1 : ((!va || !vb) ? 0 : (a == b)), 1u)
+    case D_INDF_FF:
+      ROWS((!va && !vb) ? 1 : ((!va || !vb) ? 0 : (AF == BF)), 1u)
+    case D_MAGIC_IF_NOT:  // :559-566
+      ROWS((va && truthy(a, FA)) ? b : (FB ? bits(MAGIC_F64) : (uint64_t)MAGIC_I64),
+           (va && truthy(a, FA)) ? vb : 0u)
+    case D_FIRST_NON_MAGIC: {  // :567-574
+      ROWS((FA ? (as_f(a) == MAGIC_F64) : ((int64_t)a == MAGIC_I64)) ? b : a,
+           (FA ? (as_f(a) == MAGIC_F64) : ((int64_t)a == MAGIC_I64)) ? vb : va)
     }
-    return;
-  }
-  if (code == D_LOAD) {
-    const DevInCol& c = plan.in[op.a];
-    load_rows<R>(c, row0, nrows, D);
-    VD = load_valid<R>(c, row0, nrows);
-  } else if (code == D_LOADV) {
-    VD = load_valid<R>(plan.in[op.a], row0, nrows);
-#pragma unroll
-    for (int k = 0; k < R; ++k) D[k] = 0;
-  } else {
-    // operand fetch: the only reads of the register file
-    uint64_t A[R], B[R];
-    uint32_t VA = 0, VB = 0;
-    const bool a_imm = op.flags & F_A_IMM, b_imm = op.flags & F_B_IMM;
-    const bool has_b = code < D_NOT && code != D_KILL && code != D_SEL;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      A[k] = a_imm ? (uint64_t)op.imm : rv.at(k, op.a);
-      VA |= (a_imm ? 1u : (op.flags & F_A_NULL) ? 0u : ((rvalid[k] >> op.a) & 1u)) << k;
+    // unary ops: computed for valid inputs, validity passes through (:728-917)
+    case D_NOT: ROWS(!truthy(a, FA), va)
+    case D_ABS_I: ROWS((int64_t)a < 0 ? 0ULL - a : a, va)
+    case D_ABS_F: ROWS(bits(fabs(as_f(a))), va)
+    case D_FLOOR: ROWS(bits(floor(AF)), va)
+    case D_CEIL: ROWS(bits(ceil(AF)), va)
+    case D_IS_NULL: ROWS(va ^ 1u, 1u)
+    case D_IS_NOT_NULL: ROWS(va, 1u)
+    case D_IS_TRUE: ROWS(FA ? (uint64_t)f64_to_i64(as_f(a)) : a, va)
+    case D_IS_NOT_TRUE: ROWS(!(va && (FA ? as_f(a) == 1.0 : a == 1)), 1u)
+    case D_IS_NOT_FALSE: ROWS(!(va && (FA ? as_f(a) == 0.0 : a == 0)), 1u)
+    case D_TO_I64: ROWS(FA ? (uint64_t)f64_to_i64(as_f(a)) : a, va)
+    case D_TO_F64: ROWS(bits(AF), va)
+    case D_CAST_DATE: {
+      const bool sub_day = is_ts(op.aux2) && op.aux2 != BSQL_TIMESTAMP_DAYS;
+      const int64_t tpd = ticks_per_day(op.aux2);
+      ROWS(sub_day ? (uint64_t)((FA ? f64_to_i64(as_f(a)) : (int64_t)a) / tpd)
+                   : (uint64_t)(FA ? f64_to_i64(as_f(a)) : (int64_t)a), va)
     }
-    if (has_b) {
-#pragma unroll
-      for (int k = 0; k < R; ++k) {
-        B[k] = b_imm ? (uint64_t)op.imm : rv.at(k, op.b);
-        VB |= (b_imm ? 1u : (op.flags & F_B_NULL) ? 0u : ((rvalid[k] >> op.b) & 1u)) << k;
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < R; ++k) B[k] = 0;
+    case D_CAST_TS: {
+      const uint64_t f = is_ts(op.aux2) ? (uint64_t)(86400000000000LL / ticks_per_day(op.aux2)) : 1ULL;
+      ROWS((FA ? (uint64_t)f64_to_i64(as_f(a)) : a) * f, va)
     }
-    if (code == D_KILL || code == D_SEL) {
-#pragma unroll
-      for (int k = 0; k < R; ++k) {
-        const bool t = code == D_KILL ? truthy(A[k], FA) : (A[k] != 0);
-        if (!(((VA >> k) & 1u) && t)) alive &= ~(1u << k);
-      }
-      return;
-    }
-    if (code >= D_EQ_SS && code <= D_GE_SS) {
+    case D_MOD_FF: case D_POW: case D_ROUND:
+      ROWS(compute_heavy(op.code, op.aux, op.aux2, op.imm, AF, BF, a, rs.row[k]), va & vb)
+    case D_SIN: case D_COS: case D_ASIN: case D_ACOS: case D_TAN: case D_COTAN: case D_ATAN:
+    case D_LN: case D_LOG: case D_EXTRACT:
+      ROWS(compute_heavy(op.code, op.aux, op.aux2, op.imm, AF, BF, a, rs.row[k]), va)
+    case D_RAND:
+      ROWS(compute_heavy(op.code, op.aux, op.aux2, op.imm, 0.0, 0.0, 0, rs.pos0 + k), 1u)
+    case D_EQ_SS: case D_NE_SS: case D_LT_SS: case D_GT_SS: case D_LE_SS: case D_GE_SS: {
+      const int kind = op.code - D_EQ_SS;
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         D[k] = 0;
-        if (k < nrows && ((VA >> k) & 1u) && ((VB >> k) & 1u)) {
+        if (((rs.exist >> k) & 1u) && ((VA >> k) & 1u) && ((VB >> k) & 1u)) {
           const uint8_t *pa, *pb;
           int32_t na, nb;
-          str_at(plan, op.aux, op.imm, row0 + k, &pa, &na);
-          str_at(plan, op.aux2, op.imm, row0 + k, &pb, &nb);
-          D[k] = (uint64_t)cmp_result(code - D_EQ_SS, str_cmp(pa, na, pb, nb));
+          str_at(plan, op.aux, op.imm, rs.row[k], &pa, &na);
+          str_at(plan, op.aux2, op.imm, rs.row[k], &pb, &nb);
+          D[k] = (uint64_t)cmp_result(kind, str_cmp(pa, na, pb, nb));
           VD |= 1u << k;
         }
       }
-    } else if (code == D_CHAR_LENGTH) {
+      return;
+    }
+    case D_CHAR_LENGTH: {
       const DevInCol& c = plan.in[op.aux];
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         D[k] = 0;
-        if (k < nrows && ((VA >> k) & 1u)) {
-          const int32_t s = c.offsets[row0 + k], e = c.offsets[row0 + k + 1];
+        if (((rs.exist >> k) & 1u) && ((VA >> k) & 1u)) {
+          const int32_t s = c.offsets[rs.row[k]], e = c.offsets[rs.row[k] + 1];
           const uint8_t* p = (const uint8_t*)c.data;
           int64_t n = 0;
           for (int32_t i = s; i < e; ++i) n += (p[i] & 0xC0) != 0x80;
@@ -546,69 +426,17 @@ __device__ __forceinline__ void vm_step(const DevPlan& plan, const DevOp& op,
           VD |= 1u << k;
         }
       }
-    } else {
-#pragma unroll
-      for (int k = 0; k < R; ++k) {
-        Operands o{A[k], B[k], (VA >> k) & 1u, (VB >> k) & 1u};
-        uint32_t vd;
-        compute(op, o, D[k], vd, row0 + k);
-        VD |= vd << k;
-      }
+      return;
     }
-  }
-  // result write: the only write of the register file
-  const uint32_t keep = ~(1u << op.dst);
+    default:
 #pragma unroll
-  for (int k = 0; k < R; ++k) {
-    rv.at(k, op.dst) = D[k];
-    rvalid[k] = (rvalid[k] & keep) | (((VD >> k) & 1u) << op.dst);
+      for (int k = 0; k < R; ++k) D[k] = A[k];
+      return;
   }
 }
+#undef ROWS
 
-// ------------------------------------------------------------- execute path
-
-// perform_interpreter_operation: outputs at the input row positions.
-template <int NREG, int R>
-__global__ __launch_bounds__(BLOCK) void vm_execute(DevPlan plan) {
-  BSQL_REGFILE(rv);
-  uint32_t rvalid[R];
-#pragma unroll
-  for (int k = 0; k < R; ++k) rvalid[k] = 0;
-  const int64_t tile_rows = (int64_t)BLOCK * R;
-  const int64_t row0 = (int64_t)blockIdx.x * tile_rows + (int64_t)threadIdx.x * R;
-  const int64_t left = plan.num_rows - row0;
-  const int nrows = left <= 0 ? 0 : (left >= R ? R : (int)left);
-  uint32_t alive = (1u << nrows) - 1u;
-  const int lane = threadIdx.x & 63;
-
-  for (int q = 0; q < plan.n_ops; ++q) {
-    const DevOp op = plan.ops[q];
-    if (op.code == D_STORE) {
-      const DevOutCol& c = plan.out[op.aux];
-      uint64_t vals[R];
-      uint32_t vm = 0;
-#pragma unroll
-      for (int k = 0; k < R; ++k) {
-        vals[k] = rv.at(k, op.a);
-        vm |= ((rvalid[k] >> op.a) & 1u) << k;
-      }
-      if (nrows > 0) store_rows<R>(c, row0, nrows, vals);
-      if (c.valid) {
-        // lanes covering one 32-row word: 32/R of them; OR their bits together
-        constexpr int LPW = 32 / R;
-        uint32_t word = (vm & alive) << ((lane % LPW) * R);
-#pragma unroll
-        for (int s = 1; s < LPW; s <<= 1) word |= __shfl_xor(word, s, 64);
-        if ((lane % LPW) == 0 && nrows > 0) c.valid[row0 >> 5] = word;
-      }
-      continue;
-    }
-    if (nrows == 0) continue;
-    vm_step<NREG, R>(plan, op, rv, rvalid, alive, row0, nrows);
-  }
-}
-
-// --------------------------------------------------------- compaction path
+// --------------------------------------------------------- compaction state
 
 // Tile status words: flag in bits 62-63, value (row count) in bits 0-61.
 constexpr uint64_t ST_AGG = 1ULL << 62;
@@ -636,7 +464,8 @@ __device__ __forceinline__ int64_t wave_sum(int64_t v) {
   return v;
 }
 
-// Exclusive prefix of this tile across all earlier tiles (wave 0 only).
+// Exclusive prefix of this tile across all earlier tiles (wave 0 only):
+// decoupled look-back over a 64-tile window per iteration.
 __device__ int64_t look_back(const CompactArgs& ca, int64_t tile, int64_t agg, int lane) {
   if (tile == 0) {
     if (lane == 0) st_status(&ca.status[0], ST_PREFIX | (uint64_t)agg);
@@ -665,14 +494,298 @@ __device__ int64_t look_back(const CompactArgs& ca, int64_t tile, int64_t agg, i
   return excl;
 }
 
-// filter / filter+project: selection (phase 1), compaction, outputs (phase 2)
+// -------------------------------------------------------------- rows
+
+// The rows one lane works on: row[k] for k < R where bit k of `exist` is set.
+// `contig` means row[k] == row0 + k (vector loads and stores allowed).
+// pos0 + k is the row's position in the table the ops see (RAND's sequence
+// number): the input row, or the output row after a filter.
+template <int R>
+struct Rows {
+  int64_t row[R];
+  uint32_t exist;
+  int64_t row0;
+  int64_t pos0;
+  int nrows;
+  bool contig;
+};
+
+// gathers R elements of column c at arbitrary rows (converted to slot bits)
+template <int R>
+__device__ __forceinline__ void gather_rows(const DevInCol& c, const Rows<R>& rs, uint64_t (&v)[R]) {
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    v[k] = 0;
+    if (!((rs.exist >> k) & 1u)) continue;
+    const int64_t r = rs.row[k];
+    switch (c.type) {
+      case BSQL_INT8: v[k] = (uint64_t)(int64_t)((const int8_t*)c.data)[r]; break;
+      case BSQL_INT16: v[k] = (uint64_t)(int64_t)((const int16_t*)c.data)[r]; break;
+      case BSQL_INT32: case BSQL_TIMESTAMP_DAYS: case BSQL_DURATION_DAYS:
+        v[k] = (uint64_t)(int64_t)((const int32_t*)c.data)[r]; break;
+      case BSQL_UINT8: v[k] = ((const uint8_t*)c.data)[r]; break;
+      case BSQL_BOOL8: v[k] = ((const uint8_t*)c.data)[r] != 0; break;
+      case BSQL_UINT16: v[k] = ((const uint16_t*)c.data)[r]; break;
+      case BSQL_UINT32: v[k] = ((const uint32_t*)c.data)[r]; break;
+      case BSQL_FLOAT32: v[k] = bits((double)((const float*)c.data)[r]); break;
+      default: v[k] = ((const uint64_t*)c.data)[r]; break;  // 8-byte types
+    }
+  }
+}
+
+template <int R>
+__device__ __forceinline__ uint32_t gather_valid(const DevInCol& c, const Rows<R>& rs) {
+  if (!c.valid) return rs.exist;
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    if (!((rs.exist >> k) & 1u)) continue;
+    const int64_t b = (int64_t)c.vbit + rs.row[k];
+    m |= ((c.valid[b >> 5] >> (b & 31)) & 1u) << k;
+  }
+  return m;
+}
+
+template <int R>
+__device__ __forceinline__ void col_load(const DevInCol& c, const Rows<R>& rs, uint64_t (&v)[R], uint32_t& vm) {
+  if (rs.contig) {
+    load_rows<R>(c, rs.row0, rs.nrows, v);
+    vm = load_valid<R>(c, rs.row0, rs.nrows);
+  } else {
+    gather_rows<R>(c, rs, v);
+    vm = gather_valid<R>(c, rs);
+  }
+}
+
+// -------------------------------------------------------------- VM step
+//
+// Register file: one vector of NREG slot values per row the lane owns,
+// f[k][reg], and validity bit-planes vb (bit reg*R + k). Every op reads its
+// two operands with the wave-uniform register numbers and writes exactly one
+// register (result-less ops write the spare register NREG-1), so the file
+// stays in VGPRs and is read and written with relative (s_set_gpr_idx) moves.
+// Column loads are split into LOADN, which issues up to four columns' loads
+// into fixed temporaries so that they are all in flight together, and one
+// COMMIT per column.
+
+template <int N>
+struct RegVecT {
+  typedef uint64_t type __attribute__((ext_vector_type(N)));
+};
+
+template <int R>
+struct LoadTemps {
+  uint64_t t0[R], t1[R], t2[R], t3[R];
+  uint32_t v0, v1, v2, v3;
+};
+
+// Executes one non-store op. Returns true if it was a KILL that left the
+// wave without a selected row.
 template <int NREG, int R>
-__global__ __launch_bounds__(BLOCK) void vm_compact(DevPlan plan, CompactArgs ca) {
+__device__ __forceinline__ bool vm_step(const DevPlan& plan, const DevOp& op,
+                                        typename RegVecT<NREG>::type (&f)[R], uint64_t& vb,
+                                        LoadTemps<R>& T, const Rows<R>& rs, uint32_t& alive) {
+  constexpr uint32_t RMASK = (1u << R) - 1u;
+  const uint8_t code = op.code;
+  uint64_t A[R], B[R], D[R];
+  const int ia = op.a & (NREG - 1), ib = op.b & (NREG - 1);
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    A[k] = f[k][ia];
+    B[k] = f[k][ib];
+  }
+  uint32_t VA = (uint32_t)(vb >> (ia * R)) & RMASK, VB = (uint32_t)(vb >> (ib * R)) & RMASK, VD = 0;
+  if (op.flags & F_A_IMM) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) A[k] = (uint64_t)op.imm;
+    VA = RMASK;
+  }
+  if (op.flags & F_B_IMM) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) B[k] = (uint64_t)op.imm;
+    VB = RMASK;
+  }
+  if (op.flags & F_A_NULL) VA = 0;
+  if (op.flags & F_B_NULL) VB = 0;
+  int dst = op.dst == 0xff ? NREG - 1 : (op.dst & (NREG - 1));
+  bool dead = false;
+  if (code == D_LOADN) {
+    const int n = op.aux;
+    col_load<R>(plan.in[op.imm & 0xff], rs, T.t0, T.v0);
+    if (n > 1) col_load<R>(plan.in[(op.imm >> 8) & 0xff], rs, T.t1, T.v1);
+    if (n > 2) col_load<R>(plan.in[(op.imm >> 16) & 0xff], rs, T.t2, T.v2);
+    if (n > 3) col_load<R>(plan.in[(op.imm >> 24) & 0xff], rs, T.t3, T.v3);
+#pragma unroll
+    for (int k = 0; k < R; ++k) D[k] = A[k];
+    dst = NREG - 1;
+  } else if (code == D_COMMIT) {
+    const int j = op.aux;
+#pragma unroll
+    for (int k = 0; k < R; ++k) D[k] = j == 0 ? T.t0[k] : (j == 1 ? T.t1[k] : (j == 2 ? T.t2[k] : T.t3[k]));
+    VD = j == 0 ? T.v0 : (j == 1 ? T.v1 : (j == 2 ? T.v2 : T.v3));
+  } else if (code == D_LOADV) {
+    const DevInCol& c = plan.in[op.a];
+#pragma unroll
+    for (int k = 0; k < R; ++k) D[k] = 0;
+    VD = rs.contig ? load_valid<R>(c, rs.row0, rs.nrows) : gather_valid<R>(c, rs);
+  } else if (code == D_KILL || code == D_SEL) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const bool t = code == D_KILL ? truthy(A[k], FA) : (A[k] != 0);
+      if (!(((VA >> k) & 1u) && t)) alive &= ~(1u << k);
+      D[k] = A[k];
+    }
+    dst = NREG - 1;
+    dead = code == D_KILL && __ballot(alive != 0) == 0;
+  } else {
+    compute_rows<R>(plan, op, A, B, VA, VB, D, VD, rs);
+  }
+  // result write: the only write of the register file
+#pragma unroll
+  for (int k = 0; k < R; ++k) f[k][dst] = D[k];
+  vb = (vb & ~((uint64_t)RMASK << (dst * R))) | ((uint64_t)(VD & RMASK) << (dst * R));
+  return dead;
+}
+
+// Writes the validity bits `bits` (bit k = lane row k) of the output rows at
+// positions [p0, p0 + R) of this lane, where the wave covers [wpos, wpos+wcnt):
+// words wholly inside the wave's range are stored, boundary words are merged
+// with disjoint-mask atomics (neighbouring waves/blocks own the other bits).
+template <int R>
+__device__ __forceinline__ void write_valid_words(uint32_t* valid, uint32_t (&s_words)[2 * R + 2], int lane,
+                                                  int64_t wpos, int64_t wcnt, int64_t p0, uint32_t bits,
+                                                  int64_t capacity) {
+  constexpr int WORDS = 2 * R + 2;
+  const int64_t w0 = wpos >> 5;
+  const int nwords = (int)(((wpos + wcnt + 31) >> 5) - w0);
+  if (lane < WORDS) s_words[lane] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    if ((bits >> k) & 1u) {
+      const int64_t rel = p0 + k - (w0 << 5);
+      atomicOr(&s_words[rel >> 5], 1u << (rel & 31));
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane < nwords) {
+    const int64_t wi = w0 + lane;
+    const int64_t lo = wpos > (wi << 5) ? wpos : (wi << 5);
+    int64_t hi = wpos + wcnt < ((wi + 1) << 5) ? wpos + wcnt : ((wi + 1) << 5);
+    if (hi > capacity) hi = capacity;
+    if (hi > lo) {
+      const int a = (int)(lo - (wi << 5)), b = (int)(hi - (wi << 5));
+      const uint32_t mask = (b >= 32 ? 0xffffffffu : ((1u << b) - 1u)) & ~((1u << a) - 1u);
+      const uint32_t word = s_words[lane] & mask;
+      if (mask == 0xffffffffu) {
+        valid[wi] = word;
+      } else {
+        atomicAnd(&valid[wi], ~mask | word);
+        atomicOr(&valid[wi], word);
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Range conjuncts (plan.ranges) over contiguous rows: alive &= in range.
+template <int R>
+__device__ __forceinline__ void range_test(const DevPlan& plan, const DevRange& rg, const uint64_t (&v)[R],
+                                           uint32_t vm, uint32_t& alive) {
+  const int ct = plan.in[rg.col].type;
+  const bool col_f = ct == BSQL_FLOAT32 || ct == BSQL_FLOAT64;
+  const uint8_t fl = rg.flags;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    bool ok;
+    if (fl & R_F) {
+      const double x = col_f ? as_f(v[k]) : (double)(int64_t)v[k];
+      const double lo = as_f((uint64_t)rg.lo), hi = as_f((uint64_t)rg.hi);
+      if (fl & R_NE) {
+        ok = x != lo;
+      } else {
+        ok = true;
+        if (fl & R_HAS_LO) ok = ok && ((fl & R_LO_STRICT) ? x > lo : x >= lo);
+        if (fl & R_HAS_HI) ok = ok && ((fl & R_HI_STRICT) ? x < hi : x <= hi);
+      }
+    } else {
+      const int64_t x = (int64_t)v[k];
+      if (fl & R_NE) {
+        ok = x != rg.lo;
+      } else {
+        ok = true;
+        if (fl & R_HAS_LO) ok = ok && ((fl & R_LO_STRICT) ? x > rg.lo : x >= rg.lo);
+        if (fl & R_HAS_HI) ok = ok && ((fl & R_HI_STRICT) ? x < rg.hi : x <= rg.hi);
+      }
+    }
+    if (!(((vm >> k) & 1u) && ok)) alive &= ~(1u << k);
+  }
+}
+
+// -------------------------------------------------------------- the kernels
+
+// perform_interpreter_operation: every op over this lane's R contiguous rows,
+// outputs at the input row positions.
+template <int NREG, int R>
+__global__ __launch_bounds__(BLOCK) void vm_execute(const DevPlan* __restrict__ pplan) {
+  typedef typename RegVecT<NREG>::type RegVec;
+  const DevPlan& plan = *pplan;  // device memory: ops are scalar-cache loads
+  const int lane = threadIdx.x & 63;
+  Rows<R> rs;
+  rs.row0 = (int64_t)blockIdx.x * (BLOCK * R) + (int64_t)threadIdx.x * R;
+  const int64_t left = plan.num_rows - rs.row0;
+  rs.nrows = left <= 0 ? 0 : (left >= R ? R : (int)left);
+  rs.exist = (1u << rs.nrows) - 1u;
+  rs.contig = true;
+  rs.pos0 = rs.row0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) rs.row[k] = rs.row0 + k;
+  uint32_t alive = rs.exist;
+  RegVec f[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
+  uint64_t vb = 0;
+  LoadTemps<R> T;
+  for (int q = 0; q < plan.n_ops; ++q) {
+    const DevOp op = plan.ops[q];
+    if (op.code != D_STORE) {
+      vm_step<NREG, R>(plan, op, f, vb, T, rs, alive);
+      continue;
+    }
+    const DevOutCol& c = plan.out[op.aux];
+    const int ia = op.a & (NREG - 1);
+    uint64_t A[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) A[k] = f[k][ia];
+    const uint32_t VA = (uint32_t)(vb >> (ia * R)) & ((1u << R) - 1u);
+    if (rs.nrows > 0) store_rows<R>(c, rs.row0, rs.nrows, A);
+    if (c.valid) {
+      // 32/R lanes cover one 32-row word: OR their bits together
+      constexpr int LPW = 32 / R;
+      uint32_t word = (VA & alive) << ((lane % LPW) * R);
+#pragma unroll
+      for (int s = 1; s < LPW; s <<= 1) word |= __shfl_xor(word, s, 64);
+      if ((lane % LPW) == 0 && rs.nrows > 0) c.valid[rs.row0 >> 5] = word;
+    }
+  }
+}
+
+// filter / filter+project: rows selected by the range conjuncts or by ops
+// [0, split) (wave-uniform early exit after each conjunct), compacted
+// (ballot/mbcnt -> workgroup prefix -> decoupled look-back), then ops
+// [split, n_ops) run over the selected rows only, packed densely.
+template <int NREG, int R>
+__global__ __launch_bounds__(BLOCK) void vm_compact(const DevPlan* __restrict__ pplan, CompactArgs ca) {
+  typedef typename RegVecT<NREG>::type RegVec;
+  constexpr int TILE = BLOCK * R;
+  const DevPlan& plan = *pplan;
   __shared__ uint32_t s_ticket;
   __shared__ int64_t s_wave_total[WAVES];
   __shared__ int64_t s_tile_prefix;
-  constexpr int WORDS = 2 * R + 2;  // validity words one wave's 64*R outputs can touch
-  __shared__ uint32_t s_words[WAVES][WORDS];
+  __shared__ uint16_t s_rows[TILE];
+  __shared__ uint32_t s_words[WAVES][2 * R + 2];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -680,31 +793,61 @@ __global__ __launch_bounds__(BLOCK) void vm_compact(DevPlan plan, CompactArgs ca
   if (tid == 0) s_ticket = atomicAdd(ca.ticket, 1u);
   __syncthreads();
   const int64_t tile = s_ticket;
+  const int64_t tile_row0 = tile * TILE;
 
-  BSQL_REGFILE(rv);
-  uint32_t rvalid[R];
+  Rows<R> rs;
+  rs.row0 = tile_row0 + (int64_t)tid * R;
+  const int64_t left = plan.num_rows - rs.row0;
+  rs.nrows = left <= 0 ? 0 : (left >= R ? R : (int)left);
+  rs.exist = (1u << rs.nrows) - 1u;
+  rs.contig = true;
+  rs.pos0 = rs.row0;
 #pragma unroll
-  for (int k = 0; k < R; ++k) rvalid[k] = 0;
-  const int64_t tile_rows = (int64_t)BLOCK * R;
-  const int64_t row0 = tile * tile_rows + (int64_t)tid * R;
-  const int64_t left = plan.num_rows - row0;
-  const int nrows = left <= 0 ? 0 : (left >= R ? R : (int)left);
-  uint32_t alive = (1u << nrows) - 1u;
+  for (int k = 0; k < R; ++k) rs.row[k] = rs.row0 + k;
+  uint32_t alive = rs.exist;
 
-  // ---- phase 1: selection with wave-uniform early exit
-  for (int q = 0; q < plan.split; ++q) {
-    const DevOp op = plan.ops[q];
-    vm_step<NREG, R>(plan, op, rv, rvalid, alive, row0, nrows);
-    if (op.code == D_KILL && __ballot(alive != 0) == 0) break;
+  RegVec f[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
+  uint64_t vb = 0;
+  LoadTemps<R> T;
+
+  // ---- phase 1: row selection
+  if (plan.n_ranges > 0) {
+    const int nr = plan.n_ranges;
+    int r = 0;
+    if (plan.range_eager) {
+      // every range column in flight before the first test
+      col_load<R>(plan.in[plan.ranges[0].col], rs, T.t0, T.v0);
+      if (nr > 1) col_load<R>(plan.in[plan.ranges[1].col], rs, T.t1, T.v1);
+      if (nr > 2) col_load<R>(plan.in[plan.ranges[2].col], rs, T.t2, T.v2);
+      if (nr > 3) col_load<R>(plan.in[plan.ranges[3].col], rs, T.t3, T.v3);
+      range_test<R>(plan, plan.ranges[0], T.t0, T.v0, alive);
+      if (nr > 1) range_test<R>(plan, plan.ranges[1], T.t1, T.v1, alive);
+      if (nr > 2) range_test<R>(plan, plan.ranges[2], T.t2, T.v2, alive);
+      if (nr > 3) range_test<R>(plan, plan.ranges[3], T.t3, T.v3, alive);
+      r = nr < 4 ? nr : 4;
+    }
+    for (; r < nr; ++r) {
+      if (__ballot(alive != 0) == 0) break;
+      const DevRange rg = plan.ranges[r];
+      uint64_t v[R];
+      uint32_t vm;
+      col_load<R>(plan.in[rg.col], rs, v, vm);
+      range_test<R>(plan, rg, v, vm, alive);
+    }
+  } else {
+    for (int q = 0; q < plan.split; ++q) {
+      const DevOp op = plan.ops[q];
+      if (vm_step<NREG, R>(plan, op, f, vb, T, rs, alive)) break;
+    }
   }
 
-  // ---- wave / workgroup prefix of the selected rows
+  // ---- compaction
   int lane_excl = 0, wave_total = 0;
-  uint64_t sel_ballot[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const uint64_t b = __ballot((alive >> k) & 1u);
-    sel_ballot[k] = b;
     lane_excl += __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
     wave_total += __popcll(b);
   }
@@ -724,67 +867,58 @@ __global__ __launch_bounds__(BLOCK) void vm_compact(DevPlan plan, CompactArgs ca
       if (tile == ca.num_tiles - 1) *ca.d_count = excl + tile_total;
     }
   }
-  __syncthreads();
-  if (wave_total == 0) return;  // nothing selected in this wave: no phase 2
-
-  const int64_t wave_pos = s_tile_prefix + wave_off;
-  int64_t pos[R];
+  // selected rows, densely packed in row order (tile-relative offsets)
   {
-    int64_t p = wave_pos + lane_excl;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      pos[k] = p;
-      p += (alive >> k) & 1u;
-    }
-  }
-  (void)sel_ballot;
-
-  // ---- phase 2: projection ops and compacted stores
-  for (int q = plan.split; q < plan.n_ops; ++q) {
-    const DevOp op = plan.ops[q];
-    if (op.code != D_STORE) {
-      vm_step<NREG, R>(plan, op, rv, rvalid, alive, row0, nrows);
-      continue;
-    }
-    const DevOutCol& c = plan.out[op.aux];
+    int d = (int)wave_off + lane_excl;
 #pragma unroll
     for (int k = 0; k < R; ++k)
-      if (((alive >> k) & 1u) && pos[k] < ca.capacity) store_elem(c, pos[k], rv.at(k, op.a));
-    if (!c.valid) continue;
-    // pack this wave's output validity words: [wave_pos, wave_pos + wave_total)
-    const int64_t w0 = wave_pos >> 5;
-    const int nwords = (int)(((wave_pos + wave_total + 31) >> 5) - w0);  // <= WORDS
-    if (lane < WORDS) s_words[wave][lane] = 0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+      if ((alive >> k) & 1u) s_rows[d++] = (uint16_t)(tid * R + k);
+  }
+  __syncthreads();
+  if (tile_total == 0 || plan.split >= plan.n_ops) return;
+  const int64_t tile_prefix = s_tile_prefix;
+
+  // ---- phase 2: outputs of the selected rows, R dense rows per lane
+  for (int64_t base = 0; base < tile_total; base += TILE) {
+    const int64_t j0 = base + (int64_t)tid * R;  // dense index of this lane's first row
+    Rows<R> ds;
+    ds.exist = 0;
+    ds.contig = false;
+    ds.row0 = 0;
+    ds.nrows = 0;
+    ds.pos0 = tile_prefix + j0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      if (((alive >> k) & 1u) && ((rvalid[k] >> op.a) & 1u)) {
-        const int64_t rel = pos[k] - (w0 << 5);
-        atomicOr(&s_words[wave][rel >> 5], 1u << (rel & 31));
-      }
+      const bool e = j0 + k < tile_total;
+      ds.row[k] = e ? tile_row0 + s_rows[j0 + k] : 0;
+      ds.exist |= (e ? 1u : 0u) << k;
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    if (lane < nwords) {
-      const int64_t wi = w0 + lane;
-      const int64_t lo = wave_pos > (wi << 5) ? wave_pos : (wi << 5);
-      int64_t hi = wave_pos + wave_total < ((wi + 1) << 5) ? wave_pos + wave_total : ((wi + 1) << 5);
-      if (hi > ca.capacity) hi = ca.capacity;
-      if (hi > lo) {
-        const int a = (int)(lo - (wi << 5)), b = (int)(hi - (wi << 5));
-        const uint32_t mask = (b >= 32 ? 0xffffffffu : ((1u << b) - 1u)) & ~((1u << a) - 1u);
-        const uint32_t word = s_words[wave][lane] & mask;
-        if (mask == 0xffffffffu) {
-          c.valid[wi] = word;
-        } else {
-          // boundary word shared with neighbouring waves: disjoint bit ranges
-          atomicAnd(&c.valid[wi], ~mask | word);
-          atomicOr(&c.valid[wi], word);
-        }
+    const int64_t wave_first = base + (int64_t)wave * 64 * R;
+    int64_t wcnt = tile_total - wave_first;
+    wcnt = wcnt < 0 ? 0 : (wcnt > 64 * R ? 64 * R : wcnt);
+    if (wcnt == 0) break;  // this wave and the following ones have no rows left
+    uint32_t alive2 = ds.exist;
+    for (int q = plan.split; q < plan.n_ops; ++q) {
+      const DevOp op = plan.ops[q];
+      if (op.code != D_STORE) {
+        vm_step<NREG, R>(plan, op, f, vb, T, ds, alive2);
+        continue;
       }
+      const DevOutCol& c = plan.out[op.aux];
+      const int ia = op.a & (NREG - 1);
+      uint64_t A[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) A[k] = f[k][ia];
+      const uint32_t VA = (uint32_t)(vb >> (ia * R)) & ((1u << R) - 1u);
+      const int64_t p0 = tile_prefix + j0;
+      int n = (int)(tile_total - j0);
+      n = n < 0 ? 0 : (n > R ? R : n);
+      if (p0 + n > ca.capacity) n = ca.capacity > p0 ? (int)(ca.capacity - p0) : 0;
+      if (n > 0) store_rows<R>(c, p0, n, A);
+      if (c.valid)
+        write_valid_words<R>(c.valid, s_words[wave], lane, tile_prefix + wave_first, wcnt, p0,
+                             VA & ds.exist, ca.capacity);
     }
-    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -802,35 +936,31 @@ namespace bsql {
 using namespace bsql_dev;
 
 template <int NREG, int R>
-static hipError_t launch_execute_t(const DevPlan& plan, hipStream_t stream) {
+static hipError_t launch_execute_t(const DevPlan* d_plan, int64_t num_rows, hipStream_t stream) {
   const int64_t tile = (int64_t)BLOCK * R;
-  const int64_t grid = (plan.num_rows + tile - 1) / tile;
+  const int64_t grid = (num_rows + tile - 1) / tile;
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((vm_execute<NREG, R>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, plan);
+  CompactArgs ca{};
+  (void)ca;
+  hipLaunchKernelGGL((vm_execute<NREG, R>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan);
   return hipGetLastError();
 }
 
 template <int NREG, int R>
-static hipError_t launch_compact_t(const DevPlan& plan, const CompactArgs& ca, hipStream_t stream) {
+static hipError_t launch_compact_t(const DevPlan* d_plan, const CompactArgs& ca, hipStream_t stream) {
   hipLaunchKernelGGL((vm_compact<NREG, R>), dim3((unsigned)ca.num_tiles), dim3(BLOCK), 0, stream,
-                     plan, ca);
+                     d_plan, ca);
   return hipGetLastError();
 }
 
 // (NREG, R) variants compiled: register file size x rows per lane.
-#define BSQL_VARIANTS(X) X(8, 2) X(8, 4) X(16, 1) X(16, 2) X(32, 1)
-
-int tile_rows_for(int nreg, int rows_per_thread) {
-  (void)nreg;
-  return BLOCK * rows_per_thread;
-}
+#define BSQL_VARIANTS(X) X(8, 2) X(8, 4) X(16, 1) X(16, 2)
 
 // Picks the variant: the smallest register file that holds the program, and
-// the rows per lane requested (0 = largest that keeps the register file small).
+// the rows per lane requested (0 = the default for that register file).
 static bool pick(int nreg, int rpt_req, int* NR, int* RR) {
   struct V { int n, r; };
-  // default rows per lane for each register-file size first
-  const V vs[] = {{8, 4}, {16, 2}, {32, 1}, {8, 2}, {16, 1}};
+  const V vs[] = {{8, 4}, {16, 2}, {8, 2}, {16, 1}};
   for (const V& v : vs) {
     if (v.n < nreg) continue;
     if (rpt_req > 0 && v.r != rpt_req) continue;
@@ -842,10 +972,12 @@ static bool pick(int nreg, int rpt_req, int* NR, int* RR) {
   return false;
 }
 
-hipError_t launch_execute(const DevPlan& plan, int nreg, int rpt, hipStream_t stream) {
+// `d_plan` is a device copy of `plan` (made by the caller on `stream`).
+hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt,
+                          hipStream_t stream) {
   int NR, RR;
   if (!pick(nreg, rpt, &NR, &RR)) return hipErrorInvalidValue;
-#define X(N, RPT) if (NR == N && RR == RPT) return launch_execute_t<N, RPT>(plan, stream);
+#define X(N, RPT) if (NR == N && RR == RPT) return launch_execute_t<N, RPT>(d_plan, plan.num_rows, stream);
   BSQL_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;
@@ -859,8 +991,8 @@ int64_t compact_tiles(int64_t num_rows, int nreg, int rpt) {
 }
 
 // `status` must hold compact_tiles(...) + 1 words (the last one is the ticket).
-hipError_t launch_compact(const DevPlan& plan, int nreg, int rpt, uint64_t* status,
-                          int64_t* d_count, int64_t capacity, hipStream_t stream) {
+hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt,
+                          uint64_t* status, int64_t* d_count, int64_t capacity, hipStream_t stream) {
   int NR, RR;
   if (!pick(nreg, rpt, &NR, &RR)) return hipErrorInvalidValue;
   const int64_t tile = (int64_t)BLOCK * RR;
@@ -873,7 +1005,7 @@ hipError_t launch_compact(const DevPlan& plan, int nreg, int rpt, uint64_t* stat
                      status, nclear, 0ULL);
   hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, stream, (uint64_t*)d_count, (int64_t)1, 0ULL);
   if (tiles == 0) return hipGetLastError();
-#define X(N, RPT) if (NR == N && RR == RPT) return launch_compact_t<N, RPT>(plan, ca, stream);
+#define X(N, RPT) if (NR == N && RR == RPT) return launch_compact_t<N, RPT>(d_plan, ca, stream);
   BSQL_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;

@@ -452,6 +452,52 @@ void conjuncts_of(const Program& p, const SsaProgram& s, const std::vector<int>&
   }
 }
 
+// The conjunct with value v as a range test on one column, if it is a
+// comparison of a column with a literal in the int64 or double domain (the
+// domain the comparison op itself would use).
+bool as_range(const Lowering& L, const SsaProgram& s, const std::vector<int>& prod, int v,
+              DevRange& rg) {
+  const int o = v >= 0 ? prod[(size_t)v] : -1;
+  if (o < 0) return false;
+  const DevOp& d = s.ops[(size_t)o].d;
+  int base;
+  if (d.code >= D_EQ_II && d.code <= D_GE_II) base = D_EQ_II;
+  else if (d.code >= D_EQ_FF && d.code <= D_GE_FF) base = D_EQ_FF;
+  else if (d.code >= D_EQ_TS && d.code <= D_GE_TS && (d.aux2 & 15) == (d.aux2 >> 4)) base = D_EQ_TS;
+  else return false;
+  if (d.flags & (F_A_NULL | F_B_NULL)) return false;
+  const bool lit_left = (d.flags & F_A_IMM) != 0, lit_right = (d.flags & F_B_IMM) != 0;
+  if (lit_left == lit_right) return false;
+  const int cv = lit_left ? s.ops[(size_t)o].vb : s.ops[(size_t)o].va;
+  if (cv < 0 || !L.values[(size_t)cv].is_load) return false;
+  const int col = L.values[(size_t)cv].column;
+  if (is_string_type(L.in_types[col])) return false;
+  int k = d.code - base;  // EQ NE LT GT LE GE
+  if (lit_left) {
+    static const int flip[6] = {0, 1, 3, 2, 5, 4};
+    k = flip[k];
+  }
+  int64_t lit = d.imm;
+  uint8_t fl = 0;
+  if (base == D_EQ_FF) {
+    fl |= R_F;
+    const bool lit_f = (d.flags & (lit_left ? F_A_F : F_B_F)) != 0;
+    if (!lit_f) lit = f64_bits((double)d.imm);
+  }
+  rg = DevRange{};
+  rg.col = (uint8_t)col;
+  switch (k) {
+    case 0: fl |= R_HAS_LO | R_HAS_HI; rg.lo = rg.hi = lit; break;
+    case 1: fl |= R_NE; rg.lo = lit; break;
+    case 2: fl |= R_HAS_HI | R_HI_STRICT; rg.hi = lit; break;
+    case 3: fl |= R_HAS_LO | R_LO_STRICT; rg.lo = lit; break;
+    case 4: fl |= R_HAS_HI; rg.hi = lit; break;
+    default: fl |= R_HAS_LO; rg.lo = lit; break;
+  }
+  rg.flags = fl;
+  return true;
+}
+
 LOp to_lop(const SourceOp& so) {
   LOp l;
   l.d = so.d;
@@ -506,10 +552,42 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
     std::vector<int> conj;
     if (opt.conjuncts) conjuncts_of(pred, s, prod, s.finals[0], conj);
     res.n_conjuncts = (int)conj.size();
-    if (conj.size() >= 2) {
-      std::vector<int32_t> order(conj.size());
-      for (size_t j = 0; j < conj.size(); ++j) order[j] = (int32_t)j;
-      if (opt.conj_order && opt.conj_order->size() == conj.size()) order = *opt.conj_order;
+    std::vector<int32_t> order(conj.size());
+    for (size_t j = 0; j < conj.size(); ++j) order[j] = (int32_t)j;
+    if (opt.conj_order && opt.conj_order->size() == conj.size()) order = *opt.conj_order;
+    // every conjunct a column-vs-literal comparison: range tests, no ops
+    std::vector<DevRange> ranges;
+    if (opt.ranges && !conj.empty()) {
+      for (int32_t j : order) {
+        DevRange rg;
+        if (!as_range(L, s, prod, conj[(size_t)j], rg)) { ranges.clear(); break; }
+        // a lower and an upper bound on the same column in a row make one test
+        if (!ranges.empty()) {
+          DevRange& pr = ranges.back();
+          const uint8_t kind = R_F | R_NE;
+          const bool lo_only = (pr.flags & (R_HAS_LO | R_HAS_HI | R_NE)) == R_HAS_LO;
+          const bool hi_only = (pr.flags & (R_HAS_LO | R_HAS_HI | R_NE)) == R_HAS_HI;
+          const uint8_t rf = rg.flags & (R_HAS_LO | R_HAS_HI | R_NE);
+          if (pr.col == rg.col && (pr.flags & R_F) == (rg.flags & R_F) && !((pr.flags | rg.flags) & R_NE)) {
+            if (lo_only && rf == R_HAS_HI) {
+              pr.flags |= rg.flags & ~kind;
+              pr.hi = rg.hi;
+              continue;
+            }
+            if (hi_only && rf == R_HAS_LO) {
+              pr.flags |= rg.flags & ~kind;
+              pr.lo = rg.lo;
+              continue;
+            }
+          }
+        }
+        ranges.push_back(rg);
+      }
+      if (ranges.size() > BSQL_DEV_MAX_RANGES) ranges.clear();
+    }
+    if (!ranges.empty()) {
+      res.plan_ranges = ranges;
+    } else if (conj.size() >= 2) {
       std::set<int> done;
       for (int32_t j : order) {
         std::set<int> sub;
@@ -534,6 +612,9 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
     }
   }
   const int split = (int)L.out.size();
+  // The output phase runs over the selected rows only, packed densely: it
+  // reloads what it reads instead of reusing selection-phase registers.
+  if (opt.mode != LowerOptions::EXECUTE) L.load_value.assign((size_t)n_in, -1);
 
   // ---- phase 2: outputs
   if (opt.mode == LowerOptions::FILTER_GATHER) {
@@ -606,7 +687,8 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
   }
   // values read in phase 2 but defined in phase 1 stay live across the split
   std::vector<int> free_regs;
-  for (int r = BSQL_DEV_MAX_REG - 1; r >= 0; --r) free_regs.push_back(r);
+  // the last register of the file is the spare that result-less ops write
+  for (int r = BSQL_DEV_MAX_REG - 2; r >= 0; --r) free_regs.push_back(r);
   int used = 0;
   for (size_t i = 0; i < L.out.size(); ++i) {
     LOp& o = L.out[i];
@@ -627,7 +709,7 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
     std::sort(free_regs.begin(), free_regs.end(), std::greater<int>());
     if (o.vdst >= 0) {
       Value& dv = L.values[(size_t)o.vdst];
-      if (free_regs.empty()) fail("program needs more than 32 live values");
+      if (free_regs.empty()) fail("program needs more than 15 live values");
       dv.reg = free_regs.back();
       free_regs.pop_back();
       used = std::max(used, dv.reg + 1);
@@ -641,36 +723,45 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
   for (LOp& o : L.out)
     if (o.d.code == D_LOAD || o.d.code == D_LOADV) o.d.a = (uint8_t)L.values[(size_t)o.vdst].column;
 
-  // merge runs of column loads (same segment) into LOADN ops of up to 4
+  // result-less ops write no register
+  for (LOp& o : L.out)
+    if (o.vdst < 0) o.d.dst = 0xff;
+  // runs of column loads (same segment) become one LOADN, which issues up to
+  // four columns' loads together, and one COMMIT per column
   {
     std::vector<LOp> merged;
-    int merged_split = new_split;
     for (size_t i = 0; i < L.out.size();) {
       if (L.out[i].d.code != D_LOAD) { merged.push_back(L.out[i]); ++i; continue; }
-      LOp m = L.out[i];
-      m.d.code = D_LOADN;
-      m.d.imm = 0;
+      LOp issue;
+      issue.d.code = D_LOADN;
+      issue.d.dst = 0xff;
+      issue.d.imm = 0;
+      std::vector<LOp> commits;
       int n = 0;
       size_t j = i;
       while (j < L.out.size() && L.out[j].d.code == D_LOAD && n < 4) {
-        const uint64_t pair = (uint64_t)L.out[j].d.a | ((uint64_t)L.out[j].d.dst << 8);
-        m.d.imm |= (int64_t)(pair << (16 * n));
+        issue.d.imm |= (int64_t)((uint64_t)L.out[j].d.a << (8 * n));
+        LOp c;
+        c.d.code = D_COMMIT;
+        c.d.aux = (uint8_t)n;
+        c.d.dst = L.out[j].d.dst;
+        commits.push_back(c);
         ++n;
         ++j;
       }
-      m.d.aux = (uint8_t)n;
-      merged.push_back(m);
+      issue.d.aux = (uint8_t)n;
+      merged.push_back(issue);
+      for (const LOp& c : commits) merged.push_back(c);
       i = j;
     }
     L.out = merged;
     if (opt.mode != LowerOptions::EXECUTE) {
-      // recompute the split: one past the last selection op
+      // the selection phase ends after its last KILL/SEL
       int last_sel = -1;
       for (size_t i = 0; i < L.out.size(); ++i)
         if (L.out[i].d.code == D_KILL || L.out[i].d.code == D_SEL) last_sel = (int)i;
-      merged_split = last_sel + 1;
+      new_split_final = last_sel + 1;
     }
-    new_split_final = merged_split;
   }
 
   if (L.out.size() > BSQL_DEV_MAX_OPS) fail("program too long for one launch");
@@ -680,8 +771,11 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
   res.plan.n_in = n_in;
   res.plan.n_out = n_out;
   for (size_t i = 0; i < L.out.size(); ++i) res.plan.ops[i] = L.out[i].d;
+  res.plan.n_ranges = (int32_t)res.plan_ranges.size();
+  res.plan.range_eager = opt.lazy_loads ? 0 : 1;
+  for (size_t i = 0; i < res.plan_ranges.size(); ++i) res.plan.ranges[i] = res.plan_ranges[i];
   (void)out_types;
-  res.nreg = std::max(used, 1);
+  res.nreg = used + 1;  // + the spare register
   return res;
 }
 

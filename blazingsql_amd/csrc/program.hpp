@@ -91,6 +91,7 @@ struct LowerOptions {
   const Program* proj = nullptr;   // EXECUTE / FILTER_PROJECT: outputs
   bool conjuncts = true;           // wave-level early exit per conjunct
   bool lazy_loads = false;         // load each conjunct's columns only when it is reached
+  bool ranges = true;              // column-vs-literal conjuncts become range tests
   const std::vector<int32_t>* conj_order = nullptr;
 };
 
@@ -98,6 +99,7 @@ struct Lowered {
   DevPlan plan;          // columns are filled by the launcher
   int nreg = 0;          // registers used
   int n_conjuncts = 0;   // conjuncts found in the predicate (0 = not an AND chain)
+  std::vector<DevRange> plan_ranges;  // the selection as range tests (empty: ops)
 };
 Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
               const int32_t* out_types, int n_out);

@@ -152,8 +152,12 @@ class Program:
             arr = (ctypes.c_int32 * len(order))(*order)
             L.check(L.lib().bsql_program_set_conjunct_order(self._h, arr, len(order)))
 
-    def set_launch(self, rows_per_thread=0, block_size=0):
-        cfg = L.LaunchConfig(block_size, rows_per_thread)
+    def set_launch(self, rows_per_thread=0, block_size=0, ranges=True, lazy_loads=False):
+        """Launch shape for this program; for a predicate also whether its
+        column-vs-literal conjuncts run as range tests and whether conjunct
+        columns are loaded lazily."""
+        flags = (0 if ranges else L.LAUNCH_NO_RANGES) | (L.LAUNCH_LAZY_LOADS if lazy_loads else 0)
+        cfg = L.LaunchConfig(block_size, rows_per_thread, flags)
         L.check(L.lib().bsql_program_set_launch(self._h, ctypes.byref(cfg)))
 
     def to_oracle_dict(self):

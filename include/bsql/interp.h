@@ -217,17 +217,25 @@ int bsql_program_set_conjunct_order(bsql_program* pred, const int32_t* order,
                                     int n_conj);
 
 /* ---- launch configuration (autotuner hook) ------------------------------ */
+enum {
+  BSQL_LAUNCH_NO_RANGES = 1,  /* predicates: evaluate column-vs-literal conjuncts
+                                 with the general ops instead of range tests */
+  BSQL_LAUNCH_LAZY_LOADS = 2  /* predicates: load a conjunct's columns only when
+                                 the wave reaches it (default: all up front) */
+};
 typedef struct bsql_launch_config {
-  int32_t block_size;     /* threads per workgroup: 64, 128, 256, 512 */
-  int32_t rows_per_thread;/* 1, 2, 4, 8 */
+  int32_t block_size;     /* threads per workgroup: 0 (auto) or 256 */
+  int32_t rows_per_thread;/* 0 (auto), 1, 2, 4 */
+  int32_t flags;          /* BSQL_LAUNCH_* */
 } bsql_launch_config;
-/* Sets the configuration used for programs of this one (0 fields = auto). */
+/* Sets the configuration used for launches with this program (a predicate's
+ * flags apply to filters it drives; 0 fields = auto). No counterpart in the
+ * reference, whose launch shape is fixed (interpreter_cpp.cu:628-660). */
 int bsql_program_set_launch(bsql_program* prog, const bsql_launch_config* cfg);
 
 /* ---- diagnostics --------------------------------------------------------- */
+/* Message of the last error on the calling thread. */
 const char* bsql_last_error(void);
-/* Device kernel statistics of the last launch on the calling thread:
- * name of the dominant kernel and the number of workgroups launched. */
 int bsql_version(void);
 
 #ifdef __cplusplus

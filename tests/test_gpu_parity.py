@@ -4,7 +4,7 @@ size-independent properties at large sizes."""
 import numpy as np
 import pytest
 
-from tests.helpers import (BOOL8, FLOAT64, INT32, INT64, TS_D, ExprGen, assert_same, device_columns,
+from tests.helpers import (BOOL8, FLOAT32, FLOAT64, INT8, INT32, INT64, TS_D, ExprGen, assert_same, device_columns,
                            eval_program_both, host_columns, make_table)
 
 pytestmark = pytest.mark.gpu
@@ -228,8 +228,9 @@ C2_PRED = "AND(<=($4, 10471), >=($2, 0.05:DECIMAL(3, 2)), <=($2, 0.07:DECIMAL(3,
 C2_PROJ = ["*(*($1, -(1, $2)), +(1, $3))", "*($1, $2)"]
 
 
+@pytest.mark.parametrize("mode", ["ranges", "ops", "ops_lazy"])
 @pytest.mark.parametrize("order", [None, [3, 1, 2, 0], [1, 0, 3, 2]])
-def test_filter_project_c2_orders_vs_oracle(bsql, gpu, order):
+def test_filter_project_c2_orders_vs_oracle(bsql, gpu, order, mode):
     import oracle
     import torch
     from blazingsql_amd.interop import DeviceColumn, filter_project
@@ -240,6 +241,7 @@ def test_filter_project_c2_orders_vs_oracle(bsql, gpu, order):
     proj = bsql.Program.compile(C2_PROJ, types)
     if order is not None:
         pred.set_conjunct_order(order)
+    pred.set_launch(ranges=mode == "ranges", lazy_loads=mode == "ops_lazy")
     outs = [DeviceColumn.empty(FLOAT64, n), DeviceColumn.empty(FLOAT64, n)]
     cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
     filter_project(pred, proj, device_columns(table), outs, cnt, n)
@@ -303,3 +305,57 @@ def test_empty_and_single_row_execute(bsql, gpu):
     perform_interpreter_operation(p, ins, outs, 1)
     v, ok = outs[0].to_numpy()
     assert v[0] == 42 and ok[0]
+
+
+def _range_predicate(rng, types, table):
+    """A random AND chain of column-vs-literal comparisons (the shape that
+    runs as range tests), sometimes with a literal on the left, a null
+    literal, or a mixed int/float domain. Literals are drawn near the column's
+    values so that the chains select a fair share of the rows."""
+    ops = ["<", ">", "<=", ">=", "<>", "="]
+    conj = []
+    for _ in range(int(rng.integers(1, 5))):
+        c = int(rng.integers(0, len(types)))
+        t = types[c]
+        v = table[c][1][int(rng.integers(0, len(table[c][1])))]
+        r = rng.random()
+        if t in (FLOAT64, FLOAT32):
+            lit = f"{float(v):.2f}:DECIMAL(6, 2)" if r < 0.7 else str(int(v))
+        elif t == TS_D:
+            d = np.datetime64(int(v), "D").astype(str)
+            lit = d if r < 0.6 and len(d) == 10 else str(int(v))
+        else:
+            lit = str(int(v)) if r < 0.8 else f"{int(v) + 0.5:.1f}:DECIMAL(8, 1)"
+        if rng.random() < 0.04:
+            lit = "null:INTEGER"
+        op = ops[int(rng.choice(6, p=[0.22, 0.22, 0.22, 0.22, 0.08, 0.04]))]
+        conj.append(f"{op}({lit}, ${c})" if rng.random() < 0.3 else f"{op}(${c}, {lit})")
+    return conj[0] if len(conj) == 1 else "AND(" + ", ".join(conj) + ")"
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_range_predicates_vs_oracle(bsql, gpu, seed):
+    """Column-vs-literal AND chains: range tests and the general ops select the
+    same rows as the oracle, with and without nulls, in both domains."""
+    import oracle
+    import torch
+    from blazingsql_amd.interop import DeviceColumn, filter_columns
+    rng = np.random.default_rng(1000 + seed)
+    n = 200_003
+    types = [INT32, FLOAT64, INT64, FLOAT32, TS_D, INT8]
+    table = make_table(rng, types, n, null_rates=[0.1, 0.05, 0.0, 0.0, 0.02, 0.0])
+    dcols = device_columns(table)
+    for _ in range(4):
+        text = _range_predicate(rng, types, table)
+        on, ores = oracle.filter_table(_full_prog(bsql, text, types), host_columns(table), n)
+        for ranges in (True, False):
+            pred = bsql.Program.compile(text, types)
+            pred.set_launch(ranges=ranges)
+            outs = [DeviceColumn.empty(t, n, nullable=v is not None) for t, _, v in table]
+            cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+            filter_columns(pred, dcols, outs, cnt, n)
+            m = int(cnt.item())
+            assert m == on, (text, ranges, m, on)
+            for k in range(len(types)):
+                gv, gval = outs[k].to_numpy(m)
+                assert_same(gv, gval, ores[k][0], ores[k][1], what=f"{text} ranges={ranges} col {k}")
