@@ -20,9 +20,10 @@ namespace bsql {
 
 hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt,
                           hipStream_t stream);
-int64_t compact_tiles(int64_t num_rows, int nreg, int rpt);
-hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt,
-                          uint64_t* status, int64_t* d_count, int64_t capacity, hipStream_t stream);
+int64_t compact_tiles(const DevPlan& plan, int nreg, int rpt);
+hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int sel_rpt, int rpt,
+                          uint64_t* status, uint32_t* idx, int64_t* d_count, int64_t capacity,
+                          hipStream_t stream);
 
 namespace {
 
@@ -331,8 +332,8 @@ int bsql_program_set_conjunct_order(bsql_program* pred, const int32_t* order, in
 int bsql_program_set_launch(bsql_program* p, const bsql_launch_config* cfg) {
   if (!p || !cfg) return set_error(BSQL_ERR_INVALID, "invalid arguments");
   const int r = cfg->rows_per_thread;
-  if (r != 0 && r != 1 && r != 2 && r != 4)
-    return set_error(BSQL_ERR_INVALID, "rows_per_thread must be 0, 1, 2 or 4");
+  if (r != 0 && r != 1 && r != 2 && r != 4 && r != 16 && r != 32)
+    return set_error(BSQL_ERR_INVALID, "rows_per_thread must be 0, 1, 2, 4, 16 or 32");
   if (cfg->block_size != 0 && cfg->block_size != 256)
     return set_error(BSQL_ERR_INVALID, "block_size must be 0 or 256");
   if (cfg->flags & ~(BSQL_LAUNCH_NO_RANGES | BSQL_LAUNCH_LAZY_LOADS))
@@ -386,6 +387,8 @@ static int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column
   if (!pred) fail("null predicate");
   if (!d_count) fail("null count pointer");
   check_inputs(in, n_in, num_rows);
+  // selected rows are listed as 32-bit row ids
+  if (num_rows > (int64_t)UINT32_MAX) fail("a filter handles at most 2^32-1 rows per call");
   for (int k = 0; k < n_out; ++k) {
     if (out[k].offset != 0) fail("output columns must have offset 0");
     if (type_size(out[k].type_id) == 0) fail("unsupported output column type");
@@ -411,15 +414,22 @@ static int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column
     plan.out[k].valid = out[k].validity;
     plan.out[k].type = out[k].type_id;
   }
+  // the predicate's launch shape drives the selection kernel, the
+  // projection's (or the predicate's, for a plain filter) the output kernel
+  const int sel_rpt = rows_per_thread(pred);
   const int rpt = rows_per_thread(proj ? proj : pred);
-  const int64_t tiles = compact_tiles(num_rows, L.nreg, rpt);
+  const int64_t tiles = compact_tiles(plan, L.nreg, sel_rpt);
   if (tiles < 0) fail("no kernel variant for this program");
+  // scratch: plan | tile status words + ticket | selected row ids (uint32)
+  const int64_t id_words = (num_rows + 1) / 2;
   uint64_t* scratch = nullptr;
-  e = get_scratch((hipStream_t)stream, tiles + 1, &scratch);
+  e = get_scratch((hipStream_t)stream, tiles + 1 + id_words, &scratch);
   if (e == hipSuccess) e = upload_plan(plan, scratch, (hipStream_t)stream);
   if (e != hipSuccess) return device_error(e, "filter scratch");
-  e = launch_compact(plan, (const DevPlan*)scratch, L.nreg, rpt, scratch + kPlanWords, d_count,
-                     capacity, (hipStream_t)stream);
+  uint64_t* status = scratch + kPlanWords;
+  uint32_t* ids = (uint32_t*)(status + tiles + 1);
+  e = launch_compact(plan, (const DevPlan*)scratch, L.nreg, sel_rpt, rpt, status, ids, d_count, capacity,
+                     (hipStream_t)stream);
   if (e != hipSuccess) return device_error(e, "filter");
   return BSQL_OK;
 }

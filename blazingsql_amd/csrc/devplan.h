@@ -62,6 +62,10 @@ typedef struct DevOp {
   int64_t imm;   // literal bits; *_SS: device pointer to {int32 len, bytes}
 } DevOp;
 
+// Every field the kernels read is 32 or 64 bits wide: gfx950 has no scalar
+// byte loads, and a narrow field read by a whole wave would become a vector
+// load (and a wait on every load in flight).
+
 // Input column as the kernel sees it: data already advanced to the view's
 // first row; validity pointer advanced by whole words, vbit = remaining bit.
 typedef struct DevInCol {
@@ -69,8 +73,9 @@ typedef struct DevInCol {
   const uint32_t* valid;  // NULL = all valid
   const int32_t* offsets; // strings
   int32_t vbit;
-  int16_t type;
-  int16_t aligned16;      // data is 16-byte aligned
+  int32_t type;
+  int32_t aligned16;      // data is 16-byte aligned
+  int32_t pad;
 } DevInCol;
 
 typedef struct DevOutCol {
@@ -80,19 +85,21 @@ typedef struct DevOutCol {
   int32_t pad;
 } DevOutCol;
 
-// One conjunct of a range predicate: column `col` (slot kind `is_f`) must be
-// valid and lie in [lo, hi] (bounds optional / strict per flags), or, with
-// R_NE, differ from lo. Literals are already in the comparison domain.
+// One conjunct (or several merged conjuncts on one column) of a predicate
+// made of column-vs-literal comparisons: column `col` must be valid and lie
+// in [lo, hi] (inclusive bounds, computed on the host from the comparison
+// operators), or with RK_NE differ from lo. RK_F: the comparison domain is
+// double (the column converted as the float ops convert it), else int64.
+// RK_EMPTY: no value passes. In the int64 domain the column value is first
+// multiplied by `mul` (wrapping): timestamps compare as nanoseconds.
 #define BSQL_DEV_MAX_RANGES 16
-enum : uint8_t {
-  R_HAS_LO = 1, R_LO_STRICT = 2, R_HAS_HI = 4, R_HI_STRICT = 8, R_NE = 16, R_F = 32
-};
+enum : int32_t { RK_F = 1, RK_NE = 2, RK_EMPTY = 4 };
 typedef struct DevRange {
-  uint8_t col;
-  uint8_t flags;
-  uint8_t pad[6];
-  int64_t lo;  // int64 or double bits (R_F)
+  int32_t col;
+  int32_t kind;
+  int64_t lo;  // int64 or double bits (RK_F)
   int64_t hi;
+  int64_t mul;
 } DevRange;
 
 typedef struct DevPlan {

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "devplan.h"
 #include "../../include/bsql/interp.h"
 
@@ -31,6 +33,16 @@ constexpr int BLOCK = 256;
 constexpr int WAVES = BLOCK / 64;
 
 typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+typedef uint32_t v2u32 __attribute__((ext_vector_type(2)));
+
+// Column pointers arrive through the plan in memory, so the compiler cannot
+// tell that they address global memory and would emit flat loads/stores;
+// every column access goes through a global (address space 1) pointer.
+#define GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const GLOBAL T* gp(const void* p) { return (const GLOBAL T*)p; }
+template <class T>
+__device__ __forceinline__ GLOBAL T* gpw(void* p) { return (GLOBAL T*)p; }
 
 constexpr int64_t MAGIC_I64 = INT64_MAX - 13LL;       // interpreter_ops.cuh:41-43
 constexpr double MAGIC_F64 = 1.7976931348623123e+308;  // interpreter_ops.cuh:46-48
@@ -136,7 +148,7 @@ __device__ double philox_uniform(uint64_t seed, uint64_t row) {
 }
 
 // string compare (cudf::string_view::compare: bytewise, shorter prefix first)
-__device__ int str_cmp(const uint8_t* a, int32_t na, const uint8_t* b, int32_t nb) {
+__device__ int str_cmp(const GLOBAL uint8_t* a, int32_t na, const GLOBAL uint8_t* b, int32_t nb) {
   const int32_t n = na < nb ? na : nb;
   for (int32_t i = 0; i < n; ++i) {
     if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
@@ -164,11 +176,11 @@ __device__ __forceinline__ void load_rows(const DevInCol& c, int64_t r0, int n, 
   const bool vec = (n == R) && c.aligned16;
 #define BSQL_LOAD_CASE(TYPE_ID, CT, EXPR)                                          \
   case TYPE_ID: {                                                                  \
-    const CT* p = (const CT*)c.data + r0;                                          \
+    const GLOBAL CT* p = gp<CT>(c.data) + r0;                                      \
     CT tmp[R];                                                                     \
     if (vec && (sizeof(CT) * R) % 16 == 0) {                                       \
       _Pragma("unroll") for (int k = 0; k < (int)(sizeof(CT) * R / 16); ++k)       \
-        ((v4u32*)tmp)[k] = __builtin_nontemporal_load((const v4u32*)p + k);        \
+        ((v4u32*)tmp)[k] = __builtin_nontemporal_load((const GLOBAL v4u32*)p + k); \
     } else {                                                                       \
       _Pragma("unroll") for (int k = 0; k < R; ++k) tmp[k] = k < n ? p[k] : CT(0); \
     }                                                                              \
@@ -190,12 +202,12 @@ __device__ __forceinline__ void load_rows(const DevInCol& c, int64_t r0, int n, 
     BSQL_LOAD_CASE(BSQL_TIMESTAMP_DAYS, int32_t, (uint64_t)(int64_t)x)
     BSQL_LOAD_CASE(BSQL_DURATION_DAYS, int32_t, (uint64_t)(int64_t)x)
     default: {  // 8-byte ticks: timestamps s/ms/us/ns, durations s/ms/us/ns
-      const int64_t* p = (const int64_t*)c.data + r0;
+      const GLOBAL int64_t* p = gp<int64_t>(c.data) + r0;
       int64_t tmp[R];
       if (vec && (8 * R) % 16 == 0) {
 #pragma unroll
         for (int k = 0; k < 8 * R / 16; ++k)
-          ((v4u32*)tmp)[k] = __builtin_nontemporal_load((const v4u32*)p + k);
+          ((v4u32*)tmp)[k] = __builtin_nontemporal_load((const GLOBAL v4u32*)p + k);
       } else {
 #pragma unroll
         for (int k = 0; k < R; ++k) tmp[k] = k < n ? p[k] : 0;
@@ -216,8 +228,9 @@ __device__ __forceinline__ uint32_t load_valid(const DevInCol& c, int64_t r0, in
   const int64_t b = (int64_t)c.vbit + r0;
   const int64_t w = b >> 5;
   const int sh = (int)(b & 31);
-  uint64_t word = c.valid[w];
-  if (sh + R > 32 && n > 32 - sh) word |= (uint64_t)c.valid[w + 1] << 32;
+  const GLOBAL uint32_t* vp = gp<uint32_t>(c.valid);
+  uint64_t word = vp[w];
+  if (sh + R > 32 && n > 32 - sh) word |= (uint64_t)vp[w + 1] << 32;
   uint32_t m = (uint32_t)(word >> sh) & ((1u << R) - 1u);
   return n >= R ? m : (m & ((1u << n) - 1u));
 }
@@ -225,13 +238,13 @@ __device__ __forceinline__ uint32_t load_valid(const DevInCol& c, int64_t r0, in
 // static_cast of slot bits to the output column type (interpreter_ops.cuh:329-395)
 __device__ __forceinline__ void store_elem(const DevOutCol& c, int64_t row, uint64_t b) {
   switch (c.type) {
-    case BSQL_INT8: case BSQL_UINT8: ((uint8_t*)c.data)[row] = (uint8_t)b; break;
-    case BSQL_BOOL8: ((uint8_t*)c.data)[row] = b != 0; break;
-    case BSQL_INT16: case BSQL_UINT16: ((uint16_t*)c.data)[row] = (uint16_t)b; break;
+    case BSQL_INT8: case BSQL_UINT8: gpw<uint8_t>(c.data)[row] = (uint8_t)b; break;
+    case BSQL_BOOL8: gpw<uint8_t>(c.data)[row] = b != 0; break;
+    case BSQL_INT16: case BSQL_UINT16: gpw<uint16_t>(c.data)[row] = (uint16_t)b; break;
     case BSQL_INT32: case BSQL_UINT32: case BSQL_TIMESTAMP_DAYS: case BSQL_DURATION_DAYS:
-      ((uint32_t*)c.data)[row] = (uint32_t)b; break;
-    case BSQL_FLOAT32: ((float*)c.data)[row] = (float)as_f(b); break;
-    default: ((uint64_t*)c.data)[row] = b; break;  // int64, uint64, float64, 8-byte ticks
+      gpw<uint32_t>(c.data)[row] = (uint32_t)b; break;
+    case BSQL_FLOAT32: gpw<float>(c.data)[row] = (float)as_f(b); break;
+    default: gpw<uint64_t>(c.data)[row] = b; break;  // int64, uint64, float64, 8-byte ticks
   }
 }
 
@@ -240,13 +253,13 @@ template <int R>
 __device__ __forceinline__ void store_rows(const DevOutCol& c, int64_t r0, int n, const uint64_t (&v)[R]) {
   if (n == R && (c.type == BSQL_FLOAT64 || c.type == BSQL_INT64) && (R % 2 == 0) &&
       ((((uintptr_t)((uint64_t*)c.data + r0)) & 15) == 0)) {
-    uint64_t* p = (uint64_t*)c.data + r0;
+    GLOBAL uint64_t* p = gpw<uint64_t>(c.data) + r0;
 #pragma unroll
     for (int k = 0; k < R / 2; ++k) {
       v4u32 q;
       q.x = (uint32_t)v[2 * k]; q.y = (uint32_t)(v[2 * k] >> 32);
       q.z = (uint32_t)v[2 * k + 1]; q.w = (uint32_t)(v[2 * k + 1] >> 32);
-      __builtin_nontemporal_store(q, (v4u32*)p + k);
+      __builtin_nontemporal_store(q, (GLOBAL v4u32*)p + k);
     }
     return;
   }
@@ -285,23 +298,25 @@ __device__ __noinline__ uint64_t compute_heavy(uint8_t code, uint8_t aux, uint8_
 }
 
 __device__ __forceinline__ void str_at(const DevPlan& plan, uint8_t col, int64_t imm, int64_t row,
-                                       const uint8_t** p, int32_t* n) {
+                                       const GLOBAL uint8_t** p, int32_t* n) {
   if (col != 0xff) {
     const DevInCol& c = plan.in[col];
-    const int32_t s = c.offsets[row], e = c.offsets[row + 1];
-    *p = (const uint8_t*)c.data + s;
+    const GLOBAL int32_t* off = gp<int32_t>(c.offsets);
+    const int32_t s = off[row], e = off[row + 1];
+    *p = gp<uint8_t>(c.data) + s;
     *n = e - s;
   } else {
-    const int32_t* lit = (const int32_t*)imm;
+    const GLOBAL int32_t* lit = gp<int32_t>((const void*)imm);
     *n = lit[0];
-    *p = (const uint8_t*)(lit + 1);
+    *p = (const GLOBAL uint8_t*)(lit + 1);
   }
 }
 
 #define FA ((op.flags & F_A_F) != 0)
 #define FB ((op.flags & F_B_F) != 0)
-#define AF to_f(a, FA)
-#define BF to_f(b, FB)
+// float ops: the lowering hands them both operands as doubles
+#define AF as_f(a)
+#define BF as_f(b)
 // one result per row: D[k] and validity bit k of VD
 #define ROWS(EXPR_D, EXPR_V)                                                   \
   _Pragma("unroll") for (int k = 0; k < R; ++k) {                              \
@@ -332,7 +347,7 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
     case D_ADD_FF: ROWS(bits(__dadd_rn(AF, BF)), va & vb)
     case D_SUB_FF: ROWS(bits(__dsub_rn(AF, BF)), va & vb)
     case D_MUL_FF: ROWS(bits(__dmul_rn(AF, BF)), va & vb)
-    case D_DIV_FF: ROWS(bits(__ddiv_rn(AF, BF)), va & vb & (FB ? !(as_f(b) == 0.0) : (b != 0)))
+    case D_DIV_FF: ROWS(bits(__ddiv_rn(AF, BF)), va & vb & !(BF == 0.0))
     case D_EQ_II: ROWS((int64_t)a == (int64_t)b, va & vb)
     case D_NE_II: ROWS((int64_t)a != (int64_t)b, va & vb)
     case D_LT_II: ROWS((int64_t)a < (int64_t)b, va & vb)
@@ -378,7 +393,7 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
     case D_IS_NOT_TRUE: ROWS(!(va && (FA ? as_f(a) == 1.0 : a == 1)), 1u)
     case D_IS_NOT_FALSE: ROWS(!(va && (FA ? as_f(a) == 0.0 : a == 0)), 1u)
     case D_TO_I64: ROWS(FA ? (uint64_t)f64_to_i64(as_f(a)) : a, va)
-    case D_TO_F64: ROWS(bits(AF), va)
+    case D_TO_F64: ROWS(bits(to_f(a, FA)), va)
     case D_CAST_DATE: {
       const bool sub_day = is_ts(op.aux2) && op.aux2 != BSQL_TIMESTAMP_DAYS;
       const int64_t tpd = ticks_per_day(op.aux2);
@@ -402,7 +417,7 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
       for (int k = 0; k < R; ++k) {
         D[k] = 0;
         if (((rs.exist >> k) & 1u) && ((VA >> k) & 1u) && ((VB >> k) & 1u)) {
-          const uint8_t *pa, *pb;
+          const GLOBAL uint8_t *pa, *pb;
           int32_t na, nb;
           str_at(plan, op.aux, op.imm, rs.row[k], &pa, &na);
           str_at(plan, op.aux2, op.imm, rs.row[k], &pb, &nb);
@@ -418,8 +433,9 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
       for (int k = 0; k < R; ++k) {
         D[k] = 0;
         if (((rs.exist >> k) & 1u) && ((VA >> k) & 1u)) {
-          const int32_t s = c.offsets[rs.row[k]], e = c.offsets[rs.row[k] + 1];
-          const uint8_t* p = (const uint8_t*)c.data;
+          const GLOBAL int32_t* off = gp<int32_t>(c.offsets);
+          const int32_t s = off[rs.row[k]], e = off[rs.row[k] + 1];
+          const GLOBAL uint8_t* p = gp<uint8_t>(c.data);
           int64_t n = 0;
           for (int32_t i = s; i < e; ++i) n += (p[i] & 0xC0) != 0x80;
           D[k] = (uint64_t)n;
@@ -452,10 +468,10 @@ struct CompactArgs {
 };
 
 __device__ __forceinline__ uint64_t ld_status(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load((const GLOBAL uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_status(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store((GLOBAL uint64_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ int64_t wave_sum(int64_t v) {
@@ -494,6 +510,24 @@ __device__ int64_t look_back(const CompactArgs& ca, int64_t tile, int64_t agg, i
   return excl;
 }
 
+// One op of the plan, read as two dwords and the literal so that the whole
+// wave reads it with scalar loads (the byte fields are unpacked in SGPRs).
+__device__ __forceinline__ DevOp load_op(const DevPlan& plan, int q) {
+  const uint32_t* p = (const uint32_t*)&plan.ops[q];
+  const uint32_t w0 = p[0], w1 = p[1];
+  DevOp op;
+  op.code = (uint8_t)w0;
+  op.dst = (uint8_t)(w0 >> 8);
+  op.a = (uint8_t)(w0 >> 16);
+  op.b = (uint8_t)(w0 >> 24);
+  op.flags = (uint8_t)w1;
+  op.aux = (uint8_t)(w1 >> 8);
+  op.aux2 = (uint8_t)(w1 >> 16);
+  op.pad = 0;
+  op.imm = *(const int64_t*)(p + 2);
+  return op;
+}
+
 // -------------------------------------------------------------- rows
 
 // The rows one lane works on: row[k] for k < R where bit k of `exist` is set.
@@ -519,16 +553,16 @@ __device__ __forceinline__ void gather_rows(const DevInCol& c, const Rows<R>& rs
     if (!((rs.exist >> k) & 1u)) continue;
     const int64_t r = rs.row[k];
     switch (c.type) {
-      case BSQL_INT8: v[k] = (uint64_t)(int64_t)((const int8_t*)c.data)[r]; break;
-      case BSQL_INT16: v[k] = (uint64_t)(int64_t)((const int16_t*)c.data)[r]; break;
+      case BSQL_INT8: v[k] = (uint64_t)(int64_t)gp<int8_t>(c.data)[r]; break;
+      case BSQL_INT16: v[k] = (uint64_t)(int64_t)gp<int16_t>(c.data)[r]; break;
       case BSQL_INT32: case BSQL_TIMESTAMP_DAYS: case BSQL_DURATION_DAYS:
-        v[k] = (uint64_t)(int64_t)((const int32_t*)c.data)[r]; break;
-      case BSQL_UINT8: v[k] = ((const uint8_t*)c.data)[r]; break;
-      case BSQL_BOOL8: v[k] = ((const uint8_t*)c.data)[r] != 0; break;
-      case BSQL_UINT16: v[k] = ((const uint16_t*)c.data)[r]; break;
-      case BSQL_UINT32: v[k] = ((const uint32_t*)c.data)[r]; break;
-      case BSQL_FLOAT32: v[k] = bits((double)((const float*)c.data)[r]); break;
-      default: v[k] = ((const uint64_t*)c.data)[r]; break;  // 8-byte types
+        v[k] = (uint64_t)(int64_t)gp<int32_t>(c.data)[r]; break;
+      case BSQL_UINT8: v[k] = gp<uint8_t>(c.data)[r]; break;
+      case BSQL_BOOL8: v[k] = gp<uint8_t>(c.data)[r] != 0; break;
+      case BSQL_UINT16: v[k] = gp<uint16_t>(c.data)[r]; break;
+      case BSQL_UINT32: v[k] = gp<uint32_t>(c.data)[r]; break;
+      case BSQL_FLOAT32: v[k] = bits((double)gp<float>(c.data)[r]); break;
+      default: v[k] = gp<uint64_t>(c.data)[r]; break;  // 8-byte types
     }
   }
 }
@@ -541,7 +575,7 @@ __device__ __forceinline__ uint32_t gather_valid(const DevInCol& c, const Rows<R
   for (int k = 0; k < R; ++k) {
     if (!((rs.exist >> k) & 1u)) continue;
     const int64_t b = (int64_t)c.vbit + rs.row[k];
-    m |= ((c.valid[b >> 5] >> (b & 31)) & 1u) << k;
+    m |= ((gp<uint32_t>(c.valid)[b >> 5] >> (b & 31)) & 1u) << k;
   }
   return m;
 }
@@ -647,210 +681,123 @@ __device__ __forceinline__ bool vm_step(const DevPlan& plan, const DevOp& op,
   return dead;
 }
 
-// Writes the validity bits `bits` (bit k = lane row k) of the output rows at
-// positions [p0, p0 + R) of this lane, where the wave covers [wpos, wpos+wcnt):
-// words wholly inside the wave's range are stored, boundary words are merged
-// with disjoint-mask atomics (neighbouring waves/blocks own the other bits).
-template <int R>
-__device__ __forceinline__ void write_valid_words(uint32_t* valid, uint32_t (&s_words)[2 * R + 2], int lane,
-                                                  int64_t wpos, int64_t wcnt, int64_t p0, uint32_t bits,
-                                                  int64_t capacity) {
-  constexpr int WORDS = 2 * R + 2;
-  const int64_t w0 = wpos >> 5;
-  const int nwords = (int)(((wpos + wcnt + 31) >> 5) - w0);
-  if (lane < WORDS) s_words[lane] = 0;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    if ((bits >> k) & 1u) {
-      const int64_t rel = p0 + k - (w0 << 5);
-      atomicOr(&s_words[rel >> 5], 1u << (rel & 31));
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  if (lane < nwords) {
-    const int64_t wi = w0 + lane;
-    const int64_t lo = wpos > (wi << 5) ? wpos : (wi << 5);
-    int64_t hi = wpos + wcnt < ((wi + 1) << 5) ? wpos + wcnt : ((wi + 1) << 5);
-    if (hi > capacity) hi = capacity;
-    if (hi > lo) {
-      const int a = (int)(lo - (wi << 5)), b = (int)(hi - (wi << 5));
-      const uint32_t mask = (b >= 32 ? 0xffffffffu : ((1u << b) - 1u)) & ~((1u << a) - 1u);
-      const uint32_t word = s_words[lane] & mask;
-      if (mask == 0xffffffffu) {
-        valid[wi] = word;
-      } else {
-        atomicAnd(&valid[wi], ~mask | word);
-        atomicOr(&valid[wi], word);
-      }
-    }
-  }
-  __builtin_amdgcn_wave_barrier();
-}
-
-// Range conjuncts (plan.ranges) over contiguous rows: alive &= in range.
-template <int R>
-__device__ __forceinline__ void range_test(const DevPlan& plan, const DevRange& rg, const uint64_t (&v)[R],
-                                           uint32_t vm, uint32_t& alive) {
-  const int ct = plan.in[rg.col].type;
-  const bool col_f = ct == BSQL_FLOAT32 || ct == BSQL_FLOAT64;
-  const uint8_t fl = rg.flags;
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    bool ok;
-    if (fl & R_F) {
-      const double x = col_f ? as_f(v[k]) : (double)(int64_t)v[k];
-      const double lo = as_f((uint64_t)rg.lo), hi = as_f((uint64_t)rg.hi);
-      if (fl & R_NE) {
-        ok = x != lo;
-      } else {
-        ok = true;
-        if (fl & R_HAS_LO) ok = ok && ((fl & R_LO_STRICT) ? x > lo : x >= lo);
-        if (fl & R_HAS_HI) ok = ok && ((fl & R_HI_STRICT) ? x < hi : x <= hi);
-      }
-    } else {
-      const int64_t x = (int64_t)v[k];
-      if (fl & R_NE) {
-        ok = x != rg.lo;
-      } else {
-        ok = true;
-        if (fl & R_HAS_LO) ok = ok && ((fl & R_LO_STRICT) ? x > rg.lo : x >= rg.lo);
-        if (fl & R_HAS_HI) ok = ok && ((fl & R_HI_STRICT) ? x < rg.hi : x <= rg.hi);
-      }
-    }
-    if (!(((vm >> k) & 1u) && ok)) alive &= ~(1u << k);
-  }
-}
-
 // -------------------------------------------------------------- the kernels
 
-// perform_interpreter_operation: every op over this lane's R contiguous rows,
-// outputs at the input row positions.
-template <int NREG, int R>
-__global__ __launch_bounds__(BLOCK) void vm_execute(const DevPlan* __restrict__ pplan) {
+// Stores the R results of one STORE op for rows at output positions
+// [pos0, pos0 + n) and their validity (`vbits`, bit k = row k). pos0 is a
+// multiple of R and a tile starts at a multiple of 32, so the 32/R lanes of
+// an aligned lane group own one whole validity word: it is assembled with
+// shuffles and written with one plain store.
+template <int R>
+__device__ __forceinline__ void store_out(const DevOutCol& c, int64_t pos0, int n, const uint64_t (&A)[R],
+                                          uint32_t vbits, int lane) {
+  if (n > 0) store_rows<R>(c, pos0, n, A);
+  if (c.valid) {
+    constexpr int LPW = 32 / R;
+    uint32_t word = (vbits & ((1u << R) - 1u)) << ((lane % LPW) * R);
+#pragma unroll
+    for (int s = 1; s < LPW; s <<= 1) word |= __shfl_xor(word, s, 64);
+    if ((lane % LPW) == 0 && n > 0) gpw<uint32_t>(c.valid)[pos0 >> 5] = word;
+  }
+}
+
+// Ops [plan.split, n_ops) over rows: perform_interpreter_operation (rows are
+// the positions [0, num_rows)) or the output phase of a filter (rows are the
+// selected row ids idx[0, count), outputs at the positions in that list).
+// Persistent: workgroups stride over tiles of BLOCK*R positions.
+template <int NREG, int R, bool GATHER>
+__global__ __launch_bounds__(BLOCK) void vm_rows(const DevPlan* __restrict__ pplan,
+                                                 const uint32_t* __restrict__ idx,
+                                                 const int64_t* __restrict__ d_count, int64_t capacity) {
   typedef typename RegVecT<NREG>::type RegVec;
+  constexpr int TILE = BLOCK * R;
   const DevPlan& plan = *pplan;  // device memory: ops are scalar-cache loads
   const int lane = threadIdx.x & 63;
-  Rows<R> rs;
-  rs.row0 = (int64_t)blockIdx.x * (BLOCK * R) + (int64_t)threadIdx.x * R;
-  const int64_t left = plan.num_rows - rs.row0;
-  rs.nrows = left <= 0 ? 0 : (left >= R ? R : (int)left);
-  rs.exist = (1u << rs.nrows) - 1u;
-  rs.contig = true;
-  rs.pos0 = rs.row0;
-#pragma unroll
-  for (int k = 0; k < R; ++k) rs.row[k] = rs.row0 + k;
-  uint32_t alive = rs.exist;
+  int64_t count = plan.num_rows;
+  if (GATHER) {
+    count = *d_count;
+    if (count > capacity) count = capacity;
+  }
   RegVec f[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
   uint64_t vb = 0;
   LoadTemps<R> T;
-  for (int q = 0; q < plan.n_ops; ++q) {
-    const DevOp op = plan.ops[q];
-    if (op.code != D_STORE) {
-      vm_step<NREG, R>(plan, op, f, vb, T, rs, alive);
-      continue;
+  for (int64_t tile = blockIdx.x; tile * TILE < count; tile += gridDim.x) {
+    Rows<R> rs;
+    rs.pos0 = tile * TILE + (int64_t)threadIdx.x * R;
+    const int64_t left = count - rs.pos0;
+    const int n = left <= 0 ? 0 : (left >= R ? R : (int)left);
+    rs.nrows = n;
+    rs.exist = (1u << n) - 1u;
+    if (GATHER) {
+      rs.contig = false;
+      rs.row0 = 0;
+#pragma unroll
+      for (int k = 0; k < R; ++k) rs.row[k] = k < n ? (int64_t)gp<uint32_t>(idx)[rs.pos0 + k] : 0;
+    } else {
+      rs.contig = true;
+      rs.row0 = rs.pos0;
+#pragma unroll
+      for (int k = 0; k < R; ++k) rs.row[k] = rs.pos0 + k;
     }
-    const DevOutCol& c = plan.out[op.aux];
-    const int ia = op.a & (NREG - 1);
-    uint64_t A[R];
+    uint32_t alive = rs.exist;
+    for (int q = plan.split; q < plan.n_ops; ++q) {
+      const DevOp op = load_op(plan, q);
+      if (op.code != D_STORE) {
+        vm_step<NREG, R>(plan, op, f, vb, T, rs, alive);
+        continue;
+      }
+      const int ia = op.a & (NREG - 1);
+      uint64_t A[R];
 #pragma unroll
-    for (int k = 0; k < R; ++k) A[k] = f[k][ia];
-    const uint32_t VA = (uint32_t)(vb >> (ia * R)) & ((1u << R) - 1u);
-    if (rs.nrows > 0) store_rows<R>(c, rs.row0, rs.nrows, A);
-    if (c.valid) {
-      // 32/R lanes cover one 32-row word: OR their bits together
-      constexpr int LPW = 32 / R;
-      uint32_t word = (VA & alive) << ((lane % LPW) * R);
-#pragma unroll
-      for (int s = 1; s < LPW; s <<= 1) word |= __shfl_xor(word, s, 64);
-      if ((lane % LPW) == 0 && rs.nrows > 0) c.valid[rs.row0 >> 5] = word;
+      for (int k = 0; k < R; ++k) A[k] = f[k][ia];
+      const uint32_t VA = (uint32_t)(vb >> (ia * R));
+      store_out<R>(plan.out[op.aux], rs.pos0, n, A, VA & rs.exist, lane);
     }
   }
 }
 
-// filter / filter+project: rows selected by the range conjuncts or by ops
-// [0, split) (wave-uniform early exit after each conjunct), compacted
-// (ballot/mbcnt -> workgroup prefix -> decoupled look-back), then ops
-// [split, n_ops) run over the selected rows only, packed densely.
-template <int NREG, int R>
-__global__ __launch_bounds__(BLOCK) void vm_compact(const DevPlan* __restrict__ pplan, CompactArgs ca) {
-  typedef typename RegVecT<NREG>::type RegVec;
-  constexpr int TILE = BLOCK * R;
-  const DevPlan& plan = *pplan;
-  __shared__ uint32_t s_ticket;
-  __shared__ int64_t s_wave_total[WAVES];
-  __shared__ int64_t s_tile_prefix;
-  __shared__ uint16_t s_rows[TILE];
-  __shared__ uint32_t s_words[WAVES][2 * R + 2];
+// Selection tiles. A workgroup takes one tile from an atomic ticket (so that
+// every earlier tile is resident or done when it looks back); the tile holds
+// S chunks of 64*R rows per wave, and lane t of a wave owns rows
+// [t*R, t*R+R) of each chunk, so every load instruction of a wave reads one
+// contiguous span. Big tiles keep the number of tickets and look-backs (a
+// serialized cost per tile) small next to the memory time.
+template <int R, int S>
+struct SelectTile {
+  static constexpr int CHUNK = 64 * R;
+  static constexpr int WAVE_ROWS = S * CHUNK;
+  static constexpr int TILE = WAVES * WAVE_ROWS;
+  static_assert(R * S <= 32, "selection bits per lane");
+};
 
+template <int R>
+__device__ __forceinline__ void chunk_rows(Rows<R>& rs, int64_t row0, int64_t num_rows) {
+  rs.row0 = row0;
+  rs.pos0 = row0;
+  const int64_t left = num_rows - row0;
+  rs.nrows = left <= 0 ? 0 : (left >= R ? R : (int)left);
+  rs.exist = (1u << rs.nrows) - 1u;
+  rs.contig = true;
+#pragma unroll
+  for (int k = 0; k < R; ++k) rs.row[k] = row0 + k;
+}
+
+// Compaction of one tile: `alive` bit k*R+j = row j of this lane in chunk k,
+// `start[k]` = the lane's first dense index within its wave for chunk k and
+// `wave_total` the wave's count. Workgroup prefix in LDS -> decoupled
+// look-back -> the selected row ids in row order through LDS to
+// idx[prefix, prefix + tile_total), one coalesced store per id.
+template <int R, int S>
+__device__ __forceinline__ void compact_ids(const CompactArgs& ca, int64_t tile, int64_t tile_row0,
+                                            uint32_t alive, const int (&start)[S], int wave_total,
+                                            uint32_t* __restrict__ idx, uint16_t* s_rows,
+                                            int64_t* s_wave_total, int64_t* s_tile_prefix) {
+  typedef SelectTile<R, S> T;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  if (tid == 0) s_ticket = atomicAdd(ca.ticket, 1u);
-  __syncthreads();
-  const int64_t tile = s_ticket;
-  const int64_t tile_row0 = tile * TILE;
-
-  Rows<R> rs;
-  rs.row0 = tile_row0 + (int64_t)tid * R;
-  const int64_t left = plan.num_rows - rs.row0;
-  rs.nrows = left <= 0 ? 0 : (left >= R ? R : (int)left);
-  rs.exist = (1u << rs.nrows) - 1u;
-  rs.contig = true;
-  rs.pos0 = rs.row0;
-#pragma unroll
-  for (int k = 0; k < R; ++k) rs.row[k] = rs.row0 + k;
-  uint32_t alive = rs.exist;
-
-  RegVec f[R];
-#pragma unroll
-  for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
-  uint64_t vb = 0;
-  LoadTemps<R> T;
-
-  // ---- phase 1: row selection
-  if (plan.n_ranges > 0) {
-    const int nr = plan.n_ranges;
-    int r = 0;
-    if (plan.range_eager) {
-      // every range column in flight before the first test
-      col_load<R>(plan.in[plan.ranges[0].col], rs, T.t0, T.v0);
-      if (nr > 1) col_load<R>(plan.in[plan.ranges[1].col], rs, T.t1, T.v1);
-      if (nr > 2) col_load<R>(plan.in[plan.ranges[2].col], rs, T.t2, T.v2);
-      if (nr > 3) col_load<R>(plan.in[plan.ranges[3].col], rs, T.t3, T.v3);
-      range_test<R>(plan, plan.ranges[0], T.t0, T.v0, alive);
-      if (nr > 1) range_test<R>(plan, plan.ranges[1], T.t1, T.v1, alive);
-      if (nr > 2) range_test<R>(plan, plan.ranges[2], T.t2, T.v2, alive);
-      if (nr > 3) range_test<R>(plan, plan.ranges[3], T.t3, T.v3, alive);
-      r = nr < 4 ? nr : 4;
-    }
-    for (; r < nr; ++r) {
-      if (__ballot(alive != 0) == 0) break;
-      const DevRange rg = plan.ranges[r];
-      uint64_t v[R];
-      uint32_t vm;
-      col_load<R>(plan.in[rg.col], rs, v, vm);
-      range_test<R>(plan, rg, v, vm, alive);
-    }
-  } else {
-    for (int q = 0; q < plan.split; ++q) {
-      const DevOp op = plan.ops[q];
-      if (vm_step<NREG, R>(plan, op, f, vb, T, rs, alive)) break;
-    }
-  }
-
-  // ---- compaction
-  int lane_excl = 0, wave_total = 0;
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    const uint64_t b = __ballot((alive >> k) & 1u);
-    lane_excl += __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
-    wave_total += __popcll(b);
-  }
   if (lane == 0) s_wave_total[wave] = wave_total;
   __syncthreads();
   int64_t wave_off = 0, tile_total = 0;
@@ -863,63 +810,282 @@ __global__ __launch_bounds__(BLOCK) void vm_compact(const DevPlan* __restrict__ 
   if (wave == 0) {
     const int64_t excl = look_back(ca, tile, tile_total, lane);
     if (lane == 0) {
-      s_tile_prefix = excl;
+      *s_tile_prefix = excl;
       if (tile == ca.num_tiles - 1) *ca.d_count = excl + tile_total;
     }
   }
-  // selected rows, densely packed in row order (tile-relative offsets)
-  {
-    int d = (int)wave_off + lane_excl;
 #pragma unroll
-    for (int k = 0; k < R; ++k)
-      if ((alive >> k) & 1u) s_rows[d++] = (uint16_t)(tid * R + k);
+  for (int k = 0; k < S; ++k) {
+    int d = (int)wave_off + start[k];
+    const int base = wave * T::WAVE_ROWS + k * T::CHUNK + lane * R;
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if ((alive >> (k * R + j)) & 1u) s_rows[d++] = (uint16_t)(base + j);
   }
   __syncthreads();
-  if (tile_total == 0 || plan.split >= plan.n_ops) return;
-  const int64_t tile_prefix = s_tile_prefix;
+  const int64_t prefix = *s_tile_prefix;
+  GLOBAL uint32_t* ids = gpw<uint32_t>(idx);
+  for (int j = tid; j < (int)tile_total; j += BLOCK) ids[prefix + j] = (uint32_t)(tile_row0 + s_rows[j]);
+}
 
-  // ---- phase 2: outputs of the selected rows, R dense rows per lane
-  for (int64_t base = 0; base < tile_total; base += TILE) {
-    const int64_t j0 = base + (int64_t)tid * R;  // dense index of this lane's first row
-    Rows<R> ds;
-    ds.exist = 0;
-    ds.contig = false;
-    ds.row0 = 0;
-    ds.nrows = 0;
-    ds.pos0 = tile_prefix + j0;
+// Counts of one chunk's selection bits `a` (R bits per lane): the lane's
+// exclusive prefix within the chunk and the chunk's total over the wave.
+template <int R>
+__device__ __forceinline__ void chunk_counts(uint32_t a, int& lane_excl, int& total) {
+  lane_excl = 0;
+  total = 0;
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const bool e = j0 + k < tile_total;
-      ds.row[k] = e ? tile_row0 + s_rows[j0 + k] : 0;
-      ds.exist |= (e ? 1u : 0u) << k;
+  for (int j = 0; j < R; ++j) {
+    const uint64_t b = __ballot((a >> j) & 1u);
+    lane_excl += __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+    total += __popcll(b);
+  }
+}
+
+// Column element kinds of the range tests: the raw element type and its
+// value in the int64 slot domain (as the VM's column loads convert it).
+struct K_I8 { typedef int8_t T; static constexpr bool F = false; };
+struct K_I16 { typedef int16_t T; static constexpr bool F = false; };
+struct K_I32 { typedef int32_t T; static constexpr bool F = false; };
+struct K_I64 { typedef int64_t T; static constexpr bool F = false; };
+struct K_U8 { typedef uint8_t T; static constexpr bool F = false; };
+struct K_U16 { typedef uint16_t T; static constexpr bool F = false; };
+struct K_U32 { typedef uint32_t T; static constexpr bool F = false; };
+struct K_B8 { typedef uint8_t T; static constexpr bool F = false; };
+struct K_F32 { typedef float T; static constexpr bool F = true; };
+struct K_F64 { typedef double T; static constexpr bool F = true; };
+
+template <class K>
+__device__ __forceinline__ int64_t kind_i64(typename K::T x) {
+  if constexpr (std::is_same<K, K_B8>::value) return x != 0;
+  else if constexpr (K::F) return 0;  // float columns are only compared as doubles
+  else return (int64_t)x;
+}
+template <class K>
+__device__ __forceinline__ double kind_f64(typename K::T x) {
+  if constexpr (K::F) return (double)x;
+  else return (double)kind_i64<K>(x);
+}
+
+// Raw elements of R consecutive rows starting at r0 (n of them exist), with
+// the widest aligned loads the element size allows.
+template <class CT, int R>
+__device__ __forceinline__ void load_raw(const DevInCol& c, int64_t r0, int n, CT (&v)[R]) {
+  const GLOBAL CT* p = gp<CT>(c.data) + r0;
+  constexpr int BYTES = (int)sizeof(CT) * R;
+  if (n == R && c.aligned16 && BYTES >= 4 && BYTES % 4 == 0) {
+    if constexpr (BYTES >= 16) {
+#pragma unroll
+      for (int k = 0; k < BYTES / 16; ++k)
+        ((v4u32*)v)[k] = __builtin_nontemporal_load((const GLOBAL v4u32*)p + k);
+    } else if constexpr (BYTES == 8) {
+      *(v2u32*)v = __builtin_nontemporal_load((const GLOBAL v2u32*)p);
+    } else {
+      *(uint32_t*)v = __builtin_nontemporal_load((const GLOBAL uint32_t*)p);
     }
-    const int64_t wave_first = base + (int64_t)wave * 64 * R;
-    int64_t wcnt = tile_total - wave_first;
-    wcnt = wcnt < 0 ? 0 : (wcnt > 64 * R ? 64 * R : wcnt);
-    if (wcnt == 0) break;  // this wave and the following ones have no rows left
-    uint32_t alive2 = ds.exist;
-    for (int q = plan.split; q < plan.n_ops; ++q) {
-      const DevOp op = plan.ops[q];
-      if (op.code != D_STORE) {
-        vm_step<NREG, R>(plan, op, f, vb, T, ds, alive2);
-        continue;
-      }
-      const DevOutCol& c = plan.out[op.aux];
-      const int ia = op.a & (NREG - 1);
-      uint64_t A[R];
+  } else {
 #pragma unroll
-      for (int k = 0; k < R; ++k) A[k] = f[k][ia];
-      const uint32_t VA = (uint32_t)(vb >> (ia * R)) & ((1u << R) - 1u);
-      const int64_t p0 = tile_prefix + j0;
-      int n = (int)(tile_total - j0);
-      n = n < 0 ? 0 : (n > R ? R : n);
-      if (p0 + n > ca.capacity) n = ca.capacity > p0 ? (int)(ca.capacity - p0) : 0;
-      if (n > 0) store_rows<R>(c, p0, n, A);
-      if (c.valid)
-        write_valid_words<R>(c.valid, s_words[wave], lane, tile_prefix + wave_first, wcnt, p0,
-                             VA & ds.exist, ca.capacity);
+    for (int k = 0; k < R; ++k) v[k] = k < n ? p[k] : CT(0);
+  }
+}
+
+// In-range bits of the R elements v (bit j) for range rg.
+template <class K, int R>
+__device__ __forceinline__ uint32_t range_bits(const DevRange& rg, const typename K::T (&v)[R]) {
+  uint32_t pass = 0;
+  if (rg.kind & RK_F) {
+    const double lo = as_f((uint64_t)rg.lo), hi = as_f((uint64_t)rg.hi);
+    if (rg.kind & RK_NE) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) pass |= (uint32_t)(kind_f64<K>(v[j]) != lo) << j;
+    } else {
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const double x = kind_f64<K>(v[j]);
+        pass |= (uint32_t)(x >= lo && x <= hi) << j;
+      }
+    }
+  } else if constexpr (!K::F) {
+    const uint64_t mul = (uint64_t)rg.mul;
+    if (rg.kind & RK_NE) {
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        pass |= (uint32_t)((int64_t)((uint64_t)kind_i64<K>(v[j]) * mul) != rg.lo) << j;
+    } else if (mul == 1) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int64_t x = kind_i64<K>(v[j]);
+        pass |= (uint32_t)(x >= rg.lo && x <= rg.hi) << j;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int64_t x = (int64_t)((uint64_t)kind_i64<K>(v[j]) * mul);
+        pass |= (uint32_t)(x >= rg.lo && x <= rg.hi) << j;
+      }
     }
   }
+  return pass;
+}
+
+// One range test over this lane's S chunks: returns alive & valid & in range
+// (bit k*R + j = row j of chunk k). A full tile issues every chunk's loads
+// before the first test; the last, partial tile goes chunk by chunk with
+// guarded loads.
+template <class K, int R, int S, bool FULL>
+__device__ __forceinline__ uint32_t range_chunks(const DevInCol& c, const DevRange& rg, int64_t row0,
+                                                 int64_t num_rows, uint32_t alive) {
+  typedef typename K::T CT;
+  uint32_t pass = 0;
+  if constexpr (FULL) {
+    CT v[S][R];
+    uint32_t vm = 0;
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const int64_t r0 = row0 + (int64_t)k * 64 * R;
+      load_raw<CT, R>(c, r0, R, v[k]);
+      vm |= load_valid<R>(c, r0, R) << (k * R);
+    }
+#pragma unroll
+    for (int k = 0; k < S; ++k) pass |= range_bits<K, R>(rg, v[k]) << (k * R);
+    return alive & vm & pass;
+  } else {
+    uint32_t vm = 0;
+#pragma unroll 1
+    for (int k = 0; k < S; ++k) {
+      const int64_t r0 = row0 + (int64_t)k * 64 * R;
+      const int64_t left = num_rows - r0;
+      const int n = left <= 0 ? 0 : (left >= R ? R : (int)left);
+      CT v[R];
+      load_raw<CT, R>(c, r0, n, v);
+      vm |= load_valid<R>(c, r0, n) << (k * R);
+      pass |= range_bits<K, R>(rg, v) << (k * R);
+    }
+    return alive & vm & pass;
+  }
+}
+
+template <int R, int S, bool FULL>
+__device__ __forceinline__ uint32_t range_test(const DevInCol& c, const DevRange& rg, int64_t row0,
+                                               int64_t num_rows, uint32_t alive) {
+  switch (c.type) {
+    case BSQL_INT8: return range_chunks<K_I8, R, S, FULL>(c, rg, row0, num_rows, alive);
+    case BSQL_INT16: return range_chunks<K_I16, R, S, FULL>(c, rg, row0, num_rows, alive);
+    case BSQL_INT32: case BSQL_TIMESTAMP_DAYS: case BSQL_DURATION_DAYS:
+      return range_chunks<K_I32, R, S, FULL>(c, rg, row0, num_rows, alive);
+    case BSQL_UINT8: return range_chunks<K_U8, R, S, FULL>(c, rg, row0, num_rows, alive);
+    case BSQL_UINT16: return range_chunks<K_U16, R, S, FULL>(c, rg, row0, num_rows, alive);
+    case BSQL_UINT32: return range_chunks<K_U32, R, S, FULL>(c, rg, row0, num_rows, alive);
+    case BSQL_BOOL8: return range_chunks<K_B8, R, S, FULL>(c, rg, row0, num_rows, alive);
+    case BSQL_FLOAT32: return range_chunks<K_F32, R, S, FULL>(c, rg, row0, num_rows, alive);
+    case BSQL_FLOAT64: return range_chunks<K_F64, R, S, FULL>(c, rg, row0, num_rows, alive);
+    default: return range_chunks<K_I64, R, S, FULL>(c, rg, row0, num_rows, alive);  // 8-byte ticks/ints
+  }
+}
+
+// Row selection by range tests (plan.ranges): one range at a time over the
+// wave's whole share of the tile, so each column is loaded with every chunk
+// in flight and its type is dispatched once per wave; no register file.
+template <int R, int S>
+__global__ __launch_bounds__(BLOCK) void select_ranges(const DevPlan* __restrict__ pplan, CompactArgs ca,
+                                                       uint32_t* __restrict__ idx) {
+  typedef SelectTile<R, S> T;
+  const DevPlan& plan = *pplan;
+  __shared__ uint32_t s_ticket;
+  __shared__ int64_t s_wave_total[WAVES];
+  __shared__ int64_t s_tile_prefix;
+  __shared__ uint16_t s_rows[T::TILE];
+  if (threadIdx.x == 0) s_ticket = atomicAdd(ca.ticket, 1u);
+  __syncthreads();
+  const int64_t tile = s_ticket;
+  const int64_t tile_row0 = tile * T::TILE;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t num_rows = plan.num_rows;
+  const int64_t row0 = tile_row0 + wave * T::WAVE_ROWS + lane * R;
+  uint32_t alive = 0;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const int64_t left = num_rows - (row0 + (int64_t)k * T::CHUNK);
+    const int n = left <= 0 ? 0 : (left >= R ? R : (int)left);
+    alive |= ((1u << n) - 1u) << (k * R);
+  }
+  const bool full = tile_row0 + T::TILE <= num_rows;
+  const int nr = plan.n_ranges;
+  for (int r = 0; r < nr; ++r) {
+    if (__ballot(alive != 0) == 0) break;
+    const DevRange rg = plan.ranges[r];
+    if (rg.kind & RK_EMPTY) {
+      alive = 0;
+      break;
+    }
+    if (full)
+      alive = range_test<R, S, true>(plan.in[rg.col], rg, row0, num_rows, alive);
+    else
+      alive = range_test<R, S, false>(plan.in[rg.col], rg, row0, num_rows, alive);
+  }
+  int start[S];
+  int wave_total = 0;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    int excl, tot;
+    chunk_counts<R>((alive >> (k * R)) & ((1u << R) - 1u), excl, tot);
+    start[k] = wave_total + excl;
+    wave_total += tot;
+  }
+  compact_ids<R, S>(ca, tile, tile_row0, alive, start, wave_total, idx, s_rows, s_wave_total,
+                    &s_tile_prefix);
+}
+
+// Row selection by ops [0, split) (KILL per conjunct with a wave-uniform
+// early exit, or one SEL), chunk by chunk.
+template <int NREG, int R, int S>
+__global__ __launch_bounds__(BLOCK) void select_vm(const DevPlan* __restrict__ pplan, CompactArgs ca,
+                                                   uint32_t* __restrict__ idx) {
+  typedef typename RegVecT<NREG>::type RegVec;
+  typedef SelectTile<R, S> T;
+  const DevPlan& plan = *pplan;
+  __shared__ uint32_t s_ticket;
+  __shared__ int64_t s_wave_total[WAVES];
+  __shared__ int64_t s_tile_prefix;
+  __shared__ uint16_t s_rows[T::TILE];
+  if (threadIdx.x == 0) s_ticket = atomicAdd(ca.ticket, 1u);
+  __syncthreads();
+  const int64_t tile = s_ticket;
+  const int64_t tile_row0 = tile * T::TILE;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  RegVec f[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
+  uint64_t vb = 0;
+  LoadTemps<R> L;
+  uint32_t alive_all = 0;
+  int start[S];
+  int wave_total = 0;
+#pragma unroll 1
+  for (int k = 0; k < S; ++k) {
+    Rows<R> rs;
+    chunk_rows<R>(rs, tile_row0 + wave * T::WAVE_ROWS + k * T::CHUNK + lane * R, plan.num_rows);
+    uint32_t alive = rs.exist;
+    if (__ballot(alive != 0) != 0) {
+      for (int q = 0; q < plan.split; ++q) {
+        const DevOp op = load_op(plan, q);
+        if (vm_step<NREG, R>(plan, op, f, vb, L, rs, alive)) break;
+      }
+    }
+    int excl, tot;
+    chunk_counts<R>(alive, excl, tot);
+    // start[] is indexed by the loop counter: keep it in registers by
+    // selecting instead of indexing
+#pragma unroll
+    for (int c = 0; c < S; ++c)
+      if (c == k) start[c] = wave_total + excl;
+    wave_total += tot;
+    alive_all |= alive << (k * R);
+  }
+  compact_ids<R, S>(ca, tile, tile_row0, alive_all, start, wave_total, idx, s_rows, s_wave_total,
+                    &s_tile_prefix);
 }
 
 __global__ void fill_u64(uint64_t* p, int64_t n, uint64_t v) {
@@ -935,21 +1101,55 @@ namespace bsql {
 
 using namespace bsql_dev;
 
+static int g_num_cus = 0;
+
+// Workgroups of a persistent launch: enough to fill every CU several times.
+static int64_t persistent_grid(int64_t tiles) {
+  if (g_num_cus == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      g_num_cus = n;
+    else
+      g_num_cus = 256;
+  }
+  const int64_t cap = (int64_t)g_num_cus * 8;
+  return tiles < cap ? tiles : cap;
+}
+
 template <int NREG, int R>
 static hipError_t launch_execute_t(const DevPlan* d_plan, int64_t num_rows, hipStream_t stream) {
   const int64_t tile = (int64_t)BLOCK * R;
-  const int64_t grid = (num_rows + tile - 1) / tile;
+  const int64_t grid = persistent_grid((num_rows + tile - 1) / tile);
   if (grid == 0) return hipSuccess;
-  CompactArgs ca{};
-  (void)ca;
-  hipLaunchKernelGGL((vm_execute<NREG, R>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan);
+  hipLaunchKernelGGL((vm_rows<NREG, R, false>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan,
+                     (const uint32_t*)nullptr, (const int64_t*)nullptr, num_rows);
+  return hipGetLastError();
+}
+
+// chunks per wave of the selection tiles: 4096-row tiles for the VM,
+// 8192-row tiles for range tests
+template <int R> struct VmChunks { static constexpr int S = 16 / R; };
+// range selection: 4 rows per chunk, 8 chunks per wave (32 rows per lane,
+// 8192-row tiles) by default, or 4 chunks (sel_rpt == 16)
+constexpr int RANGE_R = 4;
+
+template <int NREG, int R>
+static hipError_t launch_select_vm_t(const DevPlan* d_plan, const CompactArgs& ca, uint32_t* idx,
+                                     hipStream_t stream) {
+  hipLaunchKernelGGL((select_vm<NREG, R, VmChunks<R>::S>), dim3((unsigned)ca.num_tiles), dim3(BLOCK), 0,
+                     stream, d_plan, ca, idx);
   return hipGetLastError();
 }
 
 template <int NREG, int R>
-static hipError_t launch_compact_t(const DevPlan* d_plan, const CompactArgs& ca, hipStream_t stream) {
-  hipLaunchKernelGGL((vm_compact<NREG, R>), dim3((unsigned)ca.num_tiles), dim3(BLOCK), 0, stream,
-                     d_plan, ca);
+static hipError_t launch_gather_t(const DevPlan* d_plan, int64_t num_rows, const uint32_t* idx,
+                                  const int64_t* d_count, int64_t capacity, hipStream_t stream) {
+  const int64_t tile = (int64_t)BLOCK * R;
+  const int64_t grid = persistent_grid((num_rows + tile - 1) / tile);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((vm_rows<NREG, R, true>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan, idx,
+                     d_count, capacity);
   return hipGetLastError();
 }
 
@@ -983,19 +1183,29 @@ hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
   return hipErrorInvalidValue;
 }
 
-int64_t compact_tiles(int64_t num_rows, int nreg, int rpt) {
+// Rows per tile of the selection kernel the plan uses.
+static int64_t select_tile_rows(const DevPlan& plan, int nreg, int rpt) {
+  if (plan.n_ranges > 0) return rpt == 16 ? SelectTile<RANGE_R, 4>::TILE : SelectTile<RANGE_R, 8>::TILE;
   int NR, RR;
   if (!pick(nreg, rpt, &NR, &RR)) return -1;
-  const int64_t tile = (int64_t)BLOCK * RR;
-  return (num_rows + tile - 1) / tile;
+  return (int64_t)BLOCK * 16;  // R * VmChunks<R>::S == 16 rows per lane
 }
 
-// `status` must hold compact_tiles(...) + 1 words (the last one is the ticket).
-hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt,
-                          uint64_t* status, int64_t* d_count, int64_t capacity, hipStream_t stream) {
-  int NR, RR;
-  if (!pick(nreg, rpt, &NR, &RR)) return hipErrorInvalidValue;
-  const int64_t tile = (int64_t)BLOCK * RR;
+int64_t compact_tiles(const DevPlan& plan, int nreg, int rpt) {
+  const int64_t tile = select_tile_rows(plan, nreg, rpt);
+  if (tile < 0) return -1;
+  return (plan.num_rows + tile - 1) / tile;
+}
+
+// Filter (+ project): selection kernel -> ids of the selected rows in `idx`
+// (num_rows entries) and their count in *d_count, then the output ops over
+// those rows. `status` holds compact_tiles(...) + 1 words (the last one is
+// the ticket).
+hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int sel_rpt, int rpt,
+                          uint64_t* status, uint32_t* idx, int64_t* d_count, int64_t capacity,
+                          hipStream_t stream) {
+  const int64_t tile = select_tile_rows(plan, nreg, sel_rpt);
+  if (tile < 0) return hipErrorInvalidValue;
   const int64_t tiles = (plan.num_rows + tile - 1) / tile;
   CompactArgs ca{status, (uint32_t*)(status + tiles), d_count, capacity, tiles};
   // clear tile status, ticket and the count (fill kernels, so the sequence can
@@ -1005,7 +1215,27 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
                      status, nclear, 0ULL);
   hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, stream, (uint64_t*)d_count, (int64_t)1, 0ULL);
   if (tiles == 0) return hipGetLastError();
-#define X(N, RPT) if (NR == N && RR == RPT) return launch_compact_t<N, RPT>(d_plan, ca, stream);
+  hipError_t e = hipErrorInvalidValue;
+  if (plan.n_ranges > 0) {
+    if (sel_rpt == 16)
+      hipLaunchKernelGGL((select_ranges<RANGE_R, 4>), dim3((unsigned)tiles), dim3(BLOCK), 0, stream, d_plan,
+                         ca, idx);
+    else
+      hipLaunchKernelGGL((select_ranges<RANGE_R, 8>), dim3((unsigned)tiles), dim3(BLOCK), 0, stream, d_plan,
+                         ca, idx);
+    e = hipGetLastError();
+  } else {
+    int NR, RR;
+    if (!pick(nreg, sel_rpt, &NR, &RR)) return hipErrorInvalidValue;
+#define X(N, RPT) if (NR == N && RR == RPT) e = launch_select_vm_t<N, RPT>(d_plan, ca, idx, stream);
+    BSQL_VARIANTS(X)
+#undef X
+  }
+  if (e != hipSuccess || plan.split >= plan.n_ops) return e;
+  int NR, RR;
+  if (!pick(nreg, rpt, &NR, &RR)) return hipErrorInvalidValue;
+#define X(N, RPT) \
+  if (NR == N && RR == RPT) return launch_gather_t<N, RPT>(d_plan, plan.num_rows, idx, d_count, capacity, stream);
   BSQL_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;

@@ -3,6 +3,8 @@
 #include "program.hpp"
 
 #include <algorithm>
+#include <cmath>
+#include <limits>
 #include <cstring>
 #include <map>
 #include <set>
@@ -394,6 +396,56 @@ std::vector<int32_t> slot_types_for(const Program& p, const int32_t* table_types
 
 namespace {
 
+bool is_float_code(uint8_t c) {
+  switch (c) {
+    case D_ADD_FF: case D_SUB_FF: case D_MUL_FF: case D_DIV_FF: case D_MOD_FF: case D_POW: case D_ROUND:
+    case D_EQ_FF: case D_NE_FF: case D_LT_FF: case D_GT_FF: case D_LE_FF: case D_GE_FF: case D_INDF_FF:
+    case D_FLOOR: case D_CEIL: case D_SIN: case D_COS: case D_ASIN: case D_ACOS: case D_TAN:
+    case D_COTAN: case D_ATAN: case D_LN: case D_LOG:
+      return true;
+    default:
+      return false;
+  }
+}
+
+bool is_unary_code(uint8_t c) { return c >= D_NOT; }
+
+// Float ops read both operands as doubles on the device: int literals are
+// converted here and int values through an explicit D_TO_F64 op (once per
+// value), the conversion the reference applies when it reads an integer slot
+// into a double operand (interpreter_ops.cuh:600-727).
+void float_operands(Lowering& L, SsaProgram& s) {
+  std::vector<SourceOp> out;
+  std::map<int, int> conv;
+  for (SourceOp so : s.ops) {
+    if (is_float_code(so.d.code)) {
+      auto fix = [&](int& v, uint8_t fflag, uint8_t immflag, uint8_t nullflag) {
+        DevOp& d = so.d;
+        if (d.flags & fflag) return;
+        if (d.flags & immflag) { d.imm = f64_bits((double)d.imm); d.flags |= fflag; return; }
+        if (d.flags & nullflag) { d.flags |= fflag; return; }
+        if (v < 0) return;
+        auto it = conv.find(v);
+        if (it == conv.end()) {
+          SourceOp c;
+          c.prog_op = so.prog_op;
+          c.d.code = D_TO_F64;
+          c.va = v;
+          c.vdst = L.new_value();
+          out.push_back(c);
+          it = conv.emplace(v, c.vdst).first;
+        }
+        v = it->second;
+        d.flags |= fflag;
+      };
+      fix(so.va, F_A_F, F_A_IMM, F_A_NULL);
+      if (!is_unary_code(so.d.code)) fix(so.vb, F_B_F, F_B_IMM, F_B_NULL);
+    }
+    out.push_back(so);
+  }
+  s.ops = out;
+}
+
 SsaProgram to_ssa(Lowering& L, const Program& p, uint64_t seed) {
   const std::vector<int> col_of = slot_columns(p, L.n_in);
   const int n_slots_in = (int)col_of.size();
@@ -413,6 +465,7 @@ SsaProgram to_ssa(Lowering& L, const Program& p, uint64_t seed) {
     slot_val[p.out[i]] = so.vdst;
     s.ops.push_back(so);
   }
+  float_operands(L, s);
   for (int16_t f : p.final_out) {
     int v = f < n_slots_in && slot_val[f] < 0 ? L.load_of(col_of[(size_t)f]) : slot_val[f];
     if (v < 0) fail("output slot " + std::to_string(f) + " is never written");
@@ -468,7 +521,12 @@ bool as_range(const Lowering& L, const SsaProgram& s, const std::vector<int>& pr
   if (d.flags & (F_A_NULL | F_B_NULL)) return false;
   const bool lit_left = (d.flags & F_A_IMM) != 0, lit_right = (d.flags & F_B_IMM) != 0;
   if (lit_left == lit_right) return false;
-  const int cv = lit_left ? s.ops[(size_t)o].vb : s.ops[(size_t)o].va;
+  int cv = lit_left ? s.ops[(size_t)o].vb : s.ops[(size_t)o].va;
+  if (cv >= 0 && !L.values[(size_t)cv].is_load && prod[(size_t)cv] >= 0) {
+    // an int column converted for a float comparison: the range test converts
+    const SourceOp& c = s.ops[(size_t)prod[(size_t)cv]];
+    if (c.d.code == D_TO_F64 && !(c.d.flags & (F_A_F | F_A_IMM | F_A_NULL))) cv = c.va;
+  }
   if (cv < 0 || !L.values[(size_t)cv].is_load) return false;
   const int col = L.values[(size_t)cv].column;
   if (is_string_type(L.in_types[col])) return false;
@@ -477,25 +535,68 @@ bool as_range(const Lowering& L, const SsaProgram& s, const std::vector<int>& pr
     static const int flip[6] = {0, 1, 3, 2, 5, 4};
     k = flip[k];
   }
-  int64_t lit = d.imm;
-  uint8_t fl = 0;
-  if (base == D_EQ_FF) {
-    fl |= R_F;
-    const bool lit_f = (d.flags & (lit_left ? F_A_F : F_B_F)) != 0;
-    if (!lit_f) lit = f64_bits((double)d.imm);
-  }
   rg = DevRange{};
-  rg.col = (uint8_t)col;
-  switch (k) {
-    case 0: fl |= R_HAS_LO | R_HAS_HI; rg.lo = rg.hi = lit; break;
-    case 1: fl |= R_NE; rg.lo = lit; break;
-    case 2: fl |= R_HAS_HI | R_HI_STRICT; rg.hi = lit; break;
-    case 3: fl |= R_HAS_LO | R_LO_STRICT; rg.lo = lit; break;
-    case 4: fl |= R_HAS_HI; rg.hi = lit; break;
-    default: fl |= R_HAS_LO; rg.lo = lit; break;
+  rg.col = col;
+  rg.mul = 1;
+  if (base == D_EQ_FF) {
+    // double domain; the literal is already double bits (float_operands)
+    rg.kind = RK_F;
+    double lit;
+    std::memcpy(&lit, &d.imm, 8);
+    const double inf = std::numeric_limits<double>::infinity();
+    double lo = -inf, hi = inf;
+    bool empty = std::isnan(lit);  // only NE is true against NaN
+    switch (k) {
+      case 0: lo = hi = lit; break;
+      case 1: rg.kind |= RK_NE; rg.lo = f64_bits(lit); return true;
+      case 2: if (lit == -inf) empty = true; else hi = std::nextafter(lit, -inf); break;  // x < lit
+      case 3: if (lit == inf) empty = true; else lo = std::nextafter(lit, inf); break;    // x > lit
+      case 4: hi = lit; break;
+      default: lo = lit; break;
+    }
+    if (empty) rg.kind |= RK_EMPTY;
+    rg.lo = f64_bits(lo);
+    rg.hi = f64_bits(hi);
+    return true;
   }
-  rg.flags = fl;
+  // int64 domain; timestamps compare as nanoseconds (wrapping, like D_*_TS)
+  if (base == D_EQ_TS) {
+    static const int64_t ns[5] = {86400000000000LL, 1000000000LL, 1000000LL, 1000LL, 1LL};
+    rg.mul = ns[d.aux2 & 15];
+  }
+  const int64_t lit = (int64_t)((uint64_t)d.imm * (uint64_t)rg.mul);
+  int64_t lo = INT64_MIN, hi = INT64_MAX;
+  bool empty = false;
+  switch (k) {
+    case 0: lo = hi = lit; break;
+    case 1: rg.kind = RK_NE; rg.lo = lit; return true;
+    case 2: if (lit == INT64_MIN) empty = true; else hi = lit - 1; break;
+    case 3: if (lit == INT64_MAX) empty = true; else lo = lit + 1; break;
+    case 4: hi = lit; break;
+    default: lo = lit; break;
+  }
+  if (empty) rg.kind |= RK_EMPTY;
+  rg.lo = lo;
+  rg.hi = hi;
   return true;
+}
+
+// Intersects range b into a (same column and domain, neither NE).
+void intersect(DevRange& a, const DevRange& b) {
+  if (b.kind & RK_EMPTY) a.kind |= RK_EMPTY;
+  if (a.kind & RK_F) {
+    double alo, ahi, blo, bhi;
+    std::memcpy(&alo, &a.lo, 8); std::memcpy(&ahi, &a.hi, 8);
+    std::memcpy(&blo, &b.lo, 8); std::memcpy(&bhi, &b.hi, 8);
+    if (blo > alo) a.lo = b.lo;
+    if (bhi < ahi) a.hi = b.hi;
+    std::memcpy(&alo, &a.lo, 8); std::memcpy(&ahi, &a.hi, 8);
+    if (alo > ahi) a.kind |= RK_EMPTY;
+  } else {
+    a.lo = std::max(a.lo, b.lo);
+    a.hi = std::min(a.hi, b.hi);
+    if (a.lo > a.hi) a.kind |= RK_EMPTY;
+  }
 }
 
 LOp to_lop(const SourceOp& so) {
@@ -555,33 +656,26 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
     std::vector<int32_t> order(conj.size());
     for (size_t j = 0; j < conj.size(); ++j) order[j] = (int32_t)j;
     if (opt.conj_order && opt.conj_order->size() == conj.size()) order = *opt.conj_order;
-    // every conjunct a column-vs-literal comparison: range tests, no ops
+    // every conjunct a column-vs-literal comparison: range tests, no ops.
+    // Bounds on one column (same domain) merge into one test; the selected
+    // rows do not depend on the order in which conjuncts are applied.
     std::vector<DevRange> ranges;
     if (opt.ranges && !conj.empty()) {
       for (int32_t j : order) {
         DevRange rg;
         if (!as_range(L, s, prod, conj[(size_t)j], rg)) { ranges.clear(); break; }
-        // a lower and an upper bound on the same column in a row make one test
-        if (!ranges.empty()) {
-          DevRange& pr = ranges.back();
-          const uint8_t kind = R_F | R_NE;
-          const bool lo_only = (pr.flags & (R_HAS_LO | R_HAS_HI | R_NE)) == R_HAS_LO;
-          const bool hi_only = (pr.flags & (R_HAS_LO | R_HAS_HI | R_NE)) == R_HAS_HI;
-          const uint8_t rf = rg.flags & (R_HAS_LO | R_HAS_HI | R_NE);
-          if (pr.col == rg.col && (pr.flags & R_F) == (rg.flags & R_F) && !((pr.flags | rg.flags) & R_NE)) {
-            if (lo_only && rf == R_HAS_HI) {
-              pr.flags |= rg.flags & ~kind;
-              pr.hi = rg.hi;
-              continue;
-            }
-            if (hi_only && rf == R_HAS_LO) {
-              pr.flags |= rg.flags & ~kind;
-              pr.lo = rg.lo;
-              continue;
+        bool merged = false;
+        if (!(rg.kind & RK_NE)) {
+          for (DevRange& pr : ranges) {
+            if (pr.col == rg.col && !(pr.kind & RK_NE) && (pr.kind & RK_F) == (rg.kind & RK_F) &&
+                pr.mul == rg.mul) {
+              intersect(pr, rg);
+              merged = true;
+              break;
             }
           }
         }
-        ranges.push_back(rg);
+        if (!merged) ranges.push_back(rg);
       }
       if (ranges.size() > BSQL_DEV_MAX_RANGES) ranges.clear();
     }

@@ -348,9 +348,9 @@ def test_range_predicates_vs_oracle(bsql, gpu, seed):
     for _ in range(4):
         text = _range_predicate(rng, types, table)
         on, ores = oracle.filter_table(_full_prog(bsql, text, types), host_columns(table), n)
-        for ranges in (True, False):
+        for ranges, rpt in ((True, 0), (True, 16), (False, 0)):
             pred = bsql.Program.compile(text, types)
-            pred.set_launch(ranges=ranges)
+            pred.set_launch(rows_per_thread=rpt, ranges=ranges)
             outs = [DeviceColumn.empty(t, n, nullable=v is not None) for t, _, v in table]
             cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
             filter_columns(pred, dcols, outs, cnt, n)
