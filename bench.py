@@ -85,14 +85,26 @@ def cpu_baseline(sample_rows, seed):
     threads = len(os.sched_getaffinity(0))
     oracle.filter_project(pred, proj, cols, [10, 10], n, threads=threads)  # warm-up
     times = []
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         oracle.filter_project(pred, proj, cols, [10, 10], n, threads=threads)
         times.append(time.perf_counter() - t0)
     t = sorted(times)[len(times) // 2]
     return {"value": n / t, "unit": "rows/s", "cores": threads, "kind": "port",
-            "sample": f"{n} rows of the same 100M-row scan workload (seeded), median of 3 runs, "
-                      f"filter then project as two passes like the reference operators"}
+            "sample": f"{n} rows of the same 100M-row scan workload (seeded numpy tables), median of 5 "
+                      f"runs after a warm-up, filter then project as two passes like the reference "
+                      f"operators, {threads} threads on {cpu_model()}"}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def load_traffic(path):
@@ -113,6 +125,9 @@ def main():
     ap.add_argument("--rows-per-thread", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed loop, copy a 1.6 GB device buffer 3 times (a kernel of "
+                         "known traffic, to scale the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the filter/project path has no cross-GPU step in this "
@@ -149,9 +164,18 @@ def main():
     ms = start.elapsed_time(end) / args.steps
     count = int(cnt.item())
     ok, detail = check_result(cols, outs, count)
+    if args.calibrate_copy:
+        src = torch.empty(200_000_000, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
-    # algorithmic bytes per launch: every input row once (4+8+8+8+4 B, no
-    # validity), every selected row written once (2 x (8 B + 1/8 B validity))
+    # Algorithmic bytes of one bsql_filter_project call (the unit timed: its
+    # selection and output kernels): every input row once (4+8+8+8+4 B, no
+    # validity), every selected row written once (2 x (8 B + 1/8 B validity));
+    # row-id lists and other scratch are not counted.
     algo_bytes = n * 32 + count * 2 * (8 + 1 / 8)
     achieved = algo_bytes / (ms * 1e-3) / 1e9
     traffic = load_traffic(os.path.join(ROOT, "profiles", "filter_project_100M_traffic.json"))
