@@ -230,7 +230,8 @@ typedef struct bsql_launch_config {
 } bsql_launch_config;
 /* Sets the configuration used for launches with this program (a predicate's
  * flags apply to filters it drives; 0 fields = auto). No counterpart in the
- * reference, whose launch shape is fixed (interpreter_cpp.cu:628-660). */
+ * reference, which sizes its launch with the occupancy API
+ * (interpreter_cpp.cu:76-270, :473). */
 int bsql_program_set_launch(bsql_program* prog, const bsql_launch_config* cfg);
 
 /* ---- diagnostics --------------------------------------------------------- */
