@@ -40,7 +40,7 @@ OP = {name: i for i, name in enumerate(OPS)}
 
 
 class Column(ctypes.Structure):
-    _fields_ = [("type_id", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    _fields_ = [("type_id", ctypes.c_int32), ("chars_size", ctypes.c_int32),
                 ("size", ctypes.c_int64), ("offset", ctypes.c_int64),
                 ("data", ctypes.c_void_p), ("validity", ctypes.c_void_p),
                 ("offsets", ctypes.c_void_p)]

@@ -21,6 +21,10 @@ namespace bsql {
 hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt,
                           hipStream_t stream);
 int64_t compact_tiles(const DevPlan& plan, int nreg, int rpt);
+int64_t string_tiles(int64_t capacity);
+hipError_t launch_string_output(const uint64_t* slices, const uint32_t* valid, int32_t* offsets,
+                                const uint8_t* src_chars, uint8_t* dst_chars, const int64_t* d_count,
+                                int64_t count, int64_t capacity, uint64_t* status, hipStream_t stream);
 hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int sel_rpt, int rpt,
                           uint64_t* status, uint32_t* idx, int64_t* d_count, int64_t capacity,
                           hipStream_t stream);
@@ -80,7 +84,8 @@ hipError_t upload_plan(const DevPlan& plan, uint64_t* scratch, hipStream_t strea
   return hipMemcpyAsync(scratch, &plan, sizeof(DevPlan), hipMemcpyHostToDevice, stream);
 }
 
-// string literals -> one device buffer of {int32 len, bytes} records
+// string literals -> one device buffer of {int32 len, bytes} records (LIKE
+// patterns: their compiled form)
 hipError_t ensure_strings(Program& p) {
   if (p.d_strings || !p.str_offsets.empty()) return hipSuccess;
   std::vector<char> blob;
@@ -92,6 +97,11 @@ hipError_t ensure_strings(Program& p) {
       if (pos != BSQL_SCALAR_INDEX || s.type_id != BSQL_STRING) continue;
       while (blob.size() % 8) blob.push_back(0);
       offs[2 * i + side] = (int64_t)blob.size();
+      if (side == 1 && p.ops[i] == BSQL_OP_STR_LIKE) {
+        const std::vector<int32_t> units = like_compile(s.str);
+        blob.insert(blob.end(), (const char*)units.data(), (const char*)(units.data() + units.size()));
+        continue;
+      }
       int32_t len = (int32_t)s.str.size();
       blob.insert(blob.end(), (const char*)&len, (const char*)&len + 4);
       blob.insert(blob.end(), s.str.begin(), s.str.end());
@@ -157,6 +167,53 @@ std::string cache_key(int mode, const bsql_program* pred, const bsql_program* pr
     k += "|f" + std::to_string(pred->launch.flags);
   }
   return k;
+}
+
+// Output column checks shared by the launching entries (rows = capacity).
+void check_outputs(const bsql_column* out, int n_out, int64_t rows) {
+  for (int k = 0; k < n_out; ++k) {
+    const bsql_column& c = out[k];
+    if (c.offset != 0) fail("output columns must have offset 0");
+    if (c.size < rows) fail("output column " + std::to_string(k) + " holds fewer rows than requested");
+    if (c.type_id == BSQL_STRING) {
+      if (!c.offsets) fail("string output column without offsets");
+      if (c.chars_size < 0 || (c.chars_size > 0 && !c.data)) fail("string output column without chars");
+    } else {
+      if (type_size(c.type_id) == 0) fail("unsupported output column type");
+      if (rows > 0 && !c.data) fail("output column has no data");
+    }
+  }
+}
+
+// String outputs: the VM stores slices into scratch; each then needs the
+// chars buffer of its source column to fit in the output's chars.
+int count_string_outputs(const Lowered& L, const bsql_column* in, const bsql_column* out, int n_out) {
+  int n = 0;
+  for (int k = 0; k < n_out; ++k) {
+    const int src = L.out_str_col[(size_t)k];
+    if (src < 0) continue;
+    if (out[k].chars_size < in[src].chars_size)
+      fail("string output " + std::to_string(k) + ": chars_size " + std::to_string(out[k].chars_size) +
+           " is smaller than its source column's " + std::to_string(in[src].chars_size));
+    ++n;
+  }
+  return n;
+}
+
+hipError_t finish_string_outputs(const Lowered& L, const bsql_column* in, const bsql_column* out, int n_out,
+                                 uint64_t* slices, const int64_t* d_count, int64_t count, int64_t capacity,
+                                 uint64_t* status, hipStream_t stream) {
+  int j = 0;
+  for (int k = 0; k < n_out; ++k) {
+    const int src = L.out_str_col[(size_t)k];
+    if (src < 0) continue;
+    hipError_t e = launch_string_output(slices + (size_t)j * (size_t)capacity, out[k].validity, out[k].offsets,
+                                        (const uint8_t*)in[src].data, (uint8_t*)out[k].data, d_count, count,
+                                        capacity, status, stream);
+    if (e != hipSuccess) return e;
+    ++j;
+  }
+  return hipSuccess;
 }
 
 const Lowered& lowered_for(bsql_program* owner, const LowerOptions& opt, const bsql_column* in,
@@ -350,12 +407,7 @@ int bsql_interp_execute(const bsql_program* cprog, const bsql_column* in, int n_
     if (!prog) fail("null program");
     check_inputs(in, n_in, num_rows);
     if (n_out != (int)prog->final_out.size()) fail("number of outputs does not match the program");
-    for (int k = 0; k < n_out; ++k) {
-      if (out[k].offset != 0) fail("output columns must have offset 0");
-      if (type_size(out[k].type_id) == 0) fail("unsupported output column type");
-      if (out[k].size < num_rows) fail("output column is shorter than num_rows");
-      if (num_rows > 0 && !out[k].data) fail("output column has no data");
-    }
+    check_outputs(out, n_out, num_rows);
     hipError_t e = ensure_strings(*prog);
     if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
     LowerOptions opt;
@@ -365,17 +417,27 @@ int bsql_interp_execute(const bsql_program* cprog, const bsql_column* in, int n_
     DevPlan plan = L.plan;
     plan.num_rows = num_rows;
     for (int k = 0; k < n_in; ++k) plan.in[k] = to_dev_in(in[k]);
-    for (int k = 0; k < n_out; ++k) {
-      plan.out[k].data = out[k].data;
+    const int n_str = count_string_outputs(L, in, out, n_out);
+    // scratch: plan | string assembly status + ticket | one slice column per string output
+    const int64_t st_words = n_str ? string_tiles(num_rows) + 1 : 0;
+    uint64_t* scratch = nullptr;
+    e = get_scratch((hipStream_t)stream, st_words + (int64_t)n_str * num_rows, &scratch);
+    if (e != hipSuccess) return device_error(e, "bsql_interp_execute scratch");
+    uint64_t* status = scratch + kPlanWords;
+    uint64_t* slices = status + st_words;
+    for (int k = 0, j = 0; k < n_out; ++k) {
+      const bool str = L.out_str_col[(size_t)k] >= 0;
+      plan.out[k].data = str ? (void*)(slices + (size_t)(j++) * (size_t)num_rows) : out[k].data;
       plan.out[k].valid = out[k].validity;
       plan.out[k].type = out[k].type_id;
     }
-    if (num_rows == 0) return BSQL_OK;
-    uint64_t* scratch = nullptr;
-    e = get_scratch((hipStream_t)stream, 0, &scratch);
-    if (e == hipSuccess) e = upload_plan(plan, scratch, (hipStream_t)stream);
-    if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
-    e = launch_execute(plan, (const DevPlan*)scratch, L.nreg, rows_per_thread(prog), (hipStream_t)stream);
+    if (num_rows == 0 && n_str == 0) return BSQL_OK;
+    e = upload_plan(plan, scratch, (hipStream_t)stream);
+    if (e == hipSuccess && num_rows > 0)
+      e = launch_execute(plan, (const DevPlan*)scratch, L.nreg, rows_per_thread(prog), (hipStream_t)stream);
+    if (e == hipSuccess)
+      e = finish_string_outputs(L, in, out, n_out, slices, nullptr, num_rows, num_rows, status,
+                                (hipStream_t)stream);
     if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
     return BSQL_OK;
   })
@@ -389,12 +451,7 @@ static int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column
   check_inputs(in, n_in, num_rows);
   // selected rows are listed as 32-bit row ids
   if (num_rows > (int64_t)UINT32_MAX) fail("a filter handles at most 2^32-1 rows per call");
-  for (int k = 0; k < n_out; ++k) {
-    if (out[k].offset != 0) fail("output columns must have offset 0");
-    if (type_size(out[k].type_id) == 0) fail("unsupported output column type");
-    if (out[k].size < capacity) fail("output column capacity is smaller than requested");
-    if (capacity > 0 && !out[k].data) fail("output column has no data");
-  }
+  check_outputs(out, n_out, capacity);
   hipError_t e = ensure_strings(*pred);
   if (e == hipSuccess && proj) e = ensure_strings(*proj);
   if (e != hipSuccess) return device_error(e, "filter");
@@ -409,27 +466,35 @@ static int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column
   DevPlan plan = L.plan;
   plan.num_rows = num_rows;
   for (int k = 0; k < n_in; ++k) plan.in[k] = to_dev_in(in[k]);
-  for (int k = 0; k < n_out; ++k) {
-    plan.out[k].data = out[k].data;
-    plan.out[k].valid = out[k].validity;
-    plan.out[k].type = out[k].type_id;
-  }
+  const int n_str = count_string_outputs(L, in, out, n_out);
   // the predicate's launch shape drives the selection kernel, the
   // projection's (or the predicate's, for a plain filter) the output kernel
   const int sel_rpt = rows_per_thread(pred);
   const int rpt = rows_per_thread(proj ? proj : pred);
   const int64_t tiles = compact_tiles(plan, L.nreg, sel_rpt);
   if (tiles < 0) fail("no kernel variant for this program");
-  // scratch: plan | tile status words + ticket | selected row ids (uint32)
+  // scratch: plan | tile status words + ticket (shared by the selection and
+  // the string assembly) | selected row ids (uint32) | string output slices
+  const int64_t st_words = std::max(tiles, n_str ? string_tiles(capacity) : 0) + 1;
   const int64_t id_words = (num_rows + 1) / 2;
   uint64_t* scratch = nullptr;
-  e = get_scratch((hipStream_t)stream, tiles + 1 + id_words, &scratch);
-  if (e == hipSuccess) e = upload_plan(plan, scratch, (hipStream_t)stream);
+  e = get_scratch((hipStream_t)stream, st_words + id_words + (int64_t)n_str * capacity, &scratch);
   if (e != hipSuccess) return device_error(e, "filter scratch");
   uint64_t* status = scratch + kPlanWords;
-  uint32_t* ids = (uint32_t*)(status + tiles + 1);
-  e = launch_compact(plan, (const DevPlan*)scratch, L.nreg, sel_rpt, rpt, status, ids, d_count, capacity,
-                     (hipStream_t)stream);
+  uint32_t* ids = (uint32_t*)(status + st_words);
+  uint64_t* slices = status + st_words + id_words;
+  for (int k = 0, j = 0; k < n_out; ++k) {
+    const bool str = L.out_str_col[(size_t)k] >= 0;
+    plan.out[k].data = str ? (void*)(slices + (size_t)(j++) * (size_t)capacity) : out[k].data;
+    plan.out[k].valid = out[k].validity;
+    plan.out[k].type = out[k].type_id;
+  }
+  e = upload_plan(plan, scratch, (hipStream_t)stream);
+  if (e == hipSuccess)
+    e = launch_compact(plan, (const DevPlan*)scratch, L.nreg, sel_rpt, rpt, status, ids, d_count, capacity,
+                       (hipStream_t)stream);
+  if (e == hipSuccess)
+    e = finish_string_outputs(L, in, out, n_out, slices, d_count, 0, capacity, status, (hipStream_t)stream);
   if (e != hipSuccess) return device_error(e, "filter");
   return BSQL_OK;
 }

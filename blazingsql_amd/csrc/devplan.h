@@ -17,7 +17,7 @@
 enum DevCode : uint8_t {
   D_NOP = 0,
   D_LOAD,        // reg[dst] <- input column a (converted to the slot kind), validity
-  D_LOADV,       // validity of input column a only (string columns)
+  D_LOADS,       // reg[dst] <- slice of string column a: start byte | byte length << 32
   D_LOADN,       // issue loads of aux (1..4) columns (imm byte j = column j) into temporaries
   D_COMMIT,      // reg[dst] <- loaded column aux of the last LOADN (data and validity)
   D_STORE,       // output column aux <- reg[a] (static_cast to its type)
@@ -35,6 +35,10 @@ enum DevCode : uint8_t {
   D_TAN, D_COTAN, D_ATAN, D_LN, D_LOG, D_EXTRACT,
   D_IS_NULL, D_IS_NOT_NULL, D_IS_TRUE, D_IS_NOT_TRUE, D_IS_NOT_FALSE,
   D_TO_I64, D_TO_F64, D_CAST_DATE, D_CAST_TS, D_CHAR_LENGTH, D_RAND,
+  // string slices (a = slice of the chars of column aux): LIKE against the
+  // compiled pattern at imm; SLICE with aux2 = 0 from char b (0-based), 1 the
+  // first b chars, 2 the last b chars
+  D_LIKE, D_SLICE,
   D_NUM_CODES
 };
 
@@ -46,20 +50,23 @@ enum : uint8_t {
   F_B_NULL = 8,   // operand b is a null literal
   F_A_F = 16,     // operand a holds double bits (else int64)
   F_B_F = 32,     // operand b holds double bits
-  F_STORE_VALID = 64  // D_STORE: the output column has a validity buffer
+  F_STORE_VALID = 64,  // D_STORE: the output column has a validity buffer
+  F_SLICE_LIT = 128    // D_SLICE: the position is the (already clamped) literal in imm
 };
 
 typedef struct DevOp {
   uint8_t code;
   uint8_t dst;   // destination register (0xff: no result)
-  uint8_t a;     // register of operand a (D_LOAD/D_LOADV: input column)
+  uint8_t a;     // register of operand a (D_LOAD/D_LOADS: input column)
   uint8_t b;     // register of operand b
   uint8_t flags;
-  uint8_t aux;   // D_STORE: output column; D_EXTRACT: operator; *_SS/CHAR_LENGTH: column of a
+  uint8_t aux;   // D_STORE: output column; D_EXTRACT: operator; string ops: column whose
+                 // chars a slices (0xff: a is a literal)
   uint8_t aux2;  // *_TS: (unit of a) | (unit of b) << 4; *_SS: column of b or 0xff;
-                 // D_EXTRACT/D_CAST_DATE/D_CAST_TS: source type id
+                 // D_EXTRACT/D_CAST_DATE/D_CAST_TS: source type id; D_SLICE: kind
   uint8_t pad;
-  int64_t imm;   // literal bits; *_SS: device pointer to {int32 len, bytes}
+  int64_t imm;   // literal bits; *_SS: device pointer to {int32 len, bytes};
+                 // D_LIKE: device pointer to the compiled pattern (like_compile)
 } DevOp;
 
 // Every field the kernels read is 32 or 64 bits wide: gfx950 has no scalar

@@ -297,19 +297,96 @@ __device__ __noinline__ uint64_t compute_heavy(uint8_t code, uint8_t aux, uint8_
   }
 }
 
-__device__ __forceinline__ void str_at(const DevPlan& plan, uint8_t col, int64_t imm, int64_t row,
-                                       const GLOBAL uint8_t** p, int32_t* n) {
+// String values in registers are slices of a column's chars: start byte in
+// the low 32 bits, byte length in the high 32 bits. A string literal operand
+// holds the device address of its {int32 len, bytes} record instead.
+__device__ __forceinline__ uint64_t slice_pack(uint32_t start, uint32_t len) {
+  return (uint64_t)start | ((uint64_t)len << 32);
+}
+__device__ __forceinline__ void str_at(const DevPlan& plan, uint8_t col, uint64_t v, const GLOBAL uint8_t** p,
+                                       int32_t* n) {
   if (col != 0xff) {
-    const DevInCol& c = plan.in[col];
-    const GLOBAL int32_t* off = gp<int32_t>(c.offsets);
-    const int32_t s = off[row], e = off[row + 1];
-    *p = gp<uint8_t>(c.data) + s;
-    *n = e - s;
+    *p = gp<uint8_t>(plan.in[col].data) + (uint32_t)v;
+    *n = (int32_t)(v >> 32);
   } else {
-    const GLOBAL int32_t* lit = gp<int32_t>((const void*)imm);
+    const GLOBAL int32_t* lit = gp<int32_t>((const void*)v);
     *n = lit[0];
     *p = (const GLOBAL uint8_t*)(lit + 1);
   }
+}
+
+__device__ __forceinline__ bool utf8_lead(uint8_t c) { return (c & 0xC0) != 0x80; }
+// bytes of the UTF-8 character starting at s[i] (i < n)
+__device__ __forceinline__ int32_t utf8_step(const GLOBAL uint8_t* s, int32_t i, int32_t n) {
+  int32_t j = i + 1;
+  while (j < n && !utf8_lead(s[j])) ++j;
+  return j - i;
+}
+// byte offset of character `chars` of s (n bytes if s has fewer characters)
+__device__ int32_t utf8_offset(const GLOBAL uint8_t* s, int32_t n, int64_t chars) {
+  int64_t c = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    if (utf8_lead(s[i])) {
+      if (c == chars) return i;
+      ++c;
+    }
+  }
+  return n;
+}
+__device__ int64_t utf8_chars(const GLOBAL uint8_t* s, int32_t n) {
+  int64_t c = 0;
+  for (int32_t i = 0; i < n; ++i) c += utf8_lead(s[i]);
+  return c;
+}
+
+// SQL LIKE through the reference's translation to a regex searched with
+// contains_re (LogicalProject.cpp:31-47, :113-131): '%' = (?:.*?) and '_' =
+// (?:.), neither matching '\n'; anchored at the start unless the pattern
+// begins with '%', at the end unless it ends with '%'. The compiled pattern is
+// {n_units, flags, units...}: unit >= 0 a literal byte, -1 any character,
+// -2 any run of characters.
+enum : int32_t { LIKE_ANCHOR_START = 1, LIKE_ANCHOR_END = 2, LIKE_MATCH_ALL = 4 };
+__device__ bool like_from(const GLOBAL uint8_t* s, int32_t n, int32_t st, const GLOBAL int32_t* u, int32_t nu,
+                          bool anchor_end) {
+  int32_t i = st, q = 0, star_q = -1, star_i = 0;
+  while (true) {
+    if (q == nu) {
+      if (!anchor_end || i == n) return true;
+    } else if (u[q] == -2) {
+      star_q = q++;
+      star_i = i;
+      continue;
+    } else if (i < n && (u[q] == -1 ? s[i] != '\n' : (int32_t)s[i] == u[q])) {
+      i += u[q] == -1 ? utf8_step(s, i, n) : 1;
+      ++q;
+      continue;
+    }
+    // let the last '%' take one more character
+    if (star_q < 0 || star_i >= n || s[star_i] == '\n') return false;
+    star_i += utf8_step(s, star_i, n);
+    i = star_i;
+    q = star_q + 1;
+  }
+}
+__device__ bool like_match(const GLOBAL uint8_t* s, int32_t n, const GLOBAL int32_t* pat) {
+  const int32_t nu = pat[0], flags = pat[1];
+  if (flags & LIKE_MATCH_ALL) return true;
+  const GLOBAL int32_t* u = pat + 2;
+  const bool anchor_end = (flags & LIKE_ANCHOR_END) != 0;
+  if (flags & LIKE_ANCHOR_START) return like_from(s, n, 0, u, nu, anchor_end);
+  if (nu > 0 && u[0] == -2) {
+    // a leading run wildcard covers every start on the first line; a match
+    // may also start after any newline
+    if (like_from(s, n, 0, u, nu, anchor_end)) return true;
+    for (int32_t i = 0; i < n; ++i)
+      if (s[i] == '\n' && like_from(s, n, i + 1, u, nu, anchor_end)) return true;
+    return false;
+  }
+  // unanchored with a literal first unit ("%%..." leaves the first '%'
+  // literal): try every character start
+  for (int32_t i = 0; i <= n; i += (i < n ? utf8_step(s, i, n) : 1))
+    if (like_from(s, n, i, u, nu, anchor_end)) return true;
+  return false;
 }
 
 #define FA ((op.flags & F_A_F) != 0)
@@ -419,8 +496,8 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
         if (((rs.exist >> k) & 1u) && ((VA >> k) & 1u) && ((VB >> k) & 1u)) {
           const GLOBAL uint8_t *pa, *pb;
           int32_t na, nb;
-          str_at(plan, op.aux, op.imm, rs.row[k], &pa, &na);
-          str_at(plan, op.aux2, op.imm, rs.row[k], &pb, &nb);
+          str_at(plan, op.aux, A[k], &pa, &na);
+          str_at(plan, op.aux2, B[k], &pb, &nb);
           D[k] = (uint64_t)cmp_result(kind, str_cmp(pa, na, pb, nb));
           VD |= 1u << k;
         }
@@ -428,17 +505,61 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
       return;
     }
     case D_CHAR_LENGTH: {
-      const DevInCol& c = plan.in[op.aux];
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         D[k] = 0;
         if (((rs.exist >> k) & 1u) && ((VA >> k) & 1u)) {
-          const GLOBAL int32_t* off = gp<int32_t>(c.offsets);
-          const int32_t s = off[rs.row[k]], e = off[rs.row[k] + 1];
-          const GLOBAL uint8_t* p = gp<uint8_t>(c.data);
-          int64_t n = 0;
-          for (int32_t i = s; i < e; ++i) n += (p[i] & 0xC0) != 0x80;
-          D[k] = (uint64_t)n;
+          const GLOBAL uint8_t* p;
+          int32_t n;
+          str_at(plan, op.aux, A[k], &p, &n);
+          D[k] = (uint64_t)utf8_chars(p, n);
+          VD |= 1u << k;
+        }
+      }
+      return;
+    }
+    case D_LIKE: {
+      const GLOBAL int32_t* pat = gp<int32_t>((const void*)op.imm);
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        D[k] = 0;
+        if (((rs.exist >> k) & 1u) && ((VA >> k) & 1u)) {
+          const GLOBAL uint8_t* p;
+          int32_t n;
+          str_at(plan, op.aux, A[k], &p, &n);
+          D[k] = like_match(p, n, pat) ? 1 : 0;
+          VD |= 1u << k;
+        }
+      }
+      return;
+    }
+    case D_SLICE: {
+      // slice_strings positions (LogicalProject.cpp:183-306): a literal
+      // arrives clamped; a computed SUBSTRING start is made 0-based, a null
+      // start or count reads as 0, and a negative start as 0
+      const bool lit = (op.flags & F_SLICE_LIT) != 0;
+      const int kind = op.aux2;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        D[k] = A[k];
+        if (((rs.exist >> k) & 1u) && ((VA >> k) & 1u)) {
+          int64_t pos = ((VB >> k) & 1u) ? (int64_t)B[k] : 0;
+          if (!lit) pos = kind == 0 ? pos - (((VB >> k) & 1u) ? 1 : 0) : pos;
+          if (pos < 0) pos = 0;
+          const GLOBAL uint8_t* p;
+          int32_t n;
+          str_at(plan, op.aux, A[k], &p, &n);
+          const uint32_t st = (uint32_t)A[k];
+          if (kind == 0) {
+            const int32_t b0 = utf8_offset(p, n, pos);
+            D[k] = slice_pack(st + (uint32_t)b0, (uint32_t)(n - b0));
+          } else if (kind == 1) {
+            D[k] = slice_pack(st, (uint32_t)utf8_offset(p, n, pos));
+          } else {
+            const int64_t total = utf8_chars(p, n);
+            const int32_t b0 = total > pos ? utf8_offset(p, n, total - pos) : 0;
+            D[k] = slice_pack(st + (uint32_t)b0, (uint32_t)(n - b0));
+          }
           VD |= 1u << k;
         }
       }
@@ -657,10 +778,17 @@ __device__ __forceinline__ bool vm_step(const DevPlan& plan, const DevOp& op,
 #pragma unroll
     for (int k = 0; k < R; ++k) D[k] = j == 0 ? T.t0[k] : (j == 1 ? T.t1[k] : (j == 2 ? T.t2[k] : T.t3[k]));
     VD = j == 0 ? T.v0 : (j == 1 ? T.v1 : (j == 2 ? T.v2 : T.v3));
-  } else if (code == D_LOADV) {
+  } else if (code == D_LOADS) {
     const DevInCol& c = plan.in[op.a];
+    const GLOBAL int32_t* off = gp<int32_t>(c.offsets);
 #pragma unroll
-    for (int k = 0; k < R; ++k) D[k] = 0;
+    for (int k = 0; k < R; ++k) {
+      D[k] = 0;
+      if ((rs.exist >> k) & 1u) {
+        const int32_t s0 = off[rs.row[k]], s1 = off[rs.row[k] + 1];
+        D[k] = slice_pack((uint32_t)s0, (uint32_t)(s1 - s0));
+      }
+    }
     VD = rs.contig ? load_valid<R>(c, rs.row0, rs.nrows) : gather_valid<R>(c, rs);
   } else if (code == D_KILL || code == D_SEL) {
 #pragma unroll
@@ -1088,6 +1216,107 @@ __global__ __launch_bounds__(BLOCK) void select_vm(const DevPlan* __restrict__ p
                     &s_tile_prefix);
 }
 
+// ------------------------------------------------------- string outputs
+//
+// A string output is written by the VM as one slice per row (start byte |
+// length << 32, into the chars of its source column). Two kernels turn the
+// slices into an Arrow string column: offsets (exclusive scan of the lengths
+// of valid rows, decoupled look-back across 4096-row tiles, offsets[count] =
+// total), then the chars. Null rows get length 0.
+constexpr int STR_R = 16;
+constexpr int STR_TILE = BLOCK * STR_R;
+
+__device__ __forceinline__ bool row_valid(const uint32_t* valid, int64_t row) {
+  return !valid || ((gp<uint32_t>(valid)[row >> 5] >> (row & 31)) & 1u);
+}
+
+__global__ __launch_bounds__(BLOCK) void str_offsets_kernel(const uint64_t* __restrict__ slices,
+                                                            const uint32_t* __restrict__ valid,
+                                                            int32_t* __restrict__ offsets,
+                                                            const int64_t* __restrict__ d_count,
+                                                            int64_t count_h, CompactArgs ca) {
+  __shared__ uint32_t s_ticket;
+  __shared__ int64_t s_wave_total[WAVES];
+  __shared__ int64_t s_prefix;
+  if (threadIdx.x == 0) s_ticket = atomicAdd(ca.ticket, 1u);
+  __syncthreads();
+  const int64_t tile = s_ticket;
+  int64_t count = count_h;
+  if (d_count) {
+    count = *d_count;
+    if (count > ca.capacity) count = ca.capacity;
+  }
+  const int64_t tiles = count > 0 ? (count + STR_TILE - 1) / STR_TILE : 1;
+  if (tile >= tiles) return;  // rows beyond the count: nothing to do
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t r0 = tile * STR_TILE + (int64_t)tid * STR_R;
+  const GLOBAL uint64_t* sl = gp<uint64_t>(slices);
+  uint32_t len[STR_R];
+  int64_t mine = 0;
+#pragma unroll
+  for (int k = 0; k < STR_R; ++k) {
+    const int64_t row = r0 + k;
+    len[k] = (row < count && row_valid(valid, row)) ? (uint32_t)(sl[row] >> 32) : 0u;
+    mine += len[k];
+  }
+  // inclusive scan of the lane sums over the wave
+  int64_t incl = mine;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t t = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += t;
+  }
+  if (lane == 63) s_wave_total[wave] = incl;
+  __syncthreads();
+  int64_t wave_off = 0, tile_total = 0;
+#pragma unroll
+  for (int w = 0; w < WAVES; ++w) {
+    const int64_t t = s_wave_total[w];
+    if (w < wave) wave_off += t;
+    tile_total += t;
+  }
+  if (wave == 0) {
+    const int64_t excl = look_back(ca, tile, tile_total, lane);
+    if (lane == 0) s_prefix = excl;
+  }
+  __syncthreads();
+  int64_t pos = s_prefix + wave_off + incl - mine;
+  GLOBAL int32_t* off = gpw<int32_t>(offsets);
+#pragma unroll
+  for (int k = 0; k < STR_R; ++k) {
+    const int64_t row = r0 + k;
+    if (row < count) off[row] = (int32_t)pos;
+    pos += len[k];
+  }
+  if (tile == tiles - 1 && tid == BLOCK - 1) off[count] = (int32_t)(s_prefix + tile_total);
+}
+
+__global__ __launch_bounds__(BLOCK) void str_copy_kernel(const uint64_t* __restrict__ slices,
+                                                         const uint32_t* __restrict__ valid,
+                                                         const int32_t* __restrict__ offsets,
+                                                         const uint8_t* __restrict__ src,
+                                                         uint8_t* __restrict__ dst,
+                                                         const int64_t* __restrict__ d_count,
+                                                         int64_t count_h, int64_t capacity) {
+  int64_t count = count_h;
+  if (d_count) {
+    count = *d_count;
+    if (count > capacity) count = capacity;
+  }
+  const GLOBAL uint64_t* sl = gp<uint64_t>(slices);
+  const GLOBAL int32_t* off = gp<int32_t>(offsets);
+  const GLOBAL uint8_t* s = gp<uint8_t>(src);
+  GLOBAL uint8_t* d = gpw<uint8_t>(dst);
+  for (int64_t row = (int64_t)blockIdx.x * BLOCK + threadIdx.x; row < count;
+       row += (int64_t)gridDim.x * BLOCK) {
+    if (!row_valid(valid, row)) continue;
+    const uint64_t v = sl[row];
+    const uint32_t start = (uint32_t)v, n = (uint32_t)(v >> 32);
+    const int64_t o = off[row];
+    for (uint32_t i = 0; i < n; ++i) d[o + i] = s[start + i];
+  }
+}
+
 __global__ void fill_u64(uint64_t* p, int64_t n, uint64_t v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1239,6 +1468,27 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
   BSQL_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;
+}
+
+// Assembles string output columns from the slices the VM stored: offsets
+// then chars. `status` holds string_tiles(capacity) + 1 words; `d_count`
+// (device) or `count` (host) gives the rows.
+int64_t string_tiles(int64_t capacity) { return capacity > 0 ? (capacity + STR_TILE - 1) / STR_TILE : 1; }
+
+hipError_t launch_string_output(const uint64_t* slices, const uint32_t* valid, int32_t* offsets,
+                                const uint8_t* src_chars, uint8_t* dst_chars, const int64_t* d_count,
+                                int64_t count, int64_t capacity, uint64_t* status, hipStream_t stream) {
+  const int64_t tiles = string_tiles(capacity);
+  CompactArgs ca{status, (uint32_t*)(status + tiles), nullptr, capacity, tiles};
+  hipLaunchKernelGGL(fill_u64, dim3((unsigned)((tiles + 1 + 255) / 256)), dim3(256), 0, stream, status,
+                     tiles + 1, 0ULL);
+  hipLaunchKernelGGL(str_offsets_kernel, dim3((unsigned)tiles), dim3(BLOCK), 0, stream, slices, valid, offsets,
+                     d_count, count, ca);
+  const int64_t blocks = persistent_grid((capacity + BLOCK - 1) / BLOCK);
+  if (blocks > 0)
+    hipLaunchKernelGGL(str_copy_kernel, dim3((unsigned)blocks), dim3(BLOCK), 0, stream, slices, valid, offsets,
+                       src_chars, dst_chars, d_count, count, capacity);
+  return hipGetLastError();
 }
 
 }  // namespace bsql

@@ -676,14 +676,38 @@ void collect_vars(const Node* n, std::vector<int>& acc) {
   for (auto& c : n->children) collect_vars(c.get(), acc);
 }
 
+// String functions this build does not evaluate (the reference computes them
+// with libcudf before the interpreter, LogicalProject.cpp:100-742). LIKE,
+// SUBSTRING, LEFT and RIGHT run inside the interpreter on string slices.
 bool is_string_function(const std::string& v) {
-  static const char* names[] = {"LIKE", "SUBSTRING", "REGEXP_REPLACE", "REPLACE", "TO_DATE",
-                                "TO_TIMESTAMP", "||", "CONCAT", "TRIM", "LEFT", "RIGHT",
-                                "LOWER", "UPPER", "INITCAP", "REVERSE", "CAST_VARCHAR",
-                                "CAST_CHAR"};
+  static const char* names[] = {"REGEXP_REPLACE", "REPLACE", "TO_DATE", "TO_TIMESTAMP", "||",
+                                "CONCAT", "TRIM", "LOWER", "UPPER", "INITCAP", "REVERSE",
+                                "CAST_VARCHAR", "CAST_CHAR"};
   for (const char* s : names)
     if (v == s) return true;
   return false;
+}
+
+// SUBSTRING(s, a, b) -> LEFT(SUBSTRING(s, a), b). The reference slices
+// [a0, a0 + b) with a0 = max(a, 1) - 1 (LogicalProject.cpp:217-223, :240-306);
+// only its all-literal form over a plain column (:217) reads a negative
+// literal b as "to the end", which stays SUBSTRING(s, a).
+std::unique_ptr<Node> rewrite_substring(std::unique_ptr<Node> n) {
+  for (auto& c : n->children) c = rewrite_substring(std::move(c));
+  if (n->type != NodeType::Operator || n->value != "SUBSTRING") return n;
+  if (n->children.size() == 2) return n;
+  if (n->children.size() != 3) fail("SUBSTRING takes two or three arguments");
+  std::unique_ptr<Node> len = std::move(n->children[2]);
+  n->children.pop_back();
+  const bool literal_form = n->children[0]->type == NodeType::Variable &&
+                            n->children[1]->type == NodeType::Literal && len->type == NodeType::Literal;
+  if (literal_form && !is_null_literal(len->value) && std::stoll(len->value) < 0) return n;
+  auto left = std::make_unique<Node>();
+  left->type = NodeType::Operator;
+  left->value = "LEFT";
+  left->children.push_back(std::move(n));
+  left->children.push_back(std::move(len));
+  return left;
 }
 
 void check_interpretable(const Node* n) {
@@ -703,7 +727,7 @@ std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exp
     std::string e = replace_calcite_regex(fill_minus_op_with_zero(raw));
     e = expand_if_logical_op(e);
     Parser parser(e);
-    std::unique_ptr<Node> t = transform_custom(parser.parse());
+    std::unique_ptr<Node> t = rewrite_substring(transform_custom(parser.parse()));
     check_interpretable(t.get());
     if (t->type == NodeType::Literal)
       fail("literal expression: the operator layer fills it without the interpreter");

@@ -97,7 +97,8 @@ int output_type_binary(int op, int lt, int rt) {
     case BSQL_OP_MAGIC_IF_NOT: return rt;
     case BSQL_OP_FIRST_NON_MAGIC: return type_size(lt) >= type_size(rt) ? lt : rt;
     case BSQL_OP_STR_SUBSTRING: case BSQL_OP_STR_REPLACE: case BSQL_OP_STR_REGEXP_REPLACE:
-    case BSQL_OP_STR_CONCAT: case BSQL_OP_STR_TRIM: return BSQL_STRING;
+    case BSQL_OP_STR_CONCAT: case BSQL_OP_STR_TRIM: case BSQL_OP_STR_LEFT: case BSQL_OP_STR_RIGHT:
+      return BSQL_STRING;
     case BSQL_OP_TO_DATE: return BSQL_TIMESTAMP_DAYS;
     case BSQL_OP_TO_TIMESTAMP: return BSQL_TIMESTAMP_NANOSECONDS;
     default:
@@ -155,6 +156,8 @@ void validate(const Program& p, int n_in) {
     } else {
       if (!is_binary_op(op)) fail("operator " + std::to_string(op) + " is not binary");
       if (l < 0 && r < 0) fail("Operations between literals is not supported");
+      if (op == BSQL_OP_STR_LIKE && r == BSQL_SCALAR_INDEX && p.rsc[i].type_id == BSQL_STRING)
+        (void)like_compile(p.rsc[i].str);  // rejects patterns the matcher does not take
       if (l < BSQL_SCALAR_NULL_INDEX || r < BSQL_SCALAR_NULL_INDEX) fail("invalid operand index");
     }
   }
@@ -162,6 +165,58 @@ void validate(const Program& p, int n_in) {
   if (mx > BSQL_MAX_SLOT)
     fail("Interops does not support plans with an input or output index greater than 63");
   (void)n_in;
+}
+
+// One regex_replace pass of like_expression_to_regex_str
+// (LogicalProject.cpp:39-43): matches of ([^\\]?|\\{2})W are found left to
+// right and their W becomes a wildcard (here the marker `mark`). A character
+// other than a backslash right before W is consumed with it, so in "WW" the
+// first W stays a literal character, as it does in the reference.
+std::string like_pass(const std::string& in, char w, char mark) {
+  std::string out;
+  size_t i = 0;
+  while (i < in.size()) {
+    if (in[i] != '\\' && i + 1 < in.size() && in[i + 1] == w) {
+      out += in[i];
+      out += mark;
+      i += 2;
+    } else if (in[i] == w) {
+      out += mark;
+      i += 1;
+    } else {
+      out += in[i];
+      i += 1;
+    }
+  }
+  return out;
+}
+
+// LIKE pattern -> {n_units, flags, units...} for the device matcher: the
+// reference's translation to a regex (LogicalProject.cpp:31-47), with the
+// regex reduced to literal bytes (unit >= 0), '(?:.)' (-1) and '(?:.*?)' (-2),
+// anchored at the start unless the pattern's first character is '%' and at
+// the end unless its last is. Characters that would be regex syntax in the
+// reference (backslash included) are rejected rather than given a meaning.
+std::vector<int32_t> like_compile(const std::string& pat) {
+  enum { ANCHOR_START = 1, ANCHOR_END = 2, MATCH_ALL = 4 };
+  const char ANY_RUN = '\x01', ANY_CHAR = '\x02';
+  std::vector<int32_t> units;
+  int32_t flags = 0;
+  if (pat.empty()) {
+    flags = MATCH_ALL;  // the empty regex is found in every string
+  } else {
+    for (char c : pat) {
+      if (c == ANY_RUN || c == ANY_CHAR || (std::strchr(".^$*+?()[]{}|\\", c) && c != 0))
+        fail(std::string("LIKE pattern character '") + c + "' is regex syntax in the reference");
+    }
+    if (pat.front() != '%') flags |= ANCHOR_START;
+    if (pat.back() != '%') flags |= ANCHOR_END;
+    const std::string re = like_pass(like_pass(pat, '%', ANY_RUN), '_', ANY_CHAR);
+    for (char c : re) units.push_back(c == ANY_RUN ? -2 : (c == ANY_CHAR ? -1 : (int32_t)(unsigned char)c));
+  }
+  std::vector<int32_t> out{(int32_t)units.size(), flags};
+  out.insert(out.end(), units.begin(), units.end());
+  return out;
 }
 
 // ------------------------------------------------------------------ lowering
@@ -175,6 +230,7 @@ struct Value {
   int reg = -1;
   bool is_load = false;
   int column = -1;     // is_load: input column
+  int str_col = -1;    // string values: the column whose chars the slice points into
   bool emitted = false;
 };
 
@@ -205,6 +261,7 @@ struct Lowering {
       int v = new_value();
       values[v].is_load = true;
       values[v].column = col;
+      if (is_string_type(in_types[col])) values[v].str_col = col;
       load_value[col] = v;
     }
     return load_value[col];
@@ -214,7 +271,7 @@ struct Lowering {
     if (v < 0 || !values[v].is_load || values[v].emitted) return;
     LOp l;
     const int col = values[v].column;
-    l.d.code = is_string_type(in_types[col]) ? D_LOADV : D_LOAD;
+    l.d.code = is_string_type(in_types[col]) ? D_LOADS : D_LOAD;
     l.d.a = (uint8_t)col;
     l.d.aux = (uint8_t)in_types[col];
     l.vdst = v;
@@ -251,6 +308,12 @@ int64_t f64_bits(double d) { int64_t b; std::memcpy(&b, &d, 8); return b; }
 int64_t scalar_bits(const Scalar& s) {
   if (is_float_type(s.type_id)) return f64_bits(s.f64);
   return s.i64;
+}
+
+// The column whose chars a string value slices.
+int str_col_of(const Lowering& L, int v) {
+  if (v < 0 || L.values[(size_t)v].str_col < 0) fail("string operand without a source column");
+  return L.values[(size_t)v].str_col;
 }
 
 // Converts reference op i of program p into SSA form (without emitting).
@@ -294,10 +357,9 @@ SourceOp translate(Lowering& L, const Program& p, const OpTypes& t, size_t i,
         if (is_string_type(lt) || is_string_type(rt)) {
           if (!(is_string_type(lt) && is_string_type(rt)))
             fail("comparison between a string and a non-string operand");
-          if (l >= n_slots_in || r >= n_slots_in) fail("string operands must be input columns");
           d.code = (uint8_t)(D_EQ_SS + k);
-          if (l >= 0) d.aux = (uint8_t)col_of[(size_t)l]; else { d.aux = 0xff; d.imm = str_base_a; }
-          if (r >= 0) d.aux2 = (uint8_t)col_of[(size_t)r]; else { d.aux2 = 0xff; d.imm = str_base_b; }
+          if (l >= 0) d.aux = (uint8_t)str_col_of(L, s.va); else { d.aux = 0xff; d.imm = str_base_a; }
+          if (r >= 0) d.aux2 = (uint8_t)str_col_of(L, s.vb); else { d.aux2 = 0xff; d.imm = str_base_b; }
           if (l < 0 && r < 0) fail("Operations between literals is not supported");
         } else if (is_timestamp_type(lt) && is_timestamp_type(rt)) {
           d.code = (uint8_t)(D_EQ_TS + k);
@@ -312,9 +374,37 @@ SourceOp translate(Lowering& L, const Program& p, const OpTypes& t, size_t i,
       case BSQL_OP_IS_NOT_DISTINCT_FROM: d.code = ii ? D_INDF_II : D_INDF_FF; break;
       case BSQL_OP_MAGIC_IF_NOT: d.code = D_MAGIC_IF_NOT; break;
       case BSQL_OP_FIRST_NON_MAGIC: d.code = D_FIRST_NON_MAGIC; break;
+      // string functions (evaluated ahead of the interpreter in the reference,
+      // LogicalProject.cpp:100-330): here they work on slices in registers
+      case BSQL_OP_STR_LIKE:
+        if (!is_string_type(lt) || l < 0) fail("LIKE needs a string column or expression");
+        if (r != BSQL_SCALAR_INDEX || !is_string_type(rt)) fail("LIKE needs a string literal pattern");
+        d.code = D_LIKE;
+        d.aux = (uint8_t)str_col_of(L, s.va);
+        d.imm = str_base_b;  // the pattern, compiled when the literals are uploaded
+        break;
+      case BSQL_OP_STR_SUBSTRING: case BSQL_OP_STR_LEFT: case BSQL_OP_STR_RIGHT: {
+        if (!is_string_type(lt) || l < 0) fail("string function on a non-string operand");
+        if (is_string_type(rt) || rf) fail("string position arguments must be integers");
+        d.code = D_SLICE;
+        d.aux = (uint8_t)str_col_of(L, s.va);
+        d.aux2 = (uint8_t)(op == BSQL_OP_STR_SUBSTRING ? 0 : (op == BSQL_OP_STR_LEFT ? 1 : 2));
+        if (r == BSQL_SCALAR_INDEX) {
+          // literal arguments are clamped as the reference does before
+          // slice_strings: SUBSTRING start max(s, 1) - 1, LEFT/RIGHT max(n, 0)
+          const int64_t v = p.rsc[i].i64;
+          d.imm = op == BSQL_OP_STR_SUBSTRING ? std::max<int64_t>(v, 1) - 1 : std::max<int64_t>(v, 0);
+          d.flags |= F_SLICE_LIT;
+        }
+        break;
+      }
       default: fail("operator " + std::to_string(op) + " is not supported by the interpreter");
     }
-    if (is_string_type(lt) && d.code < D_EQ_SS) fail("string operand to a non-string operator");
+    if (is_string_type(lt) && !((d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE ||
+                                d.code == D_SLICE))
+      fail("string operand to a non-string operator");
+    if (is_string_type(rt) && !((d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE))
+      fail("string operand to a non-string operator");
   } else if (r == BSQL_UNARY_INDEX) {
     s.va = operand(l);
     if (lf) d.flags |= F_A_F;
@@ -348,8 +438,8 @@ SourceOp translate(Lowering& L, const Program& p, const OpTypes& t, size_t i,
       case BSQL_OP_CAST_TIMESTAMP_MILLISECONDS: case BSQL_OP_CAST_TIMESTAMP_MICROSECONDS:
         d.code = D_CAST_TS; d.aux2 = (uint8_t)lt; break;
       case BSQL_OP_CHAR_LENGTH:
-        if (!is_string_type(lt) || l >= n_slots_in) fail("CHAR_LENGTH needs a string input column");
-        d.code = D_CHAR_LENGTH; d.aux = (uint8_t)col_of[(size_t)l]; break;
+        if (!is_string_type(lt)) fail("CHAR_LENGTH needs a string operand");
+        d.code = D_CHAR_LENGTH; d.aux = (uint8_t)str_col_of(L, s.va); break;
       default: fail("operator " + std::to_string(op) + " is not supported by the interpreter");
     }
     if (is_string_type(lt) && d.code != D_CHAR_LENGTH && d.code != D_IS_NULL &&
@@ -462,6 +552,7 @@ SsaProgram to_ssa(Lowering& L, const Program& p, uint64_t seed) {
     SourceOp so = translate(L, p, t, i, slot_val, sa >= 0 ? base + sa : 0, sb >= 0 ? base + sb : 0,
                             seed, n_slots_in, col_of);
     so.vdst = L.new_value();
+    if (so.d.code == D_SLICE) L.values[(size_t)so.vdst].str_col = L.values[(size_t)so.va].str_col;
     slot_val[p.out[i]] = so.vdst;
     s.ops.push_back(so);
   }
@@ -713,10 +804,7 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
   // ---- phase 2: outputs
   if (opt.mode == LowerOptions::FILTER_GATHER) {
     if (n_out != n_in) fail("filter gathers every input column");
-    for (int k = 0; k < n_in; ++k) {
-      if (is_string_type(in_types[k])) fail("string columns are gathered by the string path");
-      store_value.push_back(L.load_of(k));
-    }
+    for (int k = 0; k < n_in; ++k) store_value.push_back(L.load_of(k));
   } else {
     const Program& proj = *opt.proj;
     if ((int)proj.final_out.size() != n_out) fail("number of outputs does not match the program");
@@ -724,7 +812,15 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
     for (const SourceOp& so : s.ops) L.emit(to_lop(so));
     store_value = s.finals;
   }
+  res.out_str_col.assign((size_t)n_out, -1);
   for (int k = 0; k < n_out; ++k) {
+    // a string value is stored as its slice; the chars are copied after the
+    // kernel from the column the slice points into
+    const int sc = L.values[(size_t)store_value[(size_t)k]].str_col;
+    if ((sc >= 0) != is_string_type(out_types[k]))
+      fail(sc >= 0 ? "a string expression needs a STRING output column"
+                   : "a STRING output column needs a string expression");
+    res.out_str_col[(size_t)k] = sc;
     LOp st;
     st.d.code = D_STORE;
     st.va = store_value[(size_t)k];
@@ -815,7 +911,7 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
   }
   // loads keep their column index in `a`
   for (LOp& o : L.out)
-    if (o.d.code == D_LOAD || o.d.code == D_LOADV) o.d.a = (uint8_t)L.values[(size_t)o.vdst].column;
+    if (o.d.code == D_LOAD || o.d.code == D_LOADS) o.d.a = (uint8_t)L.values[(size_t)o.vdst].column;
 
   // result-less ops write no register
   for (LOp& o : L.out)
@@ -868,7 +964,6 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
   res.plan.n_ranges = (int32_t)res.plan_ranges.size();
   res.plan.range_eager = opt.lazy_loads ? 0 : 1;
   for (size_t i = 0; i < res.plan_ranges.size(); ++i) res.plan.ranges[i] = res.plan_ranges[i];
-  (void)out_types;
   res.nreg = used + 1;  // + the spare register
   return res;
 }

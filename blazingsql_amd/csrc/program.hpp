@@ -77,6 +77,9 @@ struct OpTypes {
 };
 OpTypes propagate_types(const Program& p, const int32_t* in_types, int n_in);
 
+// LIKE pattern compiled for the device matcher: {n_units, flags, units...}.
+std::vector<int32_t> like_compile(const std::string& pattern);
+
 // Types of the interpreter input slots of `p` for a table schema: programs
 // compiled from text address their used columns, array-built ones the table.
 std::vector<int32_t> slot_types_for(const Program& p, const int32_t* table_types, int n_table);
@@ -100,6 +103,7 @@ struct Lowered {
   int nreg = 0;          // registers used
   int n_conjuncts = 0;   // conjuncts found in the predicate (0 = not an AND chain)
   std::vector<DevRange> plan_ranges;  // the selection as range tests (empty: ops)
+  std::vector<int> out_str_col;       // per output: source column of a string output, else -1
 };
 Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
               const int32_t* out_types, int n_out);

@@ -211,7 +211,19 @@ class DeviceColumn:
         c.data = self.data.data_ptr() if self.data is not None and self.data.numel() else None
         c.validity = self.validity.data_ptr() if self.validity is not None else None
         c.offsets = self.offsets.data_ptr() if self.offsets is not None else None
+        if self.type_id == L.STRING:
+            c.chars_size = self.chars_size
         return c
+
+    @property
+    def chars_size(self):
+        """Bytes in the chars buffer (a string column's capacity)."""
+        cap = getattr(self, "_chars_cap", None)
+        if cap is not None:
+            return cap
+        if self.offsets is not None and self.offsets.numel():
+            return int(self.offsets[-1].item())
+        return 0
 
     @staticmethod
     def from_numpy(type_id, values, valid=None, device="cuda", chars=None, offsets=None):
@@ -232,8 +244,26 @@ class DeviceColumn:
         return DeviceColumn(type_id, data, vt, size=len(arr))
 
     @staticmethod
-    def empty(type_id, n, nullable=True, device="cuda"):
+    def from_strings(strings, valid=None, device="cuda"):
+        """A STRING column from Python str/bytes values."""
+        bs = [s.encode() if isinstance(s, str) else bytes(s) for s in strings]
+        offs = np.zeros(len(bs) + 1, dtype=np.int32)
+        offs[1:] = np.cumsum([len(b) for b in bs]) if bs else []
+        chars = np.frombuffer(b"".join(bs), dtype=np.uint8)
+        return DeviceColumn.from_numpy(L.STRING, None, valid, device, chars=chars, offsets=offs)
+
+    @staticmethod
+    def empty(type_id, n, nullable=True, device="cuda", chars=0):
+        """Uninitialized output column of n rows (STRING: n+1 offsets and
+        `chars` bytes of chars)."""
         import torch
+        if type_id == L.STRING:
+            data = torch.empty(max(chars, 1), dtype=torch.uint8, device=device)
+            offs = torch.zeros(n + 1, dtype=torch.int32, device=device)
+            vt = torch.zeros(max((n + 31) // 32, 1), dtype=torch.int32, device=device) if nullable else None
+            col = DeviceColumn(type_id, data, vt, offs, size=n)
+            col._chars_cap = chars
+            return col
         dt = {1: torch.int8, 2: torch.int16, 4: torch.int32, 8: torch.int64}
         width = np.dtype(np_dtype(type_id)).itemsize
         data = torch.empty(max(n, 1), dtype=dt[width], device=device)
@@ -241,7 +271,17 @@ class DeviceColumn:
         return DeviceColumn(type_id, data, vt, size=n)
 
     def to_numpy(self, n=None):
+        """(values, valid) of the first n rows; STRING values are a list of str."""
         n = self.size if n is None else n
+        if self.type_id == L.STRING:
+            offs = self.offsets[self.offset:self.offset + n + 1].cpu().numpy().astype(np.int64)
+            chars = self.data.cpu().numpy()
+            vals = [bytes(chars[offs[i]:offs[i + 1]]).decode(errors="replace") for i in range(n)]
+            if self.validity is None:
+                valid = np.ones(n, dtype=bool)
+            else:
+                valid = unpack_validity(self.validity.cpu().numpy().view(np.uint32), self.offset + n)[self.offset:]
+            return vals, valid
         raw = self.data[self.offset:self.offset + n].cpu().numpy() if n else np.zeros(0)
         vals = raw.view(np_dtype(self.type_id)) if n else np.zeros(0, dtype=np_dtype(self.type_id))
         if self.validity is None:

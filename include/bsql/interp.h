@@ -102,13 +102,16 @@ enum {
 /* ---- columns: Arrow/cudf layout -------------------------------------------
  * Mirrors cudf::column_view as used by BlazingColumn/BlazingTable
  * (engine/src/execution_kernels/LogicPrimitives.h:23-89).
- *   data      fixed-width elements (STRING: the chars buffer)
- *   validity  32-bit words, LSB-first, 1 = valid; NULL means all valid
- *   offsets   STRING only: size+1 int32 offsets into data
- *   offset    first row of the view (row units; bit units into validity)   */
+ *   data        fixed-width elements (STRING: the chars buffer)
+ *   validity    32-bit words, LSB-first, 1 = valid; NULL means all valid
+ *   offsets     STRING only: size+1 int32 offsets into data
+ *   offset      first row of the view (row units; bit units into validity)
+ *   chars_size  STRING only: bytes in the chars buffer (an output's
+ *               capacity; it must hold the chars of the column it is made
+ *               from, which bounds any slice or selection of it)          */
 typedef struct bsql_column {
   int32_t type_id;
-  int32_t reserved;
+  int32_t chars_size;
   int64_t size;
   int64_t offset;
   void* data;

@@ -29,7 +29,7 @@ _NP = {1: np.int8, 2: np.int16, 3: np.int32, 4: np.int64, 5: np.uint8, 6: np.uin
 
 
 class _Column(ctypes.Structure):
-    _fields_ = [("type_id", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    _fields_ = [("type_id", ctypes.c_int32), ("chars_size", ctypes.c_int32),
                 ("size", ctypes.c_int64), ("offset", ctypes.c_int64),
                 ("data", ctypes.c_void_p), ("validity", ctypes.c_void_p),
                 ("offsets", ctypes.c_void_p)]
@@ -100,6 +100,7 @@ class HostColumn:
         if self.type_id == STRING:
             c.data = self.chars.ctypes.data
             c.offsets = self.offsets.ctypes.data
+            c.chars_size = int(self.offsets[-1]) if self.size else 0
         else:
             c.data = self.values.ctypes.data if self.size else None
         c.validity = self._words.ctypes.data if self._words is not None else None
@@ -158,15 +159,36 @@ def _cols(cols):
     return arr
 
 
-def _alloc_out(type_id, n):
-    vals = np.zeros(max(n, 1), dtype=_NP[type_id])
+def _alloc_out(type_id, n, chars_cap=0):
+    """Output buffers; a STRING output gets offsets and chars_cap chars."""
     words = np.zeros(max((n + 31) // 32, 1), dtype=np.uint32)
     c = _Column()
     c.type_id = type_id
     c.size = n
-    c.data = vals.ctypes.data
     c.validity = words.ctypes.data
+    if type_id == STRING:
+        chars = np.zeros(max(chars_cap, 1), dtype=np.uint8)
+        offs = np.zeros(n + 1, dtype=np.int32)
+        c.data = chars.ctypes.data
+        c.offsets = offs.ctypes.data
+        c.chars_size = chars_cap
+        return (chars, offs), words, c
+    vals = np.zeros(max(n, 1), dtype=_NP[type_id])
+    c.data = vals.ctypes.data
     return vals, words, c
+
+
+def _chars_bound(in_cols):
+    """A string output is a slice or selection of one string input."""
+    return max([int(c.offsets[-1]) for c in in_cols if c.type_id == STRING and c.size] or [0])
+
+
+def _read_out(vals, words, n):
+    valid = _unpack(words, n)
+    if isinstance(vals, tuple):
+        chars, offs = vals
+        return [bytes(chars[offs[i]:offs[i + 1]]).decode() for i in range(n)], valid
+    return vals[:n], valid
 
 
 def execute(prog, in_cols, out_types, num_rows=None, threads=0, seed=0x5eed5eed12345678):
@@ -177,14 +199,14 @@ def execute(prog, in_cols, out_types, num_rows=None, threads=0, seed=0x5eed5eed1
     outs = (_Column * max(len(out_types), 1))()
     bufs = []
     for k, t in enumerate(out_types):
-        vals, words, c = _alloc_out(t, num_rows)
+        vals, words, c = _alloc_out(t, num_rows, _chars_bound(in_cols))
         outs[k] = c
         bufs.append((vals, words))
     rc = lib().oracle_execute(ctypes.byref(ph.c), ins, len(in_cols), outs, len(out_types), num_rows,
                               threads, seed)
     if rc != 0:
         raise RuntimeError(f"oracle_execute failed ({rc})")
-    return [(v[:num_rows], _unpack(w, num_rows)) for v, w in bufs]
+    return [_read_out(v, w, num_rows) for v, w in bufs]
 
 
 def op_types(prog, in_types):
@@ -207,7 +229,7 @@ def filter_project(pred, proj, in_cols, out_types, num_rows=None, threads=0, see
     outs = (_Column * max(len(out_types), 1))()
     bufs = []
     for k, t in enumerate(out_types):
-        vals, words, c = _alloc_out(t, num_rows)
+        vals, words, c = _alloc_out(t, num_rows, _chars_bound(in_cols))
         outs[k] = c
         bufs.append((vals, words))
     count = ctypes.c_int64()
@@ -216,7 +238,7 @@ def filter_project(pred, proj, in_cols, out_types, num_rows=None, threads=0, see
     if rc != 0:
         raise RuntimeError(f"oracle_filter_project failed ({rc})")
     n = count.value
-    return n, [(v[:n], _unpack(w, n)) for v, w in bufs]
+    return n, [_read_out(v, w, n) for v, w in bufs]
 
 
 def filter_table(pred, in_cols, num_rows=None, threads=0, seed=0x5eed5eed12345678):

@@ -37,6 +37,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
+#include <regex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -185,6 +187,60 @@ int64_t utf8_length(StrView s) {
   return n;
 }
 
+// UTF-8 <-> code points (the reference's string functions count characters)
+std::u32string utf8_decode(StrView s) {
+  std::u32string out;
+  for (int32_t i = 0; i < s.n;) {
+    unsigned char c = (unsigned char)s.p[i];
+    int len = c < 0x80 ? 1 : (c >> 5) == 6 ? 2 : (c >> 4) == 14 ? 3 : 4;
+    char32_t cp = len == 1 ? c : len == 2 ? (c & 0x1F) : len == 3 ? (c & 0x0F) : (c & 0x07);
+    for (int k = 1; k < len && i + k < s.n; ++k) cp = (cp << 6) | ((unsigned char)s.p[i + k] & 0x3F);
+    out.push_back(cp);
+    i += len;
+  }
+  return out;
+}
+std::string utf8_encode(const std::u32string& s) {
+  std::string out;
+  for (char32_t cp : s) {
+    if (cp < 0x80) out.push_back((char)cp);
+    else if (cp < 0x800) { out.push_back((char)(0xC0 | (cp >> 6))); out.push_back((char)(0x80 | (cp & 0x3F))); }
+    else if (cp < 0x10000) {
+      out.push_back((char)(0xE0 | (cp >> 12))); out.push_back((char)(0x80 | ((cp >> 6) & 0x3F)));
+      out.push_back((char)(0x80 | (cp & 0x3F)));
+    } else {
+      out.push_back((char)(0xF0 | (cp >> 18))); out.push_back((char)(0x80 | ((cp >> 12) & 0x3F)));
+      out.push_back((char)(0x80 | ((cp >> 6) & 0x3F))); out.push_back((char)(0x80 | (cp & 0x3F)));
+    }
+  }
+  return out;
+}
+
+// like_expression_to_regex_str (LogicalProject.cpp:31-47): '%' and '_' not
+// escaped by a backslash become (?:.*?) and (?:.); anchors unless the pattern
+// starts/ends with '%'. The regex is searched as cudf::strings::contains_re does.
+std::string like_to_regex(const std::string& like) {
+  if (like.empty()) return like;
+  const bool match_start = like.front() != '%';
+  const bool match_end = like.back() != '%';
+  static const std::regex any_string_re{R"(([^\\]?|\\{2})%)"};
+  static const std::regex any_char_re{R"(([^\\]?|\\{2})_)"};
+  std::string re = std::regex_replace(like, any_string_re, "$1(?:.*?)");
+  re = std::regex_replace(re, any_char_re, "$1(?:.)");
+  return (match_start ? "^" : "") + re + (match_end ? "$" : "");
+}
+std::wstring to_wide(const std::u32string& s) { return std::wstring(s.begin(), s.end()); }
+
+// slice_strings on characters [start, stop) with stop < 0 meaning the end
+std::string slice_chars(StrView s, int64_t start, int64_t stop) {
+  std::u32string cps = utf8_decode(s);
+  const int64_t n = (int64_t)cps.size();
+  if (stop < 0 || stop > n) stop = n;
+  if (start < 0) start = 0;
+  if (start >= stop) return std::string();
+  return utf8_encode(cps.substr((size_t)start, (size_t)(stop - start)));
+}
+
 // ------------------------------------------------------------------- program
 struct Prog {
   std::vector<slot_t> left, right, out, final_out;
@@ -235,6 +291,7 @@ int out_type_binary(int op, int lt, int rt) {  // expression_utils.cpp:201-266
     case BSQL_OP_POW: case BSQL_OP_ROUND: return type_size(lt) <= 4 ? BSQL_FLOAT32 : BSQL_FLOAT64;
     case BSQL_OP_MAGIC_IF_NOT: return rt;
     case BSQL_OP_FIRST_NON_MAGIC: return type_size(lt) >= type_size(rt) ? lt : rt;
+    case BSQL_OP_STR_SUBSTRING: case BSQL_OP_STR_LEFT: case BSQL_OP_STR_RIGHT: return BSQL_STRING;
   }
   return BSQL_EMPTY;  // IS_NOT_DISTINCT_FROM and others fall to the default branch
 }
@@ -364,6 +421,11 @@ double philox_uniform(uint64_t seed, uint64_t row) {
 struct RowCtx {
   int64_t buf[512];
   uint64_t valids[8];  // bit per slot (512 slots)
+  std::string str[512];  // string results of SUBSTRING/LEFT/RIGHT
+  void clear() {
+    std::memset(buf, 0, sizeof(buf));
+    std::memset(valids, 0, sizeof(valids));
+  }
 };
 bool getv(const RowCtx& c, int s) { return (c.valids[s >> 6] >> (s & 63)) & 1u; }
 void setv(RowCtx& c, int s, bool v) {
@@ -375,7 +437,25 @@ struct Exec {
   const Prog* p;
   Table in;
   uint64_t seed;
+  std::vector<std::shared_ptr<std::wregex>> like_re;  // per op: LIKE's regex
 };
+
+void prepare_like(Exec& ex) {
+  ex.like_re.assign(ex.p->ops.size(), nullptr);
+  for (size_t i = 0; i < ex.p->ops.size(); ++i) {
+    if (ex.p->ops[i] != BSQL_OP_STR_LIKE || ex.p->right[i] != BSQL_SCALAR_INDEX) continue;
+    const bsql_scalar& pat = ex.p->rsc[i];
+    std::string like(pat.str, (size_t)pat.str_len);
+    std::string re = like_to_regex(like);
+    ex.like_re[i] = std::make_shared<std::wregex>(to_wide(utf8_decode(StrView{re.data(), (int32_t)re.size()})));
+  }
+}
+
+// string operand: an input column, or a string an earlier op produced
+StrView get_str(const Exec& ex, const RowCtx& c, int pos, int64_t row) {
+  if (pos < ex.in.n) return col_str(ex.in.cols[pos], row);
+  return StrView{c.str[pos].data(), (int32_t)c.str[pos].size()};
+}
 
 void store(RowCtx& c, int s, const Val& v) { c.buf[s] = v.is_f ? bits_of(v.d) : v.i; }
 void store_i(RowCtx& c, int s, int64_t v) { c.buf[s] = v; }
@@ -425,7 +505,7 @@ void eval_op(const Exec& ex, size_t i, int64_t row, RowCtx& c) {
     StrView ls{nullptr, 0}, rs{nullptr, 0};
     bool lv, rv;
     if (lpos >= 0) {
-      if (is_string_type(lt)) ls = col_str(ex.in.cols[lpos], row); else L = slot_val(c, lpos, lf);
+      if (is_string_type(lt)) ls = get_str(ex, c, lpos, row); else L = slot_val(c, lpos, lf);
       lv = getv(c, lpos);
     } else if (lpos == BSQL_SCALAR_INDEX) {
       if (is_string_type(lt)) ls = StrView{p.lsc[i].str, p.lsc[i].str_len}; else L = scalar_val(p.lsc[i], lf);
@@ -434,7 +514,7 @@ void eval_op(const Exec& ex, size_t i, int64_t row, RowCtx& c) {
       lv = false;
     }
     if (rpos >= 0) {
-      if (is_string_type(rt)) rs = col_str(ex.in.cols[rpos], row); else R = slot_val(c, rpos, rf);
+      if (is_string_type(rt)) rs = get_str(ex, c, rpos, row); else R = slot_val(c, rpos, rf);
       rv = getv(c, rpos);
     } else if (rpos == BSQL_SCALAR_INDEX) {
       if (is_string_type(rt)) rs = StrView{p.rsc[i].str, p.rsc[i].str_len}; else R = scalar_val(p.rsc[i], rf);
@@ -458,6 +538,26 @@ void eval_op(const Exec& ex, size_t i, int64_t row, RowCtx& c) {
       else if (lv) { store(c, opos, L); setv(c, opos, truthy(L)); }
       else if (rv) { store(c, opos, R); setv(c, opos, true); }
       else setv(c, opos, false);
+    } else if (op == BSQL_OP_STR_SUBSTRING || op == BSQL_OP_STR_LEFT || op == BSQL_OP_STR_RIGHT) {
+      // positions as LogicalProject.cpp:183-306 prepares them for
+      // slice_strings: a literal SUBSTRING start is max(s, 1) - 1, a computed
+      // one s - 1; a null position reads as 0; LEFT/RIGHT take max(n, 0)
+      const int64_t v = rv ? (rf ? f64_to_i64(R.d) : R.i) : 0;
+      std::string res;
+      if (lv) {
+        if (op == BSQL_OP_STR_SUBSTRING) {
+          const int64_t start = rpos == BSQL_SCALAR_INDEX ? std::max<int64_t>(v, 1) - 1 : (rv ? v - 1 : 0);
+          res = slice_chars(ls, start, -1);
+        } else if (op == BSQL_OP_STR_LEFT) {
+          res = slice_chars(ls, 0, std::max<int64_t>(v, 0));
+        } else {
+          const int64_t n = std::max<int64_t>(v, 0);
+          const int64_t len = (int64_t)utf8_decode(ls).size();
+          res = n == 0 ? std::string() : slice_chars(ls, std::max<int64_t>(len - n, 0), -1);
+        }
+      }
+      c.str[opos] = res;
+      setv(c, opos, lv);
     } else if (op == BSQL_OP_IS_NOT_DISTINCT_FROM) {
       bool res;
       if (!lv && !rv) res = true;
@@ -469,6 +569,11 @@ void eval_op(const Exec& ex, size_t i, int64_t row, RowCtx& c) {
       bool valid = true;
       switch (op) {
         case BSQL_OP_LOGICAL_AND: store_i(c, opos, (truthy(L) && truthy(R)) ? 1 : 0); break;
+        case BSQL_OP_STR_LIKE: {
+          const std::wstring w = to_wide(utf8_decode(ls));
+          store_i(c, opos, std::regex_search(w, *ex.like_re[i]) ? 1 : 0);
+          break;
+        }
         case BSQL_OP_ADD:
           if (!lf && !rf) store_i(c, opos, wrap_add(L.i, R.i)); else store_f(c, opos, as_f(L) + as_f(R));
           break;
@@ -517,7 +622,7 @@ void eval_op(const Exec& ex, size_t i, int64_t row, RowCtx& c) {
     // ---------------- unary (interpreter_ops.cuh:728-917)
     Val L{lf, 0, 0.0};
     StrView ls{nullptr, 0};
-    if (is_string_type(lt)) ls = col_str(ex.in.cols[lpos], row); else L = slot_val(c, lpos, lf);
+    if (is_string_type(lt)) ls = get_str(ex, c, lpos, row); else L = slot_val(c, lpos, lf);
     bool lv = getv(c, lpos);
     if (op == BSQL_OP_IS_NOT_TRUE) {
       bool v = (lv && (lf ? L.d == 1.0 : L.i == 1)) ? false : true;
@@ -634,27 +739,58 @@ bool load_prog(const bsql_scalar* lsc, const bsql_scalar* rsc, const int16_t* le
   return true;
 }
 
-void execute_rows(const Prog& p, const bsql_column* in, int n_in, bsql_column* out, int n_out,
+// String outputs (string-valued expression roots, as evaluate_expressions
+// returns them) are assembled after the row pass: offsets then chars, in row
+// order; a null row has length 0. Returns false if a chars buffer is short.
+bool assemble_strings(bsql_column& out, const std::vector<std::string>& vals, int64_t num_rows) {
+  int64_t pos = 0;
+  out.offsets[0] = 0;
+  for (int64_t r = 0; r < num_rows; ++r) {
+    const std::string& v = vals[(size_t)r];
+    if (pos + (int64_t)v.size() > out.chars_size) return false;
+    if (!v.empty()) std::memcpy((char*)out.data + pos, v.data(), v.size());
+    pos += (int64_t)v.size();
+    out.offsets[r + 1] = (int32_t)pos;
+  }
+  return true;
+}
+
+bool execute_rows(const Prog& p, const bsql_column* in, int n_in, bsql_column* out, int n_out,
                   int64_t num_rows, int threads, uint64_t seed) {
-  Exec ex{&p, Table{in, n_in}, seed};
+  Exec ex{&p, Table{in, n_in}, seed, {}};
+  prepare_like(ex);
+  std::vector<std::vector<std::string>> strs((size_t)n_out);
+  for (int k = 0; k < n_out; ++k)
+    if (out[k].type_id == BSQL_STRING) strs[(size_t)k].assign((size_t)num_rows, std::string());
   parallel_chunks(num_rows, threads, [&](int, int64_t b, int64_t e) {
     RowCtx c;
     for (int64_t w = b; w < e; w += 32) {
-      std::memset(&c, 0, sizeof(c));
+      c.clear();
       uint32_t vw[64] = {0};
       int64_t we = std::min(e, w + 32);
       for (int64_t row = w; row < we; ++row) {
         eval_row(ex, row, c);
         for (int k = 0; k < n_out; ++k) {
           int fs = p.final_out[k];
-          write_elem(out[k], row, c.buf[fs]);
-          if (getv(c, fs)) vw[k] |= 1u << (row - w);
+          const bool v = getv(c, fs);
+          if (out[k].type_id == BSQL_STRING) {
+            if (v) {
+              StrView sv = get_str(ex, c, fs, row);
+              strs[(size_t)k][(size_t)row].assign(sv.p, (size_t)sv.n);
+            }
+          } else {
+            write_elem(out[k], row, c.buf[fs]);
+          }
+          if (v) vw[k] |= 1u << (row - w);
         }
       }
       for (int k = 0; k < n_out; ++k)
         if (out[k].validity) out[k].validity[w >> 5] = vw[k];
     }
   });
+  for (int k = 0; k < n_out; ++k)
+    if (out[k].type_id == BSQL_STRING && !assemble_strings(out[k], strs[(size_t)k], num_rows)) return false;
+  return true;
 }
 
 void gather_column(const bsql_column& src, bsql_column& dst, const std::vector<int64_t>& rows) {
@@ -687,11 +823,12 @@ void gather_column(const bsql_column& src, bsql_column& dst, const std::vector<i
 std::vector<int64_t> select_rows(const Prog& pred, const bsql_column* in, int n_in,
                                  int64_t num_rows, int threads, uint64_t seed) {
   std::vector<uint8_t> mask((size_t)num_rows);
-  Exec ex{&pred, Table{in, n_in}, seed};
+  Exec ex{&pred, Table{in, n_in}, seed, {}};
+  prepare_like(ex);
   const int fs = pred.final_out[0];
   parallel_chunks(num_rows, threads, [&](int, int64_t b, int64_t e) {
     RowCtx c;
-    std::memset(&c, 0, sizeof(c));
+    c.clear();
     for (int64_t row = b; row < e; ++row) {
       eval_row(ex, row, c);
       // the BOOL8 output is static_cast<bool>(slot) and a null drops the row
@@ -736,7 +873,7 @@ int oracle_execute(const oracle_program* prog, const bsql_column* in, int n_in,
   Prog p;
   if (!build(prog, in, n_in, &p)) return BSQL_ERR_INVALID;
   if ((int)p.final_out.size() != n_out) return BSQL_ERR_INVALID;
-  execute_rows(p, in, n_in, out, n_out, num_rows, threads, seed);
+  if (!execute_rows(p, in, n_in, out, n_out, num_rows, threads, seed)) return BSQL_ERR_INVALID;
   return BSQL_OK;
 }
 
@@ -795,6 +932,7 @@ int oracle_filter_project(const oracle_program* pred, const oracle_program* proj
       offs[k].resize((size_t)n + 1);
       fin[k].data = chars[k].data();
       fin[k].offsets = offs[k].data();
+      fin[k].chars_size = (int32_t)total;
     } else {
       data[k].resize((size_t)(n * type_size(in[k].type_id)) + 8);
       fin[k].data = data[k].data();
@@ -805,7 +943,7 @@ int oracle_filter_project(const oracle_program* pred, const oracle_program* proj
     }
     gather_column(in[k], fin[k], rows);
   }
-  execute_rows(pj, fin.data(), n_in, out, n_out, n, threads, seed);
+  if (!execute_rows(pj, fin.data(), n_in, out, n_out, n, threads, seed)) return BSQL_ERR_INVALID;
   return BSQL_OK;
 }
 

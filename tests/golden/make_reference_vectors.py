@@ -134,6 +134,31 @@ vectors["char_length"] = {
     "expected": [2, 4, -1, 3], "expected_valid": [1, 1, 0, 1],
 }
 
+# process_project.cpp:286-303 -- LIKE with '_' and a leading '%'
+vectors["string_like"] = {
+    "source": "engine/tests/new-tests/process_project.cpp:286-303",
+    "col1": ["foo", "d", "e", "a", "hello", "k", "d", "l", "bar", ""],
+    "query": "LogicalProject(EXPR$0=[LIKE($0, '_')], EXPR$1=[LIKE($0, '%o')])",
+    "expected_col1": [0, 1, 1, 1, 0, 1, 1, 1, 0, 0],
+    "expected_col2": [1, 0, 0, 0, 1, 0, 0, 0, 0, 0],
+}
+
+# process_project.cpp:305-321 -- SUBSTRING with a negative literal length (to the end)
+vectors["string_substring"] = {
+    "source": "engine/tests/new-tests/process_project.cpp:305-321",
+    "col1": ["The", "quick", "brown", "fox", "jumps", "over", "the", "lazy", "dog"],
+    "query": "LogicalProject(EXPR$0=[SUBSTRING($0, 3, -1)])",
+    "expected": ["e", "ick", "own", "x", "mps", "er", "e", "zy", "g"],
+}
+
+# process_project.cpp:323-342 -- SUBSTRING with a computed length and a null row
+vectors["string_substring_null"] = {
+    "source": "engine/tests/new-tests/process_project.cpp:323-342",
+    "col1": ["ABC", "123", "", "XYZ"], "valid": [1, 1, 0, 1],
+    "query": "LogicalProject(EXPR$0=[SUBSTRING($0, 2, -(CHAR_LENGTH($0), 2))])",
+    "expected": ["B", "2", "", "Y"], "expected_valid": [1, 1, 0, 1],
+}
+
 # process_project.cpp:489-513 -- ROUND with 0, 2 and 5 digits (float and double)
 vectors["round"] = {
     "source": "engine/tests/new-tests/process_project.cpp:489-513",

@@ -1,0 +1,132 @@
+"""GPU parity for strings: LIKE, SUBSTRING/LEFT/RIGHT (as slices in the
+interpreter), CHAR_LENGTH and comparisons, string projection outputs, and
+string columns through filter compaction -- against the reference's golden
+vectors and the oracle, bit-exact."""
+import numpy as np
+import pytest
+
+from tests.helpers import FLOAT64, INT32, INT64, assert_same
+
+pytestmark = pytest.mark.gpu
+
+STRING, BOOL8 = 23, 11
+
+
+def _run(bsql, exprs, cols, types, n):
+    """Compiles and runs on the GPU; returns [(values, valid)] per output."""
+    from blazingsql_amd.interop import DeviceColumn, perform_interpreter_operation
+    p = bsql.Program.compile(exprs, types)
+    ot = [p.output_type(types, k) for k in range(p.num_outputs)]
+    chars = max([c.chars_size for c in cols if c.type_id == STRING] or [0])
+    outs = [DeviceColumn.empty(t, n, chars=chars) for t in ot]
+    perform_interpreter_operation(p, cols, outs, n)
+    return p, ot, [o.to_numpy(n) for o in outs]
+
+
+def _oracle(bsql, p, host_cols, ot, n):
+    import oracle
+    used = p.used_columns()
+    return oracle.execute(p.to_oracle_dict(), [host_cols[c] for c in used], ot, n)
+
+
+def _same(g, o, what):
+    (gv, gval), (ov, oval) = g, o
+    np.testing.assert_array_equal(gval, oval, err_msg=what)
+    if isinstance(gv, list):
+        assert [a for a, ok in zip(gv, gval) if ok] == [b for b, ok in zip(ov, oval) if ok], what
+    else:
+        assert_same(gv, gval, ov, oval, what=what)
+
+
+def test_golden_like(bsql, gpu, golden):
+    from blazingsql_amd.interop import DeviceColumn
+    g = golden["string_like"]
+    col = DeviceColumn.from_strings(g["col1"])
+    _, ot, res = _run(bsql, ["LIKE($0, '_')", "LIKE($0, '%o')"], [col], [STRING], len(g["col1"]))
+    assert ot == [BOOL8, BOOL8]
+    np.testing.assert_array_equal(res[0][0].astype(int), g["expected_col1"])
+    np.testing.assert_array_equal(res[1][0].astype(int), g["expected_col2"])
+
+
+def test_golden_substring(bsql, gpu, golden):
+    from blazingsql_amd.interop import DeviceColumn
+    g = golden["string_substring"]
+    col = DeviceColumn.from_strings(g["col1"])
+    _, ot, res = _run(bsql, ["SUBSTRING($0, 3, -1)"], [col], [STRING], len(g["col1"]))
+    assert ot == [STRING] and res[0][0] == g["expected"] and res[0][1].all()
+
+
+def test_golden_substring_computed_length(bsql, gpu, golden):
+    from blazingsql_amd.interop import DeviceColumn
+    g = golden["string_substring_null"]
+    valid = np.asarray(g["valid"], dtype=bool)
+    col = DeviceColumn.from_strings(g["col1"], valid)
+    _, _, res = _run(bsql, ["SUBSTRING($0, 2, -(CHAR_LENGTH($0), 2))"], [col], [STRING], len(g["col1"]))
+    vals, v = res[0]
+    np.testing.assert_array_equal(v, np.asarray(g["expected_valid"], dtype=bool))
+    assert [a for a, ok in zip(vals, v) if ok] == [e for e, ok in zip(g["expected"], g["expected_valid"]) if ok]
+
+
+def _string_table(rng, n, alphabet, lo=0, hi=12, null_rate=0.1):
+    strs = ["".join(rng.choice(alphabet, int(rng.integers(lo, hi + 1)))) for _ in range(n)]
+    valid = rng.random(n) >= null_rate
+    return strs, valid
+
+
+@pytest.mark.parametrize("n", [1, 1000, 70_001])
+def test_string_functions_vs_oracle(bsql, gpu, n):
+    """Nested slices, LIKE on slices, CHAR_LENGTH, comparisons with literals and
+    between columns, string outputs, multi-byte characters, nulls."""
+    import oracle
+    from blazingsql_amd.interop import DeviceColumn
+    rng = np.random.default_rng(n)
+    alphabet = list("abcab") + ["é", "ß", "%", "_"]
+    s0, v0 = _string_table(rng, n, alphabet)
+    s1, v1 = _string_table(rng, n, list("abc"), 0, 4, 0.05)
+    k = rng.integers(-3, 8, n).astype(np.int32)
+    kv = rng.random(n) >= 0.1
+    types = [STRING, STRING, INT32]
+    dev = [DeviceColumn.from_strings(s0, v0), DeviceColumn.from_strings(s1, v1),
+           DeviceColumn.from_numpy(INT32, k, kv)]
+    host = [oracle.HostColumn(STRING, strings=s0, valid=v0), oracle.HostColumn(STRING, strings=s1, valid=v1),
+            oracle.HostColumn(INT32, k, kv)]
+    exprs = ["LIKE($0, 'a%')", "LIKE($0, '%b_')", "LIKE(SUBSTRING($0, 2), '%é%')", "LIKE($0, '__')",
+             "LIKE($0, '%%')", "CHAR_LENGTH($0)", "CHAR_LENGTH(SUBSTRING($0, $2))",
+             "SUBSTRING($0, 2, 3)", "SUBSTRING($0, $2, 2)", "SUBSTRING($0, 2, $2)", "LEFT($0, 3)",
+             "RIGHT($0, 2)", "LEFT(RIGHT($0, 4), 2)", "=(SUBSTRING($0, 1, 2), 'ab')", ">($0, $1)",
+             "<=(SUBSTRING($0, 2), $1)", "$1", "SUBSTRING($1, 2)"]
+    for e in exprs:
+        p, ot, res = _run(bsql, [e], dev, types, n)
+        ores = _oracle(bsql, p, host, ot, n)
+        _same(res[0], ores[0], e)
+
+
+@pytest.mark.parametrize("ranges", [True, False])
+def test_filter_with_string_columns_vs_oracle(bsql, gpu, ranges):
+    """process_filter over a table with string columns: the strings are
+    compacted with the rest (apply_boolean_mask), nulls kept."""
+    import oracle
+    import torch
+    from blazingsql_amd.interop import DeviceColumn, filter_columns
+    from tests.test_gpu_parity import _full_prog
+    n = 150_003
+    rng = np.random.default_rng(5)
+    s0, v0 = _string_table(rng, n, list("abcdé"), 0, 20, 0.1)
+    x = rng.integers(-1000, 1000, n).astype(np.int64)
+    y = rng.uniform(-1, 1, n)
+    types = [STRING, INT64, FLOAT64]
+    dev = [DeviceColumn.from_strings(s0, v0), DeviceColumn.from_numpy(INT64, x), DeviceColumn.from_numpy(FLOAT64, y)]
+    host = [oracle.HostColumn(STRING, strings=s0, valid=v0), oracle.HostColumn(INT64, x), oracle.HostColumn(FLOAT64, y)]
+    for pred_text in ["AND(>($1, 0), <($2, 0.5:DECIMAL(2, 1)))", "LIKE($0, 'a%')",
+                      "AND(>(CHAR_LENGTH($0), 5), <($1, 500))"]:
+        pred = bsql.Program.compile(pred_text, types)
+        pred.set_launch(ranges=ranges)
+        outs = [DeviceColumn.empty(STRING, n, chars=dev[0].chars_size), DeviceColumn.empty(INT64, n, nullable=False),
+                DeviceColumn.empty(FLOAT64, n, nullable=False)]
+        cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+        filter_columns(pred, dev, outs, cnt, n)
+        m = int(cnt.item())
+        on, ores = oracle.filter_table(_full_prog(bsql, pred_text, types), host, n)
+        assert m == on, pred_text
+        for k in range(3):
+            _same(outs[k].to_numpy(m), ores[k], f"{pred_text} col {k}")

@@ -262,3 +262,75 @@ def test_type_propagation(oracle_lib):
     p = prog([0, 3, 3], [1, -2, 2], [3, 3, 4], ["ADD", "MUL", "LESS"], [4], right_scalars=[None, (10, 2.0), None])
     lt, rt, ot = oracle_lib.op_types(p, [3, 4, 10])
     assert (lt, rt, ot) == ([3, 4, 10], [4, 10, 10], [4, 10, 11])
+
+
+def _compiled(bsql, exprs, types):
+    p = bsql.Program.compile(exprs, types)
+    return p, [p.output_type(types, k) for k in range(p.num_outputs)]
+
+
+def test_string_like(oracle_lib, golden, bsql):
+    """LIKE through the reference's regex translation and search."""
+    g = golden["string_like"]
+    from oracle import HostColumn
+    p, ot = _compiled(bsql, ["LIKE($0, '_')", "LIKE($0, '%o')"], [23])
+    res = oracle_lib.execute(p.to_oracle_dict(), [HostColumn(23, strings=g["col1"])], ot)
+    np.testing.assert_array_equal(res[0][0].astype(int), g["expected_col1"])
+    np.testing.assert_array_equal(res[1][0].astype(int), g["expected_col2"])
+
+
+def test_string_substring(oracle_lib, golden, bsql):
+    g = golden["string_substring"]
+    from oracle import HostColumn
+    p, ot = _compiled(bsql, ["SUBSTRING($0, 3, -1)"], [23])
+    (vals, valid), = oracle_lib.execute(p.to_oracle_dict(), [HostColumn(23, strings=g["col1"])], ot)
+    assert ot == [23] and vals == g["expected"] and valid.all()
+
+
+def test_string_substring_computed_length_null(oracle_lib, golden, bsql):
+    g = golden["string_substring_null"]
+    from oracle import HostColumn
+    p, ot = _compiled(bsql, ["SUBSTRING($0, 2, -(CHAR_LENGTH($0), 2))"], [23])
+    col = HostColumn(23, strings=g["col1"], valid=[bool(x) for x in g["valid"]])
+    (vals, valid), = oracle_lib.execute(p.to_oracle_dict(), [col], ot)
+    np.testing.assert_array_equal(valid, np.asarray(g["expected_valid"], dtype=bool))
+    assert [v for v, ok in zip(vals, valid) if ok] == [e for e, ok in zip(g["expected"], g["expected_valid"]) if ok]
+
+
+def reference_like_regex(pat):
+    """like_expression_to_regex_str (LogicalProject.cpp:31-47) written with
+    Python's re: the same two left-to-right replacements and anchors."""
+    import re
+    if not pat:
+        return pat
+    rx = re.sub(r"([^\\]?|\\{2})%", lambda m: m.group(1) + "(?:.*?)", pat)
+    rx = re.sub(r"([^\\]?|\\{2})_", lambda m: m.group(1) + "(?:.)", rx)
+    # '$' is written \Z: Python's '$' also matches before a final newline,
+    # the ECMAScript '$' of the oracle's regex engine only at the end
+    return ("^" if pat[0] != "%" else "") + rx + ("\\Z" if pat[-1] != "%" else "")
+
+
+def test_like_and_slices_closed_forms(oracle_lib, bsql):
+    """LIKE against Python's regex engine running the reference's pattern
+    translation (doubled wildcards keep their quirk: '__' is '^_(?:.)$',
+    '%%' finds a '%'), on random strings with multi-byte characters and
+    newlines; SUBSTRING/LEFT/RIGHT against Python slicing."""
+    import re
+    from oracle import HostColumn
+    rng = np.random.default_rng(9)
+    alphabet = list("abcd_%") + ["é", "ß", "\n"]
+    strs = ["".join(rng.choice(alphabet, int(rng.integers(0, 9)))) for _ in range(600)]
+    col = HostColumn(23, strings=strs)
+    for pat in ["a%", "%b", "%ab%", "_a%", "a_b", "%", "%é%", "__", "a%b%c", "ß_%", "%%", "%%a", "_%_",
+                "a__", "%_%", "", "%a%b", "é"]:
+        p, ot = _compiled(bsql, [f"LIKE($0, '{pat}')"], [23])
+        (vals, _), = oracle_lib.execute(p.to_oracle_dict(), [col], ot)
+        rx = re.compile(reference_like_regex(pat))
+        expect = [1 if rx.search(s) else 0 for s in strs]
+        np.testing.assert_array_equal(vals.astype(int), expect, err_msg=repr(pat))
+    for expr, fn in [("SUBSTRING($0, 2)", lambda s: s[1:]), ("SUBSTRING($0, 0)", lambda s: s),
+                     ("SUBSTRING($0, 3, 2)", lambda s: s[2:4]), ("LEFT($0, 3)", lambda s: s[:3]),
+                     ("RIGHT($0, 2)", lambda s: s[-2:] if len(s) >= 2 else s), ("RIGHT($0, 0)", lambda s: "")]:
+        p, ot = _compiled(bsql, [expr], [23])
+        (vals, _), = oracle_lib.execute(p.to_oracle_dict(), [col], ot)
+        assert vals == [fn(s) for s in strs], expr
