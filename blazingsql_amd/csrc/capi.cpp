@@ -351,6 +351,10 @@ int bsql_program_output_type(const bsql_program* p, const int32_t* in_types, int
     std::vector<int32_t> st = slot_types_for(*p, in_types, n_in);
     OpTypes t = propagate_types(*p, st.data(), (int)st.size());
     const int slot = p->final_out[k];
+    if (slot < 0) {  // literal root
+      *type_id = p->root_literals[(size_t)k].type_id;
+      return BSQL_OK;
+    }
     int32_t ty = slot < (int)st.size() ? st[(size_t)slot] : BSQL_EMPTY;
     for (size_t i = 0; i < p->ops.size(); ++i)
       if (p->out[i] == slot) ty = t.ot[i];

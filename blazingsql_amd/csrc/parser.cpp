@@ -729,8 +729,6 @@ std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exp
     Parser parser(e);
     std::unique_ptr<Node> t = rewrite_substring(transform_custom(parser.parse()));
     check_interpretable(t.get());
-    if (t->type == NodeType::Literal)
-      fail("literal expression: the operator layer fills it without the interpreter");
     std::vector<int> vars;
     collect_vars(t.get(), vars);
     for (int v : vars) {
@@ -750,11 +748,25 @@ std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exp
   const int n_inputs = (int)col_map.size();
   const int n_exprs = (int)trees.size();
   prog->n_inputs = n_inputs;
+  prog->root_literals.assign((size_t)n_exprs, Scalar());
   for (int i = 0; i < n_exprs; ++i) {
     const int final_pos = n_inputs + i;
     prog->final_out.push_back((int16_t)final_pos);
     const Node* root = trees[(size_t)i].get();
     if (root->type == NodeType::Variable) continue;  // pass-through: final slot = input slot
+    if (root->type == NodeType::Literal) {
+      // filled with the literal, as make_column_from_scalar does
+      if (is_null_literal(root->value)) {
+        prog->final_out.back() = BSQL_SCALAR_NULL_INDEX;
+        prog->root_literals[(size_t)i].type_id = root->lit_type;
+      } else {
+        prog->final_out.back() = BSQL_SCALAR_INDEX;
+        prog->root_literals[(size_t)i] = scalar_from_string(root->value, root->lit_type);
+      }
+      if (is_string_type(prog->root_literals[(size_t)i].type_id))
+        fail("a string literal expression is not produced by the interpreter");
+      continue;
+    }
     Encoder enc(*prog, col_map, n_inputs + n_exprs);
     enc.visit(root);
     prog->out.back() = (int16_t)final_pos;

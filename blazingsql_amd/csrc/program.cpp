@@ -161,7 +161,12 @@ void validate(const Program& p, int n_in) {
       if (l < BSQL_SCALAR_NULL_INDEX || r < BSQL_SCALAR_NULL_INDEX) fail("invalid operand index");
     }
   }
-  for (int16_t f : p.final_out) mx = std::max<int>(mx, f);
+  for (size_t k = 0; k < p.final_out.size(); ++k) {
+    const int16_t f = p.final_out[k];
+    if (f < 0 && !((f == BSQL_SCALAR_INDEX || f == BSQL_SCALAR_NULL_INDEX) && k < p.root_literals.size()))
+      fail("output " + std::to_string(k) + " has no slot");
+    mx = std::max<int>(mx, f);
+  }
   if (mx > BSQL_MAX_SLOT)
     fail("Interops does not support plans with an input or output index greater than 63");
   (void)n_in;
@@ -557,7 +562,21 @@ SsaProgram to_ssa(Lowering& L, const Program& p, uint64_t seed) {
     s.ops.push_back(so);
   }
   float_operands(L, s);
-  for (int16_t f : p.final_out) {
+  for (size_t k = 0; k < p.final_out.size(); ++k) {
+    const int16_t f = p.final_out[k];
+    if (f < 0) {
+      // literal root: an identity conversion of the literal (null: no value)
+      const Scalar& lit = p.root_literals[k];
+      SourceOp c;
+      c.prog_op = -1;
+      c.d.code = is_float_type(lit.type_id) ? D_TO_F64 : D_TO_I64;
+      c.d.flags = (f == BSQL_SCALAR_INDEX ? F_A_IMM : F_A_NULL) | (is_float_type(lit.type_id) ? F_A_F : 0);
+      c.d.imm = f == BSQL_SCALAR_INDEX ? scalar_bits(lit) : 0;
+      c.vdst = L.new_value();
+      s.ops.push_back(c);
+      s.finals.push_back(c.vdst);
+      continue;
+    }
     int v = f < n_slots_in && slot_val[f] < 0 ? L.load_of(col_of[(size_t)f]) : slot_val[f];
     if (v < 0) fail("output slot " + std::to_string(f) + " is never written");
     s.finals.push_back(v);
@@ -587,7 +606,7 @@ void subtree(const SsaProgram& s, const std::vector<int>& prod, int v, std::set<
 void conjuncts_of(const Program& p, const SsaProgram& s, const std::vector<int>& prod, int v,
                   std::vector<int>& acc) {
   int o = v >= 0 ? prod[v] : -1;
-  if (o >= 0 && p.ops[s.ops[o].prog_op] == BSQL_OP_LOGICAL_AND && s.ops[o].va >= 0 &&
+  if (o >= 0 && s.ops[o].prog_op >= 0 && p.ops[s.ops[o].prog_op] == BSQL_OP_LOGICAL_AND && s.ops[o].va >= 0 &&
       s.ops[o].vb >= 0) {
     conjuncts_of(p, s, prod, s.ops[o].va, acc);
     conjuncts_of(p, s, prod, s.ops[o].vb, acc);
@@ -735,9 +754,11 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
     if (pred.final_out.size() != 1) fail("a filter predicate must have exactly one output");
     SsaProgram s = to_ssa(L, pred, pred.seed);
     const OpTypes& t = s.types;
-    int root_type = pred.final_out[0] < (int)s.slot_types.size() ? s.slot_types[pred.final_out[0]] : BSQL_EMPTY;
-    for (size_t i = 0; i < pred.ops.size(); ++i)
-      if (pred.out[i] == pred.final_out[0]) root_type = t.ot[i];
+    const int root = pred.final_out[0];
+    int root_type = root < 0 ? pred.root_literals[0].type_id
+                    : root < (int)s.slot_types.size() ? s.slot_types[(size_t)root] : BSQL_EMPTY;
+    for (size_t i = 0; i < pred.ops.size() && root >= 0; ++i)
+      if (pred.out[i] == root) root_type = t.ot[i];
     if (root_type != BSQL_BOOL8)
       fail("Expression does not evaluate to a boolean mask");
     std::vector<int> prod = producers(s, L.values.size());
@@ -784,7 +805,9 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
         k.va = conj[(size_t)j];
         // truthiness follows the kind of the conjunct's value
         int o = prod[(size_t)k.va];
-        int ty = o >= 0 ? t.ot[(size_t)s.ops[(size_t)o].prog_op] : L.in_types[L.values[(size_t)k.va].column];
+        int ty = o < 0 ? L.in_types[L.values[(size_t)k.va].column]
+                 : s.ops[(size_t)o].prog_op >= 0 ? t.ot[(size_t)s.ops[(size_t)o].prog_op]
+                 : (s.ops[(size_t)o].d.code == D_TO_F64 ? BSQL_FLOAT64 : BSQL_INT64);
         if (is_float_type(ty)) k.d.flags |= F_A_F;
         L.emit(k);
       }

@@ -40,6 +40,10 @@ struct Program {
   std::vector<int16_t> left, right, out, final_out;
   std::vector<int32_t> ops;
   std::vector<Scalar> lsc, rsc;
+  // literal expression roots (evaluate_expressions fills their column without
+  // the interpreter, LogicalProject.cpp:822-832): final_out holds
+  // BSQL_SCALAR_INDEX or BSQL_SCALAR_NULL_INDEX and root_literals[k] the value
+  std::vector<Scalar> root_literals;
   // schema hints from the compiler: number of interpreter inputs and the
   // table columns they come from (col_idx_map of LogicalProject.cpp:859-881)
   int n_inputs = -1;

@@ -359,3 +359,32 @@ def test_range_predicates_vs_oracle(bsql, gpu, seed):
             for k in range(len(types)):
                 gv, gval = outs[k].to_numpy(m)
                 assert_same(gv, gval, ores[k][0], ores[k][1], what=f"{text} ranges={ranges} col {k}")
+
+
+def test_literal_roots(bsql, gpu):
+    """Literal expressions fill their column (make_column_from_scalar,
+    LogicalProject.cpp:822-832): every row the value, or every row null; a
+    literal predicate keeps all rows or none."""
+    import torch
+    from blazingsql_amd.interop import DeviceColumn, filter_columns, perform_interpreter_operation
+    n = 5000
+    x = np.arange(n, dtype=np.int64)
+    p = bsql.Program.compile(["5", "null:DOUBLE", "2.5:DOUBLE", "+($0, 1)", "true", "-7:BIGINT"], [INT64])
+    ot = [p.output_type([INT64], k) for k in range(6)]
+    assert ot == [INT32, FLOAT64, FLOAT64, INT64, BOOL8, INT64]
+    outs = [DeviceColumn.empty(t, n) for t in ot]
+    perform_interpreter_operation(p, [DeviceColumn.from_numpy(INT64, x)], outs, n)
+    vals = [o.to_numpy(n) for o in outs]
+    assert (vals[0][0] == 5).all() and vals[0][1].all()
+    assert not vals[1][1].any()
+    assert (vals[2][0] == 2.5).all() and vals[2][1].all()
+    np.testing.assert_array_equal(vals[3][0], x + 1)
+    assert (vals[4][0] == 1).all() and (vals[5][0] == -7).all()
+    for text, expect in (("true", n), ("false", 0)):
+        pred = bsql.Program.compile(text, [INT64])
+        out = [DeviceColumn.empty(INT64, n, nullable=False)]
+        cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+        filter_columns(pred, [DeviceColumn.from_numpy(INT64, x)], out, cnt, n)
+        assert int(cnt.item()) == expect
+        if expect:
+            np.testing.assert_array_equal(out[0].to_numpy(n)[0], x)
