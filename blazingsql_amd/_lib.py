@@ -63,6 +63,11 @@ class LaunchConfig(ctypes.Structure):
 LAUNCH_NO_RANGES, LAUNCH_LAZY_LOADS = 1, 2
 
 
+class BatchDesc(ctypes.Structure):
+    _fields_ = [("prog", ctypes.c_void_p), ("inp", ctypes.c_void_p), ("n_in", ctypes.c_int),
+                ("out", ctypes.c_void_p), ("n_out", ctypes.c_int), ("num_rows", ctypes.c_int64)]
+
+
 # exported symbols and their signatures, as declared in include/bsql/interp.h
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -84,6 +89,13 @@ SIGNATURES = {
     "bsql_program_num_conjuncts": (_I, [_P]),
     "bsql_program_set_conjunct_order": (_I, [_P, _P, _I]),
     "bsql_program_set_launch": (_I, [_P, ctypes.POINTER(LaunchConfig)]),
+    "bsql_batch_prepare": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "bsql_batch_launch": (_I, [_P, _P]),
+    "bsql_batch_set_launch": (_I, [_P, ctypes.POINTER(LaunchConfig)]),
+    "bsql_batch_autotune": (_I, [_P, _P, ctypes.POINTER(LaunchConfig)]),
+    "bsql_batch_total_rows": (_I64, [_P]),
+    "bsql_batch_free": (None, [_P]),
+    "bsql_program_autotune": (_I, [_P, _P, _I, _P, _I, _I64, _P, ctypes.POINTER(LaunchConfig)]),
     "bsql_last_error": (ctypes.c_char_p, []),
     "bsql_version": (_I, []),
 }

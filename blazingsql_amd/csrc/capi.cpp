@@ -2,7 +2,9 @@
 // representation and the gfx950 kernels.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
+#include <memory>
 #include <map>
 #include <mutex>
 #include <string>
@@ -22,6 +24,9 @@ hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
                           hipStream_t stream);
 int64_t compact_tiles(const DevPlan& plan, int nreg, int rpt);
 int64_t string_tiles(int64_t capacity);
+int batch_tile_rows(int nreg, int rpt);
+hipError_t launch_batches(const DevPlan* plans, const int64_t* tile_start, int n_batches, int64_t n_items,
+                          int nreg, int rpt, hipStream_t stream);
 hipError_t launch_string_output(const uint64_t* slices, const uint32_t* valid, int32_t* offsets,
                                 const uint8_t* src_chars, uint8_t* dst_chars, const int64_t* d_count,
                                 int64_t count, int64_t capacity, uint64_t* status, hipStream_t stream);
@@ -518,6 +523,204 @@ int bsql_filter_project(const bsql_program* pred, const bsql_program* proj, cons
     if (!proj) fail("null projection");
     return run_compact(const_cast<bsql_program*>(pred), const_cast<bsql_program*>(proj), in, n_in,
                        out, n_out, num_rows, num_rows, d_count, stream);
+  })
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+
+struct bsql_batch {
+  std::vector<DevPlan> plans;  // host copies (columns filled in)
+  std::vector<int64_t> rows;
+  int nreg = 1;
+  bsql_launch_config launch{0, 0, 0};
+  DevPlan* d_plans = nullptr;
+  int64_t* d_tile_start = nullptr;
+  int64_t n_items = 0;
+  ~bsql_batch() {
+    if (d_plans) (void)hipFree(d_plans);
+    if (d_tile_start) (void)hipFree(d_tile_start);
+  }
+};
+
+namespace {
+
+// Work map for the current launch shape: prefix counts of tiles per batch.
+hipError_t upload_work_map(bsql_batch* b) {
+  const int tile = batch_tile_rows(b->nreg, b->launch.rows_per_thread);
+  if (tile <= 0) fail("no kernel variant for these programs");
+  std::vector<int64_t> start(b->plans.size() + 1, 0);
+  for (size_t i = 0; i < b->plans.size(); ++i) start[i + 1] = start[i] + (b->rows[i] + tile - 1) / tile;
+  b->n_items = start.back();
+  if (!b->d_tile_start) {
+    hipError_t e = hipMalloc((void**)&b->d_tile_start, start.size() * sizeof(int64_t));
+    if (e != hipSuccess) return e;
+  }
+  return hipMemcpy(b->d_tile_start, start.data(), start.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsql_batch_prepare(const bsql_batch_desc* descs, int n, bsql_batch** out) {
+  BSQL_TRY({
+    if (!descs || n <= 0 || !out) fail("invalid arguments");
+    auto b = std::make_unique<bsql_batch>();
+    for (int i = 0; i < n; ++i) {
+      const bsql_batch_desc& d = descs[i];
+      bsql_program* prog = const_cast<bsql_program*>(d.prog);
+      if (!prog) fail("batch " + std::to_string(i) + " has no program");
+      check_inputs(d.in, d.n_in, d.num_rows);
+      if (d.n_out != (int)prog->final_out.size()) fail("batch " + std::to_string(i) + ": outputs do not match");
+      check_outputs(d.out, d.n_out, d.num_rows);
+      for (int k = 0; k < d.n_out; ++k)
+        if (d.out[k].type_id == BSQL_STRING) fail("batched launches write fixed-width outputs only");
+      hipError_t e = ensure_strings(*prog);
+      if (e != hipSuccess) return device_error(e, "bsql_batch_prepare");
+      LowerOptions opt;
+      opt.mode = LowerOptions::EXECUTE;
+      opt.proj = prog;
+      const Lowered& L = lowered_for(prog, opt, d.in, d.n_in, d.out, d.n_out);
+      DevPlan plan = L.plan;
+      plan.num_rows = d.num_rows;
+      for (int k = 0; k < d.n_in; ++k) plan.in[k] = to_dev_in(d.in[k]);
+      for (int k = 0; k < d.n_out; ++k) {
+        plan.out[k].data = d.out[k].data;
+        plan.out[k].valid = d.out[k].validity;
+        plan.out[k].type = d.out[k].type_id;
+      }
+      b->plans.push_back(plan);
+      b->rows.push_back(d.num_rows);
+      b->nreg = std::max(b->nreg, L.nreg);
+    }
+    hipError_t e = hipMalloc((void**)&b->d_plans, b->plans.size() * sizeof(DevPlan));
+    if (e == hipSuccess)
+      e = hipMemcpy(b->d_plans, b->plans.data(), b->plans.size() * sizeof(DevPlan), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = upload_work_map(b.get());
+    if (e != hipSuccess) return device_error(e, "bsql_batch_prepare");
+    *out = b.release();
+    return BSQL_OK;
+  })
+}
+
+int bsql_batch_launch(const bsql_batch* b, void* stream) {
+  BSQL_TRY({
+    if (!b) fail("null batch");
+    hipError_t e = launch_batches(b->d_plans, b->d_tile_start, (int)b->plans.size(), b->n_items, b->nreg,
+                                  b->launch.rows_per_thread, (hipStream_t)stream);
+    if (e != hipSuccess) return device_error(e, "bsql_batch_launch");
+    return BSQL_OK;
+  })
+}
+
+int bsql_batch_set_launch(bsql_batch* b, const bsql_launch_config* cfg) {
+  BSQL_TRY({
+    if (!b || !cfg) fail("invalid arguments");
+    const int r = cfg->rows_per_thread;
+    if (r != 0 && r != 1 && r != 2 && r != 4) fail("rows_per_thread must be 0, 1, 2 or 4");
+    b->launch = *cfg;
+    hipError_t e = upload_work_map(b);
+    if (e != hipSuccess) return device_error(e, "bsql_batch_set_launch");
+    return BSQL_OK;
+  })
+}
+
+int64_t bsql_batch_total_rows(const bsql_batch* b) {
+  if (!b) return -1;
+  int64_t t = 0;
+  for (int64_t r : b->rows) t += r;
+  return t;
+}
+
+void bsql_batch_free(bsql_batch* b) { delete b; }
+
+}  // extern "C"
+
+// ----------------------------------------------------------------- autotuner
+
+namespace {
+
+// Average milliseconds of `run` on `stream` over a few launches after one
+// warm-up, timed with hipEvents on that stream.
+template <class F>
+hipError_t time_launches(hipStream_t stream, F run, float* ms) {
+  hipEvent_t a, z;
+  hipError_t e = hipEventCreate(&a);
+  if (e != hipSuccess) return e;
+  e = hipEventCreate(&z);
+  if (e != hipSuccess) { (void)hipEventDestroy(a); return e; }
+  constexpr int REPS = 3;
+  e = run();
+  if (e == hipSuccess) e = hipEventRecord(a, stream);
+  for (int i = 0; i < REPS && e == hipSuccess; ++i) e = run();
+  if (e == hipSuccess) e = hipEventRecord(z, stream);
+  if (e == hipSuccess) e = hipEventSynchronize(z);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, a, z);
+  *ms = t / REPS;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(z);
+  return e;
+}
+
+const int kShapes[] = {1, 2, 4};  // rows per lane the VM variants cover
+
+}  // namespace
+
+extern "C" {
+
+int bsql_batch_autotune(bsql_batch* b, void* stream, bsql_launch_config* chosen) {
+  BSQL_TRY({
+    if (!b) fail("null batch");
+    float best = 1e30f;
+    bsql_launch_config best_cfg = b->launch;
+    for (int r : kShapes) {
+      if (batch_tile_rows(b->nreg, r) != 256 * r) continue;  // no variant of this shape
+      bsql_launch_config cfg{0, r, 0};
+      b->launch = cfg;
+      hipError_t e = upload_work_map(b);
+      float ms = 0.f;
+      if (e == hipSuccess)
+        e = time_launches((hipStream_t)stream, [&] {
+          return launch_batches(b->d_plans, b->d_tile_start, (int)b->plans.size(), b->n_items, b->nreg, r,
+                                (hipStream_t)stream);
+        }, &ms);
+      if (e != hipSuccess) return device_error(e, "bsql_batch_autotune");
+      if (ms < best) { best = ms; best_cfg = cfg; }
+    }
+    b->launch = best_cfg;
+    hipError_t e = upload_work_map(b);
+    if (e != hipSuccess) return device_error(e, "bsql_batch_autotune");
+    if (chosen) *chosen = best_cfg;
+    return BSQL_OK;
+  })
+}
+
+int bsql_program_autotune(bsql_program* prog, const bsql_column* in, int n_in, bsql_column* out, int n_out,
+                          int64_t num_rows, void* stream, bsql_launch_config* chosen) {
+  BSQL_TRY({
+    if (!prog) fail("null program");
+    float best = 1e30f;
+    bsql_launch_config best_cfg = prog->launch;
+    for (int r : kShapes) {
+      bsql_launch_config cfg = prog->launch;
+      cfg.rows_per_thread = r;
+      prog->launch = cfg;
+      float ms = 0.f;
+      int rc = BSQL_OK;
+      hipError_t e = time_launches((hipStream_t)stream, [&] {
+        rc = bsql_interp_execute(prog, in, n_in, out, n_out, num_rows, stream);
+        return rc == BSQL_OK ? hipSuccess : hipErrorUnknown;
+      }, &ms);
+      if (rc != BSQL_OK) return rc;
+      if (e != hipSuccess) return device_error(e, "bsql_program_autotune");
+      if (ms < best) { best = ms; best_cfg = cfg; }
+    }
+    prog->launch = best_cfg;
+    if (chosen) *chosen = best_cfg;
+    return BSQL_OK;
   })
 }
 

@@ -833,11 +833,61 @@ __device__ __forceinline__ void store_out(const DevOutCol& c, int64_t pos0, int 
 // the positions [0, num_rows)) or the output phase of a filter (rows are the
 // selected row ids idx[0, count), outputs at the positions in that list).
 // Persistent: workgroups stride over tiles of BLOCK*R positions.
+// Ops [plan.split, n_ops) over one tile of 256*R positions [tile*TILE,
+// count): positions are rows (or, GATHER, indexes into the row-id list idx).
+// The register file is local to the tile: every register is written before
+// it is read, so nothing carries over from one tile to the next.
+template <int NREG, int R, bool GATHER>
+__device__ __forceinline__ void vm_tile(const DevPlan& plan, int64_t tile, int64_t count,
+                                        const uint32_t* __restrict__ idx, int lane) {
+  typedef typename RegVecT<NREG>::type RegVec;
+  constexpr int TILE = BLOCK * R;
+  RegVec f[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
+  uint64_t vb = 0;
+  LoadTemps<R> T;
+  Rows<R> rs;
+  rs.pos0 = tile * TILE + (int64_t)threadIdx.x * R;
+  const int64_t left = count - rs.pos0;
+  const int n = left <= 0 ? 0 : (left >= R ? R : (int)left);
+  rs.nrows = n;
+  rs.exist = (1u << n) - 1u;
+  if (GATHER) {
+    rs.contig = false;
+    rs.row0 = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) rs.row[k] = k < n ? (int64_t)gp<uint32_t>(idx)[rs.pos0 + k] : 0;
+  } else {
+    rs.contig = true;
+    rs.row0 = rs.pos0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) rs.row[k] = rs.pos0 + k;
+  }
+  uint32_t alive = rs.exist;
+  for (int q = plan.split; q < plan.n_ops; ++q) {
+    const DevOp op = load_op(plan, q);
+    if (op.code != D_STORE) {
+      vm_step<NREG, R>(plan, op, f, vb, T, rs, alive);
+      continue;
+    }
+    const int ia = op.a & (NREG - 1);
+    uint64_t A[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) A[k] = f[k][ia];
+    const uint32_t VA = (uint32_t)(vb >> (ia * R));
+    store_out<R>(plan.out[op.aux], rs.pos0, n, A, VA & rs.exist, lane);
+  }
+}
+
+// perform_interpreter_operation (rows are the positions [0, num_rows)) or
+// the output phase of a filter (GATHER: rows are the selected row ids
+// idx[0, count), outputs at the positions in that list). Persistent:
+// workgroups stride over tiles of BLOCK*R positions.
 template <int NREG, int R, bool GATHER>
 __global__ __launch_bounds__(BLOCK) void vm_rows(const DevPlan* __restrict__ pplan,
                                                  const uint32_t* __restrict__ idx,
                                                  const int64_t* __restrict__ d_count, int64_t capacity) {
-  typedef typename RegVecT<NREG>::type RegVec;
   constexpr int TILE = BLOCK * R;
   const DevPlan& plan = *pplan;  // device memory: ops are scalar-cache loads
   const int lane = threadIdx.x & 63;
@@ -846,43 +896,28 @@ __global__ __launch_bounds__(BLOCK) void vm_rows(const DevPlan* __restrict__ ppl
     count = *d_count;
     if (count > capacity) count = capacity;
   }
-  RegVec f[R];
-#pragma unroll
-  for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
-  uint64_t vb = 0;
-  LoadTemps<R> T;
-  for (int64_t tile = blockIdx.x; tile * TILE < count; tile += gridDim.x) {
-    Rows<R> rs;
-    rs.pos0 = tile * TILE + (int64_t)threadIdx.x * R;
-    const int64_t left = count - rs.pos0;
-    const int n = left <= 0 ? 0 : (left >= R ? R : (int)left);
-    rs.nrows = n;
-    rs.exist = (1u << n) - 1u;
-    if (GATHER) {
-      rs.contig = false;
-      rs.row0 = 0;
-#pragma unroll
-      for (int k = 0; k < R; ++k) rs.row[k] = k < n ? (int64_t)gp<uint32_t>(idx)[rs.pos0 + k] : 0;
-    } else {
-      rs.contig = true;
-      rs.row0 = rs.pos0;
-#pragma unroll
-      for (int k = 0; k < R; ++k) rs.row[k] = rs.pos0 + k;
+  for (int64_t tile = blockIdx.x; tile * TILE < count; tile += gridDim.x)
+    vm_tile<NREG, R, GATHER>(plan, tile, count, idx, lane);
+}
+
+// Batched interpreter: every batch has its own plan (program and columns);
+// the work items are all batches' tiles, batch b owning items
+// [tile_start[b], tile_start[b+1]). Workgroups stride over the items, so
+// uneven batch sizes spread over the whole chip in one persistent launch.
+template <int NREG, int R>
+__global__ __launch_bounds__(BLOCK) void vm_batches(const DevPlan* __restrict__ plans,
+                                                    const int64_t* __restrict__ tile_start, int n_batches) {
+  const int lane = threadIdx.x & 63;
+  const int64_t n_items = tile_start[n_batches];
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    int lo = 0, hi = n_batches - 1;  // last batch whose first item <= item
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tile_start[mid] <= item) lo = mid;
+      else hi = mid - 1;
     }
-    uint32_t alive = rs.exist;
-    for (int q = plan.split; q < plan.n_ops; ++q) {
-      const DevOp op = load_op(plan, q);
-      if (op.code != D_STORE) {
-        vm_step<NREG, R>(plan, op, f, vb, T, rs, alive);
-        continue;
-      }
-      const int ia = op.a & (NREG - 1);
-      uint64_t A[R];
-#pragma unroll
-      for (int k = 0; k < R; ++k) A[k] = f[k][ia];
-      const uint32_t VA = (uint32_t)(vb >> (ia * R));
-      store_out<R>(plan.out[op.aux], rs.pos0, n, A, VA & rs.exist, lane);
-    }
+    const DevPlan& plan = plans[lo];
+    vm_tile<NREG, R, false>(plan, item - tile_start[lo], plan.num_rows, nullptr, lane);
   }
 }
 
@@ -1465,6 +1500,32 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
   if (!pick(nreg, rpt, &NR, &RR)) return hipErrorInvalidValue;
 #define X(N, RPT) \
   if (NR == N && RR == RPT) return launch_gather_t<N, RPT>(d_plan, plan.num_rows, idx, d_count, capacity, stream);
+  BSQL_VARIANTS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+// Batched launch: `plans` (device) holds n_batches plans, `tile_start`
+// (device) n_batches + 1 prefix counts of tiles of BLOCK*rpt rows. No
+// allocation and no copy: the launch can be captured in a hipGraph.
+int batch_tile_rows(int nreg, int rpt) {
+  int NR, RR;
+  if (!pick(nreg, rpt, &NR, &RR)) return -1;
+  return BLOCK * RR;
+}
+
+hipError_t launch_batches(const DevPlan* plans, const int64_t* tile_start, int n_batches, int64_t n_items,
+                          int nreg, int rpt, hipStream_t stream) {
+  int NR, RR;
+  if (!pick(nreg, rpt, &NR, &RR)) return hipErrorInvalidValue;
+  const int64_t grid = persistent_grid(n_items);
+  if (grid == 0) return hipSuccess;
+#define X(N, RPT)                                                                                  \
+  if (NR == N && RR == RPT) {                                                                      \
+    hipLaunchKernelGGL((vm_batches<N, RPT>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, plans, \
+                       tile_start, n_batches);                                                     \
+    return hipGetLastError();                                                                      \
+  }
   BSQL_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;

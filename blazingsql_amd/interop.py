@@ -332,3 +332,59 @@ def filter_project(pred, proj, in_cols, out_cols, d_count, num_rows=None, stream
     L.check(L.lib().bsql_filter_project(pred.handle, proj.handle, ins, len(in_cols), outs,
                                         len(out_cols), num_rows,
                                         ctypes.c_void_p(d_count.data_ptr()), _stream_ptr(stream)))
+
+
+class Batch:
+    """Many (program, columns, rows) batches run as one persistent launch
+    (bsql_batch_*). The launch allocates and copies nothing, so it can be
+    captured in a HIP graph."""
+
+    def __init__(self, items):
+        """items: list of (Program, in_cols, out_cols, num_rows)."""
+        n = len(items)
+        self._keep = []
+        descs = (L.BatchDesc * max(n, 1))()
+        for i, (prog, ins, outs, rows) in enumerate(items):
+            ia, oa = columns_array(ins), columns_array(outs)
+            self._keep += [ia, oa, prog, ins, outs]
+            descs[i].prog = prog.handle
+            descs[i].inp = ctypes.cast(ia, ctypes.c_void_p)
+            descs[i].n_in = len(ins)
+            descs[i].out = ctypes.cast(oa, ctypes.c_void_p)
+            descs[i].n_out = len(outs)
+            descs[i].num_rows = rows
+        h = ctypes.c_void_p()
+        L.check(L.lib().bsql_batch_prepare(descs, n, ctypes.byref(h)))
+        self.handle = h.value
+
+    def launch(self, stream=None):
+        L.check(L.lib().bsql_batch_launch(self.handle, _stream_ptr(stream)))
+
+    def set_launch(self, rows_per_thread=0):
+        cfg = L.LaunchConfig(0, rows_per_thread, 0)
+        L.check(L.lib().bsql_batch_set_launch(self.handle, ctypes.byref(cfg)))
+
+    def autotune(self, stream=None):
+        cfg = L.LaunchConfig()
+        L.check(L.lib().bsql_batch_autotune(self.handle, _stream_ptr(stream), ctypes.byref(cfg)))
+        return cfg.rows_per_thread
+
+    @property
+    def total_rows(self):
+        return L.lib().bsql_batch_total_rows(self.handle)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            L.lib().bsql_batch_free(h)
+            self.handle = None
+
+
+def autotune_program(prog, in_cols, out_cols, num_rows, stream=None):
+    """Times the launch shapes for this program on these columns; keeps the
+    fastest on the program and returns its rows per lane."""
+    ins, outs = columns_array(in_cols), columns_array(out_cols)
+    cfg = L.LaunchConfig()
+    L.check(L.lib().bsql_program_autotune(prog.handle, ins, len(in_cols), outs, len(out_cols), num_rows,
+                                          _stream_ptr(stream), ctypes.byref(cfg)))
+    return cfg.rows_per_thread

@@ -237,6 +237,40 @@ typedef struct bsql_launch_config {
  * (interpreter_cpp.cu:76-270, :473). */
 int bsql_program_set_launch(bsql_program* prog, const bsql_launch_config* cfg);
 
+/* ---- batched execution ----------------------------------------------------
+ * Many batches, each with its own program and columns (the uneven batches an
+ * execution graph hands to ProjectKernel one task at a time,
+ * BatchProcessing.cpp:365-448), run as ONE persistent launch: workgroups
+ * stride over every batch's tiles. bsql_batch_prepare lowers each program
+ * and uploads all plans (synchronous, allocates); bsql_batch_launch then
+ * allocates and copies nothing, so it can be captured in a hipGraph and
+ * replayed. Outputs must be fixed-width. Replaces one
+ * perform_interpreter_operation call per batch (interpreter_cpp.h:77-86). */
+typedef struct bsql_batch_desc {
+  const bsql_program* prog;
+  const bsql_column* in;
+  int n_in;
+  const bsql_column* out;
+  int n_out;
+  int64_t num_rows;
+} bsql_batch_desc;
+typedef struct bsql_batch bsql_batch;
+int bsql_batch_prepare(const bsql_batch_desc* descs, int n, bsql_batch** batch);
+int bsql_batch_launch(const bsql_batch* batch, void* stream);
+/* rows per lane of the batched launch (re-uploads the work map; not for use
+ * while a graph holding the launch may run) */
+int bsql_batch_set_launch(bsql_batch* batch, const bsql_launch_config* cfg);
+/* Autotuner: times each launch shape the batch's programs allow on `stream`
+ * (a few launches each, hipEvents), keeps the fastest and returns it. */
+int bsql_batch_autotune(bsql_batch* batch, void* stream, bsql_launch_config* chosen);
+int64_t bsql_batch_total_rows(const bsql_batch* batch);
+void bsql_batch_free(bsql_batch* batch);
+
+/* Autotuner for one program: times bsql_interp_execute on these columns for
+ * each launch shape, sets the fastest on the program and returns it. */
+int bsql_program_autotune(bsql_program* prog, const bsql_column* in, int n_in, bsql_column* out,
+                          int n_out, int64_t num_rows, void* stream, bsql_launch_config* chosen);
+
 /* ---- diagnostics --------------------------------------------------------- */
 /* Message of the last error on the calling thread. */
 const char* bsql_last_error(void);
