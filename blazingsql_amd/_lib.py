@@ -61,6 +61,7 @@ class LaunchConfig(ctypes.Structure):
 
 
 LAUNCH_NO_RANGES, LAUNCH_LAZY_LOADS = 1, 2
+SKETCH_DIM, FEATURE_DIM, MLP_HIDDEN = 19, 24, 64
 
 
 class BatchDesc(ctypes.Structure):
@@ -96,6 +97,19 @@ SIGNATURES = {
     "bsql_batch_total_rows": (_I64, [_P]),
     "bsql_batch_free": (None, [_P]),
     "bsql_program_autotune": (_I, [_P, _P, _I, _P, _I, _I64, _P, ctypes.POINTER(LaunchConfig)]),
+    "bsql_filter_count": (_I, [_P, _P, _I, _I64, _P, _P]),
+    "bsql_filter_project_sized": (_I, [_P, _P, _P, _I, _P, _I, _I64, _I64, _P, _P]),
+    "bsql_sketch": (_I, [_P, _I, _I64, _P, _P]),
+    "bsql_mlp_create": (_I, [ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "bsql_mlp_num_params": (_I, [_P]),
+    "bsql_mlp_get_params": (_I, [_P, _P]),
+    "bsql_mlp_set_params": (_I, [_P, _P]),
+    "bsql_mlp_forward": (_I, [_P, _P, _I, _P, _P]),
+    "bsql_mlp_gradients": (_I, [_P, _P, _P, _I, ctypes.c_float, _P, _P, _P]),
+    "bsql_mlp_train_step": (_I, [_P, _P, _P, _I, ctypes.c_float, ctypes.c_float, _P, _P]),
+    "bsql_mlp_free": (None, [_P]),
+    "bsql_conjunct_features": (_I, [_P, _P, _I, _P, _P, _I]),
+    "bsql_order_by_prediction": (_I, [_P, _P, _P, _I]),
     "bsql_last_error": (ctypes.c_char_p, []),
     "bsql_version": (_I, []),
 }

@@ -13,10 +13,7 @@
 #include "../../include/bsql/interp.h"
 #include "program.hpp"
 
-struct bsql_program : bsql::Program {
-  std::mutex cache_mu;
-  std::map<std::string, bsql::Lowered> cache;
-};
+#include "capi_internal.h"
 
 namespace bsql {
 
@@ -35,8 +32,8 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
                           hipStream_t stream);
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
 int set_error(int code, const std::string& msg) {
   g_last_error = msg;
@@ -48,6 +45,8 @@ int device_error(hipError_t e, const char* where) {
     return set_error(BSQL_ERR_OOM, std::string(where) + ": out of device memory");
   return set_error(BSQL_ERR_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
 }
+
+namespace {
 
 // Per-stream scratch: the device copy of the launch plan, then the compaction
 // tile status words + ticket. One buffer per stream, so that concurrent calls
@@ -465,7 +464,8 @@ static int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column
   if (e == hipSuccess && proj) e = ensure_strings(*proj);
   if (e != hipSuccess) return device_error(e, "filter");
   LowerOptions opt;
-  opt.mode = proj ? LowerOptions::FILTER_PROJECT : LowerOptions::FILTER_GATHER;
+  opt.mode = proj ? LowerOptions::FILTER_PROJECT
+                  : (n_out == 0 && !out ? LowerOptions::FILTER_COUNT : LowerOptions::FILTER_GATHER);
   opt.pred = pred;
   opt.proj = proj;
   opt.conj_order = &pred->conj_order;
@@ -513,6 +513,25 @@ int bsql_filter(const bsql_program* pred, const bsql_column* in, int n_in, bsql_
   BSQL_TRY({
     return run_compact(const_cast<bsql_program*>(pred), nullptr, in, n_in, out, n_in, num_rows,
                        num_rows, d_count, stream);
+  })
+}
+
+int bsql_filter_count(const bsql_program* pred, const bsql_column* in, int n_in, int64_t num_rows,
+                      int64_t* d_count, void* stream) {
+  BSQL_TRY({
+    return run_compact(const_cast<bsql_program*>(pred), nullptr, in, n_in, nullptr, 0, num_rows, num_rows,
+                       d_count, stream);
+  })
+}
+
+int bsql_filter_project_sized(const bsql_program* pred, const bsql_program* proj, const bsql_column* in,
+                              int n_in, bsql_column* out, int n_out, int64_t num_rows, int64_t capacity,
+                              int64_t* d_count, void* stream) {
+  BSQL_TRY({
+    if (!proj) fail("null projection");
+    if (capacity < 0 || capacity > num_rows) fail("capacity must be in [0, num_rows]");
+    return run_compact(const_cast<bsql_program*>(pred), const_cast<bsql_program*>(proj), in, n_in, out, n_out,
+                       num_rows, capacity, d_count, stream);
   })
 }
 

@@ -1099,15 +1099,26 @@ template <class K, int R, int S, bool FULL>
 __device__ __forceinline__ uint32_t range_chunks(const DevInCol& c, const DevRange& rg, int64_t row0,
                                                  int64_t num_rows, uint32_t alive) {
   typedef typename K::T CT;
+  constexpr uint32_t RMASK = (1u << R) - 1u;
   uint32_t pass = 0;
   if constexpr (FULL) {
+    // a chunk whose rows are all dead in the whole wave is not read: the
+    // earlier (more selective) conjuncts spare the later columns' bytes
+    uint32_t live = 0;
+#pragma unroll
+    for (int k = 0; k < S; ++k) live |= (__ballot(((alive >> (k * R)) & RMASK) != 0) != 0 ? 1u : 0u) << k;
     CT v[S][R];
     uint32_t vm = 0;
 #pragma unroll
     for (int k = 0; k < S; ++k) {
-      const int64_t r0 = row0 + (int64_t)k * 64 * R;
-      load_raw<CT, R>(c, r0, R, v[k]);
-      vm |= load_valid<R>(c, r0, R) << (k * R);
+      if ((live >> k) & 1u) {
+        const int64_t r0 = row0 + (int64_t)k * 64 * R;
+        load_raw<CT, R>(c, r0, R, v[k]);
+        vm |= load_valid<R>(c, r0, R) << (k * R);
+      } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[k][j] = CT(0);
+      }
     }
 #pragma unroll
     for (int k = 0; k < S; ++k) pass |= range_bits<K, R>(rg, v[k]) << (k * R);

@@ -739,6 +739,23 @@ int count_conjuncts(const Program& p) {
   }
 }
 
+std::vector<DevRange> conjunct_ranges(const Program& p, const int32_t* in_types, int n_in) {
+  Lowering L{in_types, n_in, {}, {}, {}};
+  L.load_value.assign((size_t)n_in, -1);
+  SsaProgram s = to_ssa(L, p, 0);
+  if (s.finals.size() != 1) fail("a predicate has one output");
+  std::vector<int> prod = producers(s, L.values.size());
+  std::vector<int> conj;
+  conjuncts_of(p, s, prod, s.finals[0], conj);
+  std::vector<DevRange> out;
+  for (int v : conj) {
+    DevRange rg;
+    if (!as_range(L, s, prod, v, rg)) fail("conjunct is not a column-vs-literal comparison");
+    out.push_back(rg);
+  }
+  return out;
+}
+
 Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
               const int32_t* out_types, int n_out) {
   if (n_in > BSQL_DEV_MAX_IN) fail("too many input columns for one launch");
@@ -825,7 +842,9 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
   if (opt.mode != LowerOptions::EXECUTE) L.load_value.assign((size_t)n_in, -1);
 
   // ---- phase 2: outputs
-  if (opt.mode == LowerOptions::FILTER_GATHER) {
+  if (opt.mode == LowerOptions::FILTER_COUNT) {
+    if (n_out != 0) fail("a count has no output columns");
+  } else if (opt.mode == LowerOptions::FILTER_GATHER) {
     if (n_out != n_in) fail("filter gathers every input column");
     for (int k = 0; k < n_in; ++k) store_value.push_back(L.load_of(k));
   } else {

@@ -93,7 +93,7 @@ void validate(const Program& p, int n_in);
 
 // Which register-file size / rows-per-thread a lowered plan needs.
 struct LowerOptions {
-  enum Mode { EXECUTE, FILTER_PROJECT, FILTER_GATHER } mode = EXECUTE;
+  enum Mode { EXECUTE, FILTER_PROJECT, FILTER_GATHER, FILTER_COUNT } mode = EXECUTE;
   const Program* pred = nullptr;   // FILTER_*: predicate (1 BOOL8 output)
   const Program* proj = nullptr;   // EXECUTE / FILTER_PROJECT: outputs
   bool conjuncts = true;           // wave-level early exit per conjunct
@@ -114,6 +114,10 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
 
 // Conjunct decomposition of a predicate whose root is a left-deep AND chain.
 int count_conjuncts(const Program& p);
+
+// The range test of each conjunct of a predicate, in plan order (not merged);
+// fails unless every conjunct is a column-vs-literal comparison.
+std::vector<DevRange> conjunct_ranges(const Program& p, const int32_t* in_types, int n_in);
 
 // Text front-end (parser.cpp): Calcite expression strings -> Program.
 std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exprs,

@@ -210,6 +210,19 @@ int bsql_filter_project(const bsql_program* pred, const bsql_program* proj,
                         int n_out, int64_t num_rows, int64_t* d_count,
                         void* stream);
 
+/* Row count of the filter only (the selection kernel; nothing is gathered):
+ * the count process_filter's output would have. */
+int bsql_filter_count(const bsql_program* pred, const bsql_column* in, int n_in, int64_t num_rows,
+                      int64_t* d_count, void* stream);
+
+/* bsql_filter_project into outputs pre-sized for `capacity` rows (e.g. from
+ * a predicted selectivity): one pass writes the first min(count, capacity)
+ * rows and *d_count receives the full count, so count > capacity tells the
+ * caller to run again with outputs of that size. */
+int bsql_filter_project_sized(const bsql_program* pred, const bsql_program* proj, const bsql_column* in,
+                              int n_in, bsql_column* out, int n_out, int64_t num_rows, int64_t capacity,
+                              int64_t* d_count, void* stream);
+
 /* Conjunct order: for a predicate whose root is a left-deep AND chain (the
  * shape expand_if_logical_op produces, CalciteExpressionParsing.cpp:330-398),
  * evaluate its conjuncts in `order` (a permutation of 0..n_conj-1) with a
@@ -270,6 +283,45 @@ void bsql_batch_free(bsql_batch* batch);
  * each launch shape, sets the fastest on the program and returns it. */
 int bsql_program_autotune(bsql_program* prog, const bsql_column* in, int n_in, bsql_column* out,
                           int n_out, int64_t num_rows, void* stream, bsql_launch_config* chosen);
+
+/* ---- learned conjunct ordering -------------------------------------------
+ * No counterpart in the reference, which evaluates conjuncts in plan order.
+ * A small FP32 MLP predicts, for each range conjunct of a predicate, its
+ * selectivity (sigmoid of output 0) and its log cost (output 1) from the
+ * sketch of the conjunct's column; bsql_order_by_prediction then evaluates
+ * cheap, selective conjuncts first. The rows selected do not depend on the
+ * order; the plan order stays the default. */
+#define BSQL_SKETCH_DIM 19    /* null fraction, min, max, 16-bin histogram */
+#define BSQL_FEATURE_DIM 24
+#define BSQL_MLP_HIDDEN 64
+/* Sketch of each column over a 1024-row evenly spaced sample (device output
+ * of n_cols * BSQL_SKETCH_DIM floats; synchronizes the stream). */
+int bsql_sketch(const bsql_column* cols, int n_cols, int64_t num_rows, float* d_sketch, void* stream);
+typedef struct bsql_mlp bsql_mlp;   /* BSQL_FEATURE_DIM -> 64 -> 64 -> 2, ReLU */
+int bsql_mlp_create(uint64_t seed, bsql_mlp** mlp);
+int bsql_mlp_num_params(const bsql_mlp* mlp);
+int bsql_mlp_get_params(const bsql_mlp* mlp, float* host_params);
+int bsql_mlp_set_params(bsql_mlp* mlp, const float* host_params);
+/* d_x: batch x BSQL_FEATURE_DIM, d_y: batch x 2 (device) */
+int bsql_mlp_forward(bsql_mlp* mlp, const float* d_x, int batch, float* d_y, void* stream);
+/* Gradient of the mean loss (binary cross-entropy of output 0 against the
+ * observed selectivity + cost_weight * squared error of output 1 against the
+ * observed log cost; targets batch x 2) into d_grads, per-sample losses in
+ * d_loss. */
+int bsql_mlp_gradients(bsql_mlp* mlp, const float* d_x, const float* d_target, int batch, float cost_weight,
+                       float* d_grads, float* d_loss, void* stream);
+/* One Adam step on that gradient. */
+int bsql_mlp_train_step(bsql_mlp* mlp, const float* d_x, const float* d_target, int batch, float lr,
+                        float cost_weight, float* d_loss, void* stream);
+void bsql_mlp_free(bsql_mlp* mlp);
+/* Features (host, cap x BSQL_FEATURE_DIM) of each conjunct of a predicate
+ * whose conjuncts are all column-vs-literal comparisons, from the host copy
+ * of the table's sketch; returns the number of conjuncts (written only if
+ * cap is large enough), or minus an error code. */
+int bsql_conjunct_features(const bsql_program* pred, const int32_t* in_types, int n_in, const float* sketch,
+                           float* features, int cap);
+/* Sets the conjunct order: ascending cost / (1 - selectivity). */
+int bsql_order_by_prediction(bsql_program* pred, const float* selectivity, const float* cost, int n_conj);
 
 /* ---- diagnostics --------------------------------------------------------- */
 /* Message of the last error on the calling thread. */

@@ -334,3 +334,22 @@ def test_like_and_slices_closed_forms(oracle_lib, bsql):
         p, ot = _compiled(bsql, [expr], [23])
         (vals, _), = oracle_lib.execute(p.to_oracle_dict(), [col], ot)
         assert vals == [fn(s) for s in strs], expr
+
+
+def test_mlp_reference_gradients_match_finite_differences():
+    """The host MLP reference (the checker of the GPU gradients) against
+    central differences of its own loss: a dropped term, a wrong sign or a
+    transposed weight in the backward pass fails this."""
+    from oracle import mlp_ref
+    rng = np.random.default_rng(0)
+    n_params = 24 * 64 + 64 + 64 * 64 + 64 + 64 * 2 + 2
+    p = rng.normal(0, 0.2, n_params)
+    x = rng.normal(0, 1, (16, 24))
+    t = np.stack([rng.random(16), rng.normal(0, 1, 16)], axis=1)
+    g = mlp_ref.gradients(p, x, t, cost_weight=0.5)
+    h = 1e-6
+    for i in rng.choice(n_params, 60, replace=False):
+        e = np.zeros(n_params)
+        e[i] = h
+        fd = (mlp_ref.loss(p + e, x, t, 0.5) - mlp_ref.loss(p - e, x, t, 0.5)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, g[i])
