@@ -329,7 +329,7 @@ def run_learned(args):
         ref_outs = [DeviceColumn.empty(10, n), DeviceColumn.empty(10, n)]
         cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
         for _ in range(args.predicates):
-            text = learned.random_predicate(rng, int(rng.integers(4, 9)))
+            text, terms = learned.random_predicate(rng, int(rng.integers(4, 9)))
             proj = bsql.Program.compile(proj_text, types)
             static = bsql.Program.compile(text, types)
             t_static = timed(static, proj, cols, ref_outs, cnt)
@@ -342,7 +342,6 @@ def run_learned(args):
                 torch.equal(o.data[:m_learned], r.data[:m_learned]) for o, r in zip(outs, ref_outs))
             exact = exact and same
             # the order the exact selectivities and costs give (the regret's reference)
-            terms = text[4:-1].split(", ") if text.startswith("AND(") else [text]
             true_sel, true_t = learned.measure_terms(terms, cols, n, reps=2)
             op = bsql.Program.compile(text, types)
             learned.order_by(op, true_sel, true_t)

@@ -150,10 +150,12 @@ def range_term(rng, col, sel):
 
 
 def random_predicate(rng, n_terms):
+    """(text, terms): an AND of n_terms range terms on distinct columns with
+    selectivities log-uniform in [0.001, 0.9]."""
     cols = rng.permutation(len(COPULA_TYPES))[:n_terms]
     sels = np.exp(rng.uniform(np.log(0.001), np.log(0.9), n_terms))
     terms = [range_term(rng, int(c), float(s)) for c, s in zip(cols, sels)]
-    return "AND(" + ", ".join(terms) + ")" if n_terms > 1 else terms[0]
+    return ("AND(" + ", ".join(terms) + ")" if n_terms > 1 else terms[0]), terms
 
 
 def measure_terms(terms, cols, n, reps=3):
