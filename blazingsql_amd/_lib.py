@@ -110,6 +110,24 @@ SIGNATURES = {
     "bsql_mlp_free": (None, [_P]),
     "bsql_conjunct_features": (_I, [_P, _P, _I, _P, _P, _I]),
     "bsql_order_by_prediction": (_I, [_P, _P, _P, _I]),
+    "bsql_set_allocator": (_I, [_P, _P, _P]),
+    "bsql_table_create": (_I, [_P, _P, _P, _P, _I, _I64, _P, ctypes.POINTER(_P)]),
+    "bsql_table_from_views": (_I, [_P, _P, _I, _I64, ctypes.POINTER(_P)]),
+    "bsql_table_free": (None, [_P]),
+    "bsql_table_num_rows": (_I64, [_P]),
+    "bsql_table_num_columns": (_I, [_P]),
+    "bsql_table_column": (_I, [_P, _I, ctypes.POINTER(Column)]),
+    "bsql_table_name": (ctypes.c_char_p, [_P, _I]),
+    "bsql_table_set_names": (_I, [_P, _P, _I]),
+    "bsql_table_record_stream": (_I, [_P, _P]),
+    "bsql_process_filter": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_P), _P]),
+    "bsql_process_project": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_P), _P]),
+    "bsql_evaluate_expressions": (_I, [_P, _P, _I, ctypes.POINTER(_P), _P]),
+    "bsql_apply_boolean_filter": (_I, [_P, ctypes.POINTER(Column), ctypes.POINTER(_P), _P]),
+    "bsql_project_bypass": (_I, [ctypes.c_char_p, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "bsql_table_set_project_aliases": (_I, [_P, ctypes.c_char_p]),
+    "bsql_table_fix_column_aliases": (_I, [_P, ctypes.c_char_p]),
+    "bsql_scan_projections": (_I, [ctypes.c_char_p, _I, _P, _I]),
     "bsql_last_error": (ctypes.c_char_p, []),
     "bsql_version": (_I, []),
 }

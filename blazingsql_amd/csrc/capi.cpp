@@ -19,7 +19,7 @@ namespace bsql {
 
 hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt,
                           hipStream_t stream);
-int64_t compact_tiles(const DevPlan& plan, int nreg, int rpt);
+int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt);
 int64_t string_tiles(int64_t capacity);
 int batch_tile_rows(int nreg, int rpt);
 hipError_t launch_batches(const DevPlan* plans, const int64_t* tile_start, int n_batches, int64_t n_items,
@@ -28,8 +28,8 @@ hipError_t launch_string_output(const uint64_t* slices, const uint32_t* valid, i
                                 const uint8_t* src_chars, uint8_t* dst_chars, const int64_t* d_count,
                                 int64_t count, int64_t capacity, uint64_t* status, hipStream_t stream);
 hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int sel_rpt, int rpt,
-                          uint64_t* status, uint32_t* idx, int64_t* d_count, int64_t capacity,
-                          hipStream_t stream);
+                          uint64_t* sel, uint32_t* idx, int64_t* d_count, int64_t capacity,
+                          hipStream_t stream, int phases);
 
 namespace {
 thread_local std::string g_last_error;
@@ -191,12 +191,13 @@ void check_outputs(const bsql_column* out, int n_out, int64_t rows) {
 
 // String outputs: the VM stores slices into scratch; each then needs the
 // chars buffer of its source column to fit in the output's chars.
-int count_string_outputs(const Lowered& L, const bsql_column* in, const bsql_column* out, int n_out) {
+int count_string_outputs(const Lowered& L, const bsql_column* in, const bsql_column* out, int n_out,
+                         bool check = true) {
   int n = 0;
   for (int k = 0; k < n_out; ++k) {
     const int src = L.out_str_col[(size_t)k];
     if (src < 0) continue;
-    if (out[k].chars_size < in[src].chars_size)
+    if (check && out[k].chars_size < in[src].chars_size)
       fail("string output " + std::to_string(k) + ": chars_size " + std::to_string(out[k].chars_size) +
            " is smaller than its source column's " + std::to_string(in[src].chars_size));
     ++n;
@@ -241,16 +242,72 @@ int rows_per_thread(const bsql_program* p) { return p ? p->launch.rows_per_threa
 
 using namespace bsql;
 
-#define BSQL_TRY(...)                                              \
-  try {                                                            \
-    __VA_ARGS__                                                    \
-  } catch (const bsql::Error& e) {                                 \
-    return set_error(e.code, e.what());                            \
-  } catch (const std::bad_alloc&) {                                \
-    return set_error(BSQL_ERR_OOM, "host allocation failed");      \
-  } catch (const std::exception& e) {                              \
-    return set_error(BSQL_ERR_INVALID, e.what());                  \
+namespace bsql {
+
+// Filter (+ project) launches; `phases` as launch_compact's. Scratch is
+// sized for num_rows whatever the capacity, so that a selection phase and a
+// later output phase on the same stream find the same layout.
+int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, int n_in,
+                bsql_column* out, int n_out, int64_t num_rows, int64_t capacity,
+                int64_t* d_count, void* stream, int phases) {
+  if (!pred) fail("null predicate");
+  if (!d_count) fail("null count pointer");
+  check_inputs(in, n_in, num_rows);
+  // selected rows are listed as 32-bit row ids
+  if (num_rows > (int64_t)UINT32_MAX) fail("a filter handles at most 2^32-1 rows per call");
+  const bool outputs = (phases & PHASE_OUTPUT) != 0;
+  if (outputs) check_outputs(out, n_out, capacity);
+  hipError_t e = ensure_strings(*pred);
+  if (e == hipSuccess && proj) e = ensure_strings(*proj);
+  if (e != hipSuccess) return device_error(e, "filter");
+  LowerOptions opt;
+  opt.mode = proj ? LowerOptions::FILTER_PROJECT
+                  : (n_out == 0 && !out ? LowerOptions::FILTER_COUNT : LowerOptions::FILTER_GATHER);
+  opt.pred = pred;
+  opt.proj = proj;
+  opt.conj_order = &pred->conj_order;
+  opt.ranges = !(pred->launch.flags & BSQL_LAUNCH_NO_RANGES);
+  opt.lazy_loads = (pred->launch.flags & BSQL_LAUNCH_LAZY_LOADS) != 0;
+  const Lowered& L = lowered_for(pred, opt, in, n_in, out, n_out);
+  DevPlan plan = L.plan;
+  plan.num_rows = num_rows;
+  for (int k = 0; k < n_in; ++k) plan.in[k] = to_dev_in(in[k]);
+  const int n_str = count_string_outputs(L, in, out, n_out, outputs);
+  // the predicate's launch shape drives the selection kernel, the
+  // projection's (or the predicate's, for a plain filter) the output kernel
+  const int sel_rpt = rows_per_thread(pred);
+  const int rpt = rows_per_thread(proj ? proj : pred);
+  const int64_t sel_words = select_scratch_words(plan, L.nreg, sel_rpt);
+  if (sel_words < 0) fail("no kernel variant for this program");
+  // scratch: plan | selection ballots + slice counts/offsets | string
+  // assembly tile status + ticket | selected row ids (uint32) | string
+  // output slices
+  const int64_t st_words = n_str ? string_tiles(num_rows) + 1 : 0;
+  const int64_t id_words = (num_rows + 1) / 2;
+  uint64_t* scratch = nullptr;
+  e = get_scratch((hipStream_t)stream, sel_words + st_words + id_words + (int64_t)n_str * num_rows, &scratch);
+  if (e != hipSuccess) return device_error(e, "filter scratch");
+  uint64_t* sel = scratch + kPlanWords;
+  uint64_t* status = sel + sel_words;
+  uint32_t* ids = (uint32_t*)(status + st_words);
+  uint64_t* slices = status + st_words + id_words;
+  for (int k = 0, j = 0; k < n_out; ++k) {
+    const bool str = L.out_str_col[(size_t)k] >= 0;
+    plan.out[k].data = str ? (void*)(slices + (size_t)(j++) * (size_t)capacity) : (outputs ? out[k].data : nullptr);
+    plan.out[k].valid = outputs ? out[k].validity : nullptr;
+    plan.out[k].type = out[k].type_id;
   }
+  e = upload_plan(plan, scratch, (hipStream_t)stream);
+  if (e == hipSuccess)
+    e = launch_compact(plan, (const DevPlan*)scratch, L.nreg, sel_rpt, rpt, sel, ids, d_count, capacity,
+                       (hipStream_t)stream, phases);
+  if (e == hipSuccess && outputs)
+    e = finish_string_outputs(L, in, out, n_out, slices, d_count, 0, capacity, status, (hipStream_t)stream);
+  if (e != hipSuccess) return device_error(e, "filter");
+  return BSQL_OK;
+}
+
+}  // namespace bsql
 
 extern "C" {
 
@@ -451,68 +508,11 @@ int bsql_interp_execute(const bsql_program* cprog, const bsql_column* in, int n_
   })
 }
 
-static int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, int n_in,
-                       bsql_column* out, int n_out, int64_t num_rows, int64_t capacity,
-                       int64_t* d_count, void* stream) {
-  if (!pred) fail("null predicate");
-  if (!d_count) fail("null count pointer");
-  check_inputs(in, n_in, num_rows);
-  // selected rows are listed as 32-bit row ids
-  if (num_rows > (int64_t)UINT32_MAX) fail("a filter handles at most 2^32-1 rows per call");
-  check_outputs(out, n_out, capacity);
-  hipError_t e = ensure_strings(*pred);
-  if (e == hipSuccess && proj) e = ensure_strings(*proj);
-  if (e != hipSuccess) return device_error(e, "filter");
-  LowerOptions opt;
-  opt.mode = proj ? LowerOptions::FILTER_PROJECT
-                  : (n_out == 0 && !out ? LowerOptions::FILTER_COUNT : LowerOptions::FILTER_GATHER);
-  opt.pred = pred;
-  opt.proj = proj;
-  opt.conj_order = &pred->conj_order;
-  opt.ranges = !(pred->launch.flags & BSQL_LAUNCH_NO_RANGES);
-  opt.lazy_loads = (pred->launch.flags & BSQL_LAUNCH_LAZY_LOADS) != 0;
-  const Lowered& L = lowered_for(pred, opt, in, n_in, out, n_out);
-  DevPlan plan = L.plan;
-  plan.num_rows = num_rows;
-  for (int k = 0; k < n_in; ++k) plan.in[k] = to_dev_in(in[k]);
-  const int n_str = count_string_outputs(L, in, out, n_out);
-  // the predicate's launch shape drives the selection kernel, the
-  // projection's (or the predicate's, for a plain filter) the output kernel
-  const int sel_rpt = rows_per_thread(pred);
-  const int rpt = rows_per_thread(proj ? proj : pred);
-  const int64_t tiles = compact_tiles(plan, L.nreg, sel_rpt);
-  if (tiles < 0) fail("no kernel variant for this program");
-  // scratch: plan | tile status words + ticket (shared by the selection and
-  // the string assembly) | selected row ids (uint32) | string output slices
-  const int64_t st_words = std::max(tiles, n_str ? string_tiles(capacity) : 0) + 1;
-  const int64_t id_words = (num_rows + 1) / 2;
-  uint64_t* scratch = nullptr;
-  e = get_scratch((hipStream_t)stream, st_words + id_words + (int64_t)n_str * capacity, &scratch);
-  if (e != hipSuccess) return device_error(e, "filter scratch");
-  uint64_t* status = scratch + kPlanWords;
-  uint32_t* ids = (uint32_t*)(status + st_words);
-  uint64_t* slices = status + st_words + id_words;
-  for (int k = 0, j = 0; k < n_out; ++k) {
-    const bool str = L.out_str_col[(size_t)k] >= 0;
-    plan.out[k].data = str ? (void*)(slices + (size_t)(j++) * (size_t)capacity) : out[k].data;
-    plan.out[k].valid = out[k].validity;
-    plan.out[k].type = out[k].type_id;
-  }
-  e = upload_plan(plan, scratch, (hipStream_t)stream);
-  if (e == hipSuccess)
-    e = launch_compact(plan, (const DevPlan*)scratch, L.nreg, sel_rpt, rpt, status, ids, d_count, capacity,
-                       (hipStream_t)stream);
-  if (e == hipSuccess)
-    e = finish_string_outputs(L, in, out, n_out, slices, d_count, 0, capacity, status, (hipStream_t)stream);
-  if (e != hipSuccess) return device_error(e, "filter");
-  return BSQL_OK;
-}
-
 int bsql_filter(const bsql_program* pred, const bsql_column* in, int n_in, bsql_column* out,
                 int64_t num_rows, int64_t* d_count, void* stream) {
   BSQL_TRY({
     return run_compact(const_cast<bsql_program*>(pred), nullptr, in, n_in, out, n_in, num_rows,
-                       num_rows, d_count, stream);
+                       num_rows, d_count, stream, PHASE_ALL);
   })
 }
 
@@ -520,7 +520,7 @@ int bsql_filter_count(const bsql_program* pred, const bsql_column* in, int n_in,
                       int64_t* d_count, void* stream) {
   BSQL_TRY({
     return run_compact(const_cast<bsql_program*>(pred), nullptr, in, n_in, nullptr, 0, num_rows, num_rows,
-                       d_count, stream);
+                       d_count, stream, PHASE_ALL);
   })
 }
 
@@ -531,7 +531,7 @@ int bsql_filter_project_sized(const bsql_program* pred, const bsql_program* proj
     if (!proj) fail("null projection");
     if (capacity < 0 || capacity > num_rows) fail("capacity must be in [0, num_rows]");
     return run_compact(const_cast<bsql_program*>(pred), const_cast<bsql_program*>(proj), in, n_in, out, n_out,
-                       num_rows, capacity, d_count, stream);
+                       num_rows, capacity, d_count, stream, PHASE_ALL);
   })
 }
 
@@ -541,7 +541,7 @@ int bsql_filter_project(const bsql_program* pred, const bsql_program* proj, cons
   BSQL_TRY({
     if (!proj) fail("null projection");
     return run_compact(const_cast<bsql_program*>(pred), const_cast<bsql_program*>(proj), in, n_in,
-                       out, n_out, num_rows, num_rows, d_count, stream);
+                       out, n_out, num_rows, num_rows, d_count, stream, PHASE_ALL);
   })
 }
 

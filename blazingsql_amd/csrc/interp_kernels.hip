@@ -947,48 +947,98 @@ __device__ __forceinline__ void chunk_rows(Rows<R>& rs, int64_t row0, int64_t nu
   for (int k = 0; k < R; ++k) rs.row[k] = row0 + k;
 }
 
-// Compaction of one tile: `alive` bit k*R+j = row j of this lane in chunk k,
-// `start[k]` = the lane's first dense index within its wave for chunk k and
-// `wave_total` the wave's count. Workgroup prefix in LDS -> decoupled
-// look-back -> the selected row ids in row order through LDS to
-// idx[prefix, prefix + tile_total), one coalesced store per id.
+// ------------------------------------------------- selection output
+//
+// The selection kernels impose no order between waves: each wave owns
+// "slices" of S*R*64 rows and, for each, writes its selection as S*R 64-bit
+// ballots (ballot q = k*R + j, bit t = row k*64*R + t*R + j of the slice)
+// and its selected-row count. slice_scan (one workgroup) turns the counts
+// into slice offsets and the total; slice_ids writes the selected row ids in
+// row order. No workgroup waits on another, so no ticket and no look-back.
+struct SelArgs {
+  uint64_t* bits;      // n_slices * S * R ballots
+  uint32_t* counts;    // n_slices
+  uint32_t* offsets;   // n_slices, exclusive prefix of counts
+  int64_t n_slices;
+};
+
 template <int R, int S>
-__device__ __forceinline__ void compact_ids(const CompactArgs& ca, int64_t tile, int64_t tile_row0,
-                                            uint32_t alive, const int (&start)[S], int wave_total,
-                                            uint32_t* __restrict__ idx, uint16_t* s_rows,
-                                            int64_t* s_wave_total, int64_t* s_tile_prefix) {
-  typedef SelectTile<R, S> T;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  if (lane == 0) s_wave_total[wave] = wave_total;
-  __syncthreads();
-  int64_t wave_off = 0, tile_total = 0;
+__device__ __forceinline__ void emit_slice(const SelArgs& sa, int64_t slice, uint32_t alive) {
+  const int lane = threadIdx.x & 63;
+  uint64_t mine = 0;
+  uint32_t total = 0;
 #pragma unroll
-  for (int w = 0; w < WAVES; ++w) {
-    const int64_t t = s_wave_total[w];
-    if (w < wave) wave_off += t;
-    tile_total += t;
+  for (int q = 0; q < S * R; ++q) {
+    const uint64_t b = __ballot((alive >> q) & 1u);
+    total += (uint32_t)__popcll(b);
+    mine = lane == q ? b : mine;
   }
-  if (wave == 0) {
-    const int64_t excl = look_back(ca, tile, tile_total, lane);
-    if (lane == 0) {
-      *s_tile_prefix = excl;
-      if (tile == ca.num_tiles - 1) *ca.d_count = excl + tile_total;
+  if (lane < S * R) gpw<uint64_t>(sa.bits)[slice * (S * R) + lane] = mine;
+  if (lane == 0) gpw<uint32_t>(sa.counts)[slice] = total;
+}
+
+// Exclusive prefix of the slice counts and the total (one workgroup of
+// SCAN_THREADS; each thread scans a contiguous run of counts).
+constexpr int SCAN_THREADS = 1024;
+__global__ __launch_bounds__(SCAN_THREADS) void slice_scan(const uint32_t* __restrict__ counts,
+                                                           uint32_t* __restrict__ offsets, int64_t n,
+                                                           int64_t* __restrict__ d_count) {
+  __shared__ uint64_t s_sum[SCAN_THREADS];
+  const int t = threadIdx.x;
+  const int64_t per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
+  const int64_t lo = per * t < n ? per * t : n;
+  const int64_t hi = lo + per < n ? lo + per : n;
+  uint64_t sum = 0;
+  for (int64_t i = lo; i < hi; ++i) sum += gp<uint32_t>(counts)[i];
+  s_sum[t] = sum;
+  __syncthreads();
+  // Hillis-Steele inclusive scan over the per-thread sums
+  for (int d = 1; d < SCAN_THREADS; d <<= 1) {
+    const uint64_t v = t >= d ? s_sum[t - d] : 0;
+    __syncthreads();
+    s_sum[t] += v;
+    __syncthreads();
+  }
+  uint64_t run = s_sum[t] - sum;
+  for (int64_t i = lo; i < hi; ++i) {
+    gpw<uint32_t>(offsets)[i] = (uint32_t)run;
+    run += gp<uint32_t>(counts)[i];
+  }
+  if (t == SCAN_THREADS - 1) *gpw<int64_t>(d_count) = (int64_t)s_sum[t];
+}
+
+// Row ids of the selected rows, in row order: wave-strided over slices.
+template <int R, int S>
+__global__ __launch_bounds__(BLOCK) void slice_ids(SelArgs sa, uint32_t* __restrict__ idx) {
+  constexpr int CHUNK = 64 * R;
+  constexpr int SLICE = S * CHUNK;
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = lane ? (~0ULL >> (64 - lane)) : 0ULL;
+  GLOBAL uint32_t* ids = gpw<uint32_t>(idx);
+  const int64_t waves = (int64_t)gridDim.x * WAVES;
+  for (int64_t slice = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6); slice < sa.n_slices; slice += waves) {
+    const uint64_t mine = lane < S * R ? gp<uint64_t>(sa.bits)[slice * (S * R) + lane] : 0ULL;
+    uint32_t off = gp<uint32_t>(sa.offsets)[slice];
+    const uint32_t row0 = (uint32_t)(slice * SLICE) + (uint32_t)(lane * R);
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      uint64_t b[R];
+      uint32_t before = 0, chunk_total = 0;
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)mine, k * R + j);
+        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(mine >> 32), k * R + j);
+        b[j] = ((uint64_t)hi << 32) | lo;
+        before += (uint32_t)__popcll(b[j] & below);
+        chunk_total += (uint32_t)__popcll(b[j]);
+      }
+      uint32_t pos = off + before;
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if ((b[j] >> lane) & 1ULL) ids[pos++] = row0 + (uint32_t)(k * CHUNK + j);
+      off += chunk_total;
     }
   }
-#pragma unroll
-  for (int k = 0; k < S; ++k) {
-    int d = (int)wave_off + start[k];
-    const int base = wave * T::WAVE_ROWS + k * T::CHUNK + lane * R;
-#pragma unroll
-    for (int j = 0; j < R; ++j)
-      if ((alive >> (k * R + j)) & 1u) s_rows[d++] = (uint16_t)(base + j);
-  }
-  __syncthreads();
-  const int64_t prefix = *s_tile_prefix;
-  GLOBAL uint32_t* ids = gpw<uint32_t>(idx);
-  for (int j = tid; j < (int)tile_total; j += BLOCK) ids[prefix + j] = (uint32_t)(tile_row0 + s_rows[j]);
 }
 
 // Counts of one chunk's selection bits `a` (R bits per lane): the lane's
@@ -1158,108 +1208,75 @@ __device__ __forceinline__ uint32_t range_test(const DevInCol& c, const DevRange
 }
 
 // Row selection by range tests (plan.ranges): one range at a time over the
-// wave's whole share of the tile, so each column is loaded with every chunk
-// in flight and its type is dispatched once per wave; no register file.
+// wave's whole slice, so each column is loaded with every chunk in flight
+// and its type is dispatched once per slice; no register file. Waves stride
+// over slices independently.
 template <int R, int S>
-__global__ __launch_bounds__(BLOCK) void select_ranges(const DevPlan* __restrict__ pplan, CompactArgs ca,
-                                                       uint32_t* __restrict__ idx) {
+__global__ __launch_bounds__(BLOCK) void select_ranges(const DevPlan* __restrict__ pplan, SelArgs sa) {
   typedef SelectTile<R, S> T;
   const DevPlan& plan = *pplan;
-  __shared__ uint32_t s_ticket;
-  __shared__ int64_t s_wave_total[WAVES];
-  __shared__ int64_t s_tile_prefix;
-  __shared__ uint16_t s_rows[T::TILE];
-  if (threadIdx.x == 0) s_ticket = atomicAdd(ca.ticket, 1u);
-  __syncthreads();
-  const int64_t tile = s_ticket;
-  const int64_t tile_row0 = tile * T::TILE;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
   const int64_t num_rows = plan.num_rows;
-  const int64_t row0 = tile_row0 + wave * T::WAVE_ROWS + lane * R;
-  uint32_t alive = 0;
-#pragma unroll
-  for (int k = 0; k < S; ++k) {
-    const int64_t left = num_rows - (row0 + (int64_t)k * T::CHUNK);
-    const int n = left <= 0 ? 0 : (left >= R ? R : (int)left);
-    alive |= ((1u << n) - 1u) << (k * R);
-  }
-  const bool full = tile_row0 + T::TILE <= num_rows;
   const int nr = plan.n_ranges;
-  for (int r = 0; r < nr; ++r) {
-    if (__ballot(alive != 0) == 0) break;
-    const DevRange rg = plan.ranges[r];
-    if (rg.kind & RK_EMPTY) {
-      alive = 0;
-      break;
-    }
-    if (full)
-      alive = range_test<R, S, true>(plan.in[rg.col], rg, row0, num_rows, alive);
-    else
-      alive = range_test<R, S, false>(plan.in[rg.col], rg, row0, num_rows, alive);
-  }
-  int start[S];
-  int wave_total = 0;
+  const int64_t waves = (int64_t)gridDim.x * WAVES;
+  for (int64_t slice = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6); slice < sa.n_slices; slice += waves) {
+    const int64_t slice_row0 = slice * T::WAVE_ROWS;
+    const int64_t row0 = slice_row0 + lane * R;
+    uint32_t alive = 0;
 #pragma unroll
-  for (int k = 0; k < S; ++k) {
-    int excl, tot;
-    chunk_counts<R>((alive >> (k * R)) & ((1u << R) - 1u), excl, tot);
-    start[k] = wave_total + excl;
-    wave_total += tot;
+    for (int k = 0; k < S; ++k) {
+      const int64_t left = num_rows - (row0 + (int64_t)k * T::CHUNK);
+      const int n = left <= 0 ? 0 : (left >= R ? R : (int)left);
+      alive |= ((1u << n) - 1u) << (k * R);
+    }
+    const bool full = slice_row0 + T::WAVE_ROWS <= num_rows;
+    for (int r = 0; r < nr; ++r) {
+      if (__ballot(alive != 0) == 0) break;
+      const DevRange rg = plan.ranges[r];
+      if (rg.kind & RK_EMPTY) {
+        alive = 0;
+        break;
+      }
+      if (full)
+        alive = range_test<R, S, true>(plan.in[rg.col], rg, row0, num_rows, alive);
+      else
+        alive = range_test<R, S, false>(plan.in[rg.col], rg, row0, num_rows, alive);
+    }
+    emit_slice<R, S>(sa, slice, alive);
   }
-  compact_ids<R, S>(ca, tile, tile_row0, alive, start, wave_total, idx, s_rows, s_wave_total,
-                    &s_tile_prefix);
 }
 
 // Row selection by ops [0, split) (KILL per conjunct with a wave-uniform
-// early exit, or one SEL), chunk by chunk.
+// early exit, or one SEL), chunk by chunk over the wave's slice.
 template <int NREG, int R, int S>
-__global__ __launch_bounds__(BLOCK) void select_vm(const DevPlan* __restrict__ pplan, CompactArgs ca,
-                                                   uint32_t* __restrict__ idx) {
+__global__ __launch_bounds__(BLOCK) void select_vm(const DevPlan* __restrict__ pplan, SelArgs sa) {
   typedef typename RegVecT<NREG>::type RegVec;
   typedef SelectTile<R, S> T;
   const DevPlan& plan = *pplan;
-  __shared__ uint32_t s_ticket;
-  __shared__ int64_t s_wave_total[WAVES];
-  __shared__ int64_t s_tile_prefix;
-  __shared__ uint16_t s_rows[T::TILE];
-  if (threadIdx.x == 0) s_ticket = atomicAdd(ca.ticket, 1u);
-  __syncthreads();
-  const int64_t tile = s_ticket;
-  const int64_t tile_row0 = tile * T::TILE;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  RegVec f[R];
+  const int64_t waves = (int64_t)gridDim.x * WAVES;
+  for (int64_t slice = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6); slice < sa.n_slices; slice += waves) {
+    RegVec f[R];
 #pragma unroll
-  for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
-  uint64_t vb = 0;
-  LoadTemps<R> L;
-  uint32_t alive_all = 0;
-  int start[S];
-  int wave_total = 0;
+    for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
+    uint64_t vb = 0;
+    LoadTemps<R> L;
+    uint32_t alive_all = 0;
 #pragma unroll 1
-  for (int k = 0; k < S; ++k) {
-    Rows<R> rs;
-    chunk_rows<R>(rs, tile_row0 + wave * T::WAVE_ROWS + k * T::CHUNK + lane * R, plan.num_rows);
-    uint32_t alive = rs.exist;
-    if (__ballot(alive != 0) != 0) {
-      for (int q = 0; q < plan.split; ++q) {
-        const DevOp op = load_op(plan, q);
-        if (vm_step<NREG, R>(plan, op, f, vb, L, rs, alive)) break;
+    for (int k = 0; k < S; ++k) {
+      Rows<R> rs;
+      chunk_rows<R>(rs, slice * T::WAVE_ROWS + k * T::CHUNK + lane * R, plan.num_rows);
+      uint32_t alive = rs.exist;
+      if (__ballot(alive != 0) != 0) {
+        for (int q = 0; q < plan.split; ++q) {
+          const DevOp op = load_op(plan, q);
+          if (vm_step<NREG, R>(plan, op, f, vb, L, rs, alive)) break;
+        }
       }
+      alive_all |= alive << (k * R);
     }
-    int excl, tot;
-    chunk_counts<R>(alive, excl, tot);
-    // start[] is indexed by the loop counter: keep it in registers by
-    // selecting instead of indexing
-#pragma unroll
-    for (int c = 0; c < S; ++c)
-      if (c == k) start[c] = wave_total + excl;
-    wave_total += tot;
-    alive_all |= alive << (k * R);
+    emit_slice<R, S>(sa, slice, alive_all);
   }
-  compact_ids<R, S>(ca, tile, tile_row0, alive_all, start, wave_total, idx, s_rows, s_wave_total,
-                    &s_tile_prefix);
 }
 
 // ------------------------------------------------------- string outputs
@@ -1409,11 +1426,25 @@ template <int R> struct VmChunks { static constexpr int S = 16 / R; };
 // 8192-row tiles) by default, or 4 chunks (sel_rpt == 16)
 constexpr int RANGE_R = 4;
 
+// One selection launch: waves stride over the slices; then the scan, and
+// the row ids when an output phase will read them.
+template <int R, int S>
+static int64_t slice_grid(int64_t n_slices) {
+  return persistent_grid((n_slices + WAVES - 1) / WAVES);
+}
+
 template <int NREG, int R>
-static hipError_t launch_select_vm_t(const DevPlan* d_plan, const CompactArgs& ca, uint32_t* idx,
-                                     hipStream_t stream) {
-  hipLaunchKernelGGL((select_vm<NREG, R, VmChunks<R>::S>), dim3((unsigned)ca.num_tiles), dim3(BLOCK), 0,
-                     stream, d_plan, ca, idx);
+static hipError_t launch_select_vm_t(const DevPlan* d_plan, const SelArgs& sa, hipStream_t stream) {
+  constexpr int S = VmChunks<R>::S;
+  hipLaunchKernelGGL((select_vm<NREG, R, S>), dim3((unsigned)slice_grid<R, S>(sa.n_slices)), dim3(BLOCK), 0,
+                     stream, d_plan, sa);
+  return hipGetLastError();
+}
+
+template <int R, int S>
+static hipError_t launch_slice_ids(const SelArgs& sa, uint32_t* idx, hipStream_t stream) {
+  hipLaunchKernelGGL((slice_ids<R, S>), dim3((unsigned)slice_grid<R, S>(sa.n_slices)), dim3(BLOCK), 0, stream,
+                     sa, idx);
   return hipGetLastError();
 }
 
@@ -1459,54 +1490,37 @@ hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
 }
 
 // Rows per tile of the selection kernel the plan uses.
-static int64_t select_tile_rows(const DevPlan& plan, int nreg, int rpt) {
-  if (plan.n_ranges > 0) return rpt == 16 ? SelectTile<RANGE_R, 4>::TILE : SelectTile<RANGE_R, 8>::TILE;
-  int NR, RR;
-  if (!pick(nreg, rpt, &NR, &RR)) return -1;
-  return (int64_t)BLOCK * 16;  // R * VmChunks<R>::S == 16 rows per lane
-}
-
-int64_t compact_tiles(const DevPlan& plan, int nreg, int rpt) {
-  const int64_t tile = select_tile_rows(plan, nreg, rpt);
-  if (tile < 0) return -1;
-  return (plan.num_rows + tile - 1) / tile;
-}
-
-// Filter (+ project): selection kernel -> ids of the selected rows in `idx`
-// (num_rows entries) and their count in *d_count, then the output ops over
-// those rows. `status` holds compact_tiles(...) + 1 words (the last one is
-// the ticket).
-hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int sel_rpt, int rpt,
-                          uint64_t* status, uint32_t* idx, int64_t* d_count, int64_t capacity,
-                          hipStream_t stream) {
-  const int64_t tile = select_tile_rows(plan, nreg, sel_rpt);
-  if (tile < 0) return hipErrorInvalidValue;
-  const int64_t tiles = (plan.num_rows + tile - 1) / tile;
-  CompactArgs ca{status, (uint32_t*)(status + tiles), d_count, capacity, tiles};
-  // clear tile status, ticket and the count (fill kernels, so the sequence can
-  // be captured in a hipGraph)
-  const int64_t nclear = tiles + 1;
-  hipLaunchKernelGGL(fill_u64, dim3((unsigned)((nclear + 255) / 256)), dim3(256), 0, stream,
-                     status, nclear, 0ULL);
-  hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, stream, (uint64_t*)d_count, (int64_t)1, 0ULL);
-  if (tiles == 0) return hipGetLastError();
-  hipError_t e = hipErrorInvalidValue;
+// Rows per selection slice and ballots per slice (R * S) of the kernel a
+// plan selects with.
+static bool slice_shape(const DevPlan& plan, int nreg, int rpt, int64_t* rows, int* ballots) {
   if (plan.n_ranges > 0) {
-    if (sel_rpt == 16)
-      hipLaunchKernelGGL((select_ranges<RANGE_R, 4>), dim3((unsigned)tiles), dim3(BLOCK), 0, stream, d_plan,
-                         ca, idx);
-    else
-      hipLaunchKernelGGL((select_ranges<RANGE_R, 8>), dim3((unsigned)tiles), dim3(BLOCK), 0, stream, d_plan,
-                         ca, idx);
-    e = hipGetLastError();
-  } else {
-    int NR, RR;
-    if (!pick(nreg, sel_rpt, &NR, &RR)) return hipErrorInvalidValue;
-#define X(N, RPT) if (NR == N && RR == RPT) e = launch_select_vm_t<N, RPT>(d_plan, ca, idx, stream);
-    BSQL_VARIANTS(X)
-#undef X
+    *rows = rpt == 16 ? SelectTile<RANGE_R, 4>::WAVE_ROWS : SelectTile<RANGE_R, 8>::WAVE_ROWS;
+    *ballots = rpt == 16 ? RANGE_R * 4 : RANGE_R * 8;
+    return true;
   }
-  if (e != hipSuccess || plan.split >= plan.n_ops) return e;
+  int NR, RR;
+  if (!pick(nreg, rpt, &NR, &RR)) return false;
+  *rows = 64 * 16;  // R * VmChunks<R>::S == 16 rows per lane
+  *ballots = 16;
+  return true;
+}
+
+// Scratch words of the selection: the ballots, then the slice counts and
+// offsets (two uint32 per slice).
+int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt) {
+  int64_t rows;
+  int ballots;
+  if (!slice_shape(plan, nreg, rpt, &rows, &ballots)) return -1;
+  const int64_t n = (plan.num_rows + rows - 1) / rows;
+  return n * ballots + n;
+}
+
+// The output phase of a filter: ops [split, n_ops) over the selected row ids
+// (written by the selection phase, count in *d_count).
+static hipError_t launch_compact_gather(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt,
+                                        const uint32_t* idx, const int64_t* d_count, int64_t capacity,
+                                        hipStream_t stream) {
+  if (plan.split >= plan.n_ops) return hipSuccess;
   int NR, RR;
   if (!pick(nreg, rpt, &NR, &RR)) return hipErrorInvalidValue;
 #define X(N, RPT) \
@@ -1514,6 +1528,64 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
   BSQL_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;
+}
+
+// Filter (+ project): selection kernel -> ids of the selected rows in `idx`
+// (num_rows entries) and their count in *d_count, then the output ops over
+// those rows. `sel` holds select_scratch_words(...) words. `phases`: bit 0 =
+// selection (+ scan + ids), bit 1 = output phase (a caller that sizes its
+// outputs from the count runs them as two calls on one stream, with idx and
+// *d_count kept in between). Nothing is cleared first: every slice writes
+// its ballots and count, and the scan writes *d_count.
+hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int sel_rpt, int rpt,
+                          uint64_t* sel, uint32_t* idx, int64_t* d_count, int64_t capacity,
+                          hipStream_t stream, int phases) {
+  if (!(phases & 1)) return launch_compact_gather(plan, d_plan, nreg, rpt, idx, d_count, capacity, stream);
+  int64_t rows;
+  int ballots;
+  if (!slice_shape(plan, nreg, sel_rpt, &rows, &ballots)) return hipErrorInvalidValue;
+  const int64_t n = (plan.num_rows + rows - 1) / rows;
+  SelArgs sa{sel, (uint32_t*)(sel + n * ballots), (uint32_t*)(sel + n * ballots) + n, n};
+  hipError_t e = hipSuccess;
+  const bool ids = plan.split < plan.n_ops;
+  if (n > 0) {
+    e = hipErrorInvalidValue;
+    if (plan.n_ranges > 0) {
+      if (sel_rpt == 16) {
+        hipLaunchKernelGGL((select_ranges<RANGE_R, 4>), dim3((unsigned)slice_grid<RANGE_R, 4>(n)), dim3(BLOCK), 0,
+                           stream, d_plan, sa);
+      } else {
+        hipLaunchKernelGGL((select_ranges<RANGE_R, 8>), dim3((unsigned)slice_grid<RANGE_R, 8>(n)), dim3(BLOCK), 0,
+                           stream, d_plan, sa);
+      }
+      e = hipGetLastError();
+    } else {
+      int NR, RR;
+      if (!pick(nreg, sel_rpt, &NR, &RR)) return hipErrorInvalidValue;
+#define X(N, RPT) if (NR == N && RR == RPT) e = launch_select_vm_t<N, RPT>(d_plan, sa, stream);
+      BSQL_VARIANTS(X)
+#undef X
+    }
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(slice_scan, dim3(1), dim3(SCAN_THREADS), 0, stream, (const uint32_t*)sa.counts, sa.offsets,
+                     n, d_count);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (ids && n > 0) {
+    if (plan.n_ranges > 0) {
+      e = sel_rpt == 16 ? launch_slice_ids<RANGE_R, 4>(sa, idx, stream)
+                        : launch_slice_ids<RANGE_R, 8>(sa, idx, stream);
+    } else {
+      int NR, RR;
+      if (!pick(nreg, sel_rpt, &NR, &RR)) return hipErrorInvalidValue;
+      e = RR == 1 ? launch_slice_ids<1, VmChunks<1>::S>(sa, idx, stream)
+          : RR == 2 ? launch_slice_ids<2, VmChunks<2>::S>(sa, idx, stream)
+                    : launch_slice_ids<4, VmChunks<4>::S>(sa, idx, stream);
+    }
+  }
+  if (e != hipSuccess || !(phases & 2)) return e;
+  return launch_compact_gather(plan, d_plan, nreg, rpt, idx, d_count, capacity, stream);
 }
 
 // Batched launch: `plans` (device) holds n_batches plans, `tile_start`

@@ -11,6 +11,8 @@
 //   get_scalar_from_string       engine/src/parser/CalciteExpressionParsing.cpp:138-262
 //   used-column remap            engine/src/execution_kernels/LogicalProject.cpp:859-881
 //   expr_to_plan_visitor         engine/src/Interpreter/interpreter_cpp.cu:274-438
+//   get_named_expression         engine/src/parser/expression_utils.cpp:537-547
+//   get_query_part               engine/src/parser/expression_utils.cpp:764-771
 // The slot numbering is the reference's: inputs 0..n-1 (used columns in table
 // order), final outputs n..n+e-1, temporaries from n+e with first-free reuse.
 #include <algorithm>
@@ -779,6 +781,40 @@ std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exp
   validate(*prog, n_inputs);
   (void)in_types;
   return prog;
+}
+
+// ------------------------------------------------------------ query parts
+
+std::string named_expression(const std::string& query_part, const std::string& name) {
+  const size_t single = query_part.find(name + "=[");
+  if (single == std::string::npos) return "";
+  const size_t dbl = query_part.find(name + "=[[");
+  const size_t start = dbl != std::string::npos ? dbl + name.size() + 3 : single + name.size() + 2;
+  const size_t end = query_part.find(']', start);
+  return query_part.substr(start, end == std::string::npos ? std::string::npos : end - start);
+}
+
+std::string query_part_body(const std::string& logical_plan) {
+  const size_t open = logical_plan.find('(');
+  const size_t close = logical_plan.rfind(')');
+  if (open == std::string::npos || close == std::string::npos || close <= open)
+    fail("query part without a parenthesized body: " + logical_plan);
+  return logical_plan.substr(open + 1, close - open - 1);
+}
+
+std::vector<std::string> expression_list(const std::string& combined) {
+  return split_expression_list(replace_calcite_regex(combined), true);
+}
+
+void project_items(const std::string& query_part, std::vector<std::string>* names,
+                   std::vector<std::string>* exprs) {
+  for (const std::string& item : expression_list(query_part_body(query_part))) {
+    const size_t eq = item.find("=[");
+    if (eq == std::string::npos || item.size() < eq + 3 || item.back() != ']')
+      fail("projection item is not name=[expression]: " + item);
+    names->push_back(item.substr(0, eq));
+    exprs->push_back(item.substr(eq + 2, item.size() - eq - 3));
+  }
 }
 
 }  // namespace bsql

@@ -845,8 +845,10 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
   if (opt.mode == LowerOptions::FILTER_COUNT) {
     if (n_out != 0) fail("a count has no output columns");
   } else if (opt.mode == LowerOptions::FILTER_GATHER) {
-    if (n_out != n_in) fail("filter gathers every input column");
-    for (int k = 0; k < n_in; ++k) store_value.push_back(L.load_of(k));
+    // the first n_out input columns are gathered (the rest only feed the
+    // predicate, e.g. an appended mask column)
+    if (n_out > n_in) fail("a filter gathers input columns only");
+    for (int k = 0; k < n_out; ++k) store_value.push_back(L.load_of(k));
   } else {
     const Program& proj = *opt.proj;
     if ((int)proj.final_out.size() != n_out) fail("number of outputs does not match the program");

@@ -123,4 +123,16 @@ std::vector<DevRange> conjunct_ranges(const Program& p, const int32_t* in_types,
 std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exprs,
                                              const int32_t* in_types, int n_in);
 
+// Query-part text of the operators (parser.cpp).
+// `name=[[...]]` or `name=[...]` of a query part; "" when absent.
+std::string named_expression(const std::string& query_part, const std::string& name);
+// The text between the first '(' and the last ')'.
+std::string query_part_body(const std::string& logical_plan);
+// Comma-separated list at bracket/paren depth 0 outside quotes, after the
+// Calcite regex rewrites; items trimmed.
+std::vector<std::string> expression_list(const std::string& combined);
+// `name=[expr]` items of a LogicalProject: (names, expressions).
+void project_items(const std::string& query_part, std::vector<std::string>* names,
+                   std::vector<std::string>* exprs);
+
 }  // namespace bsql

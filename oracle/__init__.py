@@ -92,6 +92,15 @@ class HostColumn:
         self.valid = None if valid is None else np.asarray(valid, dtype=bool)
         self._words = None if self.valid is None else _pack(self.valid)
 
+    @staticmethod
+    def strings_from_arrays(offsets, chars, valid=None):
+        """A STRING column straight from int32 offsets (n+1) and uint8 chars."""
+        col = HostColumn(STRING, strings=[], valid=valid)
+        col.offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+        col.chars = np.concatenate([np.asarray(chars, dtype=np.uint8), np.zeros(1, dtype=np.uint8)])
+        col.size = len(col.offsets) - 1
+        return col
+
     def c(self):
         c = _Column()
         c.type_id = self.type_id

@@ -15,8 +15,10 @@ INT32, INT64, FLOAT64, BOOL8, TS_D, TS_S, STRING = 3, 4, 10, 11, 12, 13, 23
 
 def test_abi_exports_every_declared_symbol(bsql):
     from blazingsql_amd import _lib
-    header = open(os.path.join(ROOT, "include", "bsql", "interp.h")).read()
-    declared = set(re.findall(r"\b(bsql_[a-z_]+)\s*\(", header))
+    declared = set()
+    for h in ("interp.h", "table.h"):
+        header = open(os.path.join(ROOT, "include", "bsql", h)).read()
+        declared |= set(re.findall(r"\b(bsql_[a-z_]+)\s*\(", header))
     lib = _lib.lib()
     missing = [s for s in sorted(declared) if not hasattr(lib, s)]
     assert not missing, missing

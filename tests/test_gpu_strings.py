@@ -130,3 +130,51 @@ def test_filter_with_string_columns_vs_oracle(bsql, gpu, ranges):
         assert m == on, pred_text
         for k in range(3):
             _same(outs[k].to_numpy(m), ores[k], f"{pred_text} col {k}")
+
+
+@pytest.mark.parametrize("n", [5000, 200_003])
+def test_strings_math_queries_vs_oracle(bsql, gpu, n):
+    """The strings+math workload's four filter+project queries (LIKE prefix,
+    CHAR_LENGTH, SUBSTRING equality, their AND; SIN/LN/POWER/FLOOR, a wrapping
+    int64 product, int64 division by a column with zeros) against the
+    oracle: bit-exact except the transcendentals (within 1 ulp of libm)."""
+    import oracle
+    import torch
+    import bench
+    from blazingsql_amd.interop import DeviceColumn, filter_project
+    from tests.test_gpu_parity import _full_prog
+    rng = np.random.default_rng(n)
+    lens = rng.integers(4, 33, n)
+    offs = np.zeros(n + 1, dtype=np.int32)
+    offs[1:] = np.cumsum(lens)
+    # a small alphabet so that every predicate selects rows at these sizes
+    chars = rng.choice(np.frombuffer(b"abcd", dtype=np.uint8), int(offs[-1]))
+    x = rng.integers(-(1 << 63), (1 << 63) - 1, n, dtype=np.int64, endpoint=True)
+    y = (1.0 - rng.random(n)) * 1000.0
+    z = (rng.integers(1, 1000, n) * (rng.random(n) >= 0.1)).astype(np.int64)
+    types = bench.STR_TYPES
+    dev = [DeviceColumn.from_numpy(STRING, None, chars=chars, offsets=offs), DeviceColumn.from_numpy(INT64, x),
+           DeviceColumn.from_numpy(FLOAT64, y), DeviceColumn.from_numpy(INT64, z)]
+    host = [oracle.HostColumn.strings_from_arrays(offs, chars), oracle.HostColumn(INT64, x),
+            oracle.HostColumn(FLOAT64, y), oracle.HostColumn(INT64, z)]
+    proj = bsql.Program.compile(bench.STR_PROJ, types)
+    ot = [proj.output_type(types, k) for k in range(proj.num_outputs)]
+    assert ot == [FLOAT64, FLOAT64, FLOAT64, FLOAT64, INT64, INT64]
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    for pred_text in bench.STR_PREDS:
+        pred = bsql.Program.compile(pred_text, types)
+        outs = [DeviceColumn.empty(t, n) for t in ot]
+        filter_project(pred, proj, dev, outs, cnt, n)
+        m = int(cnt.item())
+        om, ores = oracle.filter_project(_full_prog(bsql, pred_text, types), _full_prog(bsql, bench.STR_PROJ, types),
+                                         host, ot, n)
+        assert m == om and m > 0, pred_text
+        for k in range(len(ot)):
+            gv, gval = outs[k].to_numpy(m)
+            ov, oval = ores[k]
+            np.testing.assert_array_equal(gval, oval, err_msg=f"{pred_text} out {k}")
+            if k < 3:
+                a, b = gv[gval], ov[oval]
+                assert np.all(np.abs(a - b) <= np.spacing(np.abs(b))), f"{pred_text} out {k}"
+            else:
+                assert_same(gv, gval, ov, oval, what=f"{pred_text} out {k}")
