@@ -865,8 +865,12 @@ __device__ __forceinline__ void vm_tile(const DevPlan& plan, int64_t tile, int64
     for (int k = 0; k < R; ++k) rs.row[k] = rs.pos0 + k;
   }
   uint32_t alive = rs.exist;
+  // the next op's scalar loads are issued before this op runs, so their
+  // latency hides behind it
+  DevOp next = load_op(plan, plan.split);
   for (int q = plan.split; q < plan.n_ops; ++q) {
-    const DevOp op = load_op(plan, q);
+    const DevOp op = next;
+    next = load_op(plan, q + 1 < BSQL_DEV_MAX_OPS ? q + 1 : q);
     if (op.code != D_STORE) {
       vm_step<NREG, R>(plan, op, f, vb, T, rs, alive);
       continue;
@@ -949,21 +953,25 @@ __device__ __forceinline__ void chunk_rows(Rows<R>& rs, int64_t row0, int64_t nu
 
 // ------------------------------------------------- selection output
 //
-// The selection kernels impose no order between waves: each wave owns
-// "slices" of S*R*64 rows and, for each, writes its selection as S*R 64-bit
-// ballots (ballot q = k*R + j, bit t = row k*64*R + t*R + j of the slice)
-// and its selected-row count. slice_scan (one workgroup) turns the counts
-// into slice offsets and the total; slice_ids writes the selected row ids in
-// row order. No workgroup waits on another, so no ticket and no look-back.
+// The selection kernels impose no order between workgroups: workgroup g owns
+// the contiguous block of "slices" [g*per, (g+1)*per) (a slice is one wave's
+// S*R*64 rows). For each slice a wave writes S*R 64-bit ballots (ballot
+// q = k*R + j, bit t = row k*64*R + t*R + j of the slice) and the selected
+// count; the workgroup writes its block's total. block_scan (one workgroup)
+// turns the block totals into block bases and the row total; slice_ids
+// rebuilds the slice offsets of its block in LDS and writes the selected row
+// ids in row order. No workgroup waits on another: no ticket, no look-back.
 struct SelArgs {
-  uint64_t* bits;      // n_slices * S * R ballots
-  uint32_t* counts;    // n_slices
-  uint32_t* offsets;   // n_slices, exclusive prefix of counts
+  uint64_t* bits;        // n_slices * S * R ballots
+  uint32_t* counts;      // n_slices
+  uint32_t* block_sums;  // grid: selected rows per workgroup block
+  uint32_t* block_base;  // grid: exclusive prefix of block_sums
   int64_t n_slices;
+  int64_t per;           // slices per workgroup block
 };
 
 template <int R, int S>
-__device__ __forceinline__ void emit_slice(const SelArgs& sa, int64_t slice, uint32_t alive) {
+__device__ __forceinline__ uint32_t emit_slice(const SelArgs& sa, int64_t slice, uint32_t alive) {
   const int lane = threadIdx.x & 63;
   uint64_t mine = 0;
   uint32_t total = 0;
@@ -975,69 +983,112 @@ __device__ __forceinline__ void emit_slice(const SelArgs& sa, int64_t slice, uin
   }
   if (lane < S * R) gpw<uint64_t>(sa.bits)[slice * (S * R) + lane] = mine;
   if (lane == 0) gpw<uint32_t>(sa.counts)[slice] = total;
+  return total;
 }
 
-// Exclusive prefix of the slice counts and the total (one workgroup of
-// SCAN_THREADS; each thread scans a contiguous run of counts).
-constexpr int SCAN_THREADS = 1024;
-__global__ __launch_bounds__(SCAN_THREADS) void slice_scan(const uint32_t* __restrict__ counts,
-                                                           uint32_t* __restrict__ offsets, int64_t n,
-                                                           int64_t* __restrict__ d_count) {
-  __shared__ uint64_t s_sum[SCAN_THREADS];
-  const int t = threadIdx.x;
-  const int64_t per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
-  const int64_t lo = per * t < n ? per * t : n;
-  const int64_t hi = lo + per < n ? lo + per : n;
-  uint64_t sum = 0;
-  for (int64_t i = lo; i < hi; ++i) sum += gp<uint32_t>(counts)[i];
-  s_sum[t] = sum;
+// The block total of a selection workgroup (every thread calls it once,
+// after its wave's last slice).
+__device__ __forceinline__ void emit_block(const SelArgs& sa, uint32_t wave_total) {
+  __shared__ uint32_t s_sum[WAVES];
+  if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = wave_total;
   __syncthreads();
-  // Hillis-Steele inclusive scan over the per-thread sums
-  for (int d = 1; d < SCAN_THREADS; d <<= 1) {
-    const uint64_t v = t >= d ? s_sum[t - d] : 0;
-    __syncthreads();
-    s_sum[t] += v;
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) t += s_sum[w];
+    gpw<uint32_t>(sa.block_sums)[blockIdx.x] = t;
   }
-  uint64_t run = s_sum[t] - sum;
-  for (int64_t i = lo; i < hi; ++i) {
-    gpw<uint32_t>(offsets)[i] = (uint32_t)run;
-    run += gp<uint32_t>(counts)[i];
-  }
-  if (t == SCAN_THREADS - 1) *gpw<int64_t>(d_count) = (int64_t)s_sum[t];
 }
 
-// Row ids of the selected rows, in row order: wave-strided over slices.
+// Exclusive prefix of the block totals, and the row total (one workgroup).
+constexpr int SCAN_THREADS = 1024;
+__global__ __launch_bounds__(SCAN_THREADS) void block_scan(const uint32_t* __restrict__ sums,
+                                                           uint32_t* __restrict__ base, int64_t n,
+                                                           int64_t* __restrict__ d_count) {
+  __shared__ uint64_t s_part[SCAN_THREADS / 64];
+  uint64_t carry = 0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t i0 = 0; i0 < n; i0 += SCAN_THREADS) {
+    const int64_t i = i0 + threadIdx.x;
+    const uint64_t v = i < n ? gp<uint32_t>(sums)[i] : 0;
+    uint64_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t u = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += u;
+    }
+    if (lane == 63) s_part[wave] = incl;
+    __syncthreads();
+    uint64_t before = carry;
+    for (int w = 0; w < wave; ++w) before += s_part[w];
+    if (i < n) gpw<uint32_t>(base)[i] = (uint32_t)(before + incl - v);
+    uint64_t all = 0;
+    for (int w = 0; w < SCAN_THREADS / 64; ++w) all += s_part[w];
+    __syncthreads();
+    carry += all;
+  }
+  if (threadIdx.x == 0) *gpw<int64_t>(d_count) = (int64_t)carry;
+}
+
+// Row ids of the selected rows, in row order. Workgroup g takes the same
+// block of slices as in the selection; the block's slice offsets are scanned
+// in LDS, BLOCK slices at a time.
 template <int R, int S>
 __global__ __launch_bounds__(BLOCK) void slice_ids(SelArgs sa, uint32_t* __restrict__ idx) {
   constexpr int CHUNK = 64 * R;
   constexpr int SLICE = S * CHUNK;
-  const int lane = threadIdx.x & 63;
+  __shared__ uint32_t s_off[BLOCK];
+  __shared__ uint32_t s_part[WAVES];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t below = lane ? (~0ULL >> (64 - lane)) : 0ULL;
   GLOBAL uint32_t* ids = gpw<uint32_t>(idx);
-  const int64_t waves = (int64_t)gridDim.x * WAVES;
-  for (int64_t slice = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6); slice < sa.n_slices; slice += waves) {
-    const uint64_t mine = lane < S * R ? gp<uint64_t>(sa.bits)[slice * (S * R) + lane] : 0ULL;
-    uint32_t off = gp<uint32_t>(sa.offsets)[slice];
-    const uint32_t row0 = (uint32_t)(slice * SLICE) + (uint32_t)(lane * R);
+  const int64_t s0 = (int64_t)blockIdx.x * sa.per;
+  const int64_t s1 = s0 + sa.per < sa.n_slices ? s0 + sa.per : sa.n_slices;
+  uint32_t carry = gp<uint32_t>(sa.block_base)[blockIdx.x];
+  for (int64_t c0 = s0; c0 < s1; c0 += BLOCK) {
+    // offsets of slices [c0, c0 + BLOCK) within the block
+    const int64_t mys = c0 + threadIdx.x;
+    const uint32_t v = mys < s1 ? gp<uint32_t>(sa.counts)[mys] : 0u;
+    uint32_t incl = v;
 #pragma unroll
-    for (int k = 0; k < S; ++k) {
-      uint64_t b[R];
-      uint32_t before = 0, chunk_total = 0;
-#pragma unroll
-      for (int j = 0; j < R; ++j) {
-        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)mine, k * R + j);
-        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(mine >> 32), k * R + j);
-        b[j] = ((uint64_t)hi << 32) | lo;
-        before += (uint32_t)__popcll(b[j] & below);
-        chunk_total += (uint32_t)__popcll(b[j]);
-      }
-      uint32_t pos = off + before;
-#pragma unroll
-      for (int j = 0; j < R; ++j)
-        if ((b[j] >> lane) & 1ULL) ids[pos++] = row0 + (uint32_t)(k * CHUNK + j);
-      off += chunk_total;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t u = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += u;
     }
+    if (lane == 63) s_part[wave] = incl;
+    __syncthreads();
+    uint32_t before = carry;
+    for (int w = 0; w < wave; ++w) before += s_part[w];
+    s_off[threadIdx.x] = before + incl - v;
+    uint32_t all = 0;
+    for (int w = 0; w < WAVES; ++w) all += s_part[w];
+    __syncthreads();
+    const int64_t c1 = c0 + BLOCK < s1 ? c0 + BLOCK : s1;
+    for (int64_t slice = c0 + wave; slice < c1; slice += WAVES) {
+      const uint64_t mine = lane < S * R ? gp<uint64_t>(sa.bits)[slice * (S * R) + lane] : 0ULL;
+      uint32_t off = s_off[slice - c0];
+      const uint32_t row0 = (uint32_t)(slice * SLICE) + (uint32_t)(lane * R);
+#pragma unroll
+      for (int k = 0; k < S; ++k) {
+        uint64_t b[R];
+        uint32_t before_k = 0, chunk_total = 0;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)mine, k * R + j);
+          const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(mine >> 32), k * R + j);
+          b[j] = ((uint64_t)hi << 32) | lo;
+          before_k += (uint32_t)__popcll(b[j] & below);
+          chunk_total += (uint32_t)__popcll(b[j]);
+        }
+        uint32_t pos = off + before_k;
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+          if ((b[j] >> lane) & 1ULL) ids[pos++] = row0 + (uint32_t)(k * CHUNK + j);
+        off += chunk_total;
+      }
+    }
+    __syncthreads();  // s_off is rewritten for the next BLOCK slices
+    carry += all;
   }
 }
 
@@ -1218,8 +1269,10 @@ __global__ __launch_bounds__(BLOCK) void select_ranges(const DevPlan* __restrict
   const int lane = threadIdx.x & 63;
   const int64_t num_rows = plan.num_rows;
   const int nr = plan.n_ranges;
-  const int64_t waves = (int64_t)gridDim.x * WAVES;
-  for (int64_t slice = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6); slice < sa.n_slices; slice += waves) {
+  const int64_t s0 = (int64_t)blockIdx.x * sa.per;
+  const int64_t s1 = s0 + sa.per < sa.n_slices ? s0 + sa.per : sa.n_slices;
+  uint32_t wave_total = 0;
+  for (int64_t slice = s0 + (threadIdx.x >> 6); slice < s1; slice += WAVES) {
     const int64_t slice_row0 = slice * T::WAVE_ROWS;
     const int64_t row0 = slice_row0 + lane * R;
     uint32_t alive = 0;
@@ -1242,8 +1295,9 @@ __global__ __launch_bounds__(BLOCK) void select_ranges(const DevPlan* __restrict
       else
         alive = range_test<R, S, false>(plan.in[rg.col], rg, row0, num_rows, alive);
     }
-    emit_slice<R, S>(sa, slice, alive);
+    wave_total += emit_slice<R, S>(sa, slice, alive);
   }
+  emit_block(sa, wave_total);
 }
 
 // Row selection by ops [0, split) (KILL per conjunct with a wave-uniform
@@ -1254,8 +1308,10 @@ __global__ __launch_bounds__(BLOCK) void select_vm(const DevPlan* __restrict__ p
   typedef SelectTile<R, S> T;
   const DevPlan& plan = *pplan;
   const int lane = threadIdx.x & 63;
-  const int64_t waves = (int64_t)gridDim.x * WAVES;
-  for (int64_t slice = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6); slice < sa.n_slices; slice += waves) {
+  const int64_t s0 = (int64_t)blockIdx.x * sa.per;
+  const int64_t s1 = s0 + sa.per < sa.n_slices ? s0 + sa.per : sa.n_slices;
+  uint32_t wave_total = 0;
+  for (int64_t slice = s0 + (threadIdx.x >> 6); slice < s1; slice += WAVES) {
     RegVec f[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
@@ -1268,15 +1324,18 @@ __global__ __launch_bounds__(BLOCK) void select_vm(const DevPlan* __restrict__ p
       chunk_rows<R>(rs, slice * T::WAVE_ROWS + k * T::CHUNK + lane * R, plan.num_rows);
       uint32_t alive = rs.exist;
       if (__ballot(alive != 0) != 0) {
+        DevOp next = load_op(plan, 0);
         for (int q = 0; q < plan.split; ++q) {
-          const DevOp op = load_op(plan, q);
+          const DevOp op = next;
+          next = load_op(plan, q + 1 < BSQL_DEV_MAX_OPS ? q + 1 : q);
           if (vm_step<NREG, R>(plan, op, f, vb, L, rs, alive)) break;
         }
       }
       alive_all |= alive << (k * R);
     }
-    emit_slice<R, S>(sa, slice, alive_all);
+    wave_total += emit_slice<R, S>(sa, slice, alive_all);
   }
+  emit_block(sa, wave_total);
 }
 
 // ------------------------------------------------------- string outputs
@@ -1426,25 +1485,15 @@ template <int R> struct VmChunks { static constexpr int S = 16 / R; };
 // 8192-row tiles) by default, or 4 chunks (sel_rpt == 16)
 constexpr int RANGE_R = 4;
 
-// One selection launch: waves stride over the slices; then the scan, and
-// the row ids when an output phase will read them.
-template <int R, int S>
-static int64_t slice_grid(int64_t n_slices) {
-  return persistent_grid((n_slices + WAVES - 1) / WAVES);
-}
-
 template <int NREG, int R>
-static hipError_t launch_select_vm_t(const DevPlan* d_plan, const SelArgs& sa, hipStream_t stream) {
-  constexpr int S = VmChunks<R>::S;
-  hipLaunchKernelGGL((select_vm<NREG, R, S>), dim3((unsigned)slice_grid<R, S>(sa.n_slices)), dim3(BLOCK), 0,
-                     stream, d_plan, sa);
+static hipError_t launch_select_vm_t(const DevPlan* d_plan, const SelArgs& sa, int64_t grid, hipStream_t stream) {
+  hipLaunchKernelGGL((select_vm<NREG, R, VmChunks<R>::S>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan, sa);
   return hipGetLastError();
 }
 
 template <int R, int S>
-static hipError_t launch_slice_ids(const SelArgs& sa, uint32_t* idx, hipStream_t stream) {
-  hipLaunchKernelGGL((slice_ids<R, S>), dim3((unsigned)slice_grid<R, S>(sa.n_slices)), dim3(BLOCK), 0, stream,
-                     sa, idx);
+static hipError_t launch_slice_ids(const SelArgs& sa, uint32_t* idx, int64_t grid, hipStream_t stream) {
+  hipLaunchKernelGGL((slice_ids<R, S>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, sa, idx);
   return hipGetLastError();
 }
 
@@ -1506,13 +1555,14 @@ static bool slice_shape(const DevPlan& plan, int nreg, int rpt, int64_t* rows, i
 }
 
 // Scratch words of the selection: the ballots, then the slice counts and
-// offsets (two uint32 per slice).
+// the block totals and bases (uint32 each).
 int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt) {
   int64_t rows;
   int ballots;
   if (!slice_shape(plan, nreg, rpt, &rows, &ballots)) return -1;
   const int64_t n = (plan.num_rows + rows - 1) / rows;
-  return n * ballots + n;
+  const int64_t grid = persistent_grid((n + WAVES - 1) / WAVES);
+  return n * ballots + (n + 2 * grid + 1) / 2;
 }
 
 // The output phase of a filter: ops [split, n_ops) over the selected row ids
@@ -1545,43 +1595,43 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
   int ballots;
   if (!slice_shape(plan, nreg, sel_rpt, &rows, &ballots)) return hipErrorInvalidValue;
   const int64_t n = (plan.num_rows + rows - 1) / rows;
-  SelArgs sa{sel, (uint32_t*)(sel + n * ballots), (uint32_t*)(sel + n * ballots) + n, n};
+  const int64_t grid = persistent_grid((n + WAVES - 1) / WAVES);
+  uint32_t* counts = (uint32_t*)(sel + n * ballots);
+  SelArgs sa{sel, counts, counts + n, counts + n + grid, n, grid ? (n + grid - 1) / grid : 0};
   hipError_t e = hipSuccess;
   const bool ids = plan.split < plan.n_ops;
   if (n > 0) {
     e = hipErrorInvalidValue;
     if (plan.n_ranges > 0) {
       if (sel_rpt == 16) {
-        hipLaunchKernelGGL((select_ranges<RANGE_R, 4>), dim3((unsigned)slice_grid<RANGE_R, 4>(n)), dim3(BLOCK), 0,
-                           stream, d_plan, sa);
+        hipLaunchKernelGGL((select_ranges<RANGE_R, 4>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan, sa);
       } else {
-        hipLaunchKernelGGL((select_ranges<RANGE_R, 8>), dim3((unsigned)slice_grid<RANGE_R, 8>(n)), dim3(BLOCK), 0,
-                           stream, d_plan, sa);
+        hipLaunchKernelGGL((select_ranges<RANGE_R, 8>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan, sa);
       }
       e = hipGetLastError();
     } else {
       int NR, RR;
       if (!pick(nreg, sel_rpt, &NR, &RR)) return hipErrorInvalidValue;
-#define X(N, RPT) if (NR == N && RR == RPT) e = launch_select_vm_t<N, RPT>(d_plan, sa, stream);
+#define X(N, RPT) if (NR == N && RR == RPT) e = launch_select_vm_t<N, RPT>(d_plan, sa, grid, stream);
       BSQL_VARIANTS(X)
 #undef X
     }
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(slice_scan, dim3(1), dim3(SCAN_THREADS), 0, stream, (const uint32_t*)sa.counts, sa.offsets,
-                     n, d_count);
+  hipLaunchKernelGGL(block_scan, dim3(1), dim3(SCAN_THREADS), 0, stream, (const uint32_t*)sa.block_sums,
+                     sa.block_base, n > 0 ? grid : (int64_t)0, d_count);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (ids && n > 0) {
     if (plan.n_ranges > 0) {
-      e = sel_rpt == 16 ? launch_slice_ids<RANGE_R, 4>(sa, idx, stream)
-                        : launch_slice_ids<RANGE_R, 8>(sa, idx, stream);
+      e = sel_rpt == 16 ? launch_slice_ids<RANGE_R, 4>(sa, idx, grid, stream)
+                        : launch_slice_ids<RANGE_R, 8>(sa, idx, grid, stream);
     } else {
       int NR, RR;
       if (!pick(nreg, sel_rpt, &NR, &RR)) return hipErrorInvalidValue;
-      e = RR == 1 ? launch_slice_ids<1, VmChunks<1>::S>(sa, idx, stream)
-          : RR == 2 ? launch_slice_ids<2, VmChunks<2>::S>(sa, idx, stream)
-                    : launch_slice_ids<4, VmChunks<4>::S>(sa, idx, stream);
+      e = RR == 1 ? launch_slice_ids<1, VmChunks<1>::S>(sa, idx, grid, stream)
+          : RR == 2 ? launch_slice_ids<2, VmChunks<2>::S>(sa, idx, grid, stream)
+                    : launch_slice_ids<4, VmChunks<4>::S>(sa, idx, grid, stream);
     }
   }
   if (e != hipSuccess || !(phases & 2)) return e;

@@ -254,9 +254,11 @@ def test_filter_project_c2_orders_vs_oracle(bsql, gpu, order, mode):
         assert_same(gv, gval, ores[k][0], ores[k][1], what=f"C2 output {k}")
 
 
-@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 255, 256, 257, 1023, 1024, 1025, 4096 + 3, 100003])
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 255, 256, 257, 1023, 1024, 1025, 2047, 2048, 2049, 4096 + 3,
+                               100003, 2048 * 64 * 3 + 5])
 @pytest.mark.parametrize("sel", ["none", "all", "half", "sparse"])
 def test_filter_gather_edges_vs_oracle(bsql, gpu, n, sel):
+    """Sizes around slice, chunk, workgroup-block and validity-word boundaries."""
     import oracle
     import torch
     from blazingsql_amd.interop import DeviceColumn, filter_columns
