@@ -1212,7 +1212,9 @@ __device__ __forceinline__ uint32_t range_chunks(const DevInCol& c, const DevRan
     uint32_t vm = 0;
 #pragma unroll
     for (int k = 0; k < S; ++k) {
-      if ((live >> k) & 1u) {
+      // only lanes with a live row load (exec-masked loads): a cache line
+      // whose rows are all dead is not fetched
+      if (((live >> k) & 1u) && ((alive >> (k * R)) & RMASK)) {
         const int64_t r0 = row0 + (int64_t)k * 64 * R;
         load_raw<CT, R>(c, r0, R, v[k]);
         vm |= load_valid<R>(c, r0, R) << (k * R);
@@ -1230,7 +1232,7 @@ __device__ __forceinline__ uint32_t range_chunks(const DevInCol& c, const DevRan
     for (int k = 0; k < S; ++k) {
       const int64_t r0 = row0 + (int64_t)k * 64 * R;
       const int64_t left = num_rows - r0;
-      const int n = left <= 0 ? 0 : (left >= R ? R : (int)left);
+      const int n = ((alive >> (k * R)) & RMASK) == 0 ? 0 : (left <= 0 ? 0 : (left >= R ? R : (int)left));
       CT v[R];
       load_raw<CT, R>(c, r0, n, v);
       vm |= load_valid<R>(c, r0, n) << (k * R);
