@@ -454,8 +454,8 @@ int bsql_program_set_conjunct_order(bsql_program* pred, const int32_t* order, in
 int bsql_program_set_launch(bsql_program* p, const bsql_launch_config* cfg) {
   if (!p || !cfg) return set_error(BSQL_ERR_INVALID, "invalid arguments");
   const int r = cfg->rows_per_thread;
-  if (r != 0 && r != 1 && r != 2 && r != 4 && r != 16 && r != 32)
-    return set_error(BSQL_ERR_INVALID, "rows_per_thread must be 0, 1, 2, 4, 16 or 32");
+  if (r != 0 && r != 1 && r != 2 && r != 4 && r != 8 && r != 16 && r != 32)
+    return set_error(BSQL_ERR_INVALID, "rows_per_thread must be 0, 1, 2, 4, 8, 16 or 32");
   if (cfg->block_size != 0 && cfg->block_size != 256)
     return set_error(BSQL_ERR_INVALID, "block_size must be 0 or 256");
   if (cfg->flags & ~(BSQL_LAUNCH_NO_RANGES | BSQL_LAUNCH_LAZY_LOADS))

@@ -1483,9 +1483,10 @@ static hipError_t launch_execute_t(const DevPlan* d_plan, int64_t num_rows, hipS
 // chunks per wave of the selection tiles: 4096-row tiles for the VM,
 // 8192-row tiles for range tests
 template <int R> struct VmChunks { static constexpr int S = 16 / R; };
-// range selection: 4 rows per chunk, 8 chunks per wave (32 rows per lane,
-// 8192-row tiles) by default, or 4 chunks (sel_rpt == 16)
+// range selection: 4 rows per chunk; 2, 4 or 8 chunks per wave (8, 16 or 32
+// rows per lane: sel_rpt 8, 16, 32), 4 by default
 constexpr int RANGE_R = 4;
+static int range_chunks_of(int sel_rpt) { return sel_rpt == 8 ? 2 : (sel_rpt == 32 ? 8 : 4); }
 
 template <int NREG, int R>
 static hipError_t launch_select_vm_t(const DevPlan* d_plan, const SelArgs& sa, int64_t grid, hipStream_t stream) {
@@ -1545,8 +1546,9 @@ hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
 // plan selects with.
 static bool slice_shape(const DevPlan& plan, int nreg, int rpt, int64_t* rows, int* ballots) {
   if (plan.n_ranges > 0) {
-    *rows = rpt == 16 ? SelectTile<RANGE_R, 4>::WAVE_ROWS : SelectTile<RANGE_R, 8>::WAVE_ROWS;
-    *ballots = rpt == 16 ? RANGE_R * 4 : RANGE_R * 8;
+    const int chunks = range_chunks_of(rpt);
+    *rows = (int64_t)64 * RANGE_R * chunks;
+    *ballots = RANGE_R * chunks;
     return true;
   }
   int NR, RR;
@@ -1605,10 +1607,15 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
   if (n > 0) {
     e = hipErrorInvalidValue;
     if (plan.n_ranges > 0) {
-      if (sel_rpt == 16) {
-        hipLaunchKernelGGL((select_ranges<RANGE_R, 4>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan, sa);
-      } else {
-        hipLaunchKernelGGL((select_ranges<RANGE_R, 8>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan, sa);
+      switch (range_chunks_of(sel_rpt)) {
+        case 2:
+          hipLaunchKernelGGL((select_ranges<RANGE_R, 2>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan, sa);
+          break;
+        case 8:
+          hipLaunchKernelGGL((select_ranges<RANGE_R, 8>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan, sa);
+          break;
+        default:
+          hipLaunchKernelGGL((select_ranges<RANGE_R, 4>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan, sa);
       }
       e = hipGetLastError();
     } else {
@@ -1626,8 +1633,10 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
   if (e != hipSuccess) return e;
   if (ids && n > 0) {
     if (plan.n_ranges > 0) {
-      e = sel_rpt == 16 ? launch_slice_ids<RANGE_R, 4>(sa, idx, grid, stream)
-                        : launch_slice_ids<RANGE_R, 8>(sa, idx, grid, stream);
+      const int chunks = range_chunks_of(sel_rpt);
+      e = chunks == 2 ? launch_slice_ids<RANGE_R, 2>(sa, idx, grid, stream)
+          : chunks == 8 ? launch_slice_ids<RANGE_R, 8>(sa, idx, grid, stream)
+                        : launch_slice_ids<RANGE_R, 4>(sa, idx, grid, stream);
     } else {
       int NR, RR;
       if (!pick(nreg, sel_rpt, &NR, &RR)) return hipErrorInvalidValue;

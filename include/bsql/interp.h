@@ -241,7 +241,7 @@ enum {
 };
 typedef struct bsql_launch_config {
   int32_t block_size;     /* threads per workgroup: 0 (auto) or 256 */
-  int32_t rows_per_thread;/* 0 (auto), 1, 2, 4; range-test selection: 16, 32 */
+  int32_t rows_per_thread;/* 0 (auto), 1, 2, 4; range-test selection: 8, 16, 32 */
   int32_t flags;          /* BSQL_LAUNCH_* */
 } bsql_launch_config;
 /* Sets the configuration used for launches with this program (a predicate's

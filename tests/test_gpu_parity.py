@@ -350,7 +350,7 @@ def test_range_predicates_vs_oracle(bsql, gpu, seed):
     for _ in range(4):
         text = _range_predicate(rng, types, table)
         on, ores = oracle.filter_table(_full_prog(bsql, text, types), host_columns(table), n)
-        for ranges, rpt in ((True, 0), (True, 16), (False, 0)):
+        for ranges, rpt in ((True, 0), (True, 8), (True, 32), (False, 0)):
             pred = bsql.Program.compile(text, types)
             pred.set_launch(rows_per_thread=rpt, ranges=ranges)
             outs = [DeviceColumn.empty(t, n, nullable=v is not None) for t, _, v in table]
