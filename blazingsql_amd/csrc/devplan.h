@@ -39,6 +39,10 @@ enum DevCode : uint8_t {
   // compiled pattern at imm; SLICE with aux2 = 0 from char b (0-based), 1 the
   // first b chars, 2 the last b chars
   D_LIKE, D_SLICE,
+  // TRIM: the slice without the leading (aux2 bit 0) / trailing (bit 1)
+  // characters found in the literal at imm ({int32 len, bytes}; empty: those
+  // <= ' '); STR_TO_I64 / STR_TO_F64: the slice parsed as a number
+  D_TRIM, D_STR_TO_I64, D_STR_TO_F64,
   D_NUM_CODES
 };
 

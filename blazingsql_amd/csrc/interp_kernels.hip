@@ -25,6 +25,7 @@
 #include <type_traits>
 
 #include "devplan.h"
+#include "pow10.h"
 #include "../../include/bsql/interp.h"
 
 namespace bsql_dev {
@@ -339,6 +340,128 @@ __device__ int64_t utf8_chars(const GLOBAL uint8_t* s, int32_t n) {
   return c;
 }
 
+// The UTF-8 character (its encoded bytes packed little-endian) at s[i].
+__device__ __forceinline__ uint32_t utf8_char_at(const GLOBAL uint8_t* s, int32_t i, int32_t len) {
+  uint32_t c = 0;
+  for (int32_t k = 0; k < len; ++k) c |= (uint32_t)s[i + k] << (8 * k);
+  return c;
+}
+
+// Whether character c is one to strip: one of the characters of `set`, or
+// for an empty set a byte <= ' ' (cudf::strings::strip's rule).
+__device__ bool strip_char(uint32_t c, const GLOBAL uint8_t* set, int32_t nset) {
+  if (nset == 0) return c <= (uint32_t)' ';
+  for (int32_t i = 0; i < nset;) {
+    const int32_t l = utf8_step(set, i, nset);  // >= 1
+    if (utf8_char_at(set, i, l) == c) return true;
+    i += l;
+  }
+  return false;
+}
+
+// [b0, b1) of s[0, n) after stripping (side bit 0: leading, bit 1: trailing).
+__device__ void strip_range(const GLOBAL uint8_t* s, int32_t n, const GLOBAL uint8_t* set, int32_t nset, int side,
+                            int32_t* b0, int32_t* b1) {
+  int32_t lo = 0, hi = n;
+  if (side & 1) {
+    while (lo < hi) {
+      const int32_t l = utf8_step(s, lo, hi);  // >= 1
+      if (!strip_char(utf8_char_at(s, lo, l), set, nset)) break;
+      lo += l;
+    }
+  }
+  if (side & 2) {
+    while (hi > lo) {
+      int32_t st = hi - 1;
+      while (st > lo && !utf8_lead(s[st])) --st;
+      if (!strip_char(utf8_char_at(s, st, hi - st), set, nset)) break;
+      hi = st;
+    }
+  }
+  *b0 = lo;
+  *b1 = hi;
+}
+
+// cudf::strings::to_integers (libcudf 21.08, string_to_integer): an optional
+// sign, then decimal digits up to the first other character; wraps.
+__device__ int64_t str_to_i64(const GLOBAL uint8_t* s, int32_t n) {
+  int32_t i = 0;
+  uint64_t sign = 1;
+  if (n > 0 && (s[0] == '-' || s[0] == '+')) {
+    sign = s[0] == '-' ? ~0ULL : 1ULL;
+    i = 1;
+  }
+  uint64_t v = 0;
+  for (; i < n; ++i) {
+    const uint8_t c = s[i];
+    if (c < '0' || c > '9') break;
+    v = v * 10ULL + (uint64_t)(c - '0');
+  }
+  return (int64_t)(v * sign);
+}
+
+__constant__ double POW10[BSQL_POW10_MAX - BSQL_POW10_MIN + 1] = {BSQL_POW10_VALUES};
+
+// cudf::strings::to_floats (libcudf 21.08, the parser LogicalProject.cpp:439-468
+// hands CAST(varchar AS DOUBLE/FLOAT) to): exactly "NaN", "Inf" or "-Inf";
+// else an optional sign, digits and '.'s (every '.' marks the fraction) kept
+// while they fit 52 bits, an optional exponent; a decimal exponent above 308
+// is infinity and below -307 is +0; else sign * digits * 10^e, with 10^e the
+// double nearest it.
+__device__ double str_to_f64(const GLOBAL uint8_t* s, int32_t n) {
+  if (n == 0) return 0.0;
+  if (n == 3 && s[0] == 'N' && s[1] == 'a' && s[2] == 'N') return __longlong_as_double(0x7ff8000000000000LL);
+  if (n == 3 && s[0] == 'I' && s[1] == 'n' && s[2] == 'f') return __longlong_as_double(0x7ff0000000000000LL);
+  if (n == 4 && s[0] == '-' && s[1] == 'I' && s[2] == 'n' && s[3] == 'f')
+    return __longlong_as_double((long long)0xfff0000000000000ULL);
+  int32_t i = 0;
+  double sign = 1.0;
+  if (s[0] == '-' || s[0] == '+') {
+    sign = s[0] == '-' ? -1.0 : 1.0;
+    i = 1;
+  }
+  const uint64_t max_mantissa = 0x0FFFFFFFFFFFFFULL;
+  uint64_t digits = 0;
+  uint32_t exp_off = 0;  // int32 arithmetic, wrapping
+  bool decimal = false;
+  for (; i < n; ++i) {
+    const uint8_t c = s[i];
+    if (c == '.') { decimal = true; continue; }
+    if (c < '0' || c > '9') break;
+    if (digits > max_mantissa) {
+      exp_off += !decimal;
+    } else {
+      digits = digits * 10ULL + (uint64_t)(c - '0');
+      if (digits > max_mantissa) {
+        digits /= 10ULL;
+        exp_off += !decimal;
+      } else {
+        exp_off -= decimal;
+      }
+    }
+  }
+  uint32_t exp_ten = 0;
+  bool exp_neg = false;
+  if (i < n) {
+    const uint8_t c = s[i++];
+    if ((c == 'e' || c == 'E') && i < n) {
+      if (s[i] == '-' || s[i] == '+') exp_neg = s[i++] == '-';
+      while (i < n) {
+        const uint8_t d = s[i++];
+        if (d < '0' || d > '9') break;
+        exp_ten = exp_ten * 10u + (uint32_t)(d - '0');
+      }
+    }
+  }
+  const int32_t e = (int32_t)((exp_neg ? 0u - exp_ten : exp_ten) + exp_off);
+  if (e > BSQL_POW10_MAX) return sign > 0 ? __longlong_as_double(0x7ff0000000000000LL)
+                                          : __longlong_as_double((long long)0xfff0000000000000ULL);
+  if (e < BSQL_POW10_MIN) return 0.0;
+  double base = __dmul_rn(sign, (double)digits);
+  if (e) base = __dmul_rn(base, POW10[e - BSQL_POW10_MIN]);
+  return base;
+}
+
 // SQL LIKE through the reference's translation to a regex searched with
 // contains_re (LogicalProject.cpp:31-47, :113-131): '%' = (?:.*?) and '_' =
 // (?:.), neither matching '\n'; anchored at the start unless the pattern
@@ -560,6 +683,39 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
             const int32_t b0 = total > pos ? utf8_offset(p, n, total - pos) : 0;
             D[k] = slice_pack(st + (uint32_t)b0, (uint32_t)(n - b0));
           }
+          VD |= 1u << k;
+        }
+      }
+      return;
+    }
+    case D_TRIM: {
+      const GLOBAL int32_t* lit = gp<int32_t>((const void*)op.imm);
+      const int32_t nset = lit[0];
+      const GLOBAL uint8_t* set = (const GLOBAL uint8_t*)(lit + 1);
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        D[k] = A[k];
+        if (((rs.exist >> k) & 1u) && ((VA >> k) & 1u)) {
+          const GLOBAL uint8_t* p;
+          int32_t n;
+          str_at(plan, op.aux, A[k], &p, &n);
+          int32_t b0, b1;
+          strip_range(p, n, set, nset, op.aux2, &b0, &b1);
+          D[k] = slice_pack((uint32_t)A[k] + (uint32_t)b0, (uint32_t)(b1 - b0));
+          VD |= 1u << k;
+        }
+      }
+      return;
+    }
+    case D_STR_TO_I64: case D_STR_TO_F64: {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        D[k] = 0;
+        if (((rs.exist >> k) & 1u) && ((VA >> k) & 1u)) {
+          const GLOBAL uint8_t* p;
+          int32_t n;
+          str_at(plan, op.aux, A[k], &p, &n);
+          D[k] = op.code == D_STR_TO_I64 ? (uint64_t)str_to_i64(p, n) : bits(str_to_f64(p, n));
           VD |= 1u << k;
         }
       }

@@ -63,6 +63,11 @@ std::string replace_calcite_regex(const std::string& expression) {
   replace_all(ret, "EXTRACT(FLAG(HOUR), ", "BL_HOUR(");
   replace_all(ret, "EXTRACT(FLAG(MINUTE), ", "BL_MINUTE(");
   replace_all(ret, "EXTRACT(FLAG(SECOND), ", "BL_SECOND(");
+  // TRIM's side flag (expression_utils.cpp:1102-1105 rewrites it to a quoted
+  // word); here the side goes into the function name
+  replace_all(ret, "TRIM(FLAG(BOTH), ", "BLZ_TRIM_BOTH(");
+  replace_all(ret, "TRIM(FLAG(LEADING), ", "BLZ_TRIM_LEADING(");
+  replace_all(ret, "TRIM(FLAG(TRAILING), ", "BLZ_TRIM_TRAILING(");
   replace_all(ret, "DAY TO SECOND", "SECOND");
   replace_all(ret, "HOUR TO SECOND", "SECOND");
   replace_all(ret, "MINUTE TO SECOND", "SECOND");
@@ -466,6 +471,8 @@ int map_to_operator(const std::string& tok) {
       {"REGEXP_REPLACE", BSQL_OP_STR_REGEXP_REPLACE}, {"REPLACE", BSQL_OP_STR_REPLACE},
       {"TO_DATE", BSQL_OP_TO_DATE}, {"TO_TIMESTAMP", BSQL_OP_TO_TIMESTAMP},
       {"||", BSQL_OP_STR_CONCAT}, {"CONCAT", BSQL_OP_STR_CONCAT}, {"TRIM", BSQL_OP_STR_TRIM},
+      {"BLZ_TRIM_BOTH", BSQL_OP_STR_TRIM}, {"BLZ_TRIM_LEADING", BSQL_OP_STR_TRIM},
+      {"BLZ_TRIM_TRAILING", BSQL_OP_STR_TRIM},
       {"LEFT", BSQL_OP_STR_LEFT}, {"RIGHT", BSQL_OP_STR_RIGHT},
       {"IS_NOT_DISTINCT_FROM", BSQL_OP_IS_NOT_DISTINCT_FROM},
   };
@@ -594,6 +601,14 @@ Scalar scalar_from_string(const std::string& v, int32_t type) {
 
 // ---------------------------------------------------------------- encoder
 
+// TRIM's side: 1 leading, 2 trailing, 3 both (0: not a side-flagged TRIM).
+int trim_side(const std::string& v) {
+  if (v == "BLZ_TRIM_LEADING") return 1;
+  if (v == "BLZ_TRIM_TRAILING") return 2;
+  if (v == "BLZ_TRIM_BOTH") return 3;
+  return 0;
+}
+
 struct Encoder {
   Program& p;
   const std::map<int, int>& col_map;
@@ -652,6 +667,7 @@ struct Encoder {
       } else if (R->type == NodeType::Literal) {
         int code;
         Scalar s = literal_scalar(R, &code);
+        if (op == BSQL_OP_STR_TRIM) s.i64 = trim_side(n->value);  // with the characters
         push(lp, code, none, s, op);
       } else {
         push(lp, rp, none, none, op);
@@ -685,6 +701,7 @@ bool is_string_function(const std::string& v) {
   static const char* names[] = {"REGEXP_REPLACE", "REPLACE", "TO_DATE", "TO_TIMESTAMP", "||",
                                 "CONCAT", "TRIM", "LOWER", "UPPER", "INITCAP", "REVERSE",
                                 "CAST_VARCHAR", "CAST_CHAR"};
+  // TRIM with its side flag is interpreted (BLZ_TRIM_*); a bare TRIM is not
   for (const char* s : names)
     if (v == s) return true;
   return false;
@@ -712,6 +729,18 @@ std::unique_ptr<Node> rewrite_substring(std::unique_ptr<Node> n) {
   return left;
 }
 
+// BLZ_TRIM_<side>('chars', s) -> the string first and the characters to
+// strip second, as the interpreter's string ops take them; the side is kept
+// in the name and goes into the literal's scalar when it is encoded.
+std::unique_ptr<Node> rewrite_trim(std::unique_ptr<Node> n) {
+  for (auto& c : n->children) c = rewrite_trim(std::move(c));
+  if (n->type != NodeType::Operator || !trim_side(n->value)) return n;
+  if (n->children.size() != 2 || n->children[0]->type != NodeType::Literal)
+    fail("TRIM takes a side flag, a literal set of characters and a string");
+  std::swap(n->children[0], n->children[1]);
+  return n;
+}
+
 void check_interpretable(const Node* n) {
   if (n->type == NodeType::Operator && is_string_function(n->value))
     fail("string function " + n->value +
@@ -729,7 +758,7 @@ std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exp
     std::string e = replace_calcite_regex(fill_minus_op_with_zero(raw));
     e = expand_if_logical_op(e);
     Parser parser(e);
-    std::unique_ptr<Node> t = rewrite_substring(transform_custom(parser.parse()));
+    std::unique_ptr<Node> t = rewrite_trim(rewrite_substring(transform_custom(parser.parse())));
     check_interpretable(t.get());
     std::vector<int> vars;
     collect_vars(t.get(), vars);

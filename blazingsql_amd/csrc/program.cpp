@@ -403,12 +403,25 @@ SourceOp translate(Lowering& L, const Program& p, const OpTypes& t, size_t i,
         }
         break;
       }
+      case BSQL_OP_STR_TRIM: {
+        // the characters to strip and the side come in the right scalar
+        if (!is_string_type(lt) || l < 0) fail("TRIM needs a string column or expression");
+        if (r != BSQL_SCALAR_INDEX || !is_string_type(rt)) fail("TRIM needs a literal set of characters");
+        const int64_t side = p.rsc[i].i64;
+        if (side < 1 || side > 3) fail("TRIM needs a side (LEADING, TRAILING or BOTH)");
+        d.code = D_TRIM;
+        d.aux = (uint8_t)str_col_of(L, s.va);
+        d.aux2 = (uint8_t)side;
+        d.imm = str_base_b;
+        break;
+      }
       default: fail("operator " + std::to_string(op) + " is not supported by the interpreter");
     }
     if (is_string_type(lt) && !((d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE ||
-                                d.code == D_SLICE))
+                                d.code == D_SLICE || d.code == D_TRIM))
       fail("string operand to a non-string operator");
-    if (is_string_type(rt) && !((d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE))
+    if (is_string_type(rt) && !((d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE ||
+                                d.code == D_TRIM))
       fail("string operand to a non-string operator");
   } else if (r == BSQL_UNARY_INDEX) {
     s.va = operand(l);
@@ -435,9 +448,17 @@ SourceOp translate(Lowering& L, const Program& p, const OpTypes& t, size_t i,
       case BSQL_OP_IS_TRUE: d.code = D_IS_TRUE; break;
       case BSQL_OP_IS_NOT_TRUE: d.code = D_IS_NOT_TRUE; break;
       case BSQL_OP_IS_NOT_FALSE: d.code = D_IS_NOT_FALSE; break;
+      // a string operand is parsed (cudf::strings::to_integers / to_floats,
+      // LogicalProject.cpp:379-468)
       case BSQL_OP_CAST_TINYINT: case BSQL_OP_CAST_SMALLINT: case BSQL_OP_CAST_INTEGER:
-      case BSQL_OP_CAST_BIGINT: d.code = D_TO_I64; break;
-      case BSQL_OP_CAST_FLOAT: case BSQL_OP_CAST_DOUBLE: d.code = D_TO_F64; break;
+      case BSQL_OP_CAST_BIGINT:
+        d.code = is_string_type(lt) ? D_STR_TO_I64 : D_TO_I64;
+        if (is_string_type(lt)) d.aux = (uint8_t)str_col_of(L, s.va);
+        break;
+      case BSQL_OP_CAST_FLOAT: case BSQL_OP_CAST_DOUBLE:
+        d.code = is_string_type(lt) ? D_STR_TO_F64 : D_TO_F64;
+        if (is_string_type(lt)) d.aux = (uint8_t)str_col_of(L, s.va);
+        break;
       case BSQL_OP_CAST_DATE: d.code = D_CAST_DATE; d.aux2 = (uint8_t)lt; break;
       case BSQL_OP_CAST_TIMESTAMP: case BSQL_OP_CAST_TIMESTAMP_SECONDS:
       case BSQL_OP_CAST_TIMESTAMP_MILLISECONDS: case BSQL_OP_CAST_TIMESTAMP_MICROSECONDS:
@@ -448,7 +469,7 @@ SourceOp translate(Lowering& L, const Program& p, const OpTypes& t, size_t i,
       default: fail("operator " + std::to_string(op) + " is not supported by the interpreter");
     }
     if (is_string_type(lt) && d.code != D_CHAR_LENGTH && d.code != D_IS_NULL &&
-        d.code != D_IS_NOT_NULL)
+        d.code != D_IS_NOT_NULL && d.code != D_STR_TO_I64 && d.code != D_STR_TO_F64)
       fail("string operand to a non-string operator");
   } else {
     d.code = D_RAND;
@@ -557,7 +578,8 @@ SsaProgram to_ssa(Lowering& L, const Program& p, uint64_t seed) {
     SourceOp so = translate(L, p, t, i, slot_val, sa >= 0 ? base + sa : 0, sb >= 0 ? base + sb : 0,
                             seed, n_slots_in, col_of);
     so.vdst = L.new_value();
-    if (so.d.code == D_SLICE) L.values[(size_t)so.vdst].str_col = L.values[(size_t)so.va].str_col;
+    if (so.d.code == D_SLICE || so.d.code == D_TRIM)
+      L.values[(size_t)so.vdst].str_col = L.values[(size_t)so.va].str_col;
     slot_val[p.out[i]] = so.vdst;
     s.ops.push_back(so);
   }

@@ -127,7 +127,9 @@ class Program:
         if s.type_id == L.EMPTY:
             return None
         if s.type_id == L.STRING:
-            return (s.type_id, ctypes.string_at(s.str, s.str_len).decode())
+            text = ctypes.string_at(s.str, s.str_len).decode()
+            # TRIM's literal also carries the side (1 leading, 2 trailing, 3 both)
+            return (s.type_id, text, s.v.i64) if s.v.i64 else (s.type_id, text)
         return (s.type_id, s.v.f64 if is_float(s.type_id) else s.v.i64)
 
     def output_type(self, in_types, k=0):
@@ -175,13 +177,15 @@ def _scalar_array(scalars, n):
     for i, sc in enumerate(scalars):
         if sc is None:
             continue
-        t, v = sc
+        t, v = sc[0], sc[1]
         arr[i].type_id = t
         if t == L.STRING:
             b = v.encode() if isinstance(v, str) else bytes(v)
             keep.append(b)
             arr[i].str = b
             arr[i].str_len = len(b)
+            if len(sc) > 2:
+                arr[i].v.i64 = int(sc[2])
         elif is_float(t):
             arr[i].v.f64 = float(v)
         else:

@@ -145,13 +145,15 @@ class _ProgHolder:
             for i, sc in enumerate(prog.get(key) or [None] * n):
                 if sc is None:
                     continue
-                t, v = sc
+                t, v = sc[0], sc[1]
                 arr[i].type_id = t
                 if t == STRING:
                     b = v.encode() if isinstance(v, str) else bytes(v)
                     self._keep.append(b)
                     arr[i].str = b
                     arr[i].str_len = len(b)
+                    if len(sc) > 2:
+                        arr[i].v.i64 = int(sc[2])  # TRIM's side
                 elif t in (FLOAT32, FLOAT64):
                     arr[i].v.f64 = float(v)
                 else:

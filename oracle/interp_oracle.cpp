@@ -23,6 +23,13 @@
 //   per-op operand type propagation                  engine/src/Interpreter/interpreter_cpp.cu:502-550
 //   output type rules get_output_type                engine/src/parser/expression_utils.cpp:115-266
 //   filter: null/false drops the row, stable order  engine/src/execution_kernels/LogicalFilter.cpp:22-52
+//   string functions inside programs                 engine/src/execution_kernels/LogicalProject.cpp:
+//       LIKE :31-47 and :113-131, SUBSTRING/LEFT/RIGHT :183-306, CAST(varchar)
+//       :379-468 (libcudf 21.08 to_integers / to_floats), TRIM :566-579 (strip)
+//
+// CAST(varchar AS FLOAT/DOUBLE) multiplies by exp10(e); here that factor is
+// the double nearest 10^e, so its last bit is parity unpinned where the
+// reference's exp10 is not correctly rounded.
 //
 // Values the reference leaves undefined are given one fixed definition here
 // and in the HIP kernels alike (they are "parity unpinned" against the
@@ -34,9 +41,12 @@
 // Build: g++ -O2 -std=c++17 -ffp-contract=off -fno-fast-math -fPIC -shared
 //        -pthread (see oracle/build_oracle.py).
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <regex>
 #include <string>
@@ -291,7 +301,8 @@ int out_type_binary(int op, int lt, int rt) {  // expression_utils.cpp:201-266
     case BSQL_OP_POW: case BSQL_OP_ROUND: return type_size(lt) <= 4 ? BSQL_FLOAT32 : BSQL_FLOAT64;
     case BSQL_OP_MAGIC_IF_NOT: return rt;
     case BSQL_OP_FIRST_NON_MAGIC: return type_size(lt) >= type_size(rt) ? lt : rt;
-    case BSQL_OP_STR_SUBSTRING: case BSQL_OP_STR_LEFT: case BSQL_OP_STR_RIGHT: return BSQL_STRING;
+    case BSQL_OP_STR_SUBSTRING: case BSQL_OP_STR_LEFT: case BSQL_OP_STR_RIGHT: case BSQL_OP_STR_TRIM:
+      return BSQL_STRING;
   }
   return BSQL_EMPTY;  // IS_NOT_DISTINCT_FROM and others fall to the default branch
 }
@@ -452,6 +463,125 @@ void prepare_like(Exec& ex) {
 }
 
 // string operand: an input column, or a string an earlier op produced
+// cudf::strings::strip (libcudf 21.08, LogicalProject.cpp:567-580): drops
+// the leading (side bit 0) and trailing (bit 1) characters that are in
+// `set`, or for an empty set those whose encoded value is <= ' '.
+std::string strip_chars(StrView s, StrView set, int side) {
+  const std::vector<std::string> chars = [&] {
+    std::vector<std::string> v;
+    for (int32_t i = 0; i < s.n;) {
+      int32_t j = i + 1;
+      while (j < s.n && (((uint8_t)s.p[j]) & 0xC0) == 0x80) ++j;
+      v.emplace_back(s.p + i, (size_t)(j - i));
+      i = j;
+    }
+    return v;
+  }();
+  std::vector<std::string> strip;
+  for (int32_t i = 0; i < set.n;) {
+    int32_t j = i + 1;
+    while (j < set.n && (((uint8_t)set.p[j]) & 0xC0) == 0x80) ++j;
+    strip.emplace_back(set.p + i, (size_t)(j - i));
+    i = j;
+  }
+  auto is_strip = [&](const std::string& ch) {
+    if (strip.empty()) {
+      uint32_t v = 0;
+      for (size_t k = 0; k < ch.size(); ++k) v |= (uint32_t)(uint8_t)ch[k] << (8 * k);
+      return v <= (uint32_t)' ';
+    }
+    for (const std::string& x : strip)
+      if (x == ch) return true;
+    return false;
+  };
+  size_t lo = 0, hi = chars.size();
+  if (side & 1) while (lo < hi && is_strip(chars[lo])) ++lo;
+  if (side & 2) while (hi > lo && is_strip(chars[hi - 1])) --hi;
+  std::string out;
+  for (size_t k = lo; k < hi; ++k) out += chars[k];
+  return out;
+}
+
+// cudf::strings::to_integers (libcudf 21.08): optional sign, then digits up
+// to the first other character, in wrapping arithmetic.
+int64_t parse_int(StrView s) {
+  int32_t i = 0;
+  uint64_t sign = 1;
+  if (s.n > 0 && (s.p[0] == '-' || s.p[0] == '+')) { sign = s.p[0] == '-' ? ~0ULL : 1ULL; i = 1; }
+  uint64_t v = 0;
+  for (; i < s.n; ++i) {
+    const char c = s.p[i];
+    if (c < '0' || c > '9') break;
+    v = v * 10ULL + (uint64_t)(c - '0');
+  }
+  return (int64_t)(v * sign);
+}
+
+// The double nearest 10^e, from the C library's correctly rounded strtod.
+double pow10_nearest(int e) {
+  static const std::vector<double> table = [] {
+    std::vector<double> t;
+    for (int k = -307; k <= 308; ++k) t.push_back(std::strtod(("1e" + std::to_string(k)).c_str(), nullptr));
+    return t;
+  }();
+  return table[(size_t)(e + 307)];
+}
+
+// cudf::strings::to_floats (libcudf 21.08), which LogicalProject.cpp:439-468
+// calls for CAST(varchar AS DOUBLE/FLOAT): the string exactly "NaN", "Inf" or
+// "-Inf"; else an optional sign, then digits and '.' (any '.' puts the digits
+// after it in the fraction), digits kept while the value stays within 52 bits
+// (later integer digits only raise the exponent), then an optional exponent
+// 'e'/'E' with a sign. A decimal exponent above 308 gives +-infinity, below
+// -307 gives +0, otherwise sign * digits * exp10(e). exp10 is taken as the
+// double nearest 10^e.
+double parse_double(StrView s) {
+  auto at = [&](int32_t k) { return (uint8_t)s.p[k]; };
+  const std::string str(s.p, (size_t)s.n);
+  if (s.n == 0) return 0.0;
+  if (str == "NaN") return std::numeric_limits<double>::quiet_NaN();
+  if (str == "Inf") return std::numeric_limits<double>::infinity();
+  if (str == "-Inf") return -std::numeric_limits<double>::infinity();
+  int32_t i = 0;
+  double sign = 1.0;
+  if (at(0) == '-' || at(0) == '+') { sign = at(0) == '-' ? -1.0 : 1.0; ++i; }
+  const uint64_t limit = (1ULL << 52) - 1;
+  uint64_t mant = 0;
+  int64_t shift = 0;
+  bool frac = false;
+  for (; i < s.n; ++i) {
+    const uint8_t c = at(i);
+    if (c == '.') { frac = true; continue; }
+    if (!std::isdigit(c)) break;
+    if (mant > limit) { if (!frac) ++shift; continue; }
+    mant = mant * 10 + (uint64_t)(c - '0');
+    if (mant > limit) { mant /= 10; if (!frac) ++shift; }
+    else if (frac) --shift;
+  }
+  int64_t ex = 0;
+  if (i < s.n) {
+    const uint8_t c = at(i++);
+    if ((c == 'e' || c == 'E') && i < s.n) {
+      bool neg = false;
+      if (at(i) == '-' || at(i) == '+') neg = at(i++) == '-';
+      uint32_t v = 0;  // the reference accumulates in a 32-bit int, wrapping
+      while (i < s.n) {
+        const uint8_t d = at(i++);
+        if (!std::isdigit(d)) break;
+        v = v * 10u + (uint32_t)(d - '0');
+      }
+      ex = (int32_t)v;
+      if (neg) ex = -(int64_t)(int32_t)v;
+    }
+  }
+  const int32_t e = (int32_t)(uint32_t)(ex + shift);
+  if (e > 308) return sign > 0 ? std::numeric_limits<double>::infinity() : -std::numeric_limits<double>::infinity();
+  if (e < -307) return 0.0;
+  double base = sign * (double)mant;
+  if (e) base *= pow10_nearest(e);
+  return base;
+}
+
 StrView get_str(const Exec& ex, const RowCtx& c, int pos, int64_t row) {
   if (pos < ex.in.n) return col_str(ex.in.cols[pos], row);
   return StrView{c.str[pos].data(), (int32_t)c.str[pos].size()};
@@ -558,6 +688,9 @@ void eval_op(const Exec& ex, size_t i, int64_t row, RowCtx& c) {
       }
       c.str[opos] = res;
       setv(c, opos, lv);
+    } else if (op == BSQL_OP_STR_TRIM) {
+      c.str[opos] = lv ? strip_chars(ls, rs, (int)p.rsc[i].v.i64) : std::string();
+      setv(c, opos, lv);
     } else if (op == BSQL_OP_IS_NOT_DISTINCT_FROM) {
       bool res;
       if (!lv && !rv) res = true;
@@ -660,9 +793,12 @@ void eval_op(const Exec& ex, size_t i, int64_t row, RowCtx& c) {
           break;
         case BSQL_OP_CAST_TINYINT: case BSQL_OP_CAST_SMALLINT:
         case BSQL_OP_CAST_INTEGER: case BSQL_OP_CAST_BIGINT:
-          store_i(c, opos, lf ? f64_to_i64(L.d) : L.i);
+          if (is_string_type(lt)) store_i(c, opos, parse_int(ls));  // LogicalProject.cpp:379-438
+          else store_i(c, opos, lf ? f64_to_i64(L.d) : L.i);
           break;
-        case BSQL_OP_CAST_FLOAT: case BSQL_OP_CAST_DOUBLE: store_f(c, opos, x); break;
+        case BSQL_OP_CAST_FLOAT: case BSQL_OP_CAST_DOUBLE:
+          store_f(c, opos, is_string_type(lt) ? parse_double(ls) : x);  // :439-468
+          break;
         case BSQL_OP_CAST_DATE: {
           int64_t v = lf ? f64_to_i64(L.d) : L.i;
           if (is_timestamp_type(lt) && lt != BSQL_TIMESTAMP_DAYS) v = v / ticks_per_day(lt);  // truncation

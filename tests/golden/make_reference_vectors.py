@@ -159,6 +159,19 @@ vectors["string_substring_null"] = {
     "expected": ["B", "2", "", "Y"], "expected_valid": [1, 1, 0, 1],
 }
 
+# process_project.cpp:415-446 -- CAST from strings to INTEGER and DOUBLE (the
+# TIMESTAMP column of that test is a string-to-timestamp parse, not built here)
+vectors["cast_from_string"] = {
+    "source": "engine/tests/new-tests/process_project.cpp:415-446",
+    "col1": ["1", "5", "10", "15", "100", "500", "1000", "5000", "10000", "999999"],
+    "col2": ["1.0", "5.5", "10.00003", "15.45", "100.656", "500.756756", "0.45435436", "3.24e-05", "0.1",
+             "999999.001"],
+    "query": "LogicalProject(EXPR$0=[CAST($0):INTEGER], EXPR$1=[CAST($1):DOUBLE])",
+    "expected_col1": [1, 5, 10, 15, 100, 500, 1000, 5000, 10000, 999999],
+    "expected_col2": [1.0, 5.5, 10.00003, 15.45, 100.656, 500.756756, 0.45435436, 0.0000324, 0.1, 999999.001],
+    "compare_col2": "expect_tables_equivalent: floating point within cudf's ULP tolerance",
+}
+
 # process_project.cpp:489-513 -- ROUND with 0, 2 and 5 digits (float and double)
 vectors["round"] = {
     "source": "engine/tests/new-tests/process_project.cpp:489-513",

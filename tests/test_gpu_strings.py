@@ -178,3 +178,80 @@ def test_strings_math_queries_vs_oracle(bsql, gpu, n):
                 assert np.all(np.abs(a - b) <= np.spacing(np.abs(b))), f"{pred_text} out {k}"
             else:
                 assert_same(gv, gval, ov, oval, what=f"{pred_text} out {k}")
+
+
+def test_golden_cast_from_string(bsql, gpu, golden):
+    """process_project.cpp:415-446: CAST(varchar) to INTEGER exactly, to DOUBLE
+    within cudf's 4-ulp equivalence."""
+    from blazingsql_amd.interop import DeviceColumn
+    g = golden["cast_from_string"]
+    n = len(g["col1"])
+    cols = [DeviceColumn.from_strings(g["col1"]), DeviceColumn.from_strings(g["col2"])]
+    _, ot, res = _run(bsql, ["CAST($0):INTEGER", "CAST($1):DOUBLE"], cols, [STRING, STRING], n)
+    assert ot == [INT32, FLOAT64]
+    np.testing.assert_array_equal(res[0][0], g["expected_col1"])
+    exp = np.asarray(g["expected_col2"])
+    assert np.abs(res[1][0].view(np.int64) - exp.view(np.int64)).max() <= 4
+
+
+def _numeric_strings(rng, n):
+    pieces = ["", "-", "+", "12", "7", "0", ".", "5", "e", "E-", "e+3", "x", " ", "9" * 19, "NaN", "Inf",
+              "-Inf", "3.25", "1e308", "e400", "-0"]
+    strs = ["".join(rng.choice(pieces, int(rng.integers(0, 4)))) for _ in range(n)]
+    return strs, rng.random(n) >= 0.1
+
+
+@pytest.mark.parametrize("n", [1, 1000, 70_001])
+def test_trim_and_string_casts_vs_oracle(bsql, gpu, n):
+    """TRIM (three sides; a space, a multi-byte set, the empty set) as string
+    outputs, nested under CHAR_LENGTH, LIKE and comparisons; CAST of strings
+    to every integer width, FLOAT and DOUBLE, also in arithmetic; nulls."""
+    import oracle
+    from blazingsql_amd.interop import DeviceColumn
+    rng = np.random.default_rng(n + 11)
+    s0, v0 = _string_table(rng, n, list("ab ") + ["é", "ß", "\t"], 0, 12, 0.1)
+    s1, v1 = _numeric_strings(rng, n)
+    types = [STRING, STRING]
+    dev = [DeviceColumn.from_strings(s0, v0), DeviceColumn.from_strings(s1, v1)]
+    host = [oracle.HostColumn(STRING, strings=s0, valid=v0), oracle.HostColumn(STRING, strings=s1, valid=v1)]
+    exprs = ["TRIM(FLAG(BOTH), ' ', $0)", "TRIM(FLAG(LEADING), 'aé', $0)", "TRIM(FLAG(TRAILING), 'ßb ', $0)",
+             "TRIM(FLAG(BOTH), '', $0)", "CHAR_LENGTH(TRIM(FLAG(BOTH), ' ', $0))",
+             "LIKE(TRIM(FLAG(LEADING), ' ', $0), 'a%')", "=(TRIM(FLAG(BOTH), ' ', $0), 'ab')",
+             "SUBSTRING(TRIM(FLAG(BOTH), ' ', $0), 2)", "TRIM(FLAG(BOTH), 'a', SUBSTRING($0, 2))",
+             "CAST($1):TINYINT", "CAST($1):SMALLINT", "CAST($1):INTEGER", "CAST($1):BIGINT", "CAST($1):FLOAT",
+             "CAST($1):DOUBLE", "+(CAST($1):BIGINT, 1)", "*(CAST($1):DOUBLE, 2)",
+             "CAST(TRIM(FLAG(BOTH), ' ', $1)):INTEGER"]
+    for e in exprs:
+        p, ot, res = _run(bsql, [e], dev, types, n)
+        ores = _oracle(bsql, p, host, ot, n)
+        _same(res[0], ores[0], e)
+
+
+@pytest.mark.parametrize("ranges", [True, False])
+def test_filter_on_trim_and_casts_vs_oracle(bsql, gpu, ranges):
+    """process_filter with predicates over TRIM and string CASTs; the string
+    columns compacted with the rest."""
+    import oracle
+    import torch
+    from blazingsql_amd.interop import DeviceColumn, filter_columns
+    from tests.test_gpu_parity import _full_prog
+    n = 120_007
+    rng = np.random.default_rng(12)
+    s0, v0 = _string_table(rng, n, list("ab "), 0, 6, 0.1)
+    s1, v1 = _numeric_strings(rng, n)
+    types = [STRING, STRING]
+    dev = [DeviceColumn.from_strings(s0, v0), DeviceColumn.from_strings(s1, v1)]
+    host = [oracle.HostColumn(STRING, strings=s0, valid=v0), oracle.HostColumn(STRING, strings=s1, valid=v1)]
+    for pred_text in ["=(TRIM(FLAG(BOTH), ' ', $0), 'ab')", ">(CAST($1):INTEGER, 5)",
+                      "AND(<(CAST($1):DOUBLE, 100), >(CHAR_LENGTH(TRIM(FLAG(TRAILING), ' ', $0)), 2))"]:
+        pred = bsql.Program.compile(pred_text, types)
+        pred.set_launch(ranges=ranges)
+        outs = [DeviceColumn.empty(STRING, n, chars=dev[0].chars_size),
+                DeviceColumn.empty(STRING, n, chars=dev[1].chars_size)]
+        cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+        filter_columns(pred, dev, outs, cnt, n)
+        m = int(cnt.item())
+        on, ores = oracle.filter_table(_full_prog(bsql, pred_text, types), host, n)
+        assert m == on and m > 0, pred_text
+        for k in range(2):
+            _same(outs[k].to_numpy(m), ores[k], f"{pred_text} col {k}")

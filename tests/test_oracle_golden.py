@@ -353,3 +353,111 @@ def test_mlp_reference_gradients_match_finite_differences():
         e[i] = h
         fd = (mlp_ref.loss(p + e, x, t, 0.5) - mlp_ref.loss(p - e, x, t, 0.5)) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, g[i])
+
+
+def test_cast_from_string_golden(oracle_lib, golden, bsql):
+    """CAST(varchar) to INTEGER and DOUBLE (process_project.cpp:415-446): the
+    integers exactly, the doubles within 4 ulp as cudf's equivalence check."""
+    g = golden["cast_from_string"]
+    from oracle import HostColumn
+    p, ot = _compiled(bsql, ["CAST($0):INTEGER", "CAST($1):DOUBLE"], [23, 23])
+    assert ot == [3, 10]
+    (a, va), (b, vb) = oracle_lib.execute(p.to_oracle_dict(), [HostColumn(23, strings=g["col1"]),
+                                                               HostColumn(23, strings=g["col2"])], ot)
+    np.testing.assert_array_equal(a, g["expected_col1"])
+    exp = np.asarray(g["expected_col2"])
+    ulps = np.abs(b.view(np.int64) - exp.view(np.int64))
+    assert va.all() and vb.all() and ulps.max() <= 4, ulps
+
+
+def _int_closed_form(s, bits):
+    """An optional sign and the longest run of decimal digits after it, the
+    value wrapped to two's complement of `bits`."""
+    import re
+    m = re.match(r"([+-]?)([0-9]*)", s)
+    v = int(m.group(2) or "0") * (-1 if m.group(1) == "-" else 1)
+    v &= (1 << bits) - 1
+    return v - (1 << bits) if v >> (bits - 1) else v
+
+
+def test_cast_string_to_integers_closed_form(oracle_lib, bsql):
+    """cudf::strings::to_integers: sign + digit prefix, wrapping, for every
+    signed width; empty and non-numeric strings give 0; nulls stay null."""
+    from oracle import HostColumn
+    rng = np.random.default_rng(4)
+    strs = ["", "+", "-", "0", "-0", "+17", "12abc", "abc", " 5", "3.99", "-1e3", "9" * 25,
+            "-9223372036854775808", "9223372036854775808", "18446744073709551617", "-2147483649", "255", "-129"]
+    strs += ["".join(rng.choice(list("0123456789+-.a "), int(rng.integers(0, 22)))) for _ in range(400)]
+    valid = [i % 7 != 3 for i in range(len(strs))]
+    col = HostColumn(23, strings=strs, valid=valid)
+    for cast, t, bits in [("TINYINT", 1, 8), ("SMALLINT", 2, 16), ("INTEGER", 3, 32), ("BIGINT", 4, 64)]:
+        p, ot = _compiled(bsql, [f"CAST($0):{cast}"], [23])
+        assert ot == [t]
+        (v, ok), = oracle_lib.execute(p.to_oracle_dict(), [col], ot)
+        np.testing.assert_array_equal(ok, valid)
+        got = [int(x) for x, k in zip(v, ok) if k]
+        assert got == [_int_closed_form(s, bits) for s, k in zip(strs, valid) if k], cast
+
+
+def test_cast_string_to_double_closed_forms(oracle_lib, bsql):
+    """cudf::strings::to_floats. Closed forms: an integer string under 2^52
+    times 10^e for 0 <= e <= 22 is one rounding of exact operands, so it is
+    Python's correctly rounded float(); digits past 52 bits are truncated,
+    not rounded; any decimal string lands within 1 ulp of float(); the special
+    spellings are exact-match only; exponents past the double range give
+    +-inf or +0 (and a zero mantissa with a large exponent infinity)."""
+    import math
+    from oracle import HostColumn
+    rng = np.random.default_rng(5)
+    exact = []
+    for _ in range(300):
+        d = int(rng.integers(0, 2 ** 52))
+        e = int(rng.integers(0, 23))
+        txt = f"{'-' if rng.random() < 0.5 else ''}{d}e{e}"
+        exact.append((txt, float(txt)))
+        exact.append((f"{d}", float(d)))
+    fixed = [("NaN", math.nan), ("Inf", math.inf), ("-Inf", -math.inf), ("+Inf", 0.0), ("nan", 0.0),
+             ("Infinity", 0.0), ("inf", 0.0), ("", 0.0), ("abc", 0.0), ("-abc", -0.0), ("1e309", math.inf),
+             ("-1e309", -math.inf), ("1e-400", 0.0), ("-1e-400", 0.0), ("0e500", math.inf),
+             ("1.5.5", 1.55), ("12345678901234569999", 1.234567890123456e19), ("4503599627370495", 4503599627370495.0),
+             ("45035996273704951", 4503599627370495e1), ("1e", 1.0), ("2E+3", 2000.0), ("7e-0", 7.0),
+             (".5", 0.5), ("-.25", -0.25), ("3.", 3.0), ("1e308", 1e308), ("12x3", 12.0)]
+    approx = [f"{rng.integers(0, 10 ** 9)}.{rng.integers(0, 10 ** 6):06d}e{int(rng.integers(-300, 290))}"
+              for _ in range(300)]
+    strs = [s for s, _ in exact + fixed] + approx
+    col = HostColumn(23, strings=strs)
+    p, ot = _compiled(bsql, ["CAST($0):DOUBLE", "CAST($0):FLOAT"], [23])
+    assert ot == [10, 9]
+    (v, ok), (f, fok) = oracle_lib.execute(p.to_oracle_dict(), [col], ot)
+    assert ok.all() and fok.all()
+    with np.errstate(over="ignore"):
+        np.testing.assert_array_equal(f, v.astype(np.float32))
+    n1 = len(exact) + len(fixed)
+    want = np.array([x for _, x in exact + fixed])
+    np.testing.assert_array_equal(v[:n1].view(np.int64)[~np.isnan(want)], want.view(np.int64)[~np.isnan(want)])
+    assert np.isnan(v[:n1][np.isnan(want)]).all()
+    ref = np.array([float(s) for s in approx])
+    ulps = np.abs(v[n1:].view(np.int64) - ref.view(np.int64))
+    assert ulps.max() <= 1, (ulps.max(), approx[int(ulps.argmax())])
+
+
+def test_trim_closed_forms(oracle_lib, bsql):
+    """TRIM(FLAG(BOTH|LEADING|TRAILING), chars, $0) as cudf::strings::strip
+    (LogicalProject.cpp:566-579): Python's strip/lstrip/rstrip over the set of
+    characters (multi-byte ones included); an empty set strips bytes <= ' '."""
+    from oracle import HostColumn
+    rng = np.random.default_rng(6)
+    alphabet = list("ab ") + ["é", "ß", "\t", "\n", "\x01"]
+    strs = ["".join(rng.choice(alphabet, int(rng.integers(0, 9)))) for _ in range(800)] + ["", " ", "é", "ab"]
+    valid = [i % 11 != 5 for i in range(len(strs))]
+    col = HostColumn(23, strings=strs, valid=valid)
+    ws = "".join(chr(c) for c in range(33))
+    for side, fn in [("BOTH", str.strip), ("LEADING", str.lstrip), ("TRAILING", str.rstrip)]:
+        for chars in [" ", "ab", "é ", "ßé", ""]:
+            expr = f"TRIM(FLAG({side}), '{chars}', $0)"
+            p, ot = _compiled(bsql, [expr], [23])
+            assert ot == [23]
+            (v, ok), = oracle_lib.execute(p.to_oracle_dict(), [col], ot)
+            np.testing.assert_array_equal(ok, valid)
+            got = [x for x, k in zip(v, ok) if k]
+            assert got == [fn(s, chars or ws) for s, k in zip(strs, valid) if k], expr
