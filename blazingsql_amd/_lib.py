@@ -55,6 +55,17 @@ class Scalar(ctypes.Structure):
                 ("v", _ScalarValue), ("str", ctypes.c_char_p)]
 
 
+class StepInfo(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("op", ctypes.c_int32), ("out_type", ctypes.c_int32),
+                ("n_args", ctypes.c_int32), ("arg_col", ctypes.c_int32 * 3),
+                ("arg_str", ctypes.c_void_p * 3), ("arg_len", ctypes.c_int32 * 3),
+                ("sub", ctypes.c_void_p)]
+
+
+STRFN = {"CONCAT": 1, "UPPER": 2, "LOWER": 3, "INITCAP": 4, "REVERSE": 5, "REPLACE": 6, "CAST": 7,
+         "CASE": 8, "FILL": 9}
+
+
 class LaunchConfig(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_int32), ("rows_per_thread", ctypes.c_int32),
                 ("flags", ctypes.c_int32)]
@@ -84,6 +95,10 @@ SIGNATURES = {
     "bsql_program_scalar": (_I, [_P, _I, _I, ctypes.POINTER(Scalar)]),
     "bsql_program_output_type": (_I, [_P, _P, _I, _I, ctypes.POINTER(ctypes.c_int32)]),
     "bsql_program_used_columns": (_I, [_P, _P, _I]),
+    "bsql_program_num_steps": (_I, [_P]),
+    "bsql_program_table_columns": (_I, [_P]),
+    "bsql_program_step": (_I, [_P, _I, ctypes.POINTER(StepInfo)]),
+    "bsql_program_output_chars": (_I, [_P, _P, _I, _I64, ctypes.POINTER(_I64)]),
     "bsql_interp_execute": (_I, [_P, _P, _I, _P, _I, _I64, _P]),
     "bsql_filter": (_I, [_P, _P, _I, _P, _I64, _P, _P]),
     "bsql_filter_project": (_I, [_P, _P, _P, _I, _P, _I, _I64, _P, _P]),

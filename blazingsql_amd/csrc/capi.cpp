@@ -253,6 +253,19 @@ int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, i
   if (!pred) fail("null predicate");
   if (!d_count) fail("null count pointer");
   check_inputs(in, n_in, num_rows);
+  // step columns: the predicate's after the table, the projection's after those
+  StepColumns sc;
+  if (!pred->steps.empty() || (proj && !proj->steps.empty())) {
+    sc.stream = (hipStream_t)stream;
+    sc.cols.assign(in, in + n_in);
+    build_steps(*pred, sc, num_rows);
+    if (proj && !proj->steps.empty()) {
+      proj = rebased_program(proj, (int)sc.cols.size());
+      build_steps(*proj, sc, num_rows);
+    }
+    in = sc.cols.data();
+    n_in = (int)sc.cols.size();
+  }
   // selected rows are listed as 32-bit row ids
   if (num_rows > (int64_t)UINT32_MAX) fail("a filter handles at most 2^32-1 rows per call");
   const bool outputs = (phases & PHASE_OUTPUT) != 0;
@@ -304,6 +317,60 @@ int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, i
   if (e == hipSuccess && outputs)
     e = finish_string_outputs(L, in, out, n_out, slices, d_count, 0, capacity, status, (hipStream_t)stream);
   if (e != hipSuccess) return device_error(e, "filter");
+  return BSQL_OK;
+}
+
+int execute_program(bsql_program* prog, const bsql_column* in, int n_in, bsql_column* out, int n_out,
+                    int64_t num_rows, void* stream, bool private_scratch) {
+  if (!prog) fail("null program");
+  if (!prog->steps.empty() && n_in == prog->n_table) fail("the program's step columns were not built");
+  check_inputs(in, n_in, num_rows);
+  if (n_out != (int)prog->final_out.size()) fail("number of outputs does not match the program");
+  check_outputs(out, n_out, num_rows);
+  hipError_t e = ensure_strings(*prog);
+  if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
+  LowerOptions opt;
+  opt.mode = LowerOptions::EXECUTE;
+  opt.proj = prog;
+  const Lowered& L = lowered_for(prog, opt, in, n_in, out, n_out);
+  DevPlan plan = L.plan;
+  plan.num_rows = num_rows;
+  for (int k = 0; k < n_in; ++k) plan.in[k] = to_dev_in(in[k]);
+  const int n_str = count_string_outputs(L, in, out, n_out);
+  // scratch: plan | string assembly status + ticket | one slice column per string output
+  const int64_t st_words = n_str ? string_tiles(num_rows) + 1 : 0;
+  const int64_t words = st_words + (int64_t)n_str * num_rows;
+  uint64_t* scratch = nullptr;
+  if (private_scratch) {
+    e = hipMallocAsync((void**)&scratch, (size_t)(kPlanWords + std::max<int64_t>(words, 1)) * 8,
+                       (hipStream_t)stream);
+  } else {
+    e = get_scratch((hipStream_t)stream, words, &scratch);
+  }
+  if (e != hipSuccess) return device_error(e, "bsql_interp_execute scratch");
+  struct Release {
+    uint64_t* p;
+    hipStream_t s;
+    ~Release() {
+      if (p) (void)hipFreeAsync(p, s);
+    }
+  } release{private_scratch ? scratch : nullptr, (hipStream_t)stream};
+  uint64_t* status = scratch + kPlanWords;
+  uint64_t* slices = status + st_words;
+  for (int k = 0, j = 0; k < n_out; ++k) {
+    const bool str = L.out_str_col[(size_t)k] >= 0;
+    plan.out[k].data = str ? (void*)(slices + (size_t)(j++) * (size_t)num_rows) : out[k].data;
+    plan.out[k].valid = out[k].validity;
+    plan.out[k].type = out[k].type_id;
+  }
+  if (num_rows == 0 && n_str == 0) return BSQL_OK;
+  e = upload_plan(plan, scratch, (hipStream_t)stream);
+  if (e == hipSuccess && num_rows > 0)
+    e = launch_execute(plan, (const DevPlan*)scratch, L.nreg, rows_per_thread(prog), (hipStream_t)stream);
+  if (e == hipSuccess)
+    e = finish_string_outputs(L, in, out, n_out, slices, nullptr, num_rows, num_rows, status,
+                              (hipStream_t)stream);
+  if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
   return BSQL_OK;
 }
 
@@ -368,11 +435,7 @@ int bsql_program_compile(const char* const* exprs, int n_exprs, const int32_t* i
   })
 }
 
-void bsql_program_free(bsql_program* prog) {
-  if (!prog) return;
-  if (prog->d_strings) (void)hipFree(prog->d_strings);
-  delete prog;
-}
+void bsql_program_free(bsql_program* prog) { delete prog; }
 
 int bsql_program_num_ops(const bsql_program* p) { return p ? (int)p->ops.size() : -1; }
 int bsql_program_num_outputs(const bsql_program* p) { return p ? (int)p->final_out.size() : -1; }
@@ -409,7 +472,8 @@ int bsql_program_output_type(const bsql_program* p, const int32_t* in_types, int
                              int32_t* type_id) {
   BSQL_TRY({
     if (!p || !type_id || k < 0 || k >= (int)p->final_out.size()) fail("invalid arguments");
-    std::vector<int32_t> st = slot_types_for(*p, in_types, n_in);
+    const std::vector<int32_t> ext = extended_types(*p, in_types, n_in);
+    std::vector<int32_t> st = slot_types_for(*p, ext.data(), (int)ext.size());
     OpTypes t = propagate_types(*p, st.data(), (int)st.size());
     const int slot = p->final_out[k];
     if (slot < 0) {  // literal root
@@ -470,41 +534,13 @@ int bsql_interp_execute(const bsql_program* cprog, const bsql_column* in, int n_
   BSQL_TRY({
     bsql_program* prog = const_cast<bsql_program*>(cprog);
     if (!prog) fail("null program");
+    if (prog->steps.empty()) return execute_program(prog, in, n_in, out, n_out, num_rows, stream, false);
     check_inputs(in, n_in, num_rows);
-    if (n_out != (int)prog->final_out.size()) fail("number of outputs does not match the program");
-    check_outputs(out, n_out, num_rows);
-    hipError_t e = ensure_strings(*prog);
-    if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
-    LowerOptions opt;
-    opt.mode = LowerOptions::EXECUTE;
-    opt.proj = prog;
-    const Lowered& L = lowered_for(prog, opt, in, n_in, out, n_out);
-    DevPlan plan = L.plan;
-    plan.num_rows = num_rows;
-    for (int k = 0; k < n_in; ++k) plan.in[k] = to_dev_in(in[k]);
-    const int n_str = count_string_outputs(L, in, out, n_out);
-    // scratch: plan | string assembly status + ticket | one slice column per string output
-    const int64_t st_words = n_str ? string_tiles(num_rows) + 1 : 0;
-    uint64_t* scratch = nullptr;
-    e = get_scratch((hipStream_t)stream, st_words + (int64_t)n_str * num_rows, &scratch);
-    if (e != hipSuccess) return device_error(e, "bsql_interp_execute scratch");
-    uint64_t* status = scratch + kPlanWords;
-    uint64_t* slices = status + st_words;
-    for (int k = 0, j = 0; k < n_out; ++k) {
-      const bool str = L.out_str_col[(size_t)k] >= 0;
-      plan.out[k].data = str ? (void*)(slices + (size_t)(j++) * (size_t)num_rows) : out[k].data;
-      plan.out[k].valid = out[k].validity;
-      plan.out[k].type = out[k].type_id;
-    }
-    if (num_rows == 0 && n_str == 0) return BSQL_OK;
-    e = upload_plan(plan, scratch, (hipStream_t)stream);
-    if (e == hipSuccess && num_rows > 0)
-      e = launch_execute(plan, (const DevPlan*)scratch, L.nreg, rows_per_thread(prog), (hipStream_t)stream);
-    if (e == hipSuccess)
-      e = finish_string_outputs(L, in, out, n_out, slices, nullptr, num_rows, num_rows, status,
-                                (hipStream_t)stream);
-    if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
-    return BSQL_OK;
+    StepColumns sc;
+    sc.stream = (hipStream_t)stream;
+    sc.cols.assign(in, in + n_in);
+    build_steps(*prog, sc, num_rows);
+    return execute_program(prog, sc.cols.data(), (int)sc.cols.size(), out, n_out, num_rows, stream, false);
   })
 }
 
@@ -591,6 +627,8 @@ int bsql_batch_prepare(const bsql_batch_desc* descs, int n, bsql_batch** out) {
       const bsql_batch_desc& d = descs[i];
       bsql_program* prog = const_cast<bsql_program*>(d.prog);
       if (!prog) fail("batch " + std::to_string(i) + " has no program");
+      if (!prog->steps.empty())
+        fail("batch " + std::to_string(i) + ": programs that build string columns do not run batched");
       check_inputs(d.in, d.n_in, d.num_rows);
       if (d.n_out != (int)prog->final_out.size()) fail("batch " + std::to_string(i) + ": outputs do not match");
       check_outputs(d.out, d.n_out, d.num_rows);

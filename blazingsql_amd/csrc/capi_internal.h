@@ -5,7 +5,9 @@
 
 #include <map>
 #include <mutex>
+#include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/bsql/interp.h"
 #include "program.hpp"
@@ -14,6 +16,12 @@
 struct bsql_program : bsql::Program {
   std::mutex cache_mu;
   std::map<std::string, bsql::Lowered> cache;
+  // copies whose step columns are numbered from another table width (a
+  // projection's steps placed after a predicate's), by that width
+  std::map<int, std::unique_ptr<bsql_program>> rebased;
+  ~bsql_program() {
+    if (d_strings) (void)hipFree(d_strings);
+  }
 };
 
 namespace bsql {
@@ -31,6 +39,32 @@ int device_error(hipError_t e, const char* where);
 enum { PHASE_SELECT = 1, PHASE_OUTPUT = 2, PHASE_ALL = 3 };
 int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, int n_in, bsql_column* out,
                 int n_out, int64_t num_rows, int64_t capacity, int64_t* d_count, void* stream, int phases);
+
+// The body of bsql_interp_execute for a program without steps; a private
+// scratch buffer (allocated and freed on the stream) leaves the stream's
+// shared scratch, and the row ids a filter keeps there, untouched.
+int execute_program(bsql_program* prog, const bsql_column* in, int n_in, bsql_column* out, int n_out,
+                    int64_t num_rows, void* stream, bool private_scratch);
+
+// Columns of the table followed by the columns a program's steps build.
+// Owned buffers are released on the stream when this is destroyed.
+struct StepColumns {
+  std::vector<bsql_column> cols;
+  std::vector<void*> owned;
+  hipStream_t stream = nullptr;
+  StepColumns() = default;
+  StepColumns(const StepColumns&) = delete;
+  StepColumns& operator=(const StepColumns&) = delete;
+  ~StepColumns();
+};
+// Appends the columns of p's steps to sc.cols (which holds p's table).
+void build_steps(bsql_program& p, StepColumns& sc, int64_t num_rows);
+// p with its step columns numbered from table width n_table.
+bsql_program* rebased_program(bsql_program* p, int n_table);
+// Chars a string output over these columns (table + steps) can need.
+int64_t string_chars_bound(const bsql_program& p, const bsql_column* in, int n_in, int64_t num_rows);
+// Types of the table's columns followed by the steps' columns.
+std::vector<int32_t> extended_types(const Program& p, const int32_t* types, int n);
 }  // namespace bsql
 
 // Body of a C entry point: exceptions become return codes + message.

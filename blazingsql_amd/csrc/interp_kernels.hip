@@ -1836,15 +1836,29 @@ hipError_t launch_batches(const DevPlan* plans, const int64_t* tile_start, int n
 // (device) or `count` (host) gives the rows.
 int64_t string_tiles(int64_t capacity) { return capacity > 0 ? (capacity + STR_TILE - 1) / STR_TILE : 1; }
 
-hipError_t launch_string_output(const uint64_t* slices, const uint32_t* valid, int32_t* offsets,
-                                const uint8_t* src_chars, uint8_t* dst_chars, const int64_t* d_count,
-                                int64_t count, int64_t capacity, uint64_t* status, hipStream_t stream) {
+static void launch_offsets(const uint64_t* slices, const uint32_t* valid, int32_t* offsets,
+                           const int64_t* d_count, int64_t count, int64_t capacity, uint64_t* status,
+                           hipStream_t stream) {
   const int64_t tiles = string_tiles(capacity);
   CompactArgs ca{status, (uint32_t*)(status + tiles), nullptr, capacity, tiles};
   hipLaunchKernelGGL(fill_u64, dim3((unsigned)((tiles + 1 + 255) / 256)), dim3(256), 0, stream, status,
                      tiles + 1, 0ULL);
   hipLaunchKernelGGL(str_offsets_kernel, dim3((unsigned)tiles), dim3(BLOCK), 0, stream, slices, valid, offsets,
                      d_count, count, ca);
+}
+
+// Offsets (count + 1 of them) of a string column from row lengths stored as
+// length << 32 in `slices` (null rows: 0). `status`: string_tiles(count) + 1 words.
+hipError_t launch_string_offsets(const uint64_t* slices, const uint32_t* valid, int32_t* offsets, int64_t count,
+                                 uint64_t* status, hipStream_t stream) {
+  launch_offsets(slices, valid, offsets, nullptr, count, count, status, stream);
+  return hipGetLastError();
+}
+
+hipError_t launch_string_output(const uint64_t* slices, const uint32_t* valid, int32_t* offsets,
+                                const uint8_t* src_chars, uint8_t* dst_chars, const int64_t* d_count,
+                                int64_t count, int64_t capacity, uint64_t* status, hipStream_t stream) {
+  launch_offsets(slices, valid, offsets, d_count, count, capacity, status, stream);
   const int64_t blocks = persistent_grid((capacity + BLOCK - 1) / BLOCK);
   if (blocks > 0)
     hipLaunchKernelGGL(str_copy_kernel, dim3((unsigned)blocks), dim3(BLOCK), 0, stream, slices, valid, offsets,

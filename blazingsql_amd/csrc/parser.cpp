@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "program.hpp"
+#include "strfn.h"
 
 namespace bsql {
 namespace {
@@ -696,11 +697,10 @@ void collect_vars(const Node* n, std::vector<int>& acc) {
 
 // String functions this build does not evaluate (the reference computes them
 // with libcudf before the interpreter, LogicalProject.cpp:100-742). LIKE,
-// SUBSTRING, LEFT and RIGHT run inside the interpreter on string slices.
+// SUBSTRING, LEFT, RIGHT and TRIM run inside the interpreter on string
+// slices; the functions that build new strings become steps (below).
 bool is_string_function(const std::string& v) {
-  static const char* names[] = {"REGEXP_REPLACE", "REPLACE", "TO_DATE", "TO_TIMESTAMP", "||",
-                                "CONCAT", "TRIM", "LOWER", "UPPER", "INITCAP", "REVERSE",
-                                "CAST_VARCHAR", "CAST_CHAR"};
+  static const char* names[] = {"REGEXP_REPLACE", "TO_DATE", "TO_TIMESTAMP", "TRIM"};
   // TRIM with its side flag is interpreted (BLZ_TRIM_*); a bare TRIM is not
   for (const char* s : names)
     if (v == s) return true;
@@ -741,6 +741,138 @@ std::unique_ptr<Node> rewrite_trim(std::unique_ptr<Node> n) {
   return n;
 }
 
+// Result type of a node over the columns' types (get_output_type,
+// expression_utils.cpp:115-266); a null literal is EMPTY.
+int32_t node_type(const Node* n, const std::vector<int32_t>& types) {
+  if (n->type == NodeType::Literal) return is_null_literal(n->value) ? BSQL_EMPTY : n->lit_type;
+  if (n->type == NodeType::Variable) return types.at((size_t)n->var_index);
+  const int op = map_to_operator(n->value);
+  if (is_binary_op(op) && n->children.size() == 2)
+    return output_type_binary(op, node_type(n->children[0].get(), types), node_type(n->children[1].get(), types));
+  if (is_unary_op(op) && n->children.size() == 1)
+    return output_type_unary(op, node_type(n->children[0].get(), types));
+  return output_type_nullary(op);
+}
+
+std::unique_ptr<Node> var_node(int col) {
+  auto v = std::make_unique<Node>();
+  v->type = NodeType::Variable;
+  v->value = "$" + std::to_string(col);
+  v->var_index = col;
+  return v;
+}
+
+std::unique_ptr<Program> compile_trees(std::vector<std::unique_ptr<Node>> trees, int n_cols);
+
+// Replaces, bottom-up, every function that builds new strings by a column
+// built before the launch, as function_evaluator_transformer does
+// (LogicalProject.cpp:670-731): CONCAT / ||, UPPER, LOWER, INITCAP, REVERSE,
+// REPLACE, CAST(x AS VARCHAR) and a CASE whose values are strings
+// (evaluate_string_case_when_else, :612-659). An argument that is an
+// expression is first evaluated into a column of its own (the reference's
+// evaluate_expressions calls for such arguments).
+struct StepExtractor {
+  Program& prog;
+  std::vector<int32_t> types;  // the table's columns, then the steps'
+  int n_table;
+
+  int add_step(Step st) {
+    prog.steps.push_back(std::move(st));
+    types.push_back(prog.steps.back().out_type);
+    return (int)types.size() - 1;
+  }
+
+  int eval_column(std::unique_ptr<Node> e) {
+    if (e->type == NodeType::Variable) return e->var_index;
+    Step st;
+    st.kind = Step::EVAL;
+    st.out_type = node_type(e.get(), types);
+    std::vector<std::unique_ptr<Node>> one;
+    one.push_back(std::move(e));
+    st.sub = make_program_handle(compile_trees(std::move(one), (int)types.size()));
+    return add_step(std::move(st));
+  }
+
+  // a string argument: a column (evaluated first when it is an expression),
+  // a literal or null
+  StepArg string_arg(std::unique_ptr<Node>& c, const char* fn, bool literal_ok) {
+    StepArg a;
+    if (c->type == NodeType::Literal) {
+      if (is_null_literal(c->value)) { a.kind = StepArg::NULL_LITERAL; return a; }
+      if (!is_string_type(c->lit_type)) fail(std::string(fn) + " needs string arguments");
+      if (!literal_ok) fail(std::string(fn) + " operator not supported for string literals");
+      a.kind = StepArg::LITERAL;
+      a.str = scalar_from_string(c->value, BSQL_STRING).str;
+      return a;
+    }
+    if (!is_string_type(node_type(c.get(), types))) fail(std::string(fn) + " needs string arguments");
+    a.col = eval_column(std::move(c));
+    return a;
+  }
+
+  StepArg literal_arg(const Node* c, const char* fn) {
+    if (c->type != NodeType::Literal || is_null_literal(c->value) || !is_string_type(c->lit_type))
+      fail(std::string(fn) + " needs string literals for its search and replacement");
+    StepArg a;
+    a.kind = StepArg::LITERAL;
+    a.str = scalar_from_string(c->value, BSQL_STRING).str;
+    return a;
+  }
+
+  std::unique_ptr<Node> visit(std::unique_ptr<Node> n) {
+    if (n->type != NodeType::Operator) return n;
+    for (auto& c : n->children) c = visit(std::move(c));
+    const std::string& v = n->value;
+    Step st;
+    if (v == "||" || v == "CONCAT") {
+      if (n->children.size() != 2) fail("CONCAT takes two arguments");
+      if (n->children[0]->type == NodeType::Literal && n->children[1]->type == NodeType::Literal)
+        fail("CONCAT operator between literals is not supported");
+      st.op = SF_CONCAT;
+      st.args.push_back(string_arg(n->children[0], "CONCAT", true));
+      st.args.push_back(string_arg(n->children[1], "CONCAT", true));
+    } else if (v == "UPPER" || v == "LOWER" || v == "INITCAP" || v == "REVERSE") {
+      if (n->children.size() != 1) fail(v + " takes one argument");
+      st.op = v == "UPPER" ? SF_UPPER : v == "LOWER" ? SF_LOWER : v == "INITCAP" ? SF_INITCAP : SF_REVERSE;
+      st.args.push_back(string_arg(n->children[0], v.c_str(), false));
+    } else if (v == "REPLACE") {
+      if (n->children.size() != 3) fail("REPLACE takes a string, a search string and a replacement");
+      st.op = SF_REPLACE;
+      st.args.push_back(string_arg(n->children[0], "REPLACE", false));
+      st.args.push_back(literal_arg(n->children[1].get(), "REPLACE"));
+      st.args.push_back(literal_arg(n->children[2].get(), "REPLACE"));
+      // cudf::strings::replace rejects an empty target
+      if (st.args[1].str.empty()) fail("Parameter target must not be empty string.");
+    } else if (v == "CAST_VARCHAR" || v == "CAST_CHAR") {
+      if (n->children.size() != 1) fail("malformed CAST");
+      if (n->children[0]->type == NodeType::Literal) fail("CAST operator not supported for literals");
+      const int32_t t = node_type(n->children[0].get(), types);
+      if (is_duration_type(t) || t == BSQL_EMPTY || (type_size(t) == 0 && !is_string_type(t)))
+        fail("CAST to VARCHAR is not supported for type " + std::to_string(t));
+      st.op = SF_CAST;
+      StepArg a;
+      a.col = eval_column(std::move(n->children[0]));
+      st.args.push_back(a);
+    } else if (v == "FIRST_NON_MAGIC" && n->children.size() == 2 && n->children[0]->type == NodeType::Operator &&
+               n->children[0]->value == "MAGIC_IF_NOT" && n->children[0]->children.size() == 2 &&
+               (is_string_type(node_type(n->children[0]->children[1].get(), types)) ||
+                is_string_type(node_type(n->children[1].get(), types)))) {
+      std::unique_ptr<Node>& cond = n->children[0]->children[0];
+      if (cond->type == NodeType::Literal) fail("CASE operator not supported for condition expression literals");
+      if (node_type(cond.get(), types) != BSQL_BOOL8) fail("Expression does not evaluate to a boolean mask");
+      st.op = SF_CASE;
+      StepArg c;
+      c.col = eval_column(std::move(cond));
+      st.args.push_back(c);
+      st.args.push_back(string_arg(n->children[0]->children[1], "CASE", true));
+      st.args.push_back(string_arg(n->children[1], "CASE", true));
+    } else {
+      return n;
+    }
+    return var_node(add_step(std::move(st)));
+  }
+};
+
 void check_interpretable(const Node* n) {
   if (n->type == NodeType::Operator && is_string_function(n->value))
     fail("string function " + n->value +
@@ -750,27 +882,24 @@ void check_interpretable(const Node* n) {
 
 }  // namespace
 
-std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exprs,
-                                             const int32_t* in_types, int n_in) {
-  std::vector<std::unique_ptr<Node>> trees;
-  std::vector<bool> used((size_t)std::max(n_in, 0), false);
-  for (const std::string& raw : exprs) {
-    std::string e = replace_calcite_regex(fill_minus_op_with_zero(raw));
-    e = expand_if_logical_op(e);
-    Parser parser(e);
-    std::unique_ptr<Node> t = rewrite_trim(rewrite_substring(transform_custom(parser.parse())));
+namespace {
+
+// One program over n_cols columns for already transformed trees (each tree
+// one output); the columns it reads become its inputs, in column order.
+std::unique_ptr<Program> compile_trees(std::vector<std::unique_ptr<Node>> trees, int n_cols) {
+  std::vector<bool> used((size_t)std::max(n_cols, 0), false);
+  for (const auto& t : trees) {
     check_interpretable(t.get());
     std::vector<int> vars;
     collect_vars(t.get(), vars);
     for (int v : vars) {
-      if (v < 0 || v >= n_in) fail("column $" + std::to_string(v) + " not in the table");
+      if (v < 0 || v >= n_cols) fail("column $" + std::to_string(v) + " not in the table");
       used[(size_t)v] = true;
     }
-    trees.push_back(std::move(t));
   }
   auto prog = std::make_unique<Program>();
   std::map<int, int> col_map;
-  for (int i = 0; i < n_in; ++i)
+  for (int i = 0; i < n_cols; ++i)
     if (used[(size_t)i]) {
       const int slot = (int)col_map.size();
       col_map[i] = slot;
@@ -779,6 +908,7 @@ std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exp
   const int n_inputs = (int)col_map.size();
   const int n_exprs = (int)trees.size();
   prog->n_inputs = n_inputs;
+  prog->n_table = n_cols;
   prog->root_literals.assign((size_t)n_exprs, Scalar());
   for (int i = 0; i < n_exprs; ++i) {
     const int final_pos = n_inputs + i;
@@ -808,7 +938,41 @@ std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exp
     if (root->type == NodeType::Variable) prog->final_out[(size_t)i] = (int16_t)col_map.at(root->var_index);
   }
   validate(*prog, n_inputs);
-  (void)in_types;
+  return prog;
+}
+
+}  // namespace
+
+std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exprs,
+                                             const int32_t* in_types, int n_in) {
+  auto holder = std::make_unique<Program>();
+  StepExtractor ex{*holder, std::vector<int32_t>(in_types, in_types + std::max(n_in, 0)), n_in};
+  std::vector<std::unique_ptr<Node>> trees;
+  for (const std::string& raw : exprs) {
+    std::string e = replace_calcite_regex(fill_minus_op_with_zero(raw));
+    e = expand_if_logical_op(e);
+    Parser parser(e);
+    std::unique_ptr<Node> t = rewrite_trim(rewrite_substring(transform_custom(parser.parse())));
+    std::vector<int> vars;
+    collect_vars(t.get(), vars);
+    for (int v : vars)
+      if (v < 0 || v >= n_in) fail("column $" + std::to_string(v) + " not in the table");
+    t = ex.visit(std::move(t));
+    // a string literal root fills its column (make_column_from_scalar, :822-832)
+    if (t->type == NodeType::Literal && !is_null_literal(t->value) && is_string_type(t->lit_type)) {
+      Step st;
+      st.op = SF_FILL;
+      StepArg a;
+      a.kind = StepArg::LITERAL;
+      a.str = scalar_from_string(t->value, BSQL_STRING).str;
+      st.args.push_back(a);
+      t = var_node(ex.add_step(std::move(st)));
+    }
+    trees.push_back(std::move(t));
+  }
+  std::unique_ptr<Program> prog = compile_trees(std::move(trees), (int)ex.types.size());
+  prog->steps = std::move(holder->steps);
+  prog->n_table = n_in;
   return prog;
 }
 

@@ -36,6 +36,27 @@ struct Scalar {
   std::string str;
 };
 
+struct Program;
+
+// A column built before the interpreter runs and appended after the table's
+// columns (the reference builds these with libcudf and appends them as $N+k,
+// LogicalProject.cpp:670-731). EVAL runs a one-output sub-program over the
+// columns so far (an argument of a string function that is an expression, or
+// a string CASE's condition); STRFN builds a string column (strfn.h's ops).
+struct StepArg {
+  enum Kind { COLUMN, LITERAL, NULL_LITERAL } kind = COLUMN;
+  int col = -1;     // COLUMN: index among the table's columns, then earlier steps'
+  std::string str;  // LITERAL
+};
+struct Step {
+  enum Kind { EVAL, STRFN } kind = STRFN;
+  std::shared_ptr<Program> sub;  // EVAL
+  int op = 0;                    // STRFN: StrFnOp
+  std::vector<StepArg> args;
+  int32_t out_type = BSQL_STRING;
+  std::shared_ptr<void> d_lits;  // device copy of the literal arguments, made on first use
+};
+
 struct Program {
   std::vector<int16_t> left, right, out, final_out;
   std::vector<int32_t> ops;
@@ -52,11 +73,18 @@ struct Program {
   std::vector<int32_t> conj_order;
   bsql_launch_config launch{0, 0};
   uint64_t seed = 0x5eed5eed12345678ULL;
+  // columns built before the launch: step k is column n_table + k
+  std::vector<Step> steps;
+  int n_table = -1;
   // device copies of string literals, one buffer per program
   void* d_strings = nullptr;
   std::vector<int64_t> str_offsets;  // per (op, side): offset into d_strings or -1
   ~Program();
 };
+
+// A program allocated as a C ABI handle (it carries the lowering cache), for
+// the sub-programs of steps.
+std::shared_ptr<Program> make_program_handle(std::unique_ptr<Program> p);
 
 // type helpers (cudf type_id semantics)
 bool is_float_type(int t);

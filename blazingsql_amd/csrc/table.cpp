@@ -211,9 +211,8 @@ TablePtr evaluate(const bsql_table& in, const std::vector<std::string>& exprs,
   const std::vector<int32_t> types = types_of(in);
   ProgramPtr prog = compile(exprs, types);
   std::vector<bsql_column> views = views_of(in);
-  int64_t chars_bound = 0;
-  for (const bsql_column& v : views)
-    if (v.type_id == BSQL_STRING) chars_bound = std::max<int64_t>(chars_bound, v.chars_size);
+  // a string output is a slice of one column: the table's or a step's
+  const int64_t chars_bound = string_chars_bound(*prog, views.data(), (int)views.size(), in.num_rows);
   TablePtr res = new_table(stream);
   res->num_rows = in.num_rows;
   std::vector<bsql_column> outs;

@@ -54,8 +54,10 @@ def unpack_validity(words, n):
 class Program:
     """An interpreter program (reference encoding), owned by libbsql."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, owner=None):
         self._h = ctypes.c_void_p(handle)
+        # a sub-program of a step belongs to its owner's handle
+        self._owner = owner
 
     @property
     def handle(self):
@@ -63,7 +65,7 @@ class Program:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
+        if h is not None and h.value and getattr(self, "_owner", None) is None:
             try:
                 L.lib().bsql_program_free(h)
             except Exception:
@@ -142,6 +144,42 @@ class Program:
         buf = (ctypes.c_int32 * 64)()
         n = L.lib().bsql_program_used_columns(self._h, buf, 64)
         return list(buf)[:n]
+
+    @property
+    def table_columns(self):
+        """Width of the table the program was compiled for."""
+        return L.lib().bsql_program_table_columns(self._h)
+
+    def steps(self):
+        """Columns built before the interpreter runs, in order (step k is column
+        table_columns + k): dicts with kind ("eval" with a one-output "sub"
+        program, or "strfn" with "op"), "out_type" and "args" (a column index,
+        or ("literal", text) / ("null",))."""
+        out = []
+        for k in range(L.lib().bsql_program_num_steps(self._h)):
+            info = L.StepInfo()
+            L.check(L.lib().bsql_program_step(self._h, k, ctypes.byref(info)))
+            args = []
+            for i in range(info.n_args):
+                c = info.arg_col[i]
+                if c >= 0:
+                    args.append(c)
+                elif c == -1:
+                    args.append(("literal", ctypes.string_at(info.arg_str[i], info.arg_len[i]).decode()))
+                else:
+                    args.append(("null",))
+            d = dict(kind="eval" if info.kind == 0 else "strfn", op=info.op, out_type=info.out_type, args=args)
+            if info.kind == 0:
+                d["sub"] = Program(info.sub, owner=self)
+            out.append(d)
+        return out
+
+    def output_chars(self, cols, num_rows):
+        """Chars a string output over these device columns can need at most."""
+        arr = (L.Column * max(len(cols), 1))(*[c.to_c() for c in cols])
+        v = ctypes.c_int64()
+        L.check(L.lib().bsql_program_output_chars(self._h, arr, len(cols), num_rows, ctypes.byref(v)))
+        return v.value
 
     @property
     def num_conjuncts(self):

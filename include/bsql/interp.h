@@ -179,6 +179,45 @@ int bsql_program_output_type(const bsql_program* prog, const int32_t* in_types,
  * col_idx_map of LogicalProject.cpp:859-881); fills up to cap entries. */
 int bsql_program_used_columns(const bsql_program* prog, int32_t* cols, int cap);
 
+/* ---- columns built before the interpreter -------------------------------------
+ * <=> function_evaluator_transformer + evaluate_string_functions +
+ * evaluate_string_case_when_else (engine/src/execution_kernels/
+ * LogicalProject.cpp:100-742): functions that build new strings are computed
+ * before the interpreter runs and appended to the table, step k of a compiled
+ * program being column n_table + k (n_table = the table it was compiled for;
+ * the launching calls take that table and build the steps themselves, on the
+ * call's stream). A step is an expression evaluated by the interpreter into a
+ * column (an argument of a string function, a string CASE's condition) or a
+ * string function over columns and literals. */
+enum {
+  BSQL_STRFN_CONCAT = 1,  /* a || b, null if either is null (cudf::strings::concatenate) */
+  BSQL_STRFN_UPPER = 2,   /* full case mapping (cudf::strings::to_upper / to_lower) */
+  BSQL_STRFN_LOWER = 3,
+  BSQL_STRFN_INITCAP = 4, /* cudf::strings::title */
+  BSQL_STRFN_REVERSE = 5, /* characters reversed (slice_strings with step -1) */
+  BSQL_STRFN_REPLACE = 6, /* every occurrence of arg 1 by arg 2 (cudf::strings::replace) */
+  BSQL_STRFN_CAST = 7,    /* CAST(x AS VARCHAR): from_integers/booleans/floats/timestamps */
+  BSQL_STRFN_CASE = 8,    /* arg 0 (BOOL8) true ? arg 1 : arg 2 (cudf::copy_if_else) */
+  BSQL_STRFN_FILL = 9     /* the literal in every row (make_column_from_scalar) */
+};
+typedef struct bsql_step_info {
+  int32_t kind;       /* 0: expression (sub, one output); 1: string function (op) */
+  int32_t op;         /* BSQL_STRFN_* */
+  int32_t out_type;   /* type id of the column built */
+  int32_t n_args;
+  int32_t arg_col[3]; /* column index, or -1 for a literal, -2 for a null literal */
+  const char* arg_str[3];
+  int32_t arg_len[3];
+  const bsql_program* sub; /* kind 0: compiled over columns [0, n_table + k) */
+} bsql_step_info;
+int bsql_program_num_steps(const bsql_program* prog);
+int bsql_program_table_columns(const bsql_program* prog);
+int bsql_program_step(const bsql_program* prog, int k, bsql_step_info* out);
+/* Chars a string output of `prog` over this table can need at most (size
+ * string output columns' chars with it). */
+int bsql_program_output_chars(const bsql_program* prog, const bsql_column* in, int n_in,
+                              int64_t num_rows, int64_t* chars);
+
 /* ---- execution ---------------------------------------------------------------
  * bsql_interp_execute  <=> interops::perform_interpreter_operation
  * (engine/src/Interpreter/interpreter_cpp.cu:440-594). `in` is the interpreter

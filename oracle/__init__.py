@@ -291,3 +291,29 @@ def filter_table(pred, in_cols, num_rows=None, threads=0, seed=0x5eed5eed1234567
         else:
             res.append((a[:n], valid))
     return n, res
+
+
+def run_steps(prog, cols, num_rows):
+    """The table's host columns followed by the columns a compiled program's
+    steps build (function_evaluator_transformer, LogicalProject.cpp:670-731):
+    expressions through oracle_execute, string functions through strfn.
+    `prog` is a compiled program exposing steps() (blazingsql_amd Program)."""
+    from . import strfn
+    cols = list(cols)
+    for st in prog.steps():
+        if st["kind"] == "eval":
+            sub = st["sub"]
+            (vals, valid), = execute(sub.to_oracle_dict(), [cols[c] for c in sub.used_columns()],
+                                     [st["out_type"]], num_rows)
+            if st["out_type"] == STRING:
+                cols.append(HostColumn(STRING, strings=vals, valid=valid))
+            else:
+                cols.append(HostColumn(st["out_type"], vals, valid))
+            continue
+        args = [a if isinstance(a, tuple) else strfn.column_values(cols[a]) for a in st["args"]]
+        if st["op"] == strfn.CAST:
+            args = [(cols[st["args"][0]].type_id, args[0])]
+        res = strfn.build(st["op"], args, num_rows)
+        cols.append(HostColumn(STRING, strings=[r if r is not None else "" for r in res],
+                               valid=[r is not None for r in res]))
+    return cols

@@ -159,6 +159,45 @@ vectors["string_substring_null"] = {
     "expected": ["B", "2", "", "Y"], "expected_valid": [1, 1, 0, 1],
 }
 
+# process_project.cpp:368-387 -- || with a literal and between columns
+vectors["string_concat"] = {
+    "source": "engine/tests/new-tests/process_project.cpp:368-387",
+    "col1": ["foo", "d", "e", "a", "hello", "k", "d", "l", "", ""],
+    "col2": ["The", "quick", "brown", "fox", "jumps", "over", "the", "lazy", "dog", ""],
+    "query": "LogicalProject(EXPR$0=[||($0,'XD=D')], EXPR$1=[||($0,$1)])",
+    "expected_col1": ["fooXD=D", "dXD=D", "eXD=D", "aXD=D", "helloXD=D", "kXD=D", "dXD=D", "lXD=D", "XD=D", "XD=D"],
+    "expected_col2": ["fooThe", "dquick", "ebrown", "afox", "hellojumps", "kover", "dthe", "llazy", "dog", ""],
+}
+
+# process_project.cpp:389-412 -- CAST to VARCHAR of BOOL8, INT32, FLOAT64, TIMESTAMP_SECONDS
+vectors["cast_to_string"] = {
+    "source": "engine/tests/new-tests/process_project.cpp:389-412",
+    "col1": [True, False, True, True, True, False, False, True, False, False],
+    "col2": [1, 5, 10, 15, 100, 500, 1000, 5000, 10000, 999999],
+    "col3": [1.0, 5.5, 10.00003, 15.45, 100.656, 500.756756, 0.45435436, 0.0000324, 0.1, 999999.001],
+    "col4": [0, 10, 2600, 89260, 579500, 6834000, 86796400, 135768000, 715360000, 1230720000],
+    "query": "LogicalProject(EXPR$0=[CAST($0):VARCHAR], EXPR$1=[CAST($1):VARCHAR], EXPR$2=[CAST($2):VARCHAR], "
+             "EXPR$3=[CAST($3):VARCHAR])",
+    "expected_col1": ["true", "false", "true", "true", "true", "false", "false", "true", "false", "false"],
+    "expected_col2": ["1", "5", "10", "15", "100", "500", "1000", "5000", "10000", "999999"],
+    "expected_col3": ["1.0", "5.5", "10.00003", "15.45", "100.656", "500.756756", "0.45435436", "3.24e-05", "0.1",
+                      "999999.001"],
+    "expected_col4": ["1970-01-01 00:00:00", "1970-01-01 00:00:10", "1970-01-01 00:43:20", "1970-01-02 00:47:40",
+                      "1970-01-07 16:58:20", "1970-03-21 02:20:00", "1972-10-01 14:06:40", "1974-04-21 09:20:00",
+                      "1992-09-01 15:06:40", "2008-12-31 10:40:00"],
+}
+
+# process_project.cpp:447-486 -- CASE with string values, and compared to a literal
+vectors["string_case"] = {
+    "source": "engine/tests/new-tests/process_project.cpp:447-486",
+    "col1": [0, 1, 2, 3, 4, 5, 6, 7, 8],
+    "col2": ["The", "quick", "brown", "fox", "jumps", "over", "the", "lazy", "dog"],
+    "query": "LogicalProject(EXPR$0=[CASE(=(MOD($0, 2), 0), $1, 'LOL')])",
+    "expected": ["The", "LOL", "brown", "LOL", "jumps", "LOL", "the", "LOL", "dog"],
+    "nested_query": "LogicalProject(EXPR$0=[=(CASE(=(MOD($0, 2), 0), $1, 'LOL'), 'LOL')])",
+    "nested_expected": [False, True, False, True, False, True, False, True, False],
+}
+
 # process_project.cpp:415-446 -- CAST from strings to INTEGER and DOUBLE (the
 # TIMESTAMP column of that test is a string-to-timestamp parse, not built here)
 vectors["cast_from_string"] = {

@@ -202,8 +202,9 @@ def _full_prog(bsql, exprs, types):
     p = bsql.Program.compile(exprs, types)
     d = p.to_oracle_dict()
     used = p.used_columns()
-    # remap input slots back to table columns; shift other slots past the table
-    n_in, n_tab = len(used), len(types)
+    # remap input slots back to table columns (the table's, then the columns
+    # its steps build); shift other slots past them
+    n_in, n_tab = len(used), len(types) + len(p.steps())
     def remap(s):
         if s < 0:
             return s

@@ -17,7 +17,7 @@ def _run(bsql, exprs, cols, types, n):
     from blazingsql_amd.interop import DeviceColumn, perform_interpreter_operation
     p = bsql.Program.compile(exprs, types)
     ot = [p.output_type(types, k) for k in range(p.num_outputs)]
-    chars = max([c.chars_size for c in cols if c.type_id == STRING] or [0])
+    chars = p.output_chars(cols, n)
     outs = [DeviceColumn.empty(t, n, chars=chars) for t in ot]
     perform_interpreter_operation(p, cols, outs, n)
     return p, ot, [o.to_numpy(n) for o in outs]
@@ -26,7 +26,8 @@ def _run(bsql, exprs, cols, types, n):
 def _oracle(bsql, p, host_cols, ot, n):
     import oracle
     used = p.used_columns()
-    return oracle.execute(p.to_oracle_dict(), [host_cols[c] for c in used], ot, n)
+    host = oracle.run_steps(p, host_cols, n)
+    return oracle.execute(p.to_oracle_dict(), [host[c] for c in used], ot, n)
 
 
 def _same(g, o, what):
@@ -255,3 +256,123 @@ def test_filter_on_trim_and_casts_vs_oracle(bsql, gpu, ranges):
         assert m == on and m > 0, pred_text
         for k in range(2):
             _same(outs[k].to_numpy(m), ores[k], f"{pred_text} col {k}")
+
+
+def test_golden_string_building(bsql, gpu, golden):
+    """process_project.cpp:368-486 on the GPU: || with a literal and between
+    columns, CAST to VARCHAR of BOOL8 / INT32 / FLOAT64 / TIMESTAMP_SECONDS,
+    string CASE and a comparison of it."""
+    from blazingsql_amd.interop import DeviceColumn
+    g = golden["string_concat"]
+    cols = [DeviceColumn.from_strings(g["col1"]), DeviceColumn.from_strings(g["col2"])]
+    _, ot, res = _run(bsql, ["||($0,'XD=D')", "||($0,$1)"], cols, [STRING, STRING], 10)
+    assert ot == [STRING, STRING]
+    assert res[0][0] == g["expected_col1"] and res[1][0] == g["expected_col2"] and res[0][1].all()
+    g = golden["cast_to_string"]
+    types = [BOOL8, INT32, FLOAT64, 13]
+    cols = [DeviceColumn.from_numpy(BOOL8, np.asarray(g["col1"], dtype=np.uint8)),
+            DeviceColumn.from_numpy(INT32, np.asarray(g["col2"], dtype=np.int32)),
+            DeviceColumn.from_numpy(FLOAT64, np.asarray(g["col3"])),
+            DeviceColumn.from_numpy(13, np.asarray(g["col4"], dtype=np.int64))]
+    _, ot, res = _run(bsql, [f"CAST(${k}):VARCHAR" for k in range(4)], cols, types, 10)
+    for k in range(4):
+        assert res[k][0] == g[f"expected_col{k + 1}"], k
+    g = golden["string_case"]
+    cols = [DeviceColumn.from_numpy(INT32, np.asarray(g["col1"], dtype=np.int32)),
+            DeviceColumn.from_strings(g["col2"])]
+    _, ot, res = _run(bsql, ["CASE(=(MOD($0, 2), 0), $1, 'LOL')", "=(CASE(=(MOD($0, 2), 0), $1, 'LOL'), 'LOL')"],
+                      cols, [INT32, STRING], 9)
+    assert ot == [STRING, BOOL8] and res[0][0] == g["expected"]
+    np.testing.assert_array_equal(res[1][0].astype(bool), g["nested_expected"])
+
+
+def _mixed_table(rng, n):
+    from blazingsql_amd.interop import DeviceColumn
+    import oracle
+    alphabet = list("abcXYZ ,.'-3") + ["é", "ß", "Σ", "ǅ", "ΐ", "中", "ŉ", "ı"]
+    s0, v0 = _string_table(rng, n, alphabet, 0, 10, 0.1)
+    s1, v1 = _string_table(rng, n, list("ab"), 0, 4, 0.1)
+    k = rng.integers(-50, 50, n).astype(np.int32)
+    kv = rng.random(n) >= 0.1
+    x = rng.normal(0, 1, n) * 10.0 ** rng.integers(-12, 12, n)
+    xv = rng.random(n) >= 0.1
+    b = (rng.random(n) < 0.5).astype(np.uint8)
+    ts = rng.integers(-2 * 10**12, 4 * 10**12, n).astype(np.int64)
+    types = [STRING, STRING, INT32, FLOAT64, BOOL8, 14]
+    dev = [DeviceColumn.from_strings(s0, v0), DeviceColumn.from_strings(s1, v1),
+           DeviceColumn.from_numpy(INT32, k, kv), DeviceColumn.from_numpy(FLOAT64, x, xv),
+           DeviceColumn.from_numpy(BOOL8, b), DeviceColumn.from_numpy(14, ts)]
+    host = [oracle.HostColumn(STRING, strings=s0, valid=v0), oracle.HostColumn(STRING, strings=s1, valid=v1),
+            oracle.HostColumn(INT32, k, kv), oracle.HostColumn(FLOAT64, x, xv), oracle.HostColumn(BOOL8, b),
+            oracle.HostColumn(14, ts)]
+    return types, dev, host
+
+
+STRING_BUILDERS = [
+    "UPPER($0)", "LOWER($0)", "INITCAP($0)", "REVERSE($0)", "REPLACE($0, 'ab', 'xYz')", "REPLACE($0, 'é', '')",
+    "||($0, $1)", "||('pre-', $1)", "||($0, '')", "CAST($2):VARCHAR", "CAST($3):VARCHAR", "CAST($4):VARCHAR",
+    "CAST($5):VARCHAR", "CAST(+($2, 1000000)):VARCHAR", "CAST(*($3, 2.5:DECIMAL(2, 1))):VARCHAR",
+    "CASE(>($2, 0), $0, $1)", "CASE(>($2, 0), 'pos', <($2, -25), 'low', null)", "CASE(IS NULL($0), $1, 'x')",
+    "UPPER(SUBSTRING($0, 2, 3))", "||(UPPER($0), CAST($2):VARCHAR)", "LIKE(UPPER($0), 'A%')",
+    "CHAR_LENGTH(||($0, $1))", "=(LOWER($0), $1)", "SUBSTRING(INITCAP($0), 2)", "'literal'",
+    "REVERSE(REPLACE(LOWER($0), 'a', 'aa'))",
+]
+
+
+@pytest.mark.parametrize("n", [1, 1000, 70_001])
+def test_string_building_vs_oracle(bsql, gpu, n):
+    """Every string-building function (and their nesting with interpreted ops
+    and numeric expressions) on random tables with nulls, multi-byte and
+    case-changing characters: GPU columns equal the oracle's."""
+    rng = np.random.default_rng(n + 21)
+    types, dev, host = _mixed_table(rng, n)
+    for e in STRING_BUILDERS:
+        p, ot, res = _run(bsql, [e], dev, types, n)
+        ores = _oracle(bsql, p, host, ot, n)
+        _same(res[0], ores[0], e)
+
+
+def test_filter_and_project_with_string_steps_vs_oracle(bsql, gpu):
+    """Predicates whose columns are built first (process_filter with the
+    gather of every table column), and fused filter+project where the
+    projection's steps follow the predicate's."""
+    import oracle
+    import torch
+    from blazingsql_amd.interop import DeviceColumn, filter_columns, filter_project
+    from tests.test_gpu_parity import _full_prog
+    n = 90_001
+    rng = np.random.default_rng(31)
+    types, dev, host = _mixed_table(rng, n)
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    for pred_text in ["LIKE(UPPER($0), 'A%')", "AND(=(CASE(>($2, 0), $1, 'b'), 'a'), >($3, 0))",
+                      "<(CHAR_LENGTH(||($0, $1)), 4)"]:
+        pred = bsql.Program.compile(pred_text, types)
+        outs = [DeviceColumn.empty(t, n, chars=c.chars_size if t == STRING else 0) for t, c in zip(types, dev)]
+        filter_columns(pred, dev, outs, cnt, n)
+        m = int(cnt.item())
+        host_ext = oracle.run_steps(pred, host, n)
+        on, ores = oracle.filter_table(_full_prog(bsql, pred_text, types), host_ext, n)
+        assert m == on and m > 0, pred_text
+        for k in range(len(types)):
+            _same(outs[k].to_numpy(m), ores[k], f"{pred_text} col {k}")
+    proj_exprs = ["||(UPPER($0), '!')", "CAST($3):VARCHAR", "+($2, 1)", "CASE(>($2, 10), $0, 'small')"]
+    proj = bsql.Program.compile(proj_exprs, types)
+    ot = [proj.output_type(types, k) for k in range(proj.num_outputs)]
+    chars = proj.output_chars(dev, n)
+    for pred_text in ["LIKE(UPPER($0), 'A%')", ">($2, 0)"]:
+        pred = bsql.Program.compile(pred_text, types)
+        outs = [DeviceColumn.empty(t, n, chars=chars) for t in ot]
+        filter_project(pred, proj, dev, outs, cnt, n)
+        m = int(cnt.item())
+        # oracle: the filter over the table, then the projection (with its
+        # steps) over the filtered rows
+        host_ext = oracle.run_steps(pred, host, n)
+        on, filtered = oracle.filter_table(_full_prog(bsql, pred_text, types), host_ext, n)
+        assert m == on and m > 0, pred_text
+        fcols = []
+        for t, (vals, valid) in zip(types, filtered[:len(types)]):
+            fcols.append(oracle.HostColumn(t, strings=vals, valid=valid) if t == STRING
+                         else oracle.HostColumn(t, vals, valid))
+        ores = _oracle(bsql, proj, fcols, ot, m)
+        for k in range(len(ot)):
+            _same(outs[k].to_numpy(m), ores[k], f"{pred_text} -> {proj_exprs[k]}")

@@ -461,3 +461,105 @@ def test_trim_closed_forms(oracle_lib, bsql):
             np.testing.assert_array_equal(ok, valid)
             got = [x for x, k in zip(v, ok) if k]
             assert got == [fn(s, chars or ws) for s, k in zip(strs, valid) if k], expr
+
+
+def _steps_and_execute(oracle_lib, p, cols, types, n):
+    host = oracle_lib.run_steps(p, cols, n)
+    ot = [p.output_type(types, k) for k in range(p.num_outputs)]
+    return ot, oracle_lib.execute(p.to_oracle_dict(), [host[c] for c in p.used_columns()], ot, n)
+
+
+def test_string_building_golden(oracle_lib, golden, bsql):
+    """||, CAST to VARCHAR and string CASE through the steps the compiler
+    extracts, against process_project.cpp:368-486."""
+    from oracle import HostColumn
+    g = golden["string_concat"]
+    p = bsql.Program.compile(["||($0,'XD=D')", "||($0,$1)"], [23, 23])
+    ot, res = _steps_and_execute(oracle_lib, p, [HostColumn(23, strings=g["col1"]), HostColumn(23, strings=g["col2"])],
+                                 [23, 23], 10)
+    assert ot == [23, 23] and res[0][0] == g["expected_col1"] and res[1][0] == g["expected_col2"]
+    g = golden["cast_to_string"]
+    types = [11, 3, 10, 13]
+    cols = [HostColumn(11, np.asarray(g["col1"], dtype=np.uint8)), HostColumn(3, g["col2"]), HostColumn(10, g["col3"]),
+            HostColumn(13, g["col4"])]
+    p = bsql.Program.compile([f"CAST(${k}):VARCHAR" for k in range(4)], types)
+    ot, res = _steps_and_execute(oracle_lib, p, cols, types, 10)
+    assert ot == [23] * 4
+    for k in range(4):
+        assert res[k][0] == g[f"expected_col{k + 1}"] and res[k][1].all(), k
+    g = golden["string_case"]
+    cols = [HostColumn(3, g["col1"]), HostColumn(23, strings=g["col2"])]
+    p = bsql.Program.compile(["CASE(=(MOD($0, 2), 0), $1, 'LOL')", "=(CASE(=(MOD($0, 2), 0), $1, 'LOL'), 'LOL')"],
+                             [3, 23])
+    ot, res = _steps_and_execute(oracle_lib, p, cols, [3, 23], 9)
+    assert ot == [23, 11] and res[0][0] == g["expected"]
+    np.testing.assert_array_equal(res[1][0].astype(bool), g["nested_expected"])
+
+
+def test_float_text_closed_form():
+    """from_floats as restated, against Python's correctly rounded formatting:
+    values with at most 8 significant digits print, inside [1e-4, 1e9], in
+    fixed notation with ten digits counting the integer part's, and outside
+    as d.ddddddddd e+XX, trailing zeros dropped and one decimal kept; a
+    dropped rounding step or a wrong rescale exponent fails this."""
+    from oracle.strfn import float_text
+    rng = np.random.default_rng(8)
+    for _ in range(3000):
+        digits = int(rng.integers(1, 9))
+        m = int(rng.integers(10 ** (digits - 1), 10 ** digits))
+        e = int(rng.integers(-40, 40))
+        v = float(f"{'-' if rng.random() < 0.3 else ''}{m}e{e}")
+        a = abs(v)
+        if 1e-4 <= a <= 1e9:
+            # ten digits counting the integer part's: 9 - (integer digits - 1) decimals
+            places = 10 - len(str(int(a)))
+            if -e > places:
+                continue  # the value itself needs rounding: halves are the algorithm's own
+            want = format(v, f".{places}f").rstrip("0")
+            if want.endswith("."):
+                want += "0"
+        else:
+            mant, ex = format(v, ".9e").split("e")
+            mant = mant.rstrip("0")
+            mant = mant + "0" if mant.endswith(".") else mant
+            want = f"{mant}e{'-' if int(ex) < 0 else '+'}{abs(int(ex)):02d}"
+            if mant.lstrip("-") == "1.0":
+                # rescaling by inexact powers can land on 10.0 instead of 1.0
+                # (the algorithm's own rounding): the same value, one exponent lower
+                q = int(ex) - 1
+                alt = f"{mant[:-3]}10.0e{'-' if q < 0 else '+'}{abs(q):02d}"
+                if float_text(v) == alt:
+                    continue
+        assert float_text(v) == want, (v, float_text(v), want)
+    assert [float_text(x) for x in (0.0, -0.0, float("nan"), float("inf"), float("-inf"))] == \
+        ["0.0", "-0.0", "NaN", "Inf", "-Inf"]
+
+
+def test_case_functions_closed_forms():
+    """UPPER / LOWER / INITCAP / REVERSE / REPLACE as restated, against
+    Python's whole-string methods on ASCII and Latin-1 letters (where both
+    definitions agree): a mapping applied to the wrong characters, a missed
+    word boundary or an overlapping replacement fails this."""
+    from oracle import strfn
+    rng = np.random.default_rng(9)
+    alphabet = list("abcXYZ 01,.'-_") + ["é", "Ö", "ñ", "ÿ"]
+    strs = ["".join(rng.choice(alphabet, int(rng.integers(0, 16)))) for _ in range(2000)]
+    assert strfn.build(strfn.UPPER, [strs], len(strs)) == [s.upper() for s in strs]
+    assert strfn.build(strfn.LOWER, [strs], len(strs)) == [s.lower() for s in strs]
+    assert strfn.build(strfn.INITCAP, [strs], len(strs)) == [s.title() for s in strs]
+    assert strfn.build(strfn.REVERSE, [strs], len(strs)) == ["".join(reversed(s)) for s in strs]
+    rep = strfn.build(strfn.REPLACE, [strs, ("literal", "aa"), ("literal", "b")], len(strs))
+    import re
+    assert rep == [re.sub("aa", "b", s) for s in strs]
+    # characters beyond the Basic Multilingual Plane are left unchanged
+    assert strfn.upper("a\U00010428") == "A\U00010428"
+
+
+def test_string_step_errors(bsql):
+    """The reference's own rejections (LogicalProject.cpp:132-377)."""
+    for expr, types, msg in [("REPLACE($0, '', 'x')", [23], "target"),
+                             ("||('a', 'b')", [23], "between literals"),
+                             ("UPPER('abc')", [23], "literals"),
+                             ("CASE(true, 'a', 'b')", [23], "condition")]:
+        with pytest.raises(Exception, match=msg):
+            bsql.Program.compile([expr], types)
