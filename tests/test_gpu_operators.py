@@ -138,6 +138,37 @@ def test_process_project_golden(bsql, gpu, golden):
     assert list(e1) == g["expected_col1"] and list(e2) == g["expected_col2"] and w1.all() and w2.all()
 
 
+def test_process_project_string_building_golden(bsql, gpu, golden):
+    """process_project with the reference's string-building queries
+    (process_project.cpp:368-486): the owned output tables are sized by the
+    library, including chars longer than any input column's."""
+    from blazingsql_amd.interop import DeviceColumn
+    from blazingsql_amd.processor import process_filter, process_project
+    g = golden["string_concat"]
+    out = process_project(_table([_strings(g["col1"]), _strings(g["col2"])], ["a", "b"]), g["query"])
+    (c1, v1), (c2, v2) = out.to_numpy()
+    assert out.names == ["EXPR$0", "EXPR$1"] and c1 == g["expected_col1"] and c2 == g["expected_col2"]
+    assert v1.all() and v2.all()
+    g = golden["cast_to_string"]
+    cols = [DeviceColumn.from_numpy(BOOL8, np.asarray(g["col1"], dtype=np.uint8)),
+            DeviceColumn.from_numpy(INT32, np.asarray(g["col2"], dtype=np.int32)),
+            DeviceColumn.from_numpy(FLOAT64, np.asarray(g["col3"])),
+            DeviceColumn.from_numpy(13, np.asarray(g["col4"], dtype=np.int64))]
+    out = process_project(_table(cols, ["a", "b", "c", "d"]), g["query"])
+    res = out.to_numpy()
+    for k in range(4):
+        assert res[k][0] == g[f"expected_col{k + 1}"], k
+    g = golden["string_case"]
+    tab = lambda: _table([DeviceColumn.from_numpy(INT32, np.asarray(g["col1"], dtype=np.int32)),
+                          _strings(g["col2"])], ["a", "b"])
+    (c, v), = process_project(tab(), g["query"]).to_numpy()
+    assert c == g["expected"]
+    # a filter on a built column keeps the table's own columns
+    f = process_filter(tab(), "LogicalFilter(condition=[=(CASE(=(MOD($0, 2), 0), $1, 'LOL'), 'LOL')])")
+    (a, _), (b, _) = f.to_numpy()
+    assert list(a) == [1, 3, 5, 7] and b == ["quick", "fox", "over", "lazy"]
+
+
 def test_process_project_bypass(bsql, gpu):
     """bypassingProject: $0..$n-1 in order moves the table through (renamed)."""
     from blazingsql_amd.interop import DeviceColumn
