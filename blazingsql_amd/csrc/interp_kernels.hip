@@ -1194,7 +1194,10 @@ __global__ __launch_bounds__(BLOCK) void slice_ids(SelArgs sa, uint32_t* __restr
   constexpr int CHUNK = 64 * R;
   constexpr int SLICE = S * CHUNK;
   __shared__ uint32_t s_off[BLOCK];
+  __shared__ uint32_t s_cnt[BLOCK];
   __shared__ uint32_t s_part[WAVES];
+  // a block that selected nothing has no ids to write (workgroup-uniform)
+  if (gp<uint32_t>(sa.block_sums)[blockIdx.x] == 0) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t below = lane ? (~0ULL >> (64 - lane)) : 0ULL;
   GLOBAL uint32_t* ids = gpw<uint32_t>(idx);
@@ -1216,11 +1219,13 @@ __global__ __launch_bounds__(BLOCK) void slice_ids(SelArgs sa, uint32_t* __restr
     uint32_t before = carry;
     for (int w = 0; w < wave; ++w) before += s_part[w];
     s_off[threadIdx.x] = before + incl - v;
+    s_cnt[threadIdx.x] = v;
     uint32_t all = 0;
     for (int w = 0; w < WAVES; ++w) all += s_part[w];
     __syncthreads();
     const int64_t c1 = c0 + BLOCK < s1 ? c0 + BLOCK : s1;
     for (int64_t slice = c0 + wave; slice < c1; slice += WAVES) {
+      if (s_cnt[slice - c0] == 0) continue;  // wave-uniform: no rows, no ballots to read
       const uint64_t mine = lane < S * R ? gp<uint64_t>(sa.bits)[slice * (S * R) + lane] : 0ULL;
       uint32_t off = s_off[slice - c0];
       const uint32_t row0 = (uint32_t)(slice * SLICE) + (uint32_t)(lane * R);

@@ -391,3 +391,31 @@ def test_literal_roots(bsql, gpu):
         assert int(cnt.item()) == expect
         if expect:
             np.testing.assert_array_equal(out[0].to_numpy(n)[0], x)
+
+
+def test_alternating_programs_on_one_stream(bsql, gpu):
+    """The launch plan copied to the stream's scratch is reused when the same
+    program runs over the same columns again; alternating programs, input
+    columns and row counts on one stream must each get their own plan."""
+    import torch
+    from blazingsql_amd.interop import DeviceColumn, filter_columns, perform_interpreter_operation
+    rng = np.random.default_rng(77)
+    n = 50_003
+    a = rng.integers(-100, 100, n).astype(np.int64)
+    b = rng.integers(-100, 100, n).astype(np.int64)
+    da, db = DeviceColumn.from_numpy(INT64, a), DeviceColumn.from_numpy(INT64, b)
+    p1 = bsql.Program.compile(["+($0, 1)"], [INT64])
+    p2 = bsql.Program.compile(["*($0, 3)"], [INT64])
+    pred = bsql.Program.compile(">($0, 0)", [INT64])
+    out = DeviceColumn.empty(INT64, n)
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    for prog, col, host, rows, fn in [(p1, da, a, n, lambda x: x + 1), (p2, da, a, n, lambda x: x * 3),
+                                      (p1, da, a, n, lambda x: x + 1), (p1, db, b, n, lambda x: x + 1),
+                                      (p1, da, a, n // 2, lambda x: x + 1), (p1, da, a, n, lambda x: x + 1)]:
+        perform_interpreter_operation(prog, [col], [out], rows)
+        v, _ = out.to_numpy(rows)
+        np.testing.assert_array_equal(v, fn(host[:rows]))
+        fo = DeviceColumn.empty(INT64, n, nullable=False)
+        filter_columns(pred, [col], [fo], cnt, rows)
+        m = int(cnt.item())
+        np.testing.assert_array_equal(fo.to_numpy(m)[0], host[:rows][host[:rows] > 0])
