@@ -124,6 +124,11 @@ hipError_t ensure_strings(Program& p) {
         blob.insert(blob.end(), (const char*)units.data(), (const char*)(units.data() + units.size()));
         continue;
       }
+      if (side == 1 && (p.ops[i] == BSQL_OP_TO_DATE || p.ops[i] == BSQL_OP_TO_TIMESTAMP)) {
+        const std::vector<int32_t> items = timestamp_format_compile(s.str);
+        blob.insert(blob.end(), (const char*)items.data(), (const char*)(items.data() + items.size()));
+        continue;
+      }
       int32_t len = (int32_t)s.str.size();
       blob.insert(blob.end(), (const char*)&len, (const char*)&len + 4);
       blob.insert(blob.end(), s.str.begin(), s.str.end());

@@ -43,6 +43,10 @@ enum DevCode : uint8_t {
   // characters found in the literal at imm ({int32 len, bytes}; empty: those
   // <= ' '); STR_TO_I64 / STR_TO_F64: the slice parsed as a number
   D_TRIM, D_STR_TO_I64, D_STR_TO_F64,
+  // STR_TO_TS: the slice parsed as a timestamp of type aux2 by the format
+  // items at imm ({n, (spec << 8 | width)...}; 0: "%Y-%m-%d" for days,
+  // "%Y-%m-%d %H:%M:%S" otherwise)
+  D_STR_TO_TS,
   D_NUM_CODES
 };
 

@@ -462,6 +462,64 @@ __device__ double str_to_f64(const GLOBAL uint8_t* s, int32_t n) {
   return base;
 }
 
+// to_timestamps' default formats as compiled items ((spec << 8) | width,
+// spec 0 for literal characters): "%Y-%m-%d" and "%Y-%m-%d %H:%M:%S".
+__constant__ int32_t TS_FMT_DATE[6] = {5, ('Y' << 8) | 4, 1, ('m' << 8) | 2, 1, ('d' << 8) | 2};
+__constant__ int32_t TS_FMT_DATETIME[12] = {11, ('Y' << 8) | 4, 1, ('m' << 8) | 2, 1, ('d' << 8) | 2, 1,
+                                            ('H' << 8) | 2, 1, ('M' << 8) | 2, 1, ('S' << 8) | 2};
+
+// Days since 1970-01-01 of a proleptic Gregorian date (Hinnant's days_from_civil).
+__device__ int64_t days_from_ymd(int64_t y, int64_t m, int64_t d) {
+  y -= m <= 2 ? 1 : 0;
+  const int64_t era = (y >= 0 ? y : y - 399) / 400;
+  const int64_t yoe = y - era * 400;
+  const int64_t doy = (153 * (m > 2 ? m - 3 : m + 9) + 2) / 5 + d - 1;
+  const int64_t doe = yoe * 365 + yoe / 4 - yoe / 100 + doy;
+  return era * 146097 + doe - 719468;
+}
+
+// cudf::strings::to_timestamps (libcudf 21.08): each field of the format is
+// read from a fixed number of characters (decimal digits up to the first
+// other character), literal characters are stepped over, fields the string
+// does not reach keep 1970-01-01 00:00:00; the result is the date in days or
+// the date and time in the ticks of `type`.
+__device__ int64_t str_to_ticks(const GLOBAL uint8_t* s, int32_t n, const int32_t* fmt, int type) {
+  int64_t y = 1970, mo = 1, d = 1, h = 0, mi = 0, se = 0;
+  const int32_t n_items = fmt[0];
+  int32_t pos = 0;
+  for (int32_t k = 1; k <= n_items && pos < n; ++k) {
+    const int32_t it = fmt[k];
+    const int32_t spec = it >> 8, width = it & 0xFF;
+    if (spec) {
+      const int32_t take = width < n - pos ? width : n - pos;
+      int64_t v = 0;
+      for (int32_t i = 0; i < take; ++i) {
+        const uint8_t c = s[pos + i];
+        if (c < '0' || c > '9') break;
+        v = v * 10 + (c - '0');
+      }
+      switch (spec) {
+        case 'Y': y = v; break;
+        case 'm': mo = v; break;
+        case 'd': d = v; break;
+        case 'H': h = v; break;
+        case 'M': mi = v; break;
+        default: se = v; break;
+      }
+    }
+    pos += width;
+  }
+  const int64_t days = days_from_ymd(y, mo, d);
+  if (type == BSQL_TIMESTAMP_DAYS) return days;
+  const uint64_t secs = (uint64_t)days * 86400ULL + (uint64_t)(h * 3600 + mi * 60 + se);
+  switch (type) {
+    case BSQL_TIMESTAMP_MILLISECONDS: return (int64_t)(secs * 1000ULL);
+    case BSQL_TIMESTAMP_MICROSECONDS: return (int64_t)(secs * 1000000ULL);
+    case BSQL_TIMESTAMP_NANOSECONDS: return (int64_t)(secs * 1000000000ULL);
+    default: return (int64_t)secs;
+  }
+}
+
 // SQL LIKE through the reference's translation to a regex searched with
 // contains_re (LogicalProject.cpp:31-47, :113-131): '%' = (?:.*?) and '_' =
 // (?:.), neither matching '\n'; anchored at the start unless the pattern
@@ -702,6 +760,22 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
           int32_t b0, b1;
           strip_range(p, n, set, nset, op.aux2, &b0, &b1);
           D[k] = slice_pack((uint32_t)A[k] + (uint32_t)b0, (uint32_t)(b1 - b0));
+          VD |= 1u << k;
+        }
+      }
+      return;
+    }
+    case D_STR_TO_TS: {
+      const int32_t* fmt = op.imm ? (const int32_t*)op.imm
+                                  : (op.aux2 == BSQL_TIMESTAMP_DAYS ? TS_FMT_DATE : TS_FMT_DATETIME);
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        D[k] = 0;
+        if (((rs.exist >> k) & 1u) && ((VA >> k) & 1u)) {
+          const GLOBAL uint8_t* p;
+          int32_t n;
+          str_at(plan, op.aux, A[k], &p, &n);
+          D[k] = (uint64_t)str_to_ticks(p, n, fmt, op.aux2);
           VD |= 1u << k;
         }
       }

@@ -697,10 +697,11 @@ void collect_vars(const Node* n, std::vector<int>& acc) {
 
 // String functions this build does not evaluate (the reference computes them
 // with libcudf before the interpreter, LogicalProject.cpp:100-742). LIKE,
-// SUBSTRING, LEFT, RIGHT and TRIM run inside the interpreter on string
-// slices; the functions that build new strings become steps (below).
+// SUBSTRING, LEFT, RIGHT, TRIM and the string-to-number/timestamp parses
+// run inside the interpreter on string slices; the functions that build new
+// strings become steps (below).
 bool is_string_function(const std::string& v) {
-  static const char* names[] = {"REGEXP_REPLACE", "TO_DATE", "TO_TIMESTAMP", "TRIM"};
+  static const char* names[] = {"REGEXP_REPLACE", "TRIM"};
   // TRIM with its side flag is interpreted (BLZ_TRIM_*); a bare TRIM is not
   for (const char* s : names)
     if (v == s) return true;

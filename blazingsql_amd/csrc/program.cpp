@@ -158,6 +158,9 @@ void validate(const Program& p, int n_in) {
       if (l < 0 && r < 0) fail("Operations between literals is not supported");
       if (op == BSQL_OP_STR_LIKE && r == BSQL_SCALAR_INDEX && p.rsc[i].type_id == BSQL_STRING)
         (void)like_compile(p.rsc[i].str);  // rejects patterns the matcher does not take
+      if ((op == BSQL_OP_TO_DATE || op == BSQL_OP_TO_TIMESTAMP) && r == BSQL_SCALAR_INDEX &&
+          p.rsc[i].type_id == BSQL_STRING)
+        (void)timestamp_format_compile(p.rsc[i].str);
       if (l < BSQL_SCALAR_NULL_INDEX || r < BSQL_SCALAR_NULL_INDEX) fail("invalid operand index");
     }
   }
@@ -202,6 +205,40 @@ std::string like_pass(const std::string& in, char w, char mark) {
 // anchored at the start unless the pattern's first character is '%' and at
 // the end unless its last is. Characters that would be regex syntax in the
 // reference (backslash included) are rejected rather than given a meaning.
+// The format language of cudf::strings::to_timestamps (libcudf 21.08) that
+// the reference's casts and TO_DATE / TO_TIMESTAMP use
+// (LogicalProject.cpp:469-533): %Y (4 digits), %m, %d, %H, %M, %S (2 each)
+// read as fixed-width fields, %% a literal '%', any other character a
+// literal the parser steps over; other specifiers are rejected.
+std::vector<int32_t> timestamp_format_compile(const std::string& f) {
+  std::vector<int32_t> items;
+  int32_t lit = 0;
+  auto flush = [&] {
+    while (lit > 0) {
+      const int32_t w = lit > 255 ? 255 : lit;
+      items.push_back(w);
+      lit -= w;
+    }
+  };
+  for (size_t i = 0; i < f.size(); ++i) {
+    if (f[i] != '%') { ++lit; continue; }
+    if (i + 1 >= f.size()) fail("timestamp format ends in '%'");
+    const char c = f[++i];
+    if (c == '%') { ++lit; continue; }
+    int32_t width = 0;
+    switch (c) {
+      case 'Y': width = 4; break;
+      case 'm': case 'd': case 'H': case 'M': case 'S': width = 2; break;
+      default: fail(std::string("timestamp format specifier %") + c + " is not supported");
+    }
+    flush();
+    items.push_back(((int32_t)c << 8) | width);
+  }
+  flush();
+  items.insert(items.begin(), (int32_t)items.size());
+  return items;
+}
+
 std::vector<int32_t> like_compile(const std::string& pat) {
   enum { ANCHOR_START = 1, ANCHOR_END = 2, MATCH_ALL = 4 };
   const char ANY_RUN = '\x01', ANY_CHAR = '\x02';
@@ -403,6 +440,18 @@ SourceOp translate(Lowering& L, const Program& p, const OpTypes& t, size_t i,
         }
         break;
       }
+      case BSQL_OP_TO_DATE: case BSQL_OP_TO_TIMESTAMP: {
+        // to_timestamps(column, type, format), LogicalProject.cpp:510-533
+        if (!is_string_type(lt) || l < 0) fail(std::string(op == BSQL_OP_TO_DATE ? "TO_DATE" : "TO_TIMESTAMP") +
+                                               " first argument must be a column of type string");
+        if (r != BSQL_SCALAR_INDEX || !is_string_type(rt))
+          fail("TO_DATE / TO_TIMESTAMP take a string literal format");
+        d.code = D_STR_TO_TS;
+        d.aux = (uint8_t)str_col_of(L, s.va);
+        d.aux2 = (uint8_t)(op == BSQL_OP_TO_DATE ? BSQL_TIMESTAMP_DAYS : BSQL_TIMESTAMP_NANOSECONDS);
+        d.imm = str_base_b;  // the format, compiled when the literals are uploaded
+        break;
+      }
       case BSQL_OP_STR_TRIM: {
         // the characters to strip and the side come in the right scalar
         if (!is_string_type(lt) || l < 0) fail("TRIM needs a string column or expression");
@@ -418,10 +467,10 @@ SourceOp translate(Lowering& L, const Program& p, const OpTypes& t, size_t i,
       default: fail("operator " + std::to_string(op) + " is not supported by the interpreter");
     }
     if (is_string_type(lt) && !((d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE ||
-                                d.code == D_SLICE || d.code == D_TRIM))
+                                d.code == D_SLICE || d.code == D_TRIM || d.code == D_STR_TO_TS))
       fail("string operand to a non-string operator");
     if (is_string_type(rt) && !((d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE ||
-                                d.code == D_TRIM))
+                                d.code == D_TRIM || d.code == D_STR_TO_TS))
       fail("string operand to a non-string operator");
   } else if (r == BSQL_UNARY_INDEX) {
     s.va = operand(l);
@@ -459,17 +508,27 @@ SourceOp translate(Lowering& L, const Program& p, const OpTypes& t, size_t i,
         d.code = is_string_type(lt) ? D_STR_TO_F64 : D_TO_F64;
         if (is_string_type(lt)) d.aux = (uint8_t)str_col_of(L, s.va);
         break;
-      case BSQL_OP_CAST_DATE: d.code = D_CAST_DATE; d.aux2 = (uint8_t)lt; break;
-      case BSQL_OP_CAST_TIMESTAMP: case BSQL_OP_CAST_TIMESTAMP_SECONDS:
+      // a string operand is parsed (to_timestamps with "%Y-%m-%d" or
+      // "%Y-%m-%d %H:%M:%S", LogicalProject.cpp:469-508)
+      case BSQL_OP_CAST_DATE: case BSQL_OP_CAST_TIMESTAMP: case BSQL_OP_CAST_TIMESTAMP_SECONDS:
       case BSQL_OP_CAST_TIMESTAMP_MILLISECONDS: case BSQL_OP_CAST_TIMESTAMP_MICROSECONDS:
-        d.code = D_CAST_TS; d.aux2 = (uint8_t)lt; break;
+        if (is_string_type(lt)) {
+          d.code = D_STR_TO_TS;
+          d.aux = (uint8_t)str_col_of(L, s.va);
+          d.aux2 = (uint8_t)output_type_unary(op, lt);
+          d.imm = 0;
+        } else {
+          d.code = op == BSQL_OP_CAST_DATE ? D_CAST_DATE : D_CAST_TS;
+          d.aux2 = (uint8_t)lt;
+        }
+        break;
       case BSQL_OP_CHAR_LENGTH:
         if (!is_string_type(lt)) fail("CHAR_LENGTH needs a string operand");
         d.code = D_CHAR_LENGTH; d.aux = (uint8_t)str_col_of(L, s.va); break;
       default: fail("operator " + std::to_string(op) + " is not supported by the interpreter");
     }
     if (is_string_type(lt) && d.code != D_CHAR_LENGTH && d.code != D_IS_NULL &&
-        d.code != D_IS_NOT_NULL && d.code != D_STR_TO_I64 && d.code != D_STR_TO_F64)
+        d.code != D_IS_NOT_NULL && d.code != D_STR_TO_I64 && d.code != D_STR_TO_F64 && d.code != D_STR_TO_TS)
       fail("string operand to a non-string operator");
   } else {
     d.code = D_RAND;

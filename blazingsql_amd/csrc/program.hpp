@@ -112,6 +112,10 @@ OpTypes propagate_types(const Program& p, const int32_t* in_types, int n_in);
 // LIKE pattern compiled for the device matcher: {n_units, flags, units...}.
 std::vector<int32_t> like_compile(const std::string& pattern);
 
+// to_timestamps format compiled for the device parser: {n_items, items...},
+// an item (spec << 8) | width with spec 0 for literal characters.
+std::vector<int32_t> timestamp_format_compile(const std::string& format);
+
 // Types of the interpreter input slots of `p` for a table schema: programs
 // compiled from text address their used columns, array-built ones the table.
 std::vector<int32_t> slot_types_for(const Program& p, const int32_t* table_types, int n_table);

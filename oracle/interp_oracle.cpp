@@ -303,6 +303,8 @@ int out_type_binary(int op, int lt, int rt) {  // expression_utils.cpp:201-266
     case BSQL_OP_FIRST_NON_MAGIC: return type_size(lt) >= type_size(rt) ? lt : rt;
     case BSQL_OP_STR_SUBSTRING: case BSQL_OP_STR_LEFT: case BSQL_OP_STR_RIGHT: case BSQL_OP_STR_TRIM:
       return BSQL_STRING;
+    case BSQL_OP_TO_DATE: return BSQL_TIMESTAMP_DAYS;
+    case BSQL_OP_TO_TIMESTAMP: return BSQL_TIMESTAMP_NANOSECONDS;
   }
   return BSQL_EMPTY;  // IS_NOT_DISTINCT_FROM and others fall to the default branch
 }
@@ -517,6 +519,54 @@ int64_t parse_int(StrView s) {
   return (int64_t)(v * sign);
 }
 
+// cudf::strings::to_timestamps (libcudf 21.08), which LogicalProject.cpp:469-533
+// calls for CAST(varchar AS DATE/TIMESTAMP) ("%Y-%m-%d" / "%Y-%m-%d %H:%M:%S")
+// and TO_DATE / TO_TIMESTAMP (the given format): the format is walked left to
+// right; %Y takes the next 4 characters, %m %d %H %M %S the next 2, each read
+// as the decimal digits before the first non-digit among them; %% and any
+// other character skip one character of the string; parsing stops where the
+// string ends, the fields not reached staying 1970-01-01 00:00:00. The date
+// is counted in days from 1970-01-01 on the proleptic Gregorian calendar;
+// sub-day types add the time of day, in their ticks.
+int64_t parse_timestamp(StrView s, const std::string& format, int type) {
+  long y = 1970, mo = 1, d = 1, h = 0, mi = 0, se = 0;
+  size_t at = 0;
+  const size_t n = (size_t)s.n;
+  for (size_t f = 0; f < format.size() && at < n; ++f) {
+    char spec = 0;
+    if (format[f] == '%' && f + 1 < format.size() && format[f + 1] != '%') spec = format[++f];
+    else if (format[f] == '%') ++f;  // "%%" is one literal character
+    if (!spec) { ++at; continue; }
+    const size_t width = spec == 'Y' ? 4 : 2;
+    long v = 0;
+    for (size_t k = at; k < std::min(n, at + width); ++k) {
+      const char c = s.p[k];
+      if (!std::isdigit((unsigned char)c)) break;
+      v = v * 10 + (c - '0');
+    }
+    if (spec == 'Y') y = v;
+    else if (spec == 'm') mo = v;
+    else if (spec == 'd') d = v;
+    else if (spec == 'H') h = v;
+    else if (spec == 'M') mi = v;
+    else se = v;
+    at += width;
+  }
+  // days from civil: shift the year to start in March, count 400-year eras
+  const long yy = mo <= 2 ? y - 1 : y;
+  const long era = (yy >= 0 ? yy : yy - 399) / 400;
+  const long yoe = yy - era * 400;
+  const long mp = mo > 2 ? mo - 3 : mo + 9;
+  const long doe = yoe * 365 + yoe / 4 - yoe / 100 + (153 * mp + 2) / 5 + d - 1;
+  const int64_t days = (int64_t)(era * 146097 + doe - 719468);
+  if (type == BSQL_TIMESTAMP_DAYS) return days;
+  const uint64_t secs = (uint64_t)days * 86400u + (uint64_t)(h * 3600 + mi * 60 + se);
+  const uint64_t per = type == BSQL_TIMESTAMP_MILLISECONDS ? 1000u
+                       : type == BSQL_TIMESTAMP_MICROSECONDS ? 1000000u
+                       : type == BSQL_TIMESTAMP_NANOSECONDS ? 1000000000u : 1u;
+  return (int64_t)(secs * per);
+}
+
 // The double nearest 10^e, from the C library's correctly rounded strtod.
 double pow10_nearest(int e) {
   static const std::vector<double> table = [] {
@@ -691,6 +741,12 @@ void eval_op(const Exec& ex, size_t i, int64_t row, RowCtx& c) {
     } else if (op == BSQL_OP_STR_TRIM) {
       c.str[opos] = lv ? strip_chars(ls, rs, (int)p.rsc[i].v.i64) : std::string();
       setv(c, opos, lv);
+    } else if (op == BSQL_OP_TO_DATE || op == BSQL_OP_TO_TIMESTAMP) {
+      // to_timestamps(column, type, format), LogicalProject.cpp:510-533
+      const std::string fmt(rs.p, (size_t)rs.n);
+      store_i(c, opos, lv ? parse_timestamp(ls, fmt, op == BSQL_OP_TO_DATE ? BSQL_TIMESTAMP_DAYS
+                                                                           : BSQL_TIMESTAMP_NANOSECONDS) : 0);
+      setv(c, opos, lv);
     } else if (op == BSQL_OP_IS_NOT_DISTINCT_FROM) {
       bool res;
       if (!lv && !rv) res = true;
@@ -800,6 +856,10 @@ void eval_op(const Exec& ex, size_t i, int64_t row, RowCtx& c) {
           store_f(c, opos, is_string_type(lt) ? parse_double(ls) : x);  // :439-468
           break;
         case BSQL_OP_CAST_DATE: {
+          if (is_string_type(lt)) {  // LogicalProject.cpp:469-482
+            store_i(c, opos, parse_timestamp(ls, "%Y-%m-%d", BSQL_TIMESTAMP_DAYS));
+            break;
+          }
           int64_t v = lf ? f64_to_i64(L.d) : L.i;
           if (is_timestamp_type(lt) && lt != BSQL_TIMESTAMP_DAYS) v = v / ticks_per_day(lt);  // truncation
           store_i(c, opos, v);
@@ -807,6 +867,14 @@ void eval_op(const Exec& ex, size_t i, int64_t row, RowCtx& c) {
         }
         case BSQL_OP_CAST_TIMESTAMP: case BSQL_OP_CAST_TIMESTAMP_SECONDS:
         case BSQL_OP_CAST_TIMESTAMP_MILLISECONDS: case BSQL_OP_CAST_TIMESTAMP_MICROSECONDS: {
+          if (is_string_type(lt)) {  // :484-508
+            const int to = op == BSQL_OP_CAST_TIMESTAMP_SECONDS ? BSQL_TIMESTAMP_SECONDS
+                           : op == BSQL_OP_CAST_TIMESTAMP_MILLISECONDS ? BSQL_TIMESTAMP_MILLISECONDS
+                           : op == BSQL_OP_CAST_TIMESTAMP_MICROSECONDS ? BSQL_TIMESTAMP_MICROSECONDS
+                                                                       : BSQL_TIMESTAMP_NANOSECONDS;
+            store_i(c, opos, parse_timestamp(ls, "%Y-%m-%d %H:%M:%S", to));
+            break;
+          }
           int64_t v = lf ? f64_to_i64(L.d) : L.i;
           if (is_timestamp_type(lt)) v = wrap_mul(v, ns_per_tick(lt));
           store_i(c, opos, v);

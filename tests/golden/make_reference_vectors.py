@@ -198,14 +198,18 @@ vectors["string_case"] = {
     "nested_expected": [False, True, False, True, False, True, False, True, False],
 }
 
-# process_project.cpp:415-446 -- CAST from strings to INTEGER and DOUBLE (the
-# TIMESTAMP column of that test is a string-to-timestamp parse, not built here)
+# process_project.cpp:415-446 -- CAST from strings to INTEGER, DOUBLE and TIMESTAMP
 vectors["cast_from_string"] = {
     "source": "engine/tests/new-tests/process_project.cpp:415-446",
     "col1": ["1", "5", "10", "15", "100", "500", "1000", "5000", "10000", "999999"],
     "col2": ["1.0", "5.5", "10.00003", "15.45", "100.656", "500.756756", "0.45435436", "3.24e-05", "0.1",
              "999999.001"],
-    "query": "LogicalProject(EXPR$0=[CAST($0):INTEGER], EXPR$1=[CAST($1):DOUBLE])",
+    "col3": ["1970-01-01 00:00:00", "1970-01-01 00:00:10", "1970-01-01 00:43:20", "1970-01-02 00:47:40",
+             "1970-01-07 16:58:20", "1970-03-21 02:20:00", "1972-10-01 14:06:40", "1974-04-21 09:20:00",
+             "1992-09-01 15:06:40", "2008-12-31 10:40:00"],
+    "query": "LogicalProject(EXPR$0=[CAST($0):INTEGER], EXPR$1=[CAST($1):DOUBLE], EXPR$2=[CAST($2):TIMESTAMP])",
+    "expected_col3": [0, 10000000000, 2600000000000, 89260000000000, 579500000000000, 6834000000000000,
+                      86796400000000000, 135768000000000000, 715360000000000000, 1230720000000000000],
     "expected_col1": [1, 5, 10, 15, 100, 500, 1000, 5000, 10000, 999999],
     "expected_col2": [1.0, 5.5, 10.00003, 15.45, 100.656, 500.756756, 0.45435436, 0.0000324, 0.1, 999999.001],
     "compare_col2": "expect_tables_equivalent: floating point within cudf's ULP tolerance",

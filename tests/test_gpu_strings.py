@@ -376,3 +376,38 @@ def test_filter_and_project_with_string_steps_vs_oracle(bsql, gpu):
         ores = _oracle(bsql, proj, fcols, ot, m)
         for k in range(len(ot)):
             _same(outs[k].to_numpy(m), ores[k], f"{pred_text} -> {proj_exprs[k]}")
+
+
+def test_golden_cast_string_to_timestamp(bsql, gpu, golden):
+    """process_project.cpp:415-446: CAST(varchar AS TIMESTAMP), nanoseconds."""
+    from blazingsql_amd.interop import DeviceColumn
+    g = golden["cast_from_string"]
+    _, ot, res = _run(bsql, ["CAST($0):TIMESTAMP"], [DeviceColumn.from_strings(g["col3"])], [STRING], 10)
+    assert ot == [16]
+    np.testing.assert_array_equal(res[0][0], g["expected_col3"])
+
+
+@pytest.mark.parametrize("n", [1, 5000, 70_001])
+def test_string_to_timestamp_vs_oracle(bsql, gpu, n):
+    """CAST(varchar AS DATE / TIMESTAMP_*) and TO_DATE / TO_TIMESTAMP with
+    other formats, on well-formed, short and malformed strings with nulls,
+    also compared and filtered on."""
+    import oracle
+    from blazingsql_amd.interop import DeviceColumn
+    rng = np.random.default_rng(n + 5)
+    secs = rng.integers(-9 * 10**9, 9 * 10**9, n)
+    import datetime
+    epoch = datetime.datetime(1970, 1, 1)
+    good = [(epoch + datetime.timedelta(seconds=int(s))).strftime("%Y-%m-%d %H:%M:%S") for s in secs]
+    junk = ["".join(rng.choice(list("0123456789-: x"), int(rng.integers(0, 20)))) for _ in range(n)]
+    strs = [g if rng.random() < 0.7 else j for g, j in zip(good, junk)]
+    valid = rng.random(n) >= 0.1
+    dev = [DeviceColumn.from_strings(strs, valid)]
+    host = [oracle.HostColumn(STRING, strings=strs, valid=valid)]
+    for e in ["CAST($0):DATE", "CAST($0):TIMESTAMP_SECONDS", "CAST($0):TIMESTAMP_MILLISECONDS",
+              "CAST($0):TIMESTAMP_MICROSECONDS", "CAST($0):TIMESTAMP", "TO_DATE($0, '%Y-%m-%d')",
+              "TO_TIMESTAMP($0, '%Y-%m-%d %H:%M:%S')", "TO_DATE(SUBSTRING($0, 9), '%d')",
+              ">(CAST($0):DATE, 1998-09-02)", "EXTRACT(FLAG(YEAR), CAST($0):TIMESTAMP)"]:
+        p, ot, res = _run(bsql, [e], dev, [STRING], n)
+        ores = _oracle(bsql, p, host, ot, n)
+        _same(res[0], ores[0], e)

@@ -563,3 +563,57 @@ def test_string_step_errors(bsql):
                              ("CASE(true, 'a', 'b')", [23], "condition")]:
         with pytest.raises(Exception, match=msg):
             bsql.Program.compile([expr], types)
+
+
+def test_cast_string_to_timestamp_golden(oracle_lib, golden, bsql):
+    """CAST(varchar AS TIMESTAMP) of process_project.cpp:415-446 (nanoseconds)."""
+    g = golden["cast_from_string"]
+    from oracle import HostColumn
+    p, ot = _compiled(bsql, ["CAST($0):TIMESTAMP"], [23])
+    assert ot == [16]
+    (v, ok), = oracle_lib.execute(p.to_oracle_dict(), [HostColumn(23, strings=g["col3"])], ot)
+    np.testing.assert_array_equal(v, g["expected_col3"])
+    assert ok.all()
+
+
+def test_timestamp_parse_closed_forms(oracle_lib, bsql):
+    """to_timestamps as restated, against Python's datetime: well-formed
+    strings in the default formats and in TO_DATE / TO_TIMESTAMP formats with
+    other field orders and separators, every tick type; a short string keeps
+    the fields it does not reach at 1970-01-01 00:00:00."""
+    import datetime
+    from oracle import HostColumn
+    rng = np.random.default_rng(12)
+    epoch = datetime.datetime(1970, 1, 1)
+    # within +-9.2e9 s, so that nanosecond ticks stay inside int64
+    when = [epoch + datetime.timedelta(seconds=int(s)) for s in rng.integers(-9 * 10**9, 9 * 10**9, 400)]
+    when = [w for w in when if 1 <= w.year <= 9999]
+    secs = [int((w - epoch).total_seconds()) for w in when]
+    days = [(w.date() - epoch.date()).days for w in when]
+    full = [w.strftime("%Y-%m-%d %H:%M:%S") for w in when]
+    col = HostColumn(23, strings=full)
+    for cast, t, mult in [("TIMESTAMP_SECONDS", 13, 1), ("TIMESTAMP_MILLISECONDS", 14, 1000),
+                          ("TIMESTAMP_MICROSECONDS", 15, 10**6), ("TIMESTAMP", 16, 10**9)]:
+        p, ot = _compiled(bsql, [f"CAST($0):{cast}"], [23])
+        assert ot == [t]
+        (v, _), = oracle_lib.execute(p.to_oracle_dict(), [col], ot)
+        assert [int(x) for x in v] == [s * mult for s in secs], cast
+    p, ot = _compiled(bsql, ["CAST($0):DATE"], [23])
+    (v, _), = oracle_lib.execute(p.to_oracle_dict(), [HostColumn(23, strings=[w.strftime("%Y-%m-%d") for w in when])], ot)
+    assert ot == [12] and [int(x) for x in v] == days
+    for fmt in ["%d/%m/%Y", "%Y%m%d", "%m-%d-%Y %H.%M.%S", "%%%Y_%d_%m"]:
+        text = [w.strftime(fmt) for w in when]
+        for fn, t in [("TO_DATE", 12), ("TO_TIMESTAMP", 16)]:
+            p, ot = _compiled(bsql, [f"{fn}($0, '{fmt}')"], [23])
+            assert ot == [t]
+            (v, _), = oracle_lib.execute(p.to_oracle_dict(), [HostColumn(23, strings=text)], ot)
+            parsed = [datetime.datetime.strptime(x, fmt) for x in text]
+            want = [(q.date() - epoch.date()).days if t == 12 else int((q - epoch).total_seconds()) * 10**9
+                    for q in parsed]
+            assert [int(x) for x in v] == want, (fn, fmt)
+    p, ot = _compiled(bsql, ["CAST($0):TIMESTAMP_SECONDS"], [23])
+    (v, _), = oracle_lib.execute(p.to_oracle_dict(), [HostColumn(23, strings=["2001-02", "", "1999"])], ot)
+    assert [int(x) for x in v] == [int((datetime.datetime(2001, 2, 1) - epoch).total_seconds()), 0,
+                                   int((datetime.datetime(1999, 1, 1) - epoch).total_seconds())]
+    with pytest.raises(Exception, match="not supported"):
+        bsql.Program.compile(["TO_DATE($0, '%Y-%j')"], [23])
