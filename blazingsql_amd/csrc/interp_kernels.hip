@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "devplan.h"
@@ -1795,12 +1797,59 @@ static bool slice_shape(const DevPlan& plan, int nreg, int rpt, int64_t* rows, i
 
 // Scratch words of the selection: the ballots, then the slice counts and
 // the block totals and bases (uint32 each).
+// Workgroups of `kernel` that are resident on the device at once (its
+// occupancy per CU x CUs), or 8 per CU when the runtime cannot say.
+static int64_t resident_workgroups(const void* kernel) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, int64_t> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find({dev, kernel});
+  if (it != cache.end()) return it->second;
+  (void)persistent_grid(1);  // learns the CU count
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, BLOCK, 0) != hipSuccess || per_cu <= 0)
+    per_cu = 8;
+  const int64_t r = (int64_t)per_cu * g_num_cus;
+  cache.emplace(std::make_pair(dev, kernel), r);
+  return r;
+}
+
+// Workgroups of the selection launch (and of its slice_ids): no more than are
+// resident at once, so that no second round of workgroups trails the first
+// (each takes a contiguous block of slices, ceil(slices / grid) of them).
+static int64_t selection_grid(const DevPlan& plan, int nreg, int sel_rpt, int64_t n_slices) {
+  const void* k = nullptr;
+  if (plan.n_ranges > 0) {
+    switch (range_chunks_of(sel_rpt)) {
+      case 2: k = reinterpret_cast<const void*>(&select_ranges<RANGE_R, 2>); break;
+      case 8: k = reinterpret_cast<const void*>(&select_ranges<RANGE_R, 8>); break;
+      default: k = reinterpret_cast<const void*>(&select_ranges<RANGE_R, 4>);
+    }
+  } else {
+    int NR, RR;
+    if (pick(nreg, sel_rpt, &NR, &RR)) {
+#define X(N, RPT) \
+  if (NR == N && RR == RPT) k = reinterpret_cast<const void*>(&select_vm<N, RPT, VmChunks<RPT>::S>);
+      BSQL_VARIANTS(X)
+#undef X
+    }
+  }
+  int64_t grid = persistent_grid((n_slices + WAVES - 1) / WAVES);
+  if (k) {
+    const int64_t r = resident_workgroups(k);
+    if (grid > r) grid = r;
+  }
+  return grid;
+}
+
 int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt) {
   int64_t rows;
   int ballots;
   if (!slice_shape(plan, nreg, rpt, &rows, &ballots)) return -1;
   const int64_t n = (plan.num_rows + rows - 1) / rows;
-  const int64_t grid = persistent_grid((n + WAVES - 1) / WAVES);
+  const int64_t grid = selection_grid(plan, nreg, rpt, n);
   return n * ballots + (n + 2 * grid + 1) / 2;
 }
 
@@ -1834,7 +1883,7 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
   int ballots;
   if (!slice_shape(plan, nreg, sel_rpt, &rows, &ballots)) return hipErrorInvalidValue;
   const int64_t n = (plan.num_rows + rows - 1) / rows;
-  const int64_t grid = persistent_grid((n + WAVES - 1) / WAVES);
+  const int64_t grid = selection_grid(plan, nreg, sel_rpt, n);
   uint32_t* counts = (uint32_t*)(sel + n * ballots);
   SelArgs sa{sel, counts, counts + n, counts + n + grid, n, grid ? (n + grid - 1) / grid : 0};
   hipError_t e = hipSuccess;
