@@ -1263,23 +1263,38 @@ __global__ __launch_bounds__(SCAN_THREADS) void block_scan(const uint32_t* __res
 }
 
 // Row ids of the selected rows, in row order. Workgroup g takes the same
-// block of slices as in the selection; the block's slice offsets are scanned
-// in LDS, BLOCK slices at a time.
+// block of slices as in the selection. Its first id's position is the sum of
+// the block totals before it (at most a few thousand words, read from L2);
+// the last workgroup writes the row total. The block's slice offsets are
+// scanned in LDS, BLOCK slices at a time.
 template <int R, int S>
-__global__ __launch_bounds__(BLOCK) void slice_ids(SelArgs sa, uint32_t* __restrict__ idx) {
+__global__ __launch_bounds__(BLOCK) void slice_ids(SelArgs sa, uint32_t* __restrict__ idx,
+                                                   int64_t* __restrict__ d_count) {
   constexpr int CHUNK = 64 * R;
   constexpr int SLICE = S * CHUNK;
   __shared__ uint32_t s_off[BLOCK];
   __shared__ uint32_t s_cnt[BLOCK];
   __shared__ uint32_t s_part[WAVES];
-  // a block that selected nothing has no ids to write (workgroup-uniform)
-  if (gp<uint32_t>(sa.block_sums)[blockIdx.x] == 0) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const GLOBAL uint32_t* sums = gp<uint32_t>(sa.block_sums);
+  uint32_t before_block = 0;
+  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += BLOCK) before_block += sums[b];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) before_block += __shfl_xor(before_block, d, 64);
+  if (lane == 0) s_part[wave] = before_block;
+  __syncthreads();
+  uint32_t carry = 0;
+#pragma unroll
+  for (int w = 0; w < WAVES; ++w) carry += s_part[w];
+  __syncthreads();  // s_part is reused below
+  const uint32_t mine_total = sums[blockIdx.x];
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *gpw<int64_t>(d_count) = (int64_t)carry + mine_total;
+  // a block that selected nothing has no ids to write (workgroup-uniform)
+  if (mine_total == 0) return;
   const uint64_t below = lane ? (~0ULL >> (64 - lane)) : 0ULL;
   GLOBAL uint32_t* ids = gpw<uint32_t>(idx);
   const int64_t s0 = (int64_t)blockIdx.x * sa.per;
   const int64_t s1 = s0 + sa.per < sa.n_slices ? s0 + sa.per : sa.n_slices;
-  uint32_t carry = gp<uint32_t>(sa.block_base)[blockIdx.x];
   for (int64_t c0 = s0; c0 < s1; c0 += BLOCK) {
     // offsets of slices [c0, c0 + BLOCK) within the block
     const int64_t mys = c0 + threadIdx.x;
@@ -1732,8 +1747,9 @@ static hipError_t launch_select_vm_t(const DevPlan* d_plan, const SelArgs& sa, i
 }
 
 template <int R, int S>
-static hipError_t launch_slice_ids(const SelArgs& sa, uint32_t* idx, int64_t grid, hipStream_t stream) {
-  hipLaunchKernelGGL((slice_ids<R, S>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, sa, idx);
+static hipError_t launch_slice_ids(const SelArgs& sa, uint32_t* idx, int64_t* d_count, int64_t grid,
+                                   hipStream_t stream) {
+  hipLaunchKernelGGL((slice_ids<R, S>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, sa, idx, d_count);
   return hipGetLastError();
 }
 
@@ -1911,23 +1927,24 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
     }
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(block_scan, dim3(1), dim3(SCAN_THREADS), 0, stream, (const uint32_t*)sa.block_sums,
-                     sa.block_base, n > 0 ? grid : (int64_t)0, d_count);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
   if (ids && n > 0) {
+    // the ids kernel also sums the block totals and writes the row total
     if (plan.n_ranges > 0) {
       const int chunks = range_chunks_of(sel_rpt);
-      e = chunks == 2 ? launch_slice_ids<RANGE_R, 2>(sa, idx, grid, stream)
-          : chunks == 8 ? launch_slice_ids<RANGE_R, 8>(sa, idx, grid, stream)
-                        : launch_slice_ids<RANGE_R, 4>(sa, idx, grid, stream);
+      e = chunks == 2 ? launch_slice_ids<RANGE_R, 2>(sa, idx, d_count, grid, stream)
+          : chunks == 8 ? launch_slice_ids<RANGE_R, 8>(sa, idx, d_count, grid, stream)
+                        : launch_slice_ids<RANGE_R, 4>(sa, idx, d_count, grid, stream);
     } else {
       int NR, RR;
       if (!pick(nreg, sel_rpt, &NR, &RR)) return hipErrorInvalidValue;
-      e = RR == 1 ? launch_slice_ids<1, VmChunks<1>::S>(sa, idx, grid, stream)
-          : RR == 2 ? launch_slice_ids<2, VmChunks<2>::S>(sa, idx, grid, stream)
-                    : launch_slice_ids<4, VmChunks<4>::S>(sa, idx, grid, stream);
+      e = RR == 1 ? launch_slice_ids<1, VmChunks<1>::S>(sa, idx, d_count, grid, stream)
+          : RR == 2 ? launch_slice_ids<2, VmChunks<2>::S>(sa, idx, d_count, grid, stream)
+                    : launch_slice_ids<4, VmChunks<4>::S>(sa, idx, d_count, grid, stream);
     }
+  } else {
+    hipLaunchKernelGGL(block_scan, dim3(1), dim3(SCAN_THREADS), 0, stream, (const uint32_t*)sa.block_sums,
+                       sa.block_base, n > 0 ? grid : (int64_t)0, d_count);
+    e = hipGetLastError();
   }
   if (e != hipSuccess || !(phases & 2)) return e;
   return launch_compact_gather(plan, d_plan, nreg, rpt, idx, d_count, capacity, stream);
