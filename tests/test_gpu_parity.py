@@ -4,7 +4,7 @@ size-independent properties at large sizes."""
 import numpy as np
 import pytest
 
-from tests.helpers import (BOOL8, FLOAT32, FLOAT64, INT8, INT32, INT64, TS_D, ExprGen, assert_same, device_columns,
+from tests.helpers import (BOOL8, FLOAT32, FLOAT64, INT8, INT16, INT32, INT64, TS_D, ExprGen, assert_same, device_columns,
                            eval_program_both, host_columns, make_table)
 
 pytestmark = pytest.mark.gpu
@@ -419,3 +419,52 @@ def test_alternating_programs_on_one_stream(bsql, gpu):
         filter_columns(pred, [col], [fo], cnt, rows)
         m = int(cnt.item())
         np.testing.assert_array_equal(fo.to_numpy(m)[0], host[:rows][host[:rows] > 0])
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_selective_first_range_vs_oracle(bsql, gpu, seed):
+    """A very selective first range (0.05-0.5 %) followed by 2-7 ranges over
+    every column kind, with nulls (most slices end with no live row, the
+    rest with a few): range tests and the general ops select what the oracle
+    selects, also with a NOT-EQUAL among the later ranges."""
+    import oracle
+    import torch
+    from blazingsql_amd.interop import DeviceColumn, filter_columns
+    rng = np.random.default_rng(2000 + seed)
+    n = 300_007
+    types = [INT32, FLOAT64, INT64, FLOAT32, TS_D, INT8, INT16, BOOL8]
+    table = make_table(rng, types, n, null_rates=[0.0, 0.05, 0.1, 0.0, 0.02, 0.1, 0.0, 0.05])
+    dcols = device_columns(table)
+    first = table[0][1]
+    for trial in range(5):
+        # first: a narrow band of column 0 (int32 in [-1000, 1000))
+        lo = int(rng.integers(-1000, 990))
+        width = int(rng.integers(0, 9))
+        terms = [f">=($0, {lo})", f"<=($0, {lo + width})"]
+        others = rng.permutation(np.arange(1, len(types)))[:int(rng.integers(2, 7))]
+        for c in others:
+            t, v, _ = table[c]
+            if t == BOOL8:
+                terms.append(f"=(${c}, true)")
+            elif t in (FLOAT64, FLOAT32):
+                q = float(np.quantile(v, rng.uniform(0.2, 0.8)))
+                terms.append(f"<(${c}, {q:.6f}:DOUBLE)")
+            elif t == TS_D:
+                terms.append(f">=(${c}, 1990-01-01)")
+            else:
+                q = int(np.quantile(v.astype(np.int64), rng.uniform(0.2, 0.8)))
+                terms.append(f"{'<>' if trial == 3 else '>='}(${c}, {q})")
+        text = "AND(" + ", ".join(terms) + ")"
+        on, ores = oracle.filter_table(_full_prog(bsql, text, types), host_columns(table), n)
+        assert 0 < on < n // 50 or width == 0, (text, on)
+        for ranges in (True, False):
+            pred = bsql.Program.compile(text, types)
+            pred.set_launch(ranges=ranges)
+            outs = [DeviceColumn.empty(t, n, nullable=v is not None) for t, _, v in table]
+            cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+            filter_columns(pred, dcols, outs, cnt, n)
+            m = int(cnt.item())
+            assert m == on, (text, ranges, m, on)
+            for k in range(len(types)):
+                gv, gval = outs[k].to_numpy(m)
+                assert_same(gv, gval, ores[k][0], ores[k][1], what=f"{text} ranges={ranges} col {k}")
