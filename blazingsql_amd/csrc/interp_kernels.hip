@@ -1189,10 +1189,11 @@ __device__ __forceinline__ void chunk_rows(Rows<R>& rs, int64_t row0, int64_t nu
 // the contiguous block of "slices" [g*per, (g+1)*per) (a slice is one wave's
 // S*R*64 rows). For each slice a wave writes S*R 64-bit ballots (ballot
 // q = k*R + j, bit t = row k*64*R + t*R + j of the slice) and the selected
-// count; the workgroup writes its block's total. block_scan (one workgroup)
-// turns the block totals into block bases and the row total; slice_ids
-// rebuilds the slice offsets of its block in LDS and writes the selected row
-// ids in row order. No workgroup waits on another: no ticket, no look-back.
+// count; the workgroup writes its block's total. slice_ids sums the block
+// totals before its block, rebuilds the slice offsets of its block in LDS and
+// writes the selected row ids in row order; its last workgroup writes the row
+// total. Without output ops, block_scan (one workgroup) writes the row total.
+// No workgroup waits on another: no ticket, no look-back.
 struct SelArgs {
   uint64_t* bits;        // n_slices * S * R ballots
   uint32_t* counts;      // n_slices
