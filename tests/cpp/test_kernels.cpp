@@ -6,6 +6,7 @@
 // the expected results are evident. Exit status 0 = every check passed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -56,6 +57,22 @@ static std::vector<bool> read_valid(const bsql_table* t, int k, hipStream_t s) {
   hip_ok(hipMemcpyAsync(w.data(), c.validity, w.size() * 4, hipMemcpyDeviceToHost, s), "copy validity");
   hip_ok(hipStreamSynchronize(s), "sync");
   for (int64_t i = 0; i < c.size; ++i) v[(size_t)i] = (w[(size_t)(i / 32)] >> (i % 32)) & 1u;
+  return v;
+}
+
+static std::vector<std::string> read_strings(const bsql_table* t, int k, hipStream_t s) {
+  bsql_column c{};
+  bsql_table_column(t, k, &c);
+  std::vector<int32_t> off((size_t)c.size + 1);
+  hip_ok(hipMemcpyAsync(off.data(), c.offsets + c.offset, off.size() * 4, hipMemcpyDeviceToHost, s), "offsets");
+  hip_ok(hipStreamSynchronize(s), "sync");
+  std::vector<char> chars((size_t)std::max(off.back(), 1));
+  if (off.back() > 0)
+    hip_ok(hipMemcpyAsync(chars.data(), c.data, (size_t)off.back(), hipMemcpyDeviceToHost, s), "chars");
+  hip_ok(hipStreamSynchronize(s), "sync");
+  std::vector<std::string> v;
+  for (int64_t i = 0; i < c.size; ++i)
+    v.emplace_back(chars.data() + off[(size_t)i], (size_t)(off[(size_t)i + 1] - off[(size_t)i]));
   return v;
 }
 
@@ -166,6 +183,30 @@ int main() {
   CHECK(bsql_table_num_rows(scanned.get()) == 50, "scan filter row count");
   CHECK(std::string(bsql_table_name(scanned.get(), 0)) == "u", "scan aliases");
 
+  // Projection building strings (columns made before the interpreter runs):
+  // label = 'a=' || CAST(a AS VARCHAR), up = UPPER(label), and a filter on it
+  Projection strs("LogicalProject(label=[||('a=', CAST($0):VARCHAR)], up=[UPPER(||('x', CAST($0):VARCHAR))])");
+  std::vector<Table> in8;
+  in8.push_back(make_batch(1000, s));
+  r = strs.do_process(std::move(in8), sink, s, {});
+  Table labelled = sink->pull();
+  CHECK(r.status == task_status::SUCCESS && labelled, "string projection succeeds");
+  {
+    auto label = read_strings(labelled.get(), 0, s);
+    auto up = read_strings(labelled.get(), 1, s);
+    bool same = label.size() == 1000 && up.size() == 1000;
+    for (size_t i = 0; same && i < label.size(); ++i)
+      same = label[i] == "a=" + std::to_string(i % 100) && up[i] == "X" + std::to_string(i % 100);
+    CHECK(same, "built strings");
+  }
+  Filter on_label("LogicalFilter(condition=[=(SUBSTRING($0, 3), '42')])");
+  std::vector<Table> in9;
+  in9.push_back(std::move(labelled));
+  r = on_label.do_process(std::move(in9), sink, s, {});
+  Table picked = sink->pull();
+  CHECK(r.status == task_status::SUCCESS && picked && bsql_table_num_rows(picked.get()) == 10,
+        "filter on a built string column");
+
   // RETRY: the allocator reports out of memory; the input comes back
   std::vector<Table> in5;
   in5.push_back(make_batch(n, s));
@@ -202,6 +243,7 @@ int main() {
 
   hip_ok(hipStreamSynchronize(s), "final sync");
   retried.reset();
+  picked.reset();
   scanned.reset();
   renamed.reset();
   empty.reset();
