@@ -16,7 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ROCM_INC = "/opt/rocm/include"
 
 HIP_SOURCES = ["interp_kernels.hip", "learned.hip", "strfn.hip"]
-CXX_SOURCES = ["program.cpp", "parser.cpp", "capi.cpp", "learned.cpp", "table.cpp", "steps.cpp"]
+CXX_SOURCES = ["program.cpp", "parser.cpp", "capi.cpp", "learned.cpp", "table.cpp", "steps.cpp", "regex.cpp"]
 
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
              "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

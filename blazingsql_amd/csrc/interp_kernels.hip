@@ -528,7 +528,76 @@ __device__ int64_t str_to_ticks(const GLOBAL uint8_t* s, int32_t n, const int32_
 // begins with '%', at the end unless it ends with '%'. The compiled pattern is
 // {n_units, flags, units...}: unit >= 0 a literal byte, -1 any character,
 // -2 any run of characters.
-enum : int32_t { LIKE_ANCHOR_START = 1, LIKE_ANCHOR_END = 2, LIKE_MATCH_ALL = 4 };
+enum : int32_t { LIKE_ANCHOR_START = 1, LIKE_ANCHOR_END = 2, LIKE_MATCH_ALL = 4, LIKE_REGEX = 8 };
+
+// Whether the regex compiled by regex_compile (its layout is described
+// there) is found in s[0, n): a Thompson simulation over 128-bit state sets,
+// restarted at every character (a search), '^' only at 0, '$' only at n.
+// Linear in n times the states alive: no backtracking.
+__device__ bool regex_char_in_class(const GLOBAL int32_t* prog, int32_t at, uint32_t cp) {
+  const int32_t nr = prog[at];
+  bool in = false;
+  for (int32_t k = 0; k < nr && !in; ++k)
+    in = cp >= (uint32_t)prog[at + 2 + 2 * k] && cp <= (uint32_t)prog[at + 3 + 2 * k];
+  return prog[at + 1] ? !in : in;
+}
+
+__device__ bool regex_contains(const GLOBAL uint8_t* s, int32_t n, const GLOBAL int32_t* prog) {
+  const int32_t P = prog[0];
+  const GLOBAL int32_t* kind = prog + 2;
+  const GLOBAL int32_t* mid = prog + 2 + P;
+  const GLOBAL int32_t* end = prog + 2 + 5 * P;
+  const GLOBAL int32_t* start = prog + 2 + 9 * P;
+  const int32_t match = P - 1;
+  uint32_t cur[4];
+  {
+    const GLOBAL int32_t* m = start + 4 * (2 + (n == 0 ? 1 : 0));
+    for (int w = 0; w < 4; ++w) cur[w] = (uint32_t)m[w];
+  }
+  if ((cur[match >> 5] >> (match & 31)) & 1u) return true;
+  int32_t i = 0;
+  while (i < n) {
+    int32_t l = utf8_step(s, i, n);
+    uint32_t cp;
+    const uint8_t c0 = s[i];
+    if (l == 1) cp = c0;
+    else if (l == 2) cp = ((uint32_t)(c0 & 0x1F) << 6) | (s[i + 1] & 0x3F);
+    else if (l == 3) cp = ((uint32_t)(c0 & 0x0F) << 12) | ((uint32_t)(s[i + 1] & 0x3F) << 6) | (s[i + 2] & 0x3F);
+    else cp = ((uint32_t)(c0 & 0x07) << 18) | ((uint32_t)(s[i + 1] & 0x3F) << 12) |
+              ((uint32_t)(s[i + 2] & 0x3F) << 6) | (s[i + 3] & 0x3F);
+    i += l;
+    const bool at_end = i >= n;
+    const GLOBAL int32_t* after = at_end ? end : mid;
+    uint32_t nxt[4];
+    {
+      const GLOBAL int32_t* m = start + 4 * (at_end ? 1 : 0);
+      for (int w = 0; w < 4; ++w) nxt[w] = (uint32_t)m[w];
+    }
+    for (int w = 0; w < 4; ++w) {
+      uint32_t bits = cur[w];
+      while (bits) {
+        const int q = w * 32 + __builtin_ctz(bits);
+        bits &= bits - 1u;
+        const uint32_t k = (uint32_t)kind[q];
+        const uint32_t arg = k >> 3;
+        bool ok;
+        switch (k & 7u) {
+          case 0: ok = cp == arg; break;
+          case 1: ok = cp != '\n'; break;
+          case 2: ok = regex_char_in_class(prog, (int32_t)arg, cp); break;
+          default: ok = false; break;
+        }
+        if (ok) {
+          const GLOBAL int32_t* m = after + 4 * (q + 1);
+          for (int v = 0; v < 4; ++v) nxt[v] |= (uint32_t)m[v];
+        }
+      }
+    }
+    for (int w = 0; w < 4; ++w) cur[w] = nxt[w];
+    if ((cur[match >> 5] >> (match & 31)) & 1u) return true;
+  }
+  return false;
+}
 __device__ bool like_from(const GLOBAL uint8_t* s, int32_t n, int32_t st, const GLOBAL int32_t* u, int32_t nu,
                           bool anchor_end) {
   int32_t i = st, q = 0, star_q = -1, star_i = 0;
@@ -555,6 +624,7 @@ __device__ bool like_match(const GLOBAL uint8_t* s, int32_t n, const GLOBAL int3
   const int32_t nu = pat[0], flags = pat[1];
   if (flags & LIKE_MATCH_ALL) return true;
   const GLOBAL int32_t* u = pat + 2;
+  if (flags & LIKE_REGEX) return regex_contains(s, n, u);
   const bool anchor_end = (flags & LIKE_ANCHOR_END) != 0;
   if (flags & LIKE_ANCHOR_START) return like_from(s, n, 0, u, nu, anchor_end);
   if (nu > 0 && u[0] == -2) {

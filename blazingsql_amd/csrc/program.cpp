@@ -199,12 +199,6 @@ std::string like_pass(const std::string& in, char w, char mark) {
   return out;
 }
 
-// LIKE pattern -> {n_units, flags, units...} for the device matcher: the
-// reference's translation to a regex (LogicalProject.cpp:31-47), with the
-// regex reduced to literal bytes (unit >= 0), '(?:.)' (-1) and '(?:.*?)' (-2),
-// anchored at the start unless the pattern's first character is '%' and at
-// the end unless its last is. Characters that would be regex syntax in the
-// reference (backslash included) are rejected rather than given a meaning.
 // The format language of cudf::strings::to_timestamps (libcudf 21.08) that
 // the reference's casts and TO_DATE / TO_TIMESTAMP use
 // (LogicalProject.cpp:469-533): %Y (4 digits), %m, %d, %H, %M, %S (2 each)
@@ -239,22 +233,39 @@ std::vector<int32_t> timestamp_format_compile(const std::string& f) {
   return items;
 }
 
+// LIKE pattern -> {n_units, flags, units...} for the device matcher: the
+// reference's translation to a regex (LogicalProject.cpp:31-47), anchored at
+// the start unless the pattern's first character is '%' and at the end unless
+// its last is. Without regex syntax in the pattern the regex is reduced to
+// literal bytes (unit >= 0), '(?:.)' (-1) and '(?:.*?)' (-2) for the direct
+// matcher; otherwise the pattern's characters keep the regex meaning the
+// reference gives them ('a.b%' matches "axb..."), and the translated regex is
+// compiled for the bitset matcher (flags LIKE_REGEX, units = regex_compile).
 std::vector<int32_t> like_compile(const std::string& pat) {
-  enum { ANCHOR_START = 1, ANCHOR_END = 2, MATCH_ALL = 4 };
+  enum { ANCHOR_START = 1, ANCHOR_END = 2, MATCH_ALL = 4, REGEX = 8 };
   const char ANY_RUN = '\x01', ANY_CHAR = '\x02';
   std::vector<int32_t> units;
   int32_t flags = 0;
   if (pat.empty()) {
     flags = MATCH_ALL;  // the empty regex is found in every string
   } else {
+    bool syntax = false;
     for (char c : pat) {
-      if (c == ANY_RUN || c == ANY_CHAR || (std::strchr(".^$*+?()[]{}|\\", c) && c != 0))
-        fail(std::string("LIKE pattern character '") + c + "' is regex syntax in the reference");
+      if (c == ANY_RUN || c == ANY_CHAR) fail("LIKE pattern with a control character \\x01 or \\x02");
+      if (c != 0 && std::strchr(".^$*+?()[]{}|\\", c)) syntax = true;
     }
     if (pat.front() != '%') flags |= ANCHOR_START;
     if (pat.back() != '%') flags |= ANCHOR_END;
-    const std::string re = like_pass(like_pass(pat, '%', ANY_RUN), '_', ANY_CHAR);
-    for (char c : re) units.push_back(c == ANY_RUN ? -2 : (c == ANY_CHAR ? -1 : (int32_t)(unsigned char)c));
+    const std::string marked = like_pass(like_pass(pat, '%', ANY_RUN), '_', ANY_CHAR);
+    if (syntax) {
+      std::string re = (flags & ANCHOR_START) ? "^" : "";
+      for (char c : marked) re += c == ANY_RUN ? "(?:.*?)" : (c == ANY_CHAR ? "(?:.)" : std::string(1, c));
+      if (flags & ANCHOR_END) re += "$";
+      flags |= REGEX;
+      units = regex_compile(re);
+    } else {
+      for (char c : marked) units.push_back(c == ANY_RUN ? -2 : (c == ANY_CHAR ? -1 : (int32_t)(unsigned char)c));
+    }
   }
   std::vector<int32_t> out{(int32_t)units.size(), flags};
   out.insert(out.end(), units.begin(), units.end());

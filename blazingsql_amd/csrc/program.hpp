@@ -112,6 +112,9 @@ OpTypes propagate_types(const Program& p, const int32_t* in_types, int n_in);
 // LIKE pattern compiled for the device matcher: {n_units, flags, units...}.
 std::vector<int32_t> like_compile(const std::string& pattern);
 
+// A regular expression compiled for the device's bitset matcher (regex.cpp).
+std::vector<int32_t> regex_compile(const std::string& re);
+
 // to_timestamps format compiled for the device parser: {n_items, items...},
 // an item (spec << 8) | width with spec 0 for literal characters.
 std::vector<int32_t> timestamp_format_compile(const std::string& format);

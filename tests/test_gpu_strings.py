@@ -411,3 +411,25 @@ def test_string_to_timestamp_vs_oracle(bsql, gpu, n):
         p, ot, res = _run(bsql, [e], dev, [STRING], n)
         ores = _oracle(bsql, p, host, ot, n)
         _same(res[0], ores[0], e)
+
+
+@pytest.mark.parametrize("n", [1, 2000, 70_001])
+def test_like_with_regex_syntax_vs_oracle(bsql, gpu, n):
+    """LIKE patterns with characters that are regex syntax after the
+    reference's translation ('.', '+', '*', '?', '|', '[..]', '(..)', '{n}',
+    '^', '$', escapes): the device's bitset matcher against the oracle's
+    ECMAScript regex, with multi-byte characters, newlines and nulls."""
+    import oracle
+    from blazingsql_amd.interop import DeviceColumn
+    rng = np.random.default_rng(n + 3)
+    alphabet = list("abc.x-_% ") + ["é", "\n", "1", "ß"]
+    strs, valid = _string_table(rng, n, alphabet, 0, 14, 0.1)
+    dev = [DeviceColumn.from_strings(strs, valid)]
+    host = [oracle.HostColumn(STRING, strings=strs, valid=valid)]
+    pats = ["%.", "a.b%", "%a+b%", "(ab)%", "%[ab]c%", "%b{2}%", "a|b%", "%c?d", "%(a|é)_", "[^a]%",
+            "%x*x%", "%\\\\.%", "%[.-]_", "^a%", "%$", "%(a|b|c){3}%", "%[a-cé]+$"]
+    for pat in pats:
+        e = f"LIKE($0, '{pat}')"
+        p, ot, res = _run(bsql, [e], dev, [STRING], n)
+        ores = _oracle(bsql, p, host, ot, n)
+        _same(res[0], ores[0], e)

@@ -313,7 +313,8 @@ def reference_like_regex(pat):
 def test_like_and_slices_closed_forms(oracle_lib, bsql):
     """LIKE against Python's regex engine running the reference's pattern
     translation (doubled wildcards keep their quirk: '__' is '^_(?:.)$',
-    '%%' finds a '%'), on random strings with multi-byte characters and
+    '%%' finds a '%'; '.', '+', '|', classes and groups in a pattern are
+    regex syntax there), on random strings with multi-byte characters and
     newlines; SUBSTRING/LEFT/RIGHT against Python slicing."""
     import re
     from oracle import HostColumn
@@ -322,10 +323,12 @@ def test_like_and_slices_closed_forms(oracle_lib, bsql):
     strs = ["".join(rng.choice(alphabet, int(rng.integers(0, 9)))) for _ in range(600)]
     col = HostColumn(23, strings=strs)
     for pat in ["a%", "%b", "%ab%", "_a%", "a_b", "%", "%é%", "__", "a%b%c", "ß_%", "%%", "%%a", "_%_",
-                "a__", "%_%", "", "%a%b", "é"]:
+                "a__", "%_%", "", "%a%b", "é",
+                # regex syntax keeps its meaning in the reference's translation
+                "%.", "a.b%", "%a+b%", "(ab)%", "%[ab]c%", "%b{2}%", "a|b%", "%c?d", "%(a|é)_", "[^a]%"]:
         p, ot = _compiled(bsql, [f"LIKE($0, '{pat}')"], [23])
         (vals, _), = oracle_lib.execute(p.to_oracle_dict(), [col], ot)
-        rx = re.compile(reference_like_regex(pat))
+        rx = re.compile(reference_like_regex(pat), re.ASCII)
         expect = [1 if rx.search(s) else 0 for s in strs]
         np.testing.assert_array_equal(vals.astype(int), expect, err_msg=repr(pat))
     for expr, fn in [("SUBSTRING($0, 2)", lambda s: s[1:]), ("SUBSTRING($0, 0)", lambda s: s),
