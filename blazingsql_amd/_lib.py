@@ -59,11 +59,11 @@ class StepInfo(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("op", ctypes.c_int32), ("out_type", ctypes.c_int32),
                 ("n_args", ctypes.c_int32), ("arg_col", ctypes.c_int32 * 3),
                 ("arg_str", ctypes.c_void_p * 3), ("arg_len", ctypes.c_int32 * 3),
-                ("sub", ctypes.c_void_p)]
+                ("sub", ctypes.c_void_p), ("start", ctypes.c_int32)]
 
 
 STRFN = {"CONCAT": 1, "UPPER": 2, "LOWER": 3, "INITCAP": 4, "REVERSE": 5, "REPLACE": 6, "CAST": 7,
-         "CASE": 8, "FILL": 9}
+         "CASE": 8, "FILL": 9, "REGEXP_REPLACE": 10}
 
 
 class LaunchConfig(ctypes.Structure):

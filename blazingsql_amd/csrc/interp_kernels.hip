@@ -544,10 +544,10 @@ __device__ bool regex_char_in_class(const GLOBAL int32_t* prog, int32_t at, uint
 
 __device__ bool regex_contains(const GLOBAL uint8_t* s, int32_t n, const GLOBAL int32_t* prog) {
   const int32_t P = prog[0];
-  const GLOBAL int32_t* kind = prog + 2;
-  const GLOBAL int32_t* mid = prog + 2 + P;
-  const GLOBAL int32_t* end = prog + 2 + 5 * P;
-  const GLOBAL int32_t* start = prog + 2 + 9 * P;
+  const GLOBAL int32_t* kind = prog + 4;
+  const GLOBAL int32_t* mid = prog + 4 + P;
+  const GLOBAL int32_t* end = prog + 4 + 5 * P;
+  const GLOBAL int32_t* start = prog + 4 + 9 * P;
   const int32_t match = P - 1;
   uint32_t cur[4];
   {

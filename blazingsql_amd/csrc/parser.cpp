@@ -701,7 +701,7 @@ void collect_vars(const Node* n, std::vector<int>& acc) {
 // run inside the interpreter on string slices; the functions that build new
 // strings become steps (below).
 bool is_string_function(const std::string& v) {
-  static const char* names[] = {"REGEXP_REPLACE", "TRIM"};
+  static const char* names[] = {"TRIM"};
   // TRIM with its side flag is interpreted (BLZ_TRIM_*); a bare TRIM is not
   for (const char* s : names)
     if (v == s) return true;
@@ -844,6 +844,25 @@ struct StepExtractor {
       st.args.push_back(literal_arg(n->children[2].get(), "REPLACE"));
       // cudf::strings::replace rejects an empty target
       if (st.args[1].str.empty()) fail("Parameter target must not be empty string.");
+    } else if (v == "REGEXP_REPLACE") {
+      // column, pattern, replacement[, position] (LogicalProject.cpp:148-182)
+      if (n->children.size() < 3 || n->children.size() > 6)
+        fail("REGEXP_REPLACE takes a string, a pattern, a replacement and an optional position");
+      if (n->children.size() > 4) fail("Optional parameters occurrence and match_type are not yet supported.");
+      st.op = SF_REGEXP_REPLACE;
+      st.args.push_back(string_arg(n->children[0], "REGEXP_REPLACE", false));
+      st.args.push_back(literal_arg(n->children[1].get(), "REGEXP_REPLACE"));
+      st.args.push_back(literal_arg(n->children[2].get(), "REGEXP_REPLACE"));
+      (void)regex_compile(st.args[1].str);  // rejects what the device matcher does not take
+      if (n->children.size() == 4) {
+        const Node* pos = n->children[3].get();
+        if (pos->type != NodeType::Literal || is_null_literal(pos->value) || is_string_type(pos->lit_type) ||
+            is_float_type(pos->lit_type))
+          fail("REGEXP_REPLACE position must be an integer literal");
+        const long long v1 = std::stoll(pos->value) - 1;
+        if (v1 < 0) fail("Position must be greater than zero.");
+        st.start = (int32_t)std::min<long long>(v1, INT32_MAX);
+      }
     } else if (v == "CAST_VARCHAR" || v == "CAST_CHAR") {
       if (n->children.size() != 1) fail("malformed CAST");
       if (n->children[0]->type == NodeType::Literal) fail("CAST operator not supported for literals");

@@ -54,6 +54,7 @@ struct Step {
   int op = 0;                    // STRFN: StrFnOp
   std::vector<StepArg> args;
   int32_t out_type = BSQL_STRING;
+  int32_t start = 0;             // REGEXP_REPLACE: characters kept before the search
   std::shared_ptr<void> d_lits;  // device copy of the literal arguments, made on first use
 };
 

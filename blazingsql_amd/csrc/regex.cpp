@@ -12,15 +12,20 @@
 // rejected with a message.
 //
 // Layout of the compiled program (int32 words), P states, state P-1 = MATCH:
-//   [0] P   [1] offset of the class data
-//   [2 + q]            state q: kind (bits 0-2: 0 char, 1 any, 2 class,
+//   [0] P   [1] offset of the class data   [2] offset of the edge table
+//   [3] capture groups (group 0 = the whole match not counted)
+//   [4 + q]            state q: kind (bits 0-2: 0 char, 1 any, 2 class,
 //                      3 match, 4 epsilon) | argument << 3 (code point or
 //                      class offset)
-//   [2 + P + 4q]       closure of state q in the middle of the string (128-bit
+//   [4 + P + 4q]       closure of state q in the middle of the string (128-bit
 //                      set of char/any/class/match states, 4 words)
-//   [2 + 5P + 4q]      closure of state q at the end of the string ($ passes)
-//   [2 + 9P + 4k]      closure of state 0 with k = 2*(at start) + (at end)
-//   classes: n_ranges, negate, then n_ranges (lo, hi) code point pairs.
+//   [4 + 5P + 4q]      closure of state q at the end of the string ($ passes)
+//   [4 + 9P + 4k]      closure of state 0 with k = 2*(at start) + (at end)
+//   classes: n_ranges, negate, then n_ranges (lo, hi) code point pairs
+//   edges, 2 words per state (for the backtracking matcher): x | y << 16
+//   (0xFFFF: none; x is tried first) and cond | (save slot + 1) << 8 (cond
+//   1: only at the start, 2: only at the end; slot 2g / 2g+1 = start / end
+//   of group g, g >= 1).
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -41,11 +46,14 @@ struct Node {
   bool negate = false;
   std::vector<Node> kids;
   int min = 0, max = 0;  // REP; max < 0: unbounded
+  bool lazy = false;     // REP
+  int group = 0;         // CAT: capture group index (0: none)
 };
 
 struct Parser {
   const std::string& s;
   size_t i = 0;
+  int groups = 0;
   explicit Parser(const std::string& re) : s(re) {}
 
   [[noreturn]] void bad(const std::string& why) { fail("regex '" + s + "': " + why); }
@@ -131,11 +139,18 @@ struct Parser {
     const char c = s[i];
     if (c == '(') {
       ++i;
+      int g = 0;
       if (s.compare(i, 2, "?:") == 0) i += 2;
       else if (i < s.size() && s[i] == '?') bad("lookaround is not supported");
-      n = alt();
+      else if (++groups > 9) bad("more than 9 capture groups");
+      else g = groups;
+      Node inner = alt();
       if (i >= s.size() || s[i] != ')') bad("unbalanced parenthesis");
       ++i;
+      if (!g) return inner;
+      n.t = Node::CAT;
+      n.group = g;
+      n.kids.push_back(inner);
       return n;
     }
     if (c == '[') { ++i; return cls(); }
@@ -188,7 +203,10 @@ struct Parser {
         ++i;
         if (r.max >= 0 && r.max < r.min) bad("{m,n} with n < m");
       }
-      if (i < s.size() && s[i] == '?') ++i;  // lazy: the same set of matches for a search
+      if (i < s.size() && s[i] == '?') {  // lazy: the same matches for a search, fewer for a replace
+        r.lazy = true;
+        ++i;
+      }
       if (n.t == Node::BOL || n.t == Node::EOL) bad("quantified anchor");
       r.kids.push_back(n);
       n = r;
@@ -220,8 +238,9 @@ struct Parser {
 struct State {
   int kind;
   uint32_t arg = 0;                  // code point or index into classes
-  int x = -1, y = -1;                // epsilon edges (-1: none; consuming states go to +1)
+  int x = -1, y = -1;                // epsilon edges (-1: none; consuming states go to +1); x first
   int cond = 0;                      // epsilon condition: 1 start only, 2 end only
+  int save = -1;                     // capture slot written when passing
 };
 
 struct Builder {
@@ -250,9 +269,22 @@ struct Builder {
         st[(size_t)q].x = q + 1;
         return;
       }
-      case Node::CAT:
+      case Node::CAT: {
+        if (n.group) {
+          State e{K_EPS};
+          e.save = 2 * n.group;
+          const int q = add(e);
+          st[(size_t)q].x = q + 1;
+        }
         for (const Node& k : n.kids) emit(k);
+        if (n.group) {
+          State e{K_EPS};
+          e.save = 2 * n.group + 1;
+          const int q = add(e);
+          st[(size_t)q].x = q + 1;
+        }
         return;
+      }
       case Node::ALT: {
         std::vector<int> jumps;
         for (size_t k = 0; k < n.kids.size(); ++k) {
@@ -272,22 +304,25 @@ struct Builder {
       case Node::REP: {
         const Node& body = n.kids[0];
         for (int k = 0; k < n.min; ++k) emit(body);
+        // a split tries the body first (greedy) or the way out first (lazy)
+        auto order = [&](int split, int body_at, int out_at) {
+          st[(size_t)split].x = n.lazy ? out_at : body_at;
+          st[(size_t)split].y = n.lazy ? body_at : out_at;
+        };
         if (n.max < 0) {
           const int split = add({K_EPS});
-          st[(size_t)split].x = split + 1;
           emit(body);
           const int back = add({K_EPS});
           st[(size_t)back].x = split;
-          st[(size_t)split].y = (int)st.size();
+          order(split, split + 1, (int)st.size());
         } else {
           std::vector<int> splits;
           for (int k = n.min; k < n.max; ++k) {
             const int split = add({K_EPS});
-            st[(size_t)split].x = split + 1;
             splits.push_back(split);
             emit(body);
           }
-          for (int sp : splits) st[(size_t)sp].y = (int)st.size();
+          for (int sp : splits) order(sp, sp + 1, (int)st.size());
         }
         return;
       }
@@ -327,8 +362,9 @@ std::vector<int32_t> regex_compile(const std::string& re) {
   b.emit(root);
   b.add({K_MATCH});
   const int P = (int)b.st.size();
-  std::vector<int32_t> out((size_t)(2 + P + 8 * P + 16), 0);
+  std::vector<int32_t> out((size_t)(4 + P + 8 * P + 16), 0);
   out[0] = P;
+  out[3] = p.groups;
   std::vector<int32_t> cls;
   std::vector<int32_t> class_at;
   for (const Node* c : b.classes) {
@@ -345,19 +381,26 @@ std::vector<int32_t> regex_compile(const std::string& re) {
   for (int q = 0; q < P; ++q) {
     const State& s = b.st[(size_t)q];
     uint32_t arg = s.kind == K_CLASS ? (uint32_t)(class_base + class_at[s.arg]) : s.arg;
-    out[(size_t)(2 + q)] = (int32_t)((uint32_t)s.kind | (arg << 3));
+    out[(size_t)(4 + q)] = (int32_t)((uint32_t)s.kind | (arg << 3));
     uint32_t m[4];
     closure(b.st, q, false, false, m);
-    for (int w = 0; w < 4; ++w) out[(size_t)(2 + P + 4 * q + w)] = (int32_t)m[w];
+    for (int w = 0; w < 4; ++w) out[(size_t)(4 + P + 4 * q + w)] = (int32_t)m[w];
     closure(b.st, q, false, true, m);
-    for (int w = 0; w < 4; ++w) out[(size_t)(2 + 5 * P + 4 * q + w)] = (int32_t)m[w];
+    for (int w = 0; w < 4; ++w) out[(size_t)(4 + 5 * P + 4 * q + w)] = (int32_t)m[w];
   }
   for (int k = 0; k < 4; ++k) {
     uint32_t m[4];
     closure(b.st, 0, (k & 2) != 0, (k & 1) != 0, m);
-    for (int w = 0; w < 4; ++w) out[(size_t)(2 + 9 * P + 4 * k + w)] = (int32_t)m[w];
+    for (int w = 0; w < 4; ++w) out[(size_t)(4 + 9 * P + 4 * k + w)] = (int32_t)m[w];
   }
   out.insert(out.end(), cls.begin(), cls.end());
+  out[2] = (int32_t)out.size();
+  for (int q = 0; q < P; ++q) {
+    const State& s = b.st[(size_t)q];
+    const uint32_t x = s.x < 0 ? 0xFFFFu : (uint32_t)s.x, y = s.y < 0 ? 0xFFFFu : (uint32_t)s.y;
+    out.push_back((int32_t)(x | (y << 16)));
+    out.push_back((int32_t)((uint32_t)s.cond | ((uint32_t)(s.save + 1) << 8)));
+  }
   return out;
 }
 

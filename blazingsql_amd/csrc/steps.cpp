@@ -68,6 +68,17 @@ int64_t step_chars(const Step& st, const std::vector<bsql_column>& cols, int64_t
       return (t == BSQL_BOOL8 ? 5 : t <= BSQL_UINT64 ? 20 : 32) * n;
     }
     case SF_CASE: return arg_chars(st.args[1], cols, n) + arg_chars(st.args[2], cols, n);
+    case SF_REGEXP_REPLACE: {
+      // a row of b bytes has at most 2b + 1 matches (empty ones included),
+      // each giving the replacement's L bytes plus, for each of its r
+      // backreferences, at most the match's own bytes
+      const std::string& rp = st.args[2].str;
+      int64_t r = 0;
+      for (size_t k = 0; k + 1 < rp.size(); ++k)
+        if (rp[k] == '\\' && rp[k + 1] >= '1' && rp[k + 1] <= '9') ++r, ++k;
+      const int64_t L = (int64_t)rp.size();
+      return a0 * (1 + 2 * L + r) + n * L;
+    }
     case SF_FILL: return a0;
     default: return 0;
   }
@@ -76,9 +87,17 @@ int64_t step_chars(const Step& st, const std::vector<bsql_column>& cols, int64_t
 // Device copy of a step's literal arguments, one {bytes} record per argument.
 const uint8_t* step_literals(bsql_program& p, Step& st, std::vector<int32_t>* at) {
   std::vector<uint8_t> blob;
-  for (const StepArg& a : st.args) {
+  for (size_t k = 0; k < st.args.size(); ++k) {
+    const StepArg& a = st.args[k];
+    while (blob.size() % 8) blob.push_back(0);
     at->push_back((int32_t)blob.size());
-    if (a.kind == StepArg::LITERAL) blob.insert(blob.end(), a.str.begin(), a.str.end());
+    if (a.kind != StepArg::LITERAL) continue;
+    if (st.op == SF_REGEXP_REPLACE && k == 1) {  // the pattern, compiled for the matcher
+      const std::vector<int32_t> prog = regex_compile(a.str);
+      blob.insert(blob.end(), (const uint8_t*)prog.data(), (const uint8_t*)(prog.data() + prog.size()));
+    } else {
+      blob.insert(blob.end(), a.str.begin(), a.str.end());
+    }
   }
   std::lock_guard<std::mutex> lk(p.cache_mu);
   if (!st.d_lits && !blob.empty()) {
@@ -169,6 +188,7 @@ void build_steps(bsql_program& p, StepColumns& sc, int64_t num_rows) {
       f.n_args = (int32_t)st.args.size();
       for (size_t k = 0; k < st.args.size(); ++k) f.a[k] = source_of(st.args[k], sc.cols, lits, at[k]);
       f.num_rows = n;
+      f.start = st.start;
       f.lens = (uint64_t*)stream_alloc(sc, n * 8);
       f.out_valid = c.validity;
       f.out_offsets = c.offsets;
@@ -260,6 +280,7 @@ int bsql_program_step(const bsql_program* p, int k, bsql_step_info* out) {
     out->arg_len[i] = (int32_t)a.str.size();
   }
   out->sub = static_cast<const bsql_program*>(st.sub.get());
+  out->start = st.start;
   return BSQL_OK;
 }
 

@@ -25,6 +25,7 @@ enum StrFnOp : int32_t {
   SF_CAST = BSQL_STRFN_CAST,        // a[0] (integer, bool, float, timestamp or string) as text
   SF_CASE = BSQL_STRFN_CASE,        // a[0] (BOOL8) valid and true ? a[1] : a[2]
   SF_FILL = BSQL_STRFN_FILL,        // the literal a[0] in every row
+  SF_REGEXP_REPLACE = BSQL_STRFN_REGEXP_REPLACE,  // a[1] (compiled regex) -> a[2], from char `start`
 };
 
 enum StrSrcKind : int32_t { SRC_STRING = 0, SRC_LITERAL = 1, SRC_NULL = 2, SRC_FIXED = 3 };
@@ -49,6 +50,7 @@ struct StrFnArgs {
   int32_t n_args;
   StrSrc a[3];
   int64_t num_rows;
+  int32_t start;               // REGEXP_REPLACE: characters kept before the search
   uint64_t* lens;              // length pass: row length << 32 (null rows: 0)
   uint32_t* out_valid;         // length pass: validity words of the result
   const int32_t* out_offsets;  // write pass

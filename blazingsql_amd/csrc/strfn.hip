@@ -286,6 +286,173 @@ __device__ __forceinline__ T fixed_at(const StrSrc& a, int64_t row) {
   return ((const T*)a.data)[row];
 }
 
+// ---- regular expressions (the program of regex_compile, in regex.cpp)
+
+constexpr int RX_STACK = 256;       // backtracking frames per lane
+constexpr int RX_STEPS = 1 << 20;   // steps per match attempt
+constexpr int32_t RX_RANGE = 1 << 30;  // frame tag: a greedy one-character loop's positions
+
+__device__ bool rx_in_class(const int32_t* prog, int32_t at, uint32_t cp) {
+  const int32_t nr = prog[at];
+  bool in = false;
+  for (int32_t k = 0; k < nr && !in; ++k) in = cp >= (uint32_t)prog[at + 2 + 2 * k] && cp <= (uint32_t)prog[at + 3 + 2 * k];
+  return prog[at + 1] ? !in : in;
+}
+
+// Whether the one-character state k accepts the character at s[pos] (pos < n);
+// its byte length in *l.
+__device__ __forceinline__ bool rx_accepts(const int32_t* prog, uint32_t k, const uint8_t* s, int32_t pos,
+                                           int32_t n, int32_t* l) {
+  *l = char_bytes(s[pos]) <= n - pos ? char_bytes(s[pos]) : n - pos;
+  const uint32_t cp = code_point(s + pos, *l);
+  const uint32_t kk = k & 7u;
+  return kk == 0 ? cp == (k >> 3) : (kk == 1 ? cp != '\n' : rx_in_class(prog, (int32_t)(k >> 3), cp));
+}
+
+// Leftmost-first (backtracking, first alternative / greedy or lazy order as
+// compiled) match of the regex starting exactly at byte `at` of s[0, n); on
+// success caps[0..1] is the match and caps[2g..2g+1] group g (-1: unset).
+// A greedy loop over one character state runs as far as it can and leaves
+// one frame for all its shorter lengths. A match needing more than RX_STACK
+// pending choices or RX_STEPS steps counts as no match.
+__device__ bool rx_match_at(const uint8_t* s, int32_t n, int32_t at, const int32_t* prog, int32_t* caps) {
+  const int32_t G = prog[3];
+  const int32_t* kind = prog + 4;
+  const int32_t* edges = prog + prog[2];
+  int32_t stk_a[RX_STACK], stk_b[RX_STACK], stk_c[RX_STACK];
+  int sp = 0;
+  for (int g = 0; g < 2 * (G + 1); ++g) caps[g] = -1;
+  caps[0] = at;
+  int32_t pc = 0, pos = at;
+  for (int steps = 0; steps < RX_STEPS; ++steps) {
+    bool ok = true;
+    const uint32_t k = (uint32_t)kind[pc];
+    const uint32_t kk = k & 7u;
+    if (kk <= 2) {
+      int32_t l = 0;
+      ok = pos < n && rx_accepts(prog, k, s, pos, n, &l);
+      if (ok) {
+        pos += l;
+        ++pc;
+      }
+    } else if (kk == 3) {
+      caps[1] = pos;
+      return true;
+    } else {
+      const uint32_t e0 = (uint32_t)edges[2 * pc], e1 = (uint32_t)edges[2 * pc + 1];
+      const uint32_t cond = e1 & 0xFFu;
+      const int slot = (int)(e1 >> 8) - 1;
+      if ((cond == 1 && pos != 0) || (cond == 2 && pos != n)) {
+        ok = false;
+      } else {
+        if (slot >= 0) {
+          if (sp >= RX_STACK) return false;
+          stk_a[sp] = -(slot + 1);
+          stk_b[sp] = caps[slot];
+          ++sp;
+          caps[slot] = pos;
+        }
+        const uint32_t x = e0 & 0xFFFFu, y = e0 >> 16;
+        // greedy loop over one character state: split -> (body at +1, out),
+        // body -> jump back to the split
+        const bool run = slot < 0 && cond == 0 && y != 0xFFFFu && x == (uint32_t)pc + 1 &&
+                         ((uint32_t)kind[pc + 1] & 7u) <= 2 && ((uint32_t)kind[pc + 2] & 7u) == 4 &&
+                         ((uint32_t)edges[2 * (pc + 2)] & 0xFFFFu) == (uint32_t)pc &&
+                         ((uint32_t)edges[2 * (pc + 2)] >> 16) == 0xFFFFu && edges[2 * (pc + 2) + 1] == 0;
+        if (run) {
+          const uint32_t body = (uint32_t)kind[pc + 1];
+          const int32_t from = pos;
+          int32_t l = 0;
+          while (pos < n && rx_accepts(prog, body, s, pos, n, &l)) pos += l;
+          if (pos > from) {  // the shorter lengths, tried longest first on backtracking
+            if (sp >= RX_STACK) return false;
+            stk_a[sp] = RX_RANGE | (int32_t)y;
+            stk_b[sp] = pos;
+            stk_c[sp] = from;
+            ++sp;
+          }
+          pc = (int32_t)y;
+        } else {
+          if (y != 0xFFFFu) {
+            if (sp >= RX_STACK) return false;
+            stk_a[sp] = (int32_t)y;
+            stk_b[sp] = pos;
+            ++sp;
+          }
+          pc = (int32_t)x;
+        }
+      }
+    }
+    if (!ok) {
+      while (true) {
+        if (sp == 0) return false;
+        const int32_t a = stk_a[sp - 1];
+        if (a < 0) {  // restore a capture slot
+          caps[-a - 1] = stk_b[sp - 1];
+          --sp;
+          continue;
+        }
+        if (a & RX_RANGE) {  // one character shorter, the frame kept while more remain
+          int32_t p = stk_b[sp - 1] - 1;
+          while (p > stk_c[sp - 1] && (s[p] & 0xC0) == 0x80) --p;
+          if (p <= stk_c[sp - 1]) --sp; else stk_b[sp - 1] = p;
+          pc = a & ~RX_RANGE;
+          pos = p;
+          break;
+        }
+        --sp;
+        pc = a;
+        pos = stk_b[sp];
+        break;
+      }
+    }
+  }
+  return false;
+}
+
+// cudf::strings::replace_with_backrefs over s[0, n): matches left to right,
+// each replaced by `repl` with \1..\9 the groups (an unset group is empty);
+// an empty match is replaced too and the character after it copied (the rule
+// of Python's re.sub).
+__device__ void rx_replace(Out& o, const uint8_t* s, int32_t n, const int32_t* prog, const uint8_t* repl,
+                           int32_t rn) {
+  const int32_t G = prog[3];
+  int32_t caps[20];
+  int32_t i = 0;
+  while (i <= n) {
+    int32_t at = i;
+    bool found = false;
+    while (at <= n) {
+      if (rx_match_at(s, n, at, prog, caps)) { found = true; break; }
+      if (at == n) break;
+      at += char_bytes(s[at]) <= n - at ? char_bytes(s[at]) : n - at;
+    }
+    if (!found) break;
+    o.put(s + i, at - i);
+    for (int32_t k = 0; k < rn; ++k) {
+      const uint8_t c = repl[k];
+      if (c == '\\' && k + 1 < rn && repl[k + 1] >= '1' && repl[k + 1] <= '9') {
+        const int g = repl[k + 1] - '0';
+        if (g <= G && caps[2 * g] >= 0 && caps[2 * g + 1] >= caps[2 * g])
+          o.put(s + caps[2 * g], caps[2 * g + 1] - caps[2 * g]);
+        ++k;
+      } else {
+        o.put(c);
+      }
+    }
+    const int32_t e = caps[1];
+    if (e == at) {  // an empty match: keep the next character and move past it
+      if (at >= n) return;
+      const int32_t l = char_bytes(s[at]) <= n - at ? char_bytes(s[at]) : n - at;
+      o.put(s + at, l);
+      i = at + l;
+    } else {
+      i = e;
+    }
+  }
+  if (i < n) o.put(s + i, n - i);
+}
+
 // Row `row` of the result into o; false for a null row.
 __device__ bool build_row(const StrFnArgs& f, int64_t row, Out& o) {
   const uint8_t* s;
@@ -366,6 +533,15 @@ __device__ bool build_row(const StrFnArgs& f, int64_t row, Out& o) {
           ++i;
         }
       }
+      return true;
+    }
+    case bsql::SF_REGEXP_REPLACE: {
+      if (!str_arg(f.a[0], row, &s, &n)) return false;
+      // the first `start` characters are kept as they are (slice_strings)
+      int32_t b = 0;
+      for (int32_t c = 0; c < f.start && b < n; ++c) b += char_bytes(s[b]) <= n - b ? char_bytes(s[b]) : n - b;
+      o.put(s, b);
+      rx_replace(o, s + b, n - b, (const int32_t*)f.a[1].lit, f.a[2].lit, f.a[2].lit_len);
       return true;
     }
     case bsql::SF_CAST: {

@@ -168,7 +168,8 @@ class Program:
                     args.append(("literal", ctypes.string_at(info.arg_str[i], info.arg_len[i]).decode()))
                 else:
                     args.append(("null",))
-            d = dict(kind="eval" if info.kind == 0 else "strfn", op=info.op, out_type=info.out_type, args=args)
+            d = dict(kind="eval" if info.kind == 0 else "strfn", op=info.op, out_type=info.out_type, args=args,
+                     start=info.start)
             if info.kind == 0:
                 d["sub"] = Program(info.sub, owner=self)
             out.append(d)

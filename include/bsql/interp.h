@@ -198,7 +198,8 @@ enum {
   BSQL_STRFN_REPLACE = 6, /* every occurrence of arg 1 by arg 2 (cudf::strings::replace) */
   BSQL_STRFN_CAST = 7,    /* CAST(x AS VARCHAR): from_integers/booleans/floats/timestamps */
   BSQL_STRFN_CASE = 8,    /* arg 0 (BOOL8) true ? arg 1 : arg 2 (cudf::copy_if_else) */
-  BSQL_STRFN_FILL = 9     /* the literal in every row (make_column_from_scalar) */
+  BSQL_STRFN_FILL = 9,    /* the literal in every row (make_column_from_scalar) */
+  BSQL_STRFN_REGEXP_REPLACE = 10 /* every match of arg 1 by arg 2 with \1..\9 (replace_with_backrefs) */
 };
 typedef struct bsql_step_info {
   int32_t kind;       /* 0: expression (sub, one output); 1: string function (op) */
@@ -209,6 +210,7 @@ typedef struct bsql_step_info {
   const char* arg_str[3];
   int32_t arg_len[3];
   const bsql_program* sub; /* kind 0: compiled over columns [0, n_table + k) */
+  int32_t start;           /* REGEXP_REPLACE: characters kept before the search (position - 1) */
 } bsql_step_info;
 int bsql_program_num_steps(const bsql_program* prog);
 int bsql_program_table_columns(const bsql_program* prog);

@@ -313,7 +313,7 @@ def run_steps(prog, cols, num_rows):
         args = [a if isinstance(a, tuple) else strfn.column_values(cols[a]) for a in st["args"]]
         if st["op"] == strfn.CAST:
             args = [(cols[st["args"][0]].type_id, args[0])]
-        res = strfn.build(st["op"], args, num_rows)
+        res = strfn.build(st["op"], args, num_rows, st.get("start", 0))
         cols.append(HostColumn(STRING, strings=[r if r is not None else "" for r in res],
                                valid=[r is not None for r in res]))
     return cols

@@ -23,6 +23,8 @@ behaviour):
            (:355-377, cast_to_str_functor :62-91)
   CASE     cudf::copy_if_else: condition valid and true -> first value
            (:612-659)
+  REGEXP_REPLACE  cudf::strings::replace_with_backrefs after the first
+           position - 1 characters (:148-182), as regexp_replace below
 The case mappings come from Python's unicodedata; that the reference's
 libcudf character tables agree with them beyond ASCII and Latin letters is
 not shown by any vector the reference holds (parity unpinned there).
@@ -33,7 +35,7 @@ import math
 STRING, BOOL8, FLOAT32, FLOAT64 = 23, 11, 9, 10
 TS_DAYS, TS_S, TS_MS, TS_US, TS_NS = 12, 13, 14, 15, 16
 
-CONCAT, UPPER, LOWER, INITCAP, REVERSE, REPLACE, CAST, CASE, FILL = range(1, 10)
+CONCAT, UPPER, LOWER, INITCAP, REVERSE, REPLACE, CAST, CASE, FILL, REGEXP_REPLACE = range(1, 11)
 
 
 def _map_bmp(s, fn):
@@ -63,6 +65,29 @@ def initcap(s):
             out.append(ch)
         capitalize = not alpha
     return "".join(out)
+
+
+def regexp_replace(s, pattern, repl, start):
+    """cudf::strings::replace_with_backrefs on the characters from `start`
+    (the first `start` kept, LogicalProject.cpp:148-182): every match, left
+    to right (empty ones too, the character after an empty match kept), by
+    `repl` with \\1..\\9 the groups (an unset or missing group is empty).
+    Python's re with ASCII classes; '$' matches only at the end."""
+    import re
+    rx = re.compile(pattern.replace("$", r"\Z"), re.ASCII)
+
+    def expand(m):
+        out, i = [], 0
+        while i < len(repl):
+            if repl[i] == "\\" and i + 1 < len(repl) and repl[i + 1] in "123456789":
+                g = int(repl[i + 1])
+                out.append((m.group(g) or "") if g <= rx.groups else "")
+                i += 2
+            else:
+                out.append(repl[i])
+                i += 1
+        return "".join(out)
+    return s[:start] + rx.sub(expand, s[start:])
 
 
 def float_text(v):
@@ -154,7 +179,7 @@ def column_values(col):
     return [col.values[i] if valid[i] else None for i in range(n)]
 
 
-def build(op, args, n):
+def build(op, args, n, start=0):
     """One string step over `args` (lists of row values, or ("literal", s) /
     ("null",)); returns a list of strings / None."""
     def arg(k, i):
@@ -179,6 +204,9 @@ def build(op, args, n):
         elif op == CAST:
             t, x = args[0]
             out.append(None if x[i] is None else (x[i] if t == STRING else cast_text(t, x[i])))
+        elif op == REGEXP_REPLACE:
+            x = arg(0, i)
+            out.append(None if x is None else regexp_replace(x, args[1][1], args[2][1], start))
         elif op == CASE:
             c = arg(0, i)
             out.append(arg(1, i) if c is not None and c else arg(2, i))

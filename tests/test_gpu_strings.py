@@ -433,3 +433,31 @@ def test_like_with_regex_syntax_vs_oracle(bsql, gpu, n):
         p, ot, res = _run(bsql, [e], dev, [STRING], n)
         ores = _oracle(bsql, p, host, ot, n)
         _same(res[0], ores[0], e)
+
+
+@pytest.mark.parametrize("n", [1, 3000, 40_001])
+def test_regexp_replace_vs_oracle(bsql, gpu, n):
+    """REGEXP_REPLACE with groups and backreferences, greedy and lazy loops,
+    alternation, anchors, empty matches and a start position, on short and
+    long (up to 2000 characters) strings with multi-byte characters and
+    nulls; also nested in other functions and in a comparison."""
+    import oracle
+    from blazingsql_amd.interop import DeviceColumn
+    rng = np.random.default_rng(n + 17)
+    alphabet = list("abc.x- 12") + ["é"]
+    strs, valid = _string_table(rng, n, alphabet, 0, 16, 0.1)
+    for i in rng.choice(n, min(n, 20), replace=False):
+        strs[i] = "".join(rng.choice(alphabet, int(rng.integers(200, 2000))))
+    dev = [DeviceColumn.from_strings(strs, valid)]
+    host = [oracle.HostColumn(STRING, strings=strs, valid=valid)]
+    exprs = ["REGEXP_REPLACE($0, 'a', 'X')", "REGEXP_REPLACE($0, '(a)(b)', '\\2\\1')",
+             "REGEXP_REPLACE($0, '[0-9]+', '#')", "REGEXP_REPLACE($0, 'x*', '-')",
+             "REGEXP_REPLACE($0, 'a.*b', '_')", "REGEXP_REPLACE($0, 'a.*?b', '_')",
+             "REGEXP_REPLACE($0, '(é|x)+', '\\1\\1')", "REGEXP_REPLACE($0, '^a|b$', '!')",
+             "REGEXP_REPLACE($0, '(a)|(b)', '<\\1|\\2>')", "REGEXP_REPLACE($0, '\\d', 'D', 3)",
+             "REGEXP_REPLACE($0, '(.)\\.', '\\1!')", "UPPER(REGEXP_REPLACE($0, 'c', 'ç'))",
+             "=(REGEXP_REPLACE($0, '[^a]', ''), 'aa')", "CHAR_LENGTH(REGEXP_REPLACE($0, '(a+)', '\\1\\1'))"]
+    for e in exprs:
+        p, ot, res = _run(bsql, [e], dev, [STRING], n)
+        ores = _oracle(bsql, p, host, ot, n)
+        _same(res[0], ores[0], e)

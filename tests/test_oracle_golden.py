@@ -620,3 +620,21 @@ def test_timestamp_parse_closed_forms(oracle_lib, bsql):
                                    int((datetime.datetime(1999, 1, 1) - epoch).total_seconds())]
     with pytest.raises(Exception, match="not supported"):
         bsql.Program.compile(["TO_DATE($0, '%Y-%j')"], [23])
+
+
+def test_regexp_replace_rules(bsql):
+    """REGEXP_REPLACE(column, pattern, replacement[, position]) as the
+    reference takes it (LogicalProject.cpp:148-182): occurrence and
+    match_type are rejected, a position must be above zero; the pattern must
+    be one the device matcher takes."""
+    p = bsql.Program.compile(["REGEXP_REPLACE($0, '(a)(b)', '\\2\\1', 3)"], [23])
+    st = p.steps()
+    assert len(st) == 1 and st[0]["op"] == 10 and st[0]["start"] == 2
+    assert st[0]["args"][1:] == [("literal", "(a)(b)"), ("literal", "\\2\\1")]
+    for expr, msg in [("REGEXP_REPLACE($0, 'a', 'b', 1, 1)", "occurrence"),
+                      ("REGEXP_REPLACE($0, 'a', 'b', 0)", "greater than zero"),
+                      ("REGEXP_REPLACE($0, '(a', 'b')", "parenthesis"),
+                      ("REGEXP_REPLACE($0, '\\1', 'b')", "not supported"),
+                      ("REGEXP_REPLACE('abc', 'a', 'b')", "literals")]:
+        with pytest.raises(Exception, match=msg):
+            bsql.Program.compile([expr], [23])
