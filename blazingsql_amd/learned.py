@@ -2,7 +2,7 @@
 the library's own forward/backward/Adam kernels), training data recorded
 from the exact interpreter on synthetic batches, and the ordering it drives.
 
-The static plan order is the fallback and the judge: an order only permutes
+The static plan order is the fallback and the reference result: an order only permutes
 the conjuncts of an AND chain, so the selected rows never depend on the
 model; a wrong prediction costs time, not correctness.
 """
