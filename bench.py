@@ -344,7 +344,7 @@ def run_learned(args):
             # the order the exact selectivities and costs give (the regret's reference)
             true_sel, true_t = learned.measure_terms(terms, cols, n, reps=2)
             op = bsql.Program.compile(text, types)
-            learned.order_by(op, true_sel, true_t)
+            learned.order_for_scan(op, types, true_sel, true_t)
             t_oracle = timed(op, proj, cols, outs, cnt)
             rows.append({"rho": rho, "terms": len(terms), "t_static_ms": t_static, "t_learned_ms": t_learned,
                          "t_oracle_order_ms": t_oracle, "sel_mae": float(np.mean(np.abs(sel - true_sel))),

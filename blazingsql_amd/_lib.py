@@ -125,6 +125,7 @@ SIGNATURES = {
     "bsql_mlp_free": (None, [_P]),
     "bsql_conjunct_features": (_I, [_P, _P, _I, _P, _P, _I]),
     "bsql_order_by_prediction": (_I, [_P, _P, _P, _I]),
+    "bsql_order_conjuncts": (_I, [_P, _P, _I, _P, _P, _I, _P]),
     "bsql_set_allocator": (_I, [_P, _P, _P]),
     "bsql_table_create": (_I, [_P, _P, _P, _P, _I, _I64, _P, ctypes.POINTER(_P)]),
     "bsql_table_from_views": (_I, [_P, _P, _I, _I64, ctypes.POINTER(_P)]),

@@ -740,7 +740,7 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
     case D_LN: case D_LOG: case D_EXTRACT:
       ROWS(compute_heavy(op.code, op.aux, op.aux2, op.imm, AF, BF, a, rs.row[k]), va)
     case D_RAND:
-      ROWS(compute_heavy(op.code, op.aux, op.aux2, op.imm, 0.0, 0.0, 0, rs.pos0 + k), 1u)
+      ROWS(compute_heavy(op.code, op.aux, op.aux2, op.imm, 0.0, 0.0, 0, rs.pos0 + (int64_t)k * rs.pstride), 1u)
     case D_EQ_SS: case D_NE_SS: case D_LT_SS: case D_GT_SS: case D_LE_SS: case D_GE_SS: {
       const int kind = op.code - D_EQ_SS;
 #pragma unroll
@@ -955,8 +955,8 @@ __device__ __forceinline__ DevOp load_op(const DevPlan& plan, int q) {
 
 // The rows one lane works on: row[k] for k < R where bit k of `exist` is set.
 // `contig` means row[k] == row0 + k (vector loads and stores allowed).
-// pos0 + k is the row's position in the table the ops see (RAND's sequence
-// number): the input row, or the output row after a filter.
+// pos0 + k * pstride is the row's position in the table the ops see (RAND's
+// sequence number): the input row, or the output row after a filter.
 template <int R>
 struct Rows {
   int64_t row[R];
@@ -964,6 +964,7 @@ struct Rows {
   int64_t row0;
   int64_t pos0;
   int nrows;
+  int pstride;
   bool contig;
 };
 
@@ -1131,12 +1132,35 @@ __device__ __forceinline__ void store_out(const DevOutCol& c, int64_t pos0, int 
   }
 }
 
-// Ops [plan.split, n_ops) over rows: perform_interpreter_operation (rows are
-// the positions [0, num_rows)) or the output phase of a filter (rows are the
-// selected row ids idx[0, count), outputs at the positions in that list).
-// Persistent: workgroups stride over tiles of BLOCK*R positions.
+// Stores one STORE op's results for the output positions pos0 + 64*k (the
+// lane-strided rows of a gather tile; bit k of `exist` = position k is
+// below the row count) and their validity (`vbits`): for each k the wave's
+// 64 positions are one aligned 64-bit validity span, built by one ballot
+// and written as two words by lanes 0 and 32.
+template <int R>
+__device__ __forceinline__ void store_out_strided(const DevOutCol& c, int64_t pos0, uint32_t exist,
+                                                  const uint64_t (&A)[R], uint32_t vbits, int lane) {
+#pragma unroll
+  for (int k = 0; k < R; ++k)
+    if ((exist >> k) & 1u) store_elem(c, pos0 + 64 * k, A[k]);
+  if (c.valid) {
+    const int64_t w0 = (pos0 - lane) >> 5;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const uint64_t b = __ballot((vbits >> k) & 1u);
+      if ((lane & 31) == 0 && ((exist >> k) & 1u))
+        gpw<uint32_t>(c.valid)[w0 + 2 * k + (lane >> 5)] = (uint32_t)(b >> (lane & 32));
+    }
+  }
+}
+
 // Ops [plan.split, n_ops) over one tile of 256*R positions [tile*TILE,
 // count): positions are rows (or, GATHER, indexes into the row-id list idx).
+// Rows: lane t of wave w owns positions tile*TILE + w*64*R + t*R + k
+// (consecutive, vector loads and stores) or, GATHER, tile*TILE + w*64*R +
+// 64*k + t, so that each gather instruction of the wave reads 64 consecutive
+// selected rows: neighbouring lanes share 128-byte lines, which are fetched
+// once per instruction rather than once per lane.
 // The register file is local to the tile: every register is written before
 // it is read, so nothing carries over from one tile to the next.
 template <int NREG, int R, bool GATHER>
@@ -1150,17 +1174,25 @@ __device__ __forceinline__ void vm_tile(const DevPlan& plan, int64_t tile, int64
   uint64_t vb = 0;
   LoadTemps<R> T;
   Rows<R> rs;
-  rs.pos0 = tile * TILE + (int64_t)threadIdx.x * R;
-  const int64_t left = count - rs.pos0;
-  const int n = left <= 0 ? 0 : (left >= R ? R : (int)left);
-  rs.nrows = n;
-  rs.exist = (1u << n) - 1u;
+  int n;
   if (GATHER) {
+    rs.pos0 = tile * TILE + (int64_t)(threadIdx.x >> 6) * (64 * R) + lane;
+    rs.pstride = 64;
+    const int64_t left = count - rs.pos0;  // positions pos0 + 64k < count
+    n = left <= 0 ? 0 : (left >= 64 * (R - 1) + 1 ? R : (int)((left + 63) >> 6));
+    rs.nrows = n;
+    rs.exist = (1u << n) - 1u;
     rs.contig = false;
     rs.row0 = 0;
 #pragma unroll
-    for (int k = 0; k < R; ++k) rs.row[k] = k < n ? (int64_t)gp<uint32_t>(idx)[rs.pos0 + k] : 0;
+    for (int k = 0; k < R; ++k) rs.row[k] = k < n ? (int64_t)gp<uint32_t>(idx)[rs.pos0 + 64 * k] : 0;
   } else {
+    rs.pos0 = tile * TILE + (int64_t)threadIdx.x * R;
+    rs.pstride = 1;
+    const int64_t left = count - rs.pos0;
+    n = left <= 0 ? 0 : (left >= R ? R : (int)left);
+    rs.nrows = n;
+    rs.exist = (1u << n) - 1u;
     rs.contig = true;
     rs.row0 = rs.pos0;
 #pragma unroll
@@ -1182,7 +1214,8 @@ __device__ __forceinline__ void vm_tile(const DevPlan& plan, int64_t tile, int64
 #pragma unroll
     for (int k = 0; k < R; ++k) A[k] = f[k][ia];
     const uint32_t VA = (uint32_t)(vb >> (ia * R));
-    store_out<R>(plan.out[op.aux], rs.pos0, n, A, VA & rs.exist, lane);
+    if (GATHER) store_out_strided<R>(plan.out[op.aux], rs.pos0, rs.exist, A, VA & rs.exist, lane);
+    else store_out<R>(plan.out[op.aux], rs.pos0, n, A, VA & rs.exist, lane);
   }
 }
 
@@ -1245,6 +1278,7 @@ template <int R>
 __device__ __forceinline__ void chunk_rows(Rows<R>& rs, int64_t row0, int64_t num_rows) {
   rs.row0 = row0;
   rs.pos0 = row0;
+  rs.pstride = 1;
   const int64_t left = num_rows - row0;
   rs.nrows = left <= 0 ? 0 : (left >= R ? R : (int)left);
   rs.exist = (1u << rs.nrows) - 1u;

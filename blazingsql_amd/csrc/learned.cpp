@@ -277,4 +277,62 @@ int bsql_order_by_prediction(bsql_program* pred, const float* sel, const float* 
   })
 }
 
+int bsql_order_conjuncts(bsql_program* pred, const int32_t* in_types, int n_in, const float* sel,
+                         const float* cost, int n_conj, int32_t* order_out) {
+  BSQL_TRY({
+    if (!pred || !in_types || n_in <= 0 || !sel || !cost || n_conj <= 0) fail("invalid arguments");
+    std::vector<int32_t> order((size_t)n_conj);
+    std::iota(order.begin(), order.end(), 0);
+    std::vector<DevRange> rs;
+    try {
+      rs = conjunct_ranges(*pred, in_types, n_in);
+    } catch (const Error&) {
+      rs.clear();  // not all range tests: the rank rule below
+    }
+    if ((int)rs.size() == n_conj && n_conj <= 16) {
+      // Expected cost of an order, the conjuncts' selectivities taken as
+      // independent: conjunct j after a prefix passing fraction p of the rows
+      // costs cost_j * (lines(p) + kLatency * slices(p)), where lines(p) is
+      // the fraction of its 128-byte lines still holding a live row (its
+      // loads are predicated per lane) and slices(p) the fraction of
+      // 1024-row slices still alive (one memory round trip per range per
+      // live slice). Exact minimum over all orders by dynamic programming on
+      // the set of conjuncts placed first.
+      constexpr double kLatency = 0.2;
+      const int k = n_conj;
+      std::vector<double> rows_per_line((size_t)k);
+      for (int j = 0; j < k; ++j) {
+        const int es = std::max(1, type_size(in_types[rs[(size_t)j].col]));
+        rows_per_line[(size_t)j] = 128.0 / es;
+      }
+      const size_t full = (size_t)1 << k;
+      std::vector<double> pass(full, 1.0), best(full, 0.0);
+      std::vector<int8_t> last(full, -1);
+      for (size_t m = 1; m < full; ++m) {
+        const int j0 = __builtin_ctzll(m);
+        pass[m] = pass[m & (m - 1)] * std::min(1.0, std::max(0.0, (double)sel[j0]));
+        best[m] = std::numeric_limits<double>::infinity();
+        for (int j = 0; j < k; ++j) {
+          if (!((m >> j) & 1u)) continue;
+          const size_t before = m & ~((size_t)1 << j);
+          const double p = pass[before];
+          const double lines = 1.0 - std::pow(1.0 - p, rows_per_line[(size_t)j]);
+          const double slices = 1.0 - std::pow(1.0 - p, 1024.0);
+          const double c = best[before] + std::max(0.0, (double)cost[j]) * (lines + kLatency * slices);
+          if (c < best[m]) {
+            best[m] = c;
+            last[m] = (int8_t)j;
+          }
+        }
+      }
+      for (size_t m = full - 1, pos = (size_t)k; m; m &= ~((size_t)1 << last[m])) order[--pos] = last[m];
+    } else {
+      auto rank = [&](int j) { return (double)cost[j] / std::max(1e-3, 1.0 - (double)sel[j]); };
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rank(a) < rank(b); });
+    }
+    if (order_out) std::copy(order.begin(), order.end(), order_out);
+    return bsql_program_set_conjunct_order(pred, order.data(), n_conj);
+  })
+}
+
 }  // extern "C"

@@ -74,8 +74,9 @@ class Mlp:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h:
-            L.lib().bsql_mlp_free(h)
+        lib = getattr(L, "lib", None)   # module globals are gone at interpreter exit
+        if h and lib is not None:
+            lib().bsql_mlp_free(h)
             self.handle = None
 
 
@@ -106,14 +107,25 @@ def order_by(pred, selectivity, cost):
     L.check(L.lib().bsql_order_by_prediction(pred.handle, sel.ctypes.data, c.ctypes.data, len(sel)))
 
 
+def order_for_scan(pred, types, selectivity, cost):
+    """Sets the order of least expected cost for the range-test selection
+    (bsql_order_conjuncts) and returns it."""
+    sel = np.ascontiguousarray(selectivity, dtype=np.float32)
+    c = np.ascontiguousarray(cost, dtype=np.float32)
+    t = np.ascontiguousarray(types, dtype=np.int32)
+    out = np.zeros(len(sel), dtype=np.int32)
+    L.check(L.lib().bsql_order_conjuncts(pred.handle, t.ctypes.data, len(t), sel.ctypes.data, c.ctypes.data,
+                                         len(sel), out.ctypes.data))
+    return [int(x) for x in out]
+
+
 def learned_order(mlp, pred, types, cols, num_rows):
     """Sketches the columns, predicts each conjunct and sets the order.
     Returns (selectivity, cost, order) as predicted."""
     f = conjunct_features(pred, types, sketch(cols, num_rows))
     sel, cost = mlp.predict(f)
-    order_by(pred, sel, cost)
-    rank = cost / np.maximum(1e-3, 1.0 - sel)
-    return sel, cost, list(np.argsort(rank, kind="stable"))
+    order = order_for_scan(pred, types, sel, cost)
+    return sel, cost, order
 
 
 # ------------------------------------------------------------ synthetic data

@@ -363,6 +363,16 @@ int bsql_conjunct_features(const bsql_program* pred, const int32_t* in_types, in
                            float* features, int cap);
 /* Sets the conjunct order: ascending cost / (1 - selectivity). */
 int bsql_order_by_prediction(bsql_program* pred, const float* selectivity, const float* cost, int n_conj);
+/* Sets the conjunct order of least expected cost for the range-test
+ * selection, over the table schema in_types: a conjunct after others that
+ * pass fraction p of the rows costs cost * (fraction of its 128-byte lines
+ * with a live row + 0.2 * fraction of 1024-row slices alive), selectivities
+ * taken as independent; exact over all orders (up to 16 conjuncts; more, or
+ * conjuncts that are not all range tests: the rank rule above). The order
+ * is also written to order_out (n_conj entries) unless NULL. No counterpart
+ * in the reference (its plans keep Calcite's order). */
+int bsql_order_conjuncts(bsql_program* pred, const int32_t* in_types, int n_in, const float* selectivity,
+                         const float* cost, int n_conj, int32_t* order_out);
 
 /* ---- diagnostics --------------------------------------------------------- */
 /* Message of the last error on the calling thread. */

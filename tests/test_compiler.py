@@ -170,3 +170,52 @@ def test_benchmark_plans_compile(bsql):
     proj = bsql.Program.compile(["*(*($1, -(1, $2)), +(1, $3))", "*($1, $2)"],
                                 [INT32, FLOAT64, FLOAT64, FLOAT64, INT32])
     assert proj.num_outputs == 2
+
+
+def _scan_cost(order, sel, cost, rows_per_line):
+    """Expected cost of a conjunct order under the range-test selection's
+    model, summed directly over the order (no subsets)."""
+    total, p = 0.0, 1.0
+    for j in order:
+        total += cost[j] * ((1 - (1 - p) ** rows_per_line[j]) + 0.2 * (1 - (1 - p) ** 1024))
+        p *= sel[j]
+    return total
+
+
+def test_order_conjuncts_is_least_cost_order(bsql):
+    """bsql_order_conjuncts returns the cheapest order of every permutation
+    (brute force over 5 conjuncts, random selectivities and costs); a lone
+    very selective term goes first; it rejects bad arguments."""
+    import itertools
+
+    import numpy as np
+
+    from blazingsql_amd import learned
+    types = [FLOAT64, INT32, FLOAT64, INT32, FLOAT64]
+    terms = [">($0, 0.5:DOUBLE)", "<($1, 7)", "<($2, 0.1:DOUBLE)", ">($3, 100)", "<($4, 3.0:DOUBLE)"]
+    text = "AND(" + ", ".join(terms) + ")"
+    rpl = [16, 32, 16, 32, 16]
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        sel = rng.uniform(0.001, 1.0, 5).astype(np.float32)
+        cost = rng.uniform(50, 120, 5).astype(np.float32)
+        pred = bsql.Program.compile(text, types)
+        got = learned.order_for_scan(pred, types, sel, cost)
+        assert sorted(got) == list(range(5))
+        best = min(_scan_cost(o, sel, cost, rpl) for o in itertools.permutations(range(5)))
+        assert _scan_cost(got, sel, cost, rpl) <= best * (1 + 1e-9)
+    pred = bsql.Program.compile(text, types)
+    order = learned.order_for_scan(pred, types, [0.9, 0.9, 0.001, 0.9, 0.9], [100.0] * 5)
+    assert order[0] == 2
+    with pytest.raises(BsqlError):
+        learned.order_for_scan(pred, types, [0.5] * 4, [1.0] * 4)
+
+
+def test_order_conjuncts_rank_rule_fallback(bsql):
+    """Conjuncts that are not all column-versus-literal range tests fall back
+    to ascending cost / (1 - selectivity)."""
+    from blazingsql_amd import learned
+    types = [INT32, INT32, INT32]
+    pred = bsql.Program.compile("AND(>(+($0, $1), 3), <($2, 5), =(MOD($0, 2), 0))", types)
+    order = learned.order_for_scan(pred, types, [0.5, 0.9, 0.1], [10.0, 10.0, 10.0])
+    assert order == [2, 0, 1]
