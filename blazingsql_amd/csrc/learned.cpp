@@ -300,14 +300,35 @@ int bsql_order_conjuncts(bsql_program* pred, const int32_t* in_types, int n_in, 
       // the set of conjuncts placed first.
       constexpr double kLatency = 0.2;
       const int k = n_conj;
-      std::vector<double> rows_per_line((size_t)k);
+      std::vector<int> esc((size_t)k);  // log2 of the element size: 128 >> esc rows per line
       for (int j = 0; j < k; ++j) {
-        const int es = std::max(1, type_size(in_types[rs[(size_t)j].col]));
-        rows_per_line[(size_t)j] = 128.0 / es;
+        const int es = type_size(in_types[rs[(size_t)j].col]);
+        esc[(size_t)j] = es >= 8 ? 3 : (es >= 4 ? 2 : (es >= 2 ? 1 : 0));
       }
       const size_t full = (size_t)1 << k;
       std::vector<double> pass(full, 1.0), best(full, 0.0);
       std::vector<int8_t> last(full, -1);
+      // per subset placed first: lines(p) for 128, 64, 32, 16 rows per line
+      // and slices(p), p = its pass fraction (powers by repeated squaring)
+      std::vector<double> fac((size_t)5 * full);
+      auto factors = [&](size_t m) {
+        const double q = 1.0 - pass[m];
+        double q16 = q * q;
+        q16 *= q16;
+        q16 *= q16;
+        q16 *= q16;
+        const double q32 = q16 * q16, q64 = q32 * q32, q128 = q64 * q64;
+        double q1024 = q128 * q128;
+        q1024 *= q1024;
+        q1024 *= q1024;
+        double* f = &fac[5 * m];
+        f[0] = 1.0 - q128;
+        f[1] = 1.0 - q64;
+        f[2] = 1.0 - q32;
+        f[3] = 1.0 - q16;
+        f[4] = 1.0 - q1024;
+      };
+      factors(0);
       for (size_t m = 1; m < full; ++m) {
         const int j0 = __builtin_ctzll(m);
         pass[m] = pass[m & (m - 1)] * std::min(1.0, std::max(0.0, (double)sel[j0]));
@@ -315,15 +336,14 @@ int bsql_order_conjuncts(bsql_program* pred, const int32_t* in_types, int n_in, 
         for (int j = 0; j < k; ++j) {
           if (!((m >> j) & 1u)) continue;
           const size_t before = m & ~((size_t)1 << j);
-          const double p = pass[before];
-          const double lines = 1.0 - std::pow(1.0 - p, rows_per_line[(size_t)j]);
-          const double slices = 1.0 - std::pow(1.0 - p, 1024.0);
-          const double c = best[before] + std::max(0.0, (double)cost[j]) * (lines + kLatency * slices);
+          const double* f = &fac[5 * before];
+          const double c = best[before] + std::max(0.0, (double)cost[j]) * (f[esc[(size_t)j]] + kLatency * f[4]);
           if (c < best[m]) {
             best[m] = c;
             last[m] = (int8_t)j;
           }
         }
+        factors(m);
       }
       for (size_t m = full - 1, pos = (size_t)k; m; m &= ~((size_t)1 << last[m])) order[--pos] = last[m];
     } else {
