@@ -343,10 +343,12 @@ def columns_array(cols):
 
 
 def _stream_ptr(stream):
+    """hipStream_t of `stream`: None = torch's current stream, a
+    torch.cuda.Stream, or a raw handle (int)."""
     if stream is None:
         import torch
         return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(getattr(stream, "cuda_stream", stream))
 
 
 def perform_interpreter_operation(prog, in_cols, out_cols, num_rows=None, stream=None):
