@@ -289,13 +289,16 @@ def run_learned(args):
     {0, 0.5, 0.9}); conjunctions of 4-8 range terms with selectivities
     log-uniform in [0.001, 0.9], written in a random (static plan) order. The
     MLP is trained on the GPU on records the exact interpreter makes on
-    synthetic batches, then orders each predicate's terms; the filter+project
-    output is compared bit for bit with the static order's."""
+    synthetic batches, then orders each predicate's terms and bounds its row
+    count (the smallest predicted term selectivity), and the learned call
+    writes into outputs pre-sized for that bound (a count beyond it costs a
+    second, full-size call); the filter+project output is compared bit for
+    bit with the static order's."""
     import numpy as np
     import torch
     import blazingsql_amd as bsql
     from blazingsql_amd import learned
-    from blazingsql_amd.interop import DeviceColumn, filter_project
+    from blazingsql_amd.interop import DeviceColumn, filter_project, filter_project_sized
     t0 = time.perf_counter()
     X, Y = learned.collect_records(args.records, rows=1 << 18, seed=args.seed)
     t_collect = time.perf_counter() - t0
@@ -311,15 +314,30 @@ def run_learned(args):
     cur = torch.cuda.current_stream()
     a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
-    def timed(pred, proj, cols, outs, cnt):
+    def timed(pred, proj, cols, outs, cnt, bound=None):
+        """ms per call of the plain path, or (bound) of the learned path:
+        outputs pre-sized for `bound` rows, and when the count exceeds it
+        (known from the first call; the count is the same every call) a
+        second, full-size call after each sized one."""
+        def call():
+            if bound is None:
+                filter_project(pred, proj, cols, outs, cnt, n)
+                return
+            filter_project_sized(pred, proj, cols, outs, cnt, bound, n)
+            if overflow:
+                filter_project(pred, proj, cols, outs, cnt, n)
+        overflow = False
+        if bound is not None:
+            filter_project_sized(pred, proj, cols, outs, cnt, bound, n)
+            overflow = int(cnt.item()) > bound
         for _ in range(2):
-            filter_project(pred, proj, cols, outs, cnt, n)
+            call()
         a.record(cur)
         for _ in range(args.steps):
-            filter_project(pred, proj, cols, outs, cnt, n)
+            call()
         z.record(cur)
         torch.cuda.synchronize()
-        return a.elapsed_time(z) / args.steps
+        return a.elapsed_time(z) / args.steps, overflow
 
     rows = []
     exact = True
@@ -332,11 +350,12 @@ def run_learned(args):
             text, terms = learned.random_predicate(rng, int(rng.integers(4, 9)))
             proj = bsql.Program.compile(proj_text, types)
             static = bsql.Program.compile(text, types)
-            t_static = timed(static, proj, cols, ref_outs, cnt)
+            t_static, _ = timed(static, proj, cols, ref_outs, cnt)
             m_static = int(cnt.item())
             lp = bsql.Program.compile(text, types)
             sel, cost, order = learned.learned_order(mlp, lp, types, cols, n)
-            t_learned = timed(lp, proj, cols, outs, cnt)
+            bound = learned.row_bound(sel, n)
+            t_learned, overflow = timed(lp, proj, cols, outs, cnt, bound)
             m_learned = int(cnt.item())
             same = m_static == m_learned and all(
                 torch.equal(o.data[:m_learned], r.data[:m_learned]) for o, r in zip(outs, ref_outs))
@@ -345,10 +364,11 @@ def run_learned(args):
             true_sel, true_t = learned.measure_terms(terms, cols, n, reps=2)
             op = bsql.Program.compile(text, types)
             learned.order_for_scan(op, types, true_sel, true_t)
-            t_oracle = timed(op, proj, cols, outs, cnt)
+            t_oracle, _ = timed(op, proj, cols, outs, cnt, learned.row_bound(true_sel, n))
             rows.append({"rho": rho, "terms": len(terms), "t_static_ms": t_static, "t_learned_ms": t_learned,
                          "t_oracle_order_ms": t_oracle, "sel_mae": float(np.mean(np.abs(sel - true_sel))),
-                         "selected": m_learned, "exact": bool(same)})
+                         "selected": m_learned, "row_bound": bound, "overflow": bool(overflow),
+                         "exact": bool(same)})
         del cols, outs, ref_outs
     sp = np.array([r["t_static_ms"] / r["t_learned_ms"] for r in rows])
     regret = np.array([r["t_learned_ms"] / r["t_oracle_order_ms"] - 1.0 for r in rows])
@@ -361,6 +381,7 @@ def run_learned(args):
         "config": {"workload": f"{n} rows x 8 columns (4 float64, 4 int32), Gaussian copula rho in "
                                f"{{0, 0.5, 0.9}}, {args.predicates} predicates per rho of 4-8 range terms",
                    "exact": bool(exact), "speedup_min": float(sp.min()), "speedup_max": float(sp.max()),
+                   "overflows": int(sum(r["overflow"] for r in rows)),
                    "selectivity_mae": float(np.mean([r["sel_mae"] for r in rows])),
                    "time_regret_vs_exact_order": float(np.mean(regret)),
                    "training_records": int(len(X)), "collect_s": t_collect, "train_s": t_train,

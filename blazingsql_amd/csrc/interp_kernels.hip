@@ -1983,7 +1983,9 @@ static hipError_t launch_compact_gather(const DevPlan& plan, const DevPlan* d_pl
   int NR, RR;
   if (!pick(nreg, rpt, &NR, &RR)) return hipErrorInvalidValue;
 #define X(N, RPT) \
-  if (NR == N && RR == RPT) return launch_gather_t<N, RPT>(d_plan, plan.num_rows, idx, d_count, capacity, stream);
+  if (NR == N && RR == RPT)                                                                                \
+    return launch_gather_t<N, RPT>(d_plan, plan.num_rows < capacity ? plan.num_rows : capacity, idx, d_count, \
+                                   capacity, stream);
   BSQL_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;

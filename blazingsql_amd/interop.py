@@ -379,6 +379,18 @@ def filter_project(pred, proj, in_cols, out_cols, d_count, num_rows=None, stream
                                         ctypes.c_void_p(d_count.data_ptr()), _stream_ptr(stream)))
 
 
+def filter_project_sized(pred, proj, in_cols, out_cols, d_count, capacity, num_rows=None, stream=None):
+    """filter_project into outputs that hold `capacity` rows: the first
+    min(count, capacity) selected rows are written, *d_count gets the full
+    count (count > capacity: run again with larger outputs)."""
+    if num_rows is None:
+        num_rows = in_cols[0].size if in_cols else 0
+    ins, outs = columns_array(in_cols), columns_array(out_cols)
+    L.check(L.lib().bsql_filter_project_sized(pred.handle, proj.handle, ins, len(in_cols), outs, len(out_cols),
+                                              num_rows, capacity, ctypes.c_void_p(d_count.data_ptr()),
+                                              _stream_ptr(stream)))
+
+
 class Batch:
     """Many (program, columns, rows) batches run as one persistent launch
     (bsql_batch_*). The launch allocates and copies nothing, so it can be

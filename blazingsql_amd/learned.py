@@ -119,6 +119,14 @@ def order_for_scan(pred, types, selectivity, cost):
     return [int(x) for x in out]
 
 
+def row_bound(selectivity, num_rows, slack=1.25, floor=4096):
+    """Output rows to pre-size a conjunction's compaction for: it passes at
+    most the rows of its most selective term, so the smallest predicted term
+    selectivity (with slack) bounds the count whatever the correlation."""
+    s = float(np.min(np.asarray(selectivity, dtype=np.float64))) if len(selectivity) else 1.0
+    return int(min(num_rows, np.ceil(s * slack * num_rows) + floor))
+
+
 def learned_order(mlp, pred, types, cols, num_rows):
     """Sketches the columns, predicts each conjunct and sets the order.
     Returns (selectivity, cost, order) as predicted."""

@@ -151,3 +151,16 @@ def test_filter_project_sized_overflow(bsql, gpu):
     torch.cuda.synchronize()
     assert int(cnt2.item()) == m
     np.testing.assert_array_equal(small[0].to_numpy(cap)[0], full[0].to_numpy(cap)[0])
+    # capacities around the count (the output launch is sized by the
+    # capacity): every row below min(count, capacity) is written
+    from blazingsql_amd.interop import filter_project_sized
+    pred2 = bsql.Program.compile(">($0, 995)", [4])
+    filter_project(pred2, proj, cols, full, cnt, n)
+    m2 = int(cnt.item())
+    for cap2 in (0, 1, m2 - 1, m2, m2 + 1, 4 * m2 + 3):
+        out2 = [DeviceColumn.empty(4, max(cap2, 1))]
+        cnt3 = torch.full((1,), -1, dtype=torch.int64, device="cuda")
+        filter_project_sized(pred2, proj, cols, out2, cnt3, cap2, n)
+        assert int(cnt3.item()) == m2, cap2
+        w = min(m2, cap2)
+        np.testing.assert_array_equal(out2[0].to_numpy(w)[0], full[0].to_numpy(w)[0])
