@@ -986,7 +986,7 @@ __device__ __forceinline__ void gather_rows(const DevInCol& c, const Rows<R>& rs
       case BSQL_UINT16: v[k] = gp<uint16_t>(c.data)[r]; break;
       case BSQL_UINT32: v[k] = gp<uint32_t>(c.data)[r]; break;
       case BSQL_FLOAT32: v[k] = bits((double)gp<float>(c.data)[r]); break;
-      default: v[k] = gp<uint64_t>(c.data)[r]; break;  // 8-byte types
+      default: v[k] = __builtin_nontemporal_load(gp<uint64_t>(c.data) + r); break;  // 8-byte types
     }
   }
 }
@@ -1185,7 +1185,8 @@ __device__ __forceinline__ void vm_tile(const DevPlan& plan, int64_t tile, int64
     rs.contig = false;
     rs.row0 = 0;
 #pragma unroll
-    for (int k = 0; k < R; ++k) rs.row[k] = k < n ? (int64_t)gp<uint32_t>(idx)[rs.pos0 + 64 * k] : 0;
+    for (int k = 0; k < R; ++k)
+      rs.row[k] = k < n ? (int64_t)__builtin_nontemporal_load(gp<uint32_t>(idx) + rs.pos0 + 64 * k) : 0;
   } else {
     rs.pos0 = tile * TILE + (int64_t)threadIdx.x * R;
     rs.pstride = 1;
