@@ -238,7 +238,9 @@ __device__ __forceinline__ uint32_t load_valid(const DevInCol& c, int64_t r0, in
   return n >= R ? m : (m & ((1u << n) - 1u));
 }
 
-// static_cast of slot bits to the output column type (interpreter_ops.cuh:329-395)
+// static_cast of slot bits to the output column type (interpreter_ops.cuh:329-395);
+// NT: 8-byte elements with a non-temporal store
+template <bool NT = false>
 __device__ __forceinline__ void store_elem(const DevOutCol& c, int64_t row, uint64_t b) {
   switch (c.type) {
     case BSQL_INT8: case BSQL_UINT8: gpw<uint8_t>(c.data)[row] = (uint8_t)b; break;
@@ -247,7 +249,10 @@ __device__ __forceinline__ void store_elem(const DevOutCol& c, int64_t row, uint
     case BSQL_INT32: case BSQL_UINT32: case BSQL_TIMESTAMP_DAYS: case BSQL_DURATION_DAYS:
       gpw<uint32_t>(c.data)[row] = (uint32_t)b; break;
     case BSQL_FLOAT32: gpw<float>(c.data)[row] = (float)as_f(b); break;
-    default: gpw<uint64_t>(c.data)[row] = b; break;  // int64, uint64, float64, 8-byte ticks
+    default:  // int64, uint64, float64, 8-byte ticks
+      if (NT) __builtin_nontemporal_store(b, gpw<uint64_t>(c.data) + row);
+      else gpw<uint64_t>(c.data)[row] = b;
+      break;
   }
 }
 
@@ -1142,7 +1147,7 @@ __device__ __forceinline__ void store_out_strided(const DevOutCol& c, int64_t po
                                                   const uint64_t (&A)[R], uint32_t vbits, int lane) {
 #pragma unroll
   for (int k = 0; k < R; ++k)
-    if ((exist >> k) & 1u) store_elem(c, pos0 + 64 * k, A[k]);
+    if ((exist >> k) & 1u) store_elem<true>(c, pos0 + 64 * k, A[k]);
   if (c.valid) {
     const int64_t w0 = (pos0 - lane) >> 5;
 #pragma unroll
