@@ -73,12 +73,7 @@ def cpu_baseline(sample_rows, seed):
             oracle.HostColumn(10, tax), oracle.HostColumn(3, ship)]
 
     def full(p):
-        d = p.to_oracle_dict()
-        used = p.used_columns()
-        remap = lambda s: s if s < 0 else (used[s] if s < len(used) else s - len(used) + len(TYPES))
-        for k in ("left", "right", "out", "final_out"):
-            d[k] = [remap(s) for s in d[k]]
-        return d
+        return oracle.full_table_program(p, len(TYPES))
 
     pred = full(bsql.Program.compile(PRED, TYPES))
     proj = full(bsql.Program.compile(PROJ, TYPES))
@@ -476,12 +471,7 @@ def string_cpu_baseline(sample_rows, seed):
     ot = [proj.output_type(STR_TYPES, k) for k in range(proj.num_outputs)]
 
     def full(p):
-        d = p.to_oracle_dict()
-        used = p.used_columns()
-        remap = lambda s: s if s < 0 else (used[s] if s < len(used) else s - len(used) + len(STR_TYPES))
-        for k in ("left", "right", "out", "final_out"):
-            d[k] = [remap(s) for s in d[k]]
-        return d
+        return oracle.full_table_program(p, len(STR_TYPES))
 
     pj = full(proj)
     preds = [full(bsql.Program.compile(p, STR_TYPES)) for p in STR_PREDS]

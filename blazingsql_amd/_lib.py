@@ -3,6 +3,7 @@
 The library is built in-tree by blazingsql_amd/build.py. Loading fails loudly
 if it is missing: there is no fallback implementation.
 """
+import atexit
 import ctypes
 import os
 
@@ -66,12 +67,18 @@ STRFN = {"CONCAT": 1, "UPPER": 2, "LOWER": 3, "INITCAP": 4, "REVERSE": 5, "REPLA
          "CASE": 8, "FILL": 9, "REGEXP_REPLACE": 10}
 
 
+class PlanInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("n_ops", "n_regs", "n_spill", "n_spill_ops", "n_fill_ops",
+                                              "n_inputs", "n_load_issues", "max_loads_in_flight")]
+
+
 class LaunchConfig(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_int32), ("rows_per_thread", ctypes.c_int32),
                 ("flags", ctypes.c_int32)]
 
 
-LAUNCH_NO_RANGES, LAUNCH_LAZY_LOADS = 1, 2
+LAUNCH_NO_RANGES, LAUNCH_LAZY_LOADS, LAUNCH_SPILL_WIDE = 1, 2, 4
+ROOT_INTERPRETED, ROOT_COLUMN, ROOT_LITERAL = 0, 1, 2
 SKETCH_DIM, FEATURE_DIM, MLP_HIDDEN = 19, 24, 64
 
 
@@ -91,6 +98,15 @@ SIGNATURES = {
     "bsql_program_num_ops": (_I, [_P]),
     "bsql_program_num_outputs": (_I, [_P]),
     "bsql_program_num_inputs": (_I, [_P]),
+    "bsql_program_num_interpreted": (_I, [_P]),
+    "bsql_program_root": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                               ctypes.POINTER(Scalar)]),
+    "bsql_program_num_parts": (_I, [_P]),
+    "bsql_program_part": (_I, [_P, _I, ctypes.POINTER(_P), _P, _I]),
+    "bsql_expression_rewrite": (_I, [ctypes.c_char_p, _I, _P, _I, _P, ctypes.c_size_t,
+                                     ctypes.POINTER(ctypes.c_size_t)]),
+    "bsql_set_current_timestamp": (_I, [ctypes.c_char_p]),
+    "bsql_program_plan_info": (_I, [_P, _P, _I, ctypes.POINTER(PlanInfo)]),
     "bsql_program_arrays": (_I, [_P, _P, _P, _P, _P, _P]),
     "bsql_program_scalar": (_I, [_P, _I, _I, ctypes.POINTER(Scalar)]),
     "bsql_program_output_type": (_I, [_P, _P, _I, _I, ctypes.POINTER(ctypes.c_int32)]),
@@ -149,6 +165,23 @@ SIGNATURES = {
 }
 
 _lib = None
+
+# Set when the interpreter starts to exit: finalizers then leave native
+# objects to the process teardown instead of freeing device memory, since
+# the HIP runtime (or a profiler wrapping it) may already be shutting down.
+_exiting = False
+
+
+def _mark_exiting():
+    global _exiting
+    _exiting = True
+
+
+atexit.register(_mark_exiting)
+
+
+def exiting():
+    return _exiting
 
 
 def lib():

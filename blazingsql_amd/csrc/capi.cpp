@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <map>
@@ -17,23 +19,38 @@
 
 namespace bsql {
 
-hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt,
+hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt, int block,
                           hipStream_t stream);
-int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt);
+int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt, int block);
 int64_t string_tiles(int64_t capacity);
-int batch_tile_rows(int nreg, int rpt);
+int batch_tile_rows(int nreg, int rpt, int block);
+bool variant_fits(int nreg, int rpt, int block, int n_spill);
+int variant_rows(int nreg, int rpt, int block, int n_spill);
 hipError_t launch_batches(const DevPlan* plans, const int64_t* tile_start, int n_batches, int64_t n_items,
-                          int nreg, int rpt, hipStream_t stream);
+                          int nreg, int rpt, int block, int n_spill, hipStream_t stream);
 hipError_t launch_string_output(const uint64_t* slices, const uint32_t* valid, int32_t* offsets,
                                 const uint8_t* src_chars, uint8_t* dst_chars, const int64_t* d_count,
                                 int64_t count, int64_t capacity, uint64_t* status, hipStream_t stream);
-hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int sel_rpt, int rpt,
+hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int sel_rpt, int rpt, int block,
                           uint64_t* sel, uint32_t* idx, int64_t* d_count, int64_t capacity,
                           hipStream_t stream, int phases);
 
 namespace {
 thread_local std::string g_last_error;
+std::atomic<uint64_t> g_next_id{1};
 }  // namespace
+
+uint64_t next_program_id() { return g_next_id.fetch_add(1); }
+
+namespace {
+std::atomic<bool> g_exiting{false};
+void mark_exiting() { g_exiting = true; }
+// Registered when the library is loaded, after the HIP runtime it links
+// against, so it runs before that runtime's own exit handlers.
+__attribute__((constructor)) void register_exit_flag() { std::atexit(mark_exiting); }
+}  // namespace
+
+bool process_exiting() { return g_exiting.load(); }
 
 int set_error(int code, const std::string& msg) {
   g_last_error = msg;
@@ -46,16 +63,30 @@ int device_error(hipError_t e, const char* where) {
   return set_error(BSQL_ERR_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
 }
 
+void reject_capture(void* stream, const char* entry) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing((hipStream_t)stream, &st) == hipSuccess && st != hipStreamCaptureStatusNone)
+    fail(std::string(entry) +
+         " cannot be captured in a graph (it may grow its scratch and copies its launch plan from host "
+         "memory); bsql_filter_project_noalloc can");
+}
+
 namespace {
 
-// Per-stream scratch: the device copy of the launch plan, then the compaction
-// tile status words + ticket. One buffer per stream, so that concurrent calls
-// on different streams never share it.
-constexpr int64_t kPlanWords = (int64_t)((sizeof(DevPlan) + 255) / 256 * 256 / 8);
+// Per-stream scratch, one buffer per (device, stream) so that concurrent
+// calls on different streams never share it. Layout of a filter call:
+// [selection ballots, slice counts, block totals | selected row ids (uint32)
+// | string assembly status | string output slices | launch plan + ops]; the
+// first two depend only on the predicate and the row count, so a selection
+// and several output launches of the same call find them in place.
+// Growing is stream-ordered (hipFreeAsync / hipMallocAsync on the stream):
+// work still queued on the stream keeps reading the old buffer, and the host
+// never waits.
 struct Scratch {
   uint64_t* p = nullptr;
   int64_t words = 0;
-  std::vector<char> plan;  // the plan bytes last copied to p (stream order)
+  std::vector<char> plan;  // the plan blob last copied into this buffer (stream order)
+  int64_t plan_at = -1;    // where (words)
 };
 std::mutex g_scratch_mu;
 std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
@@ -68,40 +99,60 @@ hipError_t get_scratch(hipStream_t stream, int64_t words, uint64_t** out) {
   Scratch& s = g_scratch[{dev, stream}];
   if (!s.p || s.words < words) {
     if (s.p) {
-      // the previous buffer may still be read by queued work on this stream
-      e = hipStreamSynchronize(stream);
+      e = hipFreeAsync(s.p, stream);
       if (e != hipSuccess) return e;
-      (void)hipFree(s.p);
-      s.plan.clear();
       s.p = nullptr;
       s.words = 0;
+      s.plan.clear();
+      s.plan_at = -1;
     }
-    const int64_t n = (words < 4096 ? 4096 : words + words / 4) + kPlanWords;
-    e = hipMalloc((void**)&s.p, (size_t)n * sizeof(uint64_t));
+    const int64_t n = words < 4096 ? 4096 : words + words / 4;
+    e = hipMallocAsync((void**)&s.p, (size_t)n * sizeof(uint64_t), stream);
     if (e != hipSuccess) return e;
-    s.words = n - kPlanWords;
+    s.words = n;
   }
   *out = s.p;
   return hipSuccess;
 }
 
-// Copies the launch plan into scratch (stream-ordered). The stream's own
-// scratch remembers the plan last copied there: every later copy to it is
-// queued on the same stream, so an identical plan (the same program over the
+// Words of a plan blob: the DevPlan, then its ops.
+int64_t plan_words(size_t n_ops) {
+  return (int64_t)((sizeof(DevPlan) + n_ops * sizeof(DevOp) + 255) / 256 * 256 / 8);
+}
+
+// The plan blob (DevPlan with its op pointer aimed at `at`, then the ops).
+std::vector<char> plan_blob(const DevPlan& plan, const std::vector<DevOp>& ops, const void* at) {
+  std::vector<char> blob(sizeof(DevPlan) + ops.size() * sizeof(DevOp));
+  DevPlan p = plan;
+  p.ops = (const DevOp*)((const char*)at + sizeof(DevPlan));
+  std::memcpy(blob.data(), &p, sizeof(DevPlan));
+  std::memcpy(blob.data() + sizeof(DevPlan), ops.data(), ops.size() * sizeof(DevOp));
+  return blob;
+}
+
+// Copies the plan blob to `dst` in the stream's scratch (stream-ordered). The
+// scratch remembers the blob last copied there: every later copy to it is
+// queued on the same stream, so an identical blob (the same program over the
 // same columns again) is already in place and is not copied again.
-hipError_t upload_plan(const DevPlan& plan, uint64_t* scratch, hipStream_t stream) {
-  const char* bytes = (const char*)&plan;
+hipError_t upload_plan(const DevPlan& plan, const std::vector<DevOp>& ops, uint64_t* scratch, int64_t at,
+                       hipStream_t stream) {
+  uint64_t* dst = scratch + at;
+  std::vector<char> blob = plan_blob(plan, ops, dst);
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) dev = -1;
   std::lock_guard<std::mutex> lk(g_scratch_mu);
   auto it = g_scratch.find({dev, stream});
   Scratch* own = it != g_scratch.end() && it->second.p == scratch ? &it->second : nullptr;
-  if (own && own->plan.size() == sizeof(DevPlan) && std::memcmp(own->plan.data(), bytes, sizeof(DevPlan)) == 0)
-    return hipSuccess;
-  const hipError_t e = hipMemcpyAsync(scratch, &plan, sizeof(DevPlan), hipMemcpyHostToDevice, stream);
+  if (own && own->plan_at == at && own->plan == blob) return hipSuccess;
+  const hipError_t e = hipMemcpyAsync(dst, blob.data(), blob.size(), hipMemcpyHostToDevice, stream);
   if (own) {
-    if (e == hipSuccess) own->plan.assign(bytes, bytes + sizeof(DevPlan));
-    else own->plan.clear();
+    if (e == hipSuccess) {
+      own->plan = std::move(blob);
+      own->plan_at = at;
+    } else {
+      own->plan.clear();
+      own->plan_at = -1;
+    }
   }
   return e;
 }
@@ -148,6 +199,8 @@ hipError_t ensure_strings(Program& p) {
   return hipSuccess;
 }
 
+}  // namespace
+
 DevInCol to_dev_in(const bsql_column& c) {
   DevInCol d{};
   d.type = (int16_t)c.type_id;
@@ -181,23 +234,34 @@ void check_inputs(const bsql_column* in, int n_in, int64_t num_rows) {
   }
 }
 
-std::string cache_key(int mode, const bsql_program* pred, const bsql_program* proj,
-                      const bsql_column* in, int n_in, const bsql_column* out, int n_out) {
-  std::string k = std::to_string(mode) + "|" + std::to_string((uintptr_t)pred) + "|" +
-                  std::to_string((uintptr_t)proj) + "|";
+namespace {
+
+uint64_t uid_of(const Program* p) { return p ? static_cast<const bsql_program*>(p)->uid : 0; }
+
+std::string cache_key(const LowerOptions& opt, const bsql_column* in, int n_in, const bsql_column* out,
+                      int n_out) {
+  std::string k = std::to_string((int)opt.mode) + "|" + std::to_string(uid_of(opt.pred)) + "|" +
+                  std::to_string(uid_of(opt.proj)) + "|b" + std::to_string(opt.reg_budget) + "|";
   for (int i = 0; i < n_in; ++i) k += std::to_string(in[i].type_id) + ",";
   k += "|";
   for (int i = 0; i < n_out; ++i) k += std::to_string(out ? out[i].type_id : 0) + ",";
-  if (pred) {
+  if (opt.out_subset) {
+    k += "|s";
+    for (int c : *opt.out_subset) k += std::to_string(c) + ",";
+  }
+  if (opt.pred) {
     k += "|o";
-    for (int32_t c : pred->conj_order) k += std::to_string(c) + ",";
-    k += "|f" + std::to_string(pred->launch.flags);
+    for (int32_t c : opt.pred->conj_order) k += std::to_string(c) + ",";
+    k += "|f" + std::to_string(opt.pred->launch.flags);
   }
   return k;
 }
 
+}  // namespace
+
 // Output column checks shared by the launching entries (rows = capacity).
 void check_outputs(const bsql_column* out, int n_out, int64_t rows) {
+  if (n_out < 0 || (n_out > 0 && !out)) fail("invalid output columns");
   for (int k = 0; k < n_out; ++k) {
     const bsql_column& c = out[k];
     if (c.offset != 0) fail("output columns must have offset 0");
@@ -211,6 +275,8 @@ void check_outputs(const bsql_column* out, int n_out, int64_t rows) {
     }
   }
 }
+
+namespace {
 
 // String outputs: the VM stores slices into scratch; each then needs the
 // chars buffer of its source column to fit in the output's chars.
@@ -244,21 +310,58 @@ hipError_t finish_string_outputs(const Lowered& L, const bsql_column* in, const 
   return hipSuccess;
 }
 
-const Lowered& lowered_for(bsql_program* owner, const LowerOptions& opt, const bsql_column* in,
-                           int n_in, const bsql_column* out, int n_out) {
-  std::string key = cache_key((int)opt.mode, (const bsql_program*)opt.pred,
-                              (const bsql_program*)opt.proj, in, n_in, out, n_out);
+}  // namespace
+
+// The default register budget tries the 8-register file first (it runs 4
+// rows per lane) and takes the 16-register one only if the program would
+// spill there; an explicit budget (the wide spill layout) is kept.
+Lowered lower_best(const LowerOptions& opt, const int32_t* in_types, int n_in, const int32_t* out_types,
+                   int n_out) {
+  if (opt.reg_budget >= BSQL_DEV_MAX_REG - 1) {
+    LowerOptions small = opt;
+    small.reg_budget = 7;
+    Lowered l = lower(small, in_types, n_in, out_types, n_out);
+    if (l.n_spill == 0) return l;
+  }
+  return lower(opt, in_types, n_in, out_types, n_out);
+}
+
+const Lowered& lowered_for(bsql_program* owner, const LowerOptions& opt, const bsql_column* in, int n_in,
+                           const bsql_column* out, int n_out) {
+  std::string key = cache_key(opt, in, n_in, out, n_out);
   std::lock_guard<std::mutex> lk(owner->cache_mu);
   auto it = owner->cache.find(key);
   if (it != owner->cache.end()) return it->second;
   std::vector<int32_t> in_types(n_in), out_types(n_out);
   for (int i = 0; i < n_in; ++i) in_types[i] = in[i].type_id;
   for (int i = 0; i < n_out; ++i) out_types[i] = out ? out[i].type_id : 0;
-  Lowered l = lower(opt, in_types.data(), n_in, out_types.data(), n_out);
-  return owner->cache.emplace(key, l).first->second;
+  return owner->cache.emplace(key, lower_best(opt, in_types.data(), n_in, out_types.data(), n_out)).first->second;
 }
 
+namespace {
+
 int rows_per_thread(const bsql_program* p) { return p ? p->launch.rows_per_thread : 0; }
+int block_size(const bsql_program* p) { return p && p->launch.block_size ? p->launch.block_size : 256; }
+int reg_budget(const bsql_program* p) {
+  return p && (p->launch.flags & BSQL_LAUNCH_SPILL_WIDE) ? 7 : BSQL_DEV_MAX_REG - 1;
+}
+
+// The launch shape a lowered plan runs with: the requested block size and
+// rows per lane if a kernel variant holds its registers and spill slots,
+// else the closest one that does (a spilling plan may need smaller
+// workgroups for its LDS).
+void launch_shape(const Lowered& L, int rpt_req, int block_req, int* rpt, int* block) {
+  const int blocks[2] = {block_req, block_req == 256 ? 64 : 256};
+  const int rpts[4] = {rpt_req, 0, 1, 2};
+  for (int b : blocks)
+    for (int r : rpts)
+      if (variant_fits(L.nreg, r, b, L.n_spill)) {
+        *rpt = r;
+        *block = b;
+        return;
+      }
+  fail("program needs " + std::to_string(L.n_spill) + " spill slots, more than a workgroup's LDS holds");
+}
 
 }  // namespace
 }  // namespace bsql
@@ -272,7 +375,7 @@ namespace bsql {
 // later output phase on the same stream find the same layout.
 int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, int n_in,
                 bsql_column* out, int n_out, int64_t num_rows, int64_t capacity,
-                int64_t* d_count, void* stream, int phases) {
+                int64_t* d_count, void* stream, int phases, const std::vector<int>* out_subset) {
   if (!pred) fail("null predicate");
   if (!d_count) fail("null count pointer");
   check_inputs(in, n_in, num_rows);
@@ -281,7 +384,21 @@ int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, i
   if (!pred->steps.empty() || (proj && !proj->steps.empty())) {
     sc.stream = (hipStream_t)stream;
     sc.cols.assign(in, in + n_in);
-    build_steps(*pred, sc, num_rows);
+    if (!pred->steps.empty()) {
+      if (phases & PHASE_SELECT) {
+        build_steps(*pred, sc, num_rows);
+      } else {
+        // the output phase does not read the predicate's step columns:
+        // placeholders keep the projection's numbering
+        for (const Step& st : pred->steps) {
+          bsql_column c;
+          std::memset(&c, 0, sizeof(c));
+          c.type_id = st.out_type;
+          c.size = num_rows;
+          sc.cols.push_back(c);
+        }
+      }
+    }
     if (proj && !proj->steps.empty()) {
       proj = rebased_program(proj, (int)sc.cols.size());
       build_steps(*proj, sc, num_rows);
@@ -304,97 +421,210 @@ int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, i
   opt.conj_order = &pred->conj_order;
   opt.ranges = !(pred->launch.flags & BSQL_LAUNCH_NO_RANGES);
   opt.lazy_loads = (pred->launch.flags & BSQL_LAUNCH_LAZY_LOADS) != 0;
+  opt.out_subset = out_subset;
+  opt.reg_budget = std::min(reg_budget(pred), reg_budget(proj));
   const Lowered& L = lowered_for(pred, opt, in, n_in, out, n_out);
   DevPlan plan = L.plan;
   plan.num_rows = num_rows;
-  for (int k = 0; k < n_in; ++k) plan.in[k] = to_dev_in(in[k]);
+  for (size_t j = 0; j < L.in_cols.size(); ++j) plan.in[j] = to_dev_in(in[L.in_cols[j]]);
   const int n_str = count_string_outputs(L, in, out, n_out, outputs);
   // the predicate's launch shape drives the selection kernel, the
   // projection's (or the predicate's, for a plain filter) the output kernel
-  const int sel_rpt = rows_per_thread(pred);
-  const int rpt = rows_per_thread(proj ? proj : pred);
-  const int64_t sel_words = select_scratch_words(plan, L.nreg, sel_rpt);
+  int sel_rpt = rows_per_thread(pred), sel_block = 256;
+  int rpt = 0, block = 256;
+  launch_shape(L, rows_per_thread(proj ? proj : pred), block_size(proj ? proj : pred), &rpt, &block);
+  if (plan.n_ranges == 0) launch_shape(L, sel_rpt, block_size(pred), &sel_rpt, &sel_block);
+  if (sel_block != block) block = sel_block;  // one block size per plan (the spill layout is per block)
+  const int64_t sel_words = select_scratch_words(plan, L.nreg, sel_rpt, block);
   if (sel_words < 0) fail("no kernel variant for this program");
-  // scratch: plan | selection ballots + slice counts/offsets | string
-  // assembly tile status + ticket | selected row ids (uint32) | string
-  // output slices
-  const int64_t st_words = n_str ? string_tiles(num_rows) + 1 : 0;
   const int64_t id_words = (num_rows + 1) / 2;
+  const int64_t st_words = n_str ? string_tiles(num_rows) + 1 : 0;
+  const int64_t pw = plan_words(L.ops.size());
+  const int64_t at_plan = sel_words + id_words + st_words + (int64_t)n_str * num_rows;
   uint64_t* scratch = nullptr;
-  e = get_scratch((hipStream_t)stream, sel_words + st_words + id_words + (int64_t)n_str * num_rows, &scratch);
+  e = get_scratch((hipStream_t)stream, at_plan + pw, &scratch);
   if (e != hipSuccess) return device_error(e, "filter scratch");
-  uint64_t* sel = scratch + kPlanWords;
-  uint64_t* status = sel + sel_words;
-  uint32_t* ids = (uint32_t*)(status + st_words);
-  uint64_t* slices = status + st_words + id_words;
+  uint64_t* sel = scratch;
+  uint32_t* ids = (uint32_t*)(scratch + sel_words);
+  uint64_t* status = scratch + sel_words + id_words;
+  uint64_t* slices = status + st_words;
   for (int k = 0, j = 0; k < n_out; ++k) {
     const bool str = L.out_str_col[(size_t)k] >= 0;
     plan.out[k].data = str ? (void*)(slices + (size_t)(j++) * (size_t)capacity) : (outputs ? out[k].data : nullptr);
     plan.out[k].valid = outputs ? out[k].validity : nullptr;
     plan.out[k].type = out[k].type_id;
   }
-  e = upload_plan(plan, scratch, (hipStream_t)stream);
+  e = upload_plan(plan, L.ops, scratch, at_plan, (hipStream_t)stream);
   if (e == hipSuccess)
-    e = launch_compact(plan, (const DevPlan*)scratch, L.nreg, sel_rpt, rpt, sel, ids, d_count, capacity,
-                       (hipStream_t)stream, phases);
+    e = launch_compact(plan, (const DevPlan*)(scratch + at_plan), L.nreg, sel_rpt, rpt, block, sel, ids, d_count,
+                       capacity, (hipStream_t)stream, phases);
   if (e == hipSuccess && outputs)
     e = finish_string_outputs(L, in, out, n_out, slices, d_count, 0, capacity, status, (hipStream_t)stream);
   if (e != hipSuccess) return device_error(e, "filter");
   return BSQL_OK;
 }
 
+int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, int n_in, bsql_column* out,
+                int n_out, int64_t num_rows, int64_t capacity, int64_t* d_count, void* stream, int phases) {
+  return run_compact(pred, proj, in, n_in, out, n_out, num_rows, capacity, d_count, stream, phases, nullptr);
+}
+
+namespace {
+
+// Output groups of a program without parts: consecutive runs of at most
+// BSQL_DEV_MAX_OUT outputs.
+std::vector<std::vector<int>> output_groups(int n_out) {
+  std::vector<std::vector<int>> g;
+  for (int k = 0; k < n_out; k += BSQL_DEV_MAX_OUT) {
+    g.emplace_back();
+    for (int j = k; j < std::min(n_out, k + BSQL_DEV_MAX_OUT); ++j) g.back().push_back(j);
+  }
+  return g;
+}
+
+}  // namespace
+
+int run_filter_project(bsql_program* pred, bsql_program* proj, const bsql_column* in, int n_in, bsql_column* out,
+                       int n_out, int64_t num_rows, int64_t capacity, int64_t* d_count, void* stream) {
+  if (!proj) fail("null projection");
+  if (n_out != num_roots(*proj) && proj->parts.empty()) fail("number of outputs does not match the program");
+  if (proj->parts.empty() && n_out <= BSQL_DEV_MAX_OUT)
+    return run_compact(pred, proj, in, n_in, out, n_out, num_rows, capacity, d_count, stream, PHASE_ALL, nullptr);
+  // one selection, then one output launch per part and output group (the
+  // row ids stay in the stream's scratch in between)
+  std::vector<std::pair<bsql_program*, std::vector<int>>> launches;  // program, outputs of `out`
+  if (!proj->parts.empty()) {
+    for (size_t j = 0; j < proj->parts.size(); ++j) {
+      bsql_program* part = static_cast<bsql_program*>(proj->parts[j].get());
+      for (const std::vector<int>& g : output_groups(num_roots(*part))) {
+        std::vector<int> outs;
+        for (int k : g) outs.push_back(proj->part_outputs[j][(size_t)k]);
+        launches.push_back({part, outs});
+      }
+    }
+  } else {
+    for (const std::vector<int>& g : output_groups(n_out)) launches.push_back({proj, g});
+  }
+  bool first = true;
+  for (auto& l : launches) {
+    std::vector<bsql_column> o;
+    std::vector<int> local;  // outputs of l.first this launch writes
+    for (size_t i = 0; i < l.second.size(); ++i) {
+      if (l.second[i] < 0 || l.second[i] >= n_out) fail("output index out of range");
+      o.push_back(out[l.second[i]]);
+    }
+    // the part's outputs, in the part's own numbering
+    if (l.first == proj) {
+      local = l.second;
+    } else {
+      const std::vector<int>& po = proj->part_outputs[(size_t)(std::find_if(proj->parts.begin(), proj->parts.end(),
+          [&](const std::shared_ptr<Program>& q) { return q.get() == l.first; }) - proj->parts.begin())];
+      for (int k : l.second) local.push_back((int)(std::find(po.begin(), po.end(), k) - po.begin()));
+    }
+    const int rc = run_compact(pred, l.first, in, n_in, o.data(), (int)o.size(), num_rows, capacity, d_count,
+                               stream, first ? PHASE_ALL : PHASE_OUTPUT, &local);
+    if (rc != BSQL_OK) return rc;
+    first = false;
+  }
+  return BSQL_OK;
+}
+
 int execute_program(bsql_program* prog, const bsql_column* in, int n_in, bsql_column* out, int n_out,
                     int64_t num_rows, void* stream, bool private_scratch) {
   if (!prog) fail("null program");
+  if (!prog->parts.empty()) fail("internal: a split program runs part by part");
   if (!prog->steps.empty() && n_in == prog->n_table) fail("the program's step columns were not built");
   check_inputs(in, n_in, num_rows);
-  if (n_out != (int)prog->final_out.size()) fail("number of outputs does not match the program");
+  if (n_out != num_roots(*prog)) fail("number of outputs does not match the program");
   check_outputs(out, n_out, num_rows);
   hipError_t e = ensure_strings(*prog);
   if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
-  LowerOptions opt;
-  opt.mode = LowerOptions::EXECUTE;
-  opt.proj = prog;
-  const Lowered& L = lowered_for(prog, opt, in, n_in, out, n_out);
-  DevPlan plan = L.plan;
-  plan.num_rows = num_rows;
-  for (int k = 0; k < n_in; ++k) plan.in[k] = to_dev_in(in[k]);
-  const int n_str = count_string_outputs(L, in, out, n_out);
-  // scratch: plan | string assembly status + ticket | one slice column per string output
-  const int64_t st_words = n_str ? string_tiles(num_rows) + 1 : 0;
-  const int64_t words = st_words + (int64_t)n_str * num_rows;
-  uint64_t* scratch = nullptr;
-  if (private_scratch) {
-    e = hipMallocAsync((void**)&scratch, (size_t)(kPlanWords + std::max<int64_t>(words, 1)) * 8,
-                       (hipStream_t)stream);
-  } else {
-    e = get_scratch((hipStream_t)stream, words, &scratch);
-  }
-  if (e != hipSuccess) return device_error(e, "bsql_interp_execute scratch");
-  struct Release {
-    uint64_t* p;
-    hipStream_t s;
-    ~Release() {
-      if (p) (void)hipFreeAsync(p, s);
+  for (const std::vector<int>& group : output_groups(n_out)) {
+    std::vector<bsql_column> gout;
+    for (int k : group) gout.push_back(out[k]);
+    const int g_out = (int)gout.size();
+    LowerOptions opt;
+    opt.mode = LowerOptions::EXECUTE;
+    opt.proj = prog;
+    opt.reg_budget = reg_budget(prog);
+    const bool all = g_out == n_out;
+    opt.out_subset = all ? nullptr : &group;
+    const Lowered& L = lowered_for(prog, opt, in, n_in, gout.data(), g_out);
+    DevPlan plan = L.plan;
+    plan.num_rows = num_rows;
+    for (size_t j = 0; j < L.in_cols.size(); ++j) plan.in[j] = to_dev_in(in[L.in_cols[j]]);
+    const int n_str = count_string_outputs(L, in, gout.data(), g_out);
+    int rpt = 0, block = 256;
+    launch_shape(L, rows_per_thread(prog), block_size(prog), &rpt, &block);
+    // scratch: string assembly status + ticket | one slice column per string output | plan + ops
+    const int64_t st_words = n_str ? string_tiles(num_rows) + 1 : 0;
+    const int64_t at_plan = st_words + (int64_t)n_str * num_rows;
+    const int64_t words = at_plan + plan_words(L.ops.size());
+    uint64_t* scratch = nullptr;
+    if (private_scratch) {
+      e = hipMallocAsync((void**)&scratch, (size_t)words * 8, (hipStream_t)stream);
+    } else {
+      e = get_scratch((hipStream_t)stream, words, &scratch);
     }
-  } release{private_scratch ? scratch : nullptr, (hipStream_t)stream};
-  uint64_t* status = scratch + kPlanWords;
-  uint64_t* slices = status + st_words;
-  for (int k = 0, j = 0; k < n_out; ++k) {
-    const bool str = L.out_str_col[(size_t)k] >= 0;
-    plan.out[k].data = str ? (void*)(slices + (size_t)(j++) * (size_t)num_rows) : out[k].data;
-    plan.out[k].valid = out[k].validity;
-    plan.out[k].type = out[k].type_id;
+    if (e != hipSuccess) return device_error(e, "bsql_interp_execute scratch");
+    struct Release {
+      uint64_t* p;
+      hipStream_t s;
+      ~Release() {
+        if (p) (void)hipFreeAsync(p, s);
+      }
+    } release{private_scratch ? scratch : nullptr, (hipStream_t)stream};
+    uint64_t* status = scratch;
+    uint64_t* slices = status + st_words;
+    for (int k = 0, j = 0; k < g_out; ++k) {
+      const bool str = L.out_str_col[(size_t)k] >= 0;
+      plan.out[k].data = str ? (void*)(slices + (size_t)(j++) * (size_t)num_rows) : gout[k].data;
+      plan.out[k].valid = gout[k].validity;
+      plan.out[k].type = gout[k].type_id;
+    }
+    if (num_rows == 0 && n_str == 0) continue;
+    if (private_scratch) {
+      // a private buffer is fresh: the plan goes in without the reuse check
+      const std::vector<char> blob = plan_blob(plan, L.ops, scratch + at_plan);
+      e = hipMemcpyAsync(scratch + at_plan, blob.data(), blob.size(), hipMemcpyHostToDevice, (hipStream_t)stream);
+      // the host blob must outlive the copy: pageable copies are staged before the call returns
+    } else {
+      e = upload_plan(plan, L.ops, scratch, at_plan, (hipStream_t)stream);
+    }
+    if (e == hipSuccess && num_rows > 0)
+      e = launch_execute(plan, (const DevPlan*)(scratch + at_plan), L.nreg, rpt, block, (hipStream_t)stream);
+    if (e == hipSuccess)
+      e = finish_string_outputs(L, in, gout.data(), g_out, slices, nullptr, num_rows, num_rows, status,
+                                (hipStream_t)stream);
+    if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
   }
-  if (num_rows == 0 && n_str == 0) return BSQL_OK;
-  e = upload_plan(plan, scratch, (hipStream_t)stream);
-  if (e == hipSuccess && num_rows > 0)
-    e = launch_execute(plan, (const DevPlan*)scratch, L.nreg, rows_per_thread(prog), (hipStream_t)stream);
-  if (e == hipSuccess)
-    e = finish_string_outputs(L, in, out, n_out, slices, nullptr, num_rows, num_rows, status,
-                              (hipStream_t)stream);
-  if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
   return BSQL_OK;
+}
+
+int run_execute(bsql_program* prog, const bsql_column* in, int n_in, bsql_column* out, int n_out,
+                int64_t num_rows, void* stream) {
+  if (!prog) fail("null program");
+  if (!prog->parts.empty()) {
+    if (n_out != num_roots(*prog)) fail("number of outputs does not match the program");
+    for (size_t j = 0; j < prog->parts.size(); ++j) {
+      std::vector<bsql_column> o;
+      for (int k : prog->part_outputs[j]) {
+        if (k < 0 || k >= n_out) fail("output index out of range");
+        o.push_back(out[k]);
+      }
+      const int rc = run_execute(static_cast<bsql_program*>(prog->parts[j].get()), in, n_in, o.data(),
+                                 (int)o.size(), num_rows, stream);
+      if (rc != BSQL_OK) return rc;
+    }
+    return BSQL_OK;
+  }
+  if (prog->steps.empty()) return execute_program(prog, in, n_in, out, n_out, num_rows, stream, false);
+  check_inputs(in, n_in, num_rows);
+  StepColumns sc;
+  sc.stream = (hipStream_t)stream;
+  sc.cols.assign(in, in + n_in);
+  build_steps(*prog, sc, num_rows);
+  return execute_program(prog, sc.cols.data(), (int)sc.cols.size(), out, n_out, num_rows, stream, false);
 }
 
 }  // namespace bsql
@@ -402,7 +632,29 @@ int execute_program(bsql_program* prog, const bsql_column* in, int n_in, bsql_co
 extern "C" {
 
 const char* bsql_last_error(void) { return g_last_error.c_str(); }
-int bsql_version(void) { return 1; }
+int bsql_version(void) { return 2; }
+
+int bsql_expression_rewrite(const char* expr, int stage, const int32_t* types, int n_types, char* out,
+                            size_t cap, size_t* len) {
+  BSQL_TRY({
+    if (!expr || (n_types > 0 && !types) || (cap > 0 && !out)) fail("invalid arguments");
+    const std::string r = rewrite_expression(expr, stage, types, n_types);
+    if (cap > 0) {
+      const size_t n = std::min(cap - 1, r.size());
+      std::memcpy(out, r.data(), n);
+      out[n] = 0;
+    }
+    if (len) *len = r.size();
+    return BSQL_OK;
+  })
+}
+
+int bsql_set_current_timestamp(const char* ts) {
+  BSQL_TRY({
+    set_current_timestamp(ts ? ts : "");
+    return BSQL_OK;
+  })
+}
 
 int bsql_program_from_arrays(const int16_t* left, const int16_t* right, const int16_t* out,
                              const int32_t* ops, int n_ops, const int16_t* final_out, int n_final,
@@ -461,12 +713,45 @@ int bsql_program_compile(const char* const* exprs, int n_exprs, const int32_t* i
 void bsql_program_free(bsql_program* prog) { delete prog; }
 
 int bsql_program_num_ops(const bsql_program* p) { return p ? (int)p->ops.size() : -1; }
-int bsql_program_num_outputs(const bsql_program* p) { return p ? (int)p->final_out.size() : -1; }
+int bsql_program_num_outputs(const bsql_program* p) { return p ? num_roots(*p) : -1; }
 int bsql_program_num_inputs(const bsql_program* p) { return p ? p->n_inputs : -1; }
+int bsql_program_num_interpreted(const bsql_program* p) { return p ? (int)p->final_out.size() : -1; }
+
+int bsql_program_root(const bsql_program* p, int k, int32_t* kind, int32_t* index, bsql_scalar* literal) {
+  if (!p || k < 0 || k >= num_roots(*p) || !p->parts.empty())
+    return set_error(BSQL_ERR_INVALID, "invalid arguments");
+  const Root r = root_of(*p, k);
+  if (kind) *kind = r.kind == Root::VM ? BSQL_ROOT_INTERPRETED : r.kind == Root::COLUMN ? BSQL_ROOT_COLUMN
+                                                                                       : BSQL_ROOT_LITERAL;
+  if (index) *index = r.kind == Root::LITERAL ? -1 : r.index;
+  if (literal) {
+    std::memset(literal, 0, sizeof(*literal));
+    if (r.kind == Root::LITERAL) {
+      literal->type_id = r.lit.type_id;
+      if (is_float_type(r.lit.type_id)) literal->v.f64 = r.lit.f64; else literal->v.i64 = r.lit.i64;
+      // a null literal: str_len -1
+      if (r.lit.is_null) literal->str_len = -1;
+    }
+  }
+  return BSQL_OK;
+}
+
+int bsql_program_num_parts(const bsql_program* p) { return p ? (int)p->parts.size() : -1; }
+
+int bsql_program_part(const bsql_program* p, int j, const bsql_program** part, int32_t* outputs, int cap) {
+  if (!p || j < 0 || j >= (int)p->parts.size() || !part) return set_error(BSQL_ERR_INVALID, "invalid arguments");
+  *part = static_cast<const bsql_program*>(p->parts[(size_t)j].get());
+  const std::vector<int>& o = p->part_outputs[(size_t)j];
+  for (size_t i = 0; i < o.size() && (int)i < cap && outputs; ++i) outputs[i] = o[i];
+  return (int)o.size();
+}
 
 int bsql_program_arrays(const bsql_program* p, int16_t* left, int16_t* right, int16_t* out,
                         int32_t* ops, int16_t* final_out) {
   if (!p) return set_error(BSQL_ERR_INVALID, "null program");
+  if (!p->parts.empty())
+    return set_error(BSQL_ERR_INVALID, "the expression list was split into " + std::to_string(p->parts.size()) +
+                                           " programs (bsql_program_part)");
   const size_t n = p->ops.size();
   if (left) std::memcpy(left, p->left.data(), n * 2);
   if (right) std::memcpy(right, p->right.data(), n * 2);
@@ -494,15 +779,29 @@ int bsql_program_scalar(const bsql_program* p, int op, int side, bsql_scalar* ou
 int bsql_program_output_type(const bsql_program* p, const int32_t* in_types, int n_in, int k,
                              int32_t* type_id) {
   BSQL_TRY({
-    if (!p || !type_id || k < 0 || k >= (int)p->final_out.size()) fail("invalid arguments");
+    if (!p || !type_id || k < 0 || k >= num_roots(*p)) fail("invalid arguments");
+    if (!p->parts.empty()) {
+      for (size_t j = 0; j < p->parts.size(); ++j)
+        for (size_t i = 0; i < p->part_outputs[j].size(); ++i)
+          if (p->part_outputs[j][i] == k)
+            return bsql_program_output_type(static_cast<const bsql_program*>(p->parts[j].get()), in_types, n_in,
+                                            (int)i, type_id);
+      fail("output " + std::to_string(k) + " is in no part");
+    }
     const std::vector<int32_t> ext = extended_types(*p, in_types, n_in);
-    std::vector<int32_t> st = slot_types_for(*p, ext.data(), (int)ext.size());
-    OpTypes t = propagate_types(*p, st.data(), (int)st.size());
-    const int slot = p->final_out[k];
-    if (slot < 0) {  // literal root
-      *type_id = p->root_literals[(size_t)k].type_id;
+    const Root r = root_of(*p, k);
+    if (r.kind == Root::LITERAL) {
+      *type_id = r.lit.type_id;
       return BSQL_OK;
     }
+    if (r.kind == Root::COLUMN) {
+      if (r.index < 0 || r.index >= (int)ext.size()) fail("output column not in the table");
+      *type_id = ext[(size_t)r.index];
+      return BSQL_OK;
+    }
+    std::vector<int32_t> st = slot_types_for(*p, ext.data(), (int)ext.size());
+    OpTypes t = propagate_types(*p, st.data(), (int)st.size());
+    const int slot = p->final_out[(size_t)r.index];
     int32_t ty = slot < (int)st.size() ? st[(size_t)slot] : BSQL_EMPTY;
     for (size_t i = 0; i < p->ops.size(); ++i)
       if (p->out[i] == slot) ty = t.ot[i];
@@ -538,14 +837,51 @@ int bsql_program_set_conjunct_order(bsql_program* pred, const int32_t* order, in
   })
 }
 
+int bsql_program_plan_info(const bsql_program* cp, const int32_t* in_types, int n_in, bsql_plan_info* info) {
+  BSQL_TRY({
+    bsql_program* p = const_cast<bsql_program*>(cp);
+    if (!p || !info || (n_in > 0 && !in_types)) fail("invalid arguments");
+    if (!p->parts.empty()) fail("the expression list was split into parts (bsql_program_part)");
+    const std::vector<int32_t> ext = extended_types(*p, in_types, n_in);
+    const int n_out = num_roots(*p);
+    if (n_out > BSQL_DEV_MAX_OUT) fail("more outputs than one launch writes");
+    std::vector<int32_t> out_types((size_t)n_out);
+    for (int k = 0; k < n_out; ++k) {
+      const int rc = bsql_program_output_type(p, in_types, n_in, k, &out_types[(size_t)k]);
+      if (rc != BSQL_OK) return rc;
+    }
+    LowerOptions opt;
+    opt.mode = LowerOptions::EXECUTE;
+    opt.proj = p;
+    opt.reg_budget = reg_budget(p);
+    const Lowered L = lower_best(opt, ext.data(), (int)ext.size(), out_types.data(), n_out);
+    std::memset(info, 0, sizeof(*info));
+    info->n_ops = L.plan.n_ops;
+    info->n_regs = L.nreg;
+    info->n_spill = L.n_spill;
+    info->n_inputs = (int32_t)L.in_cols.size();
+    int in_flight = 0;
+    for (const DevOp& d : L.ops) {
+      if (d.code == D_LOADN) {
+        info->n_load_issues += 1;
+        in_flight = 0;
+      }
+      if (d.code == D_COMMIT) info->max_loads_in_flight = std::max<int32_t>(info->max_loads_in_flight, ++in_flight);
+      if (d.code == D_SPILL) info->n_spill_ops += 1;
+      if (d.code == D_FILL) info->n_fill_ops += 1;
+    }
+    return BSQL_OK;
+  })
+}
+
 int bsql_program_set_launch(bsql_program* p, const bsql_launch_config* cfg) {
   if (!p || !cfg) return set_error(BSQL_ERR_INVALID, "invalid arguments");
   const int r = cfg->rows_per_thread;
   if (r != 0 && r != 1 && r != 2 && r != 4 && r != 8 && r != 16 && r != 32)
     return set_error(BSQL_ERR_INVALID, "rows_per_thread must be 0, 1, 2, 4, 8, 16 or 32");
-  if (cfg->block_size != 0 && cfg->block_size != 256)
-    return set_error(BSQL_ERR_INVALID, "block_size must be 0 or 256");
-  if (cfg->flags & ~(BSQL_LAUNCH_NO_RANGES | BSQL_LAUNCH_LAZY_LOADS))
+  if (cfg->block_size != 0 && cfg->block_size != 64 && cfg->block_size != 256)
+    return set_error(BSQL_ERR_INVALID, "block_size must be 0, 64 or 256");
+  if (cfg->flags & ~(BSQL_LAUNCH_NO_RANGES | BSQL_LAUNCH_LAZY_LOADS | BSQL_LAUNCH_SPILL_WIDE))
     return set_error(BSQL_ERR_INVALID, "unknown launch flags");
   std::lock_guard<std::mutex> lk(p->cache_mu);
   p->launch = *cfg;
@@ -557,19 +893,15 @@ int bsql_interp_execute(const bsql_program* cprog, const bsql_column* in, int n_
   BSQL_TRY({
     bsql_program* prog = const_cast<bsql_program*>(cprog);
     if (!prog) fail("null program");
-    if (prog->steps.empty()) return execute_program(prog, in, n_in, out, n_out, num_rows, stream, false);
-    check_inputs(in, n_in, num_rows);
-    StepColumns sc;
-    sc.stream = (hipStream_t)stream;
-    sc.cols.assign(in, in + n_in);
-    build_steps(*prog, sc, num_rows);
-    return execute_program(prog, sc.cols.data(), (int)sc.cols.size(), out, n_out, num_rows, stream, false);
+    reject_capture(stream, "bsql_interp_execute");
+    return run_execute(prog, in, n_in, out, n_out, num_rows, stream);
   })
 }
 
 int bsql_filter(const bsql_program* pred, const bsql_column* in, int n_in, bsql_column* out,
                 int64_t num_rows, int64_t* d_count, void* stream) {
   BSQL_TRY({
+    reject_capture(stream, "bsql_filter");
     return run_compact(const_cast<bsql_program*>(pred), nullptr, in, n_in, out, n_in, num_rows,
                        num_rows, d_count, stream, PHASE_ALL);
   })
@@ -578,6 +910,7 @@ int bsql_filter(const bsql_program* pred, const bsql_column* in, int n_in, bsql_
 int bsql_filter_count(const bsql_program* pred, const bsql_column* in, int n_in, int64_t num_rows,
                       int64_t* d_count, void* stream) {
   BSQL_TRY({
+    reject_capture(stream, "bsql_filter_count");
     return run_compact(const_cast<bsql_program*>(pred), nullptr, in, n_in, nullptr, 0, num_rows, num_rows,
                        d_count, stream, PHASE_ALL);
   })
@@ -589,8 +922,9 @@ int bsql_filter_project_sized(const bsql_program* pred, const bsql_program* proj
   BSQL_TRY({
     if (!proj) fail("null projection");
     if (capacity < 0 || capacity > num_rows) fail("capacity must be in [0, num_rows]");
-    return run_compact(const_cast<bsql_program*>(pred), const_cast<bsql_program*>(proj), in, n_in, out, n_out,
-                       num_rows, capacity, d_count, stream, PHASE_ALL);
+    reject_capture(stream, "bsql_filter_project_sized");
+    return run_filter_project(const_cast<bsql_program*>(pred), const_cast<bsql_program*>(proj), in, n_in, out,
+                              n_out, num_rows, capacity, d_count, stream);
   })
 }
 
@@ -599,8 +933,9 @@ int bsql_filter_project(const bsql_program* pred, const bsql_program* proj, cons
                         void* stream) {
   BSQL_TRY({
     if (!proj) fail("null projection");
-    return run_compact(const_cast<bsql_program*>(pred), const_cast<bsql_program*>(proj), in, n_in,
-                       out, n_out, num_rows, num_rows, d_count, stream, PHASE_ALL);
+    reject_capture(stream, "bsql_filter_project");
+    return run_filter_project(const_cast<bsql_program*>(pred), const_cast<bsql_program*>(proj), in, n_in,
+                              out, n_out, num_rows, num_rows, d_count, stream);
   })
 }
 
@@ -609,33 +944,67 @@ int bsql_filter_project(const bsql_program* pred, const bsql_program* proj, cons
 // ------------------------------------------------------------------ batches
 
 struct bsql_batch {
-  std::vector<DevPlan> plans;  // host copies (columns filled in)
+  std::vector<DevPlan> plans;  // host copies (columns filled in, ops aimed at d_ops)
   std::vector<int64_t> rows;
   int nreg = 1;
+  int n_spill = 0;
   bsql_launch_config launch{0, 0, 0};
+  int block = 256;
+  int rpt = 0;
   DevPlan* d_plans = nullptr;
+  DevOp* d_ops = nullptr;
+  // one work map per launch shape (block, rows per lane), written once when
+  // the shape is first used and never rewritten, so changing the shape never
+  // touches a map that a queued launch may still read
+  std::map<std::pair<int, int>, std::pair<int64_t*, int64_t>> maps;
   int64_t* d_tile_start = nullptr;
   int64_t n_items = 0;
   ~bsql_batch() {
+    if (process_exiting()) return;
     if (d_plans) (void)hipFree(d_plans);
-    if (d_tile_start) (void)hipFree(d_tile_start);
+    if (d_ops) (void)hipFree(d_ops);
+    for (auto& m : maps) (void)hipFree(m.second.first);
   }
 };
 
 namespace {
 
-// Work map for the current launch shape: prefix counts of tiles per batch.
-hipError_t upload_work_map(bsql_batch* b) {
-  const int tile = batch_tile_rows(b->nreg, b->launch.rows_per_thread);
+// Selects the work map of a launch shape (prefix counts of tiles per batch),
+// building it on first use.
+hipError_t upload_work_map(bsql_batch* b, int rpt, int block, hipStream_t) {
+  const int tile = batch_tile_rows(b->nreg, rpt, block);
   if (tile <= 0) fail("no kernel variant for these programs");
-  std::vector<int64_t> start(b->plans.size() + 1, 0);
-  for (size_t i = 0; i < b->plans.size(); ++i) start[i + 1] = start[i] + (b->rows[i] + tile - 1) / tile;
-  b->n_items = start.back();
-  if (!b->d_tile_start) {
-    hipError_t e = hipMalloc((void**)&b->d_tile_start, start.size() * sizeof(int64_t));
+  auto it = b->maps.find({block, rpt});
+  if (it == b->maps.end()) {
+    std::vector<int64_t> start(b->plans.size() + 1, 0);
+    for (size_t i = 0; i < b->plans.size(); ++i) start[i + 1] = start[i] + (b->rows[i] + tile - 1) / tile;
+    int64_t* d = nullptr;
+    hipError_t e = hipMalloc((void**)&d, start.size() * sizeof(int64_t));
     if (e != hipSuccess) return e;
+    e = hipMemcpy(d, start.data(), start.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { (void)hipFree(d); return e; }
+    it = b->maps.emplace(std::make_pair(block, rpt), std::make_pair(d, start.back())).first;
   }
-  return hipMemcpy(b->d_tile_start, start.data(), start.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+  b->d_tile_start = it->second.first;
+  b->n_items = it->second.second;
+  b->rpt = rpt;
+  b->block = block;
+  return hipSuccess;
+}
+
+// The launch shape closest to the requested one that holds the batch's
+// registers and spill slots.
+void batch_shape(const bsql_batch* b, int rpt_req, int block_req, int* rpt, int* block) {
+  const int blocks[2] = {block_req ? block_req : 256, block_req == 64 ? 256 : 64};
+  const int rpts[4] = {rpt_req, 0, 1, 2};
+  for (int bl : blocks)
+    for (int r : rpts)
+      if (variant_fits(b->nreg, r, bl, b->n_spill)) {
+        *rpt = variant_rows(b->nreg, r, bl, b->n_spill);  // concrete: the work map's tile size
+        *block = bl;
+        return;
+      }
+  fail("the batch's programs need more spill slots than a workgroup's LDS holds");
 }
 
 }  // namespace
@@ -646,14 +1015,18 @@ int bsql_batch_prepare(const bsql_batch_desc* descs, int n, bsql_batch** out) {
   BSQL_TRY({
     if (!descs || n <= 0 || !out) fail("invalid arguments");
     auto b = std::make_unique<bsql_batch>();
+    std::vector<DevOp> all_ops;
+    std::vector<size_t> op_at;
     for (int i = 0; i < n; ++i) {
       const bsql_batch_desc& d = descs[i];
       bsql_program* prog = const_cast<bsql_program*>(d.prog);
       if (!prog) fail("batch " + std::to_string(i) + " has no program");
       if (!prog->steps.empty())
         fail("batch " + std::to_string(i) + ": programs that build string columns do not run batched");
+      if (!prog->parts.empty() || num_roots(*prog) > BSQL_DEV_MAX_OUT)
+        fail("batch " + std::to_string(i) + ": a program split over several launches does not run batched");
       check_inputs(d.in, d.n_in, d.num_rows);
-      if (d.n_out != (int)prog->final_out.size()) fail("batch " + std::to_string(i) + ": outputs do not match");
+      if (d.n_out != num_roots(*prog)) fail("batch " + std::to_string(i) + ": outputs do not match");
       check_outputs(d.out, d.n_out, d.num_rows);
       for (int k = 0; k < d.n_out; ++k)
         if (d.out[k].type_id == BSQL_STRING) fail("batched launches write fixed-width outputs only");
@@ -662,23 +1035,37 @@ int bsql_batch_prepare(const bsql_batch_desc* descs, int n, bsql_batch** out) {
       LowerOptions opt;
       opt.mode = LowerOptions::EXECUTE;
       opt.proj = prog;
+      opt.reg_budget = reg_budget(prog);
       const Lowered& L = lowered_for(prog, opt, d.in, d.n_in, d.out, d.n_out);
       DevPlan plan = L.plan;
       plan.num_rows = d.num_rows;
-      for (int k = 0; k < d.n_in; ++k) plan.in[k] = to_dev_in(d.in[k]);
+      for (size_t j = 0; j < L.in_cols.size(); ++j) plan.in[j] = to_dev_in(d.in[L.in_cols[j]]);
       for (int k = 0; k < d.n_out; ++k) {
         plan.out[k].data = d.out[k].data;
         plan.out[k].valid = d.out[k].validity;
         plan.out[k].type = d.out[k].type_id;
       }
+      op_at.push_back(all_ops.size());
+      all_ops.insert(all_ops.end(), L.ops.begin(), L.ops.end());
       b->plans.push_back(plan);
       b->rows.push_back(d.num_rows);
       b->nreg = std::max(b->nreg, L.nreg);
+      b->n_spill = std::max(b->n_spill, L.n_spill);
     }
-    hipError_t e = hipMalloc((void**)&b->d_plans, b->plans.size() * sizeof(DevPlan));
+    // one register file size for all: a spilling program fixes it at 16
+    if (b->n_spill > 0) b->nreg = BSQL_DEV_MAX_REG;
+    hipError_t e = hipMalloc((void**)&b->d_ops, all_ops.size() * sizeof(DevOp));
+    for (size_t i = 0; i < b->plans.size() && e == hipSuccess; ++i) {
+      b->plans[i].ops = b->d_ops + op_at[i];
+      b->plans[i].n_spill = b->n_spill;
+    }
+    if (e == hipSuccess) e = hipMemcpy(b->d_ops, all_ops.data(), all_ops.size() * sizeof(DevOp), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_plans, b->plans.size() * sizeof(DevPlan));
     if (e == hipSuccess)
       e = hipMemcpy(b->d_plans, b->plans.data(), b->plans.size() * sizeof(DevPlan), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = upload_work_map(b.get());
+    int rpt = 0, block = 256;
+    batch_shape(b.get(), 0, 256, &rpt, &block);
+    if (e == hipSuccess) e = upload_work_map(b.get(), rpt, block, nullptr);
     if (e != hipSuccess) return device_error(e, "bsql_batch_prepare");
     *out = b.release();
     return BSQL_OK;
@@ -689,7 +1076,7 @@ int bsql_batch_launch(const bsql_batch* b, void* stream) {
   BSQL_TRY({
     if (!b) fail("null batch");
     hipError_t e = launch_batches(b->d_plans, b->d_tile_start, (int)b->plans.size(), b->n_items, b->nreg,
-                                  b->launch.rows_per_thread, (hipStream_t)stream);
+                                  b->rpt, b->block, b->n_spill, (hipStream_t)stream);
     if (e != hipSuccess) return device_error(e, "bsql_batch_launch");
     return BSQL_OK;
   })
@@ -700,8 +1087,12 @@ int bsql_batch_set_launch(bsql_batch* b, const bsql_launch_config* cfg) {
     if (!b || !cfg) fail("invalid arguments");
     const int r = cfg->rows_per_thread;
     if (r != 0 && r != 1 && r != 2 && r != 4) fail("rows_per_thread must be 0, 1, 2 or 4");
+    if (cfg->block_size != 0 && cfg->block_size != 64 && cfg->block_size != 256)
+      fail("block_size must be 0, 64 or 256");
+    int rpt = 0, block = 256;
+    batch_shape(b, r, cfg->block_size, &rpt, &block);
     b->launch = *cfg;
-    hipError_t e = upload_work_map(b);
+    hipError_t e = upload_work_map(b, rpt, block, nullptr);
     if (e != hipSuccess) return device_error(e, "bsql_batch_set_launch");
     return BSQL_OK;
   })
@@ -745,7 +1136,8 @@ hipError_t time_launches(hipStream_t stream, F run, float* ms) {
   return e;
 }
 
-const int kShapes[] = {1, 2, 4};  // rows per lane the VM variants cover
+// launch shapes the autotuners try: (block size, rows per lane)
+const int kShapes[][2] = {{256, 1}, {256, 2}, {256, 4}, {64, 1}, {64, 2}, {64, 4}};
 
 }  // namespace
 
@@ -755,48 +1147,61 @@ int bsql_batch_autotune(bsql_batch* b, void* stream, bsql_launch_config* chosen)
   BSQL_TRY({
     if (!b) fail("null batch");
     float best = 1e30f;
-    bsql_launch_config best_cfg = b->launch;
-    for (int r : kShapes) {
-      if (batch_tile_rows(b->nreg, r) != 256 * r) continue;  // no variant of this shape
-      bsql_launch_config cfg{0, r, 0};
-      b->launch = cfg;
-      hipError_t e = upload_work_map(b);
+    int best_r = b->rpt, best_b = b->block;
+    for (const auto& sh : kShapes) {
+      const int bl = sh[0], r = sh[1];
+      if (!variant_fits(b->nreg, r, bl, b->n_spill)) continue;  // no variant of this shape
+      hipError_t e = upload_work_map(b, r, bl, (hipStream_t)stream);
       float ms = 0.f;
       if (e == hipSuccess)
         e = time_launches((hipStream_t)stream, [&] {
           return launch_batches(b->d_plans, b->d_tile_start, (int)b->plans.size(), b->n_items, b->nreg, r,
-                                (hipStream_t)stream);
+                                bl, b->n_spill, (hipStream_t)stream);
         }, &ms);
       if (e != hipSuccess) return device_error(e, "bsql_batch_autotune");
-      if (ms < best) { best = ms; best_cfg = cfg; }
+      if (ms < best) { best = ms; best_r = r; best_b = bl; }
     }
-    b->launch = best_cfg;
-    hipError_t e = upload_work_map(b);
+    hipError_t e = upload_work_map(b, best_r, best_b, (hipStream_t)stream);
     if (e != hipSuccess) return device_error(e, "bsql_batch_autotune");
-    if (chosen) *chosen = best_cfg;
+    b->launch = bsql_launch_config{best_b, best_r, 0};
+    if (chosen) *chosen = b->launch;
     return BSQL_OK;
   })
 }
 
+// Tries each launch shape and, for a program that needs more registers
+// than the small file holds, both spill layouts (16 registers, or 8
+// registers with more values in LDS slots and more rows per lane).
 int bsql_program_autotune(bsql_program* prog, const bsql_column* in, int n_in, bsql_column* out, int n_out,
                           int64_t num_rows, void* stream, bsql_launch_config* chosen) {
   BSQL_TRY({
     if (!prog) fail("null program");
     float best = 1e30f;
+    const bsql_launch_config base = prog->launch;
     bsql_launch_config best_cfg = prog->launch;
-    for (int r : kShapes) {
-      bsql_launch_config cfg = prog->launch;
-      cfg.rows_per_thread = r;
-      prog->launch = cfg;
-      float ms = 0.f;
-      int rc = BSQL_OK;
-      hipError_t e = time_launches((hipStream_t)stream, [&] {
-        rc = bsql_interp_execute(prog, in, n_in, out, n_out, num_rows, stream);
-        return rc == BSQL_OK ? hipSuccess : hipErrorUnknown;
-      }, &ms);
-      if (rc != BSQL_OK) return rc;
-      if (e != hipSuccess) return device_error(e, "bsql_program_autotune");
-      if (ms < best) { best = ms; best_cfg = cfg; }
+    for (int wide = 0; wide < 2; ++wide) {
+      for (const auto& sh : kShapes) {
+        bsql_launch_config cfg = base;
+        cfg.block_size = sh[0];
+        cfg.rows_per_thread = sh[1];
+        cfg.flags = (base.flags & ~BSQL_LAUNCH_SPILL_WIDE) | (wide ? BSQL_LAUNCH_SPILL_WIDE : 0);
+        prog->launch = cfg;
+        float ms = 0.f;
+        int rc = BSQL_OK;
+        hipError_t e = time_launches((hipStream_t)stream, [&] {
+          rc = bsql_interp_execute(prog, in, n_in, out, n_out, num_rows, stream);
+          return rc == BSQL_OK ? hipSuccess : hipErrorUnknown;
+        }, &ms);
+        if (rc != BSQL_OK) {
+          prog->launch = base;
+          return rc;
+        }
+        if (e != hipSuccess) {
+          prog->launch = base;
+          return device_error(e, "bsql_program_autotune");
+        }
+        if (ms < best) { best = ms; best_cfg = cfg; }
+      }
     }
     prog->launch = best_cfg;
     if (chosen) *chosen = best_cfg;

@@ -12,15 +12,25 @@
 #include "../../include/bsql/interp.h"
 #include "program.hpp"
 
+namespace bsql {
+// True once the process has begun to exit (a C atexit handler of this
+// library sets it): destructors then leave device memory to the process
+// teardown instead of calling into a runtime that may be shutting down.
+bool process_exiting();
+}  // namespace bsql
+
 // The opaque program handle: the program plus its cache of lowered plans.
+// `uid` is unique per handle (copies of the Program part keep their own), so
+// that a lowering cached for a pair of programs never outlives either.
 struct bsql_program : bsql::Program {
+  const uint64_t uid = bsql::next_program_id();
   std::mutex cache_mu;
   std::map<std::string, bsql::Lowered> cache;
   // copies whose step columns are numbered from another table width (a
   // projection's steps placed after a predicate's), by that width
   std::map<int, std::unique_ptr<bsql_program>> rebased;
   ~bsql_program() {
-    if (d_strings) (void)hipFree(d_strings);
+    if (d_strings && !bsql::process_exiting()) (void)hipFree(d_strings);
   }
 };
 
@@ -39,6 +49,17 @@ int device_error(hipError_t e, const char* where);
 enum { PHASE_SELECT = 1, PHASE_OUTPUT = 2, PHASE_ALL = 3 };
 int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, int n_in, bsql_column* out,
                 int n_out, int64_t num_rows, int64_t capacity, int64_t* d_count, void* stream, int phases);
+// Filter + project for any projection: one selection, then the output phase
+// per part of a split projection and per group of at most
+// BSQL_DEV_MAX_OUT outputs.
+int run_filter_project(bsql_program* pred, bsql_program* proj, const bsql_column* in, int n_in, bsql_column* out,
+                       int n_out, int64_t num_rows, int64_t capacity, int64_t* d_count, void* stream);
+// bsql_interp_execute for any program: its parts, steps and output groups.
+int run_execute(bsql_program* prog, const bsql_column* in, int n_in, bsql_column* out, int n_out,
+                int64_t num_rows, void* stream);
+// Fails with BSQL_ERR_INVALID if `stream` is capturing a graph (the entry
+// points that allocate scratch or copy host memory cannot be captured).
+void reject_capture(void* stream, const char* entry);
 
 // The body of bsql_interp_execute for a program without steps; a private
 // scratch buffer (allocated and freed on the stream) leaves the stream's

@@ -1,15 +1,22 @@
 // devplan.h -- the lowered program the gfx950 VM kernel executes.
 //
 // Shared by the host lowering (program.cpp) and the HIP kernels
-// (interp_kernels.hip). Passed by value as a kernel argument, so the whole
-// plan is read through the scalar cache with wave-uniform loads.
+// (interp_kernels.hip). The plan and its op list live in device memory and
+// are read through the scalar cache with wave-uniform loads.
 #pragma once
 
 #include <stdint.h>
 
-#define BSQL_DEV_MAX_OPS 96
-#define BSQL_DEV_MAX_IN 24
-#define BSQL_DEV_MAX_OUT 16
+// Columns one launch reads (the distinct table columns its ops and range
+// tests refer to, densely renumbered) and writes. The reference's programs
+// address at most 64 slots (inputs, outputs and temporaries together), so a
+// predicate and a projection part fit one launch; longer output lists run as
+// several output launches over the same selection.
+#define BSQL_DEV_MAX_IN 128
+#define BSQL_DEV_MAX_OUT 64
+// Register file sizes of the kernel variants (the last register is the spare
+// that result-less ops write); live values beyond the file are spilled to
+// LDS slots.
 #define BSQL_DEV_MAX_REG 16
 
 // Device opcodes: reference operators specialised by operand kind
@@ -47,6 +54,9 @@ enum DevCode : uint8_t {
   // items at imm ({n, (spec << 8 | width)...}; 0: "%Y-%m-%d" for days,
   // "%Y-%m-%d %H:%M:%S" otherwise)
   D_STR_TO_TS,
+  // register <-> LDS spill slot imm (data of the lane's rows and their validity)
+  D_SPILL,       // slot imm <- reg[a]
+  D_FILL,        // reg[dst] <- slot imm
   D_NUM_CODES
 };
 
@@ -125,8 +135,16 @@ typedef struct DevPlan {
   int64_t num_rows;
   int32_t n_ranges;  // > 0: the selection is these range conjuncts (ops [0, split) unused)
   int32_t range_eager;  // load every range column before the first test
+  int32_t n_spill;   // LDS spill slots per lane (0: the program fits the register file)
+  int32_t pad;
+  const DevOp* ops;  // n_ops ops (at least one entry), device memory
   DevInCol in[BSQL_DEV_MAX_IN];
   DevOutCol out[BSQL_DEV_MAX_OUT];
   DevRange ranges[BSQL_DEV_MAX_RANGES];
-  DevOp ops[BSQL_DEV_MAX_OPS];
 } DevPlan;
+
+// LDS bytes of the spill slots of one workgroup: per wave and slot, the
+// 64 lanes x R rows of 8-byte data, then one validity byte per lane.
+static inline int64_t spill_lds_bytes(int n_spill, int block, int rows) {
+  return (int64_t)n_spill * (block / 64) * (64 * 8 * rows + 64);
+}

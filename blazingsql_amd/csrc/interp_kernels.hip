@@ -24,6 +24,7 @@
 
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <type_traits>
 
 #include "devplan.h"
@@ -42,6 +43,7 @@ typedef uint32_t v2u32 __attribute__((ext_vector_type(2)));
 // tell that they address global memory and would emit flat loads/stores;
 // every column access goes through a global (address space 1) pointer.
 #define GLOBAL __attribute__((address_space(1)))
+#define LDS __attribute__((address_space(3)))
 template <class T>
 __device__ __forceinline__ const GLOBAL T* gp(const void* p) { return (const GLOBAL T*)p; }
 template <class T>
@@ -941,7 +943,7 @@ __device__ int64_t look_back(const CompactArgs& ca, int64_t tile, int64_t agg, i
 // One op of the plan, read as two dwords and the literal so that the whole
 // wave reads it with scalar loads (the byte fields are unpacked in SGPRs).
 __device__ __forceinline__ DevOp load_op(const DevPlan& plan, int q) {
-  const uint32_t* p = (const uint32_t*)&plan.ops[q];
+  const uint32_t* p = (const uint32_t*)(plan.ops + q);
   const uint32_t w0 = p[0], w1 = p[1];
   DevOp op;
   op.code = (uint8_t)w0;
@@ -1044,10 +1046,18 @@ struct LoadTemps {
 
 // Executes one non-store op. Returns true if it was a KILL that left the
 // wave without a selected row.
+// Spill slot q of the calling wave: 64 lanes x R rows of data (row k of lane
+// t at k*64 + t), then one validity byte per lane (R bits).
+template <int R>
+__device__ __forceinline__ LDS uint64_t* spill_slot(LDS uint64_t* sp, int64_t q) {
+  return sp + q * (64 * R + 8);
+}
+
 template <int NREG, int R>
 __device__ __forceinline__ bool vm_step(const DevPlan& plan, const DevOp& op,
                                         typename RegVecT<NREG>::type (&f)[R], uint64_t& vb,
-                                        LoadTemps<R>& T, const Rows<R>& rs, uint32_t& alive) {
+                                        LoadTemps<R>& T, const Rows<R>& rs, uint32_t& alive,
+                                        LDS uint64_t* sp) {
   constexpr uint32_t RMASK = (1u << R) - 1u;
   const uint8_t code = op.code;
   uint64_t A[R], B[R], D[R];
@@ -1098,6 +1108,22 @@ __device__ __forceinline__ bool vm_step(const DevPlan& plan, const DevOp& op,
       }
     }
     VD = rs.contig ? load_valid<R>(c, rs.row0, rs.nrows) : gather_valid<R>(c, rs);
+  } else if (code == D_SPILL) {
+    const int lane = threadIdx.x & 63;
+    LDS uint64_t* slot = spill_slot<R>(sp, op.imm);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      slot[k * 64 + lane] = A[k];
+      D[k] = A[k];
+    }
+    ((LDS uint8_t*)(slot + 64 * R))[lane] = (uint8_t)VA;
+    dst = NREG - 1;
+  } else if (code == D_FILL) {
+    const int lane = threadIdx.x & 63;
+    const LDS uint64_t* slot = spill_slot<R>(sp, op.imm);
+#pragma unroll
+    for (int k = 0; k < R; ++k) D[k] = slot[k * 64 + lane];
+    VD = ((const LDS uint8_t*)(slot + 64 * R))[lane];
   } else if (code == D_KILL || code == D_SEL) {
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -1159,7 +1185,7 @@ __device__ __forceinline__ void store_out_strided(const DevOutCol& c, int64_t po
   }
 }
 
-// Ops [plan.split, n_ops) over one tile of 256*R positions [tile*TILE,
+// Ops [plan.split, n_ops) over one tile of B*R positions [tile*TILE,
 // count): positions are rows (or, GATHER, indexes into the row-id list idx).
 // Rows: lane t of wave w owns positions tile*TILE + w*64*R + t*R + k
 // (consecutive, vector loads and stores) or, GATHER, tile*TILE + w*64*R +
@@ -1167,12 +1193,13 @@ __device__ __forceinline__ void store_out_strided(const DevOutCol& c, int64_t po
 // selected rows: neighbouring lanes share 128-byte lines, which are fetched
 // once per instruction rather than once per lane.
 // The register file is local to the tile: every register is written before
-// it is read, so nothing carries over from one tile to the next.
-template <int NREG, int R, bool GATHER>
+// it is read, so nothing carries over from one tile to the next. `sp` is the
+// wave's spill area (plan.n_spill slots).
+template <int NREG, int R, bool GATHER, int B>
 __device__ __forceinline__ void vm_tile(const DevPlan& plan, int64_t tile, int64_t count,
-                                        const uint32_t* __restrict__ idx, int lane) {
+                                        const uint32_t* __restrict__ idx, int lane, LDS uint64_t* sp) {
   typedef typename RegVecT<NREG>::type RegVec;
-  constexpr int TILE = BLOCK * R;
+  constexpr int TILE = B * R;
   RegVec f[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
@@ -1207,12 +1234,13 @@ __device__ __forceinline__ void vm_tile(const DevPlan& plan, int64_t tile, int64
   uint32_t alive = rs.exist;
   // the next op's scalar loads are issued before this op runs, so their
   // latency hides behind it
-  DevOp next = load_op(plan, plan.split);
-  for (int q = plan.split; q < plan.n_ops; ++q) {
+  const int n_ops = plan.n_ops;
+  DevOp next = load_op(plan, plan.split < n_ops ? plan.split : 0);
+  for (int q = plan.split; q < n_ops; ++q) {
     const DevOp op = next;
-    next = load_op(plan, q + 1 < BSQL_DEV_MAX_OPS ? q + 1 : q);
+    next = load_op(plan, q + 1 < n_ops ? q + 1 : q);
     if (op.code != D_STORE) {
-      vm_step<NREG, R>(plan, op, f, vb, T, rs, alive);
+      vm_step<NREG, R>(plan, op, f, vb, T, rs, alive, sp);
       continue;
     }
     const int ia = op.a & (NREG - 1);
@@ -1225,35 +1253,45 @@ __device__ __forceinline__ void vm_tile(const DevPlan& plan, int64_t tile, int64
   }
 }
 
+// The calling wave's spill area in the workgroup's dynamic LDS.
+template <int R>
+__device__ __forceinline__ LDS uint64_t* wave_spill(LDS uint64_t* base, int n_spill) {
+  return base + (int64_t)(threadIdx.x >> 6) * n_spill * (64 * R + 8);
+}
+
+extern __shared__ uint64_t s_spill_area[];
+
 // perform_interpreter_operation (rows are the positions [0, num_rows)) or
 // the output phase of a filter (GATHER: rows are the selected row ids
 // idx[0, count), outputs at the positions in that list). Persistent:
-// workgroups stride over tiles of BLOCK*R positions.
-template <int NREG, int R, bool GATHER>
-__global__ __launch_bounds__(BLOCK) void vm_rows(const DevPlan* __restrict__ pplan,
-                                                 const uint32_t* __restrict__ idx,
-                                                 const int64_t* __restrict__ d_count, int64_t capacity) {
-  constexpr int TILE = BLOCK * R;
+// workgroups stride over tiles of B*R positions.
+template <int NREG, int R, bool GATHER, int B>
+__global__ __launch_bounds__(B) void vm_rows(const DevPlan* __restrict__ pplan,
+                                             const uint32_t* __restrict__ idx,
+                                             const int64_t* __restrict__ d_count, int64_t capacity) {
+  constexpr int TILE = B * R;
   const DevPlan& plan = *pplan;  // device memory: ops are scalar-cache loads
   const int lane = threadIdx.x & 63;
+  LDS uint64_t* sp = wave_spill<R>((LDS uint64_t*)s_spill_area, plan.n_spill);
   int64_t count = plan.num_rows;
   if (GATHER) {
     count = *d_count;
     if (count > capacity) count = capacity;
   }
   for (int64_t tile = blockIdx.x; tile * TILE < count; tile += gridDim.x)
-    vm_tile<NREG, R, GATHER>(plan, tile, count, idx, lane);
+    vm_tile<NREG, R, GATHER, B>(plan, tile, count, idx, lane, sp);
 }
 
 // Batched interpreter: every batch has its own plan (program and columns);
 // the work items are all batches' tiles, batch b owning items
 // [tile_start[b], tile_start[b+1]). Workgroups stride over the items, so
 // uneven batch sizes spread over the whole chip in one persistent launch.
-template <int NREG, int R>
-__global__ __launch_bounds__(BLOCK) void vm_batches(const DevPlan* __restrict__ plans,
-                                                    const int64_t* __restrict__ tile_start, int n_batches) {
+template <int NREG, int R, int B>
+__global__ __launch_bounds__(B) void vm_batches(const DevPlan* __restrict__ plans,
+                                                const int64_t* __restrict__ tile_start, int n_batches) {
   const int lane = threadIdx.x & 63;
   const int64_t n_items = tile_start[n_batches];
+  LDS uint64_t* sp = wave_spill<R>((LDS uint64_t*)s_spill_area, plans[0].n_spill);
   for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
     int lo = 0, hi = n_batches - 1;  // last batch whose first item <= item
     while (lo < hi) {
@@ -1262,7 +1300,7 @@ __global__ __launch_bounds__(BLOCK) void vm_batches(const DevPlan* __restrict__ 
       else hi = mid - 1;
     }
     const DevPlan& plan = plans[lo];
-    vm_tile<NREG, R, false>(plan, item - tile_start[lo], plan.num_rows, nullptr, lane);
+    vm_tile<NREG, R, false, B>(plan, item - tile_start[lo], plan.num_rows, nullptr, lane, sp);
   }
 }
 
@@ -1331,14 +1369,16 @@ __device__ __forceinline__ uint32_t emit_slice(const SelArgs& sa, int64_t slice,
 
 // The block total of a selection workgroup (every thread calls it once,
 // after its wave's last slice).
+template <int B = BLOCK>
 __device__ __forceinline__ void emit_block(const SelArgs& sa, uint32_t wave_total) {
-  __shared__ uint32_t s_sum[WAVES];
+  constexpr int NW = B / 64;
+  __shared__ uint32_t s_sum[NW];
   if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = wave_total;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t t = 0;
 #pragma unroll
-    for (int w = 0; w < WAVES; ++w) t += s_sum[w];
+    for (int w = 0; w < NW; ++w) t += s_sum[w];
     gpw<uint32_t>(sa.block_sums)[blockIdx.x] = t;
   }
 }
@@ -1667,16 +1707,19 @@ __global__ __launch_bounds__(BLOCK) void select_ranges(const DevPlan* __restrict
 
 // Row selection by ops [0, split) (KILL per conjunct with a wave-uniform
 // early exit, or one SEL), chunk by chunk over the wave's slice.
-template <int NREG, int R, int S>
-__global__ __launch_bounds__(BLOCK) void select_vm(const DevPlan* __restrict__ pplan, SelArgs sa) {
+template <int NREG, int R, int S, int B>
+__global__ __launch_bounds__(B) void select_vm(const DevPlan* __restrict__ pplan, SelArgs sa) {
   typedef typename RegVecT<NREG>::type RegVec;
   typedef SelectTile<R, S> T;
+  constexpr int NW = B / 64;
   const DevPlan& plan = *pplan;
   const int lane = threadIdx.x & 63;
+  LDS uint64_t* sp = wave_spill<R>((LDS uint64_t*)s_spill_area, plan.n_spill);
   const int64_t s0 = (int64_t)blockIdx.x * sa.per;
   const int64_t s1 = s0 + sa.per < sa.n_slices ? s0 + sa.per : sa.n_slices;
   uint32_t wave_total = 0;
-  for (int64_t slice = s0 + (threadIdx.x >> 6); slice < s1; slice += WAVES) {
+  const int split = plan.split;
+  for (int64_t slice = s0 + (threadIdx.x >> 6); slice < s1; slice += NW) {
     RegVec f[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
@@ -1690,17 +1733,17 @@ __global__ __launch_bounds__(BLOCK) void select_vm(const DevPlan* __restrict__ p
       uint32_t alive = rs.exist;
       if (__ballot(alive != 0) != 0) {
         DevOp next = load_op(plan, 0);
-        for (int q = 0; q < plan.split; ++q) {
+        for (int q = 0; q < split; ++q) {
           const DevOp op = next;
-          next = load_op(plan, q + 1 < BSQL_DEV_MAX_OPS ? q + 1 : q);
-          if (vm_step<NREG, R>(plan, op, f, vb, L, rs, alive)) break;
+          next = load_op(plan, q + 1 < split ? q + 1 : q);
+          if (vm_step<NREG, R>(plan, op, f, vb, L, rs, alive, sp)) break;
         }
       }
       alive_all |= alive << (k * R);
     }
     wave_total += emit_slice<R, S>(sa, slice, alive_all);
   }
-  emit_block(sa, wave_total);
+  emit_block<B>(sa, wave_total);
 }
 
 // ------------------------------------------------------- string outputs
@@ -1833,13 +1876,59 @@ static int64_t persistent_grid(int64_t tiles) {
   return tiles < cap ? tiles : cap;
 }
 
-template <int NREG, int R>
-static hipError_t launch_execute_t(const DevPlan* d_plan, int64_t num_rows, hipStream_t stream) {
-  const int64_t tile = (int64_t)BLOCK * R;
+// (NREG, R, B) variants compiled: register file size x rows per lane x
+// workgroup size. 64-lane workgroups serve programs whose spill slots need
+// more LDS than a 256-lane workgroup can have.
+#define BSQL_VARIANTS(X) \
+  X(8, 2, 256) X(8, 4, 256) X(16, 1, 256) X(16, 2, 256) X(16, 1, 64) X(16, 2, 64) X(8, 4, 64)
+
+// LDS a workgroup may have for spill slots (the static arrays of the
+// selection kernels take < 1 KB of the 64 KB).
+static int64_t max_spill_lds() { return 60 * 1024; }
+
+// Picks the variant: the smallest register file that holds the program, and
+// the rows per lane requested (0 = the default for that register file), for
+// workgroup size `block`; false if no variant holds it (or its spill slots).
+static bool pick(int nreg, int rpt_req, int block, int n_spill, int* NR, int* RR) {
+  struct V { int n, r; };
+  const V vs[] = {{8, 4}, {16, 2}, {8, 2}, {16, 1}};
+  for (const V& v : vs) {
+    if (v.n < nreg) continue;
+    if (rpt_req > 0 && v.r != rpt_req) continue;
+    bool compiled = false;
+#define X(N, RPT, BL) if (N == v.n && RPT == v.r && BL == block) compiled = true;
+    BSQL_VARIANTS(X)
+#undef X
+    if (!compiled) continue;
+    if (spill_lds_bytes(n_spill, block, v.r) > max_spill_lds()) continue;
+    *NR = v.n;
+    *RR = v.r;
+    return true;
+  }
+  if (rpt_req > 0) return pick(nreg, 0, block, n_spill, NR, RR);
+  return false;
+}
+
+bool variant_fits(int nreg, int rpt, int block, int n_spill) {
+  int NR, RR;
+  if (!pick(nreg, rpt, block, n_spill, &NR, &RR)) return false;
+  return rpt == 0 || RR == rpt;
+}
+
+// Rows per lane of the variant a launch with this shape uses (0: none).
+int variant_rows(int nreg, int rpt, int block, int n_spill) {
+  int NR, RR;
+  return pick(nreg, rpt, block, n_spill, &NR, &RR) ? RR : 0;
+}
+
+template <int NREG, int R, int B>
+static hipError_t launch_execute_t(const DevPlan* d_plan, int64_t num_rows, int n_spill, hipStream_t stream) {
+  const int64_t tile = (int64_t)B * R;
   const int64_t grid = persistent_grid((num_rows + tile - 1) / tile);
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((vm_rows<NREG, R, false>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan,
-                     (const uint32_t*)nullptr, (const int64_t*)nullptr, num_rows);
+  hipLaunchKernelGGL((vm_rows<NREG, R, false, B>), dim3((unsigned)grid), dim3(B),
+                     (unsigned)spill_lds_bytes(n_spill, B, R), stream, d_plan, (const uint32_t*)nullptr,
+                     (const int64_t*)nullptr, num_rows);
   return hipGetLastError();
 }
 
@@ -1851,9 +1940,11 @@ template <int R> struct VmChunks { static constexpr int S = 16 / R; };
 constexpr int RANGE_R = 4;
 static int range_chunks_of(int sel_rpt) { return sel_rpt == 8 ? 2 : (sel_rpt == 32 ? 8 : 4); }
 
-template <int NREG, int R>
-static hipError_t launch_select_vm_t(const DevPlan* d_plan, const SelArgs& sa, int64_t grid, hipStream_t stream) {
-  hipLaunchKernelGGL((select_vm<NREG, R, VmChunks<R>::S>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan, sa);
+template <int NREG, int R, int B>
+static hipError_t launch_select_vm_t(const DevPlan* d_plan, const SelArgs& sa, int64_t grid, int n_spill,
+                                     hipStream_t stream) {
+  hipLaunchKernelGGL((select_vm<NREG, R, VmChunks<R>::S, B>), dim3((unsigned)grid), dim3(B),
+                     (unsigned)spill_lds_bytes(n_spill, B, R), stream, d_plan, sa);
   return hipGetLastError();
 }
 
@@ -1864,51 +1955,32 @@ static hipError_t launch_slice_ids(const SelArgs& sa, uint32_t* idx, int64_t* d_
   return hipGetLastError();
 }
 
-template <int NREG, int R>
+template <int NREG, int R, int B>
 static hipError_t launch_gather_t(const DevPlan* d_plan, int64_t num_rows, const uint32_t* idx,
-                                  const int64_t* d_count, int64_t capacity, hipStream_t stream) {
-  const int64_t tile = (int64_t)BLOCK * R;
+                                  const int64_t* d_count, int64_t capacity, int n_spill, hipStream_t stream) {
+  const int64_t tile = (int64_t)B * R;
   const int64_t grid = persistent_grid((num_rows + tile - 1) / tile);
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((vm_rows<NREG, R, true>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, d_plan, idx,
-                     d_count, capacity);
+  hipLaunchKernelGGL((vm_rows<NREG, R, true, B>), dim3((unsigned)grid), dim3(B),
+                     (unsigned)spill_lds_bytes(n_spill, B, R), stream, d_plan, idx, d_count, capacity);
   return hipGetLastError();
 }
 
-// (NREG, R) variants compiled: register file size x rows per lane.
-#define BSQL_VARIANTS(X) X(8, 2) X(8, 4) X(16, 1) X(16, 2)
-
-// Picks the variant: the smallest register file that holds the program, and
-// the rows per lane requested (0 = the default for that register file).
-static bool pick(int nreg, int rpt_req, int* NR, int* RR) {
-  struct V { int n, r; };
-  const V vs[] = {{8, 4}, {16, 2}, {8, 2}, {16, 1}};
-  for (const V& v : vs) {
-    if (v.n < nreg) continue;
-    if (rpt_req > 0 && v.r != rpt_req) continue;
-    *NR = v.n;
-    *RR = v.r;
-    return true;
-  }
-  if (rpt_req > 0) return pick(nreg, 0, NR, RR);
-  return false;
-}
-
 // `d_plan` is a device copy of `plan` (made by the caller on `stream`).
-hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt,
+hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt, int block,
                           hipStream_t stream) {
   int NR, RR;
-  if (!pick(nreg, rpt, &NR, &RR)) return hipErrorInvalidValue;
-#define X(N, RPT) if (NR == N && RR == RPT) return launch_execute_t<N, RPT>(d_plan, plan.num_rows, stream);
+  if (!pick(nreg, rpt, block, plan.n_spill, &NR, &RR)) return hipErrorInvalidValue;
+#define X(N, RPT, BL) \
+  if (NR == N && RR == RPT && block == BL) return launch_execute_t<N, RPT, BL>(d_plan, plan.num_rows, plan.n_spill, stream);
   BSQL_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;
 }
 
-// Rows per tile of the selection kernel the plan uses.
 // Rows per selection slice and ballots per slice (R * S) of the kernel a
 // plan selects with.
-static bool slice_shape(const DevPlan& plan, int nreg, int rpt, int64_t* rows, int* ballots) {
+static bool slice_shape(const DevPlan& plan, int nreg, int rpt, int block, int64_t* rows, int* ballots) {
   if (plan.n_ranges > 0) {
     const int chunks = range_chunks_of(rpt);
     *rows = (int64_t)64 * RANGE_R * chunks;
@@ -1916,38 +1988,38 @@ static bool slice_shape(const DevPlan& plan, int nreg, int rpt, int64_t* rows, i
     return true;
   }
   int NR, RR;
-  if (!pick(nreg, rpt, &NR, &RR)) return false;
+  if (!pick(nreg, rpt, block, plan.n_spill, &NR, &RR)) return false;
   *rows = 64 * 16;  // R * VmChunks<R>::S == 16 rows per lane
   *ballots = 16;
   return true;
 }
 
-// Scratch words of the selection: the ballots, then the slice counts and
-// the block totals and bases (uint32 each).
 // Workgroups of `kernel` that are resident on the device at once (its
 // occupancy per CU x CUs), or 8 per CU when the runtime cannot say.
-static int64_t resident_workgroups(const void* kernel) {
+static int64_t resident_workgroups(const void* kernel, int block, size_t lds) {
   static std::mutex mu;
-  static std::map<std::pair<int, const void*>, int64_t> cache;
+  static std::map<std::tuple<int, const void*, int, size_t>, int64_t> cache;
   int dev = 0;
   (void)hipGetDevice(&dev);
   std::lock_guard<std::mutex> lk(mu);
-  auto it = cache.find({dev, kernel});
+  auto it = cache.find(std::make_tuple(dev, kernel, block, lds));
   if (it != cache.end()) return it->second;
   (void)persistent_grid(1);  // learns the CU count
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, BLOCK, 0) != hipSuccess || per_cu <= 0)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds) != hipSuccess || per_cu <= 0)
     per_cu = 8;
   const int64_t r = (int64_t)per_cu * g_num_cus;
-  cache.emplace(std::make_pair(dev, kernel), r);
+  cache.emplace(std::make_tuple(dev, kernel, block, lds), r);
   return r;
 }
 
 // Workgroups of the selection launch (and of its slice_ids): no more than are
 // resident at once, so that no second round of workgroups trails the first
 // (each takes a contiguous block of slices, ceil(slices / grid) of them).
-static int64_t selection_grid(const DevPlan& plan, int nreg, int sel_rpt, int64_t n_slices) {
+static int64_t selection_grid(const DevPlan& plan, int nreg, int sel_rpt, int block, int64_t n_slices) {
   const void* k = nullptr;
+  int wg = BLOCK;
+  size_t lds = 0;
   if (plan.n_ranges > 0) {
     switch (range_chunks_of(sel_rpt)) {
       case 2: k = reinterpret_cast<const void*>(&select_ranges<RANGE_R, 2>); break;
@@ -1956,42 +2028,45 @@ static int64_t selection_grid(const DevPlan& plan, int nreg, int sel_rpt, int64_
     }
   } else {
     int NR, RR;
-    if (pick(nreg, sel_rpt, &NR, &RR)) {
-#define X(N, RPT) \
-  if (NR == N && RR == RPT) k = reinterpret_cast<const void*>(&select_vm<N, RPT, VmChunks<RPT>::S>);
+    if (pick(nreg, sel_rpt, block, plan.n_spill, &NR, &RR)) {
+      wg = block;
+      lds = (size_t)spill_lds_bytes(plan.n_spill, block, RR);
+#define X(N, RPT, BL) \
+  if (NR == N && RR == RPT && block == BL) k = reinterpret_cast<const void*>(&select_vm<N, RPT, VmChunks<RPT>::S, BL>);
       BSQL_VARIANTS(X)
 #undef X
     }
   }
-  int64_t grid = persistent_grid((n_slices + WAVES - 1) / WAVES);
+  const int waves = wg / 64;
+  int64_t grid = persistent_grid((n_slices + waves - 1) / waves);
   if (k) {
-    const int64_t r = resident_workgroups(k);
+    const int64_t r = resident_workgroups(k, wg, lds);
     if (grid > r) grid = r;
   }
   return grid;
 }
 
-int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt) {
+int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt, int block) {
   int64_t rows;
   int ballots;
-  if (!slice_shape(plan, nreg, rpt, &rows, &ballots)) return -1;
+  if (!slice_shape(plan, nreg, rpt, block, &rows, &ballots)) return -1;
   const int64_t n = (plan.num_rows + rows - 1) / rows;
-  const int64_t grid = selection_grid(plan, nreg, rpt, n);
+  const int64_t grid = selection_grid(plan, nreg, rpt, block, n);
   return n * ballots + (n + 2 * grid + 1) / 2;
 }
 
 // The output phase of a filter: ops [split, n_ops) over the selected row ids
 // (written by the selection phase, count in *d_count).
-static hipError_t launch_compact_gather(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt,
+static hipError_t launch_compact_gather(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt, int block,
                                         const uint32_t* idx, const int64_t* d_count, int64_t capacity,
                                         hipStream_t stream) {
   if (plan.split >= plan.n_ops) return hipSuccess;
   int NR, RR;
-  if (!pick(nreg, rpt, &NR, &RR)) return hipErrorInvalidValue;
-#define X(N, RPT) \
-  if (NR == N && RR == RPT)                                                                                \
-    return launch_gather_t<N, RPT>(d_plan, plan.num_rows < capacity ? plan.num_rows : capacity, idx, d_count, \
-                                   capacity, stream);
+  if (!pick(nreg, rpt, block, plan.n_spill, &NR, &RR)) return hipErrorInvalidValue;
+#define X(N, RPT, BL)                                                                                         \
+  if (NR == N && RR == RPT && block == BL)                                                                    \
+    return launch_gather_t<N, RPT, BL>(d_plan, plan.num_rows < capacity ? plan.num_rows : capacity, idx,      \
+                                       d_count, capacity, plan.n_spill, stream);
   BSQL_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;
@@ -2004,15 +2079,16 @@ static hipError_t launch_compact_gather(const DevPlan& plan, const DevPlan* d_pl
 // outputs from the count runs them as two calls on one stream, with idx and
 // *d_count kept in between). Nothing is cleared first: every slice writes
 // its ballots and count, and the scan writes *d_count.
-hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int sel_rpt, int rpt,
+hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int sel_rpt, int rpt, int block,
                           uint64_t* sel, uint32_t* idx, int64_t* d_count, int64_t capacity,
                           hipStream_t stream, int phases) {
-  if (!(phases & 1)) return launch_compact_gather(plan, d_plan, nreg, rpt, idx, d_count, capacity, stream);
+  if (!(phases & 1))
+    return launch_compact_gather(plan, d_plan, nreg, rpt, block, idx, d_count, capacity, stream);
   int64_t rows;
   int ballots;
-  if (!slice_shape(plan, nreg, sel_rpt, &rows, &ballots)) return hipErrorInvalidValue;
+  if (!slice_shape(plan, nreg, sel_rpt, block, &rows, &ballots)) return hipErrorInvalidValue;
   const int64_t n = (plan.num_rows + rows - 1) / rows;
-  const int64_t grid = selection_grid(plan, nreg, sel_rpt, n);
+  const int64_t grid = selection_grid(plan, nreg, sel_rpt, block, n);
   uint32_t* counts = (uint32_t*)(sel + n * ballots);
   SelArgs sa{sel, counts, counts + n, counts + n + grid, n, grid ? (n + grid - 1) / grid : 0};
   hipError_t e = hipSuccess;
@@ -2033,8 +2109,9 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
       e = hipGetLastError();
     } else {
       int NR, RR;
-      if (!pick(nreg, sel_rpt, &NR, &RR)) return hipErrorInvalidValue;
-#define X(N, RPT) if (NR == N && RR == RPT) e = launch_select_vm_t<N, RPT>(d_plan, sa, grid, stream);
+      if (!pick(nreg, sel_rpt, block, plan.n_spill, &NR, &RR)) return hipErrorInvalidValue;
+#define X(N, RPT, BL) \
+  if (NR == N && RR == RPT && block == BL) e = launch_select_vm_t<N, RPT, BL>(d_plan, sa, grid, plan.n_spill, stream);
       BSQL_VARIANTS(X)
 #undef X
     }
@@ -2049,7 +2126,7 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
                         : launch_slice_ids<RANGE_R, 4>(sa, idx, d_count, grid, stream);
     } else {
       int NR, RR;
-      if (!pick(nreg, sel_rpt, &NR, &RR)) return hipErrorInvalidValue;
+      if (!pick(nreg, sel_rpt, block, plan.n_spill, &NR, &RR)) return hipErrorInvalidValue;
       e = RR == 1 ? launch_slice_ids<1, VmChunks<1>::S>(sa, idx, d_count, grid, stream)
           : RR == 2 ? launch_slice_ids<2, VmChunks<2>::S>(sa, idx, d_count, grid, stream)
                     : launch_slice_ids<4, VmChunks<4>::S>(sa, idx, d_count, grid, stream);
@@ -2060,29 +2137,29 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
     e = hipGetLastError();
   }
   if (e != hipSuccess || !(phases & 2)) return e;
-  return launch_compact_gather(plan, d_plan, nreg, rpt, idx, d_count, capacity, stream);
+  return launch_compact_gather(plan, d_plan, nreg, rpt, block, idx, d_count, capacity, stream);
 }
 
 // Batched launch: `plans` (device) holds n_batches plans, `tile_start`
-// (device) n_batches + 1 prefix counts of tiles of BLOCK*rpt rows. No
+// (device) n_batches + 1 prefix counts of tiles of block*rpt rows. No
 // allocation and no copy: the launch can be captured in a hipGraph.
-int batch_tile_rows(int nreg, int rpt) {
+int batch_tile_rows(int nreg, int rpt, int block) {
   int NR, RR;
-  if (!pick(nreg, rpt, &NR, &RR)) return -1;
-  return BLOCK * RR;
+  if (!pick(nreg, rpt, block, 0, &NR, &RR)) return -1;
+  return block * RR;
 }
 
 hipError_t launch_batches(const DevPlan* plans, const int64_t* tile_start, int n_batches, int64_t n_items,
-                          int nreg, int rpt, hipStream_t stream) {
+                          int nreg, int rpt, int block, int n_spill, hipStream_t stream) {
   int NR, RR;
-  if (!pick(nreg, rpt, &NR, &RR)) return hipErrorInvalidValue;
+  if (!pick(nreg, rpt, block, n_spill, &NR, &RR)) return hipErrorInvalidValue;
   const int64_t grid = persistent_grid(n_items);
   if (grid == 0) return hipSuccess;
-#define X(N, RPT)                                                                                  \
-  if (NR == N && RR == RPT) {                                                                      \
-    hipLaunchKernelGGL((vm_batches<N, RPT>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, plans, \
-                       tile_start, n_batches);                                                     \
-    return hipGetLastError();                                                                      \
+#define X(N, RPT, BL)                                                                                  \
+  if (NR == N && RR == RPT && block == BL) {                                                           \
+    hipLaunchKernelGGL((vm_batches<N, RPT, BL>), dim3((unsigned)grid), dim3(BL),                       \
+                       (unsigned)spill_lds_bytes(n_spill, BL, RPT), stream, plans, tile_start, n_batches); \
+    return hipGetLastError();                                                                          \
   }
   BSQL_VARIANTS(X)
 #undef X

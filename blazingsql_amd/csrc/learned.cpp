@@ -15,6 +15,10 @@
 #include <vector>
 
 #include "../../include/bsql/interp.h"
+
+namespace bsql {
+bool process_exiting();
+}  // namespace bsql
 #include "capi_internal.h"
 #include "learned.h"
 #include "program.hpp"
@@ -33,6 +37,7 @@ struct bsql_mlp {
   float *h1 = nullptr, *h2 = nullptr, *y = nullptr, *dy = nullptr, *dh1 = nullptr, *dh2 = nullptr;
   int cap = 0;
   ~bsql_mlp() {
+    if (bsql::process_exiting()) return;
     for (float* p : {params, grads, m, v, h1, h2, y, dy, dh1, dh2})
       if (p) (void)hipFree(p);
   }

@@ -17,8 +17,12 @@
 // order), final outputs n..n+e-1, temporaries from n+e with first-free reuse.
 #include <algorithm>
 #include <cctype>
+#include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdio>
+#include <ctime>
+#include <mutex>
 #include <map>
 #include <memory>
 #include <regex>
@@ -221,6 +225,188 @@ std::string fill_minus_op_with_zero(std::string e) {
   const size_t vars = (size_t)std::count(e.begin(), e.end(), '$');
   if (!e.empty() && e[0] == '-' && vars == 1 && e.find(',') == std::string::npos)
     e = e.substr(0, 2) + "0, " + e.substr(2);
+  return e;
+}
+
+std::string replaced(std::string s, const std::string& what, const std::string& with) {
+  replace_all(s, what, with);
+  return s;
+}
+
+// n-ary CONCAT(a, b, c, ...) -> CONCAT(CONCAT(CONCAT(a, b), c), ...), each
+// CONCAT found left to right with its argument list taken up to the bracket
+// that closes it (quotes are not considered there, as in the reference);
+// arguments split at commas outside quotes and brackets and rejoined with
+// ',' (nary_to_nestedbinary, expression_utils.cpp:1520-1622). A CONCAT inside
+// another's arguments is left as written.
+std::string nested_concat(const std::string& e) {
+  const std::string name = "CONCAT";
+  std::string outp;
+  size_t from = 0;
+  size_t at = e.find(name);
+  while (at != std::string::npos) {
+    const size_t open = at + name.size();
+    // the position after the bracket closing the one at `open`
+    size_t end = open;
+    int depth = 0;
+    do {
+      if (end >= e.size()) fail("end expression reached");
+      if (e[end] == '(') ++depth;
+      else if (e[end] == ')') --depth;
+      ++end;
+    } while (end < e.size() && depth != 0);
+    if (depth != 0) fail("end expression reached");
+    outp += e.substr(from, at - from);
+    const std::string args = e.substr(open + 1, end - open - 2);
+    std::vector<std::string> parts;
+    std::string cur;
+    int br = 0;
+    bool in_str = false;
+    for (char c : args) {
+      if (c == '(') ++br;
+      else if (c == ')') --br;
+      else if (c == '\'') in_str = !in_str;
+      if (!in_str && br == 0 && c == ',') {
+        parts.push_back(cur);
+        cur.clear();
+      } else {
+        cur += c;
+      }
+    }
+    if (!cur.empty()) parts.push_back(cur);
+    if (parts.size() < 2) fail("CONCAT operator must have at least two children, as CONCAT($0, $1) .");
+    for (size_t k = 0; k + 1 < parts.size(); ++k) outp += name + "(";
+    outp += parts[0];
+    for (size_t k = 1; k < parts.size(); ++k) outp += "," + parts[k] + ")";
+    from = end;
+    at = e.find(name, end);
+  }
+  return outp + e.substr(from);
+}
+
+// CURRENT_DATE / CURRENT_TIMESTAMP / CURRENT_TIME -> the query's timestamp
+// (get_current_date_or_timestamp, expression_utils.cpp:1494-1518).
+std::string g_current_timestamp;  // "YYYY-MM-DD hh:mm:ss.mmm"; empty: the clock
+std::mutex g_current_mu;
+
+std::string current_timestamp_text() {
+  std::lock_guard<std::mutex> lk(g_current_mu);
+  if (!g_current_timestamp.empty()) return g_current_timestamp;
+  const auto now = std::chrono::system_clock::now();
+  const int64_t ms = std::chrono::duration_cast<std::chrono::milliseconds>(now.time_since_epoch()).count();
+  const std::time_t secs = (std::time_t)(ms / 1000);
+  std::tm tm{};
+  gmtime_r(&secs, &tm);
+  char buf[96];
+  std::snprintf(buf, sizeof(buf), "%04d-%02d-%02d %02d:%02d:%02d.%03d", tm.tm_year + 1900, tm.tm_mon + 1,
+                tm.tm_mday, tm.tm_hour, tm.tm_min, tm.tm_sec, (int)(ms % 1000));
+  return buf;
+}
+
+std::string current_date_or_timestamp(std::string e) {
+  if (e.find("CURRENT_TIME") != std::string::npos && e.find("CURRENT_TIMESTAMP") == std::string::npos)
+    e = replaced(e, "CURRENT_TIME", "CURRENT_TIMESTAMP");
+  const bool date = e.find("CURRENT_DATE") != std::string::npos;
+  if (!date && e.find("CURRENT_TIMESTAMP") == std::string::npos) return e;
+  std::string ts = current_timestamp_text().substr(0, 23);
+  if (date) return replaced(e, "CURRENT_DATE", ts.substr(0, 10));
+  return replaced(e, "CURRENT_TIMESTAMP", ts);
+}
+
+// Calcite states timestamp differences and interval additions in
+// milliseconds; the interpreter's timestamps here are nanoseconds, so the
+// unit factors gain six zeros (convert_ms_to_ns_units,
+// expression_utils.cpp:1316-1350).
+std::string ms_to_ns_units(const std::string& e) {
+  const bool cast_ts = e.find("CAST(") != std::string::npos && e.find(":TIMESTAMP") != std::string::npos;
+  const bool cast_date = e.find("CAST(") != std::string::npos && e.find(":DATE") != std::string::npos;
+  if (!cast_ts && !cast_date) return e;
+  const std::string zeros = "000000";
+  if (e.find("Reinterpret(") != std::string::npos) {
+    for (const char* unit : {"86400000", "3600000", "60000", "1000"})
+      if (e.find(unit) != std::string::npos) return replaced(e, unit, std::string(unit) + zeros);
+  }
+  const size_t iv = e.find(":INTERVAL");
+  if (iv != std::string::npos) {
+    const size_t start = e.find(", ") + 2;
+    const std::string value = e.substr(start, iv - start);
+    return replaced(e, value, value + zeros);
+  }
+  return e;
+}
+
+// A timestamp literal token (date with '-' or '/', a time, an optional
+// fraction of 3, 6 or 9 digits; expression_utils.cpp:401-467).
+bool is_timestamp_token(const std::string& t) {
+  static const std::regex re{
+      R"(^[0-9]{4}([-/])[0-9]{2}\1([0-9]{2})?[ T]?[0-9]{2}:[0-9]{2}:([0-9]{2})?(.[0-9]{3}|.[0-9]{6}|.[0-9]{9})?[Z]?$)"};
+  return std::regex_match(t, re);
+}
+
+// Column index of "$k" or of the first "$k" inside e.g. "CAST($k):TIMESTAMP".
+int column_of_operand(const std::string& e) {
+  if (!e.empty() && e[0] == '$') return std::stoi(e.substr(1));
+  const size_t a = e.find('$') + 1;
+  const size_t b = e.find(')');
+  return std::stoi(e.substr(a, b - a));
+}
+
+// Reinterpret(-(<timestamp literal>, $k)): the literal takes the unit of
+// column k (reinterpret_timestamp, expression_utils.cpp:1371-1421).
+std::string reinterpret_timestamp(std::string e, const int32_t* types, int n) {
+  if (n <= 0) return e;
+  const std::string key = "Reinterpret(-(";
+  const size_t at = e.find(key);
+  if (at == std::string::npos) return e;
+  std::string rest = e.substr(at + key.size());
+  rest = rest.substr(0, rest.find("))"));
+  const std::vector<std::string> ops = split_expression_list(rest, true);
+  if (ops.size() != 2) return e;
+  std::string ts;
+  int col;
+  if (is_timestamp_token(ops[0])) {
+    ts = ops[0];
+    col = column_of_operand(ops[1]);
+  } else if (is_timestamp_token(ops[1])) {
+    ts = ops[1];
+    col = column_of_operand(ops[0]);
+  } else {
+    return e;
+  }
+  if (col < 0 || col >= n) fail("Reinterpret refers to column $" + std::to_string(col) + " outside the table");
+  const char* unit = types[col] == BSQL_TIMESTAMP_SECONDS        ? ":TIMESTAMP_SECONDS"
+                     : types[col] == BSQL_TIMESTAMP_MILLISECONDS ? ":TIMESTAMP_MILLISECONDS"
+                     : types[col] == BSQL_TIMESTAMP_MICROSECONDS ? ":TIMESTAMP_MICROSECONDS"
+                                                                 : ":TIMESTAMP";
+  return replaced(e, ts, "CAST(" + ts + ")" + unit);
+}
+
+// Interval literals come in milliseconds; they take the unit of the
+// duration column they are added to (apply_interval_conversion,
+// expression_utils.cpp:1425-1469).
+std::string interval_conversion(const std::string& e, const int32_t* types, int n) {
+  const std::string iv = ":INTERVAL";
+  if (n <= 0 || e.find(iv) == std::string::npos) return e;
+  if (e.find('+') != std::string::npos || e.find('-') != std::string::npos) {
+    size_t start = 0;
+    if (e.find("+($") != std::string::npos) start = e.find("+($") + 3;
+    else if (e.find("-($") != std::string::npos) start = e.find("-($") + 3;
+    std::string idx = e.substr(start);
+    idx = idx.substr(0, idx.find(", "));
+    int col = -1;
+    try {
+      col = std::stoi(idx);
+    } catch (const std::exception&) {
+      fail("interval arithmetic without a column operand: " + e);
+    }
+    if (col < 0 || col >= n) fail("interval arithmetic on column $" + std::to_string(col) + " outside the table");
+    if (types[col] == BSQL_DURATION_SECONDS) return replaced(e, "000" + iv, iv);
+    if (types[col] == BSQL_DURATION_MICROSECONDS) return replaced(e, "000" + iv, "000000" + iv);
+    if (types[col] == BSQL_DURATION_NANOSECONDS) return replaced(e, "000" + iv, "000000000" + iv);
+    return e;
+  }
+  if (e.find('$') == std::string::npos && e.find("000:INTERVAL") != std::string::npos)
+    return replaced(e, "000" + iv, iv);
   return e;
 }
 
@@ -764,6 +950,7 @@ std::unique_ptr<Node> var_node(int col) {
 }
 
 std::unique_ptr<Program> compile_trees(std::vector<std::unique_ptr<Node>> trees, int n_cols);
+int max_slot(const Program& p);
 
 // Replaces, bottom-up, every function that builds new strings by a column
 // built before the launch, as function_evaluator_transformer does
@@ -790,7 +977,11 @@ struct StepExtractor {
     st.out_type = node_type(e.get(), types);
     std::vector<std::unique_ptr<Node>> one;
     one.push_back(std::move(e));
-    st.sub = make_program_handle(compile_trees(std::move(one), (int)types.size()));
+    std::unique_ptr<Program> sub = compile_trees(std::move(one), (int)types.size());
+    if (max_slot(*sub) > BSQL_MAX_SLOT)
+      fail("Interops does not support plans with an input or output index greater than 63");
+    validate(*sub, sub->n_inputs);
+    st.sub = make_program_handle(std::move(sub));
     return add_step(std::move(st));
   }
 
@@ -905,19 +1096,46 @@ void check_interpretable(const Node* n) {
 namespace {
 
 // One program over n_cols columns for already transformed trees (each tree
-// one output); the columns it reads become its inputs, in column order.
+// one output), as evaluate_expressions builds it (LogicalProject.cpp:
+// 810-904): a literal root fills its column, a bare column root is copied,
+// and only the other trees are encoded; their columns become the inputs, in
+// column order, and interpreted output j has slot n_inputs + j.
 std::unique_ptr<Program> compile_trees(std::vector<std::unique_ptr<Node>> trees, int n_cols) {
   std::vector<bool> used((size_t)std::max(n_cols, 0), false);
-  for (const auto& t : trees) {
-    check_interpretable(t.get());
-    std::vector<int> vars;
-    collect_vars(t.get(), vars);
-    for (int v : vars) {
-      if (v < 0 || v >= n_cols) fail("column $" + std::to_string(v) + " not in the table");
-      used[(size_t)v] = true;
-    }
-  }
+  const int n_exprs = (int)trees.size();
   auto prog = std::make_unique<Program>();
+  prog->n_table = n_cols;
+  std::vector<const Node*> interp;
+  for (int i = 0; i < n_exprs; ++i) {
+    const Node* root = trees[(size_t)i].get();
+    Root r;
+    if (root->type == NodeType::Literal) {
+      r.kind = Root::LITERAL;
+      if (is_null_literal(root->value)) {
+        r.lit.type_id = root->lit_type;
+        r.lit.is_null = true;
+      } else {
+        r.lit = scalar_from_string(root->value, root->lit_type);
+      }
+      if (is_string_type(r.lit.type_id)) fail("a string literal expression is not produced by the interpreter");
+    } else if (root->type == NodeType::Variable) {
+      r.kind = Root::COLUMN;
+      r.index = root->var_index;
+      if (r.index < 0 || r.index >= n_cols) fail("column $" + std::to_string(r.index) + " not in the table");
+    } else {
+      check_interpretable(root);
+      std::vector<int> vars;
+      collect_vars(root, vars);
+      for (int v : vars) {
+        if (v < 0 || v >= n_cols) fail("column $" + std::to_string(v) + " not in the table");
+        used[(size_t)v] = true;
+      }
+      r.kind = Root::VM;
+      r.index = (int)interp.size();
+      interp.push_back(root);
+    }
+    prog->roots.push_back(r);
+  }
   std::map<int, int> col_map;
   for (int i = 0; i < n_cols; ++i)
     if (used[(size_t)i]) {
@@ -926,50 +1144,42 @@ std::unique_ptr<Program> compile_trees(std::vector<std::unique_ptr<Node>> trees,
       prog->used_columns.push_back(i);
     }
   const int n_inputs = (int)col_map.size();
-  const int n_exprs = (int)trees.size();
+  const int n_interp = (int)interp.size();
   prog->n_inputs = n_inputs;
-  prog->n_table = n_cols;
-  prog->root_literals.assign((size_t)n_exprs, Scalar());
-  for (int i = 0; i < n_exprs; ++i) {
-    const int final_pos = n_inputs + i;
+  for (int j = 0; j < n_interp; ++j) {
+    const int final_pos = n_inputs + j;
     prog->final_out.push_back((int16_t)final_pos);
-    const Node* root = trees[(size_t)i].get();
-    if (root->type == NodeType::Variable) continue;  // pass-through: final slot = input slot
-    if (root->type == NodeType::Literal) {
-      // filled with the literal, as make_column_from_scalar does
-      if (is_null_literal(root->value)) {
-        prog->final_out.back() = BSQL_SCALAR_NULL_INDEX;
-        prog->root_literals[(size_t)i].type_id = root->lit_type;
-      } else {
-        prog->final_out.back() = BSQL_SCALAR_INDEX;
-        prog->root_literals[(size_t)i] = scalar_from_string(root->value, root->lit_type);
-      }
-      if (is_string_type(prog->root_literals[(size_t)i].type_id))
-        fail("a string literal expression is not produced by the interpreter");
-      continue;
-    }
-    Encoder enc(*prog, col_map, n_inputs + n_exprs);
-    enc.visit(root);
+    Encoder enc(*prog, col_map, n_inputs + n_interp);
+    enc.visit(interp[(size_t)j]);
     prog->out.back() = (int16_t)final_pos;
   }
-  // pass-through expressions read their column's slot directly
-  for (int i = 0; i < n_exprs; ++i) {
-    const Node* root = trees[(size_t)i].get();
-    if (root->type == NodeType::Variable) prog->final_out[(size_t)i] = (int16_t)col_map.at(root->var_index);
-  }
-  validate(*prog, n_inputs);
   return prog;
+}
+
+// Largest slot of an encoded program.
+int max_slot(const Program& p) {
+  int mx = -1;
+  for (size_t i = 0; i < p.ops.size(); ++i)
+    mx = std::max<int>(mx, std::max<int>(p.left[i], std::max<int>(p.right[i], p.out[i])));
+  for (int16_t f : p.final_out) mx = std::max<int>(mx, f);
+  return mx;
 }
 
 }  // namespace
 
-std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exprs,
-                                             const int32_t* in_types, int n_in) {
+namespace {
+
+// The program of one expression list: the string-building functions become
+// steps, the remaining trees are encoded; an encoding that reaches slot 64
+// is replaced by the two halves of the list, recursively
+// (LogicalProject.cpp:906-922). A single expression over the limit fails
+// (the reference recurses without end).
+std::unique_ptr<Program> compile_list(const std::vector<std::string>& exprs, const int32_t* in_types, int n_in) {
   auto holder = std::make_unique<Program>();
   StepExtractor ex{*holder, std::vector<int32_t>(in_types, in_types + std::max(n_in, 0)), n_in};
   std::vector<std::unique_ptr<Node>> trees;
   for (const std::string& raw : exprs) {
-    std::string e = replace_calcite_regex(fill_minus_op_with_zero(raw));
+    std::string e = replace_calcite_regex(preprocess_expression(raw, in_types, n_in));
     e = expand_if_logical_op(e);
     Parser parser(e);
     std::unique_ptr<Node> t = rewrite_trim(rewrite_substring(transform_custom(parser.parse())));
@@ -991,9 +1201,91 @@ std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exp
     trees.push_back(std::move(t));
   }
   std::unique_ptr<Program> prog = compile_trees(std::move(trees), (int)ex.types.size());
+  if (max_slot(*prog) > BSQL_MAX_SLOT) {
+    if (exprs.size() < 2)
+      fail("Interops does not support plans with an input or output index greater than 63");
+    const size_t half = exprs.size() / 2;
+    const std::vector<std::string> lo(exprs.begin(), exprs.begin() + (long)half);
+    const std::vector<std::string> hi(exprs.begin() + (long)half, exprs.end());
+    auto split = std::make_unique<Program>();
+    split->n_table = n_in;
+    split->n_inputs = 0;
+    for (int h = 0; h < 2; ++h) {
+      std::unique_ptr<Program> part = compile_list(h ? hi : lo, in_types, n_in);
+      const int base = h ? (int)half : 0;
+      auto add = [&](std::shared_ptr<Program> q, const std::vector<int>& outs) {
+        split->parts.push_back(std::move(q));
+        split->part_outputs.push_back(outs);
+      };
+      if (part->parts.empty()) {
+        std::vector<int> outs;
+        for (int k = 0; k < num_roots(*part); ++k) outs.push_back(base + k);
+        add(make_program_handle(std::move(part)), outs);
+      } else {
+        // flatten: the halves' own parts
+        for (size_t j = 0; j < part->parts.size(); ++j) {
+          std::vector<int> outs;
+          for (int k : part->part_outputs[j]) outs.push_back(base + k);
+          add(part->parts[j], outs);
+        }
+      }
+    }
+    Root r;
+    for (size_t k = 0; k < exprs.size(); ++k) split->roots.push_back(r);  // counted; produced by the parts
+    return split;
+  }
+  validate(*prog, prog->n_inputs);
   prog->steps = std::move(holder->steps);
   prog->n_table = n_in;
   return prog;
+}
+
+}  // namespace
+
+std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exprs,
+                                             const int32_t* in_types, int n_in) {
+  return compile_list(exprs, in_types, n_in);
+}
+
+std::string preprocess_expression(const std::string& e, const int32_t* types, int n) {
+  std::string r = fill_minus_op_with_zero(e);
+  r = nested_concat(r);
+  r = current_date_or_timestamp(r);
+  r = ms_to_ns_units(r);
+  r = reinterpret_timestamp(r, types, n);
+  return interval_conversion(r, types, n);
+}
+
+namespace {
+std::string rebuild(const Node* n) {
+  if (n->type != NodeType::Operator) return n->value;
+  std::string args;
+  for (const auto& c : n->children) args += (args.empty() ? "" : ", ") + rebuild(c.get());
+  return n->value + "(" + args + ")";
+}
+}  // namespace
+
+std::string rewrite_expression(const std::string& e, int stage, const int32_t* types, int n) {
+  switch (stage) {
+    case 0: return fill_minus_op_with_zero(e);
+    case 1: return nested_concat(e);
+    case 2: return ms_to_ns_units(e);
+    case 3: return reinterpret_timestamp(e, types, n);
+    case 4: return interval_conversion(e, types, n);
+    case 5: return preprocess_expression(e, types, n);
+    case 6: return replace_calcite_regex(e);
+    case 7: return expand_if_logical_op(replace_calcite_regex(e));
+    case 8: {
+      Parser parser(e);
+      return rebuild(transform_custom(parser.parse()).get());
+    }
+    default: fail("unknown rewrite stage " + std::to_string(stage));
+  }
+}
+
+void set_current_timestamp(const std::string& ts) {
+  std::lock_guard<std::mutex> lk(g_current_mu);
+  g_current_timestamp = ts;
 }
 
 // ------------------------------------------------------------ query parts

@@ -140,7 +140,7 @@ void validate(const Program& p, int n_in) {
   if (p.left.size() != n || p.right.size() != n || p.out.size() != n || p.lsc.size() != n ||
       p.rsc.size() != n)
     fail("program arrays have different lengths");
-  if (p.final_out.empty()) fail("program has no outputs");
+  if (p.final_out.empty() && p.roots.empty() && p.parts.empty()) fail("program has no outputs");
   int mx = -1;
   for (size_t i = 0; i < n; ++i) {
     mx = std::max<int>(mx, std::max<int>(p.left[i], std::max<int>(p.right[i], p.out[i])));
@@ -166,8 +166,7 @@ void validate(const Program& p, int n_in) {
   }
   for (size_t k = 0; k < p.final_out.size(); ++k) {
     const int16_t f = p.final_out[k];
-    if (f < 0 && !((f == BSQL_SCALAR_INDEX || f == BSQL_SCALAR_NULL_INDEX) && k < p.root_literals.size()))
-      fail("output " + std::to_string(k) + " has no slot");
+    if (f < 0) fail("output " + std::to_string(k) + " has no slot");
     mx = std::max<int>(mx, f);
   }
   if (mx > BSQL_MAX_SLOT)
@@ -654,21 +653,30 @@ SsaProgram to_ssa(Lowering& L, const Program& p, uint64_t seed) {
     s.ops.push_back(so);
   }
   float_operands(L, s);
-  for (size_t k = 0; k < p.final_out.size(); ++k) {
-    const int16_t f = p.final_out[k];
-    if (f < 0) {
+  for (int k = 0; k < num_roots(p); ++k) {
+    const Root rt = root_of(p, k);
+    if (rt.kind == Root::LITERAL) {
       // literal root: an identity conversion of the literal (null: no value)
-      const Scalar& lit = p.root_literals[k];
+      const Scalar& lit = rt.lit;
       SourceOp c;
       c.prog_op = -1;
       c.d.code = is_float_type(lit.type_id) ? D_TO_F64 : D_TO_I64;
-      c.d.flags = (f == BSQL_SCALAR_INDEX ? F_A_IMM : F_A_NULL) | (is_float_type(lit.type_id) ? F_A_F : 0);
-      c.d.imm = f == BSQL_SCALAR_INDEX ? scalar_bits(lit) : 0;
+      c.d.flags = (lit.is_null ? F_A_NULL : F_A_IMM) | (is_float_type(lit.type_id) ? F_A_F : 0);
+      c.d.imm = lit.is_null ? 0 : scalar_bits(lit);
       c.vdst = L.new_value();
       s.ops.push_back(c);
       s.finals.push_back(c.vdst);
       continue;
     }
+    if (rt.kind == Root::COLUMN) {
+      // bare column root: the column itself (a table column or a step's)
+      if (rt.index < 0 || rt.index >= L.n_in) fail("output column $" + std::to_string(rt.index) + " not in the table");
+      s.finals.push_back(L.load_of(rt.index));
+      continue;
+    }
+    if (rt.index < 0 || rt.index >= (int)p.final_out.size()) fail("output " + std::to_string(k) + " has no slot");
+    const int16_t f = p.final_out[(size_t)rt.index];
+    if (f < 0) fail("output " + std::to_string(k) + " has no slot");
     int v = f < n_slots_in && slot_val[f] < 0 ? L.load_of(col_of[(size_t)f]) : slot_val[f];
     if (v < 0) fail("output slot " + std::to_string(f) + " is never written");
     s.finals.push_back(v);
@@ -848,10 +856,212 @@ std::vector<DevRange> conjunct_ranges(const Program& p, const int32_t* in_types,
   return out;
 }
 
+namespace {
+
+// Ops of `s` that the values `roots` depend on (dead-code elimination), as
+// a flag per op.
+std::vector<char> needed_ops(const SsaProgram& s, const std::vector<int>& prod, const std::vector<int>& roots) {
+  std::vector<char> need(s.ops.size(), 0);
+  std::vector<int> stack(roots.begin(), roots.end());
+  while (!stack.empty()) {
+    const int v = stack.back();
+    stack.pop_back();
+    const int o = v >= 0 ? prod[(size_t)v] : -1;
+    if (o < 0 || need[(size_t)o]) continue;
+    need[(size_t)o] = 1;
+    stack.push_back(s.ops[(size_t)o].va);
+    stack.push_back(s.ops[(size_t)o].vb);
+  }
+  return need;
+}
+
+// Column loads issued as early as registers allow. `ops` holds the op list
+// with every D_LOAD right before the first op that reads it; within each
+// segment [bounds[k], bounds[k+1]) (the selection phase or, with lazy loads,
+// one conjunct; the output phase) a load moves up past earlier ops while
+// the values live there, plus it, leave room for one result within `budget`
+// registers, and never above an earlier load (so runs of loads stay in
+// first-use order and merge into multi-column LOADN issues).
+std::vector<LOp> place_loads(const std::vector<LOp>& ops, const std::vector<int>& bounds, size_t n_values,
+                             int budget) {
+  std::vector<LOp> res;
+  for (size_t sg = 0; sg + 1 < bounds.size(); ++sg) {
+    std::vector<LOp> body, loads;
+    std::vector<int> at;  // first use of each load, as an index into body
+    for (int i = bounds[sg]; i < bounds[sg + 1]; ++i) {
+      const LOp& o = ops[(size_t)i];
+      if (o.d.code == D_LOAD) {
+        loads.push_back(o);
+        at.push_back(-1);
+      } else {
+        body.push_back(o);
+      }
+    }
+    const int m = (int)body.size();
+    // live-in pressure before each body op, loads not counted
+    std::vector<int> def(n_values, -1), last(n_values, -1);
+    for (int j = 0; j < m; ++j) {
+      const LOp& o = body[(size_t)j];
+      for (int v : {o.va, o.vb})
+        if (v >= 0) last[(size_t)v] = std::max(last[(size_t)v], j);
+      if (o.vdst >= 0) def[(size_t)o.vdst] = j;
+    }
+    std::vector<int> pressure((size_t)m + 1, 0);
+    for (int j = 0; j < m; ++j) {
+      const int v = body[(size_t)j].vdst;
+      if (v >= 0 && last[(size_t)v] > j)
+        for (int q = j + 1; q <= last[(size_t)v]; ++q) ++pressure[(size_t)q];
+    }
+    for (size_t k = 0; k < loads.size(); ++k) {
+      const int v = loads[k].vdst;
+      for (int j = 0; j < m && at[k] < 0; ++j)
+        if (body[(size_t)j].va == v || body[(size_t)j].vb == v) at[k] = j;
+      if (at[k] < 0) at[k] = m;  // unused: placed at the end (never happens for emitted loads)
+    }
+    std::vector<size_t> order(loads.size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return at[x] < at[y]; });
+    std::vector<std::vector<LOp>> before((size_t)m + 1);
+    int floor_pos = 0;
+    for (size_t k : order) {
+      const int u = at[k];
+      const int lu = std::max(u, last[(size_t)loads[k].vdst]);
+      int p = u;
+      while (p > floor_pos && pressure[(size_t)(p - 1)] + 2 <= budget) --p;
+      for (int q = p; q <= std::min(lu, m); ++q) ++pressure[(size_t)q];
+      before[(size_t)p].push_back(loads[k]);
+      floor_pos = p;
+    }
+    for (int j = 0; j <= m; ++j) {
+      for (const LOp& l : before[(size_t)j]) res.push_back(l);
+      if (j < m) res.push_back(body[(size_t)j]);
+    }
+  }
+  return res;
+}
+
+// Linear-scan register assignment with spilling: values take registers
+// 0..budget-1; when none is free, the value whose next use is furthest away
+// goes to an LDS slot (D_SPILL, once per value) and comes back with D_FILL
+// before its next use. Returns the op list with registers filled in.
+std::vector<LOp> assign_registers(const std::vector<LOp>& ops, size_t n_values, int budget, int* regs_used,
+                                  int* slots_used) {
+  std::vector<std::vector<int>> uses(n_values);
+  for (size_t i = 0; i < ops.size(); ++i)
+    for (int v : {ops[i].va, ops[i].vb})
+      if (v >= 0 && (uses[(size_t)v].empty() || uses[(size_t)v].back() != (int)i)) uses[(size_t)v].push_back((int)i);
+  std::vector<int> reg_of(n_values, -1), slot_of(n_values, -1), holder((size_t)budget, -1);
+  std::vector<int> free_slots;
+  int n_slots = 0, used = 0;
+  std::vector<LOp> res;
+  auto next_use = [&](int v, int i) -> int {
+    const std::vector<int>& u = uses[(size_t)v];
+    auto it = std::upper_bound(u.begin(), u.end(), i);
+    return it == u.end() ? INT32_MAX : *it;
+  };
+  auto last_use = [&](int v) -> int { return uses[(size_t)v].empty() ? -1 : uses[(size_t)v].back(); };
+  auto take_reg = [&](int i, int keep_a, int keep_b) -> int {
+    for (int r = 0; r < budget; ++r)
+      if (holder[(size_t)r] < 0) {
+        used = std::max(used, r + 1);
+        return r;
+      }
+    int victim = -1, far = -1;
+    for (int r = 0; r < budget; ++r) {
+      const int v = holder[(size_t)r];
+      if (v == keep_a || v == keep_b) continue;
+      const int nu = next_use(v, i - 1);
+      if (nu > far) { far = nu; victim = v; }
+    }
+    if (victim < 0) fail("internal: no register to spill");
+    if (slot_of[(size_t)victim] < 0) {
+      if (!free_slots.empty()) {
+        slot_of[(size_t)victim] = free_slots.back();
+        free_slots.pop_back();
+      } else {
+        slot_of[(size_t)victim] = n_slots++;
+      }
+      LOp sp;
+      sp.d.code = D_SPILL;
+      sp.d.a = (uint8_t)reg_of[(size_t)victim];
+      sp.d.dst = 0xff;
+      sp.d.imm = slot_of[(size_t)victim];
+      res.push_back(sp);
+    }
+    const int r = reg_of[(size_t)victim];
+    reg_of[(size_t)victim] = -1;
+    holder[(size_t)r] = -1;
+    return r;
+  };
+  auto release = [&](int v) {
+    if (reg_of[(size_t)v] >= 0) holder[(size_t)reg_of[(size_t)v]] = -1;
+    reg_of[(size_t)v] = -1;
+    if (slot_of[(size_t)v] >= 0) free_slots.push_back(slot_of[(size_t)v]);
+    slot_of[(size_t)v] = -1;
+  };
+  for (size_t i = 0; i < ops.size(); ++i) {
+    LOp o = ops[i];
+    // operands come back from their slots first
+    for (int v : {o.va, o.vb}) {
+      if (v < 0 || reg_of[(size_t)v] >= 0) continue;
+      if (slot_of[(size_t)v] < 0) fail("internal: operand without a register");
+      const int r = take_reg((int)i, o.va, o.vb);
+      LOp f;
+      f.d.code = D_FILL;
+      f.d.dst = (uint8_t)r;
+      f.d.imm = slot_of[(size_t)v];
+      res.push_back(f);
+      reg_of[(size_t)v] = r;
+      holder[(size_t)r] = v;
+    }
+    if (o.va >= 0) o.d.a = (uint8_t)reg_of[(size_t)o.va];
+    if (o.vb >= 0) o.d.b = (uint8_t)reg_of[(size_t)o.vb];
+    // operands whose last use is this op free their registers (the result may reuse one)
+    for (int v : {o.va, o.vb})
+      if (v >= 0 && last_use(v) == (int)i && (reg_of[(size_t)v] >= 0 || slot_of[(size_t)v] >= 0)) release(v);
+    if (o.vdst >= 0) {
+      const int r = take_reg((int)i, -1, -1);
+      o.d.dst = (uint8_t)r;
+      if (last_use(o.vdst) > (int)i) {
+        reg_of[(size_t)o.vdst] = r;
+        holder[(size_t)r] = o.vdst;
+      }
+    } else {
+      o.d.dst = 0xff;
+    }
+    res.push_back(o);
+  }
+  *regs_used = used;
+  *slots_used = n_slots;
+  return res;
+}
+
+// Table columns an op reads (loads and the chars of string operands).
+void op_columns(const DevOp& d, std::vector<int>& cols) {
+  if (d.code == D_LOAD || d.code == D_LOADS) cols.push_back(d.a);
+  const bool str_aux = (d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE || d.code == D_SLICE ||
+                       d.code == D_TRIM || d.code == D_STR_TO_I64 || d.code == D_STR_TO_F64 ||
+                       d.code == D_STR_TO_TS || d.code == D_CHAR_LENGTH;
+  if (str_aux && d.aux != 0xff) cols.push_back(d.aux);
+  if (d.code >= D_EQ_SS && d.code <= D_GE_SS && d.aux2 != 0xff) cols.push_back(d.aux2);
+}
+
+void remap_columns(DevOp& d, const std::vector<int>& map) {
+  if (d.code == D_LOAD || d.code == D_LOADS) d.a = (uint8_t)map[d.a];
+  const bool str_aux = (d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE || d.code == D_SLICE ||
+                       d.code == D_TRIM || d.code == D_STR_TO_I64 || d.code == D_STR_TO_F64 ||
+                       d.code == D_STR_TO_TS || d.code == D_CHAR_LENGTH;
+  if (str_aux && d.aux != 0xff) d.aux = (uint8_t)map[d.aux];
+  if (d.code >= D_EQ_SS && d.code <= D_GE_SS && d.aux2 != 0xff) d.aux2 = (uint8_t)map[d.aux2];
+}
+
+}  // namespace
+
 Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
               const int32_t* out_types, int n_out) {
-  if (n_in > BSQL_DEV_MAX_IN) fail("too many input columns for one launch");
+  if (n_in > 255) fail("a launch reads columns of a table of at most 255 columns");
   if (n_out > BSQL_DEV_MAX_OUT) fail("too many output columns for one launch");
+  const int budget = std::max(3, std::min(opt.reg_budget, BSQL_DEV_MAX_REG - 1));
   Lowering L{in_types, n_in, {}, {}, {}};
   L.load_value.assign(n_in, -1);
   Lowered res;
@@ -860,14 +1070,21 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
   // ---- phase 1: row selection
   if (opt.mode != LowerOptions::EXECUTE) {
     const Program& pred = *opt.pred;
-    if (pred.final_out.size() != 1) fail("a filter predicate must have exactly one output");
+    if (num_roots(pred) != 1) fail("a filter predicate must have exactly one output");
     SsaProgram s = to_ssa(L, pred, pred.seed);
     const OpTypes& t = s.types;
-    const int root = pred.final_out[0];
-    int root_type = root < 0 ? pred.root_literals[0].type_id
-                    : root < (int)s.slot_types.size() ? s.slot_types[(size_t)root] : BSQL_EMPTY;
-    for (size_t i = 0; i < pred.ops.size() && root >= 0; ++i)
-      if (pred.out[i] == root) root_type = t.ot[i];
+    const Root rt0 = root_of(pred, 0);
+    int root_type = BSQL_EMPTY;
+    if (rt0.kind == Root::LITERAL) {
+      root_type = rt0.lit.type_id;
+    } else if (rt0.kind == Root::COLUMN) {
+      root_type = in_types[rt0.index];
+    } else {
+      const int root = pred.final_out[(size_t)rt0.index];
+      root_type = root < (int)s.slot_types.size() ? s.slot_types[(size_t)root] : BSQL_EMPTY;
+      for (size_t i = 0; i < pred.ops.size(); ++i)
+        if (pred.out[i] == root) root_type = t.ot[i];
+    }
     if (root_type != BSQL_BOOL8)
       fail("Expression does not evaluate to a boolean mask");
     std::vector<int> prod = producers(s, L.values.size());
@@ -921,7 +1138,9 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
         L.emit(k);
       }
     } else {
-      for (const SourceOp& so : s.ops) L.emit(to_lop(so));
+      const std::vector<char> need = needed_ops(s, prod, s.finals);
+      for (size_t i = 0; i < s.ops.size(); ++i)
+        if (need[i]) L.emit(to_lop(s.ops[i]));
       LOp k;
       k.d.code = D_SEL;
       k.va = s.finals[0];
@@ -934,6 +1153,7 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
   if (opt.mode != LowerOptions::EXECUTE) L.load_value.assign((size_t)n_in, -1);
 
   // ---- phase 2: outputs
+  std::vector<int> out_index;  // projection output written by launch output k
   if (opt.mode == LowerOptions::FILTER_COUNT) {
     if (n_out != 0) fail("a count has no output columns");
   } else if (opt.mode == LowerOptions::FILTER_GATHER) {
@@ -943,15 +1163,30 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
     for (int k = 0; k < n_out; ++k) store_value.push_back(L.load_of(k));
   } else {
     const Program& proj = *opt.proj;
-    if ((int)proj.final_out.size() != n_out) fail("number of outputs does not match the program");
+    const int n_roots = num_roots(proj);
+    if (opt.out_subset) {
+      out_index = *opt.out_subset;
+    } else {
+      for (int k = 0; k < n_roots; ++k) out_index.push_back(k);
+    }
+    if ((int)out_index.size() != n_out) fail("number of outputs does not match the program");
     SsaProgram s = to_ssa(L, proj, proj.seed);
-    for (const SourceOp& so : s.ops) L.emit(to_lop(so));
-    store_value = s.finals;
+    std::vector<int> finals;
+    for (int k : out_index) {
+      if (k < 0 || k >= n_roots) fail("output index out of range");
+      finals.push_back(s.finals[(size_t)k]);
+    }
+    const std::vector<int> prod = producers(s, L.values.size());
+    const std::vector<char> need = needed_ops(s, prod, finals);
+    for (size_t i = 0; i < s.ops.size(); ++i)
+      if (need[i]) L.emit(to_lop(s.ops[i]));
+    store_value = finals;
   }
   res.out_str_col.assign((size_t)n_out, -1);
   for (int k = 0; k < n_out; ++k) {
     // a string value is stored as its slice; the chars are copied after the
     // kernel from the column the slice points into
+    L.emit_value_load(store_value[(size_t)k]);
     const int sc = L.values[(size_t)store_value[(size_t)k]].str_col;
     if ((sc >= 0) != is_string_type(out_types[k]))
       fail(sc >= 0 ? "a string expression needs a STRING output column"
@@ -984,80 +1219,49 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
     L.out = final_list;
   }
   const int new_split = opt.mode == LowerOptions::EXECUTE ? 0 : split;
-  int new_split_final = new_split;
 
-  // ---- load placement: column loads move to the start of their segment so
-  // that one LOADN keeps several columns in flight. Segments: the whole
-  // selection phase (eager) or each conjunct (lazy), and the output phase.
+  // ---- load placement: each segment (the whole selection phase, or each
+  // conjunct with lazy loads; the output phase) issues its column loads as
+  // early as the registers allow, so that several are in flight together.
   {
     std::vector<int> bounds{0};
     if (opt.lazy_loads)
       for (int i = 0; i < new_split; ++i)
         if (L.out[(size_t)i].d.code == D_KILL) bounds.push_back(i + 1);
-    if (new_split > 0) bounds.push_back(new_split);
-    bounds.push_back((int)L.out.size());
-    for (size_t s = 0; s + 1 < bounds.size(); ++s) {
-      auto b = L.out.begin() + bounds[s], e = L.out.begin() + bounds[s + 1];
-      if (b < e)
-        std::stable_partition(b, e, [](const LOp& o) { return o.d.code == D_LOAD; });
-    }
+    if (new_split > 0 && bounds.back() != new_split) bounds.push_back(new_split);
+    if (bounds.back() != (int)L.out.size()) bounds.push_back((int)L.out.size());
+    L.out = place_loads(L.out, bounds, L.values.size(), budget);
   }
 
-  // ---- liveness and register assignment (linear scan)
-  for (Value& v : L.values) { v.def = -1; v.last_use = -1; v.reg = -1; }
-  for (size_t i = 0; i < L.out.size(); ++i) {
-    LOp& o = L.out[i];
-    if (o.vdst >= 0) L.values[(size_t)o.vdst].def = (int)i;
-    for (int v : {o.va, o.vb})
-      if (v >= 0) L.values[(size_t)v].last_use = std::max(L.values[(size_t)v].last_use, (int)i);
-  }
-  // values read in phase 2 but defined in phase 1 stay live across the split
-  std::vector<int> free_regs;
-  // the last register of the file is the spare that result-less ops write
-  for (int r = BSQL_DEV_MAX_REG - 2; r >= 0; --r) free_regs.push_back(r);
-  int used = 0;
-  for (size_t i = 0; i < L.out.size(); ++i) {
-    LOp& o = L.out[i];
-    for (int v : {o.va, o.vb}) {
-      if (v < 0) continue;
-      Value& val = L.values[(size_t)v];
-      if (val.reg < 0) fail("internal: operand without a register");
-    }
-    // free operands whose last use is this op (dst may reuse them)
-    for (int v : {o.va, o.vb}) {
-      if (v < 0) continue;
-      Value& val = L.values[(size_t)v];
-      if (val.last_use == (int)i && val.reg >= 0) {
-        if (std::find(free_regs.begin(), free_regs.end(), val.reg) == free_regs.end())
-          free_regs.push_back(val.reg);
-      }
-    }
-    std::sort(free_regs.begin(), free_regs.end(), std::greater<int>());
-    if (o.vdst >= 0) {
-      Value& dv = L.values[(size_t)o.vdst];
-      if (free_regs.empty()) fail("program needs more than 15 live values");
-      dv.reg = free_regs.back();
-      free_regs.pop_back();
-      used = std::max(used, dv.reg + 1);
-      if (dv.last_use < 0) free_regs.push_back(dv.reg);  // dead result
-    }
-    if (o.va >= 0) o.d.a = (uint8_t)L.values[(size_t)o.va].reg;
-    if (o.vb >= 0) o.d.b = (uint8_t)L.values[(size_t)o.vb].reg;
-    if (o.vdst >= 0) o.d.dst = (uint8_t)L.values[(size_t)o.vdst].reg;
-  }
+  // ---- registers, with spill slots beyond the budget
+  int regs_used = 0, slots = 0;
+  std::vector<LOp> alloc = assign_registers(L.out, L.values.size(), budget, &regs_used, &slots);
   // loads keep their column index in `a`
-  for (LOp& o : L.out)
+  for (LOp& o : alloc)
     if (o.d.code == D_LOAD || o.d.code == D_LOADS) o.d.a = (uint8_t)L.values[(size_t)o.vdst].column;
 
-  // result-less ops write no register
-  for (LOp& o : L.out)
-    if (o.vdst < 0) o.d.dst = 0xff;
+  // ---- dense input numbering: the distinct columns this launch reads
+  {
+    std::vector<int> cols;
+    for (const LOp& o : alloc) op_columns(o.d, cols);
+    for (const DevRange& rg : res.plan_ranges) cols.push_back(rg.col);
+    std::sort(cols.begin(), cols.end());
+    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    if ((int)cols.size() > BSQL_DEV_MAX_IN) fail("a launch reads at most 128 distinct columns");
+    std::vector<int> map((size_t)std::max(n_in, 1), -1);
+    for (size_t j = 0; j < cols.size(); ++j) map[(size_t)cols[j]] = (int)j;
+    for (LOp& o : alloc) remap_columns(o.d, map);
+    for (DevRange& rg : res.plan_ranges) rg.col = map[(size_t)rg.col];
+    res.in_cols = cols;
+  }
+
   // runs of column loads (same segment) become one LOADN, which issues up to
   // four columns' loads together, and one COMMIT per column
+  int new_split_final = new_split;
   {
     std::vector<LOp> merged;
-    for (size_t i = 0; i < L.out.size();) {
-      if (L.out[i].d.code != D_LOAD) { merged.push_back(L.out[i]); ++i; continue; }
+    for (size_t i = 0; i < alloc.size();) {
+      if (alloc[i].d.code != D_LOAD) { merged.push_back(alloc[i]); ++i; continue; }
       LOp issue;
       issue.d.code = D_LOADN;
       issue.d.dst = 0xff;
@@ -1065,12 +1269,12 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
       std::vector<LOp> commits;
       int n = 0;
       size_t j = i;
-      while (j < L.out.size() && L.out[j].d.code == D_LOAD && n < 4) {
-        issue.d.imm |= (int64_t)((uint64_t)L.out[j].d.a << (8 * n));
+      while (j < alloc.size() && alloc[j].d.code == D_LOAD && n < 4) {
+        issue.d.imm |= (int64_t)((uint64_t)alloc[j].d.a << (8 * n));
         LOp c;
         c.d.code = D_COMMIT;
         c.d.aux = (uint8_t)n;
-        c.d.dst = L.out[j].d.dst;
+        c.d.dst = alloc[j].d.dst;
         commits.push_back(c);
         ++n;
         ++j;
@@ -1080,27 +1284,31 @@ Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
       for (const LOp& c : commits) merged.push_back(c);
       i = j;
     }
-    L.out = merged;
+    alloc = merged;
     if (opt.mode != LowerOptions::EXECUTE) {
       // the selection phase ends after its last KILL/SEL
       int last_sel = -1;
-      for (size_t i = 0; i < L.out.size(); ++i)
-        if (L.out[i].d.code == D_KILL || L.out[i].d.code == D_SEL) last_sel = (int)i;
+      for (size_t i = 0; i < alloc.size(); ++i)
+        if (alloc[i].d.code == D_KILL || alloc[i].d.code == D_SEL) last_sel = (int)i;
       new_split_final = last_sel + 1;
     }
   }
 
-  if (L.out.size() > BSQL_DEV_MAX_OPS) fail("program too long for one launch");
+  if (alloc.size() > ((size_t)1 << 20)) fail("program too long for one launch");
   std::memset(&res.plan, 0, sizeof(res.plan));
-  res.plan.n_ops = (int32_t)L.out.size();
+  res.plan.n_ops = (int32_t)alloc.size();
   res.plan.split = new_split_final;
-  res.plan.n_in = n_in;
+  res.plan.n_in = (int32_t)res.in_cols.size();
   res.plan.n_out = n_out;
-  for (size_t i = 0; i < L.out.size(); ++i) res.plan.ops[i] = L.out[i].d;
+  for (const LOp& o : alloc) res.ops.push_back(o.d);
+  if (res.ops.empty()) res.ops.push_back(DevOp{});  // one NOP so that the list is never empty
   res.plan.n_ranges = (int32_t)res.plan_ranges.size();
   res.plan.range_eager = opt.lazy_loads ? 0 : 1;
+  res.plan.n_spill = slots;
   for (size_t i = 0; i < res.plan_ranges.size(); ++i) res.plan.ranges[i] = res.plan_ranges[i];
-  res.nreg = used + 1;  // + the spare register
+  res.n_spill = slots;
+  res.nreg = slots > 0 ? BSQL_DEV_MAX_REG : regs_used + 1;  // + the spare register
+  if (slots > 0 && budget < BSQL_DEV_MAX_REG - 1) res.nreg = budget + 1;
   return res;
 }
 

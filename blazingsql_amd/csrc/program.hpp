@@ -58,14 +58,29 @@ struct Step {
   std::shared_ptr<void> d_lits;  // device copy of the literal arguments, made on first use
 };
 
+// One expression of a compiled list (evaluate_expressions, LogicalProject.cpp:
+// 810-852): interpreted (output `index` of the encoded program), a bare column
+// (the reference copies table column `index`, or moves a column a step built),
+// or a literal (make_column_from_scalar; lit.is_null for a null literal).
+struct Root {
+  enum Kind { VM, COLUMN, LITERAL } kind = VM;
+  int index = 0;
+  Scalar lit;
+};
+
 struct Program {
+  // the reference encoding of the interpreted expressions only
   std::vector<int16_t> left, right, out, final_out;
   std::vector<int32_t> ops;
   std::vector<Scalar> lsc, rsc;
-  // literal expression roots (evaluate_expressions fills their column without
-  // the interpreter, LogicalProject.cpp:822-832): final_out holds
-  // BSQL_SCALAR_INDEX or BSQL_SCALAR_NULL_INDEX and root_literals[k] the value
-  std::vector<Scalar> root_literals;
+  // every expression of the list, in order (empty: output k is final_out[k],
+  // as for programs built from arrays)
+  std::vector<Root> roots;
+  // an expression list whose encoding reaches slot 64 is evaluated as halves
+  // (LogicalProject.cpp:906-922): each part is a program of its own over the
+  // same table, and part_outputs[j] lists the outputs part j produces
+  std::vector<std::shared_ptr<Program>> parts;
+  std::vector<std::vector<int>> part_outputs;
   // schema hints from the compiler: number of interpreter inputs and the
   // table columns they come from (col_idx_map of LogicalProject.cpp:859-881)
   int n_inputs = -1;
@@ -84,8 +99,23 @@ struct Program {
 };
 
 // A program allocated as a C ABI handle (it carries the lowering cache), for
-// the sub-programs of steps.
+// the sub-programs of steps and the parts of split lists.
 std::shared_ptr<Program> make_program_handle(std::unique_ptr<Program> p);
+
+// A fresh unique program id.
+uint64_t next_program_id();
+
+// Number of expressions (outputs) of a program.
+inline int num_roots(const Program& p) {
+  return p.roots.empty() ? (int)p.final_out.size() : (int)p.roots.size();
+}
+// Root k of a program (VM k for programs without a root list).
+inline Root root_of(const Program& p, int k) {
+  if (!p.roots.empty()) return p.roots[(size_t)k];
+  Root r;
+  r.index = k;
+  return r;
+}
 
 // type helpers (cudf type_id semantics)
 bool is_float_type(int t);
@@ -136,14 +166,22 @@ struct LowerOptions {
   bool lazy_loads = false;         // load each conjunct's columns only when it is reached
   bool ranges = true;              // column-vs-literal conjuncts become range tests
   const std::vector<int32_t>* conj_order = nullptr;
+  // outputs of the projection this launch writes (empty: all of them)
+  const std::vector<int>* out_subset = nullptr;
+  // registers the allocator may use before it spills (the file minus the
+  // spare): 15 for a 16-register file, 7 for an 8-register file
+  int reg_budget = BSQL_DEV_MAX_REG - 1;
 };
 
 struct Lowered {
-  DevPlan plan;          // columns are filled by the launcher
-  int nreg = 0;          // registers used
+  DevPlan plan;          // columns are filled by the launcher; plan.ops is not set
+  std::vector<DevOp> ops;  // the op list (plan.n_ops entries, at least one)
+  int nreg = 0;          // registers used (+ the spare)
+  int n_spill = 0;       // LDS spill slots
   int n_conjuncts = 0;   // conjuncts found in the predicate (0 = not an AND chain)
   std::vector<DevRange> plan_ranges;  // the selection as range tests (empty: ops)
-  std::vector<int> out_str_col;       // per output: source column of a string output, else -1
+  std::vector<int> out_str_col;       // per output: source table column of a string output, else -1
+  std::vector<int> in_cols;           // device input j reads table column in_cols[j]
 };
 Lowered lower(const LowerOptions& opt, const int32_t* in_types, int n_in,
               const int32_t* out_types, int n_out);
@@ -158,6 +196,18 @@ std::vector<DevRange> conjunct_ranges(const Program& p, const int32_t* in_types,
 // Text front-end (parser.cpp): Calcite expression strings -> Program.
 std::unique_ptr<Program> compile_expressions(const std::vector<std::string>& exprs,
                                              const int32_t* in_types, int n_in);
+
+// preprocess_expression (expression_utils.cpp:1624-1632): unary minus,
+// n-ary CONCAT, CURRENT_DATE/TIMESTAMP, ms -> ns units, Reinterpret
+// timestamp units and interval units, for a table schema.
+std::string preprocess_expression(const std::string& e, const int32_t* types, int n);
+// One text stage of the front end (for tests): 0 fill_minus_op_with_zero,
+// 1 n-ary CONCAT, 2 ms -> ns, 3 Reinterpret, 4 intervals, 5 all of
+// preprocess_expression, 6 replace_calcite_regex, 7 + expand_if_logical_op,
+// 8 parse + custom-op rewrites, rebuilt as text (rebuildExpression).
+std::string rewrite_expression(const std::string& e, int stage, const int32_t* types, int n);
+// The timestamp CURRENT_DATE / CURRENT_TIMESTAMP stand for ("" = the clock).
+void set_current_timestamp(const std::string& ts);
 
 // Query-part text of the operators (parser.cpp).
 // `name=[[...]]` or `name=[...]` of a query part; "" when absent.

@@ -108,7 +108,9 @@ const uint8_t* step_literals(bsql_program& p, Step& st, std::vector<int32_t>* at
       if (d) (void)hipFree(d);
       throw Error(device_error(e, "step literals"), "step literals");
     }
-    st.d_lits = std::shared_ptr<void>(d, [](void* q) { (void)hipFree(q); });
+    st.d_lits = std::shared_ptr<void>(d, [](void* q) {
+      if (!process_exiting()) (void)hipFree(q);
+    });
   }
   return (const uint8_t*)st.d_lits.get();
 }
@@ -212,6 +214,8 @@ bsql_program* rebased_program(bsql_program* p, int n_table) {
   auto move_cols = [&](Program& q) {
     for (int32_t& c : q.used_columns)
       if (c >= base) c += shift;
+    for (Root& rt : q.roots)
+      if (rt.kind == Root::COLUMN && rt.index >= base) rt.index += shift;
   };
   std::unique_ptr<bsql_program> r(new bsql_program());
   static_cast<Program&>(*r) = static_cast<const Program&>(*p);

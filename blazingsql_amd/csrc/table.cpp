@@ -206,6 +206,15 @@ TablePtr filter_table(const bsql_table& in, bsql_program* pred, const std::vecto
   return res;
 }
 
+// How output k of a compiled list is produced (through its part if the list
+// was split).
+Root output_root(const Program& p, int k) {
+  for (size_t j = 0; j < p.parts.size(); ++j)
+    for (size_t i = 0; i < p.part_outputs[j].size(); ++i)
+      if (p.part_outputs[j][i] == k) return output_root(*p.parts[j], (int)i);
+  return root_of(p, k);
+}
+
 TablePtr evaluate(const bsql_table& in, const std::vector<std::string>& exprs,
                   const std::vector<std::string>& names, void* stream) {
   const std::vector<int32_t> types = types_of(in);
@@ -218,8 +227,16 @@ TablePtr evaluate(const bsql_table& in, const std::vector<std::string>& exprs,
   std::vector<bsql_column> outs;
   for (size_t k = 0; k < exprs.size(); ++k) {
     const int32_t t = output_type(prog.get(), types, (int)k);
-    // outputs always carry validity (make_fixed_width_column(..., UNINITIALIZED))
-    add_column(*res, t, k < names.size() ? names[k] : std::string(), in.num_rows, true,
+    // interpreted outputs always carry validity (make_fixed_width_column(...,
+    // UNINITIALIZED)); a copied table column keeps its own nullability; a
+    // literal column has a mask only for a null literal
+    // (make_column_from_scalar) (LogicalProject.cpp:822-839)
+    const Root r = output_root(*prog, (int)k);
+    bool nullable = true;
+    if (r.kind == Root::COLUMN && r.index < (int)in.cols.size())
+      nullable = in.cols[(size_t)r.index].view.validity != nullptr;
+    if (r.kind == Root::LITERAL) nullable = r.lit.is_null;
+    add_column(*res, t, k < names.size() ? names[k] : std::string(), in.num_rows, nullable,
                t == BSQL_STRING ? chars_bound : 0);
     outs.push_back(res->cols.back().view);
   }

@@ -65,6 +65,8 @@ class Program:
 
     def __del__(self):
         h = getattr(self, "_h", None)
+        if L.exiting():
+            return
         if h is not None and h.value and getattr(self, "_owner", None) is None:
             try:
                 L.lib().bsql_program_free(h)
@@ -112,8 +114,53 @@ class Program:
     def num_inputs(self):
         return L.lib().bsql_program_num_inputs(self._h)
 
+    @property
+    def num_interpreted(self):
+        """Outputs the interpreter produces (the encoded program's final_out)."""
+        return L.lib().bsql_program_num_interpreted(self._h)
+
+    def roots(self):
+        """How each output is produced: ("vm", j) for interpreted output j,
+        ("column", c) for a copy of column c, ("literal", (type, value)) or
+        ("null", type) for a filled literal."""
+        out = []
+        for k in range(self.num_outputs):
+            kind, idx = ctypes.c_int32(), ctypes.c_int32()
+            s = L.Scalar()
+            L.check(L.lib().bsql_program_root(self._h, k, ctypes.byref(kind), ctypes.byref(idx), ctypes.byref(s)))
+            if kind.value == L.ROOT_INTERPRETED:
+                out.append(("vm", idx.value))
+            elif kind.value == L.ROOT_COLUMN:
+                out.append(("column", idx.value))
+            elif s.str_len == -1:
+                out.append(("null", s.type_id))
+            else:
+                out.append(("literal", (s.type_id, s.v.f64 if is_float(s.type_id) else s.v.i64)))
+        return out
+
+    def parts(self):
+        """[(part Program, [output indices])] of a list split at 64 slots
+        (empty for a program in one piece)."""
+        res = []
+        for j in range(L.lib().bsql_program_num_parts(self._h)):
+            h = ctypes.c_void_p()
+            buf = (ctypes.c_int32 * 4096)()
+            n = L.lib().bsql_program_part(self._h, j, ctypes.byref(h), buf, 4096)
+            if n < 0:
+                L.check(-n)
+            res.append((Program(h.value, owner=self), list(buf)[:n]))
+        return res
+
+    def plan_info(self, in_types):
+        """Shape of the device plan for bsql_interp_execute over these column
+        types (host-side lowering; no device needed)."""
+        info = L.PlanInfo()
+        t = (ctypes.c_int32 * max(len(in_types), 1))(*in_types)
+        L.check(L.lib().bsql_program_plan_info(self._h, t, len(in_types), ctypes.byref(info)))
+        return {f: getattr(info, f) for f, _ in L.PlanInfo._fields_}
+
     def arrays(self):
-        n, m = self.num_ops, self.num_outputs
+        n, m = self.num_ops, self.num_interpreted
         left = (ctypes.c_int16 * max(n, 1))()
         right = (ctypes.c_int16 * max(n, 1))()
         out = (ctypes.c_int16 * max(n, 1))()
@@ -193,21 +240,45 @@ class Program:
             arr = (ctypes.c_int32 * len(order))(*order)
             L.check(L.lib().bsql_program_set_conjunct_order(self._h, arr, len(order)))
 
-    def set_launch(self, rows_per_thread=0, block_size=0, ranges=True, lazy_loads=False):
-        """Launch shape for this program; for a predicate also whether its
-        column-vs-literal conjuncts run as range tests and whether conjunct
-        columns are loaded lazily."""
-        flags = (0 if ranges else L.LAUNCH_NO_RANGES) | (L.LAUNCH_LAZY_LOADS if lazy_loads else 0)
+    def set_launch(self, rows_per_thread=0, block_size=0, ranges=True, lazy_loads=False, spill_wide=False):
+        """Launch shape for this program (block_size 0, 64 or 256); for a
+        predicate also whether its column-vs-literal conjuncts run as range
+        tests and whether conjunct columns are loaded lazily; spill_wide keeps
+        at most 7 live values in registers and the rest in LDS slots."""
+        flags = ((0 if ranges else L.LAUNCH_NO_RANGES) | (L.LAUNCH_LAZY_LOADS if lazy_loads else 0) |
+                 (L.LAUNCH_SPILL_WIDE if spill_wide else 0))
         cfg = L.LaunchConfig(block_size, rows_per_thread, flags)
         L.check(L.lib().bsql_program_set_launch(self._h, ctypes.byref(cfg)))
 
     def to_oracle_dict(self):
-        """Arrays + scalars in the form the CPU oracle consumes."""
+        """Arrays + scalars (the interpreted outputs) in the form the CPU
+        oracle consumes; "roots" says how every output is produced."""
         a = self.arrays()
+        a["roots"] = self.roots()
         n = len(a["ops"])
         a["left_scalars"] = [self.scalar(i, 0) if a["left"][i] == L.SCALAR_INDEX else None for i in range(n)]
         a["right_scalars"] = [self.scalar(i, 1) if a["right"][i] == L.SCALAR_INDEX else None for i in range(n)]
         return a
+
+
+def rewrite(expr, stage, types=()):
+    """One text stage of the expression front end (bsql_expression_rewrite):
+    0 fill_minus_op_with_zero, 1 n-ary CONCAT, 2 ms -> ns units, 3 Reinterpret
+    timestamps, 4 interval units, 5 preprocess_expression, 6
+    replace_calcite_regex, 7 + expand_if_logical_op, 8 parse + custom-op
+    rewrites rebuilt as text."""
+    t = (ctypes.c_int32 * max(len(types), 1))(*types)
+    n = ctypes.c_size_t()
+    L.check(L.lib().bsql_expression_rewrite(expr.encode(), stage, t, len(types), None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    L.check(L.lib().bsql_expression_rewrite(expr.encode(), stage, t, len(types), buf, n.value + 1,
+                                            ctypes.byref(n)))
+    return buf.value.decode()
+
+
+def set_current_timestamp(ts):
+    """The timestamp CURRENT_DATE / CURRENT_TIMESTAMP stand for (None: the clock)."""
+    L.check(L.lib().bsql_set_current_timestamp(ts.encode() if ts else None))
 
 
 def _scalar_array(scalars, n):
@@ -432,7 +503,7 @@ class Batch:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h:
+        if h and not L.exiting():
             L.lib().bsql_batch_free(h)
             self.handle = None
 

@@ -75,7 +75,7 @@ class Mlp:
     def __del__(self):
         h = getattr(self, "handle", None)
         lib = getattr(L, "lib", None)   # module globals are gone at interpreter exit
-        if h and lib is not None:
+        if h and lib is not None and not L.exiting():
             lib().bsql_mlp_free(h)
             self.handle = None
 

@@ -180,6 +180,8 @@ class Table:
             L.lib().bsql_table_free(self._release())
 
     def __del__(self):
+        if L.exiting():
+            return
         try:
             self.free()
         except Exception:  # noqa: BLE001 -- interpreter shutdown

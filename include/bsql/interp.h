@@ -153,21 +153,45 @@ int bsql_program_from_arrays(const int16_t* left, const int16_t* right,
  * (engine/src/execution_kernels/LogicalProject.cpp:796-941): regex rewrite,
  * AND/OR expansion, parse, CASE/ROUND/CAST rewrites, slot allocation.
  * `in_types[n_in]` is the schema of the table the expressions refer to ($k).
- * Expressions whose root is a bare column or literal are still compiled
- * (as a copy) so that every expression gets an output slot.               */
+ * As in the reference, only expressions with an operator at the root are
+ * encoded for the interpreter; a bare column is copied and a literal fills
+ * its column (bsql_program_root). The launches write every output.        */
 int bsql_program_compile(const char* const* exprs, int n_exprs,
                          const int32_t* in_types, int n_in,
                          bsql_program** prog);
 
 void bsql_program_free(bsql_program* prog);
 
-/* Introspection (reference encoding, after compile): counts and arrays. */
+/* Introspection (reference encoding, after compile): counts and arrays.
+ * num_outputs counts every expression of the list; the encoded program holds
+ * only the interpreted ones (num_interpreted of them, final_out[j] = n_in + j
+ * as add_expression_to_interpreter_plan numbers them,
+ * LogicalProject.cpp:883-904). */
 int bsql_program_num_ops(const bsql_program* prog);
 int bsql_program_num_outputs(const bsql_program* prog);
 int bsql_program_num_inputs(const bsql_program* prog);
-/* Fills caller arrays of length num_ops / num_outputs; any may be NULL. */
+int bsql_program_num_interpreted(const bsql_program* prog);
+/* Fills caller arrays of length num_ops / num_interpreted; any may be NULL.
+ * Fails for a program split into parts (below). */
 int bsql_program_arrays(const bsql_program* prog, int16_t* left, int16_t* right,
                         int16_t* out, int32_t* ops, int16_t* final_out);
+/* How expression k is produced (evaluate_expressions, LogicalProject.cpp:
+ * 810-852): by the interpreter (index = its position among the interpreted
+ * outputs), as a copy of a column (index = table column; a column a step
+ * builds is moved, the table's copied), or filled with a literal (index -1;
+ * *literal gets its type and value, str_len -1 for a null literal). */
+enum { BSQL_ROOT_INTERPRETED = 0, BSQL_ROOT_COLUMN = 1, BSQL_ROOT_LITERAL = 2 };
+int bsql_program_root(const bsql_program* prog, int k, int32_t* kind, int32_t* index,
+                      bsql_scalar* literal);
+/* An expression list whose encoding would use a slot >= 64 is compiled as
+ * halves, recursively, each a program of its own over the same table
+ * (LogicalProject.cpp:906-922); launches run the parts one after another.
+ * num_parts is 0 for a program in one piece. bsql_program_part gives part j
+ * (owned by prog) and which outputs of prog it produces (up to cap written);
+ * returns their number. */
+int bsql_program_num_parts(const bsql_program* prog);
+int bsql_program_part(const bsql_program* prog, int j, const bsql_program** part,
+                      int32_t* outputs, int cap);
 /* Scalar of op i (side 0 = left, 1 = right); string pointer stays owned by prog. */
 int bsql_program_scalar(const bsql_program* prog, int op, int side,
                         bsql_scalar* out);
@@ -277,11 +301,16 @@ int bsql_program_set_conjunct_order(bsql_program* pred, const int32_t* order,
 enum {
   BSQL_LAUNCH_NO_RANGES = 1,  /* predicates: evaluate column-vs-literal conjuncts
                                  with the general ops instead of range tests */
-  BSQL_LAUNCH_LAZY_LOADS = 2  /* predicates: load a conjunct's columns only when
-                                 the wave reaches it (default: all up front) */
+  BSQL_LAUNCH_LAZY_LOADS = 2, /* predicates: load a conjunct's columns only when
+                                 the wave reaches it (default: as early as
+                                 registers allow) */
+  BSQL_LAUNCH_SPILL_WIDE = 4  /* spill layout: keep at most 7 live values in
+                                 registers (the 8-register file, up to 4 rows
+                                 per lane) and the rest in LDS slots; default
+                                 15 in registers (16-register file) */
 };
 typedef struct bsql_launch_config {
-  int32_t block_size;     /* threads per workgroup: 0 (auto) or 256 */
+  int32_t block_size;     /* threads per workgroup: 0 (auto), 64 or 256 */
   int32_t rows_per_thread;/* 0 (auto), 1, 2, 4; range-test selection: 8, 16, 32 */
   int32_t flags;          /* BSQL_LAUNCH_* */
 } bsql_launch_config;
@@ -290,6 +319,17 @@ typedef struct bsql_launch_config {
  * reference, which sizes its launch with the occupancy API
  * (interpreter_cpp.cu:76-270, :473). */
 int bsql_program_set_launch(bsql_program* prog, const bsql_launch_config* cfg);
+
+/* Shape of the device plan a program lowers to for bsql_interp_execute over
+ * a table of these column types (its lowering runs on the host, so this
+ * needs no device): device ops, register file (registers + the spare), LDS
+ * spill slots and the spill/fill ops, distinct input columns, load issues
+ * and the most columns one issue keeps in flight. No counterpart in the
+ * reference (its slots are LDS cells of the program's own numbering). */
+typedef struct bsql_plan_info {
+  int32_t n_ops, n_regs, n_spill, n_spill_ops, n_fill_ops, n_inputs, n_load_issues, max_loads_in_flight;
+} bsql_plan_info;
+int bsql_program_plan_info(const bsql_program* prog, const int32_t* in_types, int n_in, bsql_plan_info* info);
 
 /* ---- batched execution ----------------------------------------------------
  * Many batches, each with its own program and columns (the uneven batches an
@@ -373,6 +413,27 @@ int bsql_order_by_prediction(bsql_program* pred, const float* selectivity, const
  * in the reference (its plans keep Calcite's order). */
 int bsql_order_conjuncts(bsql_program* pred, const int32_t* in_types, int n_in, const float* selectivity,
                          const float* cost, int n_conj, int32_t* order_out);
+
+/* ---- front-end text stages ------------------------------------------------
+ * bsql_expression_rewrite returns one text stage of the expression front end
+ * (the rewrites bsql_program_compile applies), for tests and tooling:
+ *   0 fill_minus_op_with_zero        engine/src/parser/expression_utils.cpp:1155-1164
+ *   1 convert_nary_to_binary_concat  expression_utils.cpp:1520-1622
+ *   2 convert_ms_to_ns_units         expression_utils.cpp:1316-1350
+ *   3 reinterpret_timestamp          expression_utils.cpp:1371-1421 (types: schema)
+ *   4 apply_interval_conversion      expression_utils.cpp:1425-1469 (types: schema)
+ *   5 preprocess_expression (0..4 and CURRENT_DATE)   expression_utils.cpp:1624-1632
+ *   6 replace_calcite_regex          expression_utils.cpp:1068-1113
+ *   7 6 then expand_if_logical_op    CalciteExpressionParsing.cpp:330-398
+ *   8 parse_tree::build + transform_to_custom_op + rebuildExpression
+ *                                    engine/src/parser/expression_tree.hpp:147-162, 181-258
+ * Writes at most cap-1 bytes and a NUL, and the full length to *len. */
+int bsql_expression_rewrite(const char* expr, int stage, const int32_t* types, int n_types, char* out,
+                            size_t cap, size_t* len);
+/* The query's timestamp that CURRENT_DATE / CURRENT_TIMESTAMP / CURRENT_TIME
+ * stand for, "YYYY-MM-DD hh:mm:ss.mmm" (Context::getCurrentTimestamp in the
+ * reference); NULL or "" uses the clock at compile time. */
+int bsql_set_current_timestamp(const char* ts);
 
 /* ---- diagnostics --------------------------------------------------------- */
 /* Message of the last error on the calling thread. */

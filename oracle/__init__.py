@@ -317,3 +317,85 @@ def run_steps(prog, cols, num_rows):
         cols.append(HostColumn(STRING, strings=[r if r is not None else "" for r in res],
                                valid=[r is not None for r in res]))
     return cols
+
+
+def _column_values(col, n):
+    """(values or strings, validity) of the first n rows of a host column."""
+    valid = np.ones(n, dtype=bool) if col.valid is None else np.asarray(col.valid[:n], dtype=bool)
+    if col.type_id == STRING:
+        strs = [bytes(col.chars[col.offsets[i]:col.offsets[i + 1]]).decode() for i in range(n)]
+        return strs, valid
+    return np.asarray(col.values[:n]), valid
+
+
+def evaluate(prog, cols, num_rows=None, threads=0, seed=0x5eed5eed12345678):
+    """evaluate_expressions on the host (LogicalProject.cpp:796-941) for a
+    compiled program (blazingsql_amd Program) over the table's host columns:
+    the columns its steps build, then each output as the reference produces
+    it -- interpreted outputs by perform_interpreter_operation over the used
+    columns, a bare column root as a copy of that column, a literal root
+    filled (a null literal: all rows null) -- part by part for a list split
+    at 64 slots. Returns [(values or strings, validity)] per output."""
+    num_rows = cols[0].size if num_rows is None else num_rows
+    parts = prog.parts()
+    if parts:
+        res = [None] * prog.num_outputs
+        for part, outs in parts:
+            for k, v in zip(outs, evaluate(part, cols, num_rows, threads, seed)):
+                res[k] = v
+        return res
+    types = [c.type_id for c in cols]
+    ext = run_steps(prog, cols, num_rows) if prog.steps() else list(cols)
+    roots = prog.roots()
+    out_types = [prog.output_type(types, k) for k in range(len(roots))]
+    vm = [k for k, r in enumerate(roots) if r[0] == "vm"]
+    vm_res = {}
+    if vm:
+        d = prog.to_oracle_dict()
+        got = execute(d, [ext[c] for c in prog.used_columns()], [out_types[k] for k in vm], num_rows,
+                      threads, seed)
+        for k, v in zip(vm, got):
+            vm_res[k] = v
+    res = []
+    for k, r in enumerate(roots):
+        if r[0] == "vm":
+            res.append(vm_res[k])
+        elif r[0] == "column":
+            res.append(_column_values(ext[r[1]], num_rows))
+        elif r[0] == "literal":
+            t, v = r[1]
+            res.append((np.full(num_rows, v, dtype=_NP[t]), np.ones(num_rows, dtype=bool)))
+        else:
+            res.append((np.zeros(num_rows, dtype=_NP.get(r[1], np.int64)), np.zeros(num_rows, dtype=bool)))
+    return res
+
+
+def full_table_program(prog, n_table):
+    """The oracle program of a compiled program over its whole table (table
+    columns, then the columns its steps build), for oracle.filter_project and
+    oracle.filter_table: the interpreted encoding with its input slots mapped
+    back to table columns and its other slots shifted past them, and every
+    output listed -- an interpreted one at its shifted slot, a bare column
+    root at the column's own slot (the reference copies the column).
+    Literal roots are not expressible this way (use evaluate)."""
+    d = prog.to_oracle_dict()
+    used = prog.used_columns()
+    n_in, n_tab = len(used), n_table + len(prog.steps())
+
+    def remap(s):
+        if s < 0:
+            return s
+        return used[s] if s < n_in else s - n_in + n_tab
+
+    for key in ("left", "right", "out", "final_out"):
+        d[key] = [remap(s) for s in d[key]]
+    final = []
+    for r in d.pop("roots"):
+        if r[0] == "vm":
+            final.append(d["final_out"][r[1]])
+        elif r[0] == "column":
+            final.append(r[1])
+        else:
+            raise ValueError("a literal root has no slot in an oracle program")
+    d["final_out"] = final
+    return d

@@ -158,14 +158,12 @@ def eval_program_both(bsql, prog_text_list, table, n=None):
     from blazingsql_amd.interop import DeviceColumn, perform_interpreter_operation
     types = [t for t, _, _ in table]
     p = bsql.Program.compile(prog_text_list, types)
-    used = p.used_columns()
-    sub = [table[c] for c in used]
     out_types = [p.output_type(types, k) for k in range(p.num_outputs)]
     n = len(table[0][1]) if n is None else n
     dev_in = device_columns(table)
     dev_out = [DeviceColumn.empty(t, n) for t in out_types]
     perform_interpreter_operation(p, dev_in, dev_out, n)
-    ora = oracle.execute(p.to_oracle_dict(), host_columns(sub), out_types, n)
+    ora = oracle.evaluate(p, host_columns(table), n)
     res = []
     for k in range(len(out_types)):
         gv, gvalid = dev_out[k].to_numpy(n)

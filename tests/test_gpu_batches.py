@@ -44,8 +44,7 @@ def _make_batches(bsql, seed, n_batches, max_rows):
 def _check(batches, what):
     import oracle
     for i, (prog, table, _, outs, rows, ot) in enumerate(batches):
-        used = prog.used_columns()
-        ores = oracle.execute(prog.to_oracle_dict(), host_columns([table[c] for c in used]), ot, rows)
+        ores = oracle.evaluate(prog, host_columns(table), rows)
         for k in range(len(ot)):
             gv, gval = outs[k].to_numpy(rows)
             assert_same(gv, gval, ores[k][0], ores[k][1], what=f"{what} batch {i} output {k}")
@@ -123,8 +122,7 @@ def test_program_autotune(bsql, gpu):
     assert r in (1, 2, 4)
     perform_interpreter_operation(prog, device_columns(table), outs, n)
     torch.cuda.synchronize()
-    used = prog.used_columns()
-    ores = oracle.execute(prog.to_oracle_dict(), host_columns([table[c] for c in used]), ot, n)
+    ores = oracle.evaluate(prog, host_columns(table), n)
     for k in range(2):
         gv, gval = outs[k].to_numpy(n)
         assert_same(gv, gval, ores[k][0], ores[k][1], what=f"output {k}")

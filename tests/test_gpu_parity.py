@@ -199,19 +199,8 @@ def test_filter_project_c1_vs_oracle(bsql, gpu, n):
 
 def _full_prog(bsql, exprs, types):
     """A program over the full table for the oracle (identity column map)."""
-    p = bsql.Program.compile(exprs, types)
-    d = p.to_oracle_dict()
-    used = p.used_columns()
-    # remap input slots back to table columns (the table's, then the columns
-    # its steps build); shift other slots past them
-    n_in, n_tab = len(used), len(types) + len(p.steps())
-    def remap(s):
-        if s < 0:
-            return s
-        return used[s] if s < n_in else s - n_in + n_tab
-    for key in ("left", "right", "out", "final_out"):
-        d[key] = [remap(s) for s in d[key]]
-    return d
+    import oracle
+    return oracle.full_table_program(bsql.Program.compile(exprs, types), len(types))
 
 
 def c2_table(n, seed=11):

@@ -25,9 +25,7 @@ def _run(bsql, exprs, cols, types, n):
 
 def _oracle(bsql, p, host_cols, ot, n):
     import oracle
-    used = p.used_columns()
-    host = oracle.run_steps(p, host_cols, n)
-    return oracle.execute(p.to_oracle_dict(), [host[c] for c in used], ot, n)
+    return oracle.evaluate(p, host_cols, n)
 
 
 def _same(g, o, what):

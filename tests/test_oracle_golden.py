@@ -467,9 +467,8 @@ def test_trim_closed_forms(oracle_lib, bsql):
 
 
 def _steps_and_execute(oracle_lib, p, cols, types, n):
-    host = oracle_lib.run_steps(p, cols, n)
     ot = [p.output_type(types, k) for k in range(p.num_outputs)]
-    return ot, oracle_lib.execute(p.to_oracle_dict(), [host[c] for c in p.used_columns()], ot, n)
+    return ot, oracle_lib.evaluate(p, cols, n)
 
 
 def test_string_building_golden(oracle_lib, golden, bsql):
