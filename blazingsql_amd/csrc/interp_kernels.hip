@@ -44,6 +44,11 @@ typedef uint32_t v2u32 __attribute__((ext_vector_type(2)));
 // every column access goes through a global (address space 1) pointer.
 #define GLOBAL __attribute__((address_space(1)))
 #define LDS __attribute__((address_space(3)))
+// The launch plan and its op list are never written while a kernel runs:
+// read through the constant address space, their wave-uniform loads are
+// scalar loads (a generic pointer would make them vector loads, and every
+// op field derived from them divergent).
+#define CONSTANT __attribute__((address_space(4)))
 template <class T>
 __device__ __forceinline__ const GLOBAL T* gp(const void* p) { return (const GLOBAL T*)p; }
 template <class T>
@@ -943,7 +948,7 @@ __device__ int64_t look_back(const CompactArgs& ca, int64_t tile, int64_t agg, i
 // One op of the plan, read as two dwords and the literal so that the whole
 // wave reads it with scalar loads (the byte fields are unpacked in SGPRs).
 __device__ __forceinline__ DevOp load_op(const DevPlan& plan, int q) {
-  const uint32_t* p = (const uint32_t*)(plan.ops + q);
+  const CONSTANT uint32_t* p = (const CONSTANT uint32_t*)(plan.ops + q);
   const uint32_t w0 = p[0], w1 = p[1];
   DevOp op;
   op.code = (uint8_t)w0;
@@ -954,7 +959,7 @@ __device__ __forceinline__ DevOp load_op(const DevPlan& plan, int q) {
   op.aux = (uint8_t)(w1 >> 8);
   op.aux2 = (uint8_t)(w1 >> 16);
   op.pad = 0;
-  op.imm = *(const int64_t*)(p + 2);
+  op.imm = *(const CONSTANT int64_t*)(p + 2);
   return op;
 }
 
