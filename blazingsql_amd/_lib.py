@@ -118,6 +118,7 @@ SIGNATURES = {
     "bsql_interp_execute": (_I, [_P, _P, _I, _P, _I, _I64, _P]),
     "bsql_filter": (_I, [_P, _P, _I, _P, _I64, _P, _P]),
     "bsql_filter_project": (_I, [_P, _P, _P, _I, _P, _I, _I64, _P, _P]),
+    "bsql_filter_project_noalloc": (_I, [_P, _P, _P, _I, _P, _I, _I64, _P, _P]),
     "bsql_program_num_conjuncts": (_I, [_P]),
     "bsql_program_set_conjunct_order": (_I, [_P, _P, _I]),
     "bsql_program_set_launch": (_I, [_P, ctypes.POINTER(LaunchConfig)]),

@@ -15,7 +15,7 @@ OUT = os.path.join(HERE, "libbsql.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ROCM_INC = "/opt/rocm/include"
 
-HIP_SOURCES = ["interp_kernels.hip", "learned.hip", "strfn.hip"]
+HIP_SOURCES = ["interp_kernels.hip", "learned.hip", "strfn.hip", "noalloc.hip"]
 CXX_SOURCES = ["program.cpp", "parser.cpp", "capi.cpp", "learned.cpp", "table.cpp", "steps.cpp", "regex.cpp"]
 
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -469,6 +469,87 @@ int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, i
   return run_compact(pred, proj, in, n_in, out, n_out, num_rows, capacity, d_count, stream, phases, nullptr);
 }
 
+size_t noalloc_scratch_bytes();
+hipError_t noalloc_scratch(uint64_t** out);
+hipError_t write_words_async(uint64_t* dst, const uint64_t* src, size_t n, hipStream_t stream);
+
+namespace {
+
+bool has_string_literals(const Program& p) {
+  for (size_t i = 0; i < p.ops.size(); ++i)
+    if ((p.left[i] == BSQL_SCALAR_INDEX && p.lsc[i].type_id == BSQL_STRING) ||
+        (p.right[i] == BSQL_SCALAR_INDEX && p.rsc[i].type_id == BSQL_STRING))
+      return true;
+  return false;
+}
+
+}  // namespace
+
+// bsql_filter_project with every buffer beyond the inputs and outputs in
+// the fixed device scratch: no allocation, no host-memory copy (the plan is
+// written by kernels that carry it as arguments), nothing to clear (the
+// selection writes every ballot and count it later reads).
+int run_filter_project_noalloc(bsql_program* pred, bsql_program* proj, const bsql_column* in, int n_in,
+                               bsql_column* out, int n_out, int64_t num_rows, int64_t* d_count, void* stream) {
+  if (!pred || !proj) fail("null program");
+  if (!d_count) fail("null count pointer");
+  for (bsql_program* p : {pred, proj}) {
+    if (!p->steps.empty()) fail("bsql_filter_project_noalloc: programs that build string columns allocate them");
+    if (!p->parts.empty()) fail("bsql_filter_project_noalloc: a projection split into parts runs in several calls");
+    if (has_string_literals(*p))
+      fail("bsql_filter_project_noalloc: string literals need a device copy the library allocates");
+  }
+  check_inputs(in, n_in, num_rows);
+  if (n_out != num_roots(*proj)) fail("number of outputs does not match the program");
+  if (n_out > BSQL_DEV_MAX_OUT) fail("bsql_filter_project_noalloc writes at most 64 outputs");
+  check_outputs(out, n_out, num_rows);
+  for (int k = 0; k < n_out; ++k)
+    if (out[k].type_id == BSQL_STRING) fail("bsql_filter_project_noalloc writes fixed-width outputs only");
+  if (num_rows > (int64_t)UINT32_MAX) fail("a filter handles at most 2^32-1 rows per call");
+  LowerOptions opt;
+  opt.mode = LowerOptions::FILTER_PROJECT;
+  opt.pred = pred;
+  opt.proj = proj;
+  opt.conj_order = &pred->conj_order;
+  opt.ranges = !(pred->launch.flags & BSQL_LAUNCH_NO_RANGES);
+  opt.lazy_loads = (pred->launch.flags & BSQL_LAUNCH_LAZY_LOADS) != 0;
+  opt.reg_budget = std::min(reg_budget(pred), reg_budget(proj));
+  const Lowered& L = lowered_for(pred, opt, in, n_in, out, n_out);
+  DevPlan plan = L.plan;
+  plan.num_rows = num_rows;
+  for (size_t j = 0; j < L.in_cols.size(); ++j) plan.in[j] = to_dev_in(in[L.in_cols[j]]);
+  int sel_rpt = rows_per_thread(pred), sel_block = 256;
+  int rpt = 0, block = 256;
+  launch_shape(L, rows_per_thread(proj), block_size(proj), &rpt, &block);
+  if (plan.n_ranges == 0) launch_shape(L, sel_rpt, block_size(pred), &sel_rpt, &sel_block);
+  if (sel_block != block) block = sel_block;
+  const int64_t sel_words = select_scratch_words(plan, L.nreg, sel_rpt, block);
+  if (sel_words < 0) fail("no kernel variant for this program");
+  const int64_t id_words = (num_rows + 1) / 2;
+  const int64_t at_plan = sel_words + id_words;
+  const int64_t words = at_plan + plan_words(L.ops.size());
+  const int64_t cap = (int64_t)(noalloc_scratch_bytes() / 8);
+  if (words > cap)
+    fail("bsql_filter_project_noalloc: " + std::to_string(num_rows) + " rows need " + std::to_string(words * 8) +
+         " bytes of scratch, more than its " + std::to_string(cap * 8) + " (split the table into batches)");
+  uint64_t* scratch = nullptr;
+  hipError_t e = noalloc_scratch(&scratch);
+  if (e != hipSuccess) return device_error(e, "bsql_filter_project_noalloc scratch");
+  for (int k = 0; k < n_out; ++k) {
+    plan.out[k].data = out[k].data;
+    plan.out[k].valid = out[k].validity;
+    plan.out[k].type = out[k].type_id;
+  }
+  std::vector<char> blob = plan_blob(plan, L.ops, scratch + at_plan);
+  blob.resize((blob.size() + 7) / 8 * 8, 0);
+  e = write_words_async(scratch + at_plan, (const uint64_t*)blob.data(), blob.size() / 8, (hipStream_t)stream);
+  if (e == hipSuccess)
+    e = launch_compact(plan, (const DevPlan*)(scratch + at_plan), L.nreg, sel_rpt, rpt, block, scratch,
+                       (uint32_t*)(scratch + sel_words), d_count, num_rows, (hipStream_t)stream, PHASE_ALL);
+  if (e != hipSuccess) return device_error(e, "bsql_filter_project_noalloc");
+  return BSQL_OK;
+}
+
 namespace {
 
 // Output groups of a program without parts: consecutive runs of at most
@@ -925,6 +1006,15 @@ int bsql_filter_project_sized(const bsql_program* pred, const bsql_program* proj
     reject_capture(stream, "bsql_filter_project_sized");
     return run_filter_project(const_cast<bsql_program*>(pred), const_cast<bsql_program*>(proj), in, n_in, out,
                               n_out, num_rows, capacity, d_count, stream);
+  })
+}
+
+int bsql_filter_project_noalloc(const bsql_program* pred, const bsql_program* proj, const bsql_column* in,
+                                int n_in, bsql_column* out, int n_out, int64_t num_rows, int64_t* d_count,
+                                void* stream) {
+  BSQL_TRY({
+    return run_filter_project_noalloc(const_cast<bsql_program*>(pred), const_cast<bsql_program*>(proj), in, n_in,
+                                      out, n_out, num_rows, d_count, stream);
   })
 }
 

@@ -450,6 +450,17 @@ def filter_project(pred, proj, in_cols, out_cols, d_count, num_rows=None, stream
                                         ctypes.c_void_p(d_count.data_ptr()), _stream_ptr(stream)))
 
 
+def filter_project_noalloc(pred, proj, in_cols, out_cols, d_count, num_rows=None, stream=None):
+    """filter_project through the allocation-free entry point (fixed device
+    scratch, graph-capturable)."""
+    if num_rows is None:
+        num_rows = in_cols[0].size if in_cols else 0
+    ins, outs = columns_array(in_cols), columns_array(out_cols)
+    L.check(L.lib().bsql_filter_project_noalloc(pred.handle, proj.handle, ins, len(in_cols), outs,
+                                                len(out_cols), num_rows,
+                                                ctypes.c_void_p(d_count.data_ptr()), _stream_ptr(stream)))
+
+
 def filter_project_sized(pred, proj, in_cols, out_cols, d_count, capacity, num_rows=None, stream=None):
     """filter_project into outputs that hold `capacity` rows: the first
     min(count, capacity) selected rows are written, *d_count gets the full

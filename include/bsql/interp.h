@@ -275,6 +275,22 @@ int bsql_filter_project(const bsql_program* pred, const bsql_program* proj,
                         int n_out, int64_t num_rows, int64_t* d_count,
                         void* stream);
 
+/* bsql_filter_project for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. Same arguments and results as
+ * bsql_filter_project; every buffer beyond the inputs and outputs lives in a
+ * fixed 128 MiB device array per GPU (held until the process ends), so a
+ * call allocates nothing, copies nothing from host memory and can be
+ * captured in a graph. Limits: the scratch must hold the selection and 4
+ * bytes of row id per input row (about 32M rows; more fails with
+ * BSQL_ERR_INVALID), at most 64 fixed-width outputs, no programs with string
+ * literals or string-building functions. Calls that overlap on one GPU
+ * share the array and race. */
+int bsql_filter_project_noalloc(const bsql_program* pred, const bsql_program* proj,
+                                const bsql_column* in, int n_in, bsql_column* out,
+                                int n_out, int64_t num_rows, int64_t* d_count,
+                                void* stream);
+
 /* Row count of the filter only (the selection kernel; nothing is gathered):
  * the count process_filter's output would have. */
 int bsql_filter_count(const bsql_program* pred, const bsql_column* in, int n_in, int64_t num_rows,
