@@ -554,6 +554,8 @@ def main():
     ap.add_argument("--rows", type=int, default=100_000_000)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--rows-per-thread", type=int, default=0)
+    ap.add_argument("--fused", action="store_true",
+                    help="run the filter as one fused pass instead of selection, row ids and output pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)
     ap.add_argument("--workload", choices=["scan", "batched", "learned", "strings"], default="scan",
@@ -597,8 +599,8 @@ def main():
     outs = [DeviceColumn.empty(10, n), DeviceColumn.empty(10, n)]
     pred = bsql.Program.compile(PRED, TYPES)
     proj = bsql.Program.compile(PROJ, TYPES)
-    if args.rows_per_thread:
-        pred.set_launch(rows_per_thread=args.rows_per_thread)
+    if args.rows_per_thread or args.fused:
+        pred.set_launch(rows_per_thread=args.rows_per_thread, fused=args.fused)
         proj.set_launch(rows_per_thread=args.rows_per_thread)
     cnt = torch.zeros(1, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream()

@@ -77,7 +77,7 @@ class LaunchConfig(ctypes.Structure):
                 ("flags", ctypes.c_int32)]
 
 
-LAUNCH_NO_RANGES, LAUNCH_LAZY_LOADS, LAUNCH_SPILL_WIDE = 1, 2, 4
+LAUNCH_NO_RANGES, LAUNCH_LAZY_LOADS, LAUNCH_SPILL_WIDE, LAUNCH_FUSED = 1, 2, 4, 8
 ROOT_INTERPRETED, ROOT_COLUMN, ROOT_LITERAL = 0, 1, 2
 SKETCH_DIM, FEATURE_DIM, MLP_HIDDEN = 19, 24, 64
 

@@ -33,7 +33,9 @@ hipError_t launch_string_output(const uint64_t* slices, const uint32_t* valid, i
                                 int64_t count, int64_t capacity, uint64_t* status, hipStream_t stream);
 hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int sel_rpt, int rpt, int block,
                           uint64_t* sel, uint32_t* idx, int64_t* d_count, int64_t capacity,
-                          hipStream_t stream, int phases);
+                          hipStream_t stream, int phases, bool fused);
+bool fused_eligible(const DevPlan& plan, int nreg);
+int64_t fused_scratch_words(int64_t num_rows);
 
 namespace {
 thread_local std::string g_last_error;
@@ -435,9 +437,13 @@ int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, i
   launch_shape(L, rows_per_thread(proj ? proj : pred), block_size(proj ? proj : pred), &rpt, &block);
   if (plan.n_ranges == 0) launch_shape(L, sel_rpt, block_size(pred), &sel_rpt, &sel_block);
   if (sel_block != block) block = sel_block;  // one block size per plan (the spill layout is per block)
-  const int64_t sel_words = select_scratch_words(plan, L.nreg, sel_rpt, block);
+  const int64_t sel_words = std::max(select_scratch_words(plan, L.nreg, sel_rpt, block),
+                                     (int64_t)fused_scratch_words(num_rows));
   if (sel_words < 0) fail("no kernel variant for this program");
-  const int64_t id_words = (num_rows + 1) / 2;
+  // the one-pass kernel serves a whole call over fixed-width outputs
+  const bool fused = phases == PHASE_ALL && n_str == 0 && (pred->launch.flags & BSQL_LAUNCH_FUSED) &&
+                     fused_eligible(plan, L.nreg);
+  const int64_t id_words = fused ? 0 : (num_rows + 1) / 2;
   const int64_t st_words = n_str ? string_tiles(num_rows) + 1 : 0;
   const int64_t pw = plan_words(L.ops.size());
   const int64_t at_plan = sel_words + id_words + st_words + (int64_t)n_str * num_rows;
@@ -457,7 +463,7 @@ int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, i
   e = upload_plan(plan, L.ops, scratch, at_plan, (hipStream_t)stream);
   if (e == hipSuccess)
     e = launch_compact(plan, (const DevPlan*)(scratch + at_plan), L.nreg, sel_rpt, rpt, block, sel, ids, d_count,
-                       capacity, (hipStream_t)stream, phases);
+                       capacity, (hipStream_t)stream, phases, fused);
   if (e == hipSuccess && outputs)
     e = finish_string_outputs(L, in, out, n_out, slices, d_count, 0, capacity, status, (hipStream_t)stream);
   if (e != hipSuccess) return device_error(e, "filter");
@@ -523,9 +529,11 @@ int run_filter_project_noalloc(bsql_program* pred, bsql_program* proj, const bsq
   launch_shape(L, rows_per_thread(proj), block_size(proj), &rpt, &block);
   if (plan.n_ranges == 0) launch_shape(L, sel_rpt, block_size(pred), &sel_rpt, &sel_block);
   if (sel_block != block) block = sel_block;
-  const int64_t sel_words = select_scratch_words(plan, L.nreg, sel_rpt, block);
+  const int64_t sel_words = std::max(select_scratch_words(plan, L.nreg, sel_rpt, block),
+                                     (int64_t)fused_scratch_words(num_rows));
   if (sel_words < 0) fail("no kernel variant for this program");
-  const int64_t id_words = (num_rows + 1) / 2;
+  const bool fused = (pred->launch.flags & BSQL_LAUNCH_FUSED) && fused_eligible(plan, L.nreg);
+  const int64_t id_words = fused ? 0 : (num_rows + 1) / 2;
   const int64_t at_plan = sel_words + id_words;
   const int64_t words = at_plan + plan_words(L.ops.size());
   const int64_t cap = (int64_t)(noalloc_scratch_bytes() / 8);
@@ -545,7 +553,7 @@ int run_filter_project_noalloc(bsql_program* pred, bsql_program* proj, const bsq
   e = write_words_async(scratch + at_plan, (const uint64_t*)blob.data(), blob.size() / 8, (hipStream_t)stream);
   if (e == hipSuccess)
     e = launch_compact(plan, (const DevPlan*)(scratch + at_plan), L.nreg, sel_rpt, rpt, block, scratch,
-                       (uint32_t*)(scratch + sel_words), d_count, num_rows, (hipStream_t)stream, PHASE_ALL);
+                       (uint32_t*)(scratch + sel_words), d_count, num_rows, (hipStream_t)stream, PHASE_ALL, fused);
   if (e != hipSuccess) return device_error(e, "bsql_filter_project_noalloc");
   return BSQL_OK;
 }
@@ -962,7 +970,7 @@ int bsql_program_set_launch(bsql_program* p, const bsql_launch_config* cfg) {
     return set_error(BSQL_ERR_INVALID, "rows_per_thread must be 0, 1, 2, 4, 8, 16 or 32");
   if (cfg->block_size != 0 && cfg->block_size != 64 && cfg->block_size != 256)
     return set_error(BSQL_ERR_INVALID, "block_size must be 0, 64 or 256");
-  if (cfg->flags & ~(BSQL_LAUNCH_NO_RANGES | BSQL_LAUNCH_LAZY_LOADS | BSQL_LAUNCH_SPILL_WIDE))
+  if (cfg->flags & ~(BSQL_LAUNCH_NO_RANGES | BSQL_LAUNCH_LAZY_LOADS | BSQL_LAUNCH_SPILL_WIDE | BSQL_LAUNCH_FUSED))
     return set_error(BSQL_ERR_INVALID, "unknown launch flags");
   std::lock_guard<std::mutex> lk(p->cache_mu);
   p->launch = *cfg;

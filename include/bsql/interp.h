@@ -283,9 +283,10 @@ int bsql_filter_project(const bsql_program* pred, const bsql_program* proj,
  * call allocates nothing, copies nothing from host memory and can be
  * captured in a graph. Limits: the scratch must hold the selection and 4
  * bytes of row id per input row (about 32M rows; more fails with
- * BSQL_ERR_INVALID), at most 64 fixed-width outputs, no programs with string
- * literals or string-building functions. Calls that overlap on one GPU
- * share the array and race. */
+ * BSQL_ERR_INVALID; a predicate launched with BSQL_LAUNCH_FUSED needs no row
+ * ids), at most 64 fixed-width outputs, no programs with string literals or
+ * string-building functions. Calls that overlap on one GPU share the array
+ * and race. */
 int bsql_filter_project_noalloc(const bsql_program* pred, const bsql_program* proj,
                                 const bsql_column* in, int n_in, bsql_column* out,
                                 int n_out, int64_t num_rows, int64_t* d_count,
@@ -320,10 +321,15 @@ enum {
   BSQL_LAUNCH_LAZY_LOADS = 2, /* predicates: load a conjunct's columns only when
                                  the wave reaches it (default: as early as
                                  registers allow) */
-  BSQL_LAUNCH_SPILL_WIDE = 4  /* spill layout: keep at most 7 live values in
+  BSQL_LAUNCH_SPILL_WIDE = 4, /* spill layout: keep at most 7 live values in
                                  registers (the 8-register file, up to 4 rows
                                  per lane) and the rest in LDS slots; default
                                  15 in registers (16-register file) */
+  BSQL_LAUNCH_FUSED = 8       /* predicates: a filter whose conjuncts are range
+                                 tests, with at most 8 fixed-width outputs,
+                                 runs as one pass (selection, prefix and
+                                 outputs in one kernel, no row ids); default
+                                 selection, row ids and output pass */
 };
 typedef struct bsql_launch_config {
   int32_t block_size;     /* threads per workgroup: 0 (auto), 64 or 256 */

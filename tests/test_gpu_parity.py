@@ -457,3 +457,63 @@ def test_selective_first_range_vs_oracle(bsql, gpu, seed):
             for k in range(len(types)):
                 gv, gval = outs[k].to_numpy(m)
                 assert_same(gv, gval, ores[k][0], ores[k][1], what=f"{text} ranges={ranges} col {k}")
+
+
+FUSED_SIZES = [0, 1, 31, 32, 33, 255, 257, 2047, 2048, 2049, 8191, 8192, 8193, 3 * 8192 + 1000, 250_003,
+               1_000_000]
+
+
+@pytest.mark.parametrize("n", FUSED_SIZES)
+def test_fused_filter_project_vs_two_pass_and_oracle(bsql, gpu, n):
+    """The one-pass kernel (range predicate, outputs at look-back positions,
+    validity merged at wave boundaries) against the selection / row ids /
+    output-pass path and the oracle, at sizes around validity-word, chunk,
+    slice and tile boundaries, with nulls in predicate and output columns;
+    plus the capacity-limited form."""
+    import oracle
+    import torch
+    from blazingsql_amd.interop import DeviceColumn, filter_project, filter_project_sized
+    rng = np.random.default_rng(n + 3)
+    types = [INT32, FLOAT64, INT64, INT32]
+    table = make_table(rng, types, max(n, 1), null_rates=[0.1, 0.0, 0.3, 0.05])
+    table = [(t, v[:n], None if val is None else val[:n]) for t, v, val in table]
+    dev, host = device_columns(table), host_columns(table)
+    pred_text = "AND(>($0, -500), <($3, 800), >=($1, -900.5:DECIMAL(4, 1)))"
+    exprs = ["+($2, *($0, 3))", "*($1, 2.5:DECIMAL(2, 1))", "$3", "CASE(IS NOT NULL($2), $2, 0)", "RAND()"]
+    om = None
+    results = []
+    for fused in (True, False):
+        pred = bsql.Program.compile(pred_text, types)
+        pred.set_launch(fused=fused)
+        proj = bsql.Program.compile(exprs, types)
+        ot = [proj.output_type(types, k) for k in range(proj.num_outputs)]
+        outs = [DeviceColumn.empty(t, max(n, 1)) for t in ot]
+        cnt = torch.full((1,), -7, dtype=torch.int64, device="cuda")
+        print(f"filter_project n={n} fused={fused}", flush=True)
+        filter_project(pred, proj, dev, outs, cnt, n)
+        torch.cuda.synchronize()
+        m = int(cnt.item())
+        if om is None:
+            om, ores = oracle.filter_project(oracle.full_table_program(pred, 4), oracle.full_table_program(proj, 4),
+                                             host, ot, n)
+        assert m == om, (fused, m, om)
+        got = [o.to_numpy(m) for o in outs]
+        for k in range(5):
+            assert_same(got[k][0], got[k][1], ores[k][0], ores[k][1], what=f"fused={fused} output {k}")
+        results.append(got)
+    # RAND numbers the output rows: both paths agree (and with the oracle, above)
+    assert_same(results[0][4][0], results[0][4][1], results[1][4][0], results[1][4][1], what="RAND")
+    if n >= 1000:
+        # pre-sized outputs: the first `cap` rows written, the full count reported
+        cap = om // 2
+        pred = bsql.Program.compile(pred_text, types)
+        pred.set_launch(fused=True)
+        proj = bsql.Program.compile(exprs[:4], types)
+        outs = [DeviceColumn.empty(t, max(cap, 1)) for t in ot[:4]]
+        cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+        filter_project_sized(pred, proj, dev, outs, cnt, cap, n)
+        torch.cuda.synchronize()
+        assert int(cnt.item()) == om
+        for k in range(4):
+            gv, gval = outs[k].to_numpy(cap)
+            assert_same(gv, gval, ores[k][0][:cap], ores[k][1][:cap], what=f"sized output {k}")

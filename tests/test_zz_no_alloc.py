@@ -67,12 +67,19 @@ def test_noalloc_limits(bsql):
     from blazingsql_amd.interop import DeviceColumn, filter_project_noalloc
     n = 40_000_000
     col = DeviceColumn.from_numpy(INT32, np.zeros(n, dtype=np.int32))
+    # selection, row ids and output pass: 4 bytes of row id per row exceed
+    # the fixed scratch
     pred = bsql.Program.compile(">($0, 5)", [INT32])
     proj = bsql.Program.compile(["+($0, 1)"], [INT32])
     out = DeviceColumn.empty(INT32, n)
     cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
     with pytest.raises(BsqlError, match="scratch"):
         filter_project_noalloc(pred, proj, [col], [out], cnt, n)
+    # the fused pass of a range predicate keeps one word per 2048 rows
+    pred.set_launch(fused=True)
+    filter_project_noalloc(pred, proj, [col], [out], cnt, n)
+    torch.cuda.synchronize()
+    assert int(cnt.item()) == 0
     strs = DeviceColumn.from_strings(["ab", "cd"])
     pred = bsql.Program.compile("LIKE($0, 'a%')", [23])
     proj = bsql.Program.compile(["CHAR_LENGTH($0)"], [23])
