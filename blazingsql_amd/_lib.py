@@ -77,7 +77,7 @@ class LaunchConfig(ctypes.Structure):
                 ("flags", ctypes.c_int32)]
 
 
-LAUNCH_NO_RANGES, LAUNCH_LAZY_LOADS, LAUNCH_SPILL_WIDE, LAUNCH_FUSED = 1, 2, 4, 8
+LAUNCH_NO_RANGES, LAUNCH_LAZY_LOADS, LAUNCH_SPILL_WIDE, LAUNCH_FUSED, LAUNCH_GENERATED = 1, 2, 4, 8, 16
 ROOT_INTERPRETED, ROOT_COLUMN, ROOT_LITERAL = 0, 1, 2
 SKETCH_DIM, FEATURE_DIM, MLP_HIDDEN = 19, 24, 64
 
@@ -122,6 +122,8 @@ SIGNATURES = {
     "bsql_program_num_conjuncts": (_I, [_P]),
     "bsql_program_set_conjunct_order": (_I, [_P, _P, _I]),
     "bsql_program_set_launch": (_I, [_P, ctypes.POINTER(LaunchConfig)]),
+    "bsql_generated_kernel_stats": (_I, [ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_double)]),
+    "bsql_generated_kernel_check": (_I, [_P, _P, _P, _I, _P, _I, _I64, ctypes.c_char_p, ctypes.POINTER(_I64)]),
     "bsql_batch_prepare": (_I, [_P, _I, ctypes.POINTER(_P)]),
     "bsql_batch_launch": (_I, [_P, _P]),
     "bsql_batch_set_launch": (_I, [_P, ctypes.POINTER(LaunchConfig)]),

@@ -36,6 +36,19 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
                           hipStream_t stream, int phases, bool fused);
 bool fused_eligible(const DevPlan& plan, int nreg);
 int64_t fused_scratch_words(int64_t num_rows);
+// kernel_gen.cpp
+bool generated_filter_project_eligible(const DevPlan& plan, const std::vector<DevOp>& ops);
+hipError_t launch_generated_filter_project(const DevPlan& plan, const std::vector<DevOp>& ops,
+                                           const DevPlan* d_plan, uint64_t* status, int64_t* d_count,
+                                           int64_t capacity, hipStream_t stream);
+int64_t generated_filter_project_scratch_words(int64_t num_rows);
+bool generated_execute_eligible(const DevPlan& plan, const std::vector<DevOp>& ops);
+hipError_t launch_generated_execute(const DevPlan& plan, const std::vector<DevOp>& ops, const DevPlan* d_plan,
+                                    int nreg, int rpt, int block, hipStream_t stream);
+void generated_stats(int64_t* kernels, int64_t* launches, double* compile_seconds);
+void set_generated_check(const char* arch);
+bool generated_check_mode();
+int64_t generated_check_bytes();
 
 namespace {
 thread_local std::string g_last_error;
@@ -384,6 +397,7 @@ int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, i
   // step columns: the predicate's after the table, the projection's after those
   StepColumns sc;
   if (!pred->steps.empty() || (proj && !proj->steps.empty())) {
+    if (generated_check_mode()) fail("a program with step columns has no generated kernel to check");
     sc.stream = (hipStream_t)stream;
     sc.cols.assign(in, in + n_in);
     if (!pred->steps.empty()) {
@@ -412,9 +426,12 @@ int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, i
   if (num_rows > (int64_t)UINT32_MAX) fail("a filter handles at most 2^32-1 rows per call");
   const bool outputs = (phases & PHASE_OUTPUT) != 0;
   if (outputs) check_outputs(out, n_out, capacity);
-  hipError_t e = ensure_strings(*pred);
-  if (e == hipSuccess && proj) e = ensure_strings(*proj);
-  if (e != hipSuccess) return device_error(e, "filter");
+  hipError_t e = hipSuccess;
+  if (!generated_check_mode()) {
+    e = ensure_strings(*pred);
+    if (e == hipSuccess && proj) e = ensure_strings(*proj);
+    if (e != hipSuccess) return device_error(e, "filter");
+  }
   LowerOptions opt;
   opt.mode = proj ? LowerOptions::FILTER_PROJECT
                   : (n_out == 0 && !out ? LowerOptions::FILTER_COUNT : LowerOptions::FILTER_GATHER);
@@ -437,13 +454,26 @@ int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, i
   launch_shape(L, rows_per_thread(proj ? proj : pred), block_size(proj ? proj : pred), &rpt, &block);
   if (plan.n_ranges == 0) launch_shape(L, sel_rpt, block_size(pred), &sel_rpt, &sel_block);
   if (sel_block != block) block = sel_block;  // one block size per plan (the spill layout is per block)
-  const int64_t sel_words = std::max(select_scratch_words(plan, L.nreg, sel_rpt, block),
-                                     (int64_t)fused_scratch_words(num_rows));
+  // the one-pass kernels serve a whole call over fixed-width outputs
+  const bool one_pass = phases == PHASE_ALL && n_str == 0;
+  const bool generated = one_pass && ((pred->launch.flags & BSQL_LAUNCH_GENERATED) || generated_check_mode()) &&
+                         generated_filter_project_eligible(plan, L.ops);
+  if (generated_check_mode()) {
+    if (!generated) fail("this filter does not run as a generated kernel");
+    for (int k = 0; k < n_out; ++k) {
+      plan.out[k].valid = out[k].validity;
+      plan.out[k].type = out[k].type_id;
+    }
+    e = launch_generated_filter_project(plan, L.ops, nullptr, nullptr, d_count, capacity, nullptr);
+    return e == hipSuccess ? BSQL_OK : device_error(e, "generated kernel check");
+  }
+  const int64_t sel_words = std::max({select_scratch_words(plan, L.nreg, sel_rpt, block),
+                                      (int64_t)fused_scratch_words(num_rows),
+                                      generated_filter_project_scratch_words(num_rows)});
   if (sel_words < 0) fail("no kernel variant for this program");
-  // the one-pass kernel serves a whole call over fixed-width outputs
-  const bool fused = phases == PHASE_ALL && n_str == 0 && (pred->launch.flags & BSQL_LAUNCH_FUSED) &&
+  const bool fused = !generated && one_pass && (pred->launch.flags & BSQL_LAUNCH_FUSED) &&
                      fused_eligible(plan, L.nreg);
-  const int64_t id_words = fused ? 0 : (num_rows + 1) / 2;
+  const int64_t id_words = fused || generated ? 0 : (num_rows + 1) / 2;
   const int64_t st_words = n_str ? string_tiles(num_rows) + 1 : 0;
   const int64_t pw = plan_words(L.ops.size());
   const int64_t at_plan = sel_words + id_words + st_words + (int64_t)n_str * num_rows;
@@ -461,7 +491,10 @@ int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, i
     plan.out[k].type = out[k].type_id;
   }
   e = upload_plan(plan, L.ops, scratch, at_plan, (hipStream_t)stream);
-  if (e == hipSuccess)
+  if (e == hipSuccess && generated)
+    e = launch_generated_filter_project(plan, L.ops, (const DevPlan*)(scratch + at_plan), sel, d_count, capacity,
+                                        (hipStream_t)stream);
+  else if (e == hipSuccess)
     e = launch_compact(plan, (const DevPlan*)(scratch + at_plan), L.nreg, sel_rpt, rpt, block, sel, ids, d_count,
                        capacity, (hipStream_t)stream, phases, fused);
   if (e == hipSuccess && outputs)
@@ -626,7 +659,7 @@ int execute_program(bsql_program* prog, const bsql_column* in, int n_in, bsql_co
   check_inputs(in, n_in, num_rows);
   if (n_out != num_roots(*prog)) fail("number of outputs does not match the program");
   check_outputs(out, n_out, num_rows);
-  hipError_t e = ensure_strings(*prog);
+  hipError_t e = generated_check_mode() ? hipSuccess : ensure_strings(*prog);
   if (e != hipSuccess) return device_error(e, "bsql_interp_execute");
   for (const std::vector<int>& group : output_groups(n_out)) {
     std::vector<bsql_column> gout;
@@ -649,6 +682,17 @@ int execute_program(bsql_program* prog, const bsql_column* in, int n_in, bsql_co
     const int64_t st_words = n_str ? string_tiles(num_rows) + 1 : 0;
     const int64_t at_plan = st_words + (int64_t)n_str * num_rows;
     const int64_t words = at_plan + plan_words(L.ops.size());
+    if (generated_check_mode()) {
+      if (!generated_execute_eligible(plan, L.ops)) fail("this program does not run as a generated kernel");
+      for (int k = 0; k < g_out; ++k) {
+        plan.out[k].valid = gout[k].validity;
+        plan.out[k].type = gout[k].type_id;
+      }
+      e = launch_generated_execute(plan, L.ops, nullptr, L.nreg, variant_rows(L.nreg, rpt, block, L.n_spill), block,
+                                   nullptr);
+      if (e != hipSuccess) return device_error(e, "generated kernel check");
+      continue;
+    }
     uint64_t* scratch = nullptr;
     if (private_scratch) {
       e = hipMallocAsync((void**)&scratch, (size_t)words * 8, (hipStream_t)stream);
@@ -680,8 +724,13 @@ int execute_program(bsql_program* prog, const bsql_column* in, int n_in, bsql_co
     } else {
       e = upload_plan(plan, L.ops, scratch, at_plan, (hipStream_t)stream);
     }
-    if (e == hipSuccess && num_rows > 0)
-      e = launch_execute(plan, (const DevPlan*)(scratch + at_plan), L.nreg, rpt, block, (hipStream_t)stream);
+    if (e == hipSuccess && num_rows > 0) {
+      if ((prog->launch.flags & BSQL_LAUNCH_GENERATED) && generated_execute_eligible(plan, L.ops))
+        e = launch_generated_execute(plan, L.ops, (const DevPlan*)(scratch + at_plan), L.nreg,
+                                     variant_rows(L.nreg, rpt, block, L.n_spill), block, (hipStream_t)stream);
+      else
+        e = launch_execute(plan, (const DevPlan*)(scratch + at_plan), L.nreg, rpt, block, (hipStream_t)stream);
+    }
     if (e == hipSuccess)
       e = finish_string_outputs(L, in, gout.data(), g_out, slices, nullptr, num_rows, num_rows, status,
                                 (hipStream_t)stream);
@@ -970,11 +1019,38 @@ int bsql_program_set_launch(bsql_program* p, const bsql_launch_config* cfg) {
     return set_error(BSQL_ERR_INVALID, "rows_per_thread must be 0, 1, 2, 4, 8, 16 or 32");
   if (cfg->block_size != 0 && cfg->block_size != 64 && cfg->block_size != 256)
     return set_error(BSQL_ERR_INVALID, "block_size must be 0, 64 or 256");
-  if (cfg->flags & ~(BSQL_LAUNCH_NO_RANGES | BSQL_LAUNCH_LAZY_LOADS | BSQL_LAUNCH_SPILL_WIDE | BSQL_LAUNCH_FUSED))
+  if (cfg->flags & ~(BSQL_LAUNCH_NO_RANGES | BSQL_LAUNCH_LAZY_LOADS | BSQL_LAUNCH_SPILL_WIDE | BSQL_LAUNCH_FUSED |
+                     BSQL_LAUNCH_GENERATED))
     return set_error(BSQL_ERR_INVALID, "unknown launch flags");
   std::lock_guard<std::mutex> lk(p->cache_mu);
   p->launch = *cfg;
   return BSQL_OK;
+}
+
+int bsql_generated_kernel_stats(int64_t* kernels, int64_t* launches, double* compile_seconds) {
+  generated_stats(kernels, launches, compile_seconds);
+  return BSQL_OK;
+}
+
+int bsql_generated_kernel_check(const bsql_program* pred, const bsql_program* proj, const bsql_column* in,
+                                int n_in, const bsql_column* out, int n_out, int64_t num_rows, const char* arch,
+                                int64_t* code_bytes) {
+  if (!proj || !arch || !*arch || num_rows <= 0) return set_error(BSQL_ERR_INVALID, "invalid arguments");
+  BSQL_TRY({
+    struct Reset {
+      ~Reset() { set_generated_check(nullptr); }
+    } reset;
+    set_generated_check(arch);
+    int64_t count = 0;
+    const int rc = pred ? run_filter_project(const_cast<bsql_program*>(pred), const_cast<bsql_program*>(proj), in,
+                                             n_in, const_cast<bsql_column*>(out), n_out, num_rows, num_rows,
+                                             &count, nullptr)
+                        : run_execute(const_cast<bsql_program*>(proj), in, n_in, const_cast<bsql_column*>(out),
+                                      n_out, num_rows, nullptr);
+    if (rc != BSQL_OK) return rc;
+    if (code_bytes) *code_bytes = generated_check_bytes();
+    return BSQL_OK;
+  })
 }
 
 int bsql_interp_execute(const bsql_program* cprog, const bsql_column* in, int n_in,

@@ -143,6 +143,16 @@ typedef struct DevPlan {
   DevRange ranges[BSQL_DEV_MAX_RANGES];
 } DevPlan;
 
+// Arguments of the one-pass filter + project kernels (the tile status words
+// and tickets live in the call's scratch).
+typedef struct FusedArgs {
+  uint64_t* status;   // num_tiles words, zeroed before the launch
+  uint32_t* ticket;   // zeroed before the launch (kernels that take tiles from tickets)
+  int64_t* d_count;   // selected rows (written by the last tile)
+  int64_t capacity;   // rows the outputs hold; positions beyond are not written
+  int64_t num_tiles;
+} FusedArgs;
+
 // LDS bytes of the spill slots of one workgroup: per wave and slot, the
 // 64 lanes x R rows of 8-byte data, then one validity byte per lane.
 static inline int64_t spill_lds_bytes(int n_spill, int block, int rows) {

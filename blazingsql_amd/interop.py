@@ -241,14 +241,17 @@ class Program:
             L.check(L.lib().bsql_program_set_conjunct_order(self._h, arr, len(order)))
 
     def set_launch(self, rows_per_thread=0, block_size=0, ranges=True, lazy_loads=False, spill_wide=False,
-                   fused=False):
+                   fused=False, generated=False):
         """Launch shape for this program (block_size 0, 64 or 256); for a
         predicate also whether its column-vs-literal conjuncts run as range
         tests and whether conjunct columns are loaded lazily; spill_wide keeps
         at most 7 live values in registers and the rest in LDS slots; fused runs
-        a range-test filter with at most 8 fixed-width outputs as one pass."""
+        a range-test filter with at most 8 fixed-width outputs as one pass;
+        generated runs the program through a kernel generated for it (op
+        fields and column types compiled in, hipRTC, cached per process)."""
         flags = ((0 if ranges else L.LAUNCH_NO_RANGES) | (L.LAUNCH_LAZY_LOADS if lazy_loads else 0) |
-                 (L.LAUNCH_SPILL_WIDE if spill_wide else 0) | (L.LAUNCH_FUSED if fused else 0))
+                 (L.LAUNCH_SPILL_WIDE if spill_wide else 0) | (L.LAUNCH_FUSED if fused else 0) |
+                 (L.LAUNCH_GENERATED if generated else 0))
         cfg = L.LaunchConfig(block_size, rows_per_thread, flags)
         L.check(L.lib().bsql_program_set_launch(self._h, ctypes.byref(cfg)))
 
@@ -406,6 +409,31 @@ class DeviceColumn:
             words = self.validity.cpu().numpy().view(np.uint32)
             valid = unpack_validity(words, self.offset + n)[self.offset:]
         return vals, valid
+
+
+def generated_kernel_stats():
+    """(kernels generated so far, launches that ran one, compile seconds)."""
+    k, n, s = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+    L.check(L.lib().bsql_generated_kernel_stats(ctypes.byref(k), ctypes.byref(n), ctypes.byref(s)))
+    return k.value, n.value, s.value
+
+
+def check_generated_kernel(pred, proj, in_cols, out_cols, num_rows, arch="gfx950"):
+    """Generates and compiles, without a device, the kernel filter_project
+    (pred given) or perform_interpreter_operation (pred None) would run with
+    generated=True over columns like these (L.Column descriptors or device
+    columns; only types, validity presence and alignment are read); returns
+    the code object bytes."""
+    def arr(cols):
+        a = (L.Column * max(len(cols), 1))()
+        for k, c in enumerate(cols):
+            a[k] = c if isinstance(c, L.Column) else c.to_c()
+        return a
+    nbytes = ctypes.c_int64()
+    L.check(L.lib().bsql_generated_kernel_check(pred.handle if pred is not None else None, proj.handle,
+                                                arr(in_cols), len(in_cols), arr(out_cols), len(out_cols),
+                                                num_rows, arch.encode(), ctypes.byref(nbytes)))
+    return nbytes.value
 
 
 def columns_array(cols):

@@ -325,11 +325,22 @@ enum {
                                  registers (the 8-register file, up to 4 rows
                                  per lane) and the rest in LDS slots; default
                                  15 in registers (16-register file) */
-  BSQL_LAUNCH_FUSED = 8       /* predicates: a filter whose conjuncts are range
+  BSQL_LAUNCH_FUSED = 8,      /* predicates: a filter whose conjuncts are range
                                  tests, with at most 8 fixed-width outputs,
                                  runs as one pass (selection, prefix and
                                  outputs in one kernel, no row ids); default
                                  selection, row ids and output pass */
+  BSQL_LAUNCH_GENERATED = 16  /* run through a kernel generated for the
+                                 program: its op fields, column types and
+                                 validity presence compiled in as constants
+                                 (hipRTC, at the first launch of each program
+                                 shape on each device; kept for the process),
+                                 literals, columns and range bounds passed at
+                                 launch. bsql_interp_execute: any program;
+                                 filters with a projection (set on the
+                                 predicate): a range-test predicate with
+                                 fixed-width outputs and no RAND runs as one
+                                 pass. Other calls run on the interpreter. */
 };
 typedef struct bsql_launch_config {
   int32_t block_size;     /* threads per workgroup: 0 (auto), 64 or 256 */
@@ -341,6 +352,21 @@ typedef struct bsql_launch_config {
  * reference, which sizes its launch with the occupancy API
  * (interpreter_cpp.cu:76-270, :473). */
 int bsql_program_set_launch(bsql_program* prog, const bsql_launch_config* cfg);
+
+/* Kernels generated so far in this process (BSQL_LAUNCH_GENERATED), launches
+ * that ran one, and the seconds spent compiling them. No counterpart in the
+ * reference, which interprets every program. */
+int bsql_generated_kernel_stats(int64_t* kernels, int64_t* launches, double* compile_seconds);
+/* Generates and compiles, for architecture `arch` (e.g. "gfx950"), the
+ * kernels that bsql_filter_project(pred, proj, ...) (pred non-NULL) or
+ * bsql_interp_execute(proj, ...) (pred NULL) would run with
+ * BSQL_LAUNCH_GENERATED over these columns (only their types, validity
+ * presence and alignment are read), without a device: loads and launches
+ * nothing. BSQL_ERR_INVALID if the call would run on the interpreter;
+ * *code_bytes = size of the code objects. */
+int bsql_generated_kernel_check(const bsql_program* pred, const bsql_program* proj, const bsql_column* in,
+                                int n_in, const bsql_column* out, int n_out, int64_t num_rows, const char* arch,
+                                int64_t* code_bytes);
 
 /* Shape of the device plan a program lowers to for bsql_interp_execute over
  * a table of these column types (its lowering runs on the host, so this
