@@ -5,7 +5,9 @@ Workload (one step = one pass of the hot path over the resident table):
   tax:float64 k/100 k~U{0..8}, shipdays:int32 U{8036..10561}, no nulls;
   WHERE shipdays <= 10471 AND disc BETWEEN 0.05 AND 0.07 AND qty < 24
   SELECT price*(1-disc)*(1+tax), price*disc
-as Calcite plan text, compiled by the library and run by bsql_filter_project.
+as Calcite plan text, compiled by the library and run by bsql_filter_project
+through the one-pass kernel generated for the program (--interpreter: the
+interpreter's selection, row ids and output pass).
 
 Prints one JSON line. Timing: torch.cuda events on the stream the kernels are
 launched on, around exactly `steps` calls after `warmup` untimed ones.
@@ -556,9 +558,9 @@ def main():
     ap.add_argument("--rows-per-thread", type=int, default=0)
     ap.add_argument("--fused", action="store_true",
                     help="run the filter as one fused pass instead of selection, row ids and output pass")
-    ap.add_argument("--generated", action="store_true",
-                    help="run the filter through the kernel generated for this program (one pass; "
-                         "compiled with hipRTC before the warm-up)")
+    ap.add_argument("--interpreter", action="store_true",
+                    help="run the scan through the interpreter's selection, row ids and output pass "
+                         "instead of the kernel generated for the program")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)
     ap.add_argument("--workload", choices=["scan", "batched", "learned", "strings"], default="scan",
@@ -602,20 +604,23 @@ def main():
     outs = [DeviceColumn.empty(10, n), DeviceColumn.empty(10, n)]
     pred = bsql.Program.compile(PRED, TYPES)
     proj = bsql.Program.compile(PROJ, TYPES)
-    if args.rows_per_thread or args.fused or args.generated:
-        pred.set_launch(rows_per_thread=args.rows_per_thread, fused=args.fused, generated=args.generated)
-        proj.set_launch(rows_per_thread=args.rows_per_thread)
+    generated = not (args.interpreter or args.fused)
+    pred.set_launch(rows_per_thread=args.rows_per_thread, fused=args.fused, generated=generated)
+    proj.set_launch(rows_per_thread=args.rows_per_thread)
     cnt = torch.zeros(1, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream()
     kernel = "interpreter: selection, row ids, output pass" if not args.fused else "interpreter: fused pass"
-    if args.generated:
+    traffic_file = "filter_project_100M_traffic.json"
+    if generated:
         from blazingsql_amd.interop import generated_kernel_stats
         filter_project(pred, proj, ins, outs, cnt, n, stream=stream.cuda_stream)  # compiles the kernel
         torch.cuda.synchronize()
         k, launches, secs = generated_kernel_stats()
         if launches == 0:
             raise RuntimeError("the generated kernel did not run")
-        kernel = f"generated one-pass kernel (hipRTC compile {secs:.2f} s, before the warm-up)"
+        kernel = (f"one-pass filter_project_rows generated for the program (hipRTC compile {secs:.2f} s, "
+                  f"once per program shape, before the warm-up)")
+        traffic_file = "filter_project_100M_generated_traffic.json"
 
     for _ in range(args.warmup):
         filter_project(pred, proj, ins, outs, cnt, n, stream=stream.cuda_stream)
@@ -629,6 +634,22 @@ def main():
     ms = start.elapsed_time(end) / args.steps
     count = int(cnt.item())
     ok, detail = check_result(cols, outs, count)
+    # the same call through the interpreter (selection, row ids, output pass), for comparison
+    interp_ms = None
+    if generated:
+        ipred = bsql.Program.compile(PRED, TYPES)
+        for _ in range(3):
+            filter_project(ipred, proj, ins, outs, cnt, n, stream=stream.cuda_stream)
+        torch.cuda.synchronize()
+        reps = max(3, args.steps // 2)
+        start.record(stream)
+        for _ in range(reps):
+            filter_project(ipred, proj, ins, outs, cnt, n, stream=stream.cuda_stream)
+        end.record(stream)
+        torch.cuda.synchronize()
+        interp_ms = start.elapsed_time(end) / reps
+        iok, idetail = check_result(cols, outs, int(cnt.item()))
+        ok = ok and iok
     if args.calibrate_copy:
         src = torch.empty(200_000_000, dtype=torch.float64, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
@@ -643,7 +664,7 @@ def main():
     # row-id lists and other scratch are not counted.
     algo_bytes = n * 32 + count * 2 * (8 + 1 / 8)
     achieved = algo_bytes / (ms * 1e-3) / 1e9
-    traffic = load_traffic(os.path.join(ROOT, "profiles", "filter_project_100M_traffic.json"))
+    traffic = load_traffic(os.path.join(ROOT, "profiles", traffic_file))
     result = {
         "metric": "filter+project rows/s & HBM GB/s (% peak); learned-vs-static speedup, exact",
         "value": n / (ms * 1e-3),
@@ -659,7 +680,7 @@ def main():
         "data": "synthetic (seeded torch RNG on device)",
         "config": {"workload": "100M-row scan: 5 columns, 4-conjunct filter, 2-output project",
                    "rows": n, "selected": count, "selectivity": count / n if n else 0.0,
-                   "kernel": kernel, "exact": ok, "check": detail},
+                   "kernel": kernel, "exact": ok, "check": detail, "interpreter_ms_per_step": interp_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None},

@@ -153,6 +153,18 @@ typedef struct FusedArgs {
   int64_t num_tiles;
 } FusedArgs;
 
+// The one-pass filter + project kernel (filter_project_rows) stages up to
+// `cap` selected rows of its tile in LDS: one 8-byte slot per row and
+// output, and per output fp_null_words(cap) words of null bits (one word
+// more than the rows need, so a 32-bit window at any row offset reads two
+// words). Its dynamic LDS: n_out x cap slots, n_out x fp_null_words(cap)
+// null words, then `waves` x n_out x vw validity words (tiles that selected
+// more rows than `cap` write them from the registers).
+static constexpr inline int fp_null_words(int cap) { return cap > 0 ? cap / 32 + 1 : 0; }
+static constexpr inline int64_t fp_lds_bytes(int n_out, int cap, int waves, int vw) {
+  return (int64_t)n_out * cap * 8 + (int64_t)n_out * fp_null_words(cap) * 4 + (int64_t)waves * n_out * vw * 4;
+}
+
 // LDS bytes of the spill slots of one workgroup: per wave and slot, the
 // 64 lanes x R rows of 8-byte data, then one validity byte per lane.
 static inline int64_t spill_lds_bytes(int n_spill, int block, int rows) {

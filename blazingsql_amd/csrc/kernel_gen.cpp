@@ -14,6 +14,7 @@
 #include <hip/hip_runtime_api.h>
 #include <hip/hiprtc.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdlib>
 #include <functional>
@@ -60,13 +61,18 @@ const char* kPrelude =
 
 // The one-pass filter + project kernel's tile: kTileWaves wave slices of
 // kChunks chunks of 64 lanes x kRows rows (512-row slices, 4096-row tiles).
-// Measured on the 100M-row scan over (rows per lane, chunks, waves) from
-// (1, 4, 16) to (8, 4, 8): few rows per lane keep the registers (and the
-// kept column's values) small enough for many waves per CU, and 8-wave
-// tiles keep the look-backs few; (2, 4, 8) ran fastest (0.73 ms against
-// 0.77-1.10 ms for the others).
+// Measured on the 100M-row scan (outputs staged in LDS) over (rows per
+// lane, chunks, waves): few rows per lane keep the registers (and the kept
+// column's values) small enough for many waves per CU, and 8-wave tiles
+// keep the look-backs few; (2, 4, 8) ran fastest (0.566 ms against 0.60 to
+// 0.62 for (2, 4, 16), 0.68 for (2, 8, 4), 0.70 for (4, 4, 4)).
 constexpr int kRows = 2, kChunks = 4, kTileWaves = 8;
 constexpr int kTile = 64 * kRows * kChunks * kTileWaves;
+// LDS for a tile's staged output rows (8-byte slots): 32 KB holds 2048 rows
+// of two outputs, half of a tile; with the rest of the kernel's LDS a
+// workgroup stays within 64 KB. Measured on the 100M-row scan (12 % of
+// each 4096-row tile selected): staging 1024 or 2048 rows ran alike.
+constexpr int64_t kStageBytes = 32768, kMaxLds = 65536;
 
 struct Loaded {
   hipModule_t mod = nullptr;
@@ -227,8 +233,9 @@ const char* range_kind(int t) {
 
 // eager: the range tests load every column before the first test (keep_col's
 // values kept by its first range).
+// stage_rows: rows of a one-pass tile's outputs staged in LDS.
 std::string source_struct(const DevPlan& plan, const std::vector<DevOp>& ops, const std::vector<bool>& nt,
-                          int keep_col = -1, bool eager = false) {
+                          int keep_col = -1, bool eager = false, int stage_rows = 0) {
   std::ostringstream s;
   s << "namespace bsql_gen {\nusing namespace bsql_dev;\ntemplate <int r> struct GenRange {};\n";
   bool kept = false;
@@ -244,6 +251,7 @@ std::string source_struct(const DevPlan& plan, const std::vector<DevOp>& ops, co
     << "  static constexpr int kKeepCol = " << keep_col << ";\n"
     << "  static constexpr bool kEager = " << (eager ? "true" : "false") << ";\n"
     << "  static constexpr int kRanges = " << (eager ? plan.n_ranges : 0) << ";\n"
+    << "  static constexpr int kStageRows = " << stage_rows << ";\n"
     << "  template <int r> using RangeAt = GenRange<r>;\n";
   s << "  __device__ static __forceinline__ DevInCol in(const DevPlan& p, int j) {\n"
        "    DevInCol c = p.in[j];\n    switch (j) {\n";
@@ -334,13 +342,20 @@ hipError_t launch_generated_filter_project(const DevPlan& plan, const std::vecto
   int range_bytes = 0;
   for (int r = 0; r < plan.n_ranges; ++r) range_bytes += type_size(plan.in[plan.ranges[r].col].type);
   const bool eager = plan.n_ranges <= 8 && range_bytes <= 16;
-  const hipFunction_t fn = kernel_for(name.str(), full_source(source_struct(plan, ops, nt, keep, eager)));
+  // rows staged per tile: whole waves, within the LDS budget
+  const int vw = (int)(slice / 32 + 2);
+  int cap = 0;
+  if (plan.n_out > 0) {
+    cap = (int)std::min<int64_t>(tile_rows, kStageBytes / (8 * plan.n_out) / 64 * 64);
+    while (cap > 0 && fp_lds_bytes(plan.n_out, cap, kTileWaves, vw) > kMaxLds) cap -= 64;
+  }
+  const hipFunction_t fn = kernel_for(name.str(), full_source(source_struct(plan, ops, nt, keep, eager, cap)));
   if (!fn) return hipSuccess;  // check mode
   hipError_t e = fill_words(status, tiles, 0, stream);
   if (e != hipSuccess) return e;
   FusedArgs fa{status, nullptr, d_count, capacity, tiles};
   void* args[] = {(void*)&d_plan, (void*)&fa};
-  const unsigned lds = (unsigned)(kTileWaves * plan.n_out * (slice / 32 + 2) * 4);
+  const unsigned lds = (unsigned)fp_lds_bytes(plan.n_out, cap, kTileWaves, vw);
   return hipModuleLaunchKernel(fn, (unsigned)tiles, 1, 1, 64 * kTileWaves, 1, 1, lds, stream, args, nullptr);
 }
 

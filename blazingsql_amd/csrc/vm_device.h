@@ -929,6 +929,31 @@ static __device__ int64_t look_back(const CompactArgs& ca, int64_t tile, int64_t
   return excl;
 }
 
+// Exclusive prefix of tile `tile` > 0 whose aggregate `agg` is already
+// published (wave 0 only): the look-back of look_back() without its first
+// store; publishes the tile's inclusive prefix.
+__device__ __forceinline__ int64_t look_back_published(uint64_t* status, int64_t tile, int64_t agg, int lane) {
+  int64_t excl = 0;
+  int64_t look = tile - 1;
+  while (true) {
+    const int64_t t = look - lane;
+    const uint64_t s = t >= 0 ? ld_status(&status[t]) : ST_PREFIX;
+    const uint64_t flag = s & ~ST_VALUE;
+    const uint64_t not_ready = __ballot(flag == 0);
+    const uint64_t is_p = __ballot(flag == ST_PREFIX);
+    const uint64_t upto = is_p ? ((is_p & (~is_p + 1)) << 1) - 1 : ~0ULL;  // lanes [0, first prefix]
+    if (not_ready & upto) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += wave_sum(((upto >> lane) & 1) ? (int64_t)(s & ST_VALUE) : 0);
+    if (is_p) break;
+    look -= 64;
+  }
+  if (lane == 0) st_status(&status[tile], ST_PREFIX | (uint64_t)(excl + agg));
+  return excl;
+}
+
 // look_back with a 64*V-tile window per round trip: each lane reads V
 // consecutive status words (their loads in flight together). For launches
 // with many tiles in flight, so that one round trip usually reaches a
@@ -1009,6 +1034,8 @@ struct PlanSrc {
   // programs, whose range kinds are constants)
   static constexpr bool kEager = false;
   static constexpr int kRanges = 0;
+  // rows of a one-pass tile's outputs staged in LDS (0: none)
+  static constexpr int kStageRows = 0;
   template <int r> struct RangeAt {};
   __device__ static __forceinline__ DevInCol in(const DevPlan& p, int j) { return p.in[j]; }
   __device__ static __forceinline__ DevOutCol out(const DevPlan& p, int o) { return p.out[o]; }
@@ -2066,18 +2093,27 @@ __global__ __launch_bounds__(64) void select_project_fused(const DevPlan* __rest
 // lanes x R rows (lane t owns rows t*R .. t*R+R-1 of each chunk).
 // Workgroups are dispatched in index order, so every earlier tile is
 // resident or done when a tile looks back, and the smallest unfinished tile
-// never waits. Each wave tests the ranges over its slice (each column's
-// loads for the whole slice in flight); the workgroup sums its waves' counts
-// and wave 0 finds the tile's output position by decoupled look-back. Then,
-// chunk by chunk, the lanes that own a selected row load their R rows of
-// the output columns (whole vectors), run the output ops on them
-// straight-line and write each selected row at its wave's position plus the
-// rows selected before it in the slice. Output validity is assembled in LDS
-// per wave and written as whole words, except the two words a wave shares
-// with its neighbours (an AND/OR atomic pair). Only the tile status words
-// need clearing before the launch. P::kKeepCol >= 0: that column's values,
-// loaded by its range test, are kept in registers and used by the output
-// ops instead of a second load.
+// never waits.
+//
+// Each wave tests the ranges over its slice (each column's loads for the
+// whole slice in flight) and the workgroup publishes its row count at once.
+// If the tile selected at most P::kStageRows rows, the lanes that own a
+// selected row then load their R rows of the output columns (whole
+// vectors), run the output ops straight-line and put each result in LDS at
+// its position within the tile (8-byte slots, a null bit per row). Only then
+// does wave 0 look back for the tile's output position: by that time the
+// earlier tiles have mostly published theirs, so one round trip usually
+// settles it, and the output loads were in flight meanwhile instead of
+// after it. The workgroup writes its outputs as one contiguous run per
+// column (consecutive lanes, consecutive rows, non-temporal) and its
+// validity as whole words; the two words at the run's ends, shared with
+// the neighbouring tiles, are merged with atomics that change only the
+// tile's own bits (one OR when none of its rows is null, else an AND/OR
+// pair). A tile that selected more rows than LDS holds takes its position
+// first and writes each selected row from the registers (per-wave validity
+// words in LDS). Only the tile status words need clearing before the
+// launch. P::kKeepCol >= 0: that column's values, loaded by its range test,
+// are kept in registers and used by the output ops instead of a second load.
 template <int R>
 __device__ __forceinline__ void temp_set(LoadTemps<R>& T, int j, const uint64_t (&v)[R], uint32_t vm) {
 #pragma unroll
@@ -2138,22 +2174,124 @@ __device__ __forceinline__ uint32_t eager_ranges(const DevPlan& plan, int64_t ro
   }
 }
 
+// The output ops over a wave's selected rows, chunk by chunk: the lanes
+// that own a selected row load their R rows (whole vectors) and run the ops
+// on them. The rows selected before a lane's first row of chunk k give its
+// position: pos_start + (rows in chunks < k) + (rows of lower lanes in k).
+// CAP > 0: results go to LDS slot o*CAP + position, a null result sets its
+// bit in nulls[o*fp_null_words(CAP) ..] and bit o of *nullmask. CAP == 0:
+// results go to base + position (below capacity), their validity bits to
+// vbits[o*vw ..] at (base & 31) + position.
+template <class P, int NREG, int R, int S, int CAP>
+__device__ __forceinline__ void out_rows(const DevPlan& plan, uint32_t alive, int64_t row0, int64_t num_rows,
+                                         int pos_start, bool have_kept, const uint64_t (&kept)[S][R],
+                                         LDS uint64_t* slots, LDS uint32_t* nulls, LDS uint32_t* nullmask,
+                                         int64_t base, int64_t capacity, LDS uint32_t* vbits, int vw) {
+  typedef typename RegVecT<NREG>::type RegVec;
+  constexpr int CHUNK = 64 * R;
+  constexpr uint32_t RMASK = (1u << R) - 1u;
+  constexpr int KEEP = P::kKeepCol;
+  const int shift = (int)(base & 31);
+  int run = pos_start;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const uint32_t a = (alive >> (k * R)) & RMASK;
+    int lane_excl, chunk_total;
+    chunk_counts<R>(a, lane_excl, chunk_total);
+    if (chunk_total == 0) continue;  // wave-uniform
+    Rows<R> rs;
+    rs.row0 = row0 + (int64_t)k * CHUNK;
+    rs.pos0 = rs.row0;
+    rs.pstride = 1;
+    rs.contig = true;
+    const int64_t left = num_rows - rs.row0;
+    const int exist = left <= 0 ? 0 : (left >= R ? R : (int)left);
+    rs.nrows = a ? exist : 0;  // lanes without a selected row load nothing
+    rs.exist = (1u << rs.nrows) - 1u;
+#pragma unroll
+    for (int j = 0; j < R; ++j) rs.row[j] = rs.row0 + j;
+    RegVec f[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) f[j] = (RegVec)(0);
+    uint64_t vb = 0;
+    LoadTemps<R> T;
+    uint32_t al = rs.exist;
+    const int p0 = run + lane_excl;  // position of the lane's first selected row
+    P::ops(plan, P::split(plan), P::n_ops(plan), [&](const DevOp& op) __attribute__((always_inline)) -> bool {
+      if (KEEP >= 0 && op.code == D_LOADN) {
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          if (jj >= op.aux) break;
+          const int col = (int)((op.imm >> (8 * jj)) & 0xff);
+          uint64_t v[R];
+          uint32_t vm = 0;
+          if (col == KEEP && have_kept) {
+            // selected rows passed the column's range test: valid
+#pragma unroll
+            for (int i = 0; i < R; ++i) v[i] = kept[k][i];
+            vm = rs.exist;
+          } else if (P::nt(col, true)) {
+            col_load<R, true>(P::in(plan, col), rs, v, vm);
+          } else {
+            col_load<R, false>(P::in(plan, col), rs, v, vm);
+          }
+          temp_set<R>(T, jj, v, vm);
+        }
+        return false;
+      }
+      if (op.code != D_STORE) {
+        vm_step<NREG, R, true, P>(plan, op, f, vb, T, rs, al, nullptr);
+        return false;
+      }
+      const int ia = op.a & (NREG - 1);
+      const uint32_t VA = (uint32_t)(vb >> (ia * R)) & a;
+      const DevOutCol oc = P::out(plan, op.aux);
+      int q = p0;
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        if (!((a >> j) & 1u)) continue;
+        if constexpr (CAP > 0) {
+          slots[op.aux * CAP + q] = f[j][ia];
+          if (oc.valid && !((VA >> j) & 1u)) {
+            lds_set_bit(nulls + op.aux * fp_null_words(CAP), q);
+            __hip_atomic_fetch_or(nullmask, 1u << op.aux, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        } else if (base + q < capacity) {
+          // cached stores: the lanes' pieces of a line meet in L2
+          store_elem<false>(oc, base + q, f[j][ia]);
+          if (oc.valid && ((VA >> j) & 1u)) lds_set_bit(vbits + op.aux * vw, shift + q);
+        }
+        ++q;
+      }
+      return false;
+    });
+    run += chunk_total;
+  }
+}
+
 template <class P, int NREG, int R, int S, int W>
 __global__ __launch_bounds__(64 * W) void filter_project_rows(const DevPlan* __restrict__ pplan, FusedArgs fa) {
   typedef typename RegVecT<NREG>::type RegVec;
   constexpr int CHUNK = 64 * R;
   constexpr int SLICE = S * CHUNK;
   constexpr int VW = SLICE / 32 + 2;  // validity words a slice's outputs can touch
-  constexpr uint32_t RMASK = (1u << R) - 1u;
+  constexpr int CAP = P::kStageRows;
+  constexpr int NULLW = fp_null_words(CAP);
   static_assert(R * S <= 32, "selection bits per lane");
+  static_assert(CAP % 64 == 0 && CAP <= W * SLICE, "staged rows: whole waves, at most a tile");
   __shared__ int s_cnt[W];
   __shared__ int64_t s_base;
+  __shared__ uint32_t s_nullmask;  // bit o: a staged row of output o is null
   const DevPlan& plan = *pplan;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t num_rows = plan.num_rows;
   const int64_t tile = blockIdx.x;
   const int n_out = plan.n_out;
-  LDS uint32_t* vbits = (LDS uint32_t*)s_spill_area + (int64_t)wave * n_out * VW;
+  // dynamic LDS (fp_lds_bytes): staged slots, their null bits, then each
+  // wave's validity words for tiles written from the registers
+  LDS uint64_t* s_slots = (LDS uint64_t*)s_spill_area;
+  LDS uint32_t* s_nulls = (LDS uint32_t*)(s_slots + (int64_t)n_out * CAP);
+  LDS uint32_t* vbits = s_nulls + (int64_t)n_out * NULLW + (int64_t)wave * n_out * VW;
   // ---- range tests over the wave's slice
   const int64_t slice_row0 = (tile * W + wave) * SLICE;
   const int64_t row0 = slice_row0 + lane * R;
@@ -2199,18 +2337,27 @@ __global__ __launch_bounds__(64 * W) void filter_project_rows(const DevPlan* __r
   }
   const int cnt = (int)wave_sum(__popc(alive));
   for (int i = lane; i < n_out * VW; i += 64) vbits[i] = 0u;
+  if constexpr (CAP > 0) {
+    for (int i = threadIdx.x; i < n_out * NULLW; i += 64 * W) s_nulls[i] = 0u;
+    if (threadIdx.x == 0) s_nullmask = 0u;
+  }
   if (lane == 0) s_cnt[wave] = cnt;
   __syncthreads();
-  // ---- the tile's output position, then the wave's
   int before = 0, total = 0;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
     before += w < wave ? s_cnt[w] : 0;
     total += s_cnt[w];
   }
+  // the tile's count goes out now; its position is resolved after staging
+  if (threadIdx.x == 0) st_status(&fa.status[tile], (tile == 0 ? ST_PREFIX : ST_AGG) | (uint64_t)total);
+  const bool stage = CAP > 0 && total <= CAP;  // workgroup-uniform
+  if (stage && cnt > 0)
+    out_rows<P, NREG, R, S, CAP>(plan, alive, row0, num_rows, before, have_kept, kept, s_slots, s_nulls,
+                                 (LDS uint32_t*)&s_nullmask, 0, fa.capacity, nullptr, 0);
+  __syncthreads();  // staged rows are in LDS
   if (wave == 0) {
-    CompactArgs ca{fa.status, nullptr, fa.d_count, fa.capacity, fa.num_tiles};
-    const int64_t b = look_back(ca, tile, total, lane);
+    const int64_t b = tile == 0 ? 0 : look_back_published(fa.status, tile, total, lane);
     if (lane == 0) {
       s_base = b;
       if (tile == fa.num_tiles - 1) *gpw<int64_t>(fa.d_count) = b + total;
@@ -2218,83 +2365,58 @@ __global__ __launch_bounds__(64 * W) void filter_project_rows(const DevPlan* __r
   }
   __syncthreads();
   if (total == 0) return;  // workgroup-uniform
-  const int64_t base = s_base + before;
-  if (cnt > 0) {
-    // ---- outputs of the selected rows, by the lanes that own them
-    const int shift = (int)(base & 31);
-    int run = 0;  // rows selected in the slice before chunk k
-#pragma unroll
-    for (int k = 0; k < S; ++k) {
-      const uint32_t a = (alive >> (k * R)) & RMASK;
-      int lane_excl, chunk_total;
-      chunk_counts<R>(a, lane_excl, chunk_total);
-      if (chunk_total == 0) continue;  // wave-uniform
-      Rows<R> rs;
-      rs.row0 = row0 + (int64_t)k * CHUNK;
-      rs.pos0 = rs.row0;
-      rs.pstride = 1;
-      rs.contig = true;
-      const int64_t left = num_rows - rs.row0;
-      const int exist = left <= 0 ? 0 : (left >= R ? R : (int)left);
-      rs.nrows = a ? exist : 0;  // lanes without a selected row load nothing
-      rs.exist = (1u << rs.nrows) - 1u;
-#pragma unroll
-      for (int j = 0; j < R; ++j) rs.row[j] = rs.row0 + j;
-      RegVec f[R];
-#pragma unroll
-      for (int j = 0; j < R; ++j) f[j] = (RegVec)(0);
-      uint64_t vb = 0;
-      LoadTemps<R> T;
-      uint32_t al = rs.exist;
-      const int p0 = run + lane_excl;  // slice position of the lane's first selected row
-      P::ops(plan, P::split(plan), P::n_ops(plan), [&](const DevOp& op) __attribute__((always_inline)) -> bool {
-        if (KEEP >= 0 && op.code == D_LOADN) {
-#pragma unroll
-          for (int jj = 0; jj < 4; ++jj) {
-            if (jj >= op.aux) break;
-            const int col = (int)((op.imm >> (8 * jj)) & 0xff);
-            uint64_t v[R];
-            uint32_t vm = 0;
-            if (col == KEEP && have_kept) {
-              // selected rows passed the column's range test: valid
-#pragma unroll
-              for (int i = 0; i < R; ++i) v[i] = kept[k][i];
-              vm = rs.exist;
-            } else if (P::nt(col, true)) {
-              col_load<R, true>(P::in(plan, col), rs, v, vm);
-            } else {
-              col_load<R, false>(P::in(plan, col), rs, v, vm);
-            }
-            temp_set<R>(T, jj, v, vm);
-          }
-          return false;
-        }
-        if (op.code != D_STORE) {
-          vm_step<NREG, R, true, P>(plan, op, f, vb, T, rs, al, nullptr);
-          return false;
-        }
-        const int ia = op.a & (NREG - 1);
-        const uint32_t VA = (uint32_t)(vb >> (ia * R)) & a;
-        const DevOutCol oc = P::out(plan, op.aux);
-        int q = p0;
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-          if (!((a >> j) & 1u)) continue;
-          if (base + q < fa.capacity) {
-            // cached stores: the lanes' pieces of a line meet in L2
-            store_elem<false>(oc, base + q, f[j][ia]);
-            if (oc.valid && ((VA >> j) & 1u)) lds_set_bit(vbits + op.aux * VW, shift + q);
-          }
-          ++q;
-        }
-        return false;
-      });
-      run += chunk_total;
+  const int64_t tbase = s_base;
+  if (stage) {
+    // ---- the tile's rows [tbase, tbase + nw): contiguous stores per column
+    const int64_t room = fa.capacity - tbase;
+    const int nw = room <= 0 ? 0 : (room < total ? (int)room : total);
+    for (int o = 0; o < n_out; ++o) {
+      const DevOutCol oc = P::out(plan, o);
+      const LDS uint64_t* sl = s_slots + o * CAP;
+      for (int i = threadIdx.x; i < nw; i += 64 * W) store_elem<true>(oc, tbase + i, sl[i]);
     }
+    if (nw == 0) return;
+    const int64_t end = tbase + nw;
+    const int64_t w0 = tbase >> 5, w1 = (end - 1) >> 5;
+    const uint32_t nullmask = s_nullmask;
+    for (int o = 0; o < n_out; ++o) {
+      uint32_t* words = P::out(plan, o).valid;
+      if (!words) continue;
+      const LDS uint32_t* nb = s_nulls + o * NULLW;
+      const bool any_null = (nullmask >> o) & 1u;
+      for (int64_t g = w0 + threadIdx.x; g <= w1; g += 64 * W) {
+        uint32_t own = 0xffffffffu;
+        if (g == w0) own &= 0xffffffffu << (tbase & 31);
+        if (g == w1 && (end & 31)) own &= 0xffffffffu >> (32 - (end & 31));
+        if (!any_null) {
+          if (own == 0xffffffffu) gpw<uint32_t>(words)[g] = own;
+          else __hip_atomic_fetch_or(gpw<uint32_t>(words) + g, own, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          continue;
+        }
+        // null bits of tile positions [l0, l0 + 32)
+        const int64_t l0 = g * 32 - tbase;
+        uint32_t x;
+        if (l0 < 0) {
+          x = nb[0] << (int)(-l0);
+        } else {
+          const int wi = (int)(l0 >> 5), sh = (int)(l0 & 31);
+          const uint64_t two = (uint64_t)nb[wi] | ((uint64_t)nb[wi + 1] << 32);
+          x = (uint32_t)(two >> sh);
+        }
+        merge_word(words, g, own, ~x & own);
+      }
+    }
+    return;
   }
+  // ---- more rows than LDS holds: each wave writes its selected rows at
+  // the tile position plus the rows selected before them
+  const int64_t base = tbase + before;
+  if (cnt > 0)
+    out_rows<P, NREG, R, S, 0>(plan, alive, row0, num_rows, 0, have_kept, kept, nullptr, nullptr, nullptr, base,
+                               fa.capacity, vbits, VW);
   __syncthreads();
-  // ---- validity of the wave's rows: whole words inside its range, the
-  // boundary words merged
+  // validity of the wave's rows: whole words inside its range, the boundary
+  // words merged
   int64_t end = base + cnt;
   if (end > fa.capacity) end = fa.capacity;
   if (cnt == 0 || end <= base) return;

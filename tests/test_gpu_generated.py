@@ -125,6 +125,54 @@ def test_generated_filter_project_capacity(bsql, gpu, n):
         assert_same(got[k][0], got[k][1], full[k][0][:cap], full[k][1][:cap], what=f"sized output {k}")
 
 
+# selective predicates: the one-pass kernel stages each tile's rows in LDS
+# (with five outputs, up to 768 of a 4096-row tile) and writes them as one
+# run; at about 18 % selected, tiles on either side of that limit mix the
+# staged and the direct write in one call
+SELECTIVE = {"3%": "AND(>($0, 900), <($3, 800), >=($1, -900.5:DECIMAL(4, 1)))",
+             "18%": "AND(>($0, 500), <($3, 800), >=($1, -900.5:DECIMAL(4, 1)))"}
+
+
+@pytest.mark.parametrize("sel", sorted(SELECTIVE))
+@pytest.mark.parametrize("n", [2049, 8193, 1_000_003])
+def test_generated_filter_project_staged_rows(bsql, gpu, sel, n):
+    """Outputs with nulls (a 30 %-null column, division by zero) written from
+    LDS-staged tiles: generated against the two-pass interpreter and the oracle."""
+    import oracle
+    rng = np.random.default_rng(n + 3)
+    types = [INT32, FLOAT64, INT64, INT32]
+    table = make_table(rng, types, n, null_rates=[0.1, 0.0, 0.3, 0.05])
+    pred_text = SELECTIVE[sel]
+    print(f"staged filter_project sel={sel} n={n}", flush=True)
+    pred, proj, ot, m, got = _filter_project(bsql, table, n, True, pred_text)
+    _, _, _, m2, ref = _filter_project(bsql, table, n, False, pred_text)
+    om, ores = oracle.filter_project(oracle.full_table_program(pred, 4), oracle.full_table_program(proj, 4),
+                                     host_columns(table), ot, n)
+    assert m == m2 == om, (m, m2, om)
+    assert m > 0
+    for k in range(len(PROJ)):
+        if k in (0, 4):  # +($2, ...) and /($1, $2) over the 30 %-null $2
+            assert not got[k][1].all(), f"output {k} has nulls in the test data"
+        assert_same(got[k][0], got[k][1], ref[k][0], ref[k][1], what=f"generated vs two-pass output {k}")
+        assert_same(got[k][0], got[k][1], ores[k][0], ores[k][1], what=f"generated vs oracle output {k}")
+
+
+@pytest.mark.parametrize("sel", sorted(SELECTIVE))
+def test_generated_filter_project_staged_capacity(bsql, gpu, sel):
+    """Pre-sized outputs cut inside a staged tile: the first `cap` rows written,
+    the full count reported."""
+    n = 300_007
+    rng = np.random.default_rng(17)
+    types = [INT32, FLOAT64, INT64, INT32]
+    table = make_table(rng, types, n, null_rates=[0.1, 0.0, 0.3, 0.05])
+    _, _, _, m_all, full = _filter_project(bsql, table, n, False, SELECTIVE[sel])
+    for cap in (m_all // 3 + 5, m_all - 1):
+        _, _, _, m, got = _filter_project(bsql, table, n, True, SELECTIVE[sel], cap=cap)
+        assert m == m_all
+        for k in range(len(PROJ)):
+            assert_same(got[k][0], got[k][1], full[k][0][:cap], full[k][1][:cap], what=f"sized output {k}")
+
+
 def test_generated_filter_project_scan_shape(bsql, gpu):
     """The 100M-row scan's predicate and projection (range tests on shipdays,
     disc and qty; price * (1 - disc) * (1 + tax) and price * disc) on 3M
