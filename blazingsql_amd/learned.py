@@ -136,6 +136,32 @@ def learned_order(mlp, pred, types, cols, num_rows):
     return sel, cost, order
 
 
+def filter_project_learned(mlp, pred, proj, types, cols, outs, d_count, num_rows, stream=None, order=True):
+    """One filter + project in the learned form: the conjuncts in the order of
+    least predicted cost (order=False keeps the order already set), the
+    outputs written in one pass up to the predicted row bound
+    (bsql_filter_project_sized: positions past it are not written and the
+    full count is reported), and only when the count exceeds the bound a
+    second call over the whole output. `outs` must hold num_rows rows for
+    that second call. Returns (rows selected, whether the bound overflowed,
+    the order set or None); the count is read back, so the call synchronizes
+    the stream."""
+    from .interop import filter_project, filter_project_sized
+    chosen = None
+    if order:
+        sel, _, chosen = learned_order(mlp, pred, types, cols, num_rows)
+    else:
+        f = conjunct_features(pred, types, sketch(cols, num_rows, stream))
+        sel, _ = mlp.predict(f)
+    bound = row_bound(sel, num_rows)
+    filter_project_sized(pred, proj, cols, outs, d_count, bound, num_rows, stream=stream)
+    m = int(d_count.item())
+    if m <= bound:
+        return m, False, chosen
+    filter_project(pred, proj, cols, outs, d_count, num_rows, stream=stream)
+    return int(d_count.item()), True, chosen
+
+
 # ------------------------------------------------------------ synthetic data
 
 COPULA_TYPES = [10, 10, 10, 10, 3, 3, 3, 3]   # 4 x float64, 4 x int32
@@ -176,6 +202,27 @@ def random_predicate(rng, n_terms):
     sels = np.exp(rng.uniform(np.log(0.001), np.log(0.9), n_terms))
     terms = [range_term(rng, int(c), float(s)) for c, s in zip(cols, sels)]
     return ("AND(" + ", ".join(terms) + ")" if n_terms > 1 else terms[0]), terms
+
+
+def copula_predicate(rng, n_terms, sel_range=(0.001, 0.9)):
+    """(text, terms, target): an AND of n_terms range terms on distinct
+    columns whose joint selectivity on independent columns is `target`,
+    log-uniform in sel_range. The terms all bound the same tail (all
+    `>=` high quantiles or all `<` low ones) and split the target by
+    Dirichlet weights (term k selects target^w_k); on correlated columns the
+    joint selectivity rises towards the smallest term's."""
+    cols = rng.permutation(len(COPULA_TYPES))[:n_terms]
+    target = float(np.exp(rng.uniform(np.log(sel_range[0]), np.log(sel_range[1]))))
+    w = rng.dirichlet(np.ones(n_terms))
+    upper = rng.random() < 0.5
+    terms = []
+    for c, wk in zip(cols, w):
+        s = min(max(target ** float(wk), 1e-4), 0.9999)
+        t = COPULA_TYPES[int(c)]
+        v = 1e6 * (1.0 - s) if upper else 1e6 * s
+        lit = f"{v:.2f}:DECIMAL(12, 2)" if t == 10 else str(int(round(v)))
+        terms.append(f">=(${int(c)}, {lit})" if upper else f"<(${int(c)}, {lit})")
+    return ("AND(" + ", ".join(terms) + ")" if n_terms > 1 else terms[0]), terms, target
 
 
 def measure_terms(terms, cols, n, reps=3):

@@ -164,3 +164,56 @@ def test_filter_project_sized_overflow(bsql, gpu):
         assert int(cnt3.item()) == m2, cap2
         w = min(m2, cap2)
         np.testing.assert_array_equal(out2[0].to_numpy(w)[0], full[0].to_numpy(w)[0])
+
+
+@pytest.fixture(scope="module")
+def trained_mlp(bsql, gpu):
+    from blazingsql_amd import learned
+    X, Y = learned.collect_records(1500, rows=1 << 17, seed=21)
+    mlp, _ = learned.train(X, Y, epochs=20)
+    return mlp
+
+
+@pytest.mark.parametrize("rho", [0.0, 0.5, 0.9])
+def test_copula_learned_filter_project_vs_oracle(bsql, gpu, trained_mlp, rho):
+    """The learned-ordering workload's shape at 2M rows: Gaussian-copula
+    tables (8 columns, pairwise rho), predicates of 4-8 range terms whose
+    joint selectivity spans 0.001-0.9, each run in the learned form
+    (predicted order, outputs written up to the predicted row bound, a
+    second call on overflow; half of them through the generated one-pass
+    kernel) and checked bit for bit against the oracle running the static
+    plan order."""
+    import torch
+    import oracle
+    from blazingsql_amd import learned
+    from blazingsql_amd.interop import DeviceColumn
+    n = 2_000_003
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(31 + int(rho * 10))
+    cols = learned.copula_table(n, rho, gen)
+    host = [oracle.HostColumn(c.type_id, c.to_numpy(n)[0]) for c in cols]
+    types = learned.COPULA_TYPES
+    proj_text = ["+($0, $4)", "*($2, 2:DOUBLE)"]
+    rng = np.random.default_rng(int(rho * 10) + 5)
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    reordered, selected = 0, []
+    for i in range(6):
+        text, terms, target = learned.copula_predicate(rng, int(rng.integers(4, 9)))
+        print(f"rho={rho} predicate {i}: {len(terms)} terms, target selectivity {target:.4f}", flush=True)
+        pred = bsql.Program.compile(text, types)
+        pred.set_launch(generated=bool(i % 2))
+        proj = bsql.Program.compile(proj_text, types)
+        outs = [DeviceColumn.empty(10, n), DeviceColumn.empty(10, n)]
+        m, _, order = learned.filter_project_learned(trained_mlp, pred, proj, types, cols, outs, cnt, n)
+        static = bsql.Program.compile(text, types)
+        om, ores = oracle.filter_project(oracle.full_table_program(static, 8), oracle.full_table_program(proj, 8),
+                                         host, [10, 10], n)
+        assert m == om, (text, m, om)
+        for k in range(2):
+            gv, gval = outs[k].to_numpy(m)
+            assert np.array_equal(gval, ores[k][1]), f"validity of output {k}"
+            assert np.array_equal(gv.view(np.uint64), ores[k][0].view(np.uint64)), f"output {k} of {text}"
+        selected.append(m / n)
+        reordered += int(list(order) != list(range(len(terms))))
+    assert max(selected) > 0.01, selected  # the workload selects rows, not only nothing
+    assert reordered > 0
