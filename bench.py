@@ -281,119 +281,173 @@ def run_batched(args):
 
 # ---------------------------------------------------------------- learned
 
-def run_learned(args):
-    """64M rows of 8 float64/int32 columns through a Gaussian copula (rho in
-    {0, 0.5, 0.9}); conjunctions of 4-8 range terms with selectivities
-    log-uniform in [0.001, 0.9], written in a random (static plan) order. The
-    MLP is trained on the GPU on records the exact interpreter makes on
-    synthetic batches, then orders each predicate's terms and bounds its row
-    count (the smallest predicted term selectivity), and the learned call
-    writes into outputs pre-sized for that bound (a count beyond it costs a
-    second, full-size call); the filter+project output is compared bit for
-    bit with the static order's."""
+def learned_workload(n, rhos, predicates, steps, records, seed, per_query=True):
+    """n rows of 8 float64/int32 columns through a Gaussian copula (each rho
+    in rhos); per rho, `predicates` conjunctions of 4-8 range terms whose
+    joint selectivity spans 0.001-0.9, written in a random (static plan)
+    order, each with the projection +($0, $4), *($2, 2:DOUBLE). The MLP is
+    trained on the GPU on records the exact interpreter makes on synthetic
+    batches. Timed per rho:
+      static: each query's plain call (interpreter, plan order), summed;
+      learned and batched: the rho's queries in one bsql_filter_project_batch
+        pass, each predicate in the order of least predicted cost and its
+        outputs written up to its predicted row bound (a query over it runs
+        again on its own);
+    and with per_query, each query alone: the learned order through the
+    interpreter (outputs pre-sized for the bound) and through the generated
+    one-pass kernel, and the order the measured statistics give. Every
+    output is compared bit for bit with the static order's."""
     import numpy as np
     import torch
     import blazingsql_amd as bsql
     from blazingsql_amd import learned
-    from blazingsql_amd.interop import DeviceColumn, filter_project, filter_project_sized
+    from blazingsql_amd.interop import DeviceColumn, filter_project, filter_project_sized, filter_project_batch
     t0 = time.perf_counter()
-    X, Y = learned.collect_records(args.records, rows=1 << 18, seed=args.seed)
+    X, Y = learned.collect_records(records, rows=1 << 18, seed=seed)
     t_collect = time.perf_counter() - t0
     t0 = time.perf_counter()
     mlp, hist = learned.train(X, Y, epochs=40)
     t_train = time.perf_counter() - t0
-    n = args.learned_rows
     types = learned.COPULA_TYPES
     proj_text = ["+($0, $4)", "*($2, 2:DOUBLE)"]
-    rng = np.random.default_rng(args.seed + 1)
+    rng = np.random.default_rng(seed + 1)
     gen = torch.Generator(device="cuda")
-    gen.manual_seed(args.seed + 1)
+    gen.manual_seed(seed + 1)
     cur = torch.cuda.current_stream()
     a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
-    def timed(pred, proj, cols, outs, cnt, bound=None):
-        """ms per call of the plain path, or (bound) of the learned path:
-        outputs pre-sized for `bound` rows, and when the count exceeds it
-        (known from the first call; the count is the same every call) a
-        second, full-size call after each sized one."""
-        def call():
-            if bound is None:
-                filter_project(pred, proj, cols, outs, cnt, n)
-                return
-            filter_project_sized(pred, proj, cols, outs, cnt, bound, n)
-            if overflow:
-                filter_project(pred, proj, cols, outs, cnt, n)
-        overflow = False
-        if bound is not None:
-            filter_project_sized(pred, proj, cols, outs, cnt, bound, n)
-            overflow = int(cnt.item()) > bound
+    def time_calls(call):
         for _ in range(2):
             call()
         a.record(cur)
-        for _ in range(args.steps):
+        for _ in range(steps):
             call()
         z.record(cur)
         torch.cuda.synchronize()
-        return a.elapsed_time(z) / args.steps, overflow
+        return a.elapsed_time(z) / steps
 
-    rows = []
+    def sized_call(pred, proj, cols, outs, cnt, bound):
+        """The learned path of one query: outputs pre-sized for `bound` rows,
+        and a second, full-size call when the count exceeds it."""
+        filter_project_sized(pred, proj, cols, outs, cnt, bound, n)
+        overflow = int(cnt.item()) > bound
+
+        def call():
+            filter_project_sized(pred, proj, cols, outs, cnt, bound, n)
+            if overflow:
+                filter_project(pred, proj, cols, outs, cnt, n)
+        return call, overflow
+
+    def same(outs, ref, m):
+        return all(torch.equal(o.data[:m], r.data[:m]) for o, r in zip(outs, ref))
+
+    rows, sets = [], []
     exact = True
-    for rho in (0.0, 0.5, 0.9):
+    for rho in rhos:
         cols = learned.copula_table(n, rho, gen)
         outs = [DeviceColumn.empty(10, n), DeviceColumn.empty(10, n)]
         ref_outs = [DeviceColumn.empty(10, n), DeviceColumn.empty(10, n)]
         cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
-        for _ in range(args.predicates):
+        proj = bsql.Program.compile(proj_text, types)
+        texts, static_ms, counts = [], [], []
+        for _ in range(predicates):
             text, terms, target = learned.copula_predicate(rng, int(rng.integers(4, 9)))
-            proj = bsql.Program.compile(proj_text, types)
+            texts.append(text)
             static = bsql.Program.compile(text, types)
-            t_static, _ = timed(static, proj, cols, ref_outs, cnt)
+            t_static = time_calls(lambda: filter_project(static, proj, cols, ref_outs, cnt, n))
             m_static = int(cnt.item())
+            static_ms.append(t_static)
+            counts.append(m_static)
+            if not per_query:
+                continue
             lp = bsql.Program.compile(text, types)
             sel, cost, order = learned.learned_order(mlp, lp, types, cols, n)
             bound = learned.row_bound(sel, n)
-            t_learned, overflow = timed(lp, proj, cols, outs, cnt, bound)
-            m_learned = int(cnt.item())
-            same = m_static == m_learned and all(
-                torch.equal(o.data[:m_learned], r.data[:m_learned]) for o, r in zip(outs, ref_outs))
+            call, overflow = sized_call(lp, proj, cols, outs, cnt, bound)
+            t_learned = time_calls(call)
+            ok = int(cnt.item()) == m_static and same(outs, ref_outs, m_static)
             # the same learned order through the one-pass kernel generated for the program
             gp = bsql.Program.compile(text, types)
             gp.set_conjunct_order(order)
             gp.set_launch(generated=True)
             filter_project(gp, proj, cols, outs, cnt, n)  # compiles the kernel
-            t_learned_gen, _ = timed(gp, proj, cols, outs, cnt, bound)
-            same = same and int(cnt.item()) == m_static and all(
-                torch.equal(o.data[:m_static], r.data[:m_static]) for o, r in zip(outs, ref_outs))
-            exact = exact and same
+            call, _ = sized_call(gp, proj, cols, outs, cnt, bound)
+            t_learned_gen = time_calls(call)
+            ok = ok and int(cnt.item()) == m_static and same(outs, ref_outs, m_static)
+            exact = exact and ok
             # the order the exact selectivities and costs give (the regret's reference)
             true_sel, true_t = learned.measure_terms(terms, cols, n, reps=2)
             op = bsql.Program.compile(text, types)
             learned.order_for_scan(op, types, true_sel, true_t)
-            t_oracle, _ = timed(op, proj, cols, outs, cnt, learned.row_bound(true_sel, n))
+            call, _ = sized_call(op, proj, cols, outs, cnt, learned.row_bound(true_sel, n))
+            t_oracle = time_calls(call)
             rows.append({"rho": rho, "terms": len(terms), "target_selectivity": target,
                          "t_static_ms": t_static, "t_learned_ms": t_learned,
-                         "t_learned_generated_ms": t_learned_gen, "t_oracle_order_ms": t_oracle, "sel_mae": float(np.mean(np.abs(sel - true_sel))),
-                         "selected": m_learned, "row_bound": bound, "overflow": bool(overflow),
-                         "exact": bool(same)})
-        del cols, outs, ref_outs
-    sp = np.array([r["t_static_ms"] / r["t_learned_ms"] for r in rows])
-    regret = np.array([r["t_learned_ms"] / r["t_oracle_order_ms"] - 1.0 for r in rows])
-    geo = float(np.exp(np.mean(np.log(sp))))
-    sp_gen = np.array([r["t_static_ms"] / r["t_learned_generated_ms"] for r in rows])
+                         "t_learned_generated_ms": t_learned_gen, "t_oracle_order_ms": t_oracle,
+                         "sel_mae": float(np.mean(np.abs(sel - true_sel))), "selected": m_static,
+                         "row_bound": bound, "overflow": bool(overflow), "exact": bool(ok)})
+        # the learned and batched form of the rho's queries: one pass
+        bpreds = [bsql.Program.compile(t, types) for t in texts]
+        bprojs = [bsql.Program.compile(proj_text, types) for _ in texts]
+        t0 = time.perf_counter()
+        bounds = learned.plan_batch_learned(mlp, bpreds, types, cols, n)
+        plan_ms = (time.perf_counter() - t0) * 1e3
+        bouts = [[DeviceColumn.empty(10, n), DeviceColumn.empty(10, n)] for _ in texts]
+        bcnt = torch.zeros(len(texts), dtype=torch.int64, device="cuda")
+        t0 = time.perf_counter()
+        bc, over = learned.filter_project_batch_learned(mlp, bpreds, bprojs, types, cols, bouts, bcnt, n,
+                                                        bounds=bounds)  # compiles the kernel
+        first_s = time.perf_counter() - t0
+
+        def bcall():
+            filter_project_batch(bpreds, bprojs, cols, bouts, bcnt, n, capacities=bounds)
+            for q, o in enumerate(over):
+                if o:
+                    filter_project(bpreds[q], bprojs[q], cols, bouts[q], bcnt[q:q + 1], n)
+        t_batched = time_calls(bcall)
+        got = [int(c) for c in bcnt.cpu()]
+        ok = got == counts
+        for q, text in enumerate(texts):
+            static = bsql.Program.compile(text, types)
+            filter_project(static, proj, cols, ref_outs, cnt, n)
+            ok = ok and int(cnt.item()) == got[q] and same(bouts[q], ref_outs, got[q])
+        exact = exact and ok
+        sets.append({"rho": rho, "queries": len(texts), "t_static_sum_ms": float(sum(static_ms)),
+                     "t_learned_batched_ms": t_batched, "speedup": float(sum(static_ms)) / t_batched,
+                     "plan_ms": plan_ms, "first_call_s_with_compile": first_s,
+                     "selected": got, "row_bounds": bounds, "overflows": int(sum(over)), "exact": bool(ok)})
+        del cols, outs, ref_outs, bouts
+    geo_set = float(np.exp(np.mean(np.log([st["speedup"] for st in sets]))))
+    res = {"speedup_learned_batched": geo_set, "exact": bool(exact), "sets": sets,
+           "training_records": int(len(X)), "collect_s": t_collect, "train_s": t_train,
+           "final_train_loss": float(hist[-1])}
+    if rows:
+        sp = np.array([r["t_static_ms"] / r["t_learned_ms"] for r in rows])
+        sp_gen = np.array([r["t_static_ms"] / r["t_learned_generated_ms"] for r in rows])
+        regret = np.array([r["t_learned_ms"] / r["t_oracle_order_ms"] - 1.0 for r in rows])
+        res.update({"speedup_learned_per_query": float(np.exp(np.mean(np.log(sp)))),
+                    "speedup_learned_generated_per_query": float(np.exp(np.mean(np.log(sp_gen)))),
+                    "overflows": int(sum(r["overflow"] for r in rows)),
+                    "selectivity_mae": float(np.mean([r["sel_mae"] for r in rows])),
+                    "time_regret_vs_exact_order": float(np.mean(regret)), "per_query": rows})
+    return res
+
+
+def run_learned(args):
+    """The learned workload at full size: 64M rows, rho in {0, 0.5, 0.9}."""
+    import numpy as np
+    n = args.learned_rows
+    res = learned_workload(n, (0.0, 0.5, 0.9), args.predicates, args.steps, args.records, args.seed)
+    sets = res["sets"]
     result = {
-        "metric": "learned-vs-static speedup (filter+project, geomean over predicates), exact",
-        "value": geo, "unit": "x", "n_gpus": 1, "steps": args.steps, "warmup": 2,
-        "ms_per_step": float(np.mean([r["t_learned_ms"] for r in rows])), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64/int32", "data": "synthetic (Gaussian copula)",
-        "config": {"workload": f"{n} rows x 8 columns (4 float64, 4 int32), Gaussian copula rho in "
-                               f"{{0, 0.5, 0.9}}, {args.predicates} predicates per rho of 4-8 range terms",
-                   "exact": bool(exact), "speedup_min": float(sp.min()), "speedup_max": float(sp.max()),
-                   "speedup_learned_generated": float(np.exp(np.mean(np.log(sp_gen)))),
-                   "overflows": int(sum(r["overflow"] for r in rows)),
-                   "selectivity_mae": float(np.mean([r["sel_mae"] for r in rows])),
-                   "time_regret_vs_exact_order": float(np.mean(regret)),
-                   "training_records": int(len(X)), "collect_s": t_collect, "train_s": t_train,
-                   "final_train_loss": float(hist[-1]), "per_predicate": rows},
+        "metric": "learned-and-batched vs static speedup (filter+project of a table's predicates), exact",
+        "value": res["speedup_learned_batched"], "unit": "x", "n_gpus": 1, "steps": args.steps, "warmup": 2,
+        "ms_per_step": float(np.mean([st["t_learned_batched_ms"] for st in sets])) if sets else None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64/int32",
+        "data": "synthetic (Gaussian copula)",
+        "config": dict(workload=f"{n} rows x 8 columns (4 float64, 4 int32), Gaussian copula rho in "
+                                f"{{0, 0.5, 0.9}}, {args.predicates} predicates per rho of 4-8 range terms",
+                       **res),
     }
     print(json.dumps(result))
 
@@ -573,6 +627,8 @@ def main():
                     help="run the scan through the interpreter's selection, row ids and output pass "
                          "instead of the kernel generated for the program")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="leave out the reduced learned-and-batched measurement of the scan line")
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)
     ap.add_argument("--workload", choices=["scan", "batched", "learned", "strings"], default="scan",
                     help="scan: the 100M-row filter+project (the headline line); batched: 512 "
@@ -696,6 +752,20 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None},
     }
+    if not args.no_learned:
+        # the learned and batched form against the static plan order, reduced
+        # to one correlated 16M-row table and 8 predicates (--workload learned:
+        # 64M rows, three correlations, per-query detail)
+        del cols, ins, outs
+        torch.cuda.empty_cache()
+        lw = learned_workload(16 << 20, (0.9,), 8, 10, 4000, args.seed, per_query=False)
+        st = lw["sets"][0]
+        result["learned"] = {"metric": "learned-and-batched vs static speedup", "value": lw["speedup_learned_batched"],
+                             "unit": "x", "exact": lw["exact"],
+                             "workload": "16M rows x 8 columns (4 float64, 4 int32), Gaussian copula rho 0.9, "
+                                         "8 predicates of 4-8 range terms, joint selectivity 0.001-0.9",
+                             "t_static_sum_ms": st["t_static_sum_ms"], "t_learned_batched_ms": st["t_learned_batched_ms"],
+                             "row_bound_overflows": st["overflows"], "training_records": lw["training_records"]}
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_rows, args.seed)
     print(json.dumps(result))

@@ -133,6 +133,9 @@ SIGNATURES = {
     "bsql_program_autotune": (_I, [_P, _P, _I, _P, _I, _I64, _P, ctypes.POINTER(LaunchConfig)]),
     "bsql_filter_count": (_I, [_P, _P, _I, _I64, _P, _P]),
     "bsql_filter_project_sized": (_I, [_P, _P, _P, _I, _P, _I, _I64, _I64, _P, _P]),
+    "bsql_filter_project_batch": (_I, [_P, _P, _I, _P, _I, _P, _P, _I64, _P, _P, _P]),
+    "bsql_generated_batch_kernel_check": (_I, [_P, _P, _I, _P, _I, _P, _P, _I64, ctypes.c_char_p,
+                                               ctypes.POINTER(_I64)]),
     "bsql_sketch": (_I, [_P, _I, _I64, _P, _P]),
     "bsql_mlp_create": (_I, [ctypes.c_uint64, ctypes.POINTER(_P)]),
     "bsql_mlp_num_params": (_I, [_P]),

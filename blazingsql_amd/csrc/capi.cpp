@@ -38,6 +38,12 @@ bool fused_eligible(const DevPlan& plan, int nreg);
 int64_t fused_scratch_words(int64_t num_rows);
 // kernel_gen.cpp
 bool generated_filter_project_eligible(const DevPlan& plan, const std::vector<DevOp>& ops);
+hipError_t launch_generated_filter_project_batch(const std::vector<DevPlan>& plans,
+                                                 const std::vector<std::vector<DevOp>>& ops,
+                                                 const std::vector<std::vector<int>>& in_maps,
+                                                 const std::vector<const DevPlan*>& d_plans, uint64_t* status,
+                                                 int64_t* d_counts, const std::vector<int64_t>& capacity,
+                                                 hipStream_t stream);
 hipError_t launch_generated_filter_project(const DevPlan& plan, const std::vector<DevOp>& ops,
                                            const DevPlan* d_plan, uint64_t* status, int64_t* d_count,
                                            int64_t capacity, hipStream_t stream);
@@ -506,6 +512,121 @@ int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, i
 int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, int n_in, bsql_column* out,
                 int n_out, int64_t num_rows, int64_t capacity, int64_t* d_count, void* stream, int phases) {
   return run_compact(pred, proj, in, n_in, out, n_out, num_rows, capacity, d_count, stream, phases, nullptr);
+}
+
+// Several (predicate, projection) queries over the same table: one launch of
+// the kernel generated for all of them when every query can run as a
+// generated one-pass filter (range-test predicate, fixed-width outputs, no
+// step columns), else one bsql_filter_project call per query. Query q writes
+// out[q][0 .. n_out[q]) up to capacity[q] rows and its count to d_counts[q].
+int run_filter_project_batch(bsql_program* const* preds, bsql_program* const* projs, int nq, const bsql_column* in,
+                             int n_in, bsql_column* const* out, const int32_t* n_out, int64_t num_rows,
+                             const int64_t* capacity, int64_t* d_counts, void* stream) {
+  if (nq < 0 || nq > BSQL_MAX_BATCH_QUERIES) fail("a batch holds at most 16 queries");
+  if (nq > 0 && (!preds || !projs || !out || !n_out || !capacity)) fail("null batch arrays");
+  if (!d_counts) fail("null count pointer");
+  check_inputs(in, n_in, num_rows);
+  if (num_rows > (int64_t)UINT32_MAX) fail("a filter handles at most 2^32-1 rows per call");
+  std::vector<DevPlan> plans;
+  std::vector<std::vector<DevOp>> ops;
+  std::vector<std::vector<int>> in_maps;
+  std::vector<int64_t> caps;
+  bool together = nq > 0;
+  for (int q = 0; q < nq; ++q) {
+    bsql_program* pred = preds[q];
+    bsql_program* proj = projs[q];
+    if (!pred || !proj) fail("null program in batch");
+    if (capacity[q] < 0 || capacity[q] > num_rows) fail("capacity must be in [0, num_rows]");
+    if (n_out[q] != num_roots(*proj) && proj->parts.empty()) fail("number of outputs does not match the program");
+    check_outputs(out[q], n_out[q], capacity[q]);
+    if (!together) continue;
+    if (!pred->steps.empty() || !proj->steps.empty() || !proj->parts.empty() || n_out[q] > BSQL_DEV_MAX_OUT) {
+      together = false;
+      continue;
+    }
+    if (!generated_check_mode()) {
+      hipError_t e = ensure_strings(*pred);
+      if (e == hipSuccess) e = ensure_strings(*proj);
+      if (e != hipSuccess) return device_error(e, "filter batch");
+    }
+    LowerOptions opt;
+    opt.mode = LowerOptions::FILTER_PROJECT;
+    opt.pred = pred;
+    opt.proj = proj;
+    opt.conj_order = &pred->conj_order;
+    opt.ranges = !(pred->launch.flags & BSQL_LAUNCH_NO_RANGES);
+    opt.lazy_loads = (pred->launch.flags & BSQL_LAUNCH_LAZY_LOADS) != 0;
+    opt.reg_budget = std::min(reg_budget(pred), reg_budget(proj));
+    const Lowered& L = lowered_for(pred, opt, in, n_in, out[q], n_out[q]);
+    DevPlan plan = L.plan;
+    plan.num_rows = num_rows;
+    for (size_t j = 0; j < L.in_cols.size(); ++j) plan.in[j] = to_dev_in(in[L.in_cols[j]]);
+    if (count_string_outputs(L, in, out[q], n_out[q]) != 0 || !generated_filter_project_eligible(plan, L.ops)) {
+      together = false;
+      continue;
+    }
+    for (int k = 0; k < n_out[q]; ++k) {
+      plan.out[k].data = out[q][k].data;
+      plan.out[k].valid = out[q][k].validity;
+      plan.out[k].type = out[q][k].type_id;
+    }
+    plans.push_back(plan);
+    ops.push_back(L.ops);
+    in_maps.emplace_back(L.in_cols.begin(), L.in_cols.end());
+    caps.push_back(capacity[q]);
+  }
+  if (!together) {
+    if (generated_check_mode()) fail("this batch does not run as one generated kernel");
+    for (int q = 0; q < nq; ++q) {
+      const int rc = run_filter_project(preds[q], projs[q], in, n_in, out[q], n_out[q], num_rows, capacity[q],
+                                        d_counts + q, stream);
+      if (rc != BSQL_OK) return rc;
+    }
+    return BSQL_OK;
+  }
+  if (generated_check_mode()) {
+    const std::vector<const DevPlan*> none((size_t)nq, nullptr);
+    const hipError_t e =
+        launch_generated_filter_project_batch(plans, ops, in_maps, none, nullptr, d_counts, caps, nullptr);
+    return e == hipSuccess ? BSQL_OK : device_error(e, "generated batch kernel check");
+  }
+  // scratch: every query's tile status words, then the plan blobs
+  const int64_t tiles = generated_filter_project_scratch_words(num_rows);
+  int64_t at = tiles * nq;
+  std::vector<int64_t> plan_at;
+  for (int q = 0; q < nq; ++q) {
+    plan_at.push_back(at);
+    at += plan_words(ops[(size_t)q].size());
+  }
+  uint64_t* scratch = nullptr;
+  hipError_t e = get_scratch((hipStream_t)stream, at, &scratch);
+  if (e != hipSuccess) return device_error(e, "filter batch scratch");
+  // one copy of all the plan blobs (their op pointers aimed at the scratch)
+  std::vector<char> blob((size_t)(at - plan_at[0]) * sizeof(uint64_t), 0);
+  std::vector<const DevPlan*> d_plans;
+  for (int q = 0; q < nq; ++q) {
+    uint64_t* dst = scratch + plan_at[(size_t)q];
+    const std::vector<char> b = plan_blob(plans[(size_t)q], ops[(size_t)q], dst);
+    std::memcpy(blob.data() + (size_t)(plan_at[(size_t)q] - plan_at[0]) * sizeof(uint64_t), b.data(), b.size());
+    d_plans.push_back((const DevPlan*)dst);
+  }
+  {
+    // the stream's scratch no longer holds the single plan it remembers
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find({dev, (hipStream_t)stream});
+    if (it != g_scratch.end()) {
+      it->second.plan.clear();
+      it->second.plan_at = -1;
+    }
+  }
+  e = hipMemcpyAsync(scratch + plan_at[0], blob.data(), blob.size(), hipMemcpyHostToDevice, (hipStream_t)stream);
+  if (e == hipSuccess)
+    e = launch_generated_filter_project_batch(plans, ops, in_maps, d_plans, scratch, d_counts, caps,
+                                              (hipStream_t)stream);
+  if (e != hipSuccess) return device_error(e, "filter batch");
+  return BSQL_OK;
 }
 
 size_t noalloc_scratch_bytes();
@@ -1099,6 +1220,36 @@ int bsql_filter_project_noalloc(const bsql_program* pred, const bsql_program* pr
   BSQL_TRY({
     return run_filter_project_noalloc(const_cast<bsql_program*>(pred), const_cast<bsql_program*>(proj), in, n_in,
                                       out, n_out, num_rows, d_count, stream);
+  })
+}
+
+int bsql_filter_project_batch(const bsql_program* const* preds, const bsql_program* const* projs, int n_queries,
+                              const bsql_column* in, int n_in, bsql_column* const* out, const int32_t* n_out,
+                              int64_t num_rows, const int64_t* capacity, int64_t* d_counts, void* stream) {
+  BSQL_TRY({
+    reject_capture(stream, "bsql_filter_project_batch");
+    return run_filter_project_batch(const_cast<bsql_program* const*>(preds), const_cast<bsql_program* const*>(projs),
+                                    n_queries, in, n_in, out, n_out, num_rows, capacity, d_counts, stream);
+  })
+}
+
+int bsql_generated_batch_kernel_check(const bsql_program* const* preds, const bsql_program* const* projs,
+                                      int n_queries, const bsql_column* in, int n_in, bsql_column* const* out,
+                                      const int32_t* n_out, int64_t num_rows, const char* arch,
+                                      int64_t* code_bytes) {
+  if (!arch || !*arch || num_rows <= 0 || n_queries <= 0) return set_error(BSQL_ERR_INVALID, "invalid arguments");
+  BSQL_TRY({
+    struct Reset {
+      ~Reset() { set_generated_check(nullptr); }
+    } reset;
+    set_generated_check(arch);
+    std::vector<int64_t> caps((size_t)n_queries, num_rows), counts((size_t)n_queries, 0);
+    const int rc = run_filter_project_batch(const_cast<bsql_program* const*>(preds),
+                                            const_cast<bsql_program* const*>(projs), n_queries, in, n_in, out, n_out,
+                                            num_rows, caps.data(), counts.data(), nullptr);
+    if (rc != BSQL_OK) return rc;
+    if (code_bytes) *code_bytes = generated_check_bytes();
+    return BSQL_OK;
   })
 }
 

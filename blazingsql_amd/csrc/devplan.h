@@ -165,6 +165,15 @@ static constexpr inline int64_t fp_lds_bytes(int n_out, int cap, int waves, int 
   return (int64_t)n_out * cap * 8 + (int64_t)n_out * fp_null_words(cap) * 4 + (int64_t)waves * n_out * vw * 4;
 }
 
+// Arguments of the launch that runs several filter + project queries over
+// one table (bsql_filter_project_batch): each query's plan and one-pass
+// arguments.
+#define BSQL_MAX_BATCH_QUERIES 16
+typedef struct MultiArgs {
+  const DevPlan* plan[BSQL_MAX_BATCH_QUERIES];
+  FusedArgs fa[BSQL_MAX_BATCH_QUERIES];
+} MultiArgs;
+
 // LDS bytes of the spill slots of one workgroup: per wave and slot, the
 // 64 lanes x R rows of 8-byte data, then one validity byte per lane.
 static inline int64_t spill_lds_bytes(int n_spill, int block, int rows) {

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <functional>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -233,26 +234,28 @@ const char* range_kind(int t) {
 
 // eager: the range tests load every column before the first test (keep_col's
 // values kept by its first range).
-// stage_rows: rows of a one-pass tile's outputs staged in LDS.
+// stage_rows: rows of a one-pass tile's outputs staged in LDS; name: the
+// struct's name (several programs in one kernel use one struct each).
 std::string source_struct(const DevPlan& plan, const std::vector<DevOp>& ops, const std::vector<bool>& nt,
-                          int keep_col = -1, bool eager = false, int stage_rows = 0) {
+                          int keep_col = -1, bool eager = false, int stage_rows = 0, const char* name = "Src") {
   std::ostringstream s;
-  s << "namespace bsql_gen {\nusing namespace bsql_dev;\ntemplate <int r> struct GenRange {};\n";
+  const std::string ranges = std::string(name) + "Range";
+  s << "namespace bsql_gen {\nusing namespace bsql_dev;\ntemplate <int r> struct " << ranges << " {};\n";
   bool kept = false;
   for (int r = 0; eager && r < plan.n_ranges; ++r) {
     const int c = plan.ranges[r].col;
     const bool keep_here = c == keep_col && !kept;
     kept = kept || keep_here;
-    s << "template <> struct GenRange<" << r << "> {\n  typedef " << range_kind(plan.in[c].type)
+    s << "template <> struct " << ranges << "<" << r << "> {\n  typedef " << range_kind(plan.in[c].type)
       << " K;\n  static constexpr int col = " << c << ";\n  static constexpr bool keep = "
       << (keep_here ? "true" : "false") << ";\n};\n";
   }
-  s << "struct Src {\n  static constexpr bool kPlanOps = false;\n"
+  s << "struct " << name << " {\n  static constexpr bool kPlanOps = false;\n"
     << "  static constexpr int kKeepCol = " << keep_col << ";\n"
     << "  static constexpr bool kEager = " << (eager ? "true" : "false") << ";\n"
     << "  static constexpr int kRanges = " << (eager ? plan.n_ranges : 0) << ";\n"
     << "  static constexpr int kStageRows = " << stage_rows << ";\n"
-    << "  template <int r> using RangeAt = GenRange<r>;\n";
+    << "  template <int r> using RangeAt = " << ranges << "<r>;\n";
   s << "  __device__ static __forceinline__ DevInCol in(const DevPlan& p, int j) {\n"
        "    DevInCol c = p.in[j];\n    switch (j) {\n";
   for (int j = 0; j < plan.n_in; ++j) {
@@ -314,13 +317,27 @@ bool generated_filter_project_eligible(const DevPlan& plan, const std::vector<De
   return true;
 }
 
-hipError_t launch_generated_filter_project(const DevPlan& plan, const std::vector<DevOp>& ops,
-                                           const DevPlan* d_plan, uint64_t* status, int64_t* d_count,
-                                           int64_t capacity, hipStream_t stream) {
+namespace {
+
+// How the one-pass kernel runs one program: the source of its constants
+// (struct `name`) and the dynamic LDS it needs.
+struct OnePassShape {
+  std::string source;
+  int64_t lds = 0;
+};
+
+// shared[j]: input column j is read by other programs of the same kernel
+// too (its loads stay in the caches for them).
+// batch: for the batched kernel, whose selection is generated separately
+// (no range columns kept, no eager range loads).
+OnePassShape one_pass_shape(const DevPlan& plan, const std::vector<DevOp>& ops, const char* name,
+                            const std::vector<bool>* shared = nullptr, bool batch = false) {
   // the output ops' columns that a range test reads: the widest one stays
   // in registers, the others are read again through the cache
   std::vector<bool> out_cols = loaded_columns(plan, ops, plan.split, plan.n_ops);
   std::vector<bool> nt((size_t)plan.n_in, true);
+  if (shared)
+    for (int j = 0; j < plan.n_in; ++j) nt[(size_t)j] = !(*shared)[(size_t)j];
   int keep = -1;
   for (int r = 0; r < plan.n_ranges; ++r) {
     const int c = plan.ranges[r].col;
@@ -328,35 +345,195 @@ hipError_t launch_generated_filter_project(const DevPlan& plan, const std::vecto
     nt[(size_t)c] = false;
     if (keep < 0 || type_size(plan.in[c].type) > type_size(plan.in[keep].type)) keep = c;
   }
-  if (keep >= 0) nt[(size_t)keep] = true;
-  const int R = kRows, S = kChunks;
-  const int64_t slice = 64 * R * S, tile_rows = slice * kTileWaves;
-  const int64_t tiles = (plan.num_rows + tile_rows - 1) / tile_rows;
-  if (tiles == 0) return fill_words((uint64_t*)d_count, 1, 0, stream);
-  if (tiles > (int64_t)INT32_MAX) fail("a generated filter handles at most 2^31-1 tiles per call");
-  std::ostringstream name;
-  name << "&bsql_dev::filter_project_rows<bsql_gen::Src, 16, " << R << ", " << S << ", "
-       << kTileWaves << ">";
+  if (keep >= 0) nt[(size_t)keep] = !(shared && (*shared)[(size_t)keep]);
+  if (batch) keep = -1;
+  const int64_t slice = 64 * kRows * kChunks;
   // every range column in flight at once when their elements fit the
   // registers (at most 16 bytes per row over the ranges)
   int range_bytes = 0;
   for (int r = 0; r < plan.n_ranges; ++r) range_bytes += type_size(plan.in[plan.ranges[r].col].type);
-  const bool eager = plan.n_ranges <= 8 && range_bytes <= 16;
+  const bool eager = !batch && plan.n_ranges <= 8 && range_bytes <= 16;
   // rows staged per tile: whole waves, within the LDS budget
   const int vw = (int)(slice / 32 + 2);
   int cap = 0;
   if (plan.n_out > 0) {
-    cap = (int)std::min<int64_t>(tile_rows, kStageBytes / (8 * plan.n_out) / 64 * 64);
+    cap = (int)std::min<int64_t>(kTile, kStageBytes / (8 * plan.n_out) / 64 * 64);
     while (cap > 0 && fp_lds_bytes(plan.n_out, cap, kTileWaves, vw) > kMaxLds) cap -= 64;
   }
-  const hipFunction_t fn = kernel_for(name.str(), full_source(source_struct(plan, ops, nt, keep, eager, cap)));
+  OnePassShape sh;
+  sh.source = source_struct(plan, ops, nt, keep, eager, cap, name);
+  sh.lds = fp_lds_bytes(plan.n_out, cap, kTileWaves, vw);
+  return sh;
+}
+
+int64_t one_pass_tiles(int64_t num_rows) {
+  const int64_t tiles = (num_rows + kTile - 1) / kTile;
+  if (tiles > (int64_t)INT32_MAX) fail("a generated filter handles at most 2^31-1 tiles per call");
+  return tiles;
+}
+
+}  // namespace
+
+hipError_t launch_generated_filter_project(const DevPlan& plan, const std::vector<DevOp>& ops,
+                                           const DevPlan* d_plan, uint64_t* status, int64_t* d_count,
+                                           int64_t capacity, hipStream_t stream) {
+  const int64_t tiles = one_pass_tiles(plan.num_rows);
+  if (tiles == 0) return fill_words((uint64_t*)d_count, 1, 0, stream);
+  std::ostringstream name;
+  name << "&bsql_dev::filter_project_rows<bsql_gen::Src, 16, " << kRows << ", " << kChunks << ", " << kTileWaves
+       << ">";
+  const OnePassShape sh = one_pass_shape(plan, ops, "Src");
+  const hipFunction_t fn = kernel_for(name.str(), full_source(sh.source));
   if (!fn) return hipSuccess;  // check mode
   hipError_t e = fill_words(status, tiles, 0, stream);
   if (e != hipSuccess) return e;
   FusedArgs fa{status, nullptr, d_count, capacity, tiles};
   void* args[] = {(void*)&d_plan, (void*)&fa};
-  const unsigned lds = (unsigned)fp_lds_bytes(plan.n_out, cap, kTileWaves, vw);
-  return hipModuleLaunchKernel(fn, (unsigned)tiles, 1, 1, 64 * kTileWaves, 1, 1, lds, stream, args, nullptr);
+  return hipModuleLaunchKernel(fn, (unsigned)tiles, 1, 1, 64 * kTileWaves, 1, 1, (unsigned)sh.lds, stream, args,
+                               nullptr);
+}
+
+namespace {
+
+// The selection of a batch of queries (struct Batch): for each table column
+// some range reads, in `order`, one load of its chunks for the lanes where a
+// query testing it still has a row (cached when `cached[c]`: the queries'
+// outputs read it again), then every range on it applied to its query's
+// rows. A column that no lane needs is not read.
+std::string batch_select_source(const std::vector<DevPlan>& plans, const std::vector<std::vector<int>>& in_maps,
+                                const std::vector<int>& order, const std::map<int, bool>& cached) {
+  const int nq = (int)plans.size();
+  std::ostringstream s;
+  s << "namespace bsql_gen {\nstruct Batch {\n  template <int R, int S, bool FULL>\n"
+    << "  __device__ static __forceinline__ void select(const MultiArgs& ma, int64_t row0, int64_t num_rows,\n"
+    << "                                                uint32_t (&alive)[" << nq << "]) {\n"
+    << "    constexpr uint32_t RMASK = (1u << R) - 1u;\n";
+  for (int c : order) {
+    struct Use {
+      int q, r, j;
+    };
+    std::vector<Use> uses;
+    for (int q = 0; q < nq; ++q)
+      for (int r = 0; r < plans[(size_t)q].n_ranges; ++r) {
+        const int j = plans[(size_t)q].ranges[r].col;
+        if (in_maps[(size_t)q][(size_t)j] == c) uses.push_back({q, r, j});
+      }
+    if (uses.empty()) continue;
+    const DevPlan& p0 = plans[(size_t)uses[0].q];
+    std::ostringstream need;
+    std::vector<bool> seen((size_t)nq, false);
+    for (const Use& u : uses) {
+      if (seen[(size_t)u.q]) continue;
+      seen[(size_t)u.q] = true;
+      need << (need.tellp() > 0 ? " | " : "") << "alive[" << u.q << "]";
+    }
+    auto ci = cached.find(c);
+    const bool cache = ci != cached.end() && ci->second;
+    s << "    {  // table column " << c << "\n"
+      << "      const uint32_t need = " << need.str() << ";\n"
+      << "      if (__ballot(need != 0) != 0) {\n"
+      << "        typedef " << range_kind(p0.in[uses[0].j].type) << " K;\n"
+      << "        typedef K::T CT;\n"
+      << "        const DevInCol c = Src" << uses[0].q << "::in(*ma.plan[" << uses[0].q << "], " << uses[0].j << ");\n"
+      << "        CT v[S][R];\n        uint32_t vm = 0;\n"
+      << "#pragma unroll\n        for (int k = 0; k < S; ++k) {\n"
+      << "          const int64_t r0 = row0 + (int64_t)k * 64 * R;\n"
+      << "          int n = R;\n"
+      << "          if (!FULL) {\n"
+      << "            const int64_t left = num_rows - r0;\n"
+      << "            n = left <= 0 ? 0 : (left >= R ? R : (int)left);\n"
+      << "          }\n"
+      << "          if (((need >> (k * R)) & RMASK) == 0) n = 0;\n"
+      << "          load_raw<CT, R, " << (cache ? "false" : "true") << ">(c, r0, n, v[k]);\n"
+      << "          vm |= load_valid<R>(c, r0, n) << (k * R);\n"
+      << "        }\n";
+    for (const Use& u : uses) {
+      s << "        {\n          const DevRange rg = Src" << u.q << "::range(*ma.plan[" << u.q << "], " << u.r << ");\n"
+        << "          uint32_t pass = 0;\n"
+        << "          if (!(rg.kind & RK_EMPTY)) {\n"
+        << "#pragma unroll\n          for (int k = 0; k < S; ++k) pass |= range_bits<K, R>(rg, v[k]) << (k * R);\n"
+        << "          }\n"
+        << "          alive[" << u.q << "] &= vm & pass;\n        }\n";
+    }
+    s << "      }\n    }\n";
+  }
+  s << "  }\n};\n}  // namespace bsql_gen\n";
+  return s.str();
+}
+
+}  // namespace
+
+// in_maps[q][j]: the table column that input j of plans[q] is.
+hipError_t launch_generated_filter_project_batch(const std::vector<DevPlan>& plans,
+                                                 const std::vector<std::vector<DevOp>>& ops,
+                                                 const std::vector<std::vector<int>>& in_maps,
+                                                 const std::vector<const DevPlan*>& d_plans, uint64_t* status,
+                                                 int64_t* d_counts, const std::vector<int64_t>& capacity,
+                                                 hipStream_t stream) {
+  const int nq = (int)plans.size();
+  if (nq < 1 || nq > BSQL_MAX_BATCH_QUERIES) fail("a batch holds 1 to 16 queries");
+  const int64_t tiles = one_pass_tiles(plans[0].num_rows);
+  if (tiles == 0) return fill_words((uint64_t*)d_counts, nq, 0, stream);
+  // who reads each table column: the selection (once, for all queries) and
+  // each query's outputs; a column read again after the selection is
+  // loaded through the caches
+  std::map<int, int> readers;
+  std::map<int, int> first;  // table column -> earliest position of a range on it in a query's order
+  std::map<int, int> uses;
+  for (int q = 0; q < nq; ++q) {
+    const DevPlan& pl = plans[(size_t)q];
+    const std::vector<bool> out_cols = loaded_columns(pl, ops[(size_t)q], pl.split, pl.n_ops);
+    for (int j = 0; j < pl.n_in; ++j)
+      if (out_cols[(size_t)j]) ++readers[in_maps[(size_t)q][(size_t)j]];
+    for (int r = 0; r < pl.n_ranges; ++r) {
+      const int c = in_maps[(size_t)q][(size_t)pl.ranges[r].col];
+      auto f = first.find(c);
+      if (f == first.end() || r < f->second) first[c] = r;
+      ++uses[c];
+    }
+  }
+  std::map<int, bool> cached;
+  std::vector<int> order;
+  for (auto& f : first) {
+    order.push_back(f.first);
+    cached[f.first] = readers[f.first] > 0;
+    ++readers[f.first];
+  }
+  // the columns in the order the predicates' (learned) conjunct orders
+  // put them: earliest first, then the ones more queries test
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return first[a] != first[b] ? first[a] < first[b] : uses[a] > uses[b];
+  });
+  std::ostringstream name, body;
+  name << "&bsql_dev::filter_project_batch<16, " << kRows << ", " << kChunks << ", " << kTileWaves
+       << ", bsql_gen::Batch";
+  int64_t lds = 0;
+  for (int q = 0; q < nq; ++q) {
+    const std::string sname = "Src" + std::to_string(q);
+    const DevPlan& pl = plans[(size_t)q];
+    std::vector<bool> shared((size_t)pl.n_in, false);
+    for (int j = 0; j < pl.n_in; ++j) shared[(size_t)j] = readers[in_maps[(size_t)q][(size_t)j]] > 1;
+    const OnePassShape sh = one_pass_shape(pl, ops[(size_t)q], sname.c_str(), &shared, true);
+    body << sh.source;
+    lds = std::max(lds, sh.lds);
+    name << ", bsql_gen::" << sname;
+  }
+  name << ">";
+  body << batch_select_source(plans, in_maps, order, cached);
+  const hipFunction_t fn = kernel_for(name.str(), full_source(body.str()));
+  if (!fn) return hipSuccess;  // check mode
+  // each query's status words: tiles of them from status + q * tiles
+  hipError_t e = fill_words(status, tiles * nq, 0, stream);
+  if (e != hipSuccess) return e;
+  MultiArgs ma;
+  std::memset(&ma, 0, sizeof(ma));
+  for (int q = 0; q < nq; ++q) {
+    ma.plan[q] = d_plans[(size_t)q];
+    ma.fa[q] = FusedArgs{status + (size_t)q * (size_t)tiles, nullptr, d_counts + q, capacity[(size_t)q], tiles};
+  }
+  void* args[] = {(void*)&ma};
+  return hipModuleLaunchKernel(fn, (unsigned)tiles, 1, 1, 64 * kTileWaves, 1, 1, (unsigned)lds, stream, args,
+                               nullptr);
 }
 
 int64_t generated_filter_project_scratch_words(int64_t num_rows) {

@@ -954,6 +954,18 @@ __device__ __forceinline__ int64_t look_back_published(uint64_t* status, int64_t
   return excl;
 }
 
+// Exclusive prefix of the tile (wave 0 of its workgroup, or the wave that
+// owns the query; the tile's aggregate is published): *s_base, and the row
+// count once the last tile knows it.
+__device__ __forceinline__ void tile_position(const FusedArgs& fa, int64_t tile, int total, int lane,
+                                              int64_t* s_base) {
+  const int64_t b = tile == 0 ? 0 : look_back_published(fa.status, tile, total, lane);
+  if (lane == 0) {
+    *s_base = b;
+    if (tile == fa.num_tiles - 1) *gpw<int64_t>(fa.d_count) = b + total;
+  }
+}
+
 // look_back with a 64*V-tile window per round trip: each lane reads V
 // consecutive status words (their loads in flight together). For launches
 // with many tiles in flight, so that one round trip usually reaches a
@@ -2153,7 +2165,10 @@ __device__ __forceinline__ uint32_t eager_ranges(const DevPlan& plan, int64_t ro
         const int64_t left = num_rows - r0;
         n = left <= 0 ? 0 : (left >= R ? R : (int)left);
       }
-      load_raw<CT, R, true>(c, r0, n, v[k]);
+      if (P::nt(RA::col, true))
+        load_raw<CT, R, true>(c, r0, n, v[k]);
+      else
+        load_raw<CT, R, false>(c, r0, n, v[k]);
       vm |= load_valid<R>(c, r0, n) << (k * R);
     }
     // the later ranges' loads go out before this range's test
@@ -2269,8 +2284,94 @@ __device__ __forceinline__ void out_rows(const DevPlan& plan, uint32_t alive, in
   }
 }
 
+// A one-pass tile's staged rows (LDS slots s_slots + o * CAP, position i of
+// the tile) written to positions tbase + i below the capacity, one
+// contiguous run per output (consecutive lanes, consecutive rows,
+// non-temporal), then their validity as whole words: the two words at the
+// run's ends, shared with the neighbouring tiles, merged with atomics that
+// change only the tile's bits (one OR when no row of the output is null,
+// bit o of nullmask; else an AND/OR pair). s_nulls + o * fp_null_words(CAP):
+// the null bits by tile position.
+template <class P, int W, int CAP>
+__device__ __forceinline__ void fp_flush_staged(const DevPlan& plan, const FusedArgs& fa, int64_t tbase, int total,
+                                                const LDS uint64_t* s_slots, const LDS uint32_t* s_nulls,
+                                                uint32_t nullmask) {
+  constexpr int NULLW = fp_null_words(CAP);
+  const int n_out = plan.n_out;
+  const int64_t room = fa.capacity - tbase;
+  const int nw = room <= 0 ? 0 : (room < total ? (int)room : total);
+  for (int o = 0; o < n_out; ++o) {
+    const DevOutCol oc = P::out(plan, o);
+    const LDS uint64_t* sl = s_slots + o * CAP;
+    for (int i = threadIdx.x; i < nw; i += 64 * W) store_elem<true>(oc, tbase + i, sl[i]);
+  }
+  if (nw == 0) return;
+  const int64_t end = tbase + nw;
+  const int64_t w0 = tbase >> 5, w1 = (end - 1) >> 5;
+  for (int o = 0; o < n_out; ++o) {
+    uint32_t* words = P::out(plan, o).valid;
+    if (!words) continue;
+    const LDS uint32_t* nb = s_nulls + o * NULLW;
+    const bool any_null = (nullmask >> o) & 1u;
+    for (int64_t g = w0 + threadIdx.x; g <= w1; g += 64 * W) {
+      uint32_t own = 0xffffffffu;
+      if (g == w0) own &= 0xffffffffu << (tbase & 31);
+      if (g == w1 && (end & 31)) own &= 0xffffffffu >> (32 - (end & 31));
+      if (!any_null) {
+        if (own == 0xffffffffu) gpw<uint32_t>(words)[g] = own;
+        else __hip_atomic_fetch_or(gpw<uint32_t>(words) + g, own, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        continue;
+      }
+      // null bits of tile positions [l0, l0 + 32)
+      const int64_t l0 = g * 32 - tbase;
+      uint32_t x;
+      if (l0 < 0) {
+        x = nb[0] << (int)(-l0);
+      } else {
+        const int wi = (int)(l0 >> 5), sh = (int)(l0 & 31);
+        const uint64_t two = (uint64_t)nb[wi] | ((uint64_t)nb[wi + 1] << 32);
+        x = (uint32_t)(two >> sh);
+      }
+      merge_word(words, g, own, ~x & own);
+    }
+  }
+}
+
+// A wave's selected rows (alive, its slice from row0) written from the
+// registers at base, base + 1, .. (below the capacity), then their validity:
+// whole words inside the wave's range, the boundary words merged. vbits: the
+// wave's LDS validity words (VW per output), cleared by the caller before a
+// barrier.
+template <class P, int NREG, int R, int S>
+__device__ __forceinline__ void fp_write_direct(const DevPlan& plan, const FusedArgs& fa, uint32_t alive,
+                                                int64_t row0, int64_t base, int cnt, bool have_kept,
+                                                const uint64_t (&kept)[S][R], LDS uint32_t* vbits) {
+  constexpr int VW = S * 64 * R / 32 + 2;
+  const int lane = threadIdx.x & 63;
+  const int n_out = plan.n_out;
+  if (cnt > 0)
+    out_rows<P, NREG, R, S, 0>(plan, alive, row0, plan.num_rows, 0, have_kept, kept, nullptr, nullptr, nullptr, base,
+                               fa.capacity, vbits, VW);
+  __syncthreads();
+  int64_t end = base + cnt;
+  if (end > fa.capacity) end = fa.capacity;
+  if (cnt == 0 || end <= base) return;
+  const int64_t w0 = base >> 5, w1 = (end - 1) >> 5;
+  for (int o = 0; o < n_out; ++o) {
+    uint32_t* words = P::out(plan, o).valid;
+    if (!words) continue;
+    for (int64_t g = w0 + lane; g <= w1; g += 64) {
+      uint32_t own = 0xffffffffu;
+      if (g == w0) own &= 0xffffffffu << (base & 31);
+      if (g == w1 && (end & 31)) own &= 0xffffffffu >> (32 - (end & 31));
+      merge_word(words, g, own, vbits[o * VW + (int)(g - w0)] & own);
+    }
+  }
+}
+
+// One tile (blockIdx.x) of the one-pass filter + project of plan / fa.
 template <class P, int NREG, int R, int S, int W>
-__global__ __launch_bounds__(64 * W) void filter_project_rows(const DevPlan* __restrict__ pplan, FusedArgs fa) {
+__device__ __forceinline__ void fp_tile(const DevPlan& plan, const FusedArgs& fa) {
   typedef typename RegVecT<NREG>::type RegVec;
   constexpr int CHUNK = 64 * R;
   constexpr int SLICE = S * CHUNK;
@@ -2282,7 +2383,6 @@ __global__ __launch_bounds__(64 * W) void filter_project_rows(const DevPlan* __r
   __shared__ int s_cnt[W];
   __shared__ int64_t s_base;
   __shared__ uint32_t s_nullmask;  // bit o: a staged row of output o is null
-  const DevPlan& plan = *pplan;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t num_rows = plan.num_rows;
   const int64_t tile = blockIdx.x;
@@ -2356,81 +2456,135 @@ __global__ __launch_bounds__(64 * W) void filter_project_rows(const DevPlan* __r
     out_rows<P, NREG, R, S, CAP>(plan, alive, row0, num_rows, before, have_kept, kept, s_slots, s_nulls,
                                  (LDS uint32_t*)&s_nullmask, 0, fa.capacity, nullptr, 0);
   __syncthreads();  // staged rows are in LDS
-  if (wave == 0) {
-    const int64_t b = tile == 0 ? 0 : look_back_published(fa.status, tile, total, lane);
-    if (lane == 0) {
-      s_base = b;
-      if (tile == fa.num_tiles - 1) *gpw<int64_t>(fa.d_count) = b + total;
-    }
-  }
+  if (wave == 0) tile_position(fa, tile, total, lane, &s_base);
   __syncthreads();
   if (total == 0) return;  // workgroup-uniform
-  const int64_t tbase = s_base;
   if (stage) {
-    // ---- the tile's rows [tbase, tbase + nw): contiguous stores per column
-    const int64_t room = fa.capacity - tbase;
-    const int nw = room <= 0 ? 0 : (room < total ? (int)room : total);
-    for (int o = 0; o < n_out; ++o) {
-      const DevOutCol oc = P::out(plan, o);
-      const LDS uint64_t* sl = s_slots + o * CAP;
-      for (int i = threadIdx.x; i < nw; i += 64 * W) store_elem<true>(oc, tbase + i, sl[i]);
-    }
-    if (nw == 0) return;
-    const int64_t end = tbase + nw;
-    const int64_t w0 = tbase >> 5, w1 = (end - 1) >> 5;
-    const uint32_t nullmask = s_nullmask;
-    for (int o = 0; o < n_out; ++o) {
-      uint32_t* words = P::out(plan, o).valid;
-      if (!words) continue;
-      const LDS uint32_t* nb = s_nulls + o * NULLW;
-      const bool any_null = (nullmask >> o) & 1u;
-      for (int64_t g = w0 + threadIdx.x; g <= w1; g += 64 * W) {
-        uint32_t own = 0xffffffffu;
-        if (g == w0) own &= 0xffffffffu << (tbase & 31);
-        if (g == w1 && (end & 31)) own &= 0xffffffffu >> (32 - (end & 31));
-        if (!any_null) {
-          if (own == 0xffffffffu) gpw<uint32_t>(words)[g] = own;
-          else __hip_atomic_fetch_or(gpw<uint32_t>(words) + g, own, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          continue;
-        }
-        // null bits of tile positions [l0, l0 + 32)
-        const int64_t l0 = g * 32 - tbase;
-        uint32_t x;
-        if (l0 < 0) {
-          x = nb[0] << (int)(-l0);
-        } else {
-          const int wi = (int)(l0 >> 5), sh = (int)(l0 & 31);
-          const uint64_t two = (uint64_t)nb[wi] | ((uint64_t)nb[wi + 1] << 32);
-          x = (uint32_t)(two >> sh);
-        }
-        merge_word(words, g, own, ~x & own);
-      }
-    }
+    fp_flush_staged<P, W, CAP>(plan, fa, s_base, total, s_slots, s_nulls, s_nullmask);
     return;
   }
   // ---- more rows than LDS holds: each wave writes its selected rows at
   // the tile position plus the rows selected before them
-  const int64_t base = tbase + before;
-  if (cnt > 0)
-    out_rows<P, NREG, R, S, 0>(plan, alive, row0, num_rows, 0, have_kept, kept, nullptr, nullptr, nullptr, base,
-                               fa.capacity, vbits, VW);
-  __syncthreads();
-  // validity of the wave's rows: whole words inside its range, the boundary
-  // words merged
-  int64_t end = base + cnt;
-  if (end > fa.capacity) end = fa.capacity;
-  if (cnt == 0 || end <= base) return;
-  const int64_t w0 = base >> 5, w1 = (end - 1) >> 5;
-  for (int o = 0; o < n_out; ++o) {
-    uint32_t* words = P::out(plan, o).valid;
-    if (!words) continue;
-    for (int64_t g = w0 + lane; g <= w1; g += 64) {
-      uint32_t own = 0xffffffffu;
-      if (g == w0) own &= 0xffffffffu << (base & 31);
-      if (g == w1 && (end & 31)) own &= 0xffffffffu >> (32 - (end & 31));
-      merge_word(words, g, own, vbits[o * VW + (int)(g - w0)] & own);
-    }
+  fp_write_direct<P, NREG, R, S>(plan, fa, alive, row0, s_base + before, cnt, have_kept, kept, vbits);
+}
+
+template <class P, int NREG, int R, int S, int W>
+__global__ __launch_bounds__(64 * W) void filter_project_rows(const DevPlan* __restrict__ pplan, FusedArgs fa) {
+  fp_tile<P, NREG, R, S, W>(*pplan, fa);
+}
+
+// The outputs of one tile of plan / fa whose position tbase is already
+// known (the batched launch finds every query's position before writing
+// any): the rows `alive` selects (the lane's bits, wave slices as in
+// fp_tile), `before` of them in the tile's lower waves, `cnt` in this wave,
+// `total` in the tile. Staged in LDS and written as runs when total <=
+// P::kStageRows, else written from the registers; then their validity.
+template <class P, int NREG, int R, int S, int W>
+__device__ __forceinline__ void fp_write_at(const DevPlan& plan, const FusedArgs& fa, uint32_t alive, int64_t tbase,
+                                            int before, int cnt, int total) {
+  constexpr int CHUNK = 64 * R;
+  constexpr int SLICE = S * CHUNK;
+  constexpr int VW = SLICE / 32 + 2;
+  constexpr int CAP = P::kStageRows;
+  constexpr int NULLW = fp_null_words(CAP);
+  static_assert(CAP % 64 == 0 && CAP <= W * SLICE, "staged rows: whole waves, at most a tile");
+  __shared__ uint32_t s_nullmask;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n_out = plan.n_out;
+  const int64_t tile = blockIdx.x;
+  LDS uint64_t* s_slots = (LDS uint64_t*)s_spill_area;
+  LDS uint32_t* s_nulls = (LDS uint32_t*)(s_slots + (int64_t)n_out * CAP);
+  LDS uint32_t* vbits = s_nulls + (int64_t)n_out * NULLW + (int64_t)wave * n_out * VW;
+  const int64_t row0 = (tile * W + wave) * SLICE + lane * R;
+  for (int i = lane; i < n_out * VW; i += 64) vbits[i] = 0u;
+  if constexpr (CAP > 0) {
+    for (int i = threadIdx.x; i < n_out * NULLW; i += 64 * W) s_nulls[i] = 0u;
+    if (threadIdx.x == 0) s_nullmask = 0u;
   }
+  __syncthreads();
+  if (total == 0) return;  // workgroup-uniform
+  uint64_t none[S][R];
+  if (CAP > 0 && total <= CAP) {
+    if (cnt > 0)
+      out_rows<P, NREG, R, S, CAP>(plan, alive, row0, plan.num_rows, before, false, none, s_slots, s_nulls,
+                                   (LDS uint32_t*)&s_nullmask, 0, fa.capacity, nullptr, 0);
+    __syncthreads();
+    fp_flush_staged<P, W, CAP>(plan, fa, tbase, total, s_slots, s_nulls, s_nullmask);
+    return;
+  }
+  fp_write_direct<P, NREG, R, S>(plan, fa, alive, row0, tbase + before, cnt, false, none, vbits);
+}
+
+// Several filter + project queries over the same table in one launch, for
+// kernels generated for the set (B: its selection; Ps: one source per
+// query). Workgroup tile blockIdx.x holds W wave slices (as fp_tile). B::select
+// tests the queries' ranges column by column: each table column that some
+// range reads is loaded once, for the lanes where a query testing it still
+// has a row, and every such range is applied to it. The workgroup then
+// publishes every query's count, waves look back for the queries' output
+// positions in parallel (wave w for queries w, w + W, ..), and each query's
+// outputs are written (fp_write_at). Each query has its own plan, tile status
+// words, count and outputs; a tile waits only on smaller tiles of the same
+// query, which are resident or done (dispatch in index order).
+template <int NREG, int R, int S, int W, class B, class... Ps>
+__global__ __launch_bounds__(64 * W) void filter_project_batch(MultiArgs ma) {
+  constexpr int NQ = sizeof...(Ps);
+  constexpr int CHUNK = 64 * R;
+  constexpr int SLICE = S * CHUNK;
+  static_assert(R * S <= 32, "selection bits per lane");
+  __shared__ int s_cnt[NQ][W];
+  __shared__ int64_t s_base[NQ];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t tile = blockIdx.x;
+  const int64_t num_rows = ma.plan[0]->num_rows;
+  const int64_t slice_row0 = (tile * W + wave) * SLICE;
+  const int64_t row0 = slice_row0 + lane * R;
+  uint32_t exist = 0;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const int64_t left = num_rows - (row0 + (int64_t)k * CHUNK);
+    const int n = left <= 0 ? 0 : (left >= R ? R : (int)left);
+    exist |= ((1u << n) - 1u) << (k * R);
+  }
+  uint32_t alive[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) alive[q] = exist;
+  if (slice_row0 + SLICE <= num_rows)
+    B::template select<R, S, true>(ma, row0, num_rows, alive);
+  else
+    B::template select<R, S, false>(ma, row0, num_rows, alive);
+  int cnt[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    cnt[q] = __builtin_amdgcn_readfirstlane((int)wave_sum(__popc(alive[q])));
+    if (lane == 0) s_cnt[q][wave] = cnt[q];
+  }
+  __syncthreads();
+  int before[NQ], total[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    int b = 0, t = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const int v = s_cnt[q][w];
+      b += w < wave ? v : 0;
+      t += v;
+    }
+    before[q] = b;
+    total[q] = t;
+    if (threadIdx.x == q)
+      st_status(&ma.fa[q].status[tile], (tile == 0 ? ST_PREFIX : ST_AGG) | (uint64_t)t);
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+    if (wave == q % W) tile_position(ma.fa[q], tile, total[q], lane, &s_base[q]);
+  __syncthreads();
+  int q = 0;
+  ((fp_write_at<Ps, NREG, R, S, W>(*ma.plan[q], ma.fa[q], alive[q], s_base[q], before[q], cnt[q], total[q]),
+    __syncthreads(), ++q),
+   ...);
 }
 
 // Row selection by ops [0, split) (KILL per conjunct with a wave-uniform

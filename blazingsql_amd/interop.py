@@ -503,6 +503,53 @@ def filter_project_sized(pred, proj, in_cols, out_cols, d_count, capacity, num_r
                                               _stream_ptr(stream)))
 
 
+def _batch_arrays(preds, projs, out_lists):
+    nq = len(preds)
+    if len(projs) != nq or len(out_lists) != nq:
+        raise ValueError("one projection and one output list per predicate")
+    pa = (ctypes.c_void_p * max(nq, 1))(*[p.handle for p in preds])
+    ja = (ctypes.c_void_p * max(nq, 1))(*[p.handle for p in projs])
+    outs = [columns_array(o) for o in out_lists]
+    oa = (ctypes.c_void_p * max(nq, 1))(*[ctypes.cast(o, ctypes.c_void_p) for o in outs])
+    na = (ctypes.c_int32 * max(nq, 1))(*[len(o) for o in out_lists])
+    return pa, ja, outs, oa, na
+
+
+def filter_project_batch(preds, projs, in_cols, out_lists, d_counts, num_rows=None, capacities=None,
+                         stream=None):
+    """Queries (preds[q], projs[q]) over the same columns in one pass
+    (bsql_filter_project_batch): query q writes out_lists[q] up to
+    capacities[q] rows (default: num_rows) and its full count to d_counts[q]
+    (an int64 CUDA tensor with one entry per query)."""
+    if num_rows is None:
+        num_rows = in_cols[0].size if in_cols else 0
+    nq = len(preds)
+    caps = [num_rows] * nq if capacities is None else [int(c) for c in capacities]
+    pa, ja, outs, oa, na = _batch_arrays(preds, projs, out_lists)
+    ca = (ctypes.c_int64 * max(nq, 1))(*caps)
+    ins = columns_array(in_cols)
+    L.check(L.lib().bsql_filter_project_batch(pa, ja, nq, ins, len(in_cols), oa, na, num_rows, ca,
+                                              ctypes.c_void_p(d_counts.data_ptr()), _stream_ptr(stream)))
+
+
+def check_generated_batch_kernel(preds, projs, in_cols, out_lists, num_rows, arch="gfx950"):
+    """Compile-only check of filter_project_batch's generated kernel over
+    columns like these (L.Column descriptors or device columns): the code
+    size in bytes. No GPU needed."""
+    def arr(cols):
+        return (L.Column * max(len(cols), 1))(*[c if isinstance(c, L.Column) else c.to_c() for c in cols])
+    nq = len(preds)
+    pa = (ctypes.c_void_p * nq)(*[p.handle for p in preds])
+    ja = (ctypes.c_void_p * nq)(*[p.handle for p in projs])
+    outs = [arr(o) for o in out_lists]
+    oa = (ctypes.c_void_p * nq)(*[ctypes.cast(o, ctypes.c_void_p) for o in outs])
+    na = (ctypes.c_int32 * nq)(*[len(o) for o in out_lists])
+    nbytes = ctypes.c_int64()
+    L.check(L.lib().bsql_generated_batch_kernel_check(pa, ja, nq, arr(in_cols), len(in_cols), oa, na, num_rows,
+                                                      arch.encode(), ctypes.byref(nbytes)))
+    return nbytes.value
+
+
 class Batch:
     """Many (program, columns, rows) batches run as one persistent launch
     (bsql_batch_*). The launch allocates and copies nothing, so it can be

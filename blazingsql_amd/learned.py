@@ -162,6 +162,44 @@ def filter_project_learned(mlp, pred, proj, types, cols, outs, d_count, num_rows
     return int(d_count.item()), True, chosen
 
 
+def plan_batch_learned(mlp, preds, types, cols, num_rows, stream=None):
+    """Orders every predicate's conjuncts by least predicted cost and returns
+    each one's output row bound: one sketch of the table and one forward
+    pass of the MLP over all the predicates' conjuncts."""
+    sk = sketch(cols, num_rows, stream)
+    feats = [conjunct_features(p, types, sk) for p in preds]
+    sel, cost = mlp.predict(np.concatenate(feats, axis=0))
+    bounds, at = [], 0
+    for p, f in zip(preds, feats):
+        s, c = sel[at:at + len(f)], cost[at:at + len(f)]
+        at += len(f)
+        order_for_scan(p, types, s, c)
+        bounds.append(row_bound(s, num_rows))
+    return bounds
+
+
+def filter_project_batch_learned(mlp, preds, projs, types, cols, out_lists, d_counts, num_rows, stream=None,
+                                 bounds=None):
+    """The learned and batched form of several queries over one table: each
+    predicate's conjuncts in the order of least predicted cost, each query's
+    outputs written up to its predicted row bound, all the queries in one
+    pass (bsql_filter_project_batch); a query whose count exceeds its bound
+    runs again on its own over the whole output (`out_lists[q]` must hold
+    num_rows rows for that). bounds: from plan_batch_learned (planned here
+    when None). Returns (counts, which queries overflowed); the counts are
+    read back, so the call synchronizes the stream."""
+    from .interop import filter_project, filter_project_batch
+    if bounds is None:
+        bounds = plan_batch_learned(mlp, preds, types, cols, num_rows, stream)
+    filter_project_batch(preds, projs, cols, out_lists, d_counts, num_rows, capacities=bounds, stream=stream)
+    counts = [int(c) for c in d_counts.cpu()]
+    over = [c > b for c, b in zip(counts, bounds)]
+    for q, o in enumerate(over):
+        if o:
+            filter_project(preds[q], projs[q], cols, out_lists[q], d_counts[q:q + 1], num_rows, stream=stream)
+    return counts, over
+
+
 # ------------------------------------------------------------ synthetic data
 
 COPULA_TYPES = [10, 10, 10, 10, 3, 3, 3, 3]   # 4 x float64, 4 x int32

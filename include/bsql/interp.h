@@ -305,6 +305,30 @@ int bsql_filter_project_sized(const bsql_program* pred, const bsql_program* proj
                               int n_in, bsql_column* out, int n_out, int64_t num_rows, int64_t capacity,
                               int64_t* d_count, void* stream);
 
+/* Several filter + project queries over the same table in one pass; no
+ * counterpart in the reference, whose executor runs each query's Filter and
+ * Projection kernels on its own. For callers with many predicates over one
+ * table (the learned form of several queries, dashboards): query q is
+ * preds[q] then projs[q], writes out[q][0 .. n_out[q]) up to capacity[q]
+ * rows and its full count to d_counts[q], exactly as bsql_filter_project_sized
+ * would. When every query is a range-test predicate with fixed-width outputs
+ * and no string-building steps, one kernel generated for all of them runs
+ * tile by tile over the table, each query reading the tile's columns soon
+ * after the queries before it (hipRTC compile on first use of the set's
+ * shape); otherwise the queries run one call each. At most 16 queries. */
+int bsql_filter_project_batch(const bsql_program* const* preds, const bsql_program* const* projs,
+                              int n_queries, const bsql_column* in, int n_in, bsql_column* const* out,
+                              const int32_t* n_out, int64_t num_rows, const int64_t* capacity,
+                              int64_t* d_counts, void* stream);
+
+/* Compile-only check of bsql_filter_project_batch's generated kernel for
+ * `arch` (no GPU needed; nothing is launched): BSQL_OK and the code size, or
+ * BSQL_ERR_INVALID when the batch would not run as one generated kernel. */
+int bsql_generated_batch_kernel_check(const bsql_program* const* preds, const bsql_program* const* projs,
+                                      int n_queries, const bsql_column* in, int n_in,
+                                      bsql_column* const* out, const int32_t* n_out, int64_t num_rows,
+                                      const char* arch, int64_t* code_bytes);
+
 /* Conjunct order: for a predicate whose root is a left-deep AND chain (the
  * shape expand_if_logical_op produces, CalciteExpressionParsing.cpp:330-398),
  * evaluate its conjuncts in `order` (a permutation of 0..n_conj-1) with a

@@ -61,3 +61,20 @@ def test_calls_that_stay_on_the_interpreter(bsql):
         check_generated_kernel(pred, proj, cols, [_col(4, n, True)], n)
     with pytest.raises(BsqlError):
         check_generated_kernel(pred, proj, cols, [_col(4, n, True)], 0)
+
+
+def test_batch_kernel_compiles(bsql):
+    """Three queries over one table in one generated kernel (the scan's
+    predicate and projection, a range on another column with a
+    one-output projection, and a nullable table column)."""
+    from blazingsql_amd.interop import check_generated_batch_kernel
+    n = 1_000_000
+    preds = [bsql.Program.compile(SCAN_PRED, SCAN_TYPES), bsql.Program.compile("<($1, 5000.0:DECIMAL(5, 1))", SCAN_TYPES),
+             bsql.Program.compile("AND(>($0, 10), <($4, 9000))", SCAN_TYPES)]
+    projs = [bsql.Program.compile(SCAN_PROJ, SCAN_TYPES), bsql.Program.compile(["+($0, $4)"], SCAN_TYPES),
+             bsql.Program.compile(["*($3, 2:DOUBLE)", "$0"], SCAN_TYPES)]
+    ins = [_col(t, n, valid=(i == 0)) for i, t in enumerate(SCAN_TYPES)]
+    outs = [[_col(FLOAT64, n, True), _col(FLOAT64, n, True)], [_col(4, n, True)],
+            [_col(FLOAT64, n, True), _col(INT32, n, True)]]
+    nbytes = check_generated_batch_kernel(preds, projs, ins, outs, n)
+    assert nbytes > 1000

@@ -294,3 +294,88 @@ def test_generated_kernel_cache_reuses_shapes(bsql, gpu):
     _, ref = _execute(bsql, ["+(*($0, 7), -2)", "*($1, 4.25:DECIMAL(3, 2))"], table, n, False)
     for k in range(2):
         assert_same(got[k][0], got[k][1], ref[k][0], ref[k][1], what=f"reused kernel output {k}")
+
+
+BATCH_QUERIES = [
+    ("AND(>($0, 900), <($3, 800), >=($1, -900.5:DECIMAL(4, 1)))", PROJ),
+    ("<($1, -400.0:DECIMAL(5, 1))", ["*($2, 3)", "+($1, $0)"]),
+    ("AND(>=($3, -200), <=($3, 200))", ["/($1, $2)"]),
+    (">($2, 0)", ["$0", "CASE(IS NOT NULL($3), $3, -1)"]),
+]
+
+
+def _batch_run(bsql, table, n, queries, caps=None):
+    import torch
+    from blazingsql_amd.interop import DeviceColumn, filter_project_batch
+    types = [t for t, _, _ in table]
+    preds = [bsql.Program.compile(p, types) for p, _ in queries]
+    projs = [bsql.Program.compile(j, types) for _, j in queries]
+    ots = [[pj.output_type(types, k) for k in range(pj.num_outputs)] for pj in projs]
+    outs = [[DeviceColumn.empty(t, max(n, 1)) for t in ot] for ot in ots]
+    cnt = torch.full((len(queries),), -7, dtype=torch.int64, device="cuda")
+    filter_project_batch(preds, projs, device_columns(table), outs, cnt, n, capacities=caps)
+    torch.cuda.synchronize()
+    counts = [int(c) for c in cnt.cpu()]
+    got = []
+    for q in range(len(queries)):
+        k = counts[q] if caps is None else min(counts[q], caps[q])
+        got.append([o.to_numpy(k) for o in outs[q]])
+    return preds, projs, ots, counts, got
+
+
+@pytest.mark.parametrize("n", [31, 8193, 1_000_003])
+def test_filter_project_batch_vs_calls_and_oracle(bsql, gpu, n):
+    """Four queries over one table in one generated launch: each query's
+    count and outputs equal its own bsql_filter_project call's and the
+    oracle's."""
+    import oracle
+    rng = np.random.default_rng(n + 41)
+    types = [INT32, FLOAT64, INT64, INT32]
+    table = make_table(rng, types, n, null_rates=[0.1, 0.0, 0.3, 0.05])
+    print(f"batch n={n}", flush=True)
+    _, l0, _ = _stats()
+    preds, projs, ots, counts, got = _batch_run(bsql, table, n, BATCH_QUERIES)
+    _, l1, _ = _stats()
+    assert l1 > l0, "the generated kernel did not run"
+    for q, (pt, jt) in enumerate(BATCH_QUERIES):
+        _, _, _, m, ref = _filter_project(bsql, table, n, False, pt, jt)
+        om, ores = oracle.filter_project(oracle.full_table_program(preds[q], 4),
+                                         oracle.full_table_program(projs[q], 4), host_columns(table), ots[q], n)
+        assert counts[q] == m == om, (q, counts[q], m, om)
+        for k in range(len(jt)):
+            assert_same(got[q][k][0], got[q][k][1], ref[k][0], ref[k][1], what=f"query {q} output {k} vs call")
+            assert_same(got[q][k][0], got[q][k][1], ores[k][0], ores[k][1], what=f"query {q} output {k} vs oracle")
+
+
+def test_filter_project_batch_capacities(bsql, gpu):
+    """Per-query capacities: the first capacity rows written, full counts."""
+    n = 300_007
+    rng = np.random.default_rng(77)
+    types = [INT32, FLOAT64, INT64, INT32]
+    table = make_table(rng, types, n, null_rates=[0.1, 0.0, 0.3, 0.05])
+    _, _, _, full_counts, full = _batch_run(bsql, table, n, BATCH_QUERIES)
+    caps = [c // 2 + 3 for c in full_counts]
+    _, _, _, counts, got = _batch_run(bsql, table, n, BATCH_QUERIES, caps=caps)
+    assert counts == full_counts
+    for q in range(len(BATCH_QUERIES)):
+        for k in range(len(got[q])):
+            assert_same(got[q][k][0], got[q][k][1], full[q][k][0][:caps[q]], full[q][k][1][:caps[q]],
+                        what=f"sized query {q} output {k}")
+
+
+def test_filter_project_batch_falls_back_per_query(bsql, gpu):
+    """A query whose predicate is not range tests: the batch runs each
+    query on its own, with the same results."""
+    import oracle
+    n = 50_001
+    rng = np.random.default_rng(5)
+    types = [INT32, FLOAT64, INT64, INT32]
+    table = make_table(rng, types, n, null_rates=[0.1, 0.0, 0.3, 0.05])
+    queries = BATCH_QUERIES[:2] + [(">(*($0, 2), $3)", ["+($0, 1)"])]
+    preds, projs, ots, counts, got = _batch_run(bsql, table, n, queries)
+    for q in range(len(queries)):
+        om, ores = oracle.filter_project(oracle.full_table_program(preds[q], 4),
+                                         oracle.full_table_program(projs[q], 4), host_columns(table), ots[q], n)
+        assert counts[q] == om
+        for k in range(len(got[q])):
+            assert_same(got[q][k][0], got[q][k][1], ores[k][0], ores[k][1], what=f"fallback query {q} output {k}")
