@@ -25,7 +25,9 @@
 //   edges, 2 words per state (for the backtracking matcher): x | y << 16
 //   (0xFFFF: none; x is tried first) and cond | (save slot + 1) << 8 (cond
 //   1: only at the start, 2: only at the end; slot 2g / 2g+1 = start / end
-//   of group g, g >= 1).
+//   of group g, g >= 1; cond bit 7: the split of an unbounded loop, whose
+//   body starts at the next state and whose last body state jumps back to
+//   it).
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -239,7 +241,7 @@ struct State {
   int kind;
   uint32_t arg = 0;                  // code point or index into classes
   int x = -1, y = -1;                // epsilon edges (-1: none; consuming states go to +1); x first
-  int cond = 0;                      // epsilon condition: 1 start only, 2 end only
+  int cond = 0;                      // epsilon condition: 1 start only, 2 end only; 128: loop split
   int save = -1;                     // capture slot written when passing
 };
 
@@ -311,6 +313,7 @@ struct Builder {
         };
         if (n.max < 0) {
           const int split = add({K_EPS});
+          st[(size_t)split].cond = 128;  // the matcher repeats no iteration that consumed nothing
           emit(body);
           const int back = add({K_EPS});
           st[(size_t)back].x = split;

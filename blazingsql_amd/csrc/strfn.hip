@@ -289,8 +289,11 @@ __device__ __forceinline__ T fixed_at(const StrSrc& a, int64_t row) {
 // ---- regular expressions (the program of regex_compile, in regex.cpp)
 
 constexpr int RX_STACK = 256;       // backtracking frames per lane
-constexpr int RX_STEPS = 1 << 20;   // steps per match attempt
+constexpr int RX_STEPS = 1 << 20;   // matcher steps per row (over all start positions)
 constexpr int32_t RX_RANGE = 1 << 30;  // frame tag: a greedy one-character loop's positions
+constexpr int32_t RX_LOOP = 1 << 29;   // frame tag: the body of a lazy loop, entered on backtracking
+constexpr int32_t RX_LOOP_SPLIT = 128;  // edge condition bit: the split of an unbounded loop
+constexpr int RX_MAX_STATES = 128;
 
 __device__ bool rx_in_class(const int32_t* prog, int32_t at, uint32_t cp) {
   const int32_t nr = prog[at];
@@ -313,18 +316,36 @@ __device__ __forceinline__ bool rx_accepts(const int32_t* prog, uint32_t k, cons
 // compiled) match of the regex starting exactly at byte `at` of s[0, n); on
 // success caps[0..1] is the match and caps[2g..2g+1] group g (-1: unset).
 // A greedy loop over one character state runs as far as it can and leaves
-// one frame for all its shorter lengths. A match needing more than RX_STACK
-// pending choices or RX_STEPS steps counts as no match.
-__device__ bool rx_match_at(const uint8_t* s, int32_t n, int32_t at, const int32_t* prog, int32_t* caps) {
+// one frame for all its shorter lengths. An iteration of an unbounded loop
+// that consumed nothing is not repeated: the match goes on after the loop
+// with that iteration's captures (Python's rule). nonempty: a match that
+// consumes nothing does not count (the search goes on to the next choice).
+// Returns 1 (match), 0 (no match) or -1 when the match needs more than
+// RX_STACK pending choices or the row's remaining *steps.
+__device__ int rx_match_at(const uint8_t* s, int32_t n, int32_t at, const int32_t* prog, int32_t* caps,
+                           int32_t* steps, bool nonempty) {
   const int32_t G = prog[3];
   const int32_t* kind = prog + 4;
   const int32_t* edges = prog + prog[2];
   int32_t stk_a[RX_STACK], stk_b[RX_STACK], stk_c[RX_STACK];
+  int32_t iter_start[RX_MAX_STATES];  // an unbounded loop's current iteration: its start (by split state)
   int sp = 0;
   for (int g = 0; g < 2 * (G + 1); ++g) caps[g] = -1;
   caps[0] = at;
   int32_t pc = 0, pos = at;
-  for (int steps = 0; steps < RX_STEPS; ++steps) {
+  // enters the body of the loop whose split is `split` at pos (its
+  // iteration start kept, restored on backtracking)
+  auto enter_body = [&](int32_t split) -> bool {
+    if (sp >= RX_STACK) return false;
+    stk_a[sp] = -(64 + split + 1);
+    stk_b[sp] = iter_start[split];
+    ++sp;
+    iter_start[split] = pos;
+    pc = split + 1;
+    return true;
+  };
+  while (true) {
+    if (--*steps < 0) return -1;
     bool ok = true;
     const uint32_t k = (uint32_t)kind[pc];
     const uint32_t kk = k & 7u;
@@ -336,23 +357,32 @@ __device__ bool rx_match_at(const uint8_t* s, int32_t n, int32_t at, const int32
         ++pc;
       }
     } else if (kk == 3) {
-      caps[1] = pos;
-      return true;
-    } else {
-      const uint32_t e0 = (uint32_t)edges[2 * pc], e1 = (uint32_t)edges[2 * pc + 1];
-      const uint32_t cond = e1 & 0xFFu;
-      const int slot = (int)(e1 >> 8) - 1;
-      if ((cond == 1 && pos != 0) || (cond == 2 && pos != n)) {
+      if (nonempty && pos == at) {
         ok = false;
       } else {
+        caps[1] = pos;
+        return 1;
+      }
+    } else {
+      const uint32_t e0 = (uint32_t)edges[2 * pc], e1 = (uint32_t)edges[2 * pc + 1];
+      const uint32_t cond = e1 & 3u;
+      const bool loop = (e1 & (uint32_t)RX_LOOP_SPLIT) != 0;
+      const int slot = (int)(e1 >> 8) - 1;
+      const uint32_t x = e0 & 0xFFFFu, y = e0 >> 16;
+      if ((cond == 1 && pos != 0) || (cond == 2 && pos != n)) {
+        ok = false;
+      } else if (x < (uint32_t)pc && y == 0xFFFFu && slot < 0 &&
+                 ((uint32_t)edges[2 * x + 1] & (uint32_t)RX_LOOP_SPLIT) && iter_start[x] == pos) {
+        // the jump back of a loop whose iteration consumed nothing: leave it
+        ++pc;
+      } else {
         if (slot >= 0) {
-          if (sp >= RX_STACK) return false;
+          if (sp >= RX_STACK) return -1;
           stk_a[sp] = -(slot + 1);
           stk_b[sp] = caps[slot];
           ++sp;
           caps[slot] = pos;
         }
-        const uint32_t x = e0 & 0xFFFFu, y = e0 >> 16;
         // greedy loop over one character state: split -> (body at +1, out),
         // body -> jump back to the split
         const bool run = slot < 0 && cond == 0 && y != 0xFFFFu && x == (uint32_t)pc + 1 &&
@@ -365,7 +395,7 @@ __device__ bool rx_match_at(const uint8_t* s, int32_t n, int32_t at, const int32
           int32_t l = 0;
           while (pos < n && rx_accepts(prog, body, s, pos, n, &l)) pos += l;
           if (pos > from) {  // the shorter lengths, tried longest first on backtracking
-            if (sp >= RX_STACK) return false;
+            if (sp >= RX_STACK) return -1;
             stk_a[sp] = RX_RANGE | (int32_t)y;
             stk_b[sp] = pos;
             stk_c[sp] = from;
@@ -374,21 +404,27 @@ __device__ bool rx_match_at(const uint8_t* s, int32_t n, int32_t at, const int32
           pc = (int32_t)y;
         } else {
           if (y != 0xFFFFu) {
-            if (sp >= RX_STACK) return false;
-            stk_a[sp] = (int32_t)y;
+            if (sp >= RX_STACK) return -1;
+            // the second choice of a lazy loop enters its body
+            stk_a[sp] = (loop && y == (uint32_t)pc + 1) ? (RX_LOOP | pc) : (int32_t)y;
             stk_b[sp] = pos;
             ++sp;
           }
-          pc = (int32_t)x;
+          if (loop && x == (uint32_t)pc + 1) {
+            if (!enter_body(pc)) return -1;
+          } else {
+            pc = (int32_t)x;
+          }
         }
       }
     }
     if (!ok) {
       while (true) {
-        if (sp == 0) return false;
+        if (sp == 0) return 0;
         const int32_t a = stk_a[sp - 1];
-        if (a < 0) {  // restore a capture slot
-          caps[-a - 1] = stk_b[sp - 1];
+        if (a < 0) {  // restore a capture slot or a loop's iteration start
+          if (a <= -65) iter_start[-a - 65] = stk_b[sp - 1];
+          else caps[-a - 1] = stk_b[sp - 1];
           --sp;
           continue;
         }
@@ -401,29 +437,42 @@ __device__ bool rx_match_at(const uint8_t* s, int32_t n, int32_t at, const int32
           break;
         }
         --sp;
-        pc = a;
         pos = stk_b[sp];
+        if (a & RX_LOOP) {
+          if (!enter_body(a & ~RX_LOOP)) return -1;
+        } else {
+          pc = a;
+        }
         break;
       }
     }
   }
-  return false;
 }
 
 // cudf::strings::replace_with_backrefs over s[0, n): matches left to right,
-// each replaced by `repl` with \1..\9 the groups (an unset group is empty);
-// an empty match is replaced too and the character after it copied (the rule
-// of Python's re.sub).
-__device__ void rx_replace(Out& o, const uint8_t* s, int32_t n, const int32_t* prog, const uint8_t* repl,
+// each replaced by `repl` with \1..\9 the groups (an unset group is empty).
+// Empty matches are replaced too; after an empty match at p the next match
+// may start at p only if it is not empty (the rule of Python's re.sub,
+// whose results the oracle holds). False when a match needs more than the
+// matcher's frames or the row's steps: the row's result is null.
+__device__ bool rx_replace(Out& o, const uint8_t* s, int32_t n, const int32_t* prog, const uint8_t* repl,
                            int32_t rn) {
   const int32_t G = prog[3];
   int32_t caps[20];
-  int32_t i = 0;
-  while (i <= n) {
-    int32_t at = i;
+  int32_t steps = RX_STEPS;
+  int32_t i = 0;              // copied up to here
+  int32_t from = 0;           // the next search starts here
+  bool must_advance = false;  // the last match was empty and ended at `from`
+  while (from <= n) {
+    int32_t at = from;
     bool found = false;
     while (at <= n) {
-      if (rx_match_at(s, n, at, prog, caps)) { found = true; break; }
+      const int r = rx_match_at(s, n, at, prog, caps, &steps, must_advance && at == from);
+      if (r < 0) return false;
+      if (r > 0) {
+        found = true;
+        break;
+      }
       if (at == n) break;
       at += char_bytes(s[at]) <= n - at ? char_bytes(s[at]) : n - at;
     }
@@ -441,16 +490,12 @@ __device__ void rx_replace(Out& o, const uint8_t* s, int32_t n, const int32_t* p
       }
     }
     const int32_t e = caps[1];
-    if (e == at) {  // an empty match: keep the next character and move past it
-      if (at >= n) return;
-      const int32_t l = char_bytes(s[at]) <= n - at ? char_bytes(s[at]) : n - at;
-      o.put(s + at, l);
-      i = at + l;
-    } else {
-      i = e;
-    }
+    i = e;
+    from = e;
+    must_advance = e == at;
   }
   if (i < n) o.put(s + i, n - i);
+  return true;
 }
 
 // Row `row` of the result into o; false for a null row.
@@ -541,8 +586,7 @@ __device__ bool build_row(const StrFnArgs& f, int64_t row, Out& o) {
       int32_t b = 0;
       for (int32_t c = 0; c < f.start && b < n; ++c) b += char_bytes(s[b]) <= n - b ? char_bytes(s[b]) : n - b;
       o.put(s, b);
-      rx_replace(o, s + b, n - b, (const int32_t*)f.a[1].lit, f.a[2].lit, f.a[2].lit_len);
-      return true;
+      return rx_replace(o, s + b, n - b, (const int32_t*)f.a[1].lit, f.a[2].lit, f.a[2].lit_len);
     }
     case bsql::SF_CAST: {
       const StrSrc& a = f.a[0];

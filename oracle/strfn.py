@@ -70,7 +70,8 @@ def initcap(s):
 def regexp_replace(s, pattern, repl, start):
     """cudf::strings::replace_with_backrefs on the characters from `start`
     (the first `start` kept, LogicalProject.cpp:148-182): every match, left
-    to right (empty ones too, the character after an empty match kept), by
+    to right (empty ones too; after an empty match at p the next match may
+    start at p only if it is not empty, Python 3.7's re.sub rule), by
     `repl` with \\1..\\9 the groups (an unset or missing group is empty).
     Python's re with ASCII classes; '$' matches only at the end."""
     import re

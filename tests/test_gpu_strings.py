@@ -459,3 +459,43 @@ def test_regexp_replace_vs_oracle(bsql, gpu, n):
         p, ot, res = _run(bsql, [e], dev, [STRING], n)
         ores = _oracle(bsql, p, host, ot, n)
         _same(res[0], ores[0], e)
+
+
+def test_regexp_replace_loops_that_match_empty(bsql, gpu):
+    """Unbounded loops whose body can match nothing ('(a*)*', '(x?)*',
+    '(a|)*'), lazy and greedy, with captures: no iteration that consumed
+    nothing is repeated, and the captures of that last empty iteration are
+    kept (Python's rule), against the oracle (Python's re)."""
+    import oracle
+    from blazingsql_amd.interop import DeviceColumn
+    strs = ["baab", "axxb", "", "aaaa", "b", "xax", "ababcab", "aaxaab"]
+    valid = np.ones(len(strs), dtype=bool)
+    dev = [DeviceColumn.from_strings(strs, valid)]
+    host = [oracle.HostColumn(STRING, strings=strs, valid=valid)]
+    for pat, rep in [("(a*)*", "[\\1]"), ("(x?)*", "[\\1]"), ("(ab)*", "[\\1]"), ("(a|)*", "<\\1>"),
+                     ("(a*?)*", "[\\1]"), ("(a*)*?b", "[\\1]"), ("((a)|(x))*", "\\1\\2\\3|")]:
+        e = f"REGEXP_REPLACE($0, '{pat}', '{rep}')"
+        p, ot, res = _run(bsql, [e], dev, [STRING], len(strs))
+        ores = _oracle(bsql, p, host, ot, len(strs))
+        _same(res[0], ores[0], e)
+        assert res[0][1].all(), e
+
+
+def test_regexp_replace_deep_matches(bsql, gpu):
+    """A loop with an alternation inside leaves backtracking frames per
+    iteration: '(a|b)*c' over runs of 40 characters matches as the oracle
+    does; over a 300-character run the match needs more frames than a lane
+    holds, and that row's result is null instead of a wrong string."""
+    import oracle
+    from blazingsql_amd.interop import DeviceColumn
+    strs = ["ab" * 20 + "c", "x" + "ba" * 20 + "cab", "ab" * 150 + "c", "cc", "abab"]
+    valid = np.ones(len(strs), dtype=bool)
+    dev = [DeviceColumn.from_strings(strs, valid)]
+    host = [oracle.HostColumn(STRING, strings=strs, valid=valid)]
+    e = "REGEXP_REPLACE($0, '(a|b)*c', '[\\1]')"
+    p, ot, res = _run(bsql, [e], dev, [STRING], len(strs))
+    ores = _oracle(bsql, p, host, ot, len(strs))
+    vals, ok = res[0]
+    assert list(ok) == [True, True, False, True, True]
+    for i in (0, 1, 3, 4):
+        assert vals[i] == ores[0][0][i], (i, vals[i], ores[0][0][i])
