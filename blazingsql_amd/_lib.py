@@ -162,6 +162,10 @@ SIGNATURES = {
     "bsql_process_project": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_P), _P]),
     "bsql_evaluate_expressions": (_I, [_P, _P, _I, ctypes.POINTER(_P), _P]),
     "bsql_apply_boolean_filter": (_I, [_P, ctypes.POINTER(Column), ctypes.POINTER(_P), _P]),
+    "bsql_aggregation_input": (_I, [_P, ctypes.c_char_p, _I, ctypes.POINTER(_P), _P]),
+    "bsql_skip_data_filter": (_I, [ctypes.c_char_p, _P, _I, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "bsql_process_skipdata_for_table": (_I, [_P, _P, _I, ctypes.c_char_p, ctypes.POINTER(_P),
+                                             ctypes.POINTER(_I), _P]),
     "bsql_project_bypass": (_I, [ctypes.c_char_p, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "bsql_table_set_project_aliases": (_I, [_P, ctypes.c_char_p]),
     "bsql_table_fix_column_aliases": (_I, [_P, ctypes.c_char_p]),

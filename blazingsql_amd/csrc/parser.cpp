@@ -1265,6 +1265,24 @@ std::string rebuild(const Node* n) {
 }
 }  // namespace
 
+// modify_multi_column_count_expression (expression_utils.cpp:1467-1492):
+// the input of COUNT over several columns, "$a, $b, .." (the text between
+// the aggregation's parentheses), as the sum of their INTEGER casts, so that
+// a row is null when any of the columns is; one column stays as it is.
+std::string multi_column_count_expression(const std::string& e) {
+  const size_t first = e.find('$');
+  if (first == std::string::npos) return e;
+  std::string reduced = e.substr(first + 1);
+  reduced.erase(std::remove(reduced.begin(), reduced.end(), '$'), reduced.end());
+  const std::vector<std::string> idx = split_expression_list(reduced, true);
+  if (idx.size() <= 1) return e;
+  std::string out = "+(CAST($" + std::to_string(std::stoi(idx[0])) + "):INTEGER, CAST($" +
+                    std::to_string(std::stoi(idx[1])) + "):INTEGER)";
+  for (size_t i = 2; i < idx.size(); ++i)
+    out = "+(" + out + ", CAST($" + std::to_string(std::stoi(idx[i])) + "):INTEGER)";
+  return out;
+}
+
 std::string rewrite_expression(const std::string& e, int stage, const int32_t* types, int n) {
   switch (stage) {
     case 0: return fill_minus_op_with_zero(e);
@@ -1279,8 +1297,125 @@ std::string rewrite_expression(const std::string& e, int stage, const int32_t* t
       Parser parser(e);
       return rebuild(transform_custom(parser.parse()).get());
     }
+    case 9: return multi_column_count_expression(e);
     default: fail("unknown rewrite stage " + std::to_string(stage));
   }
+}
+
+// ------------------------------------------------------------ skip data
+//
+// ral::skip_data (SkipDataProcessor.cpp:23-232, utils.cpp:12-34): the filter
+// of a table scan rewritten over the table of row-group statistics, in which
+// projected column i has its minimum at column 2i and its maximum at 2i + 1.
+// Three post-order passes over the parsed filter, as the reference's
+// transformers: columns without statistics become NONE; operators other than
+// = > >= < <= + - AND OR become NONE, and a NONE (or unary) operand makes an
+// AND its other operand and anything else NONE; then each comparison is
+// taken over the ranges (= : AND(<=(a_min, b_max), >=(a_max, b_min)); < <= :
+// (a_min, b_max); > >= : (a_max, b_min)) and + - as the pair (a_min op
+// b_min, a_max op b_max), as the reference writes them.
+
+namespace {
+
+std::unique_ptr<Node> sd_operator(const std::string& v) {
+  auto n = std::make_unique<Node>();
+  n->type = NodeType::Operator;
+  n->value = v;
+  return n;
+}
+
+std::unique_ptr<Node> sd_clone(const Node* n) {
+  auto c = std::make_unique<Node>();
+  c->type = n->type;
+  c->value = n->value;
+  c->lit_type = n->lit_type;
+  c->var_index = n->var_index;
+  for (const auto& k : n->children) c->children.push_back(sd_clone(k.get()));
+  return c;
+}
+
+template <class F>
+std::unique_ptr<Node> sd_post_order(std::unique_ptr<Node> n, F& f) {
+  if (n->type == NodeType::Operator)
+    for (auto& c : n->children) c = sd_post_order(std::move(c), f);
+  return f(std::move(n));
+}
+
+bool sd_supported(const std::string& op) {
+  static const char* const ok[] = {"=", ">", ">=", "<", "<=", "+", "-", "AND", "OR"};
+  for (const char* o : ok)
+    if (op == o) return true;
+  return false;
+}
+
+// utils.cpp:22-34: NONE, or a unary operator
+bool sd_excluded(const Node& n) {
+  if (n.type != NodeType::Operator) return false;
+  if (n.value == "NONE") return true;
+  return is_unary_op(map_to_operator(n.value));
+}
+
+// the minimum (k = 0) or maximum (k = 1) side of an operand
+std::unique_ptr<Node> sd_side(const Node* n, int k) {
+  if (n->type == NodeType::Operator) {
+    if (n->value == "&&&" && n->children.size() == 2) return sd_clone(n->children[(size_t)k].get());
+    return sd_clone(n);
+  }
+  if (n->type == NodeType::Variable) {
+    auto v = sd_clone(n);
+    v->var_index = 2 * n->var_index + k;
+    v->value = "$" + std::to_string(v->var_index);
+    return v;
+  }
+  return sd_clone(n);
+}
+
+std::unique_ptr<Node> sd_binary(const std::string& op, std::unique_ptr<Node> a, std::unique_ptr<Node> b) {
+  auto n = sd_operator(op);
+  n->children.push_back(std::move(a));
+  n->children.push_back(std::move(b));
+  return n;
+}
+
+}  // namespace
+
+std::string skip_data_filter(const std::string& filter, const std::vector<bool>& has_stats) {
+  Parser parser(expand_if_logical_op(replace_calcite_regex(filter)));
+  std::unique_ptr<Node> root = parser.parse();
+  auto drop = [&](std::unique_ptr<Node> n) -> std::unique_ptr<Node> {
+    if (n->type == NodeType::Variable &&
+        (n->var_index < 0 || n->var_index >= (int)has_stats.size() || !has_stats[(size_t)n->var_index]))
+      return sd_operator("NONE");
+    return n;
+  };
+  root = sd_post_order(std::move(root), drop);
+  auto reduce = [&](std::unique_ptr<Node> n) -> std::unique_ptr<Node> {
+    if (n->type != NodeType::Operator) return n;
+    if (!sd_supported(n->value)) return sd_operator("NONE");
+    if (n->children.size() != 2) fail("skip data: operator " + n->value + " without two operands");
+    const bool l = sd_excluded(*n->children[0]), r = sd_excluded(*n->children[1]);
+    if (l && r) return sd_operator("NONE");
+    if (l) return n->value == "AND" ? std::move(n->children[1]) : sd_operator("NONE");
+    if (r) return n->value == "AND" ? std::move(n->children[0]) : sd_operator("NONE");
+    return n;
+  };
+  root = sd_post_order(std::move(root), reduce);
+  if (root->type == NodeType::Operator && root->value == "NONE") return "";
+  auto ranges = [&](std::unique_ptr<Node> n) -> std::unique_ptr<Node> {
+    if (n->type != NodeType::Operator || n->children.size() != 2) return n;
+    const Node* a = n->children[0].get();
+    const Node* b = n->children[1].get();
+    const std::string& op = n->value;
+    if (op == "=")
+      return sd_binary("AND", sd_binary("<=", sd_side(a, 0), sd_side(b, 1)), sd_binary(">=", sd_side(a, 1), sd_side(b, 0)));
+    if (op == "<" || op == "<=") return sd_binary(op, sd_side(a, 0), sd_side(b, 1));
+    if (op == ">" || op == ">=") return sd_binary(op, sd_side(a, 1), sd_side(b, 0));
+    if (op == "+" || op == "-")
+      return sd_binary("&&&", sd_binary(op, sd_side(a, 0), sd_side(b, 0)), sd_binary(op, sd_side(a, 1), sd_side(b, 1)));
+    return n;
+  };
+  root = sd_post_order(std::move(root), ranges);
+  return rebuild(root.get());
 }
 
 void set_current_timestamp(const std::string& ts) {

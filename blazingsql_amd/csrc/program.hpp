@@ -212,6 +212,11 @@ void set_current_timestamp(const std::string& ts);
 // Query-part text of the operators (parser.cpp).
 // `name=[[...]]` or `name=[...]` of a query part; "" when absent.
 std::string named_expression(const std::string& query_part, const std::string& name);
+// A table scan's filter over the row-group statistics table (min of
+// projected column i at 2i, max at 2i + 1); has_stats[i]: column i has
+// statistics. "" when no row group can be skipped.
+std::string skip_data_filter(const std::string& filter, const std::vector<bool>& has_stats);
+std::string multi_column_count_expression(const std::string& e);
 // The text between the first '(' and the last ')'.
 std::string query_part_body(const std::string& logical_plan);
 // Comma-separated list at bracket/paren depth 0 outside quotes, after the

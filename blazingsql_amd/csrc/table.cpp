@@ -10,6 +10,8 @@
 //   fix_column_aliases    engine/src/parser/expression_utils.cpp:512-535
 //   get_projections       engine/src/parser/expression_utils.cpp:549-580
 //                         engine/src/execution_kernels/BatchProcessing.cpp:30-46
+#include <algorithm>
+#include <cstring>
 #include <hip/hip_runtime.h>
 
 #include <memory>
@@ -487,6 +489,137 @@ int bsql_scan_projections(const char* query_part, int num_columns, int32_t* cols
   } catch (const std::exception& e) {
     return -set_error(BSQL_ERR_INVALID, e.what());
   }
+}
+
+int bsql_aggregation_input(const bsql_table* in, const char* expr, int kind, bsql_table** out, void* stream) {
+  BSQL_TRY({
+    if (!in || !expr || !out) fail("null argument");
+    *out = nullptr;
+    std::string e(expr);
+    // one column of `in`, borrowed
+    auto column = [&](int k) {
+      if (k < 0 || k >= (int)in->cols.size()) fail("aggregation input column " + std::to_string(k) + " out of range");
+      TablePtr t = new_table(stream);
+      t->num_rows = in->num_rows;
+      t->cols.emplace_back();
+      t->cols.back().view = in->cols[(size_t)k].view;
+      t->cols.back().name = in->cols[(size_t)k].name;
+      return t;
+    };
+    if (e.empty()) {
+      if (kind != BSQL_AGG_COUNT_ALL) fail("an aggregation without an input expression must be COUNT(*)");
+      *out = column(0).release();  // COUNT(*): the first column stands for the rows
+      return BSQL_OK;
+    }
+    const size_t dollars = (size_t)std::count(e.begin(), e.end(), '$');
+    if (dollars > 1 && kind == BSQL_AGG_COUNT_VALID) e = multi_column_count_expression(e);
+    // is_number (expression_utils.cpp:358-361) / get_index
+    // (CalciteExpressionParsing.cpp:48-52)
+    const bool number = !e.empty() && std::all_of(e.begin(), e.end(), [](char c) { return c >= '0' && c <= '9'; });
+    if (dollars == 1 || number) {
+      const std::string idx = e[0] == '$' ? e.substr(1) : e;
+      size_t used = 0;
+      const int k = std::stoi(idx, &used);
+      if (used != idx.size()) fail("aggregation input is not a column: " + e);
+      *out = column(k).release();
+      return BSQL_OK;
+    }
+    *out = evaluate(*in, {e}, {}, stream).release();
+    const_cast<bsql_table*>(in)->stream = stream;
+    return BSQL_OK;
+  })
+}
+
+int bsql_skip_data_filter(const char* filter, const uint8_t* has_stats, int n_cols, char* out, size_t cap,
+                          size_t* len) {
+  BSQL_TRY({
+    if (!filter || n_cols < 0 || (n_cols > 0 && !has_stats)) fail("invalid arguments");
+    std::vector<bool> st;
+    for (int i = 0; i < n_cols; ++i) st.push_back(has_stats[i] != 0);
+    const std::string r = skip_data_filter(filter, st);
+    if (len) *len = r.size();
+    if (out && cap > 0) {
+      const size_t n = std::min(cap - 1, r.size());
+      std::memcpy(out, r.data(), n);
+      out[n] = '\0';
+    }
+    return BSQL_OK;
+  })
+}
+
+int bsql_process_skipdata_for_table(const bsql_table* metadata, const char* const* names, int n_names,
+                                    const char* table_scan, bsql_table** out, int* read_all, void* stream) {
+  BSQL_TRY({
+    if (!metadata || !table_scan || !out || !read_all || n_names < 0 || (n_names > 0 && !names))
+      fail("invalid arguments");
+    *out = nullptr;
+    *read_all = 1;
+    if (metadata->cols.size() < 2) fail("the statistics table needs the file and row-group id columns");
+    const std::string scan(table_scan);
+    std::string filter = named_expression(scan, "condition");
+    if (filter.empty()) filter = named_expression(scan, "filters");
+    if (filter.empty()) return BSQL_OK;
+    // the projected columns (projects=[[..]], else all of them)
+    std::vector<int> proj;
+    const std::string projects = named_expression(scan, "projects");
+    if (projects.empty()) {
+      for (int i = 0; i < n_names; ++i) proj.push_back(i);
+    } else {
+      for (const std::string& p : expression_list(projects)) proj.push_back(std::stoi(p));
+    }
+    // their statistics: min and max columns, or placeholders the rewritten
+    // filter never reads
+    std::vector<bsql_column> cols;
+    std::vector<bool> has;
+    auto find = [&](const std::string& n) -> int {
+      for (size_t k = 0; k < metadata->cols.size(); ++k)
+        if (metadata->cols[k].name == n) return (int)k;
+      return -1;
+    };
+    for (int c : proj) {
+      if (c < 0 || c >= n_names || !names[c]) fail("projected column " + std::to_string(c) + " has no name");
+      const std::string suffix = std::to_string(c) + "_" + names[c];
+      const int lo = find("min_" + suffix), hi = find("max_" + suffix);
+      if (lo >= 0 && hi >= 0) {
+        has.push_back(true);
+        cols.push_back(metadata->cols[(size_t)lo].view);
+        cols.push_back(metadata->cols[(size_t)hi].view);
+      } else {
+        has.push_back(false);
+        bsql_column none = metadata->cols[0].view;
+        none.type_id = BSQL_INT8;
+        none.validity = nullptr;
+        cols.push_back(none);
+        cols.push_back(none);
+      }
+    }
+    const std::string expr = skip_data_filter(filter, has);
+    if (expr.empty()) return BSQL_OK;
+    TablePtr stats = new_table(stream);
+    stats->num_rows = metadata->num_rows;
+    for (const bsql_column& c : cols) {
+      stats->cols.emplace_back();
+      stats->cols.back().view = c;
+    }
+    TablePtr mask = evaluate(*stats, {expr}, {}, stream);
+    if (mask->cols.size() != 1 || mask->cols[0].view.type_id != BSQL_BOOL8)
+      fail("Expression in skip_data processing did not evaluate to a boolean mask");
+    // the file and row-group ids of the row groups to read
+    TablePtr ids = new_table(stream);
+    ids->num_rows = metadata->num_rows;
+    for (size_t k = metadata->cols.size() - 2; k < metadata->cols.size(); ++k) {
+      ids->cols.emplace_back();
+      ids->cols.back().view = metadata->cols[k].view;
+      ids->cols.back().name = metadata->cols[k].name;
+    }
+    bsql_table* kept = nullptr;
+    const int rc = bsql_apply_boolean_filter(ids.get(), &mask->cols[0].view, &kept, stream);
+    if (rc != BSQL_OK) return rc;
+    mask->stream = stream;
+    *out = kept;
+    *read_all = 0;
+    return BSQL_OK;
+  })
 }
 
 }  // extern "C"

@@ -137,7 +137,7 @@ class Table:
         c = L.Column()
         L.check(L.lib().bsql_table_column(self.handle, k, ctypes.byref(c)))
         for kept in self._keep:
-            if kept.to_c().data == c.data and kept.type_id == c.type_id:
+            if hasattr(kept, "to_c") and kept.to_c().data == c.data and kept.type_id == c.type_id:
                 return kept
 
         def buf(ptr, dtype):
@@ -229,6 +229,48 @@ def apply_boolean_filter(table, mask, stream=None):
     h = ctypes.c_void_p()
     rc = L.lib().bsql_apply_boolean_filter(table.handle, ctypes.byref(c), ctypes.byref(h), _stream_ptr(stream))
     return _out_table(rc, h)
+
+
+AGG_OTHER, AGG_COUNT_VALID, AGG_COUNT_ALL = 0, 1, 2
+
+
+def aggregation_input(table, expr, kind=AGG_OTHER, stream=None):
+    """The input column of one aggregation (GroupBy.cpp:226-249, 329-352):
+    `expr` is the text between the aggregation's parentheses; COUNT over
+    several columns (kind=AGG_COUNT_VALID) sums their INTEGER casts, one
+    column is borrowed, anything else is evaluated. A one-column Table."""
+    _ensure_allocator()
+    h = ctypes.c_void_p()
+    rc = L.lib().bsql_aggregation_input(table.handle, expr.encode(), kind, ctypes.byref(h), _stream_ptr(stream))
+    return _out_table(rc, h, keep=tuple(table._keep) + (table,))
+
+
+def skip_data_filter(filter_text, has_stats):
+    """The filter rewritten over the row-group statistics (column i: min at
+    2i, max at 2i + 1); "" when no row group can be skipped."""
+    st = (ctypes.c_uint8 * max(len(has_stats), 1))(*[1 if x else 0 for x in has_stats])
+    n = ctypes.c_size_t()
+    L.check(L.lib().bsql_skip_data_filter(filter_text.encode(), st, len(has_stats), None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    L.check(L.lib().bsql_skip_data_filter(filter_text.encode(), st, len(has_stats), buf, n.value + 1,
+                                          ctypes.byref(n)))
+    return buf.value.decode()
+
+
+def process_skipdata_for_table(metadata, names, table_scan, stream=None):
+    """ral::skip_data::process_skipdata_for_table (SkipDataProcessor.cpp:
+    242-346): (table of the file and row-group ids to read, False), or
+    (None, True) when no row group can be skipped."""
+    _ensure_allocator()
+    nm = (ctypes.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+    h = ctypes.c_void_p()
+    read_all = ctypes.c_int()
+    rc = L.lib().bsql_process_skipdata_for_table(metadata.handle, nm, len(names), table_scan.encode(),
+                                                 ctypes.byref(h), ctypes.byref(read_all), _stream_ptr(stream))
+    L.check(rc)
+    if read_all.value:
+        return None, True
+    return Table(h.value, (metadata,)), False
 
 
 def project_bypass(query_part, names):

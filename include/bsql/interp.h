@@ -499,6 +499,8 @@ int bsql_order_conjuncts(bsql_program* pred, const int32_t* in_types, int n_in, 
  *   7 6 then expand_if_logical_op    CalciteExpressionParsing.cpp:330-398
  *   8 parse_tree::build + transform_to_custom_op + rebuildExpression
  *                                    engine/src/parser/expression_tree.hpp:147-162, 181-258
+ *   9 modify_multi_column_count_expression (COUNT over several columns)
+ *                                    expression_utils.cpp:1467-1492
  * Writes at most cap-1 bytes and a NUL, and the full length to *len. */
 int bsql_expression_rewrite(const char* expr, int stage, const int32_t* types, int n_types, char* out,
                             size_t cap, size_t* len);

@@ -102,6 +102,44 @@ int bsql_evaluate_expressions(const bsql_table* in, const char* const* exprs, in
  * row drops the row. Synchronizes `stream` once. */
 int bsql_apply_boolean_filter(const bsql_table* in, const bsql_column* mask, bsql_table** out, void* stream);
 
+/* ---- the interpreter's other callers ------------------------------------- */
+
+/* The input column of one aggregation, as the aggregation kernels compute it
+ * (compute_aggregations_without_groupby / with_groupby, GroupBy.cpp:226-249
+ * and :329-352). expr is the text between the aggregation's parentheses in
+ * the LogicalAggregate plan. kind: BSQL_AGG_COUNT_VALID for COUNT(..),
+ * BSQL_AGG_COUNT_ALL for COUNT(*), 0 for the others. COUNT over several
+ * columns becomes the sum of their INTEGER casts
+ * (modify_multi_column_count_expression), so that its nulls are the rows
+ * where any of them is null; one column or a column number is that column,
+ * borrowed; COUNT(*) the first column; anything else the expression
+ * evaluated (evaluate_expressions). *out: a one-column table (its column
+ * borrows the input's when it is one of them). */
+enum { BSQL_AGG_OTHER = 0, BSQL_AGG_COUNT_VALID = 1, BSQL_AGG_COUNT_ALL = 2 };
+int bsql_aggregation_input(const bsql_table* in, const char* expr, int kind, bsql_table** out, void* stream);
+
+/* ral::skip_data::process_skipdata_for_table (SkipDataProcessor.cpp:242-346):
+ * which row groups a table scan must read. metadata: the row-group
+ * statistics table, column "min_<i>_<name>" / "max_<i>_<name>" for table
+ * column i (names[i]) where it has statistics, the last two columns the file
+ * and row-group ids. The scan's filter (condition=[..] or filters=[[..]]
+ * of table_scan, over its projects=[[..]] columns) is rewritten over the
+ * statistics (bsql_skip_data_filter) and evaluated by the interpreter; *out
+ * is the table of the last two metadata columns filtered by it. When no row
+ * group can be skipped (no filter, or nothing of it survives the rewrite),
+ * *out is NULL and *read_all is 1. */
+int bsql_process_skipdata_for_table(const bsql_table* metadata, const char* const* names, int n_names,
+                                    const char* table_scan, bsql_table** out, int* read_all, void* stream);
+
+/* The rewrite of a filter over the statistics table (projected column i:
+ * minimum at column 2i, maximum at 2i + 1; has_stats[i] != 0 when column i
+ * has statistics), the expression text process_skipdata_for_table
+ * evaluates: drop_value and apply_skip_data_rules (SkipDataProcessor.cpp:
+ * 232-246). An empty result: no row group can be skipped. Writes at most
+ * cap-1 bytes and a NUL, and the full length to *len. */
+int bsql_skip_data_filter(const char* filter, const uint8_t* has_stats, int n_cols, char* out, size_t cap,
+                          size_t* len);
+
 /* ---- query-part helpers of the operators ---------------------------------- */
 
 /* bypassingProject (expression_utils.cpp:1115-1150) for a table with these
