@@ -171,26 +171,26 @@ def device_validity(n, rate, g, dev):
     words = (n + 31) // 32
     bits = (torch.rand(words * 32, generator=g, device=dev) >= rate).to(torch.int64).view(words, 32)
     w = (bits << torch.arange(32, device=dev, dtype=torch.int64)).sum(1)
-    return w.to(torch.int64).bitwise_and(0xFFFFFFFF).to(torch.int32) if words else torch.zeros(1, dtype=torch.int32, device=dev)
+    w = torch.where(w >= 2 ** 31, w - 2 ** 32, w)      # the word's 32 bits as an int32
+    return w.to(torch.int32) if words else torch.zeros(1, dtype=torch.int32, device=dev)
 
 
-def run_batched(args):
-    """512 batches, rows log-uniform in [4K, 4M], 6 columns with null rates
-    U[0, 0.3], each with its own random 8-24-op program: one persistent
-    launch replayed from a captured HIP graph (autotuned shape), against one
-    bsql_interp_execute call per batch (the reference's launch per batch)."""
+def batched_items(n_batches, seed):
+    """The batched workload: n_batches batches, rows log-uniform in [4K, 4M],
+    BATCH_TYPES columns with null rates U[0, 0.3] (seeded torch RNG on the
+    device), one random 8-24-op program each, outputs allocated. Returns
+    (items, algorithmic bytes, rows x lowered ops, batch sizes)."""
     import numpy as np
     import torch
     import blazingsql_amd as bsql
-    from blazingsql_amd.interop import Batch, DeviceColumn, perform_interpreter_operation
+    from blazingsql_amd.interop import DeviceColumn
     dev = torch.device("cuda:0")
-    rng = np.random.default_rng(args.seed)
+    rng = np.random.default_rng(seed)
     g = torch.Generator(device=dev)
-    g.manual_seed(args.seed)
-    n_batches = args.batches
+    g.manual_seed(seed)
     sizes = np.exp(rng.uniform(np.log(4096), np.log(4194304), n_batches)).astype(np.int64)
     gen = ProgramGen(rng)
-    items, algo_bytes = [], 0.0
+    items, algo_bytes, row_ops = [], 0.0, 0
     widths = {3: 4, 4: 8, 10: 8, 12: 4}
     for b in range(n_batches):
         n = int(sizes[b])
@@ -210,9 +210,26 @@ def run_batched(args):
         ot = [prog.output_type(BATCH_TYPES, k) for k in range(prog.num_outputs)]
         outs = [DeviceColumn.empty(t, n) for t in ot]
         items.append((prog, cols, outs, n))
+        lowered = prog.plan_info(BATCH_TYPES)["n_ops"]
+        row_ops += n * lowered
         used = prog.used_columns()
         algo_bytes += n * (sum(widths[BATCH_TYPES[c]] + 1 / 8 for c in used) +
                            sum(np.dtype(bsql.interop.np_dtype(t)).itemsize + 1 / 8 for t in ot))
+    return items, algo_bytes, row_ops, sizes
+
+
+def run_batched(args):
+    """512 batches, rows log-uniform in [4K, 4M], 6 columns with null rates
+    U[0, 0.3], each with its own random 8-24-op program: one persistent
+    launch replayed from a captured HIP graph (autotuned shape), against one
+    bsql_interp_execute call per batch (the reference's launch per batch)."""
+    import numpy as np
+    import torch
+    import blazingsql_amd as bsql
+    from blazingsql_amd.interop import Batch, perform_interpreter_operation
+    dev = torch.device("cuda:0")
+    n_batches = args.batches
+    items, algo_bytes, row_ops, sizes = batched_items(n_batches, args.seed)
     total = int(sizes.sum())
     batch = Batch(items)
     chosen = batch.autotune()
@@ -258,7 +275,8 @@ def run_batched(args):
             raw_b = d.view(torch.uint8)[:nbytes]
             words_a, words_b = o.validity[:w], v[:w]
             # data is compared where valid: mask rows whose validity bit is 0
-            va = ((words_a.to(torch.int64).unsqueeze(1) >> torch.arange(32, device=dev)) & 1).view(-1)[:n].bool()
+            va = torch.remainder(words_a.to(torch.int64).unsqueeze(1) >> torch.arange(32, device=dev),
+                                 2).view(-1)[:n].bool()
             da = raw_a.view(n, width)[va]
             db = raw_b.view(n, width)[va]
             same = same and torch.equal(words_a, words_b) and torch.equal(da, db)
@@ -270,7 +288,10 @@ def run_batched(args):
         "vs_baseline": None, "dtype": "int64/f64", "data": "synthetic (seeded torch RNG on device)",
         "config": {"workload": f"{n_batches} batches, rows log-uniform in [4096, 4194304], 6 columns with "
                                f"null rates U[0, 0.3], one random 8-24-op program per batch",
-                   "total_rows": total, "rows_per_lane": chosen,
+                   "total_rows": total, "rows_per_lane": chosen, "block_size": batch.chosen[0],
+                   "register_file": "LDS" if batch.chosen[2] else "VGPR",
+                   "device_ops_x_rows": int(row_ops),
+                   "device_op_rows_per_s": row_ops / (ms_batched * 1e-3),
                    "plain_ms_per_step": ms_plain, "plain_rows_per_s": total / (ms_plain * 1e-3),
                    "speedup_vs_launch_per_batch": ms_plain / ms_batched, "same_as_plain": bool(same)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -78,6 +78,7 @@ class LaunchConfig(ctypes.Structure):
 
 
 LAUNCH_NO_RANGES, LAUNCH_LAZY_LOADS, LAUNCH_SPILL_WIDE, LAUNCH_FUSED, LAUNCH_GENERATED = 1, 2, 4, 8, 16
+LAUNCH_LDS_REGS = 32
 ROOT_INTERPRETED, ROOT_COLUMN, ROOT_LITERAL = 0, 1, 2
 SKETCH_DIM, FEATURE_DIM, MLP_HIDDEN = 19, 24, 64
 

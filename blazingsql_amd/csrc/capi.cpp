@@ -23,11 +23,11 @@ hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
                           hipStream_t stream);
 int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt, int block);
 int64_t string_tiles(int64_t capacity);
-int batch_tile_rows(int nreg, int rpt, int block);
-bool variant_fits(int nreg, int rpt, int block, int n_spill);
-int variant_rows(int nreg, int rpt, int block, int n_spill);
+int batch_tile_rows(int nreg, int rpt, int block, bool lf);
+bool variant_fits(int nreg, int rpt, int block, int n_spill, bool lf = false);
+int variant_rows(int nreg, int rpt, int block, int n_spill, bool lf = false);
 hipError_t launch_batches(const DevPlan* plans, const int64_t* tile_start, int n_batches, int64_t n_items,
-                          int nreg, int rpt, int block, int n_spill, hipStream_t stream);
+                          int nreg, int rpt, int block, int n_spill, bool lf, hipStream_t stream);
 hipError_t launch_string_output(const uint64_t* slices, const uint32_t* valid, int32_t* offsets,
                                 const uint8_t* src_chars, uint8_t* dst_chars, const int64_t* d_count,
                                 int64_t count, int64_t capacity, uint64_t* status, hipStream_t stream);
@@ -1268,26 +1268,41 @@ int bsql_filter_project(const bsql_program* pred, const bsql_program* proj, cons
 
 // ------------------------------------------------------------------ batches
 
-struct bsql_batch {
-  std::vector<DevPlan> plans;  // host copies (columns filled in, ops aimed at d_ops)
-  std::vector<int64_t> rows;
+// The batch's programs lowered one way: every plan (columns filled in, ops
+// aimed at d_ops) and the register file and spill slots they need.
+struct BatchForm {
+  std::vector<DevPlan> plans;
   int nreg = 1;
   int n_spill = 0;
+  DevPlan* d_plans = nullptr;
+  DevOp* d_ops = nullptr;
+};
+
+struct bsql_batch {
+  std::vector<int64_t> rows;
+  // forms[0]: each program with the register file it needs (16 registers
+  // for the whole batch if one program needs more than 8); forms[1]: at most
+  // 7 live values in registers and the rest in LDS spill slots, so that the
+  // 8-register file runs every program
+  BatchForm forms[2];
+  int form = 0;
   bsql_launch_config launch{0, 0, 0};
   int block = 256;
   int rpt = 0;
-  DevPlan* d_plans = nullptr;
-  DevOp* d_ops = nullptr;
+  bool lf = false;  // register file in LDS
   // one work map per launch shape (block, rows per lane), written once when
   // the shape is first used and never rewritten, so changing the shape never
   // touches a map that a queued launch may still read
   std::map<std::pair<int, int>, std::pair<int64_t*, int64_t>> maps;
   int64_t* d_tile_start = nullptr;
   int64_t n_items = 0;
+  const BatchForm& f() const { return forms[form]; }
   ~bsql_batch() {
     if (process_exiting()) return;
-    if (d_plans) (void)hipFree(d_plans);
-    if (d_ops) (void)hipFree(d_ops);
+    for (BatchForm& x : forms) {
+      if (x.d_plans) (void)hipFree(x.d_plans);
+      if (x.d_ops) (void)hipFree(x.d_ops);
+    }
     for (auto& m : maps) (void)hipFree(m.second.first);
   }
 };
@@ -1297,12 +1312,12 @@ namespace {
 // Selects the work map of a launch shape (prefix counts of tiles per batch),
 // building it on first use.
 hipError_t upload_work_map(bsql_batch* b, int rpt, int block, hipStream_t) {
-  const int tile = batch_tile_rows(b->nreg, rpt, block);
+  const int tile = batch_tile_rows(b->f().nreg, rpt, block, b->lf);
   if (tile <= 0) fail("no kernel variant for these programs");
   auto it = b->maps.find({block, rpt});
   if (it == b->maps.end()) {
-    std::vector<int64_t> start(b->plans.size() + 1, 0);
-    for (size_t i = 0; i < b->plans.size(); ++i) start[i + 1] = start[i] + (b->rows[i] + tile - 1) / tile;
+    std::vector<int64_t> start(b->rows.size() + 1, 0);
+    for (size_t i = 0; i < b->rows.size(); ++i) start[i + 1] = start[i] + (b->rows[i] + tile - 1) / tile;
     int64_t* d = nullptr;
     hipError_t e = hipMalloc((void**)&d, start.size() * sizeof(int64_t));
     if (e != hipSuccess) return e;
@@ -1324,8 +1339,8 @@ void batch_shape(const bsql_batch* b, int rpt_req, int block_req, int* rpt, int*
   const int rpts[4] = {rpt_req, 0, 1, 2};
   for (int bl : blocks)
     for (int r : rpts)
-      if (variant_fits(b->nreg, r, bl, b->n_spill)) {
-        *rpt = variant_rows(b->nreg, r, bl, b->n_spill);  // concrete: the work map's tile size
+      if (variant_fits(b->f().nreg, r, bl, b->f().n_spill, b->lf)) {
+        *rpt = variant_rows(b->f().nreg, r, bl, b->f().n_spill, b->lf);  // concrete: the work map's tile size
         *block = bl;
         return;
       }
@@ -1340,8 +1355,8 @@ int bsql_batch_prepare(const bsql_batch_desc* descs, int n, bsql_batch** out) {
   BSQL_TRY({
     if (!descs || n <= 0 || !out) fail("invalid arguments");
     auto b = std::make_unique<bsql_batch>();
-    std::vector<DevOp> all_ops;
-    std::vector<size_t> op_at;
+    std::vector<DevOp> all_ops[2];
+    std::vector<size_t> op_at[2];
     for (int i = 0; i < n; ++i) {
       const bsql_batch_desc& d = descs[i];
       bsql_program* prog = const_cast<bsql_program*>(d.prog);
@@ -1357,37 +1372,46 @@ int bsql_batch_prepare(const bsql_batch_desc* descs, int n, bsql_batch** out) {
         if (d.out[k].type_id == BSQL_STRING) fail("batched launches write fixed-width outputs only");
       hipError_t e = ensure_strings(*prog);
       if (e != hipSuccess) return device_error(e, "bsql_batch_prepare");
-      LowerOptions opt;
-      opt.mode = LowerOptions::EXECUTE;
-      opt.proj = prog;
-      opt.reg_budget = reg_budget(prog);
-      const Lowered& L = lowered_for(prog, opt, d.in, d.n_in, d.out, d.n_out);
-      DevPlan plan = L.plan;
-      plan.num_rows = d.num_rows;
-      for (size_t j = 0; j < L.in_cols.size(); ++j) plan.in[j] = to_dev_in(d.in[L.in_cols[j]]);
-      for (int k = 0; k < d.n_out; ++k) {
-        plan.out[k].data = d.out[k].data;
-        plan.out[k].valid = d.out[k].validity;
-        plan.out[k].type = d.out[k].type_id;
-      }
-      op_at.push_back(all_ops.size());
-      all_ops.insert(all_ops.end(), L.ops.begin(), L.ops.end());
-      b->plans.push_back(plan);
       b->rows.push_back(d.num_rows);
-      b->nreg = std::max(b->nreg, L.nreg);
-      b->n_spill = std::max(b->n_spill, L.n_spill);
+      for (int form = 0; form < 2; ++form) {
+        LowerOptions opt;
+        opt.mode = LowerOptions::EXECUTE;
+        opt.proj = prog;
+        opt.reg_budget = form == 0 ? reg_budget(prog) : 7;
+        const Lowered& L = lowered_for(prog, opt, d.in, d.n_in, d.out, d.n_out);
+        DevPlan plan = L.plan;
+        plan.num_rows = d.num_rows;
+        for (size_t j = 0; j < L.in_cols.size(); ++j) plan.in[j] = to_dev_in(d.in[L.in_cols[j]]);
+        for (int k = 0; k < d.n_out; ++k) {
+          plan.out[k].data = d.out[k].data;
+          plan.out[k].valid = d.out[k].validity;
+          plan.out[k].type = d.out[k].type_id;
+        }
+        BatchForm& F = b->forms[form];
+        op_at[form].push_back(all_ops[form].size());
+        all_ops[form].insert(all_ops[form].end(), L.ops.begin(), L.ops.end());
+        F.plans.push_back(plan);
+        F.nreg = std::max(F.nreg, L.nreg);
+        F.n_spill = std::max(F.n_spill, L.n_spill);
+      }
     }
-    // one register file size for all: a spilling program fixes it at 16
-    if (b->n_spill > 0) b->nreg = BSQL_DEV_MAX_REG;
-    hipError_t e = hipMalloc((void**)&b->d_ops, all_ops.size() * sizeof(DevOp));
-    for (size_t i = 0; i < b->plans.size() && e == hipSuccess; ++i) {
-      b->plans[i].ops = b->d_ops + op_at[i];
-      b->plans[i].n_spill = b->n_spill;
+    // one register file size per form: a program that spills even with 15
+    // registers fixes the first form at 16
+    if (b->forms[0].n_spill > 0) b->forms[0].nreg = BSQL_DEV_MAX_REG;
+    hipError_t e = hipSuccess;
+    for (int form = 0; form < 2 && e == hipSuccess; ++form) {
+      BatchForm& F = b->forms[form];
+      e = hipMalloc((void**)&F.d_ops, all_ops[form].size() * sizeof(DevOp));
+      for (size_t i = 0; i < F.plans.size() && e == hipSuccess; ++i) {
+        F.plans[i].ops = F.d_ops + op_at[form][i];
+        F.plans[i].n_spill = F.n_spill;
+      }
+      if (e == hipSuccess)
+        e = hipMemcpy(F.d_ops, all_ops[form].data(), all_ops[form].size() * sizeof(DevOp), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMalloc((void**)&F.d_plans, F.plans.size() * sizeof(DevPlan));
+      if (e == hipSuccess)
+        e = hipMemcpy(F.d_plans, F.plans.data(), F.plans.size() * sizeof(DevPlan), hipMemcpyHostToDevice);
     }
-    if (e == hipSuccess) e = hipMemcpy(b->d_ops, all_ops.data(), all_ops.size() * sizeof(DevOp), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc((void**)&b->d_plans, b->plans.size() * sizeof(DevPlan));
-    if (e == hipSuccess)
-      e = hipMemcpy(b->d_plans, b->plans.data(), b->plans.size() * sizeof(DevPlan), hipMemcpyHostToDevice);
     int rpt = 0, block = 256;
     batch_shape(b.get(), 0, 256, &rpt, &block);
     if (e == hipSuccess) e = upload_work_map(b.get(), rpt, block, nullptr);
@@ -1400,8 +1424,9 @@ int bsql_batch_prepare(const bsql_batch_desc* descs, int n, bsql_batch** out) {
 int bsql_batch_launch(const bsql_batch* b, void* stream) {
   BSQL_TRY({
     if (!b) fail("null batch");
-    hipError_t e = launch_batches(b->d_plans, b->d_tile_start, (int)b->plans.size(), b->n_items, b->nreg,
-                                  b->rpt, b->block, b->n_spill, (hipStream_t)stream);
+    const BatchForm& F = b->f();
+    hipError_t e = launch_batches(F.d_plans, b->d_tile_start, (int)F.plans.size(), b->n_items, F.nreg, b->rpt,
+                                  b->block, F.n_spill, b->lf, (hipStream_t)stream);
     if (e != hipSuccess) return device_error(e, "bsql_batch_launch");
     return BSQL_OK;
   })
@@ -1414,8 +1439,20 @@ int bsql_batch_set_launch(bsql_batch* b, const bsql_launch_config* cfg) {
     if (r != 0 && r != 1 && r != 2 && r != 4) fail("rows_per_thread must be 0, 1, 2 or 4");
     if (cfg->block_size != 0 && cfg->block_size != 64 && cfg->block_size != 256)
       fail("block_size must be 0, 64 or 256");
+    if (cfg->flags & ~(BSQL_LAUNCH_LDS_REGS | BSQL_LAUNCH_SPILL_WIDE))
+      fail("batched launches take only BSQL_LAUNCH_LDS_REGS and BSQL_LAUNCH_SPILL_WIDE");
+    const bool lf_was = b->lf;
+    const int form_was = b->form;
+    b->lf = (cfg->flags & BSQL_LAUNCH_LDS_REGS) != 0;
+    b->form = (cfg->flags & BSQL_LAUNCH_SPILL_WIDE) ? 1 : 0;
     int rpt = 0, block = 256;
-    batch_shape(b, r, cfg->block_size, &rpt, &block);
+    try {
+      batch_shape(b, r, cfg->block_size, &rpt, &block);
+    } catch (...) {
+      b->lf = lf_was;
+      b->form = form_was;
+      throw;
+    }
     b->launch = *cfg;
     hipError_t e = upload_work_map(b, rpt, block, nullptr);
     if (e != hipSuccess) return device_error(e, "bsql_batch_set_launch");
@@ -1472,23 +1509,34 @@ int bsql_batch_autotune(bsql_batch* b, void* stream, bsql_launch_config* chosen)
   BSQL_TRY({
     if (!b) fail("null batch");
     float best = 1e30f;
-    int best_r = b->rpt, best_b = b->block;
-    for (const auto& sh : kShapes) {
-      const int bl = sh[0], r = sh[1];
-      if (!variant_fits(b->nreg, r, bl, b->n_spill)) continue;  // no variant of this shape
-      hipError_t e = upload_work_map(b, r, bl, (hipStream_t)stream);
-      float ms = 0.f;
-      if (e == hipSuccess)
-        e = time_launches((hipStream_t)stream, [&] {
-          return launch_batches(b->d_plans, b->d_tile_start, (int)b->plans.size(), b->n_items, b->nreg, r,
-                                bl, b->n_spill, (hipStream_t)stream);
-        }, &ms);
-      if (e != hipSuccess) return device_error(e, "bsql_batch_autotune");
-      if (ms < best) { best = ms; best_r = r; best_b = bl; }
-    }
+    int best_r = b->rpt, best_b = b->block, best_form = b->form;
+    bool best_lf = b->lf;
+    // the second form only where it differs (some program needs more than 8 registers)
+    const int n_forms = b->forms[0].nreg > 8 ? 2 : 1;
+    for (int form = 0; form < n_forms; ++form)
+      for (const bool lf : {false, true})
+        for (const auto& sh : kShapes) {
+          const int bl = sh[0], r = sh[1];
+          const BatchForm& F = b->forms[form];
+          if (!variant_fits(F.nreg, r, bl, F.n_spill, lf)) continue;  // no variant of this shape
+          b->lf = lf;
+          b->form = form;
+          hipError_t e = upload_work_map(b, r, bl, (hipStream_t)stream);
+          float ms = 0.f;
+          if (e == hipSuccess)
+            e = time_launches((hipStream_t)stream, [&] {
+              return launch_batches(F.d_plans, b->d_tile_start, (int)F.plans.size(), b->n_items, F.nreg, r, bl,
+                                    F.n_spill, lf, (hipStream_t)stream);
+            }, &ms);
+          if (e != hipSuccess) return device_error(e, "bsql_batch_autotune");
+          if (ms < best) { best = ms; best_r = r; best_b = bl; best_lf = lf; best_form = form; }
+        }
+    b->lf = best_lf;
+    b->form = best_form;
     hipError_t e = upload_work_map(b, best_r, best_b, (hipStream_t)stream);
     if (e != hipSuccess) return device_error(e, "bsql_batch_autotune");
-    b->launch = bsql_launch_config{best_b, best_r, 0};
+    b->launch = bsql_launch_config{best_b, best_r, (best_lf ? BSQL_LAUNCH_LDS_REGS : 0) |
+                                                       (best_form ? BSQL_LAUNCH_SPILL_WIDE : 0)};
     if (chosen) *chosen = b->launch;
     return BSQL_OK;
   })

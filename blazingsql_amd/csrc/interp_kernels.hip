@@ -183,6 +183,12 @@ hipError_t fill_words(uint64_t* p, int64_t n, uint64_t v, hipStream_t stream) {
 // more LDS than a 256-lane workgroup can have.
 #define BSQL_VARIANTS(X) \
   X(8, 2, 256) X(8, 4, 256) X(16, 1, 256) X(16, 2, 256) X(16, 1, 64) X(16, 2, 64) X(8, 4, 64)
+// the batched interpreter's variants with the register file in LDS (block *
+// NREG * R * 8 bytes of it per workgroup)
+#define BSQL_LF_VARIANTS(X) \
+  X(8, 2, 256) X(16, 1, 256) X(8, 2, 64) X(8, 4, 64) X(16, 1, 64) X(16, 2, 64)
+
+static int64_t regfile_lds_bytes(int nreg, int block, int rows) { return (int64_t)block * nreg * rows * 8; }
 
 // LDS a workgroup may have for spill slots (the static arrays of the
 // selection kernels take < 1 KB of the 64 KB).
@@ -191,7 +197,7 @@ static int64_t max_spill_lds() { return 60 * 1024; }
 // Picks the variant: the smallest register file that holds the program, and
 // the rows per lane requested (0 = the default for that register file), for
 // workgroup size `block`; false if no variant holds it (or its spill slots).
-static bool pick(int nreg, int rpt_req, int block, int n_spill, int* NR, int* RR) {
+static bool pick(int nreg, int rpt_req, int block, int n_spill, int* NR, int* RR, bool lf = false) {
   struct V { int n, r; };
   const V vs[] = {{8, 4}, {16, 2}, {8, 2}, {16, 1}};
   for (const V& v : vs) {
@@ -199,28 +205,33 @@ static bool pick(int nreg, int rpt_req, int block, int n_spill, int* NR, int* RR
     if (rpt_req > 0 && v.r != rpt_req) continue;
     bool compiled = false;
 #define X(N, RPT, BL) if (N == v.n && RPT == v.r && BL == block) compiled = true;
-    BSQL_VARIANTS(X)
+    if (lf) {
+      BSQL_LF_VARIANTS(X)
+    } else {
+      BSQL_VARIANTS(X)
+    }
 #undef X
     if (!compiled) continue;
-    if (spill_lds_bytes(n_spill, block, v.r) > max_spill_lds()) continue;
+    const int64_t lds = spill_lds_bytes(n_spill, block, v.r) + (lf ? regfile_lds_bytes(v.n, block, v.r) : 0);
+    if (lds > max_spill_lds()) continue;
     *NR = v.n;
     *RR = v.r;
     return true;
   }
-  if (rpt_req > 0) return pick(nreg, 0, block, n_spill, NR, RR);
+  if (rpt_req > 0) return pick(nreg, 0, block, n_spill, NR, RR, lf);
   return false;
 }
 
-bool variant_fits(int nreg, int rpt, int block, int n_spill) {
+bool variant_fits(int nreg, int rpt, int block, int n_spill, bool lf) {
   int NR, RR;
-  if (!pick(nreg, rpt, block, n_spill, &NR, &RR)) return false;
+  if (!pick(nreg, rpt, block, n_spill, &NR, &RR, lf)) return false;
   return rpt == 0 || RR == rpt;
 }
 
 // Rows per lane of the variant a launch with this shape uses (0: none).
-int variant_rows(int nreg, int rpt, int block, int n_spill) {
+int variant_rows(int nreg, int rpt, int block, int n_spill, bool lf) {
   int NR, RR;
-  return pick(nreg, rpt, block, n_spill, &NR, &RR) ? RR : 0;
+  return pick(nreg, rpt, block, n_spill, &NR, &RR, lf) ? RR : 0;
 }
 
 template <int NREG, int R, int B>
@@ -484,25 +495,35 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
 // Batched launch: `plans` (device) holds n_batches plans, `tile_start`
 // (device) n_batches + 1 prefix counts of tiles of block*rpt rows. No
 // allocation and no copy: the launch can be captured in a hipGraph.
-int batch_tile_rows(int nreg, int rpt, int block) {
+int batch_tile_rows(int nreg, int rpt, int block, bool lf) {
   int NR, RR;
-  if (!pick(nreg, rpt, block, 0, &NR, &RR)) return -1;
+  if (!pick(nreg, rpt, block, 0, &NR, &RR, lf)) return -1;
   return block * RR;
 }
 
+// lf: the variant with the register file in LDS
 hipError_t launch_batches(const DevPlan* plans, const int64_t* tile_start, int n_batches, int64_t n_items,
-                          int nreg, int rpt, int block, int n_spill, hipStream_t stream) {
+                          int nreg, int rpt, int block, int n_spill, bool lf, hipStream_t stream) {
   int NR, RR;
-  if (!pick(nreg, rpt, block, n_spill, &NR, &RR)) return hipErrorInvalidValue;
+  if (!pick(nreg, rpt, block, n_spill, &NR, &RR, lf)) return hipErrorInvalidValue;
   const int64_t grid = persistent_grid(n_items);
   if (grid == 0) return hipSuccess;
 #define X(N, RPT, BL)                                                                                  \
-  if (NR == N && RR == RPT && block == BL) {                                                           \
+  if (!lf && NR == N && RR == RPT && block == BL) {                                                    \
     hipLaunchKernelGGL((vm_batches<N, RPT, BL>), dim3((unsigned)grid), dim3(BL),                       \
                        (unsigned)spill_lds_bytes(n_spill, BL, RPT), stream, plans, tile_start, n_batches); \
     return hipGetLastError();                                                                          \
   }
   BSQL_VARIANTS(X)
+#undef X
+#define X(N, RPT, BL)                                                                                  \
+  if (lf && NR == N && RR == RPT && block == BL) {                                                     \
+    hipLaunchKernelGGL((vm_batches<N, RPT, BL, true>), dim3((unsigned)grid), dim3(BL),                 \
+                       (unsigned)(spill_lds_bytes(n_spill, BL, RPT) + regfile_lds_bytes(N, BL, RPT)), stream, \
+                       plans, tile_start, n_batches);                                                  \
+    return hipGetLastError();                                                                          \
+  }
+  BSQL_LF_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;
 }

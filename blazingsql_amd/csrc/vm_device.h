@@ -85,6 +85,23 @@ __device__ __forceinline__ int64_t floor_div(int64_t a, int64_t b) {
   return q;
 }
 
+// YEAR, MONTH or DAY of a day number (days since 1970-01-01), Hinnant's
+// civil_from_days; inline, so that EXTRACT over a DATE column costs no call
+__device__ __forceinline__ int64_t extract_ymd(int op, int64_t days) {
+  const int64_t z = days + 719468;
+  const int64_t era = (z >= 0 ? z : z - 146096) / 146097;
+  const unsigned doe = (unsigned)(z - era * 146097);
+  const unsigned yoe = (doe - doe / 1460 + doe / 36524 - doe / 146096) / 365;
+  const int64_t yy = (int64_t)yoe + era * 400;
+  const unsigned doy = doe - (365 * yoe + yoe / 4 - yoe / 100);
+  const unsigned mp = (5 * doy + 2) / 153;
+  const unsigned d = doy - (153 * mp + 2) / 5 + 1;
+  const unsigned m = mp < 10 ? mp + 3 : mp - 9;
+  if (op == BSQL_OP_DAY) return (int16_t)d;
+  if (op == BSQL_OP_MONTH) return (int16_t)m;
+  return (int16_t)(yy + (m <= 2));
+}
+
 // EXTRACT (interpreter_ops.cuh:69-99) with Hinnant's civil_from_days
 static __device__ int64_t extract(int op, int t, int64_t ticks) {
   const int64_t tpd = ticks_per_day(t);
@@ -102,27 +119,22 @@ static __device__ int64_t extract(int op, int t, int64_t ticks) {
     const int64_t wd = days >= -4 ? (days + 4) % 7 : (days + 5) % 7 + 6;
     return wd == 0 ? 7 : wd;
   }
-  int64_t z = days + 719468;
-  const int64_t era = (z >= 0 ? z : z - 146096) / 146097;
-  const unsigned doe = (unsigned)(z - era * 146097);
-  const unsigned yoe = (doe - doe / 1460 + doe / 36524 - doe / 146096) / 365;
-  const int64_t yy = (int64_t)yoe + era * 400;
-  const unsigned doy = doe - (365 * yoe + yoe / 4 - yoe / 100);
-  const unsigned mp = (5 * doy + 2) / 153;
-  const unsigned d = doy - (153 * mp + 2) / 5 + 1;
-  const unsigned m = mp < 10 ? mp + 3 : mp - 9;
-  if (op == BSQL_OP_DAY) return (int16_t)d;
-  if (op == BSQL_OP_MONTH) return (int16_t)m;
-  return (int16_t)(yy + (m <= 2));
+  return extract_ymd(op, days);
 }
 
+// Integer division and modulo with the interpreter's edge cases; operands
+// that both fit in 32 bits (the usual case) take the 32-bit divide, whose
+// truncating quotient and remainder equal the 64-bit ones there.
+__device__ __forceinline__ bool fits_i32(int64_t v) { return v == (int64_t)(int32_t)v; }
 __device__ __forceinline__ int64_t idiv(int64_t a, int64_t b) {
   if (b == 0) return 0;
   if (b == -1) return (int64_t)(0ULL - (uint64_t)a);
+  if (fits_i32(a) && fits_i32(b)) return (int64_t)((int32_t)a / (int32_t)b);
   return a / b;
 }
 __device__ __forceinline__ int64_t imod(int64_t a, int64_t b) {
   if (b == 0 || b == -1) return 0;
+  if (fits_i32(a) && fits_i32(b)) return (int64_t)((int32_t)a % (int32_t)b);
   return a % b;
 }
 
@@ -732,8 +744,14 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
     }
     case D_MOD_FF: case D_POW: case D_ROUND:
       ROWS(compute_heavy(op.code, op.aux, op.aux2, op.imm, AF, BF, a, rs.row[k]), va & vb)
+    case D_EXTRACT:
+      if (op.aux2 == BSQL_TIMESTAMP_DAYS &&
+          (op.aux == BSQL_OP_YEAR || op.aux == BSQL_OP_MONTH || op.aux == BSQL_OP_DAY)) {
+        ROWS(extract_ymd(op.aux, (int64_t)a), va)
+      }
+      ROWS(compute_heavy(op.code, op.aux, op.aux2, op.imm, AF, BF, a, rs.row[k]), va)
     case D_SIN: case D_COS: case D_ASIN: case D_ACOS: case D_TAN: case D_COTAN: case D_ATAN:
-    case D_LN: case D_LOG: case D_EXTRACT:
+    case D_LN: case D_LOG:
       ROWS(compute_heavy(op.code, op.aux, op.aux2, op.imm, AF, BF, a, rs.row[k]), va)
     case D_RAND:
       ROWS(compute_heavy(op.code, op.aux, op.aux2, op.imm, 0.0, 0.0, 0, rs.pos0 + (int64_t)k * rs.pstride), 1u)
@@ -1159,6 +1177,52 @@ struct LoadTemps {
   uint32_t v0, v1, v2, v3;
 };
 
+// The register file in VGPRs: f[k][i] = register i of the lane's row k.
+// Constant register numbers (generated kernels) make every access a plain
+// register; a dynamic number costs indexed moves and, on a write, a copy of
+// the whole vector.
+template <int NREG, int R>
+struct VRegs {
+  static constexpr bool kLds = false;
+  typename RegVecT<NREG>::type f[R];
+  __device__ __forceinline__ void tile_start() {
+#pragma unroll
+    for (int k = 0; k < R; ++k) f[k] = (typename RegVecT<NREG>::type)(0);
+  }
+  __device__ __forceinline__ uint64_t at(int k, int i) const { return f[k][i]; }
+  __device__ __forceinline__ void get(int i, uint64_t (&v)[R]) const {
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = f[k][i];
+  }
+  __device__ __forceinline__ void put(int i, const uint64_t (&v)[R]) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) f[k][i] = v[k];
+  }
+};
+
+// The register file in LDS, for plans read at run time: register i of the
+// lane's R rows is the R consecutive slots at p + i*64*R, so a dynamic
+// register number costs one address add and the access is one LDS load or
+// store of the lane's R rows (each lane touches only its own slots: no
+// barrier). The wave's file is NREG*64*R slots.
+template <int NREG, int R>
+struct LRegs {
+  static constexpr bool kLds = true;
+  LDS uint64_t* p;
+  __device__ __forceinline__ void tile_start() {}
+  __device__ __forceinline__ uint64_t at(int k, int i) const { return p[i * (64 * R) + k]; }
+  __device__ __forceinline__ void get(int i, uint64_t (&v)[R]) const {
+    const LDS uint64_t* q = p + i * (64 * R);
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = q[k];
+  }
+  __device__ __forceinline__ void put(int i, const uint64_t (&v)[R]) {
+    LDS uint64_t* q = p + i * (64 * R);
+#pragma unroll
+    for (int k = 0; k < R; ++k) q[k] = v[k];
+  }
+};
+
 // Executes one non-store op. Returns true if it was a KILL that left the
 // wave without a selected row.
 // Spill slot q of the calling wave: 64 lanes x R rows of data (row k of lane
@@ -1168,20 +1232,121 @@ __device__ __forceinline__ LDS uint64_t* spill_slot(LDS uint64_t* sp, int64_t q)
   return sp + q * (64 * R + 8);
 }
 
-template <int NREG, int R, bool NT = true, class P = PlanSrc>
-__device__ __forceinline__ bool vm_step(const DevPlan& plan, const DevOp& op,
-                                        typename RegVecT<NREG>::type (&f)[R], uint64_t& vb,
+// One op with the register file in LDS. Operands are read only by the ops
+// that use them, and ops without a result write nothing. Column loads are
+// not ops here: the tile loop runs a LOADN and its COMMITs as one step
+// (lds_loads).
+template <int NREG, int R, class P, class RF>
+__device__ __forceinline__ bool vm_step_lds(const DevPlan& plan, const DevOp& op, RF& f, uint64_t& vb,
+                                            const Rows<R>& rs, uint32_t& alive, LDS uint64_t* sp) {
+  constexpr uint32_t RMASK = (1u << R) - 1u;
+  const uint8_t code = op.code;
+  uint64_t D[R];
+  uint32_t VD = 0;
+  const int dst = op.dst == 0xff ? NREG - 1 : (op.dst & (NREG - 1));
+  if (code == D_LOADS) {
+    const DevInCol c = P::in(plan, op.a);
+    const GLOBAL int32_t* off = gp<int32_t>(c.offsets);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      D[k] = 0;
+      if ((rs.exist >> k) & 1u) {
+        const int32_t s0 = off[rs.row[k]], s1 = off[rs.row[k] + 1];
+        D[k] = slice_pack((uint32_t)s0, (uint32_t)(s1 - s0));
+      }
+    }
+    VD = rs.contig ? load_valid<R>(c, rs.row0, rs.nrows) : gather_valid<R>(c, rs);
+  } else if (code == D_FILL) {
+    const int lane = threadIdx.x & 63;
+    const LDS uint64_t* slot = spill_slot<R>(sp, op.imm);
+#pragma unroll
+    for (int k = 0; k < R; ++k) D[k] = slot[k * 64 + lane];
+    VD = ((const LDS uint8_t*)(slot + 64 * R))[lane];
+  } else if (code == D_LOADN || code == D_COMMIT) {
+    return false;  // run by lds_loads
+  } else {
+    uint64_t A[R], B[R];
+    const int ia = op.a & (NREG - 1), ib = op.b & (NREG - 1);
+    uint32_t VA, VB;
+    if (op.flags & F_A_IMM) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) A[k] = (uint64_t)op.imm;
+      VA = RMASK;
+    } else {
+      f.get(ia, A);
+      VA = (uint32_t)(vb >> (ia * R)) & RMASK;
+    }
+    if (op.flags & F_B_IMM) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) B[k] = (uint64_t)op.imm;
+      VB = RMASK;
+    } else {
+      f.get(ib, B);
+      VB = (uint32_t)(vb >> (ib * R)) & RMASK;
+    }
+    if (op.flags & F_A_NULL) VA = 0;
+    if (op.flags & F_B_NULL) VB = 0;
+    if (code == D_SPILL) {
+      const int lane = threadIdx.x & 63;
+      LDS uint64_t* slot = spill_slot<R>(sp, op.imm);
+#pragma unroll
+      for (int k = 0; k < R; ++k) slot[k * 64 + lane] = A[k];
+      ((LDS uint8_t*)(slot + 64 * R))[lane] = (uint8_t)VA;
+      return false;
+    }
+    if (code == D_KILL || code == D_SEL) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const bool t = code == D_KILL ? truthy(A[k], FA) : (A[k] != 0);
+        if (!(((VA >> k) & 1u) && t)) alive &= ~(1u << k);
+      }
+      return code == D_KILL && __ballot(alive != 0) == 0;
+    }
+    compute_rows<R>(plan, op, A, B, VA, VB, D, VD, rs);
+  }
+  f.put(dst, D);
+  vb = (vb & ~((uint64_t)RMASK << (dst * R))) | ((uint64_t)(VD & RMASK) << (dst * R));
+  return false;
+}
+
+// A LOADN and the COMMITs that follow it (ops q+1 .. q+n) with the register
+// file in LDS: the n columns' loads are issued together, then each lands in
+// its COMMIT's register.
+template <int NREG, int R, bool NT, class P, class RF>
+__device__ __forceinline__ void lds_loads(const DevPlan& plan, const DevOp& op, int q, RF& f, uint64_t& vb,
+                                          const Rows<R>& rs) {
+  constexpr uint32_t RMASK = (1u << R) - 1u;
+  const int n = op.aux;
+  uint64_t v[4][R];
+  uint32_t m[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (j >= n) break;
+    const int col = (int)((op.imm >> (8 * j)) & 0xff);
+    if (P::nt(col, NT)) col_load<R, true>(P::in(plan, col), rs, v[j], m[j]);
+    else col_load<R, false>(P::in(plan, col), rs, v[j], m[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (j >= n) break;
+    const DevOp c = load_op(plan, q + 1 + j);
+    const int dst = c.dst & (NREG - 1);
+    f.put(dst, v[j]);
+    vb = (vb & ~((uint64_t)RMASK << (dst * R))) | ((uint64_t)(m[j] & RMASK) << (dst * R));
+  }
+}
+
+template <int NREG, int R, bool NT = true, class P = PlanSrc, class RF>
+__device__ __forceinline__ bool vm_step(const DevPlan& plan, const DevOp& op, RF& f, uint64_t& vb,
                                         LoadTemps<R>& T, const Rows<R>& rs, uint32_t& alive,
                                         LDS uint64_t* sp) {
+  if constexpr (RF::kLds) return vm_step_lds<NREG, R, P>(plan, op, f, vb, rs, alive, sp);
   constexpr uint32_t RMASK = (1u << R) - 1u;
   const uint8_t code = op.code;
   uint64_t A[R], B[R], D[R];
   const int ia = op.a & (NREG - 1), ib = op.b & (NREG - 1);
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    A[k] = f[k][ia];
-    B[k] = f[k][ib];
-  }
+  f.get(ia, A);
+  f.get(ib, B);
   uint32_t VA = (uint32_t)(vb >> (ia * R)) & RMASK, VB = (uint32_t)(vb >> (ib * R)) & RMASK, VD = 0;
   if (op.flags & F_A_IMM) {
 #pragma unroll
@@ -1264,8 +1429,7 @@ __device__ __forceinline__ bool vm_step(const DevPlan& plan, const DevOp& op,
     compute_rows<R>(plan, op, A, B, VA, VB, D, VD, rs);
   }
   // result write: the only write of the register file
-#pragma unroll
-  for (int k = 0; k < R; ++k) f[k][dst] = D[k];
+  f.put(dst, D);
   vb = (vb & ~((uint64_t)RMASK << (dst * R))) | ((uint64_t)(VD & RMASK) << (dst * R));
   return dead;
 }
@@ -1322,9 +1486,8 @@ __device__ __forceinline__ void store_out_strided(const DevOutCol& c, int64_t po
 // The register file is local to the tile: every register is written before
 // it is read, so nothing carries over from one tile to the next. `sp` is the
 // wave's spill area (plan.n_spill slots).
-template <int NREG, int R, bool GATHER, class P>
-__device__ __forceinline__ void tile_op(const DevPlan& plan, const DevOp& op, typename RegVecT<NREG>::type (&f)[R],
-                                        uint64_t& vb, LoadTemps<R>& T, const Rows<R>& rs, uint32_t& alive,
+template <int NREG, int R, bool GATHER, class P, class RF>
+__device__ __forceinline__ void tile_op(const DevPlan& plan, const DevOp& op, RF& f, uint64_t& vb, LoadTemps<R>& T, const Rows<R>& rs, uint32_t& alive,
                                         LDS uint64_t* sp, int n, int lane) {
   if (op.code != D_STORE) {
     vm_step<NREG, R, true, P>(plan, op, f, vb, T, rs, alive, sp);
@@ -1332,21 +1495,17 @@ __device__ __forceinline__ void tile_op(const DevPlan& plan, const DevOp& op, ty
   }
   const int ia = op.a & (NREG - 1);
   uint64_t A[R];
-#pragma unroll
-  for (int k = 0; k < R; ++k) A[k] = f[k][ia];
+  f.get(ia, A);
   const uint32_t VA = (uint32_t)(vb >> (ia * R));
   if (GATHER) store_out_strided<R>(P::out(plan, op.aux), rs.pos0, rs.exist, A, VA & rs.exist, lane);
   else store_out<R>(P::out(plan, op.aux), rs.pos0, n, A, VA & rs.exist, lane);
 }
 
-template <int NREG, int R, bool GATHER, int B, class P = PlanSrc>
-__device__ __forceinline__ void vm_tile(const DevPlan& plan, int64_t tile, int64_t count,
+template <int NREG, int R, bool GATHER, int B, class P = PlanSrc, class RF>
+__device__ __forceinline__ void vm_tile(const DevPlan& plan, RF& f, int64_t tile, int64_t count,
                                         const uint32_t* __restrict__ idx, int lane, LDS uint64_t* sp) {
-  typedef typename RegVecT<NREG>::type RegVec;
   constexpr int TILE = B * R;
-  RegVec f[R];
-#pragma unroll
-  for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
+  f.tile_start();
   uint64_t vb = 0;
   LoadTemps<R> T;
   Rows<R> rs;
@@ -1383,6 +1542,14 @@ __device__ __forceinline__ void vm_tile(const DevPlan& plan, int64_t tile, int64
     DevOp next = load_op(plan, plan.split < n_ops ? plan.split : 0);
     for (int q = plan.split; q < n_ops; ++q) {
       const DevOp op = next;
+      if constexpr (RF::kLds) {
+        if (op.code == D_LOADN) {
+          lds_loads<NREG, R, true, P>(plan, op, q, f, vb, rs);
+          q += op.aux;
+          next = load_op(plan, q + 1 < n_ops ? q + 1 : q);
+          continue;
+        }
+      }
       next = load_op(plan, q + 1 < n_ops ? q + 1 : q);
       tile_op<NREG, R, GATHER, P>(plan, op, f, vb, T, rs, alive, sp, n, lane);
     }
@@ -1402,11 +1569,32 @@ __device__ __forceinline__ LDS uint64_t* wave_spill(LDS uint64_t* base, int n_sp
 
 extern __shared__ uint64_t s_spill_area[];
 
+// The register file a kernel's waves use: VGPRs, or (LF) LDS after the
+// workgroup's spill slots (NREG*64*R slots per wave, cleared once, since
+// every register is written before it is read within a tile).
+template <int NREG, int R, bool LF> struct RegFileT { typedef VRegs<NREG, R> type; };
+template <int NREG, int R> struct RegFileT<NREG, R, true> { typedef LRegs<NREG, R> type; };
+
+template <int NREG, int R, int B, bool LF>
+__device__ __forceinline__ typename RegFileT<NREG, R, LF>::type reg_file(int n_spill) {
+  typename RegFileT<NREG, R, LF>::type f;
+  if constexpr (LF) {
+    const int lane = threadIdx.x & 63;
+    f.p = (LDS uint64_t*)s_spill_area + (int64_t)n_spill * (B / 64) * (64 * R + 8) +
+          (int64_t)(threadIdx.x >> 6) * (NREG * 64 * R) + lane * R;
+#pragma unroll
+    for (int i = 0; i < NREG; ++i)
+#pragma unroll
+      for (int k = 0; k < R; ++k) f.p[i * (64 * R) + k] = 0;
+  }
+  return f;
+}
+
 // perform_interpreter_operation (rows are the positions [0, num_rows)) or
 // the output phase of a filter (GATHER: rows are the selected row ids
 // idx[0, count), outputs at the positions in that list). Persistent:
 // workgroups stride over tiles of B*R positions.
-template <int NREG, int R, bool GATHER, int B, class P = PlanSrc>
+template <int NREG, int R, bool GATHER, int B, class P = PlanSrc, bool LF = false>
 __global__ __launch_bounds__(B) void vm_rows(const DevPlan* __restrict__ pplan,
                                              const uint32_t* __restrict__ idx,
                                              const int64_t* __restrict__ d_count, int64_t capacity) {
@@ -1414,25 +1602,27 @@ __global__ __launch_bounds__(B) void vm_rows(const DevPlan* __restrict__ pplan,
   const DevPlan& plan = *pplan;  // device memory: ops are scalar-cache loads
   const int lane = threadIdx.x & 63;
   LDS uint64_t* sp = wave_spill<R>((LDS uint64_t*)s_spill_area, plan.n_spill);
+  typename RegFileT<NREG, R, LF>::type f = reg_file<NREG, R, B, LF>(plan.n_spill);
   int64_t count = plan.num_rows;
   if (GATHER) {
     count = *d_count;
     if (count > capacity) count = capacity;
   }
   for (int64_t tile = blockIdx.x; tile * TILE < count; tile += gridDim.x)
-    vm_tile<NREG, R, GATHER, B, P>(plan, tile, count, idx, lane, sp);
+    vm_tile<NREG, R, GATHER, B, P>(plan, f, tile, count, idx, lane, sp);
 }
 
 // Batched interpreter: every batch has its own plan (program and columns);
 // the work items are all batches' tiles, batch b owning items
 // [tile_start[b], tile_start[b+1]). Workgroups stride over the items, so
 // uneven batch sizes spread over the whole chip in one persistent launch.
-template <int NREG, int R, int B>
+template <int NREG, int R, int B, bool LF = false>
 __global__ __launch_bounds__(B) void vm_batches(const DevPlan* __restrict__ plans,
                                                 const int64_t* __restrict__ tile_start, int n_batches) {
   const int lane = threadIdx.x & 63;
   const int64_t n_items = tile_start[n_batches];
   LDS uint64_t* sp = wave_spill<R>((LDS uint64_t*)s_spill_area, plans[0].n_spill);
+  typename RegFileT<NREG, R, LF>::type f = reg_file<NREG, R, B, LF>(plans[0].n_spill);
   for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
     int lo = 0, hi = n_batches - 1;  // last batch whose first item <= item
     while (lo < hi) {
@@ -1441,7 +1631,7 @@ __global__ __launch_bounds__(B) void vm_batches(const DevPlan* __restrict__ plan
       else hi = mid - 1;
     }
     const DevPlan& plan = plans[lo];
-    vm_tile<NREG, R, false, B>(plan, item - tile_start[lo], plan.num_rows, nullptr, lane, sp);
+    vm_tile<NREG, R, false, B>(plan, f, item - tile_start[lo], plan.num_rows, nullptr, lane, sp);
   }
 }
 
@@ -1950,12 +2140,10 @@ __device__ __forceinline__ void merge_word(uint32_t* words, int64_t g, uint32_t 
 template <int NREG, int R>
 __device__ __forceinline__ void vm_dense(const DevPlan& plan, const LDS uint16_t* offs, int64_t row0, int cnt,
                                          int64_t base, int64_t capacity, int lane, LDS uint32_t* vbits) {
-  typedef typename RegVecT<NREG>::type RegVec;
   const int shift = (int)(base & 31);
   for (int c0 = 0; c0 < cnt; c0 += 64 * R) {
-    RegVec f[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
+    VRegs<NREG, R> f;
+    f.tile_start();
     uint64_t vb = 0;
     LoadTemps<R> T;
     Rows<R> rs;
@@ -1987,7 +2175,7 @@ __device__ __forceinline__ void vm_dense(const DevPlan& plan, const LDS uint16_t
       const DevOutCol& oc = plan.out[op.aux];
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        if ((rs.exist >> k) & 1u) store_elem<true>(oc, rs.pos0 + 64 * k, f[k][ia]);
+        if ((rs.exist >> k) & 1u) store_elem<true>(oc, rs.pos0 + 64 * k, f.at(k, ia));
         if ((VA >> k) & 1u) lds_set_bit(vbits + op.aux * FUSED_VWORDS, shift + c0 + 64 * k + lane);
       }
     }
@@ -2225,9 +2413,8 @@ __device__ __forceinline__ void out_rows(const DevPlan& plan, uint32_t alive, in
     rs.exist = (1u << rs.nrows) - 1u;
 #pragma unroll
     for (int j = 0; j < R; ++j) rs.row[j] = rs.row0 + j;
-    RegVec f[R];
-#pragma unroll
-    for (int j = 0; j < R; ++j) f[j] = (RegVec)(0);
+    VRegs<NREG, R> f;
+    f.tile_start();
     uint64_t vb = 0;
     LoadTemps<R> T;
     uint32_t al = rs.exist;
@@ -2266,14 +2453,14 @@ __device__ __forceinline__ void out_rows(const DevPlan& plan, uint32_t alive, in
       for (int j = 0; j < R; ++j) {
         if (!((a >> j) & 1u)) continue;
         if constexpr (CAP > 0) {
-          slots[op.aux * CAP + q] = f[j][ia];
+          slots[op.aux * CAP + q] = f.at(j, ia);
           if (oc.valid && !((VA >> j) & 1u)) {
             lds_set_bit(nulls + op.aux * fp_null_words(CAP), q);
             __hip_atomic_fetch_or(nullmask, 1u << op.aux, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           }
         } else if (base + q < capacity) {
           // cached stores: the lanes' pieces of a line meet in L2
-          store_elem<false>(oc, base + q, f[j][ia]);
+          store_elem<false>(oc, base + q, f.at(j, ia));
           if (oc.valid && ((VA >> j) & 1u)) lds_set_bit(vbits + op.aux * vw, shift + q);
         }
         ++q;
@@ -2602,9 +2789,8 @@ __global__ __launch_bounds__(B) void select_vm(const DevPlan* __restrict__ pplan
   uint32_t wave_total = 0;
   const int split = P::split(plan);
   for (int64_t slice = s0 + (threadIdx.x >> 6); slice < s1; slice += NW) {
-    RegVec f[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
+    VRegs<NREG, R> f;
+    f.tile_start();
     uint64_t vb = 0;
     LoadTemps<R> L;
     uint32_t alive_all = 0;

@@ -576,13 +576,22 @@ class Batch:
     def launch(self, stream=None):
         L.check(L.lib().bsql_batch_launch(self.handle, _stream_ptr(stream)))
 
-    def set_launch(self, rows_per_thread=0):
-        cfg = L.LaunchConfig(0, rows_per_thread, 0)
+    def set_launch(self, rows_per_thread=0, block_size=0, lds_regs=False, spill_wide=False):
+        """Launch shape: rows per lane, workgroup size (0, 64 or 256),
+        whether the interpreter's register file sits in LDS, and whether every
+        program keeps at most 7 live values in registers (the rest in LDS spill
+        slots) so that the 8-register file runs them all."""
+        flags = (L.LAUNCH_LDS_REGS if lds_regs else 0) | (L.LAUNCH_SPILL_WIDE if spill_wide else 0)
+        cfg = L.LaunchConfig(block_size, rows_per_thread, flags)
         L.check(L.lib().bsql_batch_set_launch(self.handle, ctypes.byref(cfg)))
 
     def autotune(self, stream=None):
+        """Times every launch shape; keeps the fastest. Returns its rows per
+        lane; `chosen` holds (block_size, rows_per_thread, lds_regs, spill_wide)."""
         cfg = L.LaunchConfig()
         L.check(L.lib().bsql_batch_autotune(self.handle, _stream_ptr(stream), ctypes.byref(cfg)))
+        self.chosen = (cfg.block_size, cfg.rows_per_thread, bool(cfg.flags & L.LAUNCH_LDS_REGS),
+                       bool(cfg.flags & L.LAUNCH_SPILL_WIDE))
         return cfg.rows_per_thread
 
     @property

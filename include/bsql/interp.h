@@ -354,7 +354,7 @@ enum {
                                  runs as one pass (selection, prefix and
                                  outputs in one kernel, no row ids); default
                                  selection, row ids and output pass */
-  BSQL_LAUNCH_GENERATED = 16  /* run through a kernel generated for the
+  BSQL_LAUNCH_GENERATED = 16, /* run through a kernel generated for the
                                  program: its op fields, column types and
                                  validity presence compiled in as constants
                                  (hipRTC, at the first launch of each program
@@ -365,6 +365,10 @@ enum {
                                  predicate): a range-test predicate with
                                  fixed-width outputs and no RAND runs as one
                                  pass. Other calls run on the interpreter. */
+  BSQL_LAUNCH_LDS_REGS = 32   /* batched launches: the interpreter's register
+                                 file in LDS (one address add and one LDS
+                                 access per operand) instead of VGPRs
+                                 (indexed register moves) */
 };
 typedef struct bsql_launch_config {
   int32_t block_size;     /* threads per workgroup: 0 (auto), 64 or 256 */
@@ -423,11 +427,17 @@ typedef struct bsql_batch_desc {
 typedef struct bsql_batch bsql_batch;
 int bsql_batch_prepare(const bsql_batch_desc* descs, int n, bsql_batch** batch);
 int bsql_batch_launch(const bsql_batch* batch, void* stream);
-/* rows per lane of the batched launch (re-uploads the work map; not for use
- * while a graph holding the launch may run) */
+/* Launch shape of the batched launch: block size, rows per lane and the
+ * flags BSQL_LAUNCH_LDS_REGS (register file in LDS) and
+ * BSQL_LAUNCH_SPILL_WIDE (every program lowered for the 8-register file,
+ * values beyond 7 live in LDS spill slots; without it one program that needs
+ * more than 8 registers puts the whole batch on the 16-register file).
+ * Selects the work map of that shape (built on first use; not for use while
+ * a graph holding the launch may run). */
 int bsql_batch_set_launch(bsql_batch* batch, const bsql_launch_config* cfg);
-/* Autotuner: times each launch shape the batch's programs allow on `stream`
- * (a few launches each, hipEvents), keeps the fastest and returns it. */
+/* Autotuner: times each launch shape, register file place and (where they
+ * differ) lowering form on `stream` (a few launches each, hipEvents), keeps
+ * the fastest and returns it. */
 int bsql_batch_autotune(bsql_batch* batch, void* stream, bsql_launch_config* chosen);
 int64_t bsql_batch_total_rows(const bsql_batch* batch);
 void bsql_batch_free(bsql_batch* batch);

@@ -15,7 +15,9 @@ pytestmark = pytest.mark.gpu
 TYPES = [INT32, INT64, FLOAT64, FLOAT64, TS_D, INT32]
 
 
-def _make_batches(bsql, seed, n_batches, max_rows):
+def _make_batches(bsql, seed, n_batches, max_rows, wide=0):
+    """n_batches batches of up to max_rows rows; batches 1..wide carry
+    programs that need more than 8 registers."""
     from blazingsql_amd.interop import DeviceColumn
     rng = np.random.default_rng(seed)
     batches = []
@@ -27,9 +29,15 @@ def _make_batches(bsql, seed, n_batches, max_rows):
         table = make_table(rng, TYPES, rows, null_rates=rates)
         gen = ExprGen(rng, TYPES)
         while True:
-            exprs = [gen.numeric(3) if rng.random() < 0.6 else gen.boolean(3) for _ in range(int(rng.integers(1, 4)))]
+            if 1 <= b <= wide:
+                exprs = [gen.numeric(5) for _ in range(3)]
+            else:
+                exprs = [gen.numeric(3) if rng.random() < 0.6 else gen.boolean(3)
+                         for _ in range(int(rng.integers(1, 4)))]
             try:
                 prog = bsql.Program.compile(exprs, TYPES)
+                if 1 <= b <= wide and prog.plan_info(TYPES)["n_regs"] <= 8:
+                    continue
                 break
             except BsqlError:
                 # texts the reference's front end rejects too (an OR of three
@@ -72,6 +80,24 @@ def test_batches_vs_oracle(bsql, gpu, rpt):
     _check(batches, f"rpt={rpt}")
 
 
+@pytest.mark.parametrize("rpt,block", [(0, 0), (1, 256), (2, 256), (1, 64), (2, 64), (4, 64)])
+def test_batches_lds_registers_vs_oracle(bsql, gpu, rpt, block):
+    """The register file in LDS: every launch shape against the oracle, then
+    a second launch over cleared outputs (the file carries stale values from
+    the last tile, which every program overwrites before reading)."""
+    import torch
+    from blazingsql_amd.interop import Batch
+    batches = _make_batches(bsql, 200 + rpt + block, 40, 200_000)
+    batch = Batch([(p, dev, outs, rows) for p, _, dev, outs, rows, _ in batches])
+    batch.set_launch(rpt, block_size=block, lds_regs=True)
+    for rep in range(2):
+        _clear(batches)
+        torch.cuda.synchronize()
+        batch.launch()
+        torch.cuda.synchronize()
+        _check(batches, f"lds registers rpt={rpt} block={block} launch {rep}")
+
+
 def test_batches_in_hip_graph(bsql, gpu):
     """The launch captured in a graph, replayed three times with eager work
     and a synchronization between replays; every replay matches the oracle."""
@@ -102,10 +128,11 @@ def test_batch_autotune(bsql, gpu):
     batch = Batch([(p, dev, outs, rows) for p, _, dev, outs, rows, _ in batches])
     r = batch.autotune()
     assert r in (1, 2, 4)
+    assert batch.chosen[0] in (64, 256) and batch.chosen[1] == r
     _clear(batches)
     batch.launch()
     torch.cuda.synchronize()
-    _check(batches, f"autotuned rpt={r}")
+    _check(batches, f"autotuned {batch.chosen}")
 
 
 def test_program_autotune(bsql, gpu):
@@ -126,3 +153,21 @@ def test_program_autotune(bsql, gpu):
     for k in range(2):
         gv, gval = outs[k].to_numpy(n)
         assert_same(gv, gval, ores[k][0], ores[k][1], what=f"output {k}")
+
+
+@pytest.mark.parametrize("lds", [False, True])
+def test_batches_spill_wide_vs_oracle(bsql, gpu, lds):
+    """Every program lowered for the 8-register file (values beyond 7 in LDS
+    spill slots), with the register file in VGPRs or in LDS, against the
+    oracle; the batch mixes programs that need up to 16 registers."""
+    import torch
+    from blazingsql_amd.interop import Batch
+    batches = _make_batches(bsql, 300 + int(lds), 40, 200_000, wide=3)
+    batch = Batch([(p, dev, outs, rows) for p, _, dev, outs, rows, _ in batches])
+    for rpt, block in [(4, 256), (2, 256), (4, 64), (2, 64)]:
+        batch.set_launch(rpt, block_size=block, lds_regs=lds, spill_wide=True)
+        _clear(batches)
+        torch.cuda.synchronize()
+        batch.launch()
+        torch.cuda.synchronize()
+        _check(batches, f"spill_wide lds={lds} rpt={rpt} block={block}")
