@@ -38,7 +38,9 @@ enum DevCode : uint8_t {
   D_EQ_TS, D_NE_TS, D_LT_TS, D_GT_TS, D_LE_TS, D_GE_TS,
   D_EQ_SS, D_NE_SS, D_LT_SS, D_GT_SS, D_LE_SS, D_GE_SS,
   D_AND, D_OR, D_INDF_II, D_INDF_FF, D_MAGIC_IF_NOT, D_FIRST_NON_MAGIC,
-  D_NOT, D_ABS_I, D_ABS_F, D_FLOOR, D_CEIL, D_SIN, D_COS, D_ASIN, D_ACOS,
+  D_NOT,
+  D_MAGIC_IF_NULL,  // valid(a) ? a : the magic value of a's kind; valid(a) (IS_NOT_NULL + MAGIC_IF_NOT fused)
+  D_ABS_I, D_ABS_F, D_FLOOR, D_CEIL, D_SIN, D_COS, D_ASIN, D_ACOS,
   D_TAN, D_COTAN, D_ATAN, D_LN, D_LOG, D_EXTRACT,
   D_IS_NULL, D_IS_NOT_NULL, D_IS_TRUE, D_IS_NOT_TRUE, D_IS_NOT_FALSE,
   D_TO_I64, D_TO_F64, D_CAST_DATE, D_CAST_TS, D_CHAR_LENGTH, D_RAND,
@@ -69,7 +71,11 @@ enum : uint8_t {
   F_A_F = 16,     // operand a holds double bits (else int64)
   F_B_F = 32,     // operand b holds double bits
   F_STORE_VALID = 64,  // D_STORE: the output column has a validity buffer
-  F_SLICE_LIT = 128    // D_SLICE: the position is the (already clamped) literal in imm
+  F_SLICE_LIT = 128,   // D_SLICE: the position is the (already clamped) literal in imm
+  // ops on doubles (the *_FF codes, POW, ROUND, FLOOR, CEIL and the
+  // transcendentals): operand a / b is an int64 slot read as a double
+  F_A_CVT = 64,
+  F_B_CVT = 128
 };
 
 typedef struct DevOp {

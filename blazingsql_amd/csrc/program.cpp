@@ -7,6 +7,7 @@
 #include <limits>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <set>
 
 namespace bsql {
@@ -595,39 +596,86 @@ bool is_float_code(uint8_t c) {
 
 bool is_unary_code(uint8_t c) { return c >= D_NOT; }
 
-// Float ops read both operands as doubles on the device: int literals are
-// converted here and int values through an explicit D_TO_F64 op (once per
-// value), the conversion the reference applies when it reads an integer slot
-// into a double operand (interpreter_ops.cuh:600-727).
-void float_operands(Lowering& L, SsaProgram& s) {
-  std::vector<SourceOp> out;
-  std::map<int, int> conv;
-  for (SourceOp so : s.ops) {
-    if (is_float_code(so.d.code)) {
-      auto fix = [&](int& v, uint8_t fflag, uint8_t immflag, uint8_t nullflag) {
-        DevOp& d = so.d;
-        if (d.flags & fflag) return;
-        if (d.flags & immflag) { d.imm = f64_bits((double)d.imm); d.flags |= fflag; return; }
-        if (d.flags & nullflag) { d.flags |= fflag; return; }
-        if (v < 0) return;
-        auto it = conv.find(v);
-        if (it == conv.end()) {
-          SourceOp c;
-          c.prog_op = so.prog_op;
-          c.d.code = D_TO_F64;
-          c.va = v;
-          c.vdst = L.new_value();
-          out.push_back(c);
-          it = conv.emplace(v, c.vdst).first;
-        }
-        v = it->second;
-        d.flags |= fflag;
-      };
-      fix(so.va, F_A_F, F_A_IMM, F_A_NULL);
-      if (!is_unary_code(so.d.code)) fix(so.vb, F_B_F, F_B_IMM, F_B_NULL);
-    }
-    out.push_back(so);
+// Float ops read both operands as doubles on the device, the conversion the
+// reference applies when it reads an integer slot into a double operand
+// (interpreter_ops.cuh:600-727): int literals are converted here, int values
+// by the op itself (F_A_CVT / F_B_CVT: (double)(int64) of the slot), so a
+// conversion costs no op of its own.
+void float_operands(Lowering&, SsaProgram& s) {
+  for (SourceOp& so : s.ops) {
+    if (!is_float_code(so.d.code)) continue;
+    auto fix = [&](int v, uint8_t fflag, uint8_t immflag, uint8_t nullflag, uint8_t cvtflag) {
+      DevOp& d = so.d;
+      if (d.flags & fflag) return;
+      if (d.flags & immflag) { d.imm = f64_bits((double)d.imm); d.flags |= fflag; return; }
+      if (d.flags & nullflag) { d.flags |= fflag; return; }
+      if (v < 0) return;
+      d.flags |= fflag | cvtflag;
+    };
+    fix(so.va, F_A_F, F_A_IMM, F_A_NULL, F_A_CVT);
+    if (!is_unary_code(so.d.code)) fix(so.vb, F_B_F, F_B_IMM, F_B_NULL, F_B_CVT);
   }
+}
+
+// Common subexpressions, then fused pairs.
+// CASE WHEN x IS NOT NULL THEN x ... (COALESCE, and the reference's CASE
+// lowering) reaches the interpreter as IS_NOT_NULL(x) feeding
+// MAGIC_IF_NOT(_, x); when nothing else reads the IS_NOT_NULL the pair is
+// one op, MAGIC_IF_NULL(x), with the same result and validity
+// (interpreter_ops.cuh:559-566 with a condition that is valid(x)).
+void fuse_ops(Lowering& L, SsaProgram& s) {
+  // common subexpressions: a text that repeats an expression (Calcite's CASE
+  // and COALESCE repeat their operands) computes it once; every op but RAND
+  // is a pure function of its operands and fields
+  {
+    std::vector<int> repl(L.values.size());
+    for (size_t v = 0; v < repl.size(); ++v) repl[v] = (int)v;
+    std::map<std::tuple<int, int, int, int, int64_t, int, int>, int> seen;
+    std::vector<SourceOp> kept;
+    for (SourceOp so : s.ops) {
+      if (so.va >= 0) so.va = repl[(size_t)so.va];
+      if (so.vb >= 0) so.vb = repl[(size_t)so.vb];
+      if (so.d.code != D_RAND) {
+        const auto key = std::make_tuple((int)so.d.code, (int)so.d.flags, (int)so.d.aux, (int)so.d.aux2, so.d.imm,
+                                         so.va, so.vb);
+        auto it = seen.find(key);
+        if (it != seen.end()) {
+          repl[(size_t)so.vdst] = it->second;
+          continue;
+        }
+        seen.emplace(key, so.vdst);
+      }
+      kept.push_back(so);
+    }
+    for (int& v : s.finals) v = repl[(size_t)v];
+    s.ops = kept;
+  }
+  std::vector<int> uses(L.values.size(), 0);
+  for (const SourceOp& o : s.ops) {
+    if (o.va >= 0) ++uses[(size_t)o.va];
+    if (o.vb >= 0) ++uses[(size_t)o.vb];
+  }
+  for (int v : s.finals) ++uses[(size_t)v];
+  std::vector<int> prod(L.values.size(), -1);
+  for (size_t i = 0; i < s.ops.size(); ++i) prod[(size_t)s.ops[i].vdst] = (int)i;
+  std::vector<char> dead(s.ops.size(), 0);
+  const uint8_t literal = F_A_IMM | F_A_NULL | F_B_IMM | F_B_NULL;
+  for (SourceOp& m : s.ops) {
+    if (m.d.code != D_MAGIC_IF_NOT || (m.d.flags & literal) || m.va < 0 || m.vb < 0) continue;
+    if (uses[(size_t)m.va] != 1) continue;
+    const int p = prod[(size_t)m.va];
+    if (p < 0) continue;
+    const SourceOp& t = s.ops[(size_t)p];
+    if (t.d.code != D_IS_NOT_NULL || (t.d.flags & literal) || t.va != m.vb) continue;
+    dead[(size_t)p] = 1;
+    m.d.code = D_MAGIC_IF_NULL;
+    m.d.flags = (m.d.flags & F_B_F) ? F_A_F : 0;
+    m.va = m.vb;
+    m.vb = -1;
+  }
+  std::vector<SourceOp> out;
+  for (size_t i = 0; i < s.ops.size(); ++i)
+    if (!dead[i]) out.push_back(s.ops[i]);
   s.ops = out;
 }
 
@@ -683,6 +731,7 @@ SsaProgram to_ssa(Lowering& L, const Program& p, uint64_t seed) {
   }
   s.types = t;
   s.slot_types = slot_types;
+  fuse_ops(L, s);
   return s;
 }
 

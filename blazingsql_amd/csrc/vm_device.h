@@ -653,8 +653,8 @@ static __device__ bool like_match(const GLOBAL uint8_t* s, int32_t n, const GLOB
 #define FA ((op.flags & F_A_F) != 0)
 #define FB ((op.flags & F_B_F) != 0)
 // float ops: the lowering hands them both operands as doubles
-#define AF as_f(a)
-#define BF as_f(b)
+#define AF ((op.flags & F_A_CVT) ? (double)(int64_t)a : as_f(a))
+#define BF ((op.flags & F_B_CVT) ? (double)(int64_t)b : as_f(b))
 // one result per row: D[k] and validity bit k of VD
 #define ROWS(EXPR_D, EXPR_V)                                                   \
   _Pragma("unroll") for (int k = 0; k < R; ++k) {                              \
@@ -721,6 +721,7 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
     }
     // unary ops: computed for valid inputs, validity passes through (:728-917)
     case D_NOT: ROWS(!truthy(a, FA), va)
+    case D_MAGIC_IF_NULL: ROWS(va ? a : (FA ? bits(MAGIC_F64) : (uint64_t)MAGIC_I64), va)
     case D_ABS_I: ROWS((int64_t)a < 0 ? 0ULL - a : a, va)
     case D_ABS_F: ROWS(bits(fabs(as_f(a))), va)
     case D_FLOOR: ROWS(bits(floor(AF)), va)
@@ -1241,10 +1242,41 @@ __device__ __forceinline__ bool vm_step_lds(const DevPlan& plan, const DevOp& op
                                             const Rows<R>& rs, uint32_t& alive, LDS uint64_t* sp) {
   constexpr uint32_t RMASK = (1u << R) - 1u;
   const uint8_t code = op.code;
-  uint64_t D[R];
-  uint32_t VD = 0;
+  uint64_t A[R], B[R], D[R];
+  uint32_t VA = 0, VB = 0, VD = 0;
   const int dst = op.dst == 0xff ? NREG - 1 : (op.dst & (NREG - 1));
-  if (code == D_LOADS) {
+  // operands: both LDS reads issued before either is waited for
+  auto operands = [&]() __attribute__((always_inline)) {
+    const int ia = op.a & (NREG - 1), ib = op.b & (NREG - 1);
+    f.get(ia, A);
+    f.get(ib, B);
+    VA = (uint32_t)(vb >> (ia * R)) & RMASK;
+    VB = (uint32_t)(vb >> (ib * R)) & RMASK;
+    if (op.flags & F_A_IMM) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) A[k] = (uint64_t)op.imm;
+      VA = RMASK;
+    }
+    if (op.flags & F_B_IMM) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) B[k] = (uint64_t)op.imm;
+      VB = RMASK;
+    }
+    if (op.flags & F_A_NULL) VA = 0;
+    if (op.flags & F_B_NULL) VB = 0;
+  };
+  if (code >= D_ADD_II && code < D_SPILL) {  // the computing ops: one range test, then their switch
+    operands();
+    compute_rows<R>(plan, op, A, B, VA, VB, D, VD, rs);
+  } else if (code == D_KILL || code == D_SEL) {
+    operands();
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const bool t = code == D_KILL ? truthy(A[k], FA) : (A[k] != 0);
+      if (!(((VA >> k) & 1u) && t)) alive &= ~(1u << k);
+    }
+    return code == D_KILL && __ballot(alive != 0) == 0;
+  } else if (code == D_LOADS) {
     const DevInCol c = P::in(plan, op.a);
     const GLOBAL int32_t* off = gp<int32_t>(c.offsets);
 #pragma unroll
@@ -1262,47 +1294,16 @@ __device__ __forceinline__ bool vm_step_lds(const DevPlan& plan, const DevOp& op
 #pragma unroll
     for (int k = 0; k < R; ++k) D[k] = slot[k * 64 + lane];
     VD = ((const LDS uint8_t*)(slot + 64 * R))[lane];
-  } else if (code == D_LOADN || code == D_COMMIT) {
-    return false;  // run by lds_loads
+  } else if (code == D_SPILL) {
+    operands();
+    const int lane = threadIdx.x & 63;
+    LDS uint64_t* slot = spill_slot<R>(sp, op.imm);
+#pragma unroll
+    for (int k = 0; k < R; ++k) slot[k * 64 + lane] = A[k];
+    ((LDS uint8_t*)(slot + 64 * R))[lane] = (uint8_t)VA;
+    return false;
   } else {
-    uint64_t A[R], B[R];
-    const int ia = op.a & (NREG - 1), ib = op.b & (NREG - 1);
-    uint32_t VA, VB;
-    if (op.flags & F_A_IMM) {
-#pragma unroll
-      for (int k = 0; k < R; ++k) A[k] = (uint64_t)op.imm;
-      VA = RMASK;
-    } else {
-      f.get(ia, A);
-      VA = (uint32_t)(vb >> (ia * R)) & RMASK;
-    }
-    if (op.flags & F_B_IMM) {
-#pragma unroll
-      for (int k = 0; k < R; ++k) B[k] = (uint64_t)op.imm;
-      VB = RMASK;
-    } else {
-      f.get(ib, B);
-      VB = (uint32_t)(vb >> (ib * R)) & RMASK;
-    }
-    if (op.flags & F_A_NULL) VA = 0;
-    if (op.flags & F_B_NULL) VB = 0;
-    if (code == D_SPILL) {
-      const int lane = threadIdx.x & 63;
-      LDS uint64_t* slot = spill_slot<R>(sp, op.imm);
-#pragma unroll
-      for (int k = 0; k < R; ++k) slot[k * 64 + lane] = A[k];
-      ((LDS uint8_t*)(slot + 64 * R))[lane] = (uint8_t)VA;
-      return false;
-    }
-    if (code == D_KILL || code == D_SEL) {
-#pragma unroll
-      for (int k = 0; k < R; ++k) {
-        const bool t = code == D_KILL ? truthy(A[k], FA) : (A[k] != 0);
-        if (!(((VA >> k) & 1u) && t)) alive &= ~(1u << k);
-      }
-      return code == D_KILL && __ballot(alive != 0) == 0;
-    }
-    compute_rows<R>(plan, op, A, B, VA, VB, D, VD, rs);
+    return false;  // LOADN and COMMIT are run by lds_loads
   }
   f.put(dst, D);
   vb = (vb & ~((uint64_t)RMASK << (dst * R))) | ((uint64_t)(VD & RMASK) << (dst * R));
