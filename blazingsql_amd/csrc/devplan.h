@@ -38,6 +38,7 @@ enum DevCode : uint8_t {
   D_EQ_TS, D_NE_TS, D_LT_TS, D_GT_TS, D_LE_TS, D_GE_TS,
   D_EQ_SS, D_NE_SS, D_LT_SS, D_GT_SS, D_LE_SS, D_GE_SS,
   D_AND, D_OR, D_INDF_II, D_INDF_FF, D_MAGIC_IF_NOT, D_FIRST_NON_MAGIC,
+  D_COALESCE,    // FIRST_NON_MAGIC(MAGIC_IF_NULL(a), b): COALESCE(a, b) with the magic-value rule
   D_NOT,
   D_MAGIC_IF_NULL,  // valid(a) ? a : the magic value of a's kind; valid(a) (IS_NOT_NULL + MAGIC_IF_NOT fused)
   D_ABS_I, D_ABS_F, D_FLOOR, D_CEIL, D_SIN, D_COS, D_ASIN, D_ACOS,

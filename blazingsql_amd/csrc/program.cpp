@@ -673,6 +673,21 @@ void fuse_ops(Lowering& L, SsaProgram& s) {
     m.va = m.vb;
     m.vb = -1;
   }
+  // FIRST_NON_MAGIC(MAGIC_IF_NULL(x), y), the MAGIC_IF_NULL read by nothing
+  // else: COALESCE(x, y) (y may be a literal); both ops must agree on x's kind
+  for (size_t i = 0; i < s.ops.size(); ++i) {
+    if (dead[i]) continue;
+    SourceOp& f = s.ops[i];
+    if (f.d.code != D_FIRST_NON_MAGIC || (f.d.flags & (F_A_IMM | F_A_NULL)) || f.va < 0) continue;
+    if (uses[(size_t)f.va] != 1) continue;
+    const int p = prod[(size_t)f.va];
+    if (p < 0 || dead[(size_t)p]) continue;
+    const SourceOp& m = s.ops[(size_t)p];
+    if (m.d.code != D_MAGIC_IF_NULL || ((m.d.flags ^ f.d.flags) & F_A_F)) continue;
+    dead[(size_t)p] = 1;
+    f.d.code = D_COALESCE;
+    f.va = m.va;
+  }
   std::vector<SourceOp> out;
   for (size_t i = 0; i < s.ops.size(); ++i)
     if (!dead[i]) out.push_back(s.ops[i]);

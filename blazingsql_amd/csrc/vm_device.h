@@ -89,8 +89,17 @@ __device__ __forceinline__ int64_t floor_div(int64_t a, int64_t b) {
 // civil_from_days; inline, so that EXTRACT over a DATE column costs no call
 __device__ __forceinline__ int64_t extract_ymd(int op, int64_t days) {
   const int64_t z = days + 719468;
-  const int64_t era = (z >= 0 ? z : z - 146096) / 146097;
-  const unsigned doe = (unsigned)(z - era * 146097);
+  int64_t era;
+  unsigned doe;
+  if (z == (int64_t)(int32_t)z) {  // every int32 day number: 32-bit divide by the constant
+    const int32_t z32 = (int32_t)z;
+    const int32_t e32 = (z32 >= 0 ? z32 : z32 - 146096) / 146097;
+    era = e32;
+    doe = (unsigned)(z32 - e32 * 146097);
+  } else {
+    era = (z >= 0 ? z : z - 146096) / 146097;
+    doe = (unsigned)(z - era * 146097);
+  }
   const unsigned yoe = (doe - doe / 1460 + doe / 36524 - doe / 146096) / 365;
   const int64_t yy = (int64_t)yoe + era * 400;
   const unsigned doy = doe - (365 * yoe + yoe / 4 - yoe / 100);
@@ -722,6 +731,17 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
     // unary ops: computed for valid inputs, validity passes through (:728-917)
     case D_NOT: ROWS(!truthy(a, FA), va)
     case D_MAGIC_IF_NULL: ROWS(va ? a : (FA ? bits(MAGIC_F64) : (uint64_t)MAGIC_I64), va)
+    case D_COALESCE: {  // FIRST_NON_MAGIC(MAGIC_IF_NULL(a), b)
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const uint32_t va = (VA >> k) & 1u, vb = (VB >> k) & 1u;
+        const uint64_t m = va ? A[k] : (FA ? bits(MAGIC_F64) : (uint64_t)MAGIC_I64);
+        const bool magic = FA ? as_f(m) == MAGIC_F64 : (int64_t)m == MAGIC_I64;
+        D[k] = magic ? B[k] : m;
+        VD |= (magic ? vb : va) << k;
+      }
+      return;
+    }
     case D_ABS_I: ROWS((int64_t)a < 0 ? 0ULL - a : a, va)
     case D_ABS_F: ROWS(bits(fabs(as_f(a))), va)
     case D_FLOOR: ROWS(bits(floor(AF)), va)
