@@ -1198,29 +1198,6 @@ struct LoadTemps {
   uint32_t v0, v1, v2, v3;
 };
 
-// The register file in VGPRs: f[k][i] = register i of the lane's row k.
-// Constant register numbers (generated kernels) make every access a plain
-// register; a dynamic number costs indexed moves and, on a write, a copy of
-// the whole vector.
-template <int NREG, int R>
-struct VRegs {
-  static constexpr bool kLds = false;
-  typename RegVecT<NREG>::type f[R];
-  __device__ __forceinline__ void tile_start() {
-#pragma unroll
-    for (int k = 0; k < R; ++k) f[k] = (typename RegVecT<NREG>::type)(0);
-  }
-  __device__ __forceinline__ uint64_t at(int k, int i) const { return f[k][i]; }
-  __device__ __forceinline__ void get(int i, uint64_t (&v)[R]) const {
-#pragma unroll
-    for (int k = 0; k < R; ++k) v[k] = f[k][i];
-  }
-  __device__ __forceinline__ void put(int i, const uint64_t (&v)[R]) {
-#pragma unroll
-    for (int k = 0; k < R; ++k) f[k][i] = v[k];
-  }
-};
-
 // The register file in LDS, for plans read at run time: register i of the
 // lane's R rows is the R consecutive slots at p + i*64*R, so a dynamic
 // register number costs one address add and the access is one LDS load or
@@ -1228,9 +1205,7 @@ struct VRegs {
 // barrier). The wave's file is NREG*64*R slots.
 template <int NREG, int R>
 struct LRegs {
-  static constexpr bool kLds = true;
   LDS uint64_t* p;
-  __device__ __forceinline__ void tile_start() {}
   __device__ __forceinline__ uint64_t at(int k, int i) const { return p[i * (64 * R) + k]; }
   __device__ __forceinline__ void get(int i, uint64_t (&v)[R]) const {
     const LDS uint64_t* q = p + i * (64 * R);
@@ -1243,6 +1218,8 @@ struct LRegs {
     for (int k = 0; k < R; ++k) q[k] = v[k];
   }
 };
+template <class T> struct IsLRegs { static constexpr bool value = false; };
+template <int NREG, int R> struct IsLRegs<LRegs<NREG, R>> { static constexpr bool value = true; };
 
 // Executes one non-store op. Returns true if it was a KILL that left the
 // wave without a selected row.
@@ -1357,17 +1334,26 @@ __device__ __forceinline__ void lds_loads(const DevPlan& plan, const DevOp& op, 
   }
 }
 
-template <int NREG, int R, bool NT = true, class P = PlanSrc, class RF>
-__device__ __forceinline__ bool vm_step(const DevPlan& plan, const DevOp& op, RF& f, uint64_t& vb,
+template <int NREG, int R, bool NT = true, class P = PlanSrc>
+__device__ __forceinline__ bool vm_step(const DevPlan& plan, const DevOp& op, LRegs<NREG, R>& f, uint64_t& vb,
+                                        LoadTemps<R>&, const Rows<R>& rs, uint32_t& alive, LDS uint64_t* sp) {
+  return vm_step_lds<NREG, R, P>(plan, op, f, vb, rs, alive, sp);
+}
+
+template <int NREG, int R, bool NT = true, class P = PlanSrc>
+__device__ __forceinline__ bool vm_step(const DevPlan& plan, const DevOp& op,
+                                        typename RegVecT<NREG>::type (&f)[R], uint64_t& vb,
                                         LoadTemps<R>& T, const Rows<R>& rs, uint32_t& alive,
                                         LDS uint64_t* sp) {
-  if constexpr (RF::kLds) return vm_step_lds<NREG, R, P>(plan, op, f, vb, rs, alive, sp);
   constexpr uint32_t RMASK = (1u << R) - 1u;
   const uint8_t code = op.code;
   uint64_t A[R], B[R], D[R];
   const int ia = op.a & (NREG - 1), ib = op.b & (NREG - 1);
-  f.get(ia, A);
-  f.get(ib, B);
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    A[k] = f[k][ia];
+    B[k] = f[k][ib];
+  }
   uint32_t VA = (uint32_t)(vb >> (ia * R)) & RMASK, VB = (uint32_t)(vb >> (ib * R)) & RMASK, VD = 0;
   if (op.flags & F_A_IMM) {
 #pragma unroll
@@ -1450,7 +1436,8 @@ __device__ __forceinline__ bool vm_step(const DevPlan& plan, const DevOp& op, RF
     compute_rows<R>(plan, op, A, B, VA, VB, D, VD, rs);
   }
   // result write: the only write of the register file
-  f.put(dst, D);
+#pragma unroll
+  for (int k = 0; k < R; ++k) f[k][dst] = D[k];
   vb = (vb & ~((uint64_t)RMASK << (dst * R))) | ((uint64_t)(VD & RMASK) << (dst * R));
   return dead;
 }
@@ -1516,17 +1503,33 @@ __device__ __forceinline__ void tile_op(const DevPlan& plan, const DevOp& op, RF
   }
   const int ia = op.a & (NREG - 1);
   uint64_t A[R];
-  f.get(ia, A);
+  if constexpr (IsLRegs<RF>::value) {
+    f.get(ia, A);
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) A[k] = f[k][ia];
+  }
   const uint32_t VA = (uint32_t)(vb >> (ia * R));
   if (GATHER) store_out_strided<R>(P::out(plan, op.aux), rs.pos0, rs.exist, A, VA & rs.exist, lane);
   else store_out<R>(P::out(plan, op.aux), rs.pos0, n, A, VA & rs.exist, lane);
 }
 
-template <int NREG, int R, bool GATHER, int B, class P = PlanSrc, class RF>
-__device__ __forceinline__ void vm_tile(const DevPlan& plan, RF& f, int64_t tile, int64_t count,
+// One tile: the register file in VGPRs (a fresh array per tile, every
+// register written before it is read) or, LF, the wave's LDS file at lfp.
+template <int NREG, int R, bool GATHER, int B, class P = PlanSrc, bool LF = false>
+__device__ __forceinline__ void vm_tile(const DevPlan& plan, LDS uint64_t* lfp, int64_t tile, int64_t count,
                                         const uint32_t* __restrict__ idx, int lane, LDS uint64_t* sp) {
+  constexpr bool kLds = LF;
   constexpr int TILE = B * R;
-  f.tile_start();
+  typedef typename RegVecT<NREG>::type RegVec;
+  RegVec fv[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) fv[k] = (RegVec)(0);
+  LRegs<NREG, R> fl{lfp};
+  auto& f = [&]() -> auto& {
+    if constexpr (LF) return fl;
+    else return fv;
+  }();
   uint64_t vb = 0;
   LoadTemps<R> T;
   Rows<R> rs;
@@ -1563,7 +1566,7 @@ __device__ __forceinline__ void vm_tile(const DevPlan& plan, RF& f, int64_t tile
     DevOp next = load_op(plan, plan.split < n_ops ? plan.split : 0);
     for (int q = plan.split; q < n_ops; ++q) {
       const DevOp op = next;
-      if constexpr (RF::kLds) {
+      if constexpr (kLds) {
         if (op.code == D_LOADN) {
           lds_loads<NREG, R, true, P>(plan, op, q, f, vb, rs);
           q += op.aux;
@@ -1590,25 +1593,23 @@ __device__ __forceinline__ LDS uint64_t* wave_spill(LDS uint64_t* base, int n_sp
 
 extern __shared__ uint64_t s_spill_area[];
 
-// The register file a kernel's waves use: VGPRs, or (LF) LDS after the
-// workgroup's spill slots (NREG*64*R slots per wave, cleared once, since
-// every register is written before it is read within a tile).
-template <int NREG, int R, bool LF> struct RegFileT { typedef VRegs<NREG, R> type; };
-template <int NREG, int R> struct RegFileT<NREG, R, true> { typedef LRegs<NREG, R> type; };
-
+// The LDS register file of the calling lane (LF kernels): after the
+// workgroup's spill slots, NREG*64*R slots per wave, cleared once (every
+// register is written before it is read within a tile). nullptr otherwise.
 template <int NREG, int R, int B, bool LF>
-__device__ __forceinline__ typename RegFileT<NREG, R, LF>::type reg_file(int n_spill) {
-  typename RegFileT<NREG, R, LF>::type f;
-  if constexpr (LF) {
+__device__ __forceinline__ LDS uint64_t* lds_file(int n_spill) {
+  if constexpr (!LF) {
+    return nullptr;
+  } else {
     const int lane = threadIdx.x & 63;
-    f.p = (LDS uint64_t*)s_spill_area + (int64_t)n_spill * (B / 64) * (64 * R + 8) +
-          (int64_t)(threadIdx.x >> 6) * (NREG * 64 * R) + lane * R;
+    LDS uint64_t* p = (LDS uint64_t*)s_spill_area + (int64_t)n_spill * (B / 64) * (64 * R + 8) +
+                      (int64_t)(threadIdx.x >> 6) * (NREG * 64 * R) + lane * R;
 #pragma unroll
     for (int i = 0; i < NREG; ++i)
 #pragma unroll
-      for (int k = 0; k < R; ++k) f.p[i * (64 * R) + k] = 0;
+      for (int k = 0; k < R; ++k) p[i * (64 * R) + k] = 0;
+    return p;
   }
-  return f;
 }
 
 // perform_interpreter_operation (rows are the positions [0, num_rows)) or
@@ -1623,14 +1624,14 @@ __global__ __launch_bounds__(B) void vm_rows(const DevPlan* __restrict__ pplan,
   const DevPlan& plan = *pplan;  // device memory: ops are scalar-cache loads
   const int lane = threadIdx.x & 63;
   LDS uint64_t* sp = wave_spill<R>((LDS uint64_t*)s_spill_area, plan.n_spill);
-  typename RegFileT<NREG, R, LF>::type f = reg_file<NREG, R, B, LF>(plan.n_spill);
+  LDS uint64_t* lfp = lds_file<NREG, R, B, LF>(plan.n_spill);
   int64_t count = plan.num_rows;
   if (GATHER) {
     count = *d_count;
     if (count > capacity) count = capacity;
   }
   for (int64_t tile = blockIdx.x; tile * TILE < count; tile += gridDim.x)
-    vm_tile<NREG, R, GATHER, B, P>(plan, f, tile, count, idx, lane, sp);
+    vm_tile<NREG, R, GATHER, B, P, LF>(plan, lfp, tile, count, idx, lane, sp);
 }
 
 // Batched interpreter: every batch has its own plan (program and columns);
@@ -1643,7 +1644,7 @@ __global__ __launch_bounds__(B) void vm_batches(const DevPlan* __restrict__ plan
   const int lane = threadIdx.x & 63;
   const int64_t n_items = tile_start[n_batches];
   LDS uint64_t* sp = wave_spill<R>((LDS uint64_t*)s_spill_area, plans[0].n_spill);
-  typename RegFileT<NREG, R, LF>::type f = reg_file<NREG, R, B, LF>(plans[0].n_spill);
+  LDS uint64_t* lfp = lds_file<NREG, R, B, LF>(plans[0].n_spill);
   for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
     int lo = 0, hi = n_batches - 1;  // last batch whose first item <= item
     while (lo < hi) {
@@ -1652,7 +1653,7 @@ __global__ __launch_bounds__(B) void vm_batches(const DevPlan* __restrict__ plan
       else hi = mid - 1;
     }
     const DevPlan& plan = plans[lo];
-    vm_tile<NREG, R, false, B>(plan, f, item - tile_start[lo], plan.num_rows, nullptr, lane, sp);
+    vm_tile<NREG, R, false, B, PlanSrc, LF>(plan, lfp, item - tile_start[lo], plan.num_rows, nullptr, lane, sp);
   }
 }
 
@@ -2161,10 +2162,12 @@ __device__ __forceinline__ void merge_word(uint32_t* words, int64_t g, uint32_t 
 template <int NREG, int R>
 __device__ __forceinline__ void vm_dense(const DevPlan& plan, const LDS uint16_t* offs, int64_t row0, int cnt,
                                          int64_t base, int64_t capacity, int lane, LDS uint32_t* vbits) {
+  typedef typename RegVecT<NREG>::type RegVec;
   const int shift = (int)(base & 31);
   for (int c0 = 0; c0 < cnt; c0 += 64 * R) {
-    VRegs<NREG, R> f;
-    f.tile_start();
+    RegVec f[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
     uint64_t vb = 0;
     LoadTemps<R> T;
     Rows<R> rs;
@@ -2196,7 +2199,7 @@ __device__ __forceinline__ void vm_dense(const DevPlan& plan, const LDS uint16_t
       const DevOutCol& oc = plan.out[op.aux];
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        if ((rs.exist >> k) & 1u) store_elem<true>(oc, rs.pos0 + 64 * k, f.at(k, ia));
+        if ((rs.exist >> k) & 1u) store_elem<true>(oc, rs.pos0 + 64 * k, f[k][ia]);
         if ((VA >> k) & 1u) lds_set_bit(vbits + op.aux * FUSED_VWORDS, shift + c0 + 64 * k + lane);
       }
     }
@@ -2434,8 +2437,9 @@ __device__ __forceinline__ void out_rows(const DevPlan& plan, uint32_t alive, in
     rs.exist = (1u << rs.nrows) - 1u;
 #pragma unroll
     for (int j = 0; j < R; ++j) rs.row[j] = rs.row0 + j;
-    VRegs<NREG, R> f;
-    f.tile_start();
+    RegVec f[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) f[j] = (RegVec)(0);
     uint64_t vb = 0;
     LoadTemps<R> T;
     uint32_t al = rs.exist;
@@ -2474,14 +2478,14 @@ __device__ __forceinline__ void out_rows(const DevPlan& plan, uint32_t alive, in
       for (int j = 0; j < R; ++j) {
         if (!((a >> j) & 1u)) continue;
         if constexpr (CAP > 0) {
-          slots[op.aux * CAP + q] = f.at(j, ia);
+          slots[op.aux * CAP + q] = f[j][ia];
           if (oc.valid && !((VA >> j) & 1u)) {
             lds_set_bit(nulls + op.aux * fp_null_words(CAP), q);
             __hip_atomic_fetch_or(nullmask, 1u << op.aux, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           }
         } else if (base + q < capacity) {
           // cached stores: the lanes' pieces of a line meet in L2
-          store_elem<false>(oc, base + q, f.at(j, ia));
+          store_elem<false>(oc, base + q, f[j][ia]);
           if (oc.valid && ((VA >> j) & 1u)) lds_set_bit(vbits + op.aux * vw, shift + q);
         }
         ++q;
@@ -2810,8 +2814,9 @@ __global__ __launch_bounds__(B) void select_vm(const DevPlan* __restrict__ pplan
   uint32_t wave_total = 0;
   const int split = P::split(plan);
   for (int64_t slice = s0 + (threadIdx.x >> 6); slice < s1; slice += NW) {
-    VRegs<NREG, R> f;
-    f.tile_start();
+    RegVec f[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
     uint64_t vb = 0;
     LoadTemps<R> L;
     uint32_t alive_all = 0;
