@@ -302,6 +302,61 @@ def run_batched(args):
 
 # ---------------------------------------------------------------- learned
 
+def run_opcost(args):
+    """Cost of one interpreted op: chains of k ops of one kind (k = 1 and 16)
+    over 16 batches of 4M rows with the batched columns, timed for each
+    register file and launch shape; the slope over k is the op's cost. Prints
+    one JSON line per (kind, k)."""
+    import torch
+    import blazingsql_amd as bsql
+    from blazingsql_amd.interop import Batch, DeviceColumn
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+    n, nb = 4 << 20, 16
+    tables = []
+    for _ in range(nb):
+        cols = []
+        for t in BATCH_TYPES:
+            if t == 10:
+                data = torch.rand(n, generator=g, device=dev, dtype=torch.float64) * 2000 - 1000
+            elif t == 4:
+                data = torch.randint(-10**9, 10**9, (n,), generator=g, device=dev, dtype=torch.int64)
+            elif t == 12:
+                data = torch.randint(-20000, 40000, (n,), generator=g, device=dev, dtype=torch.int32)
+            else:
+                data = torch.randint(-1000, 1000, (n,), generator=g, device=dev, dtype=torch.int32)
+            cols.append(DeviceColumn(t, data, device_validity(n, 0.1, g, dev)))
+        tables.append(cols)
+    step = {"add_f64": lambda e: f"+({e}, $3)", "add_i64": lambda e: f"+({e}, $5)",
+            "div_f64": lambda e: f"/({e}, $3)", "extract_year": lambda e: f"+({e}, EXTRACT(FLAG(YEAR), $4))",
+            "coalesce": lambda e: f"CASE(IS NOT NULL($3), $3, {e})",
+            "case": lambda e: f"CASE(>($3, 0), {e}, $2)"}
+    start = {"add_i64": "$0"}
+    shapes = [(False, 2, 256), (False, 4, 256), (True, 2, 256), (True, 4, 64)]
+    for kind, f in step.items():
+        for k in (1, 16):
+            e = start.get(kind, "$2")
+            for _ in range(k):
+                e = f(e)
+            prog = bsql.Program.compile([e], BATCH_TYPES)
+            ot = prog.output_type(BATCH_TYPES, 0)
+            batch = Batch([(prog, tables[b], [DeviceColumn.empty(ot, n)], n) for b in range(nb)])
+            row = {"kind": kind, "k": k, "device_ops": prog.plan_info(BATCH_TYPES)["n_ops"], "rows": n * nb}
+            for lds, rpt, block in shapes:
+                batch.set_launch(rpt, block_size=block, lds_regs=lds, spill_wide=True)
+                batch.launch()
+                torch.cuda.synchronize()
+                a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                for _ in range(args.steps):
+                    batch.launch()
+                z.record()
+                torch.cuda.synchronize()
+                row[f"{'lds' if lds else 'vgpr'}_r{rpt}_b{block}_ms"] = round(a.elapsed_time(z) / args.steps, 3)
+            print(json.dumps(row), flush=True)
+
+
 def learned_workload(n, rhos, predicates, steps, records, seed, per_query=True):
     """n rows of 8 float64/int32 columns through a Gaussian copula (each rho
     in rhos); per rho, `predicates` conjunctions of 4-8 range terms whose
@@ -651,10 +706,11 @@ def main():
     ap.add_argument("--no-learned", action="store_true",
                     help="leave out the reduced learned-and-batched measurement of the scan line")
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)
-    ap.add_argument("--workload", choices=["scan", "batched", "learned", "strings"], default="scan",
+    ap.add_argument("--workload", choices=["scan", "batched", "learned", "strings", "opcost"], default="scan",
                     help="scan: the 100M-row filter+project (the headline line); batched: 512 "
                          "programs in one persistent launch; learned: MLP conjunct ordering vs static; "
-                         "strings: 50M-row LIKE/SUBSTRING/CHAR_LENGTH filters with math projections")
+                         "strings: 50M-row LIKE/SUBSTRING/CHAR_LENGTH filters with math projections; "
+                         "opcost: the batched interpreter's cost per op by kind and launch shape")
     ap.add_argument("--string-rows", type=int, default=50_000_000)
     ap.add_argument("--string-cpu-rows", type=int, default=10_000_000)
     ap.add_argument("--batches", type=int, default=512)
@@ -679,6 +735,9 @@ def main():
         return
     if args.workload == "strings":
         run_strings(args)
+        return
+    if args.workload == "opcost":
+        run_opcost(args)
         return
 
     import torch

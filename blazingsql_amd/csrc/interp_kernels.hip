@@ -13,8 +13,9 @@
 //     through the scalar cache and dispatched with scalar branches;
 //   * the slot buffer of the reference (8 bytes per slot per row, in LDS) is a
 //     register file reg[NREG][R] in VGPRs, indexed with the wave-uniform
-//     register number (s_set_gpr_idx relative addressing); validity is one bit
-//     per register per row;
+//     register number (s_set_gpr_idx relative addressing), or, for batched
+//     launches with BSQL_LAUNCH_LDS_REGS, in LDS at one address add per
+//     operand; validity is one bit per register per row in a VGPR;
 //   * compaction (filters): a selection kernel writes 64-bit ballots and a
 //     count per wave slice and a total per workgroup block, with no order
 //     between workgroups; slice_ids turns them into the selected row ids in

@@ -98,13 +98,17 @@ def test_batches_lds_registers_vs_oracle(bsql, gpu, rpt, block):
         _check(batches, f"lds registers rpt={rpt} block={block} launch {rep}")
 
 
-def test_batches_in_hip_graph(bsql, gpu):
+@pytest.mark.parametrize("lds", [False, True])
+def test_batches_in_hip_graph(bsql, gpu, lds):
     """The launch captured in a graph, replayed three times with eager work
-    and a synchronization between replays; every replay matches the oracle."""
+    and a synchronization between replays; every replay matches the oracle
+    (register file in VGPRs, or in LDS with the 8-register form)."""
     import torch
     from blazingsql_amd.interop import Batch
-    batches = _make_batches(bsql, 7, 24, 100_000)
+    batches = _make_batches(bsql, 7, 24, 100_000, wide=2 if lds else 0)
     batch = Batch([(p, dev, outs, rows) for p, _, dev, outs, rows, _ in batches])
+    if lds:
+        batch.set_launch(4, block_size=64, lds_regs=True, spill_wide=True)
     batch.launch()  # warm-up outside capture
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
@@ -171,3 +175,36 @@ def test_batches_spill_wide_vs_oracle(bsql, gpu, lds):
         batch.launch()
         torch.cuda.synchronize()
         _check(batches, f"spill_wide lds={lds} rpt={rpt} block={block}")
+
+
+def test_c3_shape_vs_oracle(bsql, gpu):
+    """The batched workload's shape: programs of 8-24 ops (CASE, COALESCE,
+    CAST, EXTRACT, arithmetic, comparisons), batches of 4K-4M rows with null
+    rates up to 30 %, autotuned launch, every batch against the oracle."""
+    import torch
+    from blazingsql_amd.interop import Batch, DeviceColumn
+    rng = np.random.default_rng(2024)
+    batches = []
+    sizes = [4096, 4194304] + [int(np.exp(rng.uniform(np.log(4096), np.log(4194304)))) for _ in range(10)]
+    for b, rows in enumerate(sizes):
+        table = make_table(rng, TYPES, rows, null_rates=list(rng.uniform(0, 0.3, len(TYPES))))
+        gen = ExprGen(rng, TYPES)
+        while True:
+            exprs = [gen.numeric(4) if rng.random() < 0.7 else gen.boolean(3)
+                     for _ in range(int(rng.integers(1, 4)))]
+            try:
+                prog = bsql.Program.compile(exprs, TYPES)
+            except BsqlError:
+                continue
+            if 8 <= prog.num_ops <= 24:
+                break
+        ot = [prog.output_type(TYPES, k) for k in range(prog.num_outputs)]
+        outs = [DeviceColumn.empty(t, rows) for t in ot]
+        batches.append((prog, table, device_columns(table), outs, rows, ot))
+    batch = Batch([(p, dev, outs, rows) for p, _, dev, outs, rows, _ in batches])
+    batch.autotune()
+    _clear(batches)
+    torch.cuda.synchronize()
+    batch.launch()
+    torch.cuda.synchronize()
+    _check(batches, f"C3 shape, autotuned {batch.chosen}")
