@@ -161,6 +161,9 @@ SIGNATURES = {
     "bsql_table_record_stream": (_I, [_P, _P]),
     "bsql_process_filter": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_P), _P]),
     "bsql_process_project": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_P), _P]),
+    "bsql_process_project_batches": (_I, [ctypes.POINTER(_P), ctypes.c_int, ctypes.c_char_p,
+                                          ctypes.POINTER(_P), _P]),
+    "bsql_table_size_in_bytes": (ctypes.c_int64, [_P]),
     "bsql_evaluate_expressions": (_I, [_P, _P, _I, ctypes.POINTER(_P), _P]),
     "bsql_apply_boolean_filter": (_I, [_P, ctypes.POINTER(Column), ctypes.POINTER(_P), _P]),
     "bsql_aggregation_input": (_I, [_P, ctypes.c_char_p, _I, ctypes.POINTER(_P), _P]),

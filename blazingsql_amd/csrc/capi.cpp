@@ -1412,8 +1412,20 @@ int bsql_batch_prepare(const bsql_batch_desc* descs, int n, bsql_batch** out) {
       if (e == hipSuccess)
         e = hipMemcpy(F.d_plans, F.plans.data(), F.plans.size() * sizeof(DevPlan), hipMemcpyHostToDevice);
     }
+    // default: the register file in LDS and every program on the 8-register
+    // file (the fastest on every batch measured); the VGPR file and each
+    // program's own form where that does not fit a workgroup's LDS
     int rpt = 0, block = 256;
-    batch_shape(b.get(), 0, 256, &rpt, &block);
+    b->lf = true;
+    b->form = 1;
+    try {
+      batch_shape(b.get(), 0, 256, &rpt, &block);
+    } catch (const Error&) {
+      b->lf = false;
+      b->form = 0;
+      batch_shape(b.get(), 0, 256, &rpt, &block);
+    }
+    b->launch = bsql_launch_config{block, rpt, (b->lf ? BSQL_LAUNCH_LDS_REGS : 0) | (b->form ? BSQL_LAUNCH_SPILL_WIDE : 0)};
     if (e == hipSuccess) e = upload_work_map(b.get(), rpt, block, nullptr);
     if (e != hipSuccess) return device_error(e, "bsql_batch_prepare");
     *out = b.release();

@@ -67,12 +67,23 @@ void reject_capture(void* stream, const char* entry);
 int execute_program(bsql_program* prog, const bsql_column* in, int n_in, bsql_column* out, int n_out,
                     int64_t num_rows, void* stream, bool private_scratch);
 
+// The device allocator installed with bsql_set_allocator (stream-ordered
+// hipMallocAsync / hipFreeAsync when none is): alloc returns 0, 1 for out of
+// memory, 2 for another failure.
+struct AllocRef {
+  int (*alloc)(void* ctx, void** ptr, size_t bytes, void* stream);
+  void (*release)(void* ctx, void* ptr, size_t bytes, void* stream);
+  void* ctx;
+};
+AllocRef installed_allocator();
+
 // Columns of the table followed by the columns a program's steps build.
 // Owned buffers are released on the stream when this is destroyed.
 struct StepColumns {
   std::vector<bsql_column> cols;
-  std::vector<void*> owned;
+  std::vector<std::pair<void*, size_t>> owned;  // freed with `alloc` on `stream`
   hipStream_t stream = nullptr;
+  AllocRef alloc = installed_allocator();
   StepColumns() = default;
   StepColumns(const StepColumns&) = delete;
   StepColumns& operator=(const StepColumns&) = delete;

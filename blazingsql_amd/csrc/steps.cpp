@@ -3,9 +3,10 @@
 // they take as arguments), the role function_evaluator_transformer plays in
 // the reference (engine/src/execution_kernels/LogicalProject.cpp:670-731).
 //
-// Every buffer is allocated and released with the stream-ordered allocator on
-// the call's stream, so a step column lives exactly as long as the launches
-// that read it.
+// Every buffer is allocated and released with the installed allocator
+// (bsql_set_allocator; stream-ordered hipMallocAsync by default) on the call's
+// stream, so a step column lives exactly as long as the launches that read it
+// and an allocator with a pool limit sees the step columns' memory too.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,11 +25,14 @@ hipError_t launch_strfn(StrFnArgs f, uint64_t* status, hipStream_t stream);
 
 namespace {
 
+// A step column's buffer from the installed allocator, on the call's stream.
 void* stream_alloc(StepColumns& sc, int64_t bytes) {
   void* p = nullptr;
-  const hipError_t e = hipMallocAsync(&p, (size_t)std::max<int64_t>(bytes, 8), sc.stream);
-  if (e != hipSuccess) throw Error(device_error(e, "step column"), "step column: out of device memory");
-  sc.owned.push_back(p);
+  const size_t n = (size_t)std::max<int64_t>(bytes, 8);
+  const int rc = sc.alloc.alloc(sc.alloc.ctx, &p, n, sc.stream);
+  if (rc == 1) throw Error(BSQL_ERR_OOM, "step column: device allocation of " + std::to_string(n) + " bytes failed");
+  if (rc != 0 || !p) throw Error(BSQL_ERR_DEVICE, "step column: device allocation failed");
+  sc.owned.emplace_back(p, n);
   return p;
 }
 
@@ -147,7 +151,7 @@ StrSrc source_of(const StepArg& a, const std::vector<bsql_column>& cols, const u
 }  // namespace
 
 StepColumns::~StepColumns() {
-  for (void* p : owned) (void)hipFreeAsync(p, stream);
+  for (const auto& b : owned) alloc.release(alloc.ctx, b.first, b.second, stream);
 }
 
 std::vector<int32_t> extended_types(const Program& p, const int32_t* types, int n) {

@@ -25,11 +25,7 @@
 
 namespace {
 
-struct Allocator {
-  bsql_alloc_fn alloc;
-  bsql_free_fn release;
-  void* ctx;
-};
+using Allocator = bsql::AllocRef;
 
 int default_alloc(void*, void** p, size_t bytes, void* stream) {
   const hipError_t e = hipMallocAsync(p, bytes, (hipStream_t)stream);
@@ -49,6 +45,8 @@ Allocator current_allocator() {
 }
 
 }  // namespace
+
+bsql::AllocRef bsql::installed_allocator() { return current_allocator(); }
 
 // Columns with their names; each buffer the table allocated is freed with
 // the allocator it came from, on the table's stream.
@@ -217,6 +215,19 @@ Root output_root(const Program& p, int k) {
   return root_of(p, k);
 }
 
+// Whether output k of the projection over `in` carries a validity mask:
+// interpreted outputs always do (make_fixed_width_column(..., UNINITIALIZED));
+// a copied table column keeps its own nullability; a literal column has a
+// mask only for a null literal (make_column_from_scalar)
+// (LogicalProject.cpp:822-839).
+bool output_nullable(const bsql_program& prog, const bsql_table& in, int k) {
+  const Root r = output_root(prog, k);
+  if (r.kind == Root::COLUMN && r.index < (int)in.cols.size())
+    return in.cols[(size_t)r.index].view.validity != nullptr;
+  if (r.kind == Root::LITERAL) return r.lit.is_null;
+  return true;
+}
+
 TablePtr evaluate(const bsql_table& in, const std::vector<std::string>& exprs,
                   const std::vector<std::string>& names, void* stream) {
   const std::vector<int32_t> types = types_of(in);
@@ -229,23 +240,73 @@ TablePtr evaluate(const bsql_table& in, const std::vector<std::string>& exprs,
   std::vector<bsql_column> outs;
   for (size_t k = 0; k < exprs.size(); ++k) {
     const int32_t t = output_type(prog.get(), types, (int)k);
-    // interpreted outputs always carry validity (make_fixed_width_column(...,
-    // UNINITIALIZED)); a copied table column keeps its own nullability; a
-    // literal column has a mask only for a null literal
-    // (make_column_from_scalar) (LogicalProject.cpp:822-839)
-    const Root r = output_root(*prog, (int)k);
-    bool nullable = true;
-    if (r.kind == Root::COLUMN && r.index < (int)in.cols.size())
-      nullable = in.cols[(size_t)r.index].view.validity != nullptr;
-    if (r.kind == Root::LITERAL) nullable = r.lit.is_null;
-    add_column(*res, t, k < names.size() ? names[k] : std::string(), in.num_rows, nullable,
-               t == BSQL_STRING ? chars_bound : 0);
+    add_column(*res, t, k < names.size() ? names[k] : std::string(), in.num_rows,
+               output_nullable(*prog, in, (int)k), t == BSQL_STRING ? chars_bound : 0);
     outs.push_back(res->cols.back().view);
   }
   if (in.num_rows > 0 && !outs.empty()) {
     const int rc = bsql_interp_execute(prog.get(), views.data(), (int)views.size(), outs.data(), (int)outs.size(),
                                        in.num_rows, stream);
     if (rc != BSQL_OK) throw Error(rc, bsql_last_error());
+  }
+  return res;
+}
+
+struct BatchFree {
+  void operator()(bsql_batch* b) const { bsql_batch_free(b); }
+};
+
+// The projection over several batches of one schema as ONE persistent launch
+// (bsql_batch_*): the program is compiled once, each batch's outputs are
+// allocated as evaluate allocates them, and the batches' tiles are the work
+// items of one kernel, so many small batches cost one launch instead of one
+// each. Programs the batched launch does not run (string steps or outputs,
+// programs split over several launches) and batches of different schemas
+// are evaluated batch by batch.
+std::vector<TablePtr> evaluate_batches(const std::vector<const bsql_table*>& ins,
+                                       const std::vector<std::string>& exprs,
+                                       const std::vector<std::string>& names, void* stream) {
+  const std::vector<int32_t> types = types_of(*ins[0]);
+  bool batchable = true;
+  for (const bsql_table* t : ins) batchable = batchable && types_of(*t) == types;
+  std::vector<TablePtr> res;
+  ProgramPtr prog = compile(exprs, types);
+  batchable = batchable && prog->steps.empty() && prog->parts.empty() && num_roots(*prog) <= BSQL_DEV_MAX_OUT;
+  std::vector<int32_t> ot;
+  for (size_t k = 0; k < exprs.size(); ++k) {
+    ot.push_back(output_type(prog.get(), types, (int)k));
+    batchable = batchable && ot.back() != BSQL_STRING;
+  }
+  if (!batchable) {
+    for (const bsql_table* t : ins) res.push_back(evaluate(*t, exprs, names, stream));
+    return res;
+  }
+  std::vector<std::vector<bsql_column>> views(ins.size()), outs(ins.size());
+  std::vector<bsql_batch_desc> descs;
+  for (size_t i = 0; i < ins.size(); ++i) {
+    const bsql_table& in = *ins[i];
+    TablePtr r = new_table(stream);
+    r->num_rows = in.num_rows;
+    for (size_t k = 0; k < exprs.size(); ++k)
+      add_column(*r, ot[k], k < names.size() ? names[k] : std::string(), in.num_rows,
+                 output_nullable(*prog, in, (int)k), 0);
+    views[i] = views_of(in);
+    outs[i] = views_of(*r);
+    if (in.num_rows > 0 && !exprs.empty())
+      descs.push_back(bsql_batch_desc{prog.get(), views[i].data(), (int)views[i].size(), outs[i].data(),
+                                      (int)outs[i].size(), in.num_rows});
+    res.push_back(std::move(r));
+  }
+  if (!descs.empty()) {
+    bsql_batch* b = nullptr;
+    int rc = bsql_batch_prepare(descs.data(), (int)descs.size(), &b);
+    if (rc != BSQL_OK) throw Error(rc, bsql_last_error());
+    std::unique_ptr<bsql_batch, BatchFree> hold(b);
+    rc = bsql_batch_launch(b, stream);
+    if (rc != BSQL_OK) throw Error(rc, bsql_last_error());
+    // the batch's device plans must outlive the launch that reads them
+    const hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) throw Error(device_error(e, "batched projection"), "batched projection failed");
   }
   return res;
 }
@@ -423,6 +484,60 @@ int bsql_process_project(bsql_table* in, const char* query_part, bsql_table** ou
     in->stream = stream;
     delete in;
     *out = res.release();
+    return BSQL_OK;
+  })
+}
+
+int64_t bsql_table_size_in_bytes(const bsql_table* t) {
+  if (!t) return -1;
+  int64_t total = 0;
+  for (const bsql_table::Col& c : t->cols) {
+    const int64_t rows = c.view.size;
+    if (c.view.type_id == BSQL_STRING)
+      total += (int64_t)c.view.chars_size + (rows > 0 ? (rows + 1) * 4 : 0);
+    else
+      total += rows * type_size(c.view.type_id);
+    if (c.view.validity) total += (int64_t)validity_bytes(rows);
+  }
+  return total;
+}
+
+int bsql_process_project_batches(bsql_table* const* ins, int n, const char* query_part, bsql_table** outs,
+                                 void* stream) {
+  BSQL_TRY({
+    if (!ins || n <= 0 || !query_part || !outs) fail("invalid arguments");
+    for (int i = 0; i < n; ++i) {
+      if (!ins[i]) fail("batch " + std::to_string(i) + " is null");
+      outs[i] = nullptr;
+    }
+    // an identity projection moves every batch through (renamed), as
+    // bsql_process_project does batch by batch
+    bool all_bypass = true;
+    for (int i = 0; i < n && all_bypass; ++i) {
+      bool wa = false;
+      std::vector<std::string> al;
+      all_bypass = bypass_kind(query_part, names_of(*ins[i]), &wa, &al);
+    }
+    if (all_bypass) {
+      for (int i = 0; i < n; ++i) {
+        bool wa = false;
+        std::vector<std::string> aliases;
+        bypass_kind(query_part, names_of(*ins[i]), &wa, &aliases);
+        for (size_t k = 0; k < aliases.size(); ++k) ins[i]->cols[k].name = aliases[k];
+        ins[i]->stream = stream;
+        outs[i] = ins[i];
+      }
+      return BSQL_OK;
+    }
+    std::vector<std::string> names, exprs;
+    project_items(query_part, &names, &exprs);
+    std::vector<const bsql_table*> v(ins, ins + n);
+    std::vector<TablePtr> res = evaluate_batches(v, exprs, names, stream);
+    for (int i = 0; i < n; ++i) {
+      ins[i]->stream = stream;
+      delete ins[i];
+      outs[i] = res[(size_t)i].release();
+    }
     return BSQL_OK;
   })
 }

@@ -213,6 +213,25 @@ def process_project(table, query_part, stream=None):
     return Table(h.value, keep)
 
 
+def process_project_batches(tables, query_part, stream=None):
+    """The projection over a cache's waiting batches of one schema in one
+    persistent launch (bsql_process_project_batches; the batches a
+    ConcatenatingCacheMachine collects, CacheMachine.cpp:866-900). Returns one
+    table per batch, in order; consumes every input on success, none on
+    OutOfMemory."""
+    _ensure_allocator()
+    n = len(tables)
+    ins = (ctypes.c_void_p * max(n, 1))(*[t.handle for t in tables])
+    outs = (ctypes.c_void_p * max(n, 1))()
+    L.check(L.lib().bsql_process_project_batches(ins, n, query_part.encode(), outs, _stream_ptr(stream)))
+    result = []
+    for t, h in zip(tables, outs):
+        keep = t._keep
+        t._release()
+        result.append(Table(h, keep))
+    return result
+
+
 def evaluate_expressions(table, exprs, stream=None):
     """ral::processor::evaluate_expressions (LogicalProject.cpp:796-941)."""
     _ensure_allocator()

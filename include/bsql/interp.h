@@ -415,7 +415,11 @@ int bsql_program_plan_info(const bsql_program* prog, const int32_t* in_types, in
  * and uploads all plans (synchronous, allocates); bsql_batch_launch then
  * allocates and copies nothing, so it can be captured in a hipGraph and
  * replayed. Outputs must be fixed-width. Replaces one
- * perform_interpreter_operation call per batch (interpreter_cpp.h:77-86). */
+ * perform_interpreter_operation call per batch (interpreter_cpp.h:77-86).
+ * The prepared launch keeps the register file in LDS with every program on
+ * the 8-register file (BSQL_LAUNCH_LDS_REGS | BSQL_LAUNCH_SPILL_WIDE) where
+ * the programs' spill slots fit, else the VGPR file; bsql_batch_set_launch
+ * and bsql_batch_autotune change it. */
 typedef struct bsql_batch_desc {
   const bsql_program* prog;
   const bsql_column* in;

@@ -7,6 +7,8 @@
 //   Filter::do_process             engine/src/execution_kernels/BatchProcessing.cpp:454-473
 //   Projection::do_process + run   engine/src/execution_kernels/BatchProcessing.cpp:365-448
 //   BindableTableScan::do_process  engine/src/execution_kernels/BatchProcessing.cpp:261-284
+//   ConcatenatingCacheMachine      engine/src/cache_machine/CacheMachine.h:149-175, .cpp:866-900
+//     (QueueSink::pull_batches + BatchedProjection: the waiting batches in one launch)
 // A task hands each kernel one batch (a table) with the stream it runs on; the
 // kernel calls the processor function and maps its outcome: BSQL_OK ->
 // SUCCESS (the result goes to the output sink), BSQL_ERR_OOM -> RETRY with
@@ -18,6 +20,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <deque>
 #include <map>
 #include <memory>
@@ -71,6 +74,24 @@ class QueueSink : public CacheSink {
   size_t size() const {
     std::lock_guard<std::mutex> lk(mu_);
     return q_.size();
+  }
+  // The batches a consumer takes at once, as ConcatenatingCacheMachine::
+  // pullCacheData collects them (CacheMachine.cpp:866-900): popped while the
+  // total with the next batch stays within max_bytes (at least one), in
+  // arrival order, and not concatenated: the batched projection runs them in
+  // one launch as they are.
+  std::vector<Table> pull_batches(size_t max_bytes) {
+    std::lock_guard<std::mutex> lk(mu_);
+    std::vector<Table> out;
+    size_t total = 0;
+    while (!q_.empty()) {
+      const size_t next = (size_t)std::max<int64_t>(bsql_table_size_in_bytes(q_.front().get()), 0);
+      if (!out.empty() && total + next > max_bytes) break;
+      total += next;
+      out.push_back(std::move(q_.front()));
+      q_.pop_front();
+    }
+    return out;
   }
 
  private:
@@ -134,6 +155,33 @@ class Projection : public Kernel {
     if (rc == BSQL_OK) {
       (void)inputs[0].release();  // consumed by the projection
       output->add_to_cache(Table(out));
+    }
+    return outcome(rc, inputs);
+  }
+};
+
+// The Projection kernel over a set of waiting batches of one schema: one
+// task for all of them, the program compiled once and every batch computed
+// in one persistent launch (bsql_process_project_batches). SUCCESS puts each
+// batch's result in the output cache in input order; RETRY hands every input
+// back (none is consumed); FAIL reports the message. An identity projection
+// moves the batches through as Projection does.
+class BatchedProjection : public Projection {
+ public:
+  using Projection::Projection;
+  task_result do_process(std::vector<Table> inputs, std::shared_ptr<CacheSink> output, hipStream_t stream,
+                         const std::map<std::string, std::string>&) override {
+    if (inputs.empty()) return {task_status::FAIL, "Projection: no input batch", {}};
+    std::vector<bsql_table*> in;
+    for (Table& t : inputs) {
+      if (!t) return {task_status::FAIL, "Projection: a null input batch", {}};
+      in.push_back(t.get());
+    }
+    std::vector<bsql_table*> out(in.size(), nullptr);
+    const int rc = bsql_process_project_batches(in.data(), (int)in.size(), expression_.c_str(), out.data(), stream);
+    if (rc == BSQL_OK) {
+      for (Table& t : inputs) (void)t.release();  // consumed (or moved through) by the projection
+      for (bsql_table* t : out) output->add_to_cache(Table(t));
     }
     return outcome(rc, inputs);
   }

@@ -69,6 +69,10 @@ int bsql_table_column(const bsql_table* table, int k, bsql_column* out);
 const char* bsql_table_name(const bsql_table* table, int k);
 int bsql_table_set_names(bsql_table* table, const char* const* names, int n);
 int bsql_table_record_stream(bsql_table* table, void* stream);
+/* Device bytes of the table's columns: data (chars and offsets for strings)
+ * plus validity masks (BlazingTable::sizeInBytes,
+ * engine/src/execution_kernels/LogicPrimitives.cpp:75-110); -1 for NULL. */
+int64_t bsql_table_size_in_bytes(const bsql_table* table);
 
 /* ---- processor functions ------------------------------------------------- */
 
@@ -90,6 +94,20 @@ int bsql_process_filter(const bsql_table* in, const char* query_part, bsql_table
  * (bypassingProject, expression_utils.cpp:1115-1150, as Projection::run does,
  * BatchProcessing.cpp:394-406). */
 int bsql_process_project(bsql_table* in, const char* query_part, bsql_table** out, void* stream);
+
+/* Projection over several batches of one schema (a cache's waiting batches,
+ * as ConcatenatingCacheMachine::pullCacheData collects them,
+ * engine/src/cache_machine/CacheMachine.cpp:866-900, for the Projection kernel,
+ * engine/src/execution_kernels/BatchProcessing.cpp:365-448): the program is
+ * compiled once and every batch's outputs are computed in ONE persistent
+ * launch (bsql_batch_*), without concatenating the batches. outs[i] is batch
+ * i's result; on success every input is consumed (as bsql_process_project
+ * consumes its input). On BSQL_ERR_OOM nothing is consumed and nothing is
+ * returned (the caller retries the whole set). Programs the batched launch
+ * does not run (string steps or outputs) and batches of different schemas
+ * are evaluated batch by batch. */
+int bsql_process_project_batches(bsql_table* const* ins, int n, const char* query_part, bsql_table** outs,
+                                 void* stream);
 
 /* ral::processor::evaluate_expressions (LogicalProject.cpp:796-941): one
  * output column per expression, each with validity (the reference's outputs

@@ -15,6 +15,7 @@
 #include "bsql/kernels.hpp"
 
 using bsql::Table;
+using bsql::execution::BatchedProjection;
 using bsql::execution::BindableTableScan;
 using bsql::execution::Filter;
 using bsql::execution::Projection;
@@ -224,6 +225,65 @@ int main() {
   CHECK(r.status == task_status::SUCCESS && retried &&
             bsql_table_num_rows(retried.get()) == (int64_t)expect_rows.size(),
         "retry succeeds");
+
+  // Batched projection: 24 waiting batches of uneven sizes (one empty) taken
+  // from the queue within a byte budget and projected in one launch; every
+  // batch's result in input order, values from the closed forms.
+  {
+    auto queue = std::make_shared<QueueSink>();
+    std::vector<int64_t> sizes;
+    for (int i = 0; i < 24; ++i) {
+      const int64_t rows = i == 5 ? 0 : (int64_t)(1000 + 37 * i * i);
+      sizes.push_back(rows);
+      queue->add_to_cache(make_batch(rows, s));
+    }
+    std::vector<Table> batches = queue->pull_batches((size_t)1 << 40);
+    CHECK(batches.size() == 24 && queue->size() == 0, "pull_batches takes every batch within the budget");
+    BatchedProjection bproj(
+        "LogicalProject(x=[+(*($0, 2), 1)], y=[CASE(IS NOT NULL($1), $1, -1.0:DOUBLE)], z=[*($1, $0)])");
+    auto bsink = std::make_shared<QueueSink>();
+    r = bproj.do_process(std::move(batches), bsink, s, {});
+    CHECK(r.status == task_status::SUCCESS && bsink->size() == 24, "batched projection succeeds");
+    bool same = true;
+    for (size_t i = 0; i < sizes.size(); ++i) {
+      Table t = bsink->pull();
+      if (!t || bsql_table_num_rows(t.get()) != sizes[i]) {
+        same = false;
+        continue;
+      }
+      auto x = read_column<int32_t>(t.get(), 0, s);
+      auto y = read_column<double>(t.get(), 1, s);
+      auto z = read_column<double>(t.get(), 2, s);
+      auto vz = read_valid(t.get(), 2, s);
+      for (int64_t j = 0; same && j < sizes[i]; ++j) {
+        const bool null_b = j % 7 == 3;
+        const double b = (double)j / 1000.0;
+        same = x[(size_t)j] == (int32_t)(j % 100) * 2 + 1 && y[(size_t)j] == (null_b ? -1.0 : b) &&
+               vz[(size_t)j] == !null_b && (null_b || z[(size_t)j] == b * (double)(j % 100));
+      }
+    }
+    CHECK(same, "batched projection values per batch");
+    // a budget takes a prefix (at least one batch)
+    for (int i = 0; i < 3; ++i) queue->add_to_cache(make_batch(1000, s));
+    const int64_t one = bsql_table_size_in_bytes(queue->pull_batches(0)[0].get());
+    CHECK(one == 1000 * 4 + 1000 * 8 + 128, "table size in bytes: data plus the padded validity mask");
+    CHECK(queue->pull_batches((size_t)(2 * one + 10)).size() == 2 && queue->size() == 0,
+          "pull_batches stops at the budget");
+    // RETRY: every input handed back, nothing cached; then the retry succeeds
+    std::vector<Table> again;
+    for (int i = 0; i < 3; ++i) again.push_back(make_batch(500 + i, s));
+    std::vector<bsql_table*> ptrs;
+    for (Table& t : again) ptrs.push_back(t.get());
+    bsql_set_allocator(failing_alloc, failing_free, nullptr);
+    r = bproj.do_process(std::move(again), bsink, s, {});
+    bsql_set_allocator(nullptr, nullptr, nullptr);
+    bool back = r.status == task_status::RETRY && r.inputs.size() == 3;
+    for (size_t i = 0; back && i < 3; ++i) back = r.inputs[i].get() == ptrs[i];
+    CHECK(back && bsink->size() == 0, "batched RETRY hands every input back");
+    r = bproj.do_process(std::move(r.inputs), bsink, s, {});
+    CHECK(r.status == task_status::SUCCESS && bsink->size() == 3, "batched retry succeeds");
+    while (bsink->size()) bsink->pull();
+  }
 
   // FAIL: a column the batch does not have
   Filter bad("LogicalFilter(condition=[>($5, 1)])");

@@ -10,7 +10,8 @@ import subprocess
 import numpy as np
 import pytest
 
-from tests.helpers import BOOL8, FLOAT64, INT32, INT64, assert_same, device_columns, host_columns, make_table
+from tests.helpers import (BOOL8, FLOAT64, INT32, INT64, TS_D, assert_same, device_columns, host_columns,
+                           make_table)
 
 pytestmark = pytest.mark.gpu
 
@@ -276,3 +277,94 @@ def test_invalid_query_parts(bsql, gpu):
         process_filter(tab, "LogicalFilter(condition=[>($3, 1)])")
     with pytest.raises(L.BsqlError):
         process_filter(tab, "LogicalFilter()")
+
+
+def _random_batches(seed, n_batches, max_rows, types):
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(n_batches):
+        rows = 0 if b == 3 else int(np.exp(rng.uniform(np.log(1), np.log(max_rows))))
+        out.append(make_table(rng, types, rows, null_rates=list(rng.uniform(0, 0.3, len(types)))))
+    return out
+
+
+@pytest.mark.parametrize("query", [
+    "LogicalProject(x=[+(*($0, 3), $1)], y=[CASE(IS NOT NULL($2), $2, *($3, 2.5:DOUBLE))], z=[>($1, $0)], "
+    "w=[$2], v=[EXTRACT(FLAG(YEAR), $4)])",
+    "LogicalProject(a=[$0], b=[$1], c=[$2], d=[$3], e=[$4])",          # identity: moved through
+    "LogicalProject(p=[$1], q=[$0])",                                 # column copies
+])
+def test_project_batches_vs_one_batch_at_a_time(bsql, gpu, query):
+    """bsql_process_project_batches (the waiting batches of a cache in one
+    launch) gives every batch the result bsql_process_project gives it alone."""
+    from blazingsql_amd.processor import Table, process_project, process_project_batches
+    types = [INT32, INT64, FLOAT64, FLOAT64, TS_D]
+    names = ["a", "b", "c", "d", "e"]
+    hosts = _random_batches(11, 20, 300_000, types)
+    one = [process_project(Table.from_columns(device_columns(h), names), query) for h in hosts]
+    many = process_project_batches([Table.from_columns(device_columns(h), names) for h in hosts], query)
+    assert len(many) == len(one)
+    for i, (a, b) in enumerate(zip(many, one)):
+        assert a.num_rows == b.num_rows and a.num_columns == b.num_columns
+        for k in range(a.num_columns):
+            assert a.names[k] == b.names[k]
+            ga, va = a.column(k).to_numpy(a.num_rows)
+            gb, vb = b.column(k).to_numpy(b.num_rows)
+            assert_same(ga, va, gb, vb, what=f"{query[:30]} batch {i} column {k}")
+
+
+def test_project_batches_strings_and_retry(bsql, gpu):
+    """A projection that builds strings runs batch by batch behind the same
+    call (same results); a failing allocation consumes no input."""
+    from blazingsql_amd import _lib as L
+    from blazingsql_amd.interop import DeviceColumn
+    from blazingsql_amd.processor import process_project, process_project_batches, use_torch_allocator
+    words = [["ab", "xyz", "", "q"], ["hello", "w"], ["a" * 40, "b"]]
+    query = "LogicalProject(u=[UPPER($0)], n=[+($1, 1)])"
+
+    def batches():
+        return [_table([_strings(w), DeviceColumn.from_numpy(INT64, np.arange(len(w), dtype=np.int64))], ["s", "i"])
+                for w in words]
+
+    one = [process_project(t, query) for t in batches()]
+    ins = batches()
+    use_torch_allocator(limit_bytes=64)
+    try:
+        with pytest.raises(L.OutOfMemory):
+            process_project_batches(ins, query)
+        assert all(t.num_rows == len(w) for t, w in zip(ins, words))  # not consumed
+    finally:
+        use_torch_allocator()
+    many = process_project_batches(ins, query)
+    for a, b in zip(many, one):
+        for k in range(2):
+            ga, va = a.column(k).to_numpy(a.num_rows)
+            gb, vb = b.column(k).to_numpy(b.num_rows)
+            assert list(ga) == list(gb) and list(va) == list(vb)
+
+
+def test_string_steps_use_installed_allocator(bsql, gpu):
+    """The columns a projection builds before the interpreter runs (here
+    UPPER of a string column, under LIKE) come from the installed allocator:
+    a budget that holds the outputs but not the step column fails the call
+    with OutOfMemory, and a passing call leaves only its outputs live."""
+    from blazingsql_amd import _lib as L
+    from blazingsql_amd.processor import process_project, use_torch_allocator
+    n = 4000
+    words = [("ab" if i % 3 else "xy") + "c" * (i % 20) for i in range(n)]
+    query = "LogicalProject(m=[LIKE(UPPER($0), 'AB%')])"
+    out_bytes = n + ((n + 31) // 32 * 4 + 63) // 64 * 64    # BOOL8 data + validity
+    tab = _table([_strings(words)], ["s"])
+    alloc = use_torch_allocator(limit_bytes=out_bytes + 1024)
+    try:
+        with pytest.raises(L.OutOfMemory):
+            process_project(tab, query)
+        assert tab.num_rows == n
+    finally:
+        alloc = use_torch_allocator()
+    out = process_project(tab, query)
+    got, valid = out.column(0).to_numpy(n)
+    assert list(np.asarray(got, dtype=bool)) == [w.upper().startswith("AB") for w in words] and all(valid)
+    import torch
+    torch.cuda.synchronize()
+    assert alloc.live_bytes == out_bytes    # the step column was released
