@@ -357,6 +357,64 @@ def run_opcost(args):
             print(json.dumps(row), flush=True)
 
 
+def run_parquet(args):
+    """Scan of a Parquet file with a pushed-down filter (parquet_scan): a
+    file of args.parquet_rows rows (a: int64 sorted, b: float64 with 10 %
+    nulls, c: int32, d: date32) in 1M-row row groups, written to a temporary
+    directory; timed end to end (footer statistics, skip-data pass, host
+    decode, copy, filter) for a filter the statistics prune and one they
+    cannot, against decoding every row group alone."""
+    import tempfile
+    import time
+    import numpy as np
+    import pyarrow as pa
+    import pyarrow.parquet as pq
+    import torch
+    from blazingsql_amd.parquet_scan import ParquetScan
+    n, group = args.parquet_rows, 1 << 20
+    rng = np.random.default_rng(args.seed)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = f"{tmp}/scan.parquet"
+        a = np.sort(rng.integers(0, 10**9, n)).astype(np.int64)
+        b = rng.uniform(-100, 100, n)
+        t = pa.table({"a": pa.array(a), "b": pa.array(b, mask=rng.random(n) < 0.1),
+                      "c": rng.integers(-1000, 1000, n).astype(np.int32),
+                      "d": pa.array((np.arange(n) // 1000).astype(np.int32)).cast(pa.date32())})
+        pq.write_table(t, path, row_group_size=group)
+        del t
+        lo, hi = int(a[n // 2]), int(a[n // 2 + n // 20])
+        cases = {"pruned": f"AND(>=($0, {lo}), <($0, {hi}))", "unpruned": "AND(>($2, 900), >($1, 0.0:DOUBLE))"}
+        res = {}
+        for name, f in cases.items():
+            scan = f"BindableTableScan(table=[[main, t]], filters=[[{f}]], projects=[[0, 1, 2, 3]], " \
+                   f"aliases=[[a, b, c, d]])"
+            best, rows_out = None, 0
+            for _ in range(args.steps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                ps = ParquetScan([path], scan)
+                outs = list(ps.batches())
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                rows_out = sum(o.num_rows for o in outs)
+                best = dt if best is None else min(best, dt)
+            res[name] = {"filter": f, "seconds": best, "file_rows_per_s": n / best,
+                         "row_groups_read": len(ps.read), "row_groups": ps.files[0].metadata.num_row_groups,
+                         "rows_out": rows_out}
+        t0 = time.perf_counter()
+        pf = pq.ParquetFile(path)
+        for rg in range(pf.metadata.num_row_groups):
+            pf.read_row_group(rg, columns=["a", "b", "c", "d"])
+        decode = time.perf_counter() - t0
+    print(json.dumps({"metric": "Parquet table scan with pushed-down filter, file rows/s (best of steps)",
+                      "value": res["pruned"]["file_rows_per_s"], "unit": "rows/s", "n_gpus": 1,
+                      "steps": args.steps, "higher_is_better": True, "data": "synthetic",
+                      "config": {"workload": f"{n} rows (a int64 sorted, b float64 10% null, c int32, d date32), "
+                                             f"1M-row row groups, snappy", **res,
+                                 "host_decode_only_seconds": decode,
+                                 "host_decode_only_rows_per_s": n / decode}}))
+
+
 def learned_workload(n, rhos, predicates, steps, records, seed, per_query=True):
     """n rows of 8 float64/int32 columns through a Gaussian copula (each rho
     in rhos); per rho, `predicates` conjunctions of 4-8 range terms whose
@@ -706,11 +764,14 @@ def main():
     ap.add_argument("--no-learned", action="store_true",
                     help="leave out the reduced learned-and-batched measurement of the scan line")
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)
-    ap.add_argument("--workload", choices=["scan", "batched", "learned", "strings", "opcost"], default="scan",
+    ap.add_argument("--parquet-rows", type=int, default=32 << 20)
+    ap.add_argument("--workload", choices=["scan", "batched", "learned", "strings", "opcost", "parquet"],
+                    default="scan",
                     help="scan: the 100M-row filter+project (the headline line); batched: 512 "
                          "programs in one persistent launch; learned: MLP conjunct ordering vs static; "
                          "strings: 50M-row LIKE/SUBSTRING/CHAR_LENGTH filters with math projections; "
-                         "opcost: the batched interpreter's cost per op by kind and launch shape")
+                         "opcost: the batched interpreter's cost per op by kind and launch shape; "
+                         "parquet: a Parquet file scanned with a pushed-down filter")
     ap.add_argument("--string-rows", type=int, default=50_000_000)
     ap.add_argument("--string-cpu-rows", type=int, default=10_000_000)
     ap.add_argument("--batches", type=int, default=512)
@@ -738,6 +799,9 @@ def main():
         return
     if args.workload == "opcost":
         run_opcost(args)
+        return
+    if args.workload == "parquet":
+        run_parquet(args)
         return
 
     import torch
