@@ -702,6 +702,7 @@ def run_strings(args):
     ins = [DeviceColumn(23, chars, offsets=offs, size=n), DeviceColumn(4, x), DeviceColumn(10, y),
            DeviceColumn(4, z)]
     proj = bsql.Program.compile(STR_PROJ, STR_TYPES)
+    proj.set_launch(lds_regs=args.lds_regs)
     ot = [proj.output_type(STR_TYPES, k) for k in range(proj.num_outputs)]
     preds = [bsql.Program.compile(p, STR_TYPES) for p in STR_PREDS]
     outs = [DeviceColumn.empty(t, n) for t in ot]
@@ -765,6 +766,9 @@ def main():
                     help="leave out the reduced learned-and-batched measurement of the scan line")
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)
     ap.add_argument("--parquet-rows", type=int, default=32 << 20)
+    ap.add_argument("--lds-regs", action="store_true",
+                    help="interpreter launches (the scan's interpreter comparison, the strings "
+                         "workload's output phase) keep the register file in LDS")
     ap.add_argument("--workload", choices=["scan", "batched", "learned", "strings", "opcost", "parquet"],
                     default="scan",
                     help="scan: the 100M-row filter+project (the headline line); batched: 512 "
@@ -817,7 +821,7 @@ def main():
     proj = bsql.Program.compile(PROJ, TYPES)
     generated = not (args.interpreter or args.fused)
     pred.set_launch(rows_per_thread=args.rows_per_thread, fused=args.fused, generated=generated)
-    proj.set_launch(rows_per_thread=args.rows_per_thread)
+    proj.set_launch(rows_per_thread=args.rows_per_thread, lds_regs=args.lds_regs)
     cnt = torch.zeros(1, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream()
     kernel = "interpreter: selection, row ids, output pass" if not args.fused else "interpreter: fused pass"

@@ -20,7 +20,7 @@
 namespace bsql {
 
 hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt, int block,
-                          hipStream_t stream);
+                          hipStream_t stream, bool lf);
 int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt, int block);
 int64_t string_tiles(int64_t capacity);
 int batch_tile_rows(int nreg, int rpt, int block, bool lf);
@@ -33,7 +33,7 @@ hipError_t launch_string_output(const uint64_t* slices, const uint32_t* valid, i
                                 int64_t count, int64_t capacity, uint64_t* status, hipStream_t stream);
 hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int sel_rpt, int rpt, int block,
                           uint64_t* sel, uint32_t* idx, int64_t* d_count, int64_t capacity,
-                          hipStream_t stream, int phases, bool fused);
+                          hipStream_t stream, int phases, bool fused, bool lf);
 bool fused_eligible(const DevPlan& plan, int nreg);
 int64_t fused_scratch_words(int64_t num_rows);
 // kernel_gen.cpp
@@ -363,6 +363,12 @@ namespace {
 
 int rows_per_thread(const bsql_program* p) { return p ? p->launch.rows_per_thread : 0; }
 int block_size(const bsql_program* p) { return p && p->launch.block_size ? p->launch.block_size : 256; }
+// The register file in LDS for a launch these programs drive
+// (BSQL_LAUNCH_LDS_REGS on either).
+bool lds_regs(const bsql_program* a, const bsql_program* b) {
+  return (a && (a->launch.flags & BSQL_LAUNCH_LDS_REGS)) || (b && (b->launch.flags & BSQL_LAUNCH_LDS_REGS));
+}
+
 int reg_budget(const bsql_program* p) {
   return p && (p->launch.flags & BSQL_LAUNCH_SPILL_WIDE) ? 7 : BSQL_DEV_MAX_REG - 1;
 }
@@ -502,7 +508,7 @@ int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, i
                                         (hipStream_t)stream);
   else if (e == hipSuccess)
     e = launch_compact(plan, (const DevPlan*)(scratch + at_plan), L.nreg, sel_rpt, rpt, block, sel, ids, d_count,
-                       capacity, (hipStream_t)stream, phases, fused);
+                       capacity, (hipStream_t)stream, phases, fused, lds_regs(pred, proj));
   if (e == hipSuccess && outputs)
     e = finish_string_outputs(L, in, out, n_out, slices, d_count, 0, capacity, status, (hipStream_t)stream);
   if (e != hipSuccess) return device_error(e, "filter");
@@ -707,7 +713,8 @@ int run_filter_project_noalloc(bsql_program* pred, bsql_program* proj, const bsq
   e = write_words_async(scratch + at_plan, (const uint64_t*)blob.data(), blob.size() / 8, (hipStream_t)stream);
   if (e == hipSuccess)
     e = launch_compact(plan, (const DevPlan*)(scratch + at_plan), L.nreg, sel_rpt, rpt, block, scratch,
-                       (uint32_t*)(scratch + sel_words), d_count, num_rows, (hipStream_t)stream, PHASE_ALL, fused);
+                       (uint32_t*)(scratch + sel_words), d_count, num_rows, (hipStream_t)stream, PHASE_ALL, fused,
+                       lds_regs(pred, proj));
   if (e != hipSuccess) return device_error(e, "bsql_filter_project_noalloc");
   return BSQL_OK;
 }
@@ -850,7 +857,8 @@ int execute_program(bsql_program* prog, const bsql_column* in, int n_in, bsql_co
         e = launch_generated_execute(plan, L.ops, (const DevPlan*)(scratch + at_plan), L.nreg,
                                      variant_rows(L.nreg, rpt, block, L.n_spill), block, (hipStream_t)stream);
       else
-        e = launch_execute(plan, (const DevPlan*)(scratch + at_plan), L.nreg, rpt, block, (hipStream_t)stream);
+        e = launch_execute(plan, (const DevPlan*)(scratch + at_plan), L.nreg, rpt, block, (hipStream_t)stream,
+                           lds_regs(prog, nullptr));
     }
     if (e == hipSuccess)
       e = finish_string_outputs(L, in, gout.data(), g_out, slices, nullptr, num_rows, num_rows, status,
@@ -1141,7 +1149,7 @@ int bsql_program_set_launch(bsql_program* p, const bsql_launch_config* cfg) {
   if (cfg->block_size != 0 && cfg->block_size != 64 && cfg->block_size != 256)
     return set_error(BSQL_ERR_INVALID, "block_size must be 0, 64 or 256");
   if (cfg->flags & ~(BSQL_LAUNCH_NO_RANGES | BSQL_LAUNCH_LAZY_LOADS | BSQL_LAUNCH_SPILL_WIDE | BSQL_LAUNCH_FUSED |
-                     BSQL_LAUNCH_GENERATED))
+                     BSQL_LAUNCH_GENERATED | BSQL_LAUNCH_LDS_REGS))
     return set_error(BSQL_ERR_INVALID, "unknown launch flags");
   std::lock_guard<std::mutex> lk(p->cache_mu);
   p->launch = *cfg;
@@ -1554,9 +1562,10 @@ int bsql_batch_autotune(bsql_batch* b, void* stream, bsql_launch_config* chosen)
   })
 }
 
-// Tries each launch shape and, for a program that needs more registers
-// than the small file holds, both spill layouts (16 registers, or 8
-// registers with more values in LDS slots and more rows per lane).
+// Tries each launch shape, the register file in LDS or in VGPRs and, for a
+// program that needs more registers than the small file holds, both spill
+// layouts (16 registers, or 8 registers with more values in LDS slots and
+// more rows per lane).
 int bsql_program_autotune(bsql_program* prog, const bsql_column* in, int n_in, bsql_column* out, int n_out,
                           int64_t num_rows, void* stream, bsql_launch_config* chosen) {
   BSQL_TRY({
@@ -1564,12 +1573,15 @@ int bsql_program_autotune(bsql_program* prog, const bsql_column* in, int n_in, b
     float best = 1e30f;
     const bsql_launch_config base = prog->launch;
     bsql_launch_config best_cfg = prog->launch;
-    for (int wide = 0; wide < 2; ++wide) {
+    const int lds_opts[2] = {(int)BSQL_LAUNCH_LDS_REGS, 0};
+    for (int wide = 0; wide < 2; ++wide)
+    for (const int lds : lds_opts) {
       for (const auto& sh : kShapes) {
         bsql_launch_config cfg = base;
         cfg.block_size = sh[0];
         cfg.rows_per_thread = sh[1];
-        cfg.flags = (base.flags & ~BSQL_LAUNCH_SPILL_WIDE) | (wide ? BSQL_LAUNCH_SPILL_WIDE : 0);
+        cfg.flags = (base.flags & ~(BSQL_LAUNCH_SPILL_WIDE | BSQL_LAUNCH_LDS_REGS)) |
+                    (wide ? BSQL_LAUNCH_SPILL_WIDE : 0) | lds;
         prog->launch = cfg;
         float ms = 0.f;
         int rc = BSQL_OK;

@@ -235,14 +235,15 @@ int variant_rows(int nreg, int rpt, int block, int n_spill, bool lf) {
   return pick(nreg, rpt, block, n_spill, &NR, &RR, lf) ? RR : 0;
 }
 
-template <int NREG, int R, int B>
+// LF: the register file in LDS after the spill slots
+template <int NREG, int R, int B, bool LF = false>
 static hipError_t launch_execute_t(const DevPlan* d_plan, int64_t num_rows, int n_spill, hipStream_t stream) {
   const int64_t tile = (int64_t)B * R;
   const int64_t grid = persistent_grid((num_rows + tile - 1) / tile);
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((vm_rows<NREG, R, false, B>), dim3((unsigned)grid), dim3(B),
-                     (unsigned)spill_lds_bytes(n_spill, B, R), stream, d_plan, (const uint32_t*)nullptr,
-                     (const int64_t*)nullptr, num_rows);
+  const unsigned lds = (unsigned)(spill_lds_bytes(n_spill, B, R) + (LF ? regfile_lds_bytes(NREG, B, R) : 0));
+  hipLaunchKernelGGL((vm_rows<NREG, R, false, B, PlanSrc, LF>), dim3((unsigned)grid), dim3(B), lds, stream, d_plan,
+                     (const uint32_t*)nullptr, (const int64_t*)nullptr, num_rows);
   return hipGetLastError();
 }
 
@@ -269,21 +270,31 @@ static hipError_t launch_slice_ids(const SelArgs& sa, uint32_t* idx, int64_t* d_
   return hipGetLastError();
 }
 
-template <int NREG, int R, int B>
+template <int NREG, int R, int B, bool LF = false>
 static hipError_t launch_gather_t(const DevPlan* d_plan, int64_t num_rows, const uint32_t* idx,
                                   const int64_t* d_count, int64_t capacity, int n_spill, hipStream_t stream) {
   const int64_t tile = (int64_t)B * R;
   const int64_t grid = persistent_grid((num_rows + tile - 1) / tile);
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((vm_rows<NREG, R, true, B>), dim3((unsigned)grid), dim3(B),
-                     (unsigned)spill_lds_bytes(n_spill, B, R), stream, d_plan, idx, d_count, capacity);
+  const unsigned lds = (unsigned)(spill_lds_bytes(n_spill, B, R) + (LF ? regfile_lds_bytes(NREG, B, R) : 0));
+  hipLaunchKernelGGL((vm_rows<NREG, R, true, B, PlanSrc, LF>), dim3((unsigned)grid), dim3(B), lds, stream, d_plan,
+                     idx, d_count, capacity);
   return hipGetLastError();
 }
 
 // `d_plan` is a device copy of `plan` (made by the caller on `stream`).
+// lf: the variant with the register file in LDS (BSQL_LAUNCH_LDS_REGS); the
+// VGPR one where no LDS variant holds the program.
 hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt, int block,
-                          hipStream_t stream) {
+                          hipStream_t stream, bool lf) {
   int NR, RR;
+  if (lf && pick(nreg, rpt, block, plan.n_spill, &NR, &RR, true)) {
+#define X(N, RPT, BL)                      \
+  if (NR == N && RR == RPT && block == BL) \
+    return launch_execute_t<N, RPT, BL, true>(d_plan, plan.num_rows, plan.n_spill, stream);
+    BSQL_LF_VARIANTS(X)
+#undef X
+  }
   if (!pick(nreg, rpt, block, plan.n_spill, &NR, &RR)) return hipErrorInvalidValue;
 #define X(N, RPT, BL) \
   if (NR == N && RR == RPT && block == BL) return launch_execute_t<N, RPT, BL>(d_plan, plan.num_rows, plan.n_spill, stream);
@@ -373,9 +384,17 @@ int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt, int block) 
 // (written by the selection phase, count in *d_count).
 static hipError_t launch_compact_gather(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt, int block,
                                         const uint32_t* idx, const int64_t* d_count, int64_t capacity,
-                                        hipStream_t stream) {
+                                        hipStream_t stream, bool lf) {
   if (plan.split >= plan.n_ops) return hipSuccess;
   int NR, RR;
+  if (lf && pick(nreg, rpt, block, plan.n_spill, &NR, &RR, true)) {
+#define X(N, RPT, BL)                                                                                    \
+  if (NR == N && RR == RPT && block == BL)                                                               \
+    return launch_gather_t<N, RPT, BL, true>(d_plan, plan.num_rows < capacity ? plan.num_rows : capacity, \
+                                             idx, d_count, capacity, plan.n_spill, stream);
+    BSQL_LF_VARIANTS(X)
+#undef X
+  }
   if (!pick(nreg, rpt, block, plan.n_spill, &NR, &RR)) return hipErrorInvalidValue;
 #define X(N, RPT, BL)                                                                                         \
   if (NR == N && RR == RPT && block == BL)                                                                    \
@@ -433,10 +452,10 @@ static hipError_t launch_fused(const DevPlan& plan, const DevPlan* d_plan, uint6
 
 hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, int sel_rpt, int rpt, int block,
                           uint64_t* sel, uint32_t* idx, int64_t* d_count, int64_t capacity,
-                          hipStream_t stream, int phases, bool fused) {
+                          hipStream_t stream, int phases, bool fused, bool lf) {
   if (fused && phases == 3) return launch_fused(plan, d_plan, sel, d_count, capacity, stream);
   if (!(phases & 1))
-    return launch_compact_gather(plan, d_plan, nreg, rpt, block, idx, d_count, capacity, stream);
+    return launch_compact_gather(plan, d_plan, nreg, rpt, block, idx, d_count, capacity, stream, lf);
   int64_t rows;
   int ballots;
   if (!slice_shape(plan, nreg, sel_rpt, block, &rows, &ballots)) return hipErrorInvalidValue;
@@ -490,7 +509,7 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
     e = hipGetLastError();
   }
   if (e != hipSuccess || !(phases & 2)) return e;
-  return launch_compact_gather(plan, d_plan, nreg, rpt, block, idx, d_count, capacity, stream);
+  return launch_compact_gather(plan, d_plan, nreg, rpt, block, idx, d_count, capacity, stream, lf);
 }
 
 // Batched launch: `plans` (device) holds n_batches plans, `tile_start`

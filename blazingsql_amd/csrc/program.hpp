@@ -87,7 +87,8 @@ struct Program {
   std::vector<int32_t> used_columns;
   // conjunct order override (empty = plan order) and launch override
   std::vector<int32_t> conj_order;
-  bsql_launch_config launch{0, 0};
+  // (the interpreter's register file in LDS unless the caller clears it)
+  bsql_launch_config launch{0, 0, BSQL_LAUNCH_LDS_REGS};
   uint64_t seed = 0x5eed5eed12345678ULL;
   // columns built before the launch: step k is column n_table + k
   std::vector<Step> steps;

@@ -241,17 +241,20 @@ class Program:
             L.check(L.lib().bsql_program_set_conjunct_order(self._h, arr, len(order)))
 
     def set_launch(self, rows_per_thread=0, block_size=0, ranges=True, lazy_loads=False, spill_wide=False,
-                   fused=False, generated=False):
+                   fused=False, generated=False, lds_regs=True):
         """Launch shape for this program (block_size 0, 64 or 256); for a
         predicate also whether its column-vs-literal conjuncts run as range
         tests and whether conjunct columns are loaded lazily; spill_wide keeps
         at most 7 live values in registers and the rest in LDS slots; fused runs
         a range-test filter with at most 8 fixed-width outputs as one pass;
         generated runs the program through a kernel generated for it (op
-        fields and column types compiled in, hipRTC, cached per process)."""
+        fields and column types compiled in, hipRTC, cached per process);
+        lds_regs keeps the interpreter's register file in LDS (execute and a
+        filter's output phase; the default, faster on every workload measured),
+        lds_regs=False in VGPRs."""
         flags = ((0 if ranges else L.LAUNCH_NO_RANGES) | (L.LAUNCH_LAZY_LOADS if lazy_loads else 0) |
                  (L.LAUNCH_SPILL_WIDE if spill_wide else 0) | (L.LAUNCH_FUSED if fused else 0) |
-                 (L.LAUNCH_GENERATED if generated else 0))
+                 (L.LAUNCH_GENERATED if generated else 0) | (L.LAUNCH_LDS_REGS if lds_regs else 0))
         cfg = L.LaunchConfig(block_size, rows_per_thread, flags)
         L.check(L.lib().bsql_program_set_launch(self._h, ctypes.byref(cfg)))
 

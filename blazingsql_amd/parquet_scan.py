@@ -20,6 +20,8 @@ over the projected columns) and renamed by the scan's aliases. Decoding
 stays on the host: the predicate columns cross PCIe once, in their decoded
 form, and never round-trip through HBM as anything but the batch itself.
 """
+import threading
+
 import numpy as np
 
 from . import _lib as L
@@ -66,6 +68,7 @@ class ParquetScan:
         self.filterable = "filters" in table_scan
         self.read = []   # (file, row group) pairs read by the last batches() call
         self.decode_threads = 4  # row groups decoded ahead, each on a worker thread
+        self._local = threading.local()
 
     def metadata(self):
         """The row-group statistics table process_skipdata_for_table reads:
@@ -109,8 +112,17 @@ class ParquetScan:
 
     def _decode(self, fi, rg):
         """The projected columns of one row group, decoded by pyarrow (which
-        releases the GIL, so row groups decode on worker threads)."""
-        return self.files[fi].read_row_group(rg, columns=[self.names[p] for p in self.projects])
+        releases the GIL, so row groups decode on worker threads). A pyarrow
+        ParquetFile is not safe to read from several threads at once, so each
+        worker thread reads through its own reader of the file."""
+        import pyarrow.parquet as pq
+        readers = getattr(self._local, "readers", None)
+        if readers is None:
+            readers = self._local.readers = {}
+        f = readers.get(fi)
+        if f is None:
+            f = readers[fi] = pq.ParquetFile(self.paths[fi])
+        return f.read_row_group(rg, columns=[self.names[p] for p in self.projects])
 
     def _upload(self, t, copy_stream, compute_stream):
         """A decoded row group's columns staged in pinned memory and copied on

@@ -365,10 +365,14 @@ enum {
                                  predicate): a range-test predicate with
                                  fixed-width outputs and no RAND runs as one
                                  pass. Other calls run on the interpreter. */
-  BSQL_LAUNCH_LDS_REGS = 32   /* batched launches: the interpreter's register
-                                 file in LDS (one address add and one LDS
-                                 access per operand) instead of VGPRs
-                                 (indexed register moves) */
+  BSQL_LAUNCH_LDS_REGS = 32   /* the interpreter's register file in LDS (one
+                                 address add and one LDS access per operand)
+                                 instead of VGPRs (indexed register moves):
+                                 batched launches, bsql_interp_execute and a
+                                 filter's output phase (set on either
+                                 program). A compiled program starts with
+                                 this flag set; a configuration without it
+                                 puts the file in VGPRs. */
 };
 typedef struct bsql_launch_config {
   int32_t block_size;     /* threads per workgroup: 0 (auto), 64 or 256 */

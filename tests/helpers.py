@@ -158,12 +158,15 @@ class ExprGen:
         return c if c else self.boolean(depth - 1)
 
 
-def eval_program_both(bsql, prog_text_list, table, n=None):
-    """Compiles expressions, runs them on the GPU and in the oracle; returns pairs."""
+def eval_program_both(bsql, prog_text_list, table, n=None, launch=None):
+    """Compiles expressions, runs them on the GPU (with `launch`, keyword
+    arguments of Program.set_launch) and in the oracle; returns pairs."""
     import oracle
     from blazingsql_amd.interop import DeviceColumn, perform_interpreter_operation
     types = [t for t, _, _ in table]
     p = bsql.Program.compile(prog_text_list, types)
+    if launch:
+        p.set_launch(**launch)
     out_types = [p.output_type(types, k) for k in range(p.num_outputs)]
     n = len(table[0][1]) if n is None else n
     dev_in = device_columns(table)

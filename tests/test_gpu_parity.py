@@ -134,8 +134,11 @@ def test_golden_strings(bsql, gpu, golden):
 TYPES6 = [INT32, INT64, FLOAT64, FLOAT64, TS_D, INT32]
 
 
+@pytest.mark.parametrize("lds", [False, True])
 @pytest.mark.parametrize("seed", range(6))
-def test_random_expressions_vs_oracle(bsql, gpu, seed):
+def test_random_expressions_vs_oracle(bsql, gpu, seed, lds):
+    """Random programs against the oracle, with the interpreter's register
+    file in VGPRs or (lds) in LDS."""
     rng = np.random.default_rng(100 + seed)
     types = TYPES6 + [BOOL8]
     n = [1, 31, 257, 1025, 4099, 70001][seed]
@@ -148,7 +151,7 @@ def test_random_expressions_vs_oracle(bsql, gpu, seed):
             exprs.append(e)
     for e in exprs:
         try:
-            res = eval_program_both(bsql, [e], table, n)
+            res = eval_program_both(bsql, [e], table, n, launch={"lds_regs": lds})
         except Exception as err:  # programs the interpreter rejects must be rejected on compile
             assert "bsql error 2" in str(err), (e, err)
             continue
@@ -169,8 +172,9 @@ C1_PRED = "AND(>($2, 50), <($1, 0.3:DECIMAL(2, 1)))"
 C1_PROJ = ["+($0, *($2, 3))", "-(*($1, 2.0:DECIMAL(2, 1)), 1.0:DECIMAL(2, 1))"]
 
 
+@pytest.mark.parametrize("lds", [False, True])
 @pytest.mark.parametrize("n", [0, 1, 1000, 1 << 20])
-def test_filter_project_c1_vs_oracle(bsql, gpu, n):
+def test_filter_project_c1_vs_oracle(bsql, gpu, n, lds):
     import oracle
     import torch
     from blazingsql_amd.interop import DeviceColumn, filter_project
@@ -178,6 +182,7 @@ def test_filter_project_c1_vs_oracle(bsql, gpu, n):
     types = [t for t, _, _ in table]
     pred = bsql.Program.compile(C1_PRED, types)
     proj = bsql.Program.compile(C1_PROJ, types)
+    proj.set_launch(lds_regs=lds)  # the output phase's register file in LDS or in VGPRs
     outs = [DeviceColumn.empty(INT64, n), DeviceColumn.empty(FLOAT64, n)]
     cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
     filter_project(pred, proj, device_columns(table), outs, cnt, n)

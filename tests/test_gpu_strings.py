@@ -131,8 +131,9 @@ def test_filter_with_string_columns_vs_oracle(bsql, gpu, ranges):
             _same(outs[k].to_numpy(m), ores[k], f"{pred_text} col {k}")
 
 
+@pytest.mark.parametrize("lds", [False, True])
 @pytest.mark.parametrize("n", [5000, 200_003])
-def test_strings_math_queries_vs_oracle(bsql, gpu, n):
+def test_strings_math_queries_vs_oracle(bsql, gpu, n, lds):
     """The strings+math workload's four filter+project queries (LIKE prefix,
     CHAR_LENGTH, SUBSTRING equality, their AND; SIN/LN/POWER/FLOOR, a wrapping
     int64 product, int64 division by a column with zeros) against the
@@ -157,6 +158,7 @@ def test_strings_math_queries_vs_oracle(bsql, gpu, n):
     host = [oracle.HostColumn.strings_from_arrays(offs, chars), oracle.HostColumn(INT64, x),
             oracle.HostColumn(FLOAT64, y), oracle.HostColumn(INT64, z)]
     proj = bsql.Program.compile(bench.STR_PROJ, types)
+    proj.set_launch(lds_regs=lds)  # the output phase's register file in LDS or in VGPRs
     ot = [proj.output_type(types, k) for k in range(proj.num_outputs)]
     assert ot == [FLOAT64, FLOAT64, FLOAT64, FLOAT64, INT64, INT64]
     cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
