@@ -21,7 +21,7 @@ namespace bsql {
 
 hipError_t launch_execute(const DevPlan& plan, const DevPlan* d_plan, int nreg, int rpt, int block,
                           hipStream_t stream, bool lf);
-int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt, int block);
+int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt, int block, bool lf);
 int64_t string_tiles(int64_t capacity);
 int batch_tile_rows(int nreg, int rpt, int block, bool lf);
 bool variant_fits(int nreg, int rpt, int block, int n_spill, bool lf = false);
@@ -479,7 +479,7 @@ int run_compact(bsql_program* pred, bsql_program* proj, const bsql_column* in, i
     e = launch_generated_filter_project(plan, L.ops, nullptr, nullptr, d_count, capacity, nullptr);
     return e == hipSuccess ? BSQL_OK : device_error(e, "generated kernel check");
   }
-  const int64_t sel_words = std::max({select_scratch_words(plan, L.nreg, sel_rpt, block),
+  const int64_t sel_words = std::max({select_scratch_words(plan, L.nreg, sel_rpt, block, lds_regs(pred, proj)),
                                       (int64_t)fused_scratch_words(num_rows),
                                       generated_filter_project_scratch_words(num_rows)});
   if (sel_words < 0) fail("no kernel variant for this program");
@@ -689,7 +689,7 @@ int run_filter_project_noalloc(bsql_program* pred, bsql_program* proj, const bsq
   launch_shape(L, rows_per_thread(proj), block_size(proj), &rpt, &block);
   if (plan.n_ranges == 0) launch_shape(L, sel_rpt, block_size(pred), &sel_rpt, &sel_block);
   if (sel_block != block) block = sel_block;
-  const int64_t sel_words = std::max(select_scratch_words(plan, L.nreg, sel_rpt, block),
+  const int64_t sel_words = std::max(select_scratch_words(plan, L.nreg, sel_rpt, block, lds_regs(pred, proj)),
                                      (int64_t)fused_scratch_words(num_rows));
   if (sel_words < 0) fail("no kernel variant for this program");
   const bool fused = (pred->launch.flags & BSQL_LAUNCH_FUSED) && fused_eligible(plan, L.nreg);

@@ -49,6 +49,10 @@ enum DevCode : uint8_t {
   // compiled pattern at imm; SLICE with aux2 = 0 from char b (0-based), 1 the
   // first b chars, 2 the last b chars
   D_LIKE, D_SLICE,
+  // LIKE 'abc%' (aux2 = 0) or LIKE 'abc' (aux2 = 1) with a literal of at most
+  // 7 bytes: imm holds the bytes (byte j at bits 8j) and their count at bits
+  // 56-63; a = slice of the chars of column aux
+  D_LIKE_PREFIX,
   // TRIM: the slice without the leading (aux2 bit 0) / trailing (bit 1)
   // characters found in the literal at imm ({int32 len, bytes}; empty: those
   // <= ' '); STR_TO_I64 / STR_TO_F64: the slice parsed as a number

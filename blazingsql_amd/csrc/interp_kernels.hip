@@ -255,11 +255,12 @@ template <int R> struct VmChunks { static constexpr int S = 16 / R; };
 constexpr int RANGE_R = 4;
 static int range_chunks_of(int sel_rpt) { return sel_rpt == 8 ? 2 : (sel_rpt == 32 ? 8 : 4); }
 
-template <int NREG, int R, int B>
+template <int NREG, int R, int B, bool LF = false>
 static hipError_t launch_select_vm_t(const DevPlan* d_plan, const SelArgs& sa, int64_t grid, int n_spill,
                                      hipStream_t stream) {
-  hipLaunchKernelGGL((select_vm<NREG, R, VmChunks<R>::S, B>), dim3((unsigned)grid), dim3(B),
-                     (unsigned)spill_lds_bytes(n_spill, B, R), stream, d_plan, sa);
+  const unsigned lds = (unsigned)(spill_lds_bytes(n_spill, B, R) + (LF ? regfile_lds_bytes(NREG, B, R) : 0));
+  hipLaunchKernelGGL((select_vm<NREG, R, VmChunks<R>::S, B, PlanSrc, LF>), dim3((unsigned)grid), dim3(B), lds,
+                     stream, d_plan, sa);
   return hipGetLastError();
 }
 
@@ -341,18 +342,26 @@ static int64_t resident_workgroups(const void* kernel, int block, size_t lds) {
 // Workgroups of the selection launch (and of its slice_ids): no more than are
 // resident at once, so that no second round of workgroups trails the first
 // (each takes a contiguous block of slices, ceil(slices / grid) of them).
-static int64_t selection_grid(const DevPlan& plan, int nreg, int sel_rpt, int block, int64_t n_slices) {
+static int64_t selection_grid(const DevPlan& plan, int nreg, int sel_rpt, int block, int64_t n_slices, bool lf) {
   const void* k = nullptr;
   int wg = BLOCK;
   size_t lds = 0;
+  int NR, RR;
   if (plan.n_ranges > 0) {
     switch (range_chunks_of(sel_rpt)) {
       case 2: k = reinterpret_cast<const void*>(&select_ranges<RANGE_R, 2>); break;
       case 8: k = reinterpret_cast<const void*>(&select_ranges<RANGE_R, 8>); break;
       default: k = reinterpret_cast<const void*>(&select_ranges<RANGE_R, 4>);
     }
+  } else if (lf && pick(nreg, sel_rpt, block, plan.n_spill, &NR, &RR, true)) {
+    wg = block;
+    lds = (size_t)(spill_lds_bytes(plan.n_spill, block, RR) + regfile_lds_bytes(NR, block, RR));
+#define X(N, RPT, BL)                      \
+  if (NR == N && RR == RPT && block == BL) \
+    k = reinterpret_cast<const void*>(&select_vm<N, RPT, VmChunks<RPT>::S, BL, PlanSrc, true>);
+    BSQL_LF_VARIANTS(X)
+#undef X
   } else {
-    int NR, RR;
     if (pick(nreg, sel_rpt, block, plan.n_spill, &NR, &RR)) {
       wg = block;
       lds = (size_t)spill_lds_bytes(plan.n_spill, block, RR);
@@ -371,12 +380,14 @@ static int64_t selection_grid(const DevPlan& plan, int nreg, int sel_rpt, int bl
   return grid;
 }
 
-int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt, int block) {
+// (lf: the selection kernel the launch will use, whose residency sizes the
+// grid and so the block totals; the same value as the launch's)
+int64_t select_scratch_words(const DevPlan& plan, int nreg, int rpt, int block, bool lf) {
   int64_t rows;
   int ballots;
   if (!slice_shape(plan, nreg, rpt, block, &rows, &ballots)) return -1;
   const int64_t n = (plan.num_rows + rows - 1) / rows;
-  const int64_t grid = selection_grid(plan, nreg, rpt, block, n);
+  const int64_t grid = selection_grid(plan, nreg, rpt, block, n, lf);
   return n * ballots + (n + 2 * grid + 1) / 2;
 }
 
@@ -460,7 +471,7 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
   int ballots;
   if (!slice_shape(plan, nreg, sel_rpt, block, &rows, &ballots)) return hipErrorInvalidValue;
   const int64_t n = (plan.num_rows + rows - 1) / rows;
-  const int64_t grid = selection_grid(plan, nreg, sel_rpt, block, n);
+  const int64_t grid = selection_grid(plan, nreg, sel_rpt, block, n, lf);
   uint32_t* counts = (uint32_t*)(sel + n * ballots);
   SelArgs sa{sel, counts, counts + n, counts + n + grid, n, grid ? (n + grid - 1) / grid : 0};
   hipError_t e = hipSuccess;
@@ -481,11 +492,19 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
       e = hipGetLastError();
     } else {
       int NR, RR;
-      if (!pick(nreg, sel_rpt, block, plan.n_spill, &NR, &RR)) return hipErrorInvalidValue;
+      if (lf && pick(nreg, sel_rpt, block, plan.n_spill, &NR, &RR, true)) {
+#define X(N, RPT, BL)                      \
+  if (NR == N && RR == RPT && block == BL) \
+    e = launch_select_vm_t<N, RPT, BL, true>(d_plan, sa, grid, plan.n_spill, stream);
+        BSQL_LF_VARIANTS(X)
+#undef X
+      } else {
+        if (!pick(nreg, sel_rpt, block, plan.n_spill, &NR, &RR)) return hipErrorInvalidValue;
 #define X(N, RPT, BL) \
   if (NR == N && RR == RPT && block == BL) e = launch_select_vm_t<N, RPT, BL>(d_plan, sa, grid, plan.n_spill, stream);
-      BSQL_VARIANTS(X)
+        BSQL_VARIANTS(X)
 #undef X
+      }
     }
     if (e != hipSuccess) return e;
   }
@@ -497,8 +516,11 @@ hipError_t launch_compact(const DevPlan& plan, const DevPlan* d_plan, int nreg, 
           : chunks == 8 ? launch_slice_ids<RANGE_R, 8>(sa, idx, d_count, grid, stream)
                         : launch_slice_ids<RANGE_R, 4>(sa, idx, d_count, grid, stream);
     } else {
+      // the ballot layout of the selection kernel that ran: its rows per lane
       int NR, RR;
-      if (!pick(nreg, sel_rpt, block, plan.n_spill, &NR, &RR)) return hipErrorInvalidValue;
+      if (!(lf && pick(nreg, sel_rpt, block, plan.n_spill, &NR, &RR, true)) &&
+          !pick(nreg, sel_rpt, block, plan.n_spill, &NR, &RR))
+        return hipErrorInvalidValue;
       e = RR == 1 ? launch_slice_ids<1, VmChunks<1>::S>(sa, idx, d_count, grid, stream)
           : RR == 2 ? launch_slice_ids<2, VmChunks<2>::S>(sa, idx, d_count, grid, stream)
                     : launch_slice_ids<4, VmChunks<4>::S>(sa, idx, d_count, grid, stream);

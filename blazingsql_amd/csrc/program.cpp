@@ -435,6 +435,26 @@ SourceOp translate(Lowering& L, const Program& p, const OpTypes& t, size_t i,
         d.code = D_LIKE;
         d.aux = (uint8_t)str_col_of(L, s.va);
         d.imm = str_base_b;  // the pattern, compiled when the literals are uploaded
+        {
+          // a literal prefix ('abc%') or a whole literal ('abc') of at most 7
+          // bytes: compared in place (the same strings match as under the
+          // matcher: anchored at the start, the trailing '%' takes any rest)
+          enum { ANCHOR_START = 1, ANCHOR_END = 2, MATCH_ALL = 4, REGEX = 8 };
+          const std::vector<int32_t> u = like_compile(p.rsc[i].str);
+          const int32_t nu = u[0], fl = u[1];
+          int32_t k = 0;
+          while (k < nu && u[(size_t)(2 + k)] >= 0) ++k;
+          const bool plain = (fl & ANCHOR_START) && !(fl & (MATCH_ALL | REGEX));
+          const bool prefix = plain && !(fl & ANCHOR_END) && k == nu - 1 && u[(size_t)(2 + k)] == -2;
+          const bool whole = plain && (fl & ANCHOR_END) && k == nu;
+          if ((prefix || whole) && k <= 7) {
+            uint64_t packed = (uint64_t)k << 56;
+            for (int32_t j = 0; j < k; ++j) packed |= (uint64_t)(uint8_t)u[(size_t)(2 + j)] << (8 * j);
+            d.code = D_LIKE_PREFIX;
+            d.imm = (int64_t)packed;
+            d.aux2 = whole ? 1 : 0;
+          }
+        }
         break;
       case BSQL_OP_STR_SUBSTRING: case BSQL_OP_STR_LEFT: case BSQL_OP_STR_RIGHT: {
         if (!is_string_type(lt) || l < 0) fail("string function on a non-string operand");
@@ -477,10 +497,10 @@ SourceOp translate(Lowering& L, const Program& p, const OpTypes& t, size_t i,
       }
       default: fail("operator " + std::to_string(op) + " is not supported by the interpreter");
     }
-    if (is_string_type(lt) && !((d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE ||
+    if (is_string_type(lt) && !((d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE || d.code == D_LIKE_PREFIX ||
                                 d.code == D_SLICE || d.code == D_TRIM || d.code == D_STR_TO_TS))
       fail("string operand to a non-string operator");
-    if (is_string_type(rt) && !((d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE ||
+    if (is_string_type(rt) && !((d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE || d.code == D_LIKE_PREFIX ||
                                 d.code == D_TRIM || d.code == D_STR_TO_TS))
       fail("string operand to a non-string operator");
   } else if (r == BSQL_UNARY_INDEX) {
@@ -1103,7 +1123,8 @@ std::vector<LOp> assign_registers(const std::vector<LOp>& ops, size_t n_values, 
 // Table columns an op reads (loads and the chars of string operands).
 void op_columns(const DevOp& d, std::vector<int>& cols) {
   if (d.code == D_LOAD || d.code == D_LOADS) cols.push_back(d.a);
-  const bool str_aux = (d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE || d.code == D_SLICE ||
+  const bool str_aux = (d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE || d.code == D_LIKE_PREFIX ||
+                       d.code == D_SLICE ||
                        d.code == D_TRIM || d.code == D_STR_TO_I64 || d.code == D_STR_TO_F64 ||
                        d.code == D_STR_TO_TS || d.code == D_CHAR_LENGTH;
   if (str_aux && d.aux != 0xff) cols.push_back(d.aux);
@@ -1112,7 +1133,8 @@ void op_columns(const DevOp& d, std::vector<int>& cols) {
 
 void remap_columns(DevOp& d, const std::vector<int>& map) {
   if (d.code == D_LOAD || d.code == D_LOADS) d.a = (uint8_t)map[d.a];
-  const bool str_aux = (d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE || d.code == D_SLICE ||
+  const bool str_aux = (d.code >= D_EQ_SS && d.code <= D_GE_SS) || d.code == D_LIKE || d.code == D_LIKE_PREFIX ||
+                       d.code == D_SLICE ||
                        d.code == D_TRIM || d.code == D_STR_TO_I64 || d.code == D_STR_TO_F64 ||
                        d.code == D_STR_TO_TS || d.code == D_CHAR_LENGTH;
   if (str_aux && d.aux != 0xff) d.aux = (uint8_t)map[d.aux];

@@ -353,10 +353,25 @@ static __device__ int32_t utf8_offset(const GLOBAL uint8_t* s, int32_t n, int64_
   }
   return n;
 }
+// Characters of the UTF-8 bytes s[0, n): the bytes that are not continuation
+// bytes (10xxxxxx), counted four at a time from the aligned 32-bit words that
+// cover them, the bytes of those words outside [s, s + n) masked off (an
+// aligned word lies in the page of the byte it was read for).
 static __device__ int64_t utf8_chars(const GLOBAL uint8_t* s, int32_t n) {
-  int64_t c = 0;
-  for (int32_t i = 0; i < n; ++i) c += utf8_lead(s[i]);
-  return c;
+  if (n <= 0) return 0;
+  const uintptr_t a = (uintptr_t)s;
+  const GLOBAL uint32_t* w = (const GLOBAL uint32_t*)(a & ~(uintptr_t)3);
+  const int32_t lead = (int32_t)(a & 3);
+  const int32_t end = lead + n;  // bytes from w[0]
+  int32_t cont = 0;
+  for (int32_t off = 0; off < end; off += 4) {
+    const uint32_t x = w[off >> 2];
+    uint32_t m = x & ~(x << 1) & 0x80808080u;  // bit 7 of every continuation byte
+    if (off == 0) m &= 0xFFFFFFFFu << (8 * lead);
+    if (end - off < 4) m &= (1u << (8 * (end - off))) - 1u;
+    cont += __popc(m);
+  }
+  return (int64_t)n - cont;
 }
 
 // The UTF-8 character (its encoded bytes packed little-endian) at s[i].
@@ -816,6 +831,31 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
           int32_t n;
           str_at(plan, op.aux, A[k], &p, &n);
           D[k] = like_match(p, n, pat) ? 1 : 0;
+          VD |= 1u << k;
+        }
+      }
+      return;
+    }
+    case D_LIKE_PREFIX: {
+      const int kb = (int)((uint64_t)op.imm >> 56);
+      const uint64_t want = (uint64_t)op.imm & 0x00FFFFFFFFFFFFFFull;
+      const bool whole = op.aux2 != 0;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        D[k] = 0;
+        if (((rs.exist >> k) & 1u) && ((VA >> k) & 1u)) {
+          const GLOBAL uint8_t* p;
+          int32_t n;
+          str_at(plan, op.aux, A[k], &p, &n);
+          bool m = whole ? n == kb : n >= kb;
+          if (m) {
+            uint64_t got = 0;
+#pragma unroll
+            for (int j = 0; j < 7; ++j)
+              if (j < kb) got |= (uint64_t)p[j] << (8 * j);
+            m = got == want;
+          }
+          D[k] = m ? 1 : 0;
           VD |= 1u << k;
         }
       }
@@ -2801,7 +2841,9 @@ __global__ __launch_bounds__(64 * W) void filter_project_batch(MultiArgs ma) {
 
 // Row selection by ops [0, split) (KILL per conjunct with a wave-uniform
 // early exit, or one SEL), chunk by chunk over the wave's slice.
-template <int NREG, int R, int S, int B, class P = PlanSrc>
+// LF: the register file in LDS (after the spill slots), a LOADN and its
+// COMMITs run as one step.
+template <int NREG, int R, int S, int B, class P = PlanSrc, bool LF = false>
 __global__ __launch_bounds__(B) void select_vm(const DevPlan* __restrict__ pplan, SelArgs sa) {
   typedef typename RegVecT<NREG>::type RegVec;
   typedef SelectTile<R, S> T;
@@ -2809,14 +2851,20 @@ __global__ __launch_bounds__(B) void select_vm(const DevPlan* __restrict__ pplan
   const DevPlan& plan = *pplan;
   const int lane = threadIdx.x & 63;
   LDS uint64_t* sp = wave_spill<R>((LDS uint64_t*)s_spill_area, plan.n_spill);
+  LDS uint64_t* lfp = lds_file<NREG, R, B, LF>(plan.n_spill);
   const int64_t s0 = (int64_t)blockIdx.x * sa.per;
   const int64_t s1 = s0 + sa.per < sa.n_slices ? s0 + sa.per : sa.n_slices;
   uint32_t wave_total = 0;
   const int split = P::split(plan);
   for (int64_t slice = s0 + (threadIdx.x >> 6); slice < s1; slice += NW) {
-    RegVec f[R];
+    RegVec fv[R];
 #pragma unroll
-    for (int k = 0; k < R; ++k) f[k] = (RegVec)(0);
+    for (int k = 0; k < R; ++k) fv[k] = (RegVec)(0);
+    LRegs<NREG, R> fl{lfp};
+    auto& f = [&]() -> auto& {
+      if constexpr (LF) return fl;
+      else return fv;
+    }();
     uint64_t vb = 0;
     LoadTemps<R> L;
     uint32_t alive_all = 0;
@@ -2830,6 +2878,14 @@ __global__ __launch_bounds__(B) void select_vm(const DevPlan* __restrict__ pplan
           DevOp next = load_op(plan, 0);
           for (int q = 0; q < split; ++q) {
             const DevOp op = next;
+            if constexpr (LF) {
+              if (op.code == D_LOADN) {
+                lds_loads<NREG, R, true, P>(plan, op, q, f, vb, rs);
+                q += op.aux;
+                next = load_op(plan, q + 1 < split ? q + 1 : q);
+                continue;
+              }
+            }
             next = load_op(plan, q + 1 < split ? q + 1 : q);
             if (vm_step<NREG, R, true, P>(plan, op, f, vb, L, rs, alive, sp)) break;
           }

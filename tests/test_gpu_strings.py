@@ -501,3 +501,34 @@ def test_regexp_replace_deep_matches(bsql, gpu):
     assert list(ok) == [True, True, False, True, True]
     for i in (0, 1, 3, 4):
         assert vals[i] == ores[0][0][i], (i, vals[i], ores[0][0][i])
+
+
+@pytest.mark.parametrize("n", [1, 3000, 70001])
+def test_like_prefix_and_char_length_fast_paths_vs_oracle(bsql, gpu, n):
+    """LIKE with a literal prefix ('ab%') or a whole literal ('ab') of up to 7
+    bytes is compared in place, and CHAR_LENGTH counts lead bytes four at a
+    time from aligned words: against the oracle on strings at every byte
+    alignment, with multi-byte UTF-8, empty strings, nulls, and patterns
+    just inside and outside the fast path (8 bytes, '_', '%' first)."""
+    import oracle
+    from blazingsql_amd.interop import DeviceColumn
+    rng = np.random.default_rng(n)
+    alphabet = ["a", "b", "é", "ab", "€", "x", ""]
+    words = ["".join(rng.choice(alphabet, int(rng.integers(0, 9)))) for _ in range(n)]
+    valid = rng.random(n) > 0.1
+    col = DeviceColumn.from_strings(words, valid)
+    exprs = ["LIKE($0, 'ab%')", "LIKE($0, 'ab')", "LIKE($0, 'a')", "LIKE($0, 'é%')", "LIKE($0, 'abababa%')",
+             "LIKE($0, 'abababab%')", "LIKE($0, 'a_%')", "LIKE($0, '%ab')", "LIKE($0, '')",
+             "CHAR_LENGTH($0)", "CHAR_LENGTH(SUBSTRING($0, 2, 5))"]
+    p = bsql.Program.compile(exprs, [STRING])
+    ot = [p.output_type([STRING], k) for k in range(p.num_outputs)]
+    outs = [DeviceColumn.empty(t, n) for t in ot]
+    from blazingsql_amd.interop import perform_interpreter_operation
+    perform_interpreter_operation(p, [col], outs, n)
+    host = [oracle.HostColumn(STRING, valid=valid, strings=words)]
+    ores = oracle.evaluate(p, host, n)
+    for k, e in enumerate(exprs):
+        gv, gval = outs[k].to_numpy(n)
+        np.testing.assert_array_equal(gval, ores[k][1], err_msg=e)
+        np.testing.assert_array_equal(np.asarray(gv)[gval].astype(np.int64),
+                                      np.asarray(ores[k][0])[gval].astype(np.int64), err_msg=e)
