@@ -343,13 +343,27 @@ __device__ __forceinline__ int32_t utf8_step(const GLOBAL uint8_t* s, int32_t i,
   return j - i;
 }
 // byte offset of character `chars` of s (n bytes if s has fewer characters)
+// (four bytes at a time from the aligned 32-bit words that cover s[0, n),
+// the bytes outside it masked off as in utf8_chars)
 static __device__ int32_t utf8_offset(const GLOBAL uint8_t* s, int32_t n, int64_t chars) {
+  if (n <= 0 || chars < 0) return n;
+  const uintptr_t a = (uintptr_t)s;
+  const GLOBAL uint32_t* w = (const GLOBAL uint32_t*)(a & ~(uintptr_t)3);
+  const int32_t lead = (int32_t)(a & 3);
+  const int32_t end = lead + n;
   int64_t c = 0;
-  for (int32_t i = 0; i < n; ++i) {
-    if (utf8_lead(s[i])) {
-      if (c == chars) return i;
-      ++c;
+  for (int32_t off = 0; off < end; off += 4) {
+    const uint32_t x = w[off >> 2];
+    uint32_t m = ~(x & ~(x << 1)) & 0x80808080u;  // bit 7 of every lead byte
+    if (off == 0) m &= 0xFFFFFFFFu << (8 * lead);
+    if (end - off < 4) m &= (1u << (8 * (end - off))) - 1u;
+    const int cnt = __popc(m);
+    if (c + cnt > chars) {
+      // the (chars - c)-th lead byte of this word
+      for (int64_t r = chars - c; r > 0; --r) m &= m - 1u;
+      return off + (__ffs(m) - 1) / 8 - lead;
     }
+    c += cnt;
   }
   return n;
 }
@@ -801,7 +815,9 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
           int32_t na, nb;
           str_at(plan, op.aux, A[k], &pa, &na);
           str_at(plan, op.aux2, B[k], &pb, &nb);
-          D[k] = (uint64_t)cmp_result(kind, str_cmp(pa, na, pb, nb));
+          // = and <> of strings of different lengths need no bytes
+          D[k] = kind <= 1 && na != nb ? (uint64_t)(kind == 1)
+                                       : (uint64_t)cmp_result(kind, str_cmp(pa, na, pb, nb));
           VD |= 1u << k;
         }
       }
