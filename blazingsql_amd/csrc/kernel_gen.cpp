@@ -69,6 +69,25 @@ const char* kPrelude =
 // 0.62 for (2, 4, 16), 0.68 for (2, 8, 4), 0.70 for (4, 4, 4)).
 constexpr int kRows = 2, kChunks = 4, kTileWaves = 8;
 constexpr int kTile = 64 * kRows * kChunks * kTileWaves;
+
+// A one-pass kernel's tile: `waves` wave slices of `chunks` chunks of 64
+// lanes x `rows` rows.
+struct TileShape {
+  int rows, chunks, waves;
+  int64_t slice() const { return 64LL * rows * chunks; }
+  int64_t tile() const { return slice() * waves; }
+};
+constexpr TileShape kOnePass{kRows, kChunks, kTileWaves};
+// The batched kernel's tile (8192 rows): every query's outputs load, compute
+// and stage once per chunk, and a query's output phase waits on its loads
+// before the next query's, so more rows per lane and fewer chunks keep more
+// loads in flight per wait. Measured over 8 queries on 64M copula rows
+// (rho 0 / 0.5 / 0.9, ms): (2, 4, 8) 3.70 / 3.84 / 3.71, (4, 2, 8) 2.94 /
+// 3.13 / 2.95, (8, 1, 8) 3.00 / 3.16 / 2.91, (8, 4, 4) 2.57 / 2.87 / 2.71,
+// (8, 2, 8) 2.52 / 2.91 / 2.66; staging 16 or 64 KB of rows instead of 32
+// KB ran no faster.
+constexpr TileShape kBatch{8, 2, 8};
+
 // LDS for a tile's staged output rows (8-byte slots): 32 KB holds 2048 rows
 // of two outputs, half of a tile; with the rest of the kernel's LDS a
 // workgroup stays within 64 KB. Measured on the 100M-row scan (12 % of
@@ -331,7 +350,8 @@ struct OnePassShape {
 // batch: for the batched kernel, whose selection is generated separately
 // (no range columns kept, no eager range loads).
 OnePassShape one_pass_shape(const DevPlan& plan, const std::vector<DevOp>& ops, const char* name,
-                            const std::vector<bool>* shared = nullptr, bool batch = false) {
+                            const std::vector<bool>* shared = nullptr, bool batch = false,
+                            TileShape ts = kOnePass) {
   // the output ops' columns that a range test reads: the widest one stays
   // in registers, the others are read again through the cache
   std::vector<bool> out_cols = loaded_columns(plan, ops, plan.split, plan.n_ops);
@@ -347,7 +367,7 @@ OnePassShape one_pass_shape(const DevPlan& plan, const std::vector<DevOp>& ops, 
   }
   if (keep >= 0) nt[(size_t)keep] = !(shared && (*shared)[(size_t)keep]);
   if (batch) keep = -1;
-  const int64_t slice = 64 * kRows * kChunks;
+  const int64_t slice = ts.slice();
   // every range column in flight at once when their elements fit the
   // registers (at most 16 bytes per row over the ranges)
   int range_bytes = 0;
@@ -357,17 +377,17 @@ OnePassShape one_pass_shape(const DevPlan& plan, const std::vector<DevOp>& ops, 
   const int vw = (int)(slice / 32 + 2);
   int cap = 0;
   if (plan.n_out > 0) {
-    cap = (int)std::min<int64_t>(kTile, kStageBytes / (8 * plan.n_out) / 64 * 64);
-    while (cap > 0 && fp_lds_bytes(plan.n_out, cap, kTileWaves, vw) > kMaxLds) cap -= 64;
+    cap = (int)std::min<int64_t>(ts.tile(), kStageBytes / (8 * plan.n_out) / 64 * 64);
+    while (cap > 0 && fp_lds_bytes(plan.n_out, cap, ts.waves, vw) > kMaxLds) cap -= 64;
   }
   OnePassShape sh;
   sh.source = source_struct(plan, ops, nt, keep, eager, cap, name);
-  sh.lds = fp_lds_bytes(plan.n_out, cap, kTileWaves, vw);
+  sh.lds = fp_lds_bytes(plan.n_out, cap, ts.waves, vw);
   return sh;
 }
 
-int64_t one_pass_tiles(int64_t num_rows) {
-  const int64_t tiles = (num_rows + kTile - 1) / kTile;
+int64_t one_pass_tiles(int64_t num_rows, TileShape ts = kOnePass) {
+  const int64_t tiles = (num_rows + ts.tile() - 1) / ts.tile();
   if (tiles > (int64_t)INT32_MAX) fail("a generated filter handles at most 2^31-1 tiles per call");
   return tiles;
 }
@@ -472,7 +492,8 @@ hipError_t launch_generated_filter_project_batch(const std::vector<DevPlan>& pla
                                                  hipStream_t stream) {
   const int nq = (int)plans.size();
   if (nq < 1 || nq > BSQL_MAX_BATCH_QUERIES) fail("a batch holds 1 to 16 queries");
-  const int64_t tiles = one_pass_tiles(plans[0].num_rows);
+  const TileShape ts = kBatch;
+  const int64_t tiles = one_pass_tiles(plans[0].num_rows, ts);
   if (tiles == 0) return fill_words((uint64_t*)d_counts, nq, 0, stream);
   // who reads each table column: the selection (once, for all queries) and
   // each query's outputs; a column read again after the selection is
@@ -505,7 +526,7 @@ hipError_t launch_generated_filter_project_batch(const std::vector<DevPlan>& pla
     return first[a] != first[b] ? first[a] < first[b] : uses[a] > uses[b];
   });
   std::ostringstream name, body;
-  name << "&bsql_dev::filter_project_batch<16, " << kRows << ", " << kChunks << ", " << kTileWaves
+  name << "&bsql_dev::filter_project_batch<16, " << ts.rows << ", " << ts.chunks << ", " << ts.waves
        << ", bsql_gen::Batch";
   int64_t lds = 0;
   for (int q = 0; q < nq; ++q) {
@@ -513,7 +534,7 @@ hipError_t launch_generated_filter_project_batch(const std::vector<DevPlan>& pla
     const DevPlan& pl = plans[(size_t)q];
     std::vector<bool> shared((size_t)pl.n_in, false);
     for (int j = 0; j < pl.n_in; ++j) shared[(size_t)j] = readers[in_maps[(size_t)q][(size_t)j]] > 1;
-    const OnePassShape sh = one_pass_shape(pl, ops[(size_t)q], sname.c_str(), &shared, true);
+    const OnePassShape sh = one_pass_shape(pl, ops[(size_t)q], sname.c_str(), &shared, true, ts);
     body << sh.source;
     lds = std::max(lds, sh.lds);
     name << ", bsql_gen::" << sname;
@@ -532,12 +553,13 @@ hipError_t launch_generated_filter_project_batch(const std::vector<DevPlan>& pla
     ma.fa[q] = FusedArgs{status + (size_t)q * (size_t)tiles, nullptr, d_counts + q, capacity[(size_t)q], tiles};
   }
   void* args[] = {(void*)&ma};
-  return hipModuleLaunchKernel(fn, (unsigned)tiles, 1, 1, 64 * kTileWaves, 1, 1, (unsigned)lds, stream, args,
+  return hipModuleLaunchKernel(fn, (unsigned)tiles, 1, 1, 64 * ts.waves, 1, 1, (unsigned)lds, stream, args,
                                nullptr);
 }
 
 int64_t generated_filter_project_scratch_words(int64_t num_rows) {
-  return (num_rows + kTile - 1) / kTile + 1;
+  const int64_t tile = std::min(kOnePass.tile(), kBatch.tile());
+  return (num_rows + tile - 1) / tile + 1;
 }
 
 bool generated_execute_eligible(const DevPlan& plan, const std::vector<DevOp>& ops) {

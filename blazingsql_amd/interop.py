@@ -65,7 +65,8 @@ class Program:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if L.exiting():
+        exiting = getattr(L, "exiting", None)  # module globals are gone late in interpreter exit
+        if exiting is None or exiting():
             return
         if h is not None and h.value and getattr(self, "_owner", None) is None:
             try:
@@ -603,7 +604,8 @@ class Batch:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h and not L.exiting():
+        exiting = getattr(L, "exiting", None)  # module globals are gone late in interpreter exit
+        if h and exiting is not None and not exiting():
             L.lib().bsql_batch_free(h)
             self.handle = None
 

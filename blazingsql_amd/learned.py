@@ -75,7 +75,8 @@ class Mlp:
     def __del__(self):
         h = getattr(self, "handle", None)
         lib = getattr(L, "lib", None)   # module globals are gone at interpreter exit
-        if h and lib is not None and not L.exiting():
+        exiting = getattr(L, "exiting", None)
+        if h and lib is not None and exiting is not None and not exiting():
             lib().bsql_mlp_free(h)
             self.handle = None
 

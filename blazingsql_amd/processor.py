@@ -180,7 +180,8 @@ class Table:
             L.lib().bsql_table_free(self._release())
 
     def __del__(self):
-        if L.exiting():
+        exiting = getattr(L, "exiting", None)  # module globals are gone late in interpreter exit
+        if exiting is None or exiting():
             return
         try:
             self.free()
