@@ -217,3 +217,76 @@ def test_copula_learned_filter_project_vs_oracle(bsql, gpu, trained_mlp, rho):
         reordered += int(list(order) != list(range(len(terms))))
     assert max(selected) > 0.01, selected  # the workload selects rows, not only nothing
     assert reordered > 0
+
+
+@pytest.mark.parametrize("rho", [0.0, 0.5, 0.9])
+def test_copula_learned_batch_vs_oracle(bsql, gpu, trained_mlp, rho):
+    """The learned and batched form at 2M rows: a copula table's 8 queries
+    (4-8 range terms each, joint selectivity 0.001-0.9) in ONE
+    bsql_filter_project_batch pass with the learned conjunct orders and the
+    predicted row bounds (a query over its bound runs again on its own);
+    every query's count, outputs and validity bit for bit against the oracle
+    running it in the static order."""
+    import torch
+    import oracle
+    from blazingsql_amd import learned
+    from blazingsql_amd.interop import DeviceColumn
+    n = 2_000_003
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(71 + int(rho * 10))
+    cols = learned.copula_table(n, rho, gen)
+    host = [oracle.HostColumn(c.type_id, c.to_numpy(n)[0]) for c in cols]
+    types = learned.COPULA_TYPES
+    proj_text = ["+($0, $4)", "*($2, 2:DOUBLE)"]
+    rng = np.random.default_rng(int(rho * 10) + 41)
+    texts = [learned.copula_predicate(rng, int(rng.integers(4, 9)))[0] for _ in range(8)]
+    print(f"rho={rho}: batch of {len(texts)} queries", flush=True)
+    preds = [bsql.Program.compile(t, types) for t in texts]
+    projs = [bsql.Program.compile(proj_text, types) for _ in texts]
+    outs = [[DeviceColumn.empty(10, n), DeviceColumn.empty(10, n)] for _ in texts]
+    bcnt = torch.zeros(len(texts), dtype=torch.int64, device="cuda")
+    counts, _ = learned.filter_project_batch_learned(trained_mlp, preds, projs, types, cols, outs, bcnt, n)
+    proj = bsql.Program.compile(proj_text, types)
+    for q, text in enumerate(texts):
+        static = bsql.Program.compile(text, types)
+        om, ores = oracle.filter_project(oracle.full_table_program(static, 8), oracle.full_table_program(proj, 8),
+                                         host, [10, 10], n)
+        assert counts[q] == om, (text, counts[q], om)
+        for k in range(2):
+            gv, gval = outs[q][k].to_numpy(om)
+            assert np.array_equal(gval, ores[k][1]), f"validity of output {k} of {text}"
+            assert np.array_equal(gv.view(np.uint64), ores[k][0].view(np.uint64)), f"output {k} of {text}"
+    assert max(counts) > 0.01 * n, counts
+
+
+def test_copula_learned_batch_full_size_matches_calls(bsql, gpu, trained_mlp):
+    """The learned and batched form at its full size (64M rows, rho 0.9):
+    the one-pass batch selects and writes exactly what each query's own
+    bsql_filter_project call in the static order does (batching must not
+    change any row, at a size where the oracle is too slow to run)."""
+    import torch
+    from blazingsql_amd import learned
+    from blazingsql_amd.interop import DeviceColumn, filter_project
+    n = 1 << 26
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(97)
+    cols = learned.copula_table(n, 0.9, gen)
+    types = learned.COPULA_TYPES
+    proj_text = ["+($0, $4)", "*($2, 2:DOUBLE)"]
+    rng = np.random.default_rng(97)
+    texts = [learned.copula_predicate(rng, int(rng.integers(4, 9)))[0] for _ in range(8)]
+    preds = [bsql.Program.compile(t, types) for t in texts]
+    projs = [bsql.Program.compile(proj_text, types) for _ in texts]
+    outs = [[DeviceColumn.empty(10, n), DeviceColumn.empty(10, n)] for _ in texts]
+    bcnt = torch.zeros(len(texts), dtype=torch.int64, device="cuda")
+    counts, _ = learned.filter_project_batch_learned(trained_mlp, preds, projs, types, cols, outs, bcnt, n)
+    ref = [DeviceColumn.empty(10, n), DeviceColumn.empty(10, n)]
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    proj = bsql.Program.compile(proj_text, types)
+    for q, text in enumerate(texts):
+        filter_project(bsql.Program.compile(text, types), proj, cols, ref, cnt, n)
+        m = int(cnt.item())
+        assert counts[q] == m, (text, counts[q], m)
+        for k in range(2):
+            assert torch.equal(outs[q][k].data[:m].view(torch.int64), ref[k].data[:m].view(torch.int64)), (q, k)
+    assert max(counts) > 0.01 * n, counts
