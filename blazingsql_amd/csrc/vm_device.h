@@ -1254,24 +1254,27 @@ struct LoadTemps {
   uint32_t v0, v1, v2, v3;
 };
 
-// The register file in LDS, for plans read at run time: register i of the
-// lane's R rows is the R consecutive slots at p + i*64*R, so a dynamic
-// register number costs one address add and the access is one LDS load or
-// store of the lane's R rows (each lane touches only its own slots: no
-// barrier). The wave's file is NREG*64*R slots.
+// The register file in LDS, for plans read at run time: row k of register
+// i of lane t is slot (i*R + k)*64 + t of the wave's file (p points at the
+// lane's slot 0), so a dynamic register number costs one address add, the
+// R rows are R loads or stores at constant offsets, and the 64 lanes of one
+// access touch 64 consecutive slots, which no two lanes share a bank in
+// (rows of a lane side by side put lanes 8*R bytes apart: 2-way conflicts
+// for R = 4). Each lane touches only its own slots: no barrier. The wave's
+// file is NREG*64*R slots.
 template <int NREG, int R>
 struct LRegs {
   LDS uint64_t* p;
-  __device__ __forceinline__ uint64_t at(int k, int i) const { return p[i * (64 * R) + k]; }
+  __device__ __forceinline__ uint64_t at(int k, int i) const { return p[(i * R + k) * 64]; }
   __device__ __forceinline__ void get(int i, uint64_t (&v)[R]) const {
     const LDS uint64_t* q = p + i * (64 * R);
 #pragma unroll
-    for (int k = 0; k < R; ++k) v[k] = q[k];
+    for (int k = 0; k < R; ++k) v[k] = q[k * 64];
   }
   __device__ __forceinline__ void put(int i, const uint64_t (&v)[R]) {
     LDS uint64_t* q = p + i * (64 * R);
 #pragma unroll
-    for (int k = 0; k < R; ++k) q[k] = v[k];
+    for (int k = 0; k < R; ++k) q[k * 64] = v[k];
   }
 };
 template <class T> struct IsLRegs { static constexpr bool value = false; };
@@ -1659,11 +1662,9 @@ __device__ __forceinline__ LDS uint64_t* lds_file(int n_spill) {
   } else {
     const int lane = threadIdx.x & 63;
     LDS uint64_t* p = (LDS uint64_t*)s_spill_area + (int64_t)n_spill * (B / 64) * (64 * R + 8) +
-                      (int64_t)(threadIdx.x >> 6) * (NREG * 64 * R) + lane * R;
+                      (int64_t)(threadIdx.x >> 6) * (NREG * 64 * R) + lane;
 #pragma unroll
-    for (int i = 0; i < NREG; ++i)
-#pragma unroll
-      for (int k = 0; k < R; ++k) p[i * (64 * R) + k] = 0;
+    for (int i = 0; i < NREG * R; ++i) p[i * 64] = 0;
     return p;
   }
 }
