@@ -370,7 +370,9 @@ static __device__ int32_t utf8_offset(const GLOBAL uint8_t* s, int32_t n, int64_
 // Characters of the UTF-8 bytes s[0, n): the bytes that are not continuation
 // bytes (10xxxxxx), counted four at a time from the aligned 32-bit words that
 // cover them, the bytes of those words outside [s, s + n) masked off (an
-// aligned word lies in the page of the byte it was read for).
+// aligned word lies in the page of the byte it was read for). The words are
+// read four per step, their loads issued together (only words that cover a
+// byte of s are read).
 static __device__ int64_t utf8_chars(const GLOBAL uint8_t* s, int32_t n) {
   if (n <= 0) return 0;
   const uintptr_t a = (uintptr_t)s;
@@ -378,12 +380,18 @@ static __device__ int64_t utf8_chars(const GLOBAL uint8_t* s, int32_t n) {
   const int32_t lead = (int32_t)(a & 3);
   const int32_t end = lead + n;  // bytes from w[0]
   int32_t cont = 0;
-  for (int32_t off = 0; off < end; off += 4) {
-    const uint32_t x = w[off >> 2];
-    uint32_t m = x & ~(x << 1) & 0x80808080u;  // bit 7 of every continuation byte
-    if (off == 0) m &= 0xFFFFFFFFu << (8 * lead);
-    if (end - off < 4) m &= (1u << (8 * (end - off))) - 1u;
-    cont += __popc(m);
+  for (int32_t off = 0; off < end; off += 16) {
+    uint32_t x[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) x[j] = off + 4 * j < end ? w[(off >> 2) + j] : 0u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int32_t o = off + 4 * j;
+      uint32_t m = x[j] & ~(x[j] << 1) & 0x80808080u;  // bit 7 of every continuation byte
+      if (o == 0) m &= 0xFFFFFFFFu << (8 * lead);
+      if (end - o < 4) m &= end - o > 0 ? (1u << (8 * (end - o))) - 1u : 0u;
+      cont += __popc(m);
+    }
   }
   return (int64_t)n - cont;
 }
