@@ -392,7 +392,7 @@ def run_parquet(args):
             for _ in range(args.steps):
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
-                ps = ParquetScan([path], scan)
+                ps = ParquetScan([path], scan, decode_threads=args.decode_threads)
                 outs = list(ps.batches())
                 torch.cuda.synchronize()
                 dt = time.perf_counter() - t0
@@ -411,6 +411,7 @@ def run_parquet(args):
                       "steps": args.steps, "higher_is_better": True, "data": "synthetic",
                       "config": {"workload": f"{n} rows (a int64 sorted, b float64 10% null, c int32, d date32), "
                                              f"1M-row row groups, snappy", **res,
+                                 "decode_threads": args.decode_threads,
                                  "host_decode_only_seconds": decode,
                                  "host_decode_only_rows_per_s": n / decode}}))
 
@@ -762,6 +763,8 @@ def main():
                     help="run the scan through the interpreter's selection, row ids and output pass "
                          "instead of the kernel generated for the program")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--decode-threads", type=int, default=8,
+                    help="parquet workload: row groups decoded and staged ahead on worker threads")
     ap.add_argument("--no-learned", action="store_true",
                     help="leave out the reduced learned-and-batched measurement of the scan line")
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)

@@ -12,9 +12,9 @@ engine/src/execution_kernels/BatchProcessing.cpp:261-284).
 Here the statistics come from the file footer (pyarrow), the skip-data pass
 (the filter rewritten over the statistics and evaluated by the interpreter)
 runs on the GPU, the kept row groups' projected columns are decoded by
-pyarrow on the host (a few row groups ahead, on worker threads), staged in
-pinned memory and copied to the device on a copy stream while the previous
-row group is filtered on the compute stream,
+pyarrow on the host and staged in pinned memory (a few row groups ahead,
+on worker threads), and copied to the device on a copy stream while the
+previous row group is filtered on the compute stream,
 and each batch is filtered by the interpreter (the scan's filters=[[...]]
 over the projected columns) and renamed by the scan's aliases. Decoding
 stays on the host: the predicate columns cross PCIe once, in their decoded
@@ -53,7 +53,7 @@ class ParquetScan:
     """BindableTableScan over Parquet files: `table_scan` is the plan text
     (table=[[...]], filters=[[...]], projects=[[...]], aliases=[[...]])."""
 
-    def __init__(self, paths, table_scan):
+    def __init__(self, paths, table_scan, decode_threads=8):
         import pyarrow.parquet as pq
         self.paths = list(paths)
         self.files = [pq.ParquetFile(p) for p in self.paths]
@@ -67,7 +67,7 @@ class ParquetScan:
         self.projects = scan_projections(table_scan, len(self.names))
         self.filterable = "filters" in table_scan
         self.read = []   # (file, row group) pairs read by the last batches() call
-        self.decode_threads = 4  # row groups decoded ahead, each on a worker thread
+        self.decode_threads = max(1, int(decode_threads))  # row groups decoded ahead, each on a worker thread
         self._local = threading.local()
 
     def metadata(self):
@@ -111,10 +111,13 @@ class ParquetScan:
         return sorted(zip(fcol[0].tolist(), rcol[0].tolist()))
 
     def _decode(self, fi, rg):
-        """The projected columns of one row group, decoded by pyarrow (which
-        releases the GIL, so row groups decode on worker threads). A pyarrow
-        ParquetFile is not safe to read from several threads at once, so each
-        worker thread reads through its own reader of the file."""
+        """The projected columns of one row group, decoded by pyarrow and
+        staged in pinned host memory, on a worker thread (pyarrow's decode
+        and numpy's copies release the GIL, so row groups decode and stage
+        side by side). A pyarrow ParquetFile is not safe to read from several
+        threads at once, so each worker thread reads through its own reader
+        of the file. Returns ([(type, rows, values, validity, offsets)], rows)
+        with pinned CPU tensors (validity and offsets None when absent)."""
         import pyarrow.parquet as pq
         readers = getattr(self._local, "readers", None)
         if readers is None:
@@ -122,49 +125,36 @@ class ParquetScan:
         f = readers.get(fi)
         if f is None:
             f = readers[fi] = pq.ParquetFile(self.paths[fi])
-        return f.read_row_group(rg, columns=[self.names[p] for p in self.projects])
-
-    def _upload(self, t, copy_stream, compute_stream):
-        """A decoded row group's columns staged in pinned memory and copied on
-        `copy_stream`; the tensors are recorded on `compute_stream`, which
-        reads them."""
-        import torch
-        cols = [self.names[p] for p in self.projects]
-        out = []
-        with torch.cuda.stream(copy_stream):
-            for name, p in zip(cols, self.projects):
-                arr = t.column(name).combine_chunks()
-                out.append(self._column(arr, self.types[p], compute_stream))
-        return out, t.num_rows
+        t = f.read_row_group(rg, columns=[self.names[p] for p in self.projects])
+        staged = [self._stage(t.column(self.names[p]).combine_chunks(), self.types[p]) for p in self.projects]
+        return staged, t.num_rows
 
     @staticmethod
-    def _to_device(np_array, compute_stream):
+    def _pinned(np_array):
         import torch
         a = np.ascontiguousarray(np_array)
         host = torch.empty(a.shape, dtype=torch.from_numpy(np.zeros(0, a.dtype)).dtype, pin_memory=True)
         host.numpy()[...] = a
-        dev = host.to("cuda", non_blocking=True)
-        dev.record_stream(compute_stream)
-        return dev
+        return host
 
-    def _column(self, arr, tid, compute_stream):
-        """One arrow array as a device column (validity words from the arrow
-        bitmap, which has the same bit order; null rows' values are zeros)."""
+    def _stage(self, arr, tid):
+        """One arrow array in pinned memory: validity words from the arrow
+        bitmap (the same bit order; null rows' values are zeros), values, and
+        for strings the offsets rebased to the first row and its chars."""
         import pyarrow as pa
         n = len(arr)
         valid = None
         if arr.null_count:
             from .interop import pack_validity
             mask = np.asarray(arr.is_valid().to_numpy(zero_copy_only=False), dtype=bool)
-            valid = self._to_device(pack_validity(mask).view(np.int32), compute_stream)
+            valid = self._pinned(pack_validity(mask).view(np.int32))
         if tid == L.STRING:
             offs = np.frombuffer(arr.buffers()[1], dtype=np.int32)[arr.offset:arr.offset + n + 1].astype(np.int64)
             data = arr.buffers()[2]
             chars = np.frombuffer(data, dtype=np.uint8) if data is not None else np.zeros(0, np.uint8)
             lo, hi = (int(offs[0]), int(offs[-1])) if n else (0, 0)
             body = chars[lo:hi] if hi > lo else np.zeros(1, np.uint8)
-            return DeviceColumn(tid, self._to_device(body.copy(), compute_stream), valid,
-                                self._to_device((offs - lo).astype(np.int32), compute_stream), size=n)
+            return tid, n, self._pinned(body), valid, self._pinned((offs - lo).astype(np.int32))
         if tid == 12:
             arr = arr.cast(pa.int32())
         if tid == L.BOOL8:
@@ -172,7 +162,29 @@ class ParquetScan:
         else:
             filled = arr.fill_null(0) if arr.null_count else arr
             values = np.asarray(filled.to_numpy(zero_copy_only=False))
-        return DeviceColumn(tid, self._to_device(values, compute_stream), valid, size=n)
+        return tid, n, self._pinned(values), valid, None
+
+    @staticmethod
+    def _upload(staged, copy_stream, compute_stream):
+        """A staged row group's columns copied to the device on
+        `copy_stream`; the device tensors are recorded on `compute_stream`,
+        which reads them."""
+        import torch
+
+        def dev(host):
+            if host is None:
+                return None
+            d = host.to("cuda", non_blocking=True)
+            d.record_stream(compute_stream)
+            return d
+        out = []
+        with torch.cuda.stream(copy_stream):
+            for tid, n, values, valid, offsets in staged:
+                if tid == L.STRING:
+                    out.append(DeviceColumn(tid, dev(values), dev(valid), dev(offsets), size=n))
+                else:
+                    out.append(DeviceColumn(tid, dev(values), dev(valid), size=n))
+        return out
 
     def batches(self, stream=None):
         """The scan's output batches: for every row group the skip-data pass
@@ -192,10 +204,10 @@ class ParquetScan:
         for k in range(len(groups) + 1):
             nxt = None
             if k < len(groups):
-                t = ahead.pop(0).result()
+                staged, rows = ahead.pop(0).result()
                 if k + self.decode_threads < len(groups):
                     ahead.append(pool.submit(self._decode, *groups[k + self.decode_threads]))
-                cols, rows = self._upload(t, copy, compute)
+                cols = self._upload(staged, copy, compute)
                 ev = torch.cuda.Event()
                 ev.record(copy)
                 nxt = (cols, rows, ev)
