@@ -1710,12 +1710,19 @@ __global__ __launch_bounds__(B) void vm_batches(const DevPlan* __restrict__ plan
   const int64_t n_items = tile_start[n_batches];
   LDS uint64_t* sp = wave_spill<R>((LDS uint64_t*)s_spill_area, plans[0].n_spill);
   LDS uint64_t* lfp = lds_file<NREG, R, B, LF>(plans[0].n_spill);
+  int lo = 0;  // last batch whose first item <= item (tile_start[0] = 0)
   for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-    int lo = 0, hi = n_batches - 1;  // last batch whose first item <= item
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (tile_start[mid] <= item) lo = mid;
-      else hi = mid - 1;
+    // items only grow, so the batch is lo or a later one: one load tells
+    // (tile_start[n_batches] = n_items > item), a binary search over the
+    // later batches otherwise
+    if (tile_start[lo + 1] <= item) {
+      int hi = n_batches - 1;
+      ++lo;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= item) lo = mid;
+        else hi = mid - 1;
+      }
     }
     const DevPlan& plan = plans[lo];
     vm_tile<NREG, R, false, B, PlanSrc, LF>(plan, lfp, item - tile_start[lo], plan.num_rows, nullptr, lane, sp);
