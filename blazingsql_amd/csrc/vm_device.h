@@ -396,6 +396,46 @@ static __device__ int64_t utf8_chars(const GLOBAL uint8_t* s, int32_t n) {
   return (int64_t)n - cont;
 }
 
+// utf8_chars of a lane's R strings (n[k] <= 0: none, 0 characters) at once:
+// every string's next four words are read in one step, the loads of all R
+// strings issued together, so the lane takes as many steps as its longest
+// string needs rather than their sum.
+template <int R>
+__device__ __forceinline__ void utf8_chars_rows(const GLOBAL uint8_t* const (&s)[R], const int32_t (&n)[R],
+                                                int64_t (&chars)[R]) {
+  const GLOBAL uint32_t* w[R];
+  int32_t lead[R], end[R], cont[R];
+  int32_t most = 0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const uintptr_t a = (uintptr_t)s[k];
+    w[k] = (const GLOBAL uint32_t*)(a & ~(uintptr_t)3);
+    lead[k] = (int32_t)(a & 3);
+    end[k] = n[k] > 0 ? lead[k] + n[k] : 0;  // bytes from w[k][0]; 0: nothing read
+    cont[k] = 0;
+    most = end[k] > most ? end[k] : most;
+  }
+  for (int32_t off = 0; off < most; off += 16) {
+    uint32_t x[R][4];
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) x[k][j] = off + 4 * j < end[k] ? w[k][(off >> 2) + j] : 0u;
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int32_t o = off + 4 * j;
+        uint32_t m = x[k][j] & ~(x[k][j] << 1) & 0x80808080u;  // bit 7 of every continuation byte
+        if (o == 0) m &= 0xFFFFFFFFu << (8 * lead[k]);
+        if (end[k] - o < 4) m &= end[k] - o > 0 ? (1u << (8 * (end[k] - o))) - 1u : 0u;
+        cont[k] += __popc(m);
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) chars[k] = n[k] > 0 ? (int64_t)n[k] - cont[k] : 0;
+}
+
 // The UTF-8 character (its encoded bytes packed little-endian) at s[i].
 __device__ __forceinline__ uint32_t utf8_char_at(const GLOBAL uint8_t* s, int32_t i, int32_t len) {
   uint32_t c = 0;
@@ -832,17 +872,21 @@ __device__ __forceinline__ void compute_rows(const DevPlan& plan, const DevOp& o
       return;
     }
     case D_CHAR_LENGTH: {
+      const GLOBAL uint8_t* p[R];
+      int32_t n[R];
+      int64_t c[R];
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        D[k] = 0;
+        p[k] = nullptr;
+        n[k] = 0;
         if (((rs.exist >> k) & 1u) && ((VA >> k) & 1u)) {
-          const GLOBAL uint8_t* p;
-          int32_t n;
-          str_at(plan, op.aux, A[k], &p, &n);
-          D[k] = (uint64_t)utf8_chars(p, n);
+          str_at(plan, op.aux, A[k], &p[k], &n[k]);
           VD |= 1u << k;
         }
       }
+      utf8_chars_rows<R>(p, n, c);
+#pragma unroll
+      for (int k = 0; k < R; ++k) D[k] = (uint64_t)c[k];
       return;
     }
     case D_LIKE: {
